@@ -1,0 +1,17 @@
+// hull.h -- host convex hull builder used at shape registration.
+#pragma once
+#include <vector>
+
+struct HullData {
+    std::vector<float> verts;      // xyz per vertex
+    std::vector<float> planes;     // nx ny nz w per face
+    std::vector<int> face_index;   // (first index, count) per face, relative to this hull
+    std::vector<int> indices;      // vertex indices per face, relative to this hull
+    std::vector<float> edges;      // unique unit edge directions, xyz
+    float center[3] = {0, 0, 0};   // vertex average
+    float aabb_min[3] = {0, 0, 0};
+    float aabb_max[3] = {0, 0, 0};
+};
+
+// Returns an mi_rb_status value.
+int rb_build_hull(const float *xyz, int stride_bytes, int n, const float scale[3], HullData &out);

@@ -1,0 +1,374 @@
+// k_broad.hip -- body setup and uniform-grid broadphase for gfx950.
+//
+//  k_setup      one pass over the body SoA: world AABB (initializeGpuAabbsFull,
+//               opencl/broadphase_benchmark/broadphaseKernel.cl:293-331), cell key
+//               (kCalcHashAABB, opencl/3dGridBroadphase/Shared/bt3dGridBroadphaseOCL.cl:26-47),
+//               gravity and world inverse inertia (setupBodiesKernel,
+//               broadphaseKernel.cl:163-212).  The reference runs these as three
+//               launches with host syncs in between; here they share one read of
+//               the body state.
+//  k_cell_start first sorted index of every occupied cell (kFindCellStart,
+//               bt3dGridBroadphaseOCL.cl:61-86).  The table is never cleared: a
+//               lookup is valid only if it points at the first entry of that key
+//               in the current sorted array, which a stale entry cannot.
+//  k_find_pairs 27-cell stencil per small body (kFindOverlappingPairs /
+//               findPairsInCell, bt3dGridBroadphaseOCL.cl:100-213), keeping the
+//               overlapping bodies of lower index in a sorted per-body list in LDS.
+//  k_large_pairs overflow list (kFindPairsLarge, :216-264): one workgroup per
+//               large body tests its scene, in index order.
+//  k_emit_pairs  int2 (x > y) pairs at scanned offsets (countOverlappingpairs /
+//               squeezePairCaches, opencl/broadphase_benchmark/computeAabbKernelOCL.cl:40-111),
+//               globally sorted by (x, y).
+#include <hip/hip_runtime.h>
+
+#include "dmath.h"
+#include "rb_internal.h"
+
+using namespace rb;
+
+namespace {
+
+constexpr uint32_t KEY_NONE = 0xffffffffu;
+
+__device__ __forceinline__ void grid_pos(float4 mn, float4 mx, float inv_cell, const int *dims,
+                                         int gp[3]) {
+    float cx = (mn.x + mx.x) * 0.5f, cy = (mn.y + mx.y) * 0.5f, cz = (mn.z + mx.z) * 0.5f;
+    gp[0] = ((int)floorf(cx * inv_cell)) & (dims[0] - 1);
+    gp[1] = ((int)floorf(cy * inv_cell)) & (dims[1] - 1);
+    gp[2] = ((int)floorf(cz * inv_cell)) & (dims[2] - 1);
+}
+__device__ __forceinline__ uint32_t pos_hash(int x, int y, int z, const int *dims) {
+    x &= dims[0] - 1;
+    y &= dims[1] - 1;
+    z &= dims[2] - 1;
+    return (uint32_t)(z * dims[1] * dims[0] + y * dims[0] + x);
+}
+
+struct GridP {
+    float inv_cell;
+    int dims[3];
+    int cell_bits;
+};
+
+__global__ __launch_bounds__(256) void k_setup(
+    int n, const float4 *__restrict__ pos, const float4 *__restrict__ quat,
+    float4 *__restrict__ linvel, float4 *__restrict__ angvel, const float *__restrict__ inv_mass,
+    const int *__restrict__ shape, const int *__restrict__ scene, const int *__restrict__ is_large,
+    const HullInfo *__restrict__ hulls, const float4 *__restrict__ inv_inertia_local,
+    float4 *__restrict__ inv_inertia_world, float4 *__restrict__ aabb_min,
+    float4 *__restrict__ aabb_max, uint2 *__restrict__ kv, GridP g, float gx, float gy, float gz,
+    float dt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 p = pos[i], q = quat[i];
+    const HullInfo *h = hulls + shape[i];
+    float4 hmn = h->aabb_min, hmx = h->aabb_max;
+    float4 he = mk4((hmx.x - hmn.x) * 0.5f, (hmx.y - hmn.y) * 0.5f, (hmx.z - hmn.z) * 0.5f, 0.f);
+    float4 lc = mk4((hmx.x + hmn.x) * 0.5f, (hmx.y + hmn.y) * 0.5f, (hmx.z + hmn.z) * 0.5f, 0.f);
+    float4 wc = xform(lc, p, q);
+    float4 m[3];
+    abs_rot(q, m);
+    float4 ext = mk4(dot3(m[0], he), dot3(m[1], he), dot3(m[2], he), 0.f);
+    float4 mn = mk4(wc.x - ext.x, wc.y - ext.y, wc.z - ext.z, 0.f);
+    float4 mx = mk4(wc.x + ext.x, wc.y + ext.y, wc.z + ext.z, 0.f);
+    aabb_min[i] = mn;
+    aabb_max[i] = mx;
+    uint32_t key = KEY_NONE;
+    if (!is_large[i]) {
+        int gp[3];
+        grid_pos(mn, mx, g.inv_cell, g.dims, gp);
+        key = pos_hash(gp[0], gp[1], gp[2], g.dims) | ((uint32_t)scene[i] << g.cell_bits);
+    }
+    kv[i] = make_uint2(key, (uint32_t)i);
+    float im = inv_mass[i];
+    float4 *I = inv_inertia_world + 3 * i;
+    if (im != 0.f) {
+        float4 v = linvel[i];
+        v.x = v.x + gx * dt;
+        v.y = v.y + gy * dt;
+        v.z = v.z + gz * dt;
+        linvel[i] = v;
+        float4 d = inv_inertia_local[i];
+        float4 mt[3] = {mk4(m[0].x, m[1].x, m[2].x, 0.f), mk4(m[0].y, m[1].y, m[2].y, 0.f),
+                        mk4(m[0].z, m[1].z, m[2].z, 0.f)};
+        float4 loc[3] = {mk4(d.x, 0.f, 0.f, 0.f), mk4(0.f, d.y, 0.f, 0.f), mk4(0.f, 0.f, d.z, 0.f)};
+        float4 tmp[3], out[3];
+        // mtMul(a, b): ans[i][j] = dot3(a.row[i], b.col[j]) (broadphaseKernel.cl:130-148)
+        for (int r = 0; r < 3; r++) {
+            float4 c0 = mk4(loc[0].x, loc[1].x, loc[2].x, 0.f);
+            float4 c1 = mk4(loc[0].y, loc[1].y, loc[2].y, 0.f);
+            float4 c2 = mk4(loc[0].z, loc[1].z, loc[2].z, 0.f);
+            tmp[r] = mk4(dot3(m[r], c0), dot3(m[r], c1), dot3(m[r], c2), 0.f);
+        }
+        for (int r = 0; r < 3; r++) {
+            float4 c0 = mk4(mt[0].x, mt[1].x, mt[2].x, 0.f);
+            float4 c1 = mk4(mt[0].y, mt[1].y, mt[2].y, 0.f);
+            float4 c2 = mk4(mt[0].z, mt[1].z, mt[2].z, 0.f);
+            out[r] = mk4(dot3(tmp[r], c0), dot3(tmp[r], c1), dot3(tmp[r], c2), 0.f);
+        }
+        I[0] = out[0];
+        I[1] = out[1];
+        I[2] = out[2];
+    } else {
+        float4 z = mk4(0.f, 0.f, 0.f, 0.f);
+        linvel[i] = z;
+        angvel[i] = z;
+        I[0] = z;
+        I[1] = z;
+        I[2] = z;
+    }
+}
+
+// Cell key of each AABB for the stand-alone grid API (no scenes).
+__global__ void k_aabb_hash(const float4 *mn, const float4 *mx, const int *is_large, int n,
+                            GridP g, uint2 *kv) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t key = KEY_NONE;
+    if (!(is_large && is_large[i])) {
+        int gp[3];
+        grid_pos(mn[i], mx[i], g.inv_cell, g.dims, gp);
+        key = pos_hash(gp[0], gp[1], gp[2], g.dims);
+    }
+    kv[i] = make_uint2(key, (uint32_t)i);
+}
+
+__global__ void k_cell_start(const uint2 *__restrict__ kv, int n, uint32_t *cell_start) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    uint32_t key = kv[s].x;
+    if (key == KEY_NONE) return;
+    if (s == 0 || kv[s - 1].x != key) cell_start[key] = (uint32_t)s;
+}
+
+constexpr int FP_BLOCK = 256;
+
+// One thread per sorted entry.  lists: per-thread sorted list in LDS,
+// stride K+1 words to spread banks.
+__global__ __launch_bounds__(FP_BLOCK) void k_find_pairs(
+    int n, const uint2 *__restrict__ kv, const uint32_t *__restrict__ cell_start,
+    const float4 *__restrict__ aabb_min, const float4 *__restrict__ aabb_max,
+    const int *__restrict__ large_list, const int *__restrict__ large_first,
+    const int *__restrict__ large_num, GridP g, int K, int *__restrict__ body_slots,
+    uint32_t *__restrict__ pair_count, int *overflow) {
+    extern __shared__ int s_lists[];
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    uint2 e = kv[t];
+    if (e.x == KEY_NONE) return;  // large body: handled by k_large_pairs
+    int *lst = s_lists + threadIdx.x * (K + 1);
+    int i = (int)e.y;
+    int sc = (int)(e.x >> g.cell_bits);
+    float4 mn0 = aabb_min[i], mx0 = aabb_max[i];
+    int gp[3];
+    grid_pos(mn0, mx0, g.inv_cell, g.dims, gp);
+    int c = 0;
+    for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+                uint32_t nk = pos_hash(gp[0] + dx, gp[1] + dy, gp[2] + dz, g.dims) |
+                              ((uint32_t)sc << g.cell_bits);
+                uint32_t s = cell_start[nk];
+                if (s >= (uint32_t)n || kv[s].x != nk || (s > 0 && kv[s - 1].x == nk)) continue;
+                for (uint32_t u = s; u < (uint32_t)n; u++) {
+                    uint2 f = kv[u];
+                    if (f.x != nk) break;
+                    int j = (int)f.y;
+                    if (j >= i) continue;
+                    if (!aabb_overlap(mn0, mx0, aabb_min[j], aabb_max[j])) continue;
+                    if (c < K) {
+                        int k = c;
+                        while (k > 0 && lst[k - 1] > j) { lst[k] = lst[k - 1]; k--; }
+                        lst[k] = j;
+                    }
+                    c++;
+                }
+            }
+    if (large_num) {
+        int l0 = large_first[sc], nl = large_num[sc];
+        for (int q = 0; q < nl; q++) {
+            int L = large_list[l0 + q];
+            if (L >= i) break;  // list sorted by index
+            if (!aabb_overlap(mn0, mx0, aabb_min[L], aabb_max[L])) continue;
+            if (c < K) {
+                int k = c;
+                while (k > 0 && lst[k - 1] > L) { lst[k] = lst[k - 1]; k--; }
+                lst[k] = L;
+            }
+            c++;
+        }
+    }
+    if (c > K) {
+        atomicOr(overflow, (int)MI_RB_OVERFLOW_BODY_PAIRS);
+        c = K;
+    }
+    int *dst = body_slots + (size_t)i * K;
+    for (int k = 0; k < c; k++) dst[k] = lst[k];
+    pair_count[i] = (uint32_t)c;
+}
+
+// One workgroup per large body x: every body j < x of its scene, in order.
+__global__ __launch_bounds__(256) void k_large_pairs(
+    const int *__restrict__ large_list, const int *__restrict__ large_off,
+    const int *__restrict__ scene, const int *__restrict__ scene_start,
+    const float4 *__restrict__ aabb_min, const float4 *__restrict__ aabb_max,
+    int *__restrict__ large_slots, uint32_t *__restrict__ pair_count) {
+    __shared__ uint32_t s_w[4];
+    __shared__ uint32_t s_base;
+    int x = large_list[blockIdx.x];
+    int j0 = scene ? scene_start[scene[x]] : 0;
+    int *dst = large_slots + large_off[blockIdx.x];
+    float4 mn0 = aabb_min[x], mx0 = aabb_max[x];
+    if (threadIdx.x == 0) s_base = 0;
+    __syncthreads();
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int c0 = j0; c0 < x; c0 += 256) {
+        int j = c0 + threadIdx.x;
+        bool hit = j < x && aabb_overlap(mn0, mx0, aabb_min[j], aabb_max[j]);
+        uint64_t b = __ballot(hit);
+        uint32_t below = (uint32_t)__popcll(b & lanemask_lt());
+        if (lane == 0) s_w[wave] = (uint32_t)__popcll(b);
+        __syncthreads();
+        uint32_t off = s_base;
+        uint32_t tot = 0;
+        for (int w = 0; w < 4; w++) {
+            if (w < wave) off += s_w[w];
+            tot += s_w[w];
+        }
+        if (hit) dst[off + below] = j;
+        __syncthreads();
+        if (threadIdx.x == 0) s_base += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) pair_count[x] = s_base;
+}
+
+__global__ __launch_bounds__(256) void k_emit_pairs(
+    int n, const uint32_t *__restrict__ pair_count, const uint32_t *__restrict__ pair_off,
+    const int *__restrict__ body_slots, int K, const int *__restrict__ is_large,
+    const int *__restrict__ large_index, const int *__restrict__ large_off,
+    const int *__restrict__ large_slots, int max_pairs, int2 *__restrict__ pairs,
+    const uint32_t *total, int *num_pairs, int *overflow) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x == 0) {
+        uint32_t tt = *total;
+        if (tt > (uint32_t)max_pairs) {
+            atomicOr(overflow, (int)MI_RB_OVERFLOW_PAIRS);
+            tt = (uint32_t)max_pairs;
+        }
+        *num_pairs = (int)tt;
+    }
+    if (x >= n) return;
+    uint32_t c = pair_count[x], off = pair_off[x];
+    const int *src = (is_large && is_large[x]) ? large_slots + large_off[large_index[x]]
+                                               : body_slots + (size_t)x * K;
+    for (uint32_t k = 0; k < c; k++) {
+        uint32_t o = off + k;
+        if (o < (uint32_t)max_pairs) pairs[o] = make_int2(x, src[k]);
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- launchers
+static GridP grid_of(const StepParams &p) {
+    GridP g;
+    g.inv_cell = p.inv_cell;
+    g.dims[0] = p.dims[0];
+    g.dims[1] = p.dims[1];
+    g.dims[2] = p.dims[2];
+    g.cell_bits = p.cell_bits;
+    return g;
+}
+
+hipError_t launch_setup(const DeviceWorld &d, const StepParams &p, hipStream_t s) {
+    if (p.n_bodies == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_setup, dim3((p.n_bodies + 255) / 256), dim3(256), 0, s, p.n_bodies,
+                       d.pos, d.quat, d.linvel, d.angvel, d.inv_mass, d.shape, d.scene, d.is_large,
+                       d.hulls, d.inv_inertia_local, d.inv_inertia_world, d.aabb_min, d.aabb_max,
+                       d.kv, grid_of(p), p.gravity[0], p.gravity[1], p.gravity[2], p.dt);
+    return hipGetLastError();
+}
+
+hipError_t launch_cell_start(const DeviceWorld &d, const StepParams &p, hipStream_t s) {
+    return launch_cell_start_generic(d.kv, p.n_bodies, d.cell_start, s);
+}
+
+hipError_t launch_cell_start_generic(const uint2 *kv, int n, uint32_t *cell_start, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_cell_start, dim3((n + 255) / 256), dim3(256), 0, s, kv, n, cell_start);
+    return hipGetLastError();
+}
+
+hipError_t launch_find_pairs(const DeviceWorld &d, const StepParams &p, hipStream_t s) {
+    int n = p.n_bodies;
+    if (n == 0) return hipSuccess;
+    int K = p.max_pairs_per_body;
+    size_t lds = sizeof(int) * (size_t)FP_BLOCK * (K + 1);
+    hipLaunchKernelGGL(k_find_pairs, dim3((n + FP_BLOCK - 1) / FP_BLOCK), dim3(FP_BLOCK), lds, s, n,
+                       d.kv, d.cell_start, d.aabb_min, d.aabb_max, d.large_list,
+                       p.n_large ? d.large_first : nullptr, p.n_large ? d.large_num : nullptr,
+                       grid_of(p), K, d.body_slots, d.pair_count, d.counters + RB_CNT_OVERFLOW);
+    if (p.n_large > 0)
+        hipLaunchKernelGGL(k_large_pairs, dim3(p.n_large), dim3(256), 0, s, d.large_list,
+                           d.large_off, d.scene, d.scene_start, d.aabb_min, d.aabb_max,
+                           d.large_slots, d.pair_count);
+    return hipGetLastError();
+}
+
+hipError_t launch_emit_pairs(const DeviceWorld &d, const StepParams &p, hipStream_t s) {
+    int n = p.n_bodies;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_emit_pairs, dim3((n + 255) / 256), dim3(256), 0, s, n, d.pair_count,
+                       d.pair_off, d.body_slots, p.max_pairs_per_body, d.is_large, d.large_index,
+                       d.large_off, d.large_slots, p.max_pairs, d.pairs,
+                       (const uint32_t *)(d.counters + RB_CNT_SORT_TILE),
+                       d.counters + RB_CNT_PAIRS, d.counters + RB_CNT_OVERFLOW);
+    return hipGetLastError();
+}
+
+hipError_t launch_aabb_hash(const float4 *mn, const float4 *mx, const int *is_large, int n,
+                            float inv_cell, const int dims[3], uint2 *kv, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    GridP g;
+    g.inv_cell = inv_cell;
+    g.dims[0] = dims[0];
+    g.dims[1] = dims[1];
+    g.dims[2] = dims[2];
+    g.cell_bits = 0;
+    hipLaunchKernelGGL(k_aabb_hash, dim3((n + 255) / 256), dim3(256), 0, s, mn, mx, is_large, n, g,
+                       kv);
+    return hipGetLastError();
+}
+
+hipError_t launch_grid_pairs(const float4 *mn, const float4 *mx, const uint2 *kv,
+                             uint32_t *cell_start, const int *large_list, int n_large, int n,
+                             float inv_cell, const int dims[3], int cell_bits, int K,
+                             int *body_slots, uint32_t *pair_count, int *overflow, hipStream_t s) {
+    (void)cell_bits;
+    if (n == 0) return hipSuccess;
+    GridP g;
+    g.inv_cell = inv_cell;
+    g.dims[0] = dims[0];
+    g.dims[1] = dims[1];
+    g.dims[2] = dims[2];
+    g.cell_bits = 31;  // one scene: (key >> 31) is 0 for every cell key
+    size_t lds = sizeof(int) * (size_t)FP_BLOCK * (K + 1);
+    // large bodies: large_list holds [n_large indices][first=0][num=n_large]
+    hipLaunchKernelGGL(k_find_pairs, dim3((n + FP_BLOCK - 1) / FP_BLOCK), dim3(FP_BLOCK), lds, s, n,
+                       kv, cell_start, mn, mx, large_list, n_large ? large_list + n_large : nullptr,
+                       n_large ? large_list + n_large + 1 : nullptr, g, K, body_slots, pair_count,
+                       overflow);
+    return hipGetLastError();
+}
+
+hipError_t launch_grid_emit_simple(const uint32_t *pair_count, const uint32_t *pair_off,
+                                   const int *body_slots, int n, int K, int max_pairs, int2 *pairs,
+                                   const uint32_t *total, int *num_pairs, int *overflow,
+                                   hipStream_t s) {
+    hipLaunchKernelGGL(k_emit_pairs, dim3((n + 255) / 256 > 0 ? (n + 255) / 256 : 1), dim3(256), 0,
+                       s, n, pair_count, pair_off, body_slots, K, (const int *)nullptr,
+                       (const int *)nullptr, (const int *)nullptr, (const int *)nullptr, max_pairs,
+                       pairs, total, num_pairs, overflow);
+    return hipGetLastError();
+}

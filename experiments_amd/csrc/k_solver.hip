@@ -1,0 +1,646 @@
+// k_solver.hip -- contact constraints, tile colouring, batched Gauss-Seidel
+// solve and integration for gfx950.
+//
+// Constraint maths follows the reference solver:
+//   setConstraint4      dynamics/basic_demo/Stubs/solverSetup.cl:520-610
+//   solveContact        solveContact.cl:272-312 (normal rows, lambda >= 0)
+//   solveFriction       solveFriction.cl:297-430 (2 tangents at the centre,
+//                       |lambda| <= mu * sum lambda_n, angular damping)
+//   integrateTransformsKernel2 + copyVelocitiesKernel
+//                       opencl/broadphase_benchmark/integrateKernel.cl:102-158,
+//                       broadphaseKernel.cl:216-233
+// The schedule replaces the reference's 16x16 solver cells with 4 parity
+// launches per iteration and the nondeterministic CreateBatches
+// (batchingKernels.cl:130-341): constraints are grouped into tiles (one scene
+// each, or XZ columns of a single large scene), each tile is coloured in LDS so
+// that no dynamic body repeats within a colour, and one workgroup walks its
+// tile's colours with barriers between them, keeping the tile's body velocities
+// in LDS.  Spatial tiles are run in 9 phases (tile x mod 3, z mod 3) per sweep
+// so that tiles sharing bodies never run together; scene tiles are independent
+// and run every sweep in one launch.
+#include <float.h>
+#include <hip/hip_runtime.h>
+
+#include "dmath.h"
+#include "rb_internal.h"
+
+using namespace rb;
+
+namespace {
+
+constexpr int TB = 256;  // threads per tile workgroup
+constexpr int HASH_SLOTS = 2 * RB_TILE_MAX_BODIES;
+
+__device__ __forceinline__ uint32_t color_prio(uint32_t k) {
+    k ^= k >> 16; k *= 0x7feb352dU; k ^= k >> 15; k *= 0x846ca68bU; k ^= k >> 16;
+    return k;
+}
+
+__device__ __forceinline__ float jac_coeff(float4 a0, float4 a1, float imA, const float4 *IA,
+                                           float imB, const float4 *IB) {
+    float j0 = imA, j1 = dot3(mt_mul3(a0, IA), a0), j2 = imB, j3 = dot3(mt_mul3(a1, IB), a1);
+    return -1.f / (((j0 + j1) + j2) + j3);
+}
+__device__ __forceinline__ float rel_vel(float4 l0, float4 l1, float4 a0, float4 a1, float4 vA,
+                                         float4 wA, float4 vB, float4 wB) {
+    return ((dot3(l0, vA) + dot3(a0, wA)) + dot3(l1, vB)) + dot3(a1, wB);
+}
+__device__ __forceinline__ void plane_space(float4 n, float4 &p, float4 &q) {
+    if (fabsf(n.z) > 0.70710678f) {
+        float a = n.y * n.y + n.z * n.z;
+        float k = 1.f / sqrtf(a);
+        p = mk4(0.f, -n.z * k, n.y * k, 0.f);
+        q = mk4(a * k, -n.x * p.z, n.x * p.y, 0.f);
+    } else {
+        float a = n.x * n.x + n.y * n.y;
+        float k = 1.f / sqrtf(a);
+        p = mk4(-n.y * k, n.x * k, 0.f, 0.f);
+        q = mk4(-n.z * p.y, n.z * p.x, a * k, 0.f);
+    }
+}
+
+__device__ __forceinline__ int tile_of(float4 p, int scene, const StepParams &sp) {
+    if (sp.tile_mode == 1) return scene;
+    int tx = (int)floorf((p.x - sp.tile_origin_x) * sp.tile_inv);
+    int tz = (int)floorf((p.z - sp.tile_origin_z) * sp.tile_inv);
+    tx = tx < 0 ? 0 : (tx >= sp.tiles_x ? sp.tiles_x - 1 : tx);
+    tz = tz < 0 ? 0 : (tz >= sp.tiles_z ? sp.tiles_z - 1 : tz);
+    return tz * sp.tiles_x + tx;
+}
+
+// ---- compaction of the pair slots + ContactToConstraintKernel ----
+__global__ __launch_bounds__(256) void k_constraints(
+    StepParams sp, const int *__restrict__ num_pairs_ptr, const uint32_t *__restrict__ has_contact,
+    const uint32_t *__restrict__ contact_off, const Contact4 *__restrict__ slots,
+    Contact4 *__restrict__ contacts, const float4 *__restrict__ pos,
+    const float4 *__restrict__ linvel, const float4 *__restrict__ angvel,
+    const float *__restrict__ inv_mass, const float4 *__restrict__ invI, const int *__restrict__ scene,
+    ConstraintSoA cs, uint2 *__restrict__ tile_kv, int *counters) {
+    int np = *num_pairs_ptr;
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np || !has_contact[p]) return;
+    int k = (int)contact_off[p];
+    if (k >= sp.max_contacts) {
+        atomicOr(counters + RB_CNT_OVERFLOW, (int)MI_RB_OVERFLOW_CONTACTS);
+        return;
+    }
+    Contact4 src = slots[p];
+    contacts[k] = src;
+    int a = abs(src.body_a), b = abs(src.body_b);
+    float4 pA = pos[a], pB = pos[b];
+    float4 vA = linvel[a], wA = angvel[a], vB = linvel[b], wB = angvel[b];
+    float imA = inv_mass[a], imB = inv_mass[b];
+    float4 IA[3] = {invI[3 * a], invI[3 * a + 1], invI[3 * a + 2]};
+    float4 IB[3] = {invI[3 * b], invI[3 * b + 1], invI[3 * b + 2]};
+    // static bodies: zero velocity and inverse inertia (setupBodiesKernel)
+    if (imA == 0.f) { vA = mk4(0, 0, 0, 0); wA = vA; }
+    if (imB == 0.f) { vB = mk4(0, 0, 0, 0); wB = vB; }
+    float dt_inv = 1.f / sp.solver_dt;
+    float4 n = src.normal;
+    float npts = n.w;
+    cs.linear[k] = mk4(-n.x, -n.y, -n.z, sp.friction);
+    float jac[4], bb[4];
+    for (int ic = 0; ic < 4; ic++) {
+        float4 wp = src.pos[ic];
+        float4 r0 = sub4(wp, pA), r1 = sub4(wp, pB);
+        if ((float)ic >= npts) { jac[ic] = 0.f; bb[ic] = 0.f; continue; }
+        float4 lin = neg4(n);
+        float4 a0 = neg4(cross3(r0, n));
+        float4 a1 = cross3(r1, n);
+        jac[ic] = jac_coeff(a0, a1, imA, IA, imB, IB);
+        float rv = rel_vel(lin, neg4(lin), a0, a1, vA, wA, vB, wB);
+        float e = 0.f;
+        float bv = e * rv;
+        bv += ((wp.w + sp.drift) * sp.baumgarte) * dt_inv;
+        bb[ic] = bv;
+    }
+    cs.jac_inv[k] = mk4(jac[0], jac[1], jac[2], jac[3]);
+    cs.b[k] = mk4(bb[0], bb[1], bb[2], bb[3]);
+    cs.lambda[k] = mk4(0, 0, 0, 0);
+    float4 fr = mk4(0, 0, 0, 0);
+    float4 cen = mk4(0, 0, 0, 0);
+    if (npts > 0) {
+        for (int i = 0; (float)i < npts; i++) cen = add4(cen, src.pos[i]);
+        cen = mk4(cen.x / npts, cen.y / npts, cen.z / npts, cen.w / npts);
+        float4 t0, t1;
+        plane_space(n, t0, t1);
+        float4 r0 = sub4(cen, pA), r1 = sub4(cen, pB);
+        fr.x = jac_coeff(neg4(cross3(r0, t0)), cross3(r1, t0), imA, IA, imB, IB);
+        fr.y = jac_coeff(neg4(cross3(r0, t1)), cross3(r1, t1), imA, IA, imB, IB);
+    }
+    cs.fric[k] = fr;
+    cs.center[k] = cen;
+    for (int i = 0; i < 4; i++) cs.pos[4 * k + i] = (float)i < npts ? src.pos[i] : mk4(0, 0, 0, 0);
+    cs.bodies[k] = make_int2(a, b);
+    cs.pair[k] = p;
+    // owner tile: body A unless static, then body B (SetSortDataKernel, solverSetup2.cl:452-479)
+    int owner = imA != 0.f ? a : b;
+    int t = tile_of(pos[owner], scene[owner], sp);
+    tile_kv[k] = make_uint2((uint32_t)t, (uint32_t)k);
+}
+
+__global__ void k_contact_total(const uint32_t *total, int *counters, int max_contacts) {
+    uint32_t t = *total;
+    if (t > (uint32_t)max_contacts) t = (uint32_t)max_contacts;
+    counters[RB_CNT_CONTACTS] = (int)t;
+}
+
+// tile_first/tile_cnt of every tile: zero, then the run boundaries of the
+// sorted tile keys
+__global__ void k_tile_clear(int n_tiles, int *tile_first, int *tile_cnt) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n_tiles) { tile_first[k] = 0; tile_cnt[k] = 0; }
+}
+__global__ void k_tile_bounds2(const uint2 *__restrict__ tile_kv, const int *counters,
+                               int *tile_first, int *tile_cnt) {
+    int nc = counters[RB_CNT_CONTACTS];
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nc) return;
+    uint32_t t = tile_kv[k].x;
+    if (k == 0 || tile_kv[k - 1].x != t) tile_first[t] = k;
+    if (k == nc - 1 || tile_kv[k + 1].x != t) tile_cnt[t] = k + 1;  // end of the run
+}
+
+// ---- per-tile colouring (Luby rounds in LDS) and colour-major order ----
+struct TileLDS {
+    int key[HASH_SLOTS];                 // body index per hash slot, -1 empty
+    unsigned long long mark[HASH_SLOTS]; // (round << 32) | priority
+    short la[RB_TILE_MAX_ROWS], lb[RB_TILE_MAX_ROWS];  // local slots (-1 static)
+    unsigned char color[RB_TILE_MAX_ROWS];
+    int ccount[RB_MAX_LOCAL_COLORS + 1];
+    int coff[RB_MAX_LOCAL_COLORS + 1];
+    int remaining;
+};
+
+// open-addressing body -> LDS slot tables (SLOTS a power of two)
+template <int SLOTS>
+__device__ __forceinline__ int hash_insert(int *key, int body) {
+    uint32_t h = ((uint32_t)body * 2654435761u) & (SLOTS - 1);
+    for (int probe = 0; probe < SLOTS; probe++) {
+        int old = atomicCAS(&key[h], -1, body);
+        if (old == -1 || old == body) return (int)h;
+        h = (h + 1) & (SLOTS - 1);
+    }
+    return -1;
+}
+template <int SLOTS>
+__device__ __forceinline__ int hash_find(const int *key, int body) {
+    uint32_t h = ((uint32_t)body * 2654435761u) & (SLOTS - 1);
+    for (int probe = 0; probe < SLOTS; probe++) {
+        int k = key[h];
+        if (k == body) return (int)h;
+        if (k == -1) return -1;
+        h = (h + 1) & (SLOTS - 1);
+    }
+    return -1;
+}
+
+__global__ __launch_bounds__(TB) void k_tile_color(
+    const uint2 *__restrict__ tile_kv, const int *__restrict__ tile_first,
+    const int *__restrict__ tile_cnt, const int2 *__restrict__ bodies,
+    const float *__restrict__ inv_mass, int *__restrict__ order, int *__restrict__ row_color,
+    int *__restrict__ tile_color_off, int *__restrict__ tile_ncolors, int *counters) {
+    __shared__ TileLDS L;
+    const int t = blockIdx.x;
+    const int first = tile_first[t];
+    int R = tile_cnt[t] - first;
+    if (R > RB_TILE_MAX_ROWS) {
+        if (threadIdx.x == 0) atomicOr(counters + RB_CNT_OVERFLOW, (int)MI_RB_OVERFLOW_TILE);
+        R = RB_TILE_MAX_ROWS;
+    }
+    for (int i = threadIdx.x; i < HASH_SLOTS; i += TB) { L.key[i] = -1; L.mark[i] = 0ull; }
+    for (int i = threadIdx.x; i <= RB_MAX_LOCAL_COLORS; i += TB) L.ccount[i] = 0;
+    if (threadIdx.x == 0) L.remaining = 0;
+    __syncthreads();
+    for (int r = threadIdx.x; r < R; r += TB) {
+        int k = (int)tile_kv[first + r].y;
+        int2 bd = bodies[k];
+        int sa = inv_mass[bd.x] != 0.f ? hash_insert<HASH_SLOTS>(L.key, bd.x) : -1;
+        int sb = inv_mass[bd.y] != 0.f ? hash_insert<HASH_SLOTS>(L.key, bd.y) : -1;
+        L.la[r] = (short)sa;
+        L.lb[r] = (short)sb;
+        L.color[r] = 0xff;
+    }
+    __syncthreads();
+    int ncol = 0;
+    for (int round = 0; round < RB_MAX_LOCAL_COLORS; round++) {
+        unsigned long long tag = ((unsigned long long)(round + 1) << 32);
+        for (int r = threadIdx.x; r < R; r += TB) {
+            if (L.color[r] != 0xff) continue;
+            unsigned long long v = tag | color_prio(tile_kv[first + r].y);
+            if (L.la[r] >= 0) atomicMax(&L.mark[L.la[r]], v);
+            if (L.lb[r] >= 0) atomicMax(&L.mark[L.lb[r]], v);
+        }
+        __syncthreads();
+        int left = 0;
+        for (int r = threadIdx.x; r < R; r += TB) {
+            if (L.color[r] != 0xff) continue;
+            unsigned long long v = tag | color_prio(tile_kv[first + r].y);
+            bool win = (L.la[r] < 0 || L.mark[L.la[r]] == v) && (L.lb[r] < 0 || L.mark[L.lb[r]] == v);
+            if (win) L.color[r] = (unsigned char)round;
+            else left++;
+        }
+        if (left) atomicAdd(&L.remaining, left);
+        __syncthreads();
+        int rem = L.remaining;
+        __syncthreads();
+        if (threadIdx.x == 0) L.remaining = 0;
+        ncol = round + 1;
+        if (rem == 0) break;
+        __syncthreads();
+    }
+    __syncthreads();
+    // rows still uncoloured: one colour each after the last (serial, rare)
+    if (threadIdx.x == 0) {
+        int extra = 0;
+        for (int r = 0; r < R; r++)
+            if (L.color[r] == 0xff) extra++;
+        if (extra) atomicOr(counters + RB_CNT_OVERFLOW, (int)MI_RB_OVERFLOW_COLORS);
+    }
+    for (int r = threadIdx.x; r < R; r += TB) {
+        if (L.color[r] == 0xff) L.color[r] = (unsigned char)(RB_MAX_LOCAL_COLORS - 1);
+        atomicAdd(&L.ccount[L.color[r]], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int c = 0; c < RB_MAX_LOCAL_COLORS; c++) {
+            L.coff[c] = run;
+            tile_color_off[t * (RB_MAX_LOCAL_COLORS + 1) + c] = run;
+            run += L.ccount[c];
+            L.ccount[c] = 0;
+        }
+        tile_color_off[t * (RB_MAX_LOCAL_COLORS + 1) + RB_MAX_LOCAL_COLORS] = run;
+        int nc = ncol;
+        for (int c = 0; c < RB_MAX_LOCAL_COLORS; c++)
+            if (L.coff[c] < run && c + 1 > nc) nc = c + 1;
+        tile_ncolors[t] = nc;
+        atomicMax(counters + RB_CNT_NCOLORS, nc);
+    }
+    __syncthreads();
+    // colour-major, index order within a colour: rows are in index order here,
+    // so a ballot-ranked stable scatter per colour keeps it
+    for (int base = 0; base < R; base += TB) {
+        int r = base + threadIdx.x;
+        int c = r < R ? L.color[r] : -1;
+        // stable rank among equal colours within this chunk: serial over waves
+        for (int w = 0; w < TB / 64; w++) {
+            if ((threadIdx.x >> 6) == w && r < R) {
+                uint64_t peers = __ballot(true);
+                for (int bit = 0; bit < 6; bit++) {
+                    uint64_t bal = __ballot((c >> bit) & 1);
+                    peers &= ((c >> bit) & 1) ? bal : ~bal;
+                }
+                int below = __popcll(peers & lanemask_lt());
+                int before = L.ccount[c];
+                wave_sync();
+                if (below == 0) L.ccount[c] = before + __popcll(peers);
+                wave_sync();
+                int dst = L.coff[c] + before + below;
+                int k = (int)tile_kv[first + r].y;
+                order[first + dst] = k;
+                row_color[k] = c;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// ---- solve ----
+constexpr int SOLVE_SLOTS = 1024;  // LDS velocity cache of one tile (36 KB)
+struct SolveLDS {
+    int key[SOLVE_SLOTS];
+    float4 v[SOLVE_SLOTS];
+    float4 w[SOLVE_SLOTS];
+};
+
+__device__ __forceinline__ void solve_row_normal(int k, const ConstraintSoA &cs, float4 &vA,
+                                                 float4 &wA, float4 &vB, float4 &wB, float4 pA,
+                                                 float4 pB, float imA, const float4 *IA, float imB,
+                                                 const float4 *IB, float &rmax, float &lmax) {
+    float4 lin0 = cs.linear[k];
+    float4 jac = cs.jac_inv[k], bb = cs.b[k], lam = cs.lambda[k];
+    float jv[4] = {jac.x, jac.y, jac.z, jac.w};
+    float bv[4] = {bb.x, bb.y, bb.z, bb.w};
+    float lv[4] = {lam.x, lam.y, lam.z, lam.w};
+    for (int ic = 0; ic < 4; ic++) {
+        if (jv[ic] == 0.f) continue;
+        float4 wp = cs.pos[4 * k + ic];
+        float4 r0 = sub4(wp, pA), r1 = sub4(wp, pB);
+        float4 n = neg4(lin0);
+        float4 lin = neg4(n);
+        float4 a0 = neg4(cross3(r0, n));
+        float4 a1 = cross3(r1, n);
+        float rl = rel_vel(lin0, neg4(lin0), a0, a1, vA, wA, vB, wB) + bv[ic];
+        rl *= jv[ic];
+        float prev = lv[ic];
+        float upd = prev + rl;
+        upd = fmaxf(upd, 0.f);
+        upd = fminf(upd, FLT_MAX);
+        rl = upd - prev;
+        lv[ic] = upd;
+        float4 li0 = scl4(scl4(lin, imA), rl);
+        float4 li1 = scl4(scl4(neg4(lin), imB), rl);
+        float4 ai0 = scl4(mt_mul1(IA, a0), rl);
+        float4 ai1 = scl4(mt_mul1(IB, a1), rl);
+        vA = add4(vA, li0); wA = add4(wA, ai0);
+        vB = add4(vB, li1); wB = add4(wB, ai1);
+        float sc = fabsf(rl) / fabsf(jv[ic]);
+        rmax = fmaxf(rmax, sc);
+        lmax = fmaxf(lmax, fabsf(upd));
+    }
+    cs.lambda[k] = mk4(lv[0], lv[1], lv[2], lv[3]);
+}
+
+__device__ __forceinline__ void solve_row_friction(int k, const ConstraintSoA &cs, float4 &vA,
+                                                   float4 &wA, float4 &vB, float4 &wB, float4 pA,
+                                                   float4 pB, float imA, const float4 *IA,
+                                                   float imB, const float4 *IB, float friction,
+                                                   float &rmax, float &lmax) {
+    float4 lam = cs.lambda[k];
+    float sum = 0;
+    sum += lam.x; sum += lam.y; sum += lam.z; sum += lam.w;
+    float mx = friction * sum, mn = -mx;
+    float4 fr = cs.fric[k];
+    if (fr.x == 0 && fr.x == 0) return;
+    float4 cen = cs.center[k];
+    float4 n = neg4(cs.linear[k]);
+    float4 t[2];
+    plane_space(n, t[0], t[1]);
+    float4 r0 = sub4(cen, pA), r1 = sub4(cen, pB);
+    float fj[2] = {fr.x, fr.y};
+    float fl[2] = {fr.z, fr.w};
+    for (int i = 0; i < 2; i++) {
+        float4 lin = neg4(t[i]);
+        float4 a0 = neg4(cross3(r0, t[i]));
+        float4 a1 = cross3(r1, t[i]);
+        float rl = rel_vel(lin, neg4(lin), a0, a1, vA, wA, vB, wB);
+        rl *= fj[i];
+        float prev = fl[i];
+        float upd = prev + rl;
+        upd = fmaxf(upd, mn);
+        upd = fminf(upd, mx);
+        rl = upd - prev;
+        fl[i] = upd;
+        float4 li0 = scl4(scl4(lin, imA), rl);
+        float4 li1 = scl4(scl4(neg4(lin), imB), rl);
+        float4 ai0 = scl4(mt_mul1(IA, a0), rl);
+        float4 ai1 = scl4(mt_mul1(IB, a1), rl);
+        vA = add4(vA, li0); wA = add4(wA, ai0);
+        vB = add4(vB, li1); wB = add4(wB, ai1);
+        float sc = fabsf(rl) / fabsf(fj[i]);
+        rmax = fmaxf(rmax, sc);
+        lmax = fmaxf(lmax, fabsf(upd));
+    }
+    cs.fric[k] = mk4(fr.x, fr.y, fl[0], fl[1]);
+    float4 ab = normalize3(sub4(pB, pA));
+    float4 ac = normalize3(sub4(cen, pA));
+    if (dot3(ab, ac) > 0.95f || (imA == 0.f || imB == 0.f)) {
+        float na = dot3(n, wA), nb = dot3(n, wB);
+        wA = sub4(wA, scl4(n, na * 0.1f));
+        wB = sub4(wB, scl4(n, nb * 0.1f));
+    }
+}
+
+// One workgroup per tile.  mode: 0 normal sweeps, 1 friction sweeps.
+// sweeps: how many sweeps to run inside (scene tiles run all of them here).
+__global__ __launch_bounds__(TB) void k_tile_solve(
+    StepParams sp, int phase, int sweep0, int nsweeps_n, int nsweeps_f,
+    const int *__restrict__ tile_first, const int *__restrict__ tile_cnt,
+    const int *__restrict__ order, const int *__restrict__ tile_color_off,
+    const int *__restrict__ tile_ncolors, ConstraintSoA cs, float4 *__restrict__ linvel,
+    float4 *__restrict__ angvel, const float4 *__restrict__ pos, const float *__restrict__ inv_mass,
+    const float4 *__restrict__ invI, float *__restrict__ residual, int *counters) {
+    __shared__ SolveLDS L;
+    int t;
+    if (sp.tile_mode == 1) {
+        t = blockIdx.x;
+    } else {
+        // tiles of this phase: (tx % 3, tz % 3) == (phase % 3, phase / 3)
+        int px = phase % 3, pz = phase / 3;
+        int nx = (sp.tiles_x - px + 2) / 3;
+        int bx = blockIdx.x % nx, bz = blockIdx.x / nx;
+        int tx = px + 3 * bx, tz = pz + 3 * bz;
+        if (tx >= sp.tiles_x || tz >= sp.tiles_z) return;
+        t = tz * sp.tiles_x + tx;
+    }
+    if (counters[RB_CNT_DONE] != 0) return;  // converged flag set by a previous sweep
+    const int first = tile_first[t];
+    int R = tile_cnt[t] - first;
+    if (R <= 0) return;
+    if (R > RB_TILE_MAX_ROWS) R = RB_TILE_MAX_ROWS;
+    const int nc = tile_ncolors[t];
+    const int *coff = tile_color_off + t * (RB_MAX_LOCAL_COLORS + 1);
+    for (int i = threadIdx.x; i < SOLVE_SLOTS; i += TB) L.key[i] = -1;
+    __syncthreads();
+    for (int r = threadIdx.x; r < R; r += TB) {
+        int k = order[first + r];
+        int2 bd = cs.bodies[k];
+        if (inv_mass[bd.x] != 0.f) {
+            int h = hash_insert<SOLVE_SLOTS>(L.key, bd.x);
+            if (h >= 0) { L.v[h] = linvel[bd.x]; L.w[h] = angvel[bd.x]; }
+            else atomicOr(counters + RB_CNT_OVERFLOW, (int)MI_RB_OVERFLOW_TILE);
+        }
+        if (inv_mass[bd.y] != 0.f) {
+            int h = hash_insert<SOLVE_SLOTS>(L.key, bd.y);
+            if (h >= 0) { L.v[h] = linvel[bd.y]; L.w[h] = angvel[bd.y]; }
+            else atomicOr(counters + RB_CNT_OVERFLOW, (int)MI_RB_OVERFLOW_TILE);
+        }
+    }
+    __syncthreads();
+    const int total = nsweeps_n + nsweeps_f;
+    for (int s = 0; s < total; s++) {
+        const bool fric = s >= nsweeps_n;
+        float rmax = 0.f, lmax = 0.f;
+        for (int c = 0; c < nc; c++) {
+            int c0 = coff[c], c1 = coff[c + 1];
+            if (c1 > R) c1 = R;
+            for (int r = c0 + threadIdx.x; r < c1; r += TB) {
+                int k = order[first + r];
+                int2 bd = cs.bodies[k];
+                int a = bd.x, b = bd.y;
+                float imA = inv_mass[a], imB = inv_mass[b];
+                int ha = imA != 0.f ? hash_find<SOLVE_SLOTS>(L.key, a) : -1;
+                int hb = imB != 0.f ? hash_find<SOLVE_SLOTS>(L.key, b) : -1;
+                float4 z = mk4(0, 0, 0, 0);
+                // dynamic bodies that did not fit the LDS cache live in global memory
+                float4 vA = ha >= 0 ? L.v[ha] : (imA != 0.f ? linvel[a] : z);
+                float4 wA = ha >= 0 ? L.w[ha] : (imA != 0.f ? angvel[a] : z);
+                float4 vB = hb >= 0 ? L.v[hb] : (imB != 0.f ? linvel[b] : z);
+                float4 wB = hb >= 0 ? L.w[hb] : (imB != 0.f ? angvel[b] : z);
+                float4 IA[3] = {invI[3 * a], invI[3 * a + 1], invI[3 * a + 2]};
+                float4 IB[3] = {invI[3 * b], invI[3 * b + 1], invI[3 * b + 2]};
+                float4 pA = pos[a], pB = pos[b];
+                if (!fric)
+                    solve_row_normal(k, cs, vA, wA, vB, wB, pA, pB, imA, IA, imB, IB, rmax, lmax);
+                else
+                    solve_row_friction(k, cs, vA, wA, vB, wB, pA, pB, imA, IA, imB, IB, sp.friction,
+                                       rmax, lmax);
+                if (ha >= 0) { L.v[ha] = vA; L.w[ha] = wA; }
+                else if (imA != 0.f) { linvel[a] = vA; angvel[a] = wA; }
+                if (hb >= 0) { L.v[hb] = vB; L.w[hb] = wB; }
+                else if (imB != 0.f) { linvel[b] = vB; angvel[b] = wB; }
+            }
+            __syncthreads();
+        }
+        // sweep residual of this tile
+        int sweep = sweep0 + s;
+        if (rmax > 0.f || lmax > 0.f) {
+            atomicMax((unsigned int *)&residual[2 * sweep], __float_as_uint(rmax));
+            atomicMax((unsigned int *)&residual[2 * sweep + 1], __float_as_uint(lmax));
+        }
+    }
+    __syncthreads();
+    for (int h = threadIdx.x; h < SOLVE_SLOTS; h += TB) {
+        int b = L.key[h];
+        if (b >= 0) { linvel[b] = L.v[h]; angvel[b] = L.w[h]; }
+    }
+}
+
+// After a sweep (spatial tiles): decide convergence for the remaining sweeps.
+__global__ void k_sweep_done(float *residual, int sweep, int is_friction, float tol,
+                             int *counters) {
+    if (counters[RB_CNT_DONE] != 0) return;
+    float rmax = residual[2 * sweep], lmax = residual[2 * sweep + 1];
+    float r = rmax / (lmax > 1.f ? lmax : 1.f);
+    int slot = is_friction ? RB_CNT_ITERS_F : RB_CNT_ITERS_N;
+    counters[slot] += 1;
+    ((float *)counters)[is_friction ? RB_CNT_RES_F : RB_CNT_RES_N] = r;
+    if (tol > 0.f && r < tol) counters[RB_CNT_DONE] = 1;
+}
+__global__ void k_clear_done(int *counters) { counters[RB_CNT_DONE] = 0; }
+
+// ---- integration ----
+__global__ __launch_bounds__(256) void k_integrate(int n, float4 *__restrict__ pos,
+                                                   float4 *__restrict__ quat,
+                                                   const float4 *__restrict__ linvel,
+                                                   float4 *__restrict__ angvel,
+                                                   const float *__restrict__ inv_mass, float dt,
+                                                   float damping, float max_ang) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 w = angvel[i];
+    w.x *= damping; w.y *= damping; w.z *= damping;
+    angvel[i] = w;
+    float l2 = (w.x * w.x + w.y * w.y) + w.z * w.z;
+    float ang = sqrtf(l2);
+    if (ang * dt > max_ang) ang = max_ang / dt;
+    float s;
+    if (ang < 0.001f)
+        s = 0.5f * dt - (((dt * dt * dt) * 0.020833333333f) * ang) * ang;
+    else
+        s = sinf((0.5f * ang) * dt) / ang;
+    float ax = w.x * s, ay = w.y * s, az = w.z * s;
+    float dw = cosf((ang * dt) * 0.5f);
+    float4 q = quat[i];
+    float rx = ((dw * q.x + ax * q.w) + ay * q.z) - az * q.y;
+    float ry = ((dw * q.y + ay * q.w) + az * q.x) - ax * q.z;
+    float rz = ((dw * q.z + az * q.w) + ax * q.y) - ay * q.x;
+    float rw = ((dw * q.w - ax * q.x) - ay * q.y) - az * q.z;
+    float len = sqrtf(((rx * rx + ry * ry) + rz * rz) + rw * rw);
+    if (len > 0.f) {
+        float inv = 1.f / len;
+        q = mk4(rx * inv, ry * inv, rz * inv, rw * inv);
+    } else {
+        q = mk4(0.f, 0.f, 0.f, 1.f);
+    }
+    quat[i] = q;
+    float4 v = linvel[i];
+    float4 p = pos[i];
+    p.x = p.x + v.x * dt;
+    p.y = p.y + v.y * dt;
+    p.z = p.z + v.z * dt;
+    pos[i] = p;
+    (void)inv_mass;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- launchers
+hipError_t launch_constraints(const DeviceWorld &d, const StepParams &p, hipStream_t s) {
+    int grid = (p.max_pairs + 255) / 256;
+    hipLaunchKernelGGL(k_constraints, dim3(grid), dim3(256), 0, s, p, d.counters + RB_CNT_PAIRS,
+                       d.has_contact, d.contact_off, d.contact_slots, d.contacts, d.pos, d.linvel,
+                       d.angvel, d.inv_mass, d.inv_inertia_world, d.scene, d.cs, d.tile_kv,
+                       d.counters);
+    return hipGetLastError();
+}
+
+hipError_t launch_contact_total(const DeviceWorld &d, const StepParams &p, hipStream_t s) {
+    hipLaunchKernelGGL(k_contact_total, dim3(1), dim3(1), 0, s,
+                       (const uint32_t *)(d.counters + RB_CNT_CONTACT_TOTAL), d.counters,
+                       p.max_contacts);
+    return hipGetLastError();
+}
+
+hipError_t launch_tile_bounds(const DeviceWorld &d, const StepParams &p, hipStream_t s) {
+    hipLaunchKernelGGL(k_tile_clear, dim3((p.n_tiles + 255) / 256), dim3(256), 0, s, p.n_tiles,
+                       d.tile_first, d.tile_cnt);
+    hipLaunchKernelGGL(k_tile_bounds2, dim3((p.max_contacts + 255) / 256), dim3(256), 0, s,
+                       d.tile_kv, d.counters, d.tile_first, d.tile_cnt);
+    return hipGetLastError();
+}
+
+hipError_t launch_tiles(const DeviceWorld &d, const StepParams &p, hipStream_t s) {
+    hipLaunchKernelGGL(k_tile_color, dim3(p.n_tiles), dim3(TB), 0, s, d.tile_kv, d.tile_first,
+                       d.tile_cnt, d.cs.bodies, d.inv_mass, d.solve_order, d.row_color,
+                       d.tile_color_off, d.tile_ncolors, d.counters);
+    return hipGetLastError();
+}
+
+hipError_t launch_solve(const DeviceWorld &d, const StepParams &p, hipStream_t s) {
+    const int S = p.normal_iters + p.friction_iters;
+    hipLaunchKernelGGL(k_clear_done, dim3(1), dim3(1), 0, s, d.counters);
+    launch_fill_u32(reinterpret_cast<uint32_t *>(d.residual), 0u, (size_t)2 * (S + 1), s);
+    if (p.tile_mode == 1) {
+        if (p.tol > 0.f) {
+            // scene tiles stop together: one launch per sweep so the residual can gate
+            for (int sw = 0; sw < S; sw++) {
+                int fr = sw >= p.normal_iters;
+                hipLaunchKernelGGL(k_tile_solve, dim3(p.n_tiles), dim3(TB), 0, s, p, 0, sw,
+                                   fr ? 0 : 1, fr ? 1 : 0, d.tile_first,
+                                   d.tile_cnt, d.solve_order, d.tile_color_off,
+                                   d.tile_ncolors, d.cs, d.linvel, d.angvel, d.pos, d.inv_mass,
+                                   d.inv_inertia_world, d.residual, d.counters);
+                hipLaunchKernelGGL(k_sweep_done, dim3(1), dim3(1), 0, s, d.residual, sw, fr, p.tol,
+                                   d.counters);
+                if (sw == p.normal_iters - 1)
+                    hipLaunchKernelGGL(k_clear_done, dim3(1), dim3(1), 0, s, d.counters);
+            }
+        } else {
+            hipLaunchKernelGGL(k_tile_solve, dim3(p.n_tiles), dim3(TB), 0, s, p, 0, 0,
+                               p.normal_iters, p.friction_iters, d.tile_first,
+                               d.tile_cnt, d.solve_order, d.tile_color_off,
+                               d.tile_ncolors, d.cs, d.linvel, d.angvel, d.pos, d.inv_mass,
+                               d.inv_inertia_world, d.residual, d.counters);
+            for (int sw = 0; sw < S; sw++)
+                hipLaunchKernelGGL(k_sweep_done, dim3(1), dim3(1), 0, s, d.residual, sw,
+                                   sw >= p.normal_iters, 0.f, d.counters);
+        }
+    } else {
+        for (int sw = 0; sw < S; sw++) {
+            int fr = sw >= p.normal_iters;
+            for (int ph = 0; ph < 9; ph++) {
+                int nx = (p.tiles_x - ph % 3 + 2) / 3, nz = (p.tiles_z - ph / 3 + 2) / 3;
+                if (nx <= 0 || nz <= 0) continue;
+                hipLaunchKernelGGL(k_tile_solve, dim3(nx * nz), dim3(TB), 0, s, p, ph, sw,
+                                   fr ? 0 : 1, fr ? 1 : 0, d.tile_first,
+                                   d.tile_cnt, d.solve_order, d.tile_color_off,
+                                   d.tile_ncolors, d.cs, d.linvel, d.angvel, d.pos, d.inv_mass,
+                                   d.inv_inertia_world, d.residual, d.counters);
+            }
+            hipLaunchKernelGGL(k_sweep_done, dim3(1), dim3(1), 0, s, d.residual, sw, fr, p.tol,
+                               d.counters);
+            if (sw == p.normal_iters - 1)
+                hipLaunchKernelGGL(k_clear_done, dim3(1), dim3(1), 0, s, d.counters);
+        }
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_integrate(const DeviceWorld &d, const StepParams &p, hipStream_t s) {
+    if (p.n_bodies == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_integrate, dim3((p.n_bodies + 255) / 256), dim3(256), 0, s, p.n_bodies,
+                       d.pos, d.quat, d.linvel, d.angvel, d.inv_mass, p.dt, p.damping, p.max_ang);
+    return hipGetLastError();
+}

@@ -1,0 +1,449 @@
+// k_sort.hip -- key/value LSD radix sort, exclusive scan and bound search for
+// gfx950.  Replaces btRadixSort32CL (opencl/broadphase_benchmark/btRadixSort32CL.cpp:161-535,
+// kernels primitives/AdlPrimitives/Sort/RadixSort32Kernels.cl), btPrefixScanCL
+// (btPrefixScanCL.cpp:46-99) and btBoundSearchCL (btBoundSearchCL.cpp:78-136).
+//
+// Sort: one histogram kernel counts every digit of every pass in a single
+// read of the keys; then one "onesweep" scatter kernel per pass ranks a
+// 4096-element tile in LDS (wave ballots give the stable rank of each key
+// among equal digits), obtains the tile's global digit offsets through a
+// decoupled look-back over the preceding tiles, stages the tile in digit
+// order in LDS and writes it out in contiguous runs.  Tiles take their index
+// from an atomic counter in launch order, so every look-back waits only on
+// tiles that are already running.
+#include <hip/hip_runtime.h>
+
+#include "dmath.h"
+#include "rb_internal.h"
+
+namespace {
+
+constexpr int SORT_BLOCK = 256;            // 4 waves
+constexpr int SORT_ITEMS = 16;             // keys per lane
+constexpr int SORT_TILE = SORT_BLOCK * SORT_ITEMS;  // 4096
+constexpr int WAVES = SORT_BLOCK / 64;
+constexpr int MAX_PASSES = 8;
+
+constexpr uint32_t FLAG_AGG = 1u << 30;
+constexpr uint32_t FLAG_INC = 2u << 30;
+constexpr uint32_t VAL_MASK = (1u << 30) - 1u;
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    int lane = __lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+template <int RBITS>
+__global__ __launch_bounds__(SORT_BLOCK) void k_histogram(const uint2 *kv, int n, int passes,
+                                                          int sort_bits, uint32_t *hist,
+                                                          const int *n_dev) {
+    constexpr int RADIX = 1 << RBITS;
+    if (n_dev && *n_dev < n) n = *n_dev;
+    if ((int)blockIdx.x * SORT_TILE >= n) return;
+    __shared__ uint32_t h[MAX_PASSES * RADIX];
+    for (int i = threadIdx.x; i < passes * RADIX; i += SORT_BLOCK) h[i] = 0;
+    __syncthreads();
+    int base = blockIdx.x * SORT_TILE;
+    for (int k = 0; k < SORT_ITEMS; k++) {
+        int i = base + k * SORT_BLOCK + threadIdx.x;
+        if (i < n) {
+            uint32_t key = kv[i].x;
+            for (int p = 0; p < passes; p++) {
+                int shift = p * RBITS;
+                int bits = sort_bits - shift < RBITS ? sort_bits - shift : RBITS;
+                uint32_t d = (key >> shift) & ((1u << bits) - 1u);
+                atomicAdd(&h[p * RADIX + d], 1u);
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < passes * RADIX; i += SORT_BLOCK)
+        if (h[i]) atomicAdd(&hist[i], h[i]);
+}
+
+template <int RBITS>
+__global__ __launch_bounds__(SORT_BLOCK) void k_onesweep(const uint2 *__restrict__ src,
+                                                         uint2 *__restrict__ dst, int n, int shift,
+                                                         int bits, const uint32_t *hist,
+                                                         uint32_t *status, uint32_t *tile_ctr,
+                                                         uint32_t *err, const int *n_dev) {
+    constexpr int RADIX = 1 << RBITS;
+    if (n_dev && *n_dev < n) n = *n_dev;
+    __shared__ uint32_t s_wave_hist[WAVES][RADIX];
+    __shared__ uint32_t s_digit_start[RADIX];   // tile-local start of each digit
+    __shared__ uint32_t s_global[RADIX];        // global base + tile prefix of each digit
+    __shared__ uint2 s_stage[SORT_TILE];
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_scan[RADIX];
+
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6;
+    const int lane = tid & 63;
+    if (tid == 0) s_tile = atomicAdd(tile_ctr, 1u);
+    for (int i = tid; i < WAVES * RADIX; i += SORT_BLOCK) (&s_wave_hist[0][0])[i] = 0;
+    __syncthreads();
+    const int tile = (int)s_tile;
+    if (tile * SORT_TILE >= n) return;  // beyond the device-side count: nothing looks back here
+    const int base = tile * SORT_TILE + wave * (SORT_ITEMS * 64);
+    const uint32_t dmask = (1u << bits) - 1u;
+
+    uint2 item[SORT_ITEMS];
+    uint32_t rank[SORT_ITEMS];
+    for (int r = 0; r < SORT_ITEMS; r++) {
+        int i = base + r * 64 + lane;
+        bool valid = i < n;
+        item[r] = valid ? src[i] : make_uint2(0xffffffffu, 0xffffffffu);
+        uint32_t d = (item[r].x >> shift) & dmask;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < RBITS; b++) {
+            uint64_t bal = __ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? bal : ~bal;
+        }
+        uint32_t below = (uint32_t)__popcll(peers & rb::lanemask_lt());
+        uint32_t before = 0;
+        if (valid) before = s_wave_hist[wave][d];
+        rb::wave_sync();
+        if (valid && below == 0) s_wave_hist[wave][d] = before + (uint32_t)__popcll(peers);
+        rb::wave_sync();
+        rank[r] = before + below;
+    }
+    __syncthreads();
+
+    // per digit: prefix over waves, tile total, publish and look back
+    for (int d = tid; d < RADIX; d += SORT_BLOCK) {
+        uint32_t run = 0;
+        for (int w = 0; w < WAVES; w++) {
+            uint32_t c = s_wave_hist[w][d];
+            s_wave_hist[w][d] = run;
+            run += c;
+        }
+        s_scan[d] = run;
+        uint32_t *st = status + (size_t)tile * RADIX + d;
+        if (tile == 0) {
+            __hip_atomic_store(st, FLAG_INC | run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_global[d] = 0;
+        } else {
+            __hip_atomic_store(st, FLAG_AGG | run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint32_t prefix = 0, spins = 0;
+            int t = tile - 1;
+            while (true) {
+                uint32_t v = __hip_atomic_load(status + (size_t)t * RADIX + d, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+                if ((v & ~VAL_MASK) == 0) {  // predecessor not published yet
+                    if (++spins > (1u << 26)) { atomicOr(err, 1u); break; }
+                    continue;
+                }
+                prefix += v & VAL_MASK;
+                if (v & FLAG_INC) break;
+                t--;
+            }
+            __hip_atomic_store(st, FLAG_INC | (prefix + run), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            s_global[d] = prefix;
+        }
+    }
+    __syncthreads();
+    // tile-local digit starts and the global digit bases of this pass: block
+    // exclusive scans over the digits (one digit per thread)
+    {
+        uint32_t c = tid < RADIX ? s_scan[tid] : 0u;
+        uint32_t g = tid < RADIX ? hist[tid] : 0u;
+        uint32_t ci = wave_incl_scan(c), gi = wave_incl_scan(g);
+        __shared__ uint32_t s_wc[WAVES], s_wg[WAVES];
+        if (lane == 63) { s_wc[wave] = ci; s_wg[wave] = gi; }
+        __syncthreads();
+        uint32_t oc = 0, og = 0;
+        for (int w = 0; w < wave; w++) { oc += s_wc[w]; og += s_wg[w]; }
+        if (tid < RADIX) {
+            s_digit_start[tid] = oc + ci - c;
+            s_global[tid] += og + gi - g;
+        }
+    }
+    __syncthreads();
+    // stage in digit order
+    for (int r = 0; r < SORT_ITEMS; r++) {
+        int i = base + r * 64 + lane;
+        if (i < n) {
+            uint32_t d = (item[r].x >> shift) & dmask;
+            uint32_t lp = s_digit_start[d] + s_wave_hist[wave][d] + rank[r];
+            s_stage[lp] = item[r];
+        }
+    }
+    __syncthreads();
+    int tile_n = n - tile * SORT_TILE;
+    if (tile_n > SORT_TILE) tile_n = SORT_TILE;
+    for (int k = tid; k < tile_n; k += SORT_BLOCK) {
+        uint2 e = s_stage[k];
+        uint32_t d = (e.x >> shift) & dmask;
+        uint32_t g = s_global[d] + ((uint32_t)k - s_digit_start[d]);
+        dst[g] = e;
+    }
+}
+
+// ---- single-pass exclusive scan of uint32 with decoupled look-back ----
+constexpr int SCAN_BLOCK = 256;
+constexpr int SCAN_ITEMS = 16;
+constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
+constexpr uint64_t SFLAG_AGG = 1ull << 62;
+constexpr uint64_t SFLAG_INC = 2ull << 62;
+constexpr uint64_t SVAL_MASK = (1ull << 62) - 1ull;
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan(const uint32_t *in, uint32_t *out, int n,
+                                                     uint32_t *total, uint64_t *status,
+                                                     uint32_t *tile_ctr, const int *n_dev) {
+    if (n_dev && *n_dev < n) n = *n_dev;
+    __shared__ uint32_t s_wave[SCAN_BLOCK / 64];
+    __shared__ uint32_t s_tile;
+    __shared__ uint64_t s_prefix;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) s_tile = atomicAdd(tile_ctr, 1u);
+    __syncthreads();
+    const int tile = (int)s_tile;
+    if (n <= 0) {
+        if (total && tile == 0 && tid == 0) *total = 0;
+        return;
+    }
+    if (tile * SCAN_TILE >= n) return;
+    const int base = tile * SCAN_TILE + tid * SCAN_ITEMS;  // blocked arrangement
+    uint32_t v[SCAN_ITEMS];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        int i = base + k;
+        v[k] = i < n ? in[i] : 0u;
+        sum += v[k];
+    }
+    uint32_t incl = wave_incl_scan(sum);
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    uint32_t wave_off = 0, tile_sum = 0;
+    for (int w = 0; w < SCAN_BLOCK / 64; w++) {
+        uint32_t c = s_wave[w];
+        if (w < wave) wave_off += c;
+        tile_sum += c;
+    }
+    if (tid == 0) {
+        uint64_t *st = status + tile;
+        if (tile == 0) {
+            __hip_atomic_store(st, SFLAG_INC | tile_sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_prefix = 0;
+        } else {
+            __hip_atomic_store(st, SFLAG_AGG | tile_sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint64_t prefix = 0;
+            uint32_t spins = 0;
+            int t = tile - 1;
+            while (true) {
+                uint64_t w = __hip_atomic_load(status + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if ((w & ~SVAL_MASK) == 0) {
+                    if (++spins > (1u << 26)) { atomicOr(tile_ctr + 1, 1u); break; }
+                    continue;
+                }
+                prefix += w & SVAL_MASK;
+                if (w & SFLAG_INC) break;
+                t--;
+            }
+            __hip_atomic_store(st, SFLAG_INC | (prefix + tile_sum), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            s_prefix = prefix;
+        }
+    }
+    __syncthreads();
+    uint32_t run = (uint32_t)s_prefix + wave_off + (incl - sum);
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        int i = base + k;
+        if (i < n) out[i] = run;
+        run += v[k];
+    }
+    int last_tile = (n - 1) / SCAN_TILE;
+    if (total && tile == last_tile && tid == SCAN_BLOCK - 1) *total = run;
+}
+
+__global__ void k_scan_empty(uint32_t *total) { if (total) *total = 0; }
+
+__global__ void k_copy_kv(const uint2 *src, uint2 *dst, int n, const int *n_dev) {
+    if (n_dev && *n_dev < n) n = *n_dev;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+__global__ void k_bound_search(const uint2 *kv, int n, uint32_t *dst, int n_dst, int mode) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    uint32_t prev = i > 0 ? kv[i - 1].x : 0xffffffffu;
+    uint32_t cur = i < n ? kv[i].x : 0xffffffffu;
+    bool start = i < n && (i == 0 || prev != cur);
+    bool end = i > 0 && (i == n || prev != cur);
+    if (mode == 0) {
+        if (start && cur < (uint32_t)n_dst) dst[cur] = (uint32_t)i;
+    } else if (mode == 1) {
+        if (end && prev < (uint32_t)n_dst) dst[prev] = (uint32_t)i;
+    } else {
+        // count = (end index) - (start index), built from two wrapping adds
+        if (start && cur < (uint32_t)n_dst) atomicAdd(&dst[cur], 0u - (uint32_t)i);
+        if (end && prev < (uint32_t)n_dst) atomicAdd(&dst[prev], (uint32_t)i);
+    }
+}
+
+template <int RBITS>
+hipError_t radix_sort_impl(uint2 *kv, int n, int sort_bits, const SortWork &w, hipStream_t s,
+                           const int *n_dev) {
+    constexpr int RADIX = 1 << RBITS;
+    const int passes = (sort_bits + RBITS - 1) / RBITS;
+    const int tiles = (n + SORT_TILE - 1) / SORT_TILE;
+    if (passes > MAX_PASSES || tiles > w.max_tiles) return hipErrorInvalidValue;
+    size_t zero_words = (size_t)passes * RADIX;
+    size_t status_words = (size_t)passes * tiles * RADIX;
+    hipLaunchKernelGGL(k_fill_u32, dim3(64), dim3(256), 0, s, w.hist, 0u, zero_words);
+    hipLaunchKernelGGL(k_fill_u32, dim3(256), dim3(256), 0, s, w.status, 0u, status_words);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, w.tile_ctr, 0u, (size_t)MAX_PASSES);  // the error word after them is sticky
+    hipLaunchKernelGGL(k_histogram<RBITS>, dim3(tiles), dim3(SORT_BLOCK), 0, s, kv, n, passes,
+                       sort_bits, w.hist, n_dev);
+    uint2 *src = kv, *dst = w.tmp;
+    for (int p = 0; p < passes; p++) {
+        int shift = p * RBITS;
+        int bits = sort_bits - shift < RBITS ? sort_bits - shift : RBITS;
+        hipLaunchKernelGGL(k_onesweep<RBITS>, dim3(tiles), dim3(SORT_BLOCK), 0, s, src, dst, n,
+                           shift, bits, w.hist + (size_t)p * RADIX,
+                           w.status + (size_t)p * tiles * RADIX, w.tile_ctr + p,
+                           w.tile_ctr + MAX_PASSES, n_dev);
+        uint2 *t = src; src = dst; dst = t;
+    }
+    if (src != kv)  // odd pass count: copy back with a kernel (graph-capture friendly)
+        hipLaunchKernelGGL(k_copy_kv, dim3(256), dim3(256), 0, s, src, kv, n, n_dev);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+size_t sort_status_words(int max_n, int digit_bits) {
+    int tiles = (max_n + SORT_TILE - 1) / SORT_TILE;
+    if (tiles < 1) tiles = 1;
+    return (size_t)MAX_PASSES * tiles * (1u << digit_bits);
+}
+
+hipError_t launch_radix_sort(uint2 *kv, int n, int sort_bits, int digit_bits, const SortWork &w,
+                             hipStream_t s, const int *n_dev) {
+    if (n <= 1 || sort_bits <= 0) return hipSuccess;
+    switch (digit_bits) {
+        case 4: return radix_sort_impl<4>(kv, n, sort_bits, w, s, n_dev);
+        case 6: return radix_sort_impl<6>(kv, n, sort_bits, w, s, n_dev);
+        case 7: return radix_sort_impl<7>(kv, n, sort_bits, w, s, n_dev);
+        case 8: return radix_sort_impl<8>(kv, n, sort_bits, w, s, n_dev);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_exclusive_scan(const uint32_t *in, uint32_t *out, int n, uint32_t *total,
+                                 uint32_t *status, hipStream_t s, const int *n_dev) {
+    if (n <= 0) {
+        hipLaunchKernelGGL(k_scan_empty, dim3(1), dim3(1), 0, s, total);
+        return hipGetLastError();
+    }
+    int tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    // status: [0] tile counter, then 64-bit words (8-byte aligned)
+    uint32_t *ctr = status;
+    uint64_t *st = reinterpret_cast<uint64_t *>(status + 2);
+    hipLaunchKernelGGL(k_fill_u32, dim3(64), dim3(256), 0, s, status, 0u, (size_t)(2 + 2 * tiles));
+    hipLaunchKernelGGL(k_scan, dim3(tiles), dim3(SCAN_BLOCK), 0, s, in, out, n, total, st, ctr,
+                       n_dev);
+    return hipGetLastError();
+}
+
+hipError_t launch_bound_search(const uint2 *kv, int n, uint32_t *dst, int n_dst, int mode,
+                               hipStream_t s) {
+    if (mode == 2)
+        hipLaunchKernelGGL(k_fill_u32, dim3(256), dim3(256), 0, s, dst, 0u, (size_t)n_dst);
+    hipLaunchKernelGGL(k_bound_search, dim3((n + 1 + 255) / 256), dim3(256), 0, s, kv, n, dst,
+                       n_dst, mode);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    int blocks = (int)((n + 255) / 256);
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- C ABI
+struct RadixSortHandle {
+    SortWork w;
+    uint32_t *scan_status;
+};
+
+extern "C" mi_rb_status mi_rb_radix_sort_create(int max_elements, void **out_sort) {
+    if (!out_sort || max_elements <= 0) return MI_RB_INVALID_ARG;
+    RadixSortHandle *h = new RadixSortHandle();
+    int tiles = (max_elements + SORT_TILE - 1) / SORT_TILE;
+    h->w.max_n = max_elements;
+    h->w.max_tiles = tiles;
+    size_t status_words = sort_status_words(max_elements, 8);
+    int scan_tiles = (max_elements + SCAN_TILE - 1) / SCAN_TILE;
+    if (hipMalloc(&h->w.tmp, sizeof(uint2) * (size_t)max_elements) != hipSuccess ||
+        hipMalloc(&h->w.hist, sizeof(uint32_t) * MAX_PASSES * 256) != hipSuccess ||
+        hipMalloc(&h->w.status, sizeof(uint32_t) * status_words) != hipSuccess ||
+        hipMalloc(&h->w.tile_ctr, sizeof(uint32_t) * (MAX_PASSES + 1)) != hipSuccess ||
+        hipMalloc(&h->scan_status, sizeof(uint32_t) * (size_t)(4 + 2 * scan_tiles)) != hipSuccess) {
+        delete h;
+        return MI_RB_DEVICE;
+    }
+    *out_sort = h;
+    return MI_RB_OK;
+}
+
+extern "C" mi_rb_status mi_rb_radix_sort_destroy(void *sort) {
+    RadixSortHandle *h = static_cast<RadixSortHandle *>(sort);
+    if (!h) return MI_RB_INVALID_ARG;
+    hipFree(h->w.tmp);
+    hipFree(h->w.hist);
+    hipFree(h->w.status);
+    hipFree(h->w.tile_ctr);
+    hipFree(h->scan_status);
+    delete h;
+    return MI_RB_OK;
+}
+
+extern "C" mi_rb_status mi_rb_radix_sort_kv(void *sort, void *d_keyvalue, int n, int sort_bits,
+                                            void *stream) {
+    RadixSortHandle *h = static_cast<RadixSortHandle *>(sort);
+    if (!h || !d_keyvalue || n < 0 || n > h->w.max_n || sort_bits < 0 || sort_bits > 32)
+        return MI_RB_INVALID_ARG;
+    hipError_t e = launch_radix_sort(static_cast<uint2 *>(d_keyvalue), n, sort_bits, 8, h->w,
+                                     static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? MI_RB_OK : MI_RB_DEVICE;
+}
+
+extern "C" mi_rb_status mi_rb_exclusive_scan_u32(void *sort, const void *d_in, void *d_out, int n,
+                                                 void *d_total, void *stream) {
+    RadixSortHandle *h = static_cast<RadixSortHandle *>(sort);
+    if (!h || (!d_in && n) || (!d_out && n) || n < 0 || n > h->w.max_n) return MI_RB_INVALID_ARG;
+    hipError_t e = launch_exclusive_scan(static_cast<const uint32_t *>(d_in),
+                                         static_cast<uint32_t *>(d_out), n,
+                                         static_cast<uint32_t *>(d_total), h->scan_status,
+                                         static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? MI_RB_OK : MI_RB_DEVICE;
+}
+
+extern "C" mi_rb_status mi_rb_bound_search(const void *d_keyvalue, int n, void *d_dst, int n_dst,
+                                           int mode, void *stream) {
+    if ((!d_keyvalue && n) || !d_dst || n < 0 || n_dst <= 0 || mode < 0 || mode > 2)
+        return MI_RB_INVALID_ARG;
+    hipError_t e = launch_bound_search(static_cast<const uint2 *>(d_keyvalue), n,
+                                       static_cast<uint32_t *>(d_dst), n_dst, mode,
+                                       static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? MI_RB_OK : MI_RB_DEVICE;
+}

@@ -1,0 +1,203 @@
+// rb_internal.h -- layouts and launch interfaces shared by the HIP kernels and
+// the host orchestration of the rigid-body step.  gfx950, wave64.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/mi_rb.h"
+
+#define RB_CHECK(expr)                                                          \
+    do {                                                                        \
+        hipError_t e_ = (expr);                                                 \
+        if (e_ != hipSuccess) return rb_device_error(e_, #expr, __FILE__, __LINE__); \
+    } while (0)
+
+mi_rb_status rb_device_error(hipError_t e, const char *what, const char *file, int line);
+
+// Convex hull record, 80 bytes (ConvexPolyhedronCL, opencl/gpu_rigidbody_pipeline2/ConvexPolyhedronCL.h:15-30,
+// plus the local AABB that CLPhysicsDemo keeps beside it).
+struct HullInfo {
+    int face_offset, num_faces, vertex_offset, num_vertices;
+    int edge_offset, num_edges, index_offset, num_indices;
+    float4 center;    // vertex average (CLPhysicsDemo.cpp:380-385)
+    float4 aabb_min;  // local AABB of the vertices
+    float4 aabb_max;
+};
+static_assert(sizeof(HullInfo) == 80, "HullInfo layout");
+
+// Contact manifold in the Contact4 layout (dynamics/basic_demo/Stubs/AdlContact4.h:26-59).
+struct Contact4 {
+    float4 pos[4];    // xyz point, w depth
+    float4 normal;    // xyz normal, w = number of points
+    uint32_t coeffs;  // restitution | friction << 16, 16-bit fixed point
+    int pair_index;   // m_batchIdx
+    int body_a;       // negated when static
+    int body_b;
+};
+static_assert(sizeof(Contact4) == 96, "Contact4 layout");
+
+// Limits of the wave-per-pair narrowphase.
+#define RB_MAX_HULL_VERTS 64
+#define RB_MAX_HULL_FACES 64
+#define RB_MAX_FACE_VERTS 64
+#define RB_MAX_HULL_EDGES 128
+#define RB_MAX_POLY 64
+
+// Solver tiles
+#define RB_TILE_MAX_ROWS 2048     // constraints per tile held by one workgroup
+#define RB_TILE_MAX_BODIES 1024   // distinct bodies per tile cached in LDS
+#define RB_MAX_LOCAL_COLORS 64    // colours per tile
+
+// Constraint rows, structure of arrays (fields of Constraint4,
+// dynamics/basic_demo/Stubs/AdlConstraint4.h:23-47).
+struct ConstraintSoA {
+    float4 *linear;     // -normal, w friction
+    float4 *pos;        // 4 per constraint
+    float4 *center;
+    float4 *jac_inv;    // 4 rows
+    float4 *b;
+    float4 *lambda;
+    float4 *fric;       // x,y f_jac_inv; z,w f_lambda
+    int2 *bodies;       // (a, b) absolute indices
+    int *pair;          // pair index (m_batchIdx)
+};
+
+// Device state of one world.
+struct DeviceWorld {
+    // shapes
+    HullInfo *hulls;
+    float4 *verts, *planes, *edges;
+    int2 *face_index;
+    int *indices;
+    // bodies
+    float4 *pos, *quat, *linvel, *angvel;
+    float *inv_mass;
+    int *shape, *scene, *is_large;
+    float4 *inv_inertia_local;   // diagonal
+    float4 *inv_inertia_world;   // 3 rows per body
+    int *scene_start, *scene_count;
+    int *large_list;             // indices of large bodies (overflow list), ascending
+    int *large_off;              // [n_large+1] offsets of their pair lists in large_slots
+    int *large_first, *large_num;// per scene: first position in large_list, count
+    int *large_index;            // per body: position in large_list, or -1
+    // broadphase
+    float4 *aabb_min, *aabb_max;
+    uint2 *kv, *kv_tmp;
+    uint32_t *cell_start;
+    int *body_slots;             // max_pairs_per_body per body
+    int *large_slots;
+    uint32_t *pair_count, *pair_off;
+    int2 *pairs;
+    // narrowphase
+    Contact4 *contact_slots;     // one per pair
+    uint32_t *has_contact, *contact_off;
+    Contact4 *contacts;          // compacted, ordered by pair
+    // solver
+    ConstraintSoA cs;
+    uint2 *tile_kv;              // (owner tile, constraint), sorted by tile
+    int *tile_first, *tile_cnt;  // per tile: first sorted position, end (one past last)
+    int *solve_order;            // constraints by tile, colour-major within a tile
+    int *row_color;              // local colour per constraint
+    int *tile_color_off;         // per tile RB_MAX_LOCAL_COLORS+1 offsets (relative)
+    int *tile_ncolors;
+    float *residual;             // per sweep: max scaled dlambda, max lambda (as uint bits)
+    // warm start
+    int2 *prev_pairs;            // previous step's contact body pairs, sorted
+    float4 *prev_lambda;         // previous step's normal lambdas
+    float4 *prev_fric;
+    int *prev_count;
+    uint16_t *mlp_w;
+    float *mlp_b;
+    // counters (device scalars)
+    int *counters;               // see RB_CNT_*
+};
+
+enum {
+    RB_CNT_PAIRS = 0,
+    RB_CNT_CONTACTS = 1,
+    RB_CNT_OVERFLOW = 2,
+    RB_CNT_NCOLORS = 3,
+    RB_CNT_ITERS_N = 4,
+    RB_CNT_ITERS_F = 5,
+    RB_CNT_RES_N = 6,
+    RB_CNT_RES_F = 7,
+    RB_CNT_SORT_TILE = 8,     // uint32 total of the pair-count scan
+    RB_CNT_CONTACT_TOTAL = 9, // uint32 total of the contact scan
+    RB_CNT_DONE = 12,         // solver converged flag for the remaining sweeps
+    RB_CNT_COUNT = 16
+};
+
+// Step parameters passed by value to kernels.
+struct StepParams {
+    int n_bodies;
+    int n_scenes;
+    int n_large;
+    float dt;
+    float solver_dt;
+    float gravity[3];
+    float inv_cell;
+    int dims[3];
+    int cell_bits;
+    int max_pairs;
+    int max_pairs_per_body;
+    int max_contacts;
+    float friction;
+    float baumgarte;
+    float drift;
+    float threshold;
+    float damping;
+    float max_ang;
+    // solver tiling
+    float tile_inv;
+    int tiles_x, tiles_z;
+    float tile_origin_x, tile_origin_z;
+    int n_tiles;
+    int tile_mode;       // 0: spatial tiles (9 phases per sweep), 1: one tile per scene
+    int normal_iters, friction_iters;
+    float tol;
+    int warm_start;
+};
+
+// ---- launch interfaces (implemented in the .hip files) ----
+struct SortWork {
+    uint2 *tmp;           // n elements
+    uint32_t *hist;       // passes x radix counters
+    uint32_t *status;     // look-back words: passes x tiles x radix
+    uint32_t *tile_ctr;   // passes counters
+    uint32_t *scan_status;// scan look-back words
+    int max_n;
+    int max_tiles;
+};
+size_t sort_status_words(int max_n, int digit_bits);
+// n is the capacity; when n_dev is given the kernels read the element count
+// from device memory (min(n, *n_dev)), so the launch needs no host sync.
+hipError_t launch_radix_sort(uint2 *kv, int n, int sort_bits, int digit_bits, const SortWork &w,
+                             hipStream_t s, const int *n_dev = nullptr);
+hipError_t launch_exclusive_scan(const uint32_t *in, uint32_t *out, int n, uint32_t *total,
+                                 uint32_t *status, hipStream_t s, const int *n_dev = nullptr);
+hipError_t launch_bound_search(const uint2 *kv, int n, uint32_t *dst, int n_dst, int mode,
+                               hipStream_t s);
+
+hipError_t launch_setup(const DeviceWorld &d, const StepParams &p, hipStream_t s);
+hipError_t launch_cell_start(const DeviceWorld &d, const StepParams &p, hipStream_t s);
+hipError_t launch_find_pairs(const DeviceWorld &d, const StepParams &p, hipStream_t s);
+hipError_t launch_emit_pairs(const DeviceWorld &d, const StepParams &p, hipStream_t s);
+hipError_t launch_narrowphase(const DeviceWorld &d, const StepParams &p, hipStream_t s);
+hipError_t launch_constraints(const DeviceWorld &d, const StepParams &p, hipStream_t s);
+hipError_t launch_contact_total(const DeviceWorld &d, const StepParams &p, hipStream_t s);
+hipError_t launch_tile_bounds(const DeviceWorld &d, const StepParams &p, hipStream_t s);
+hipError_t launch_tiles(const DeviceWorld &d, const StepParams &p, hipStream_t s);
+hipError_t launch_solve(const DeviceWorld &d, const StepParams &p, hipStream_t s);
+hipError_t launch_integrate(const DeviceWorld &d, const StepParams &p, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
+
+// generic kernels shared by the grid API and the world
+hipError_t launch_aabb_hash(const float4 *mn, const float4 *mx, const int *is_large, int n,
+                            float inv_cell, const int dims[3], uint2 *kv, hipStream_t s);
+hipError_t launch_grid_pairs(const float4 *mn, const float4 *mx, const uint2 *kv,
+                             uint32_t *cell_start, const int *large_list, int n_large, int n,
+                             float inv_cell, const int dims[3], int cell_bits, int K,
+                             int *body_slots, uint32_t *pair_count, int *overflow, hipStream_t s);
+hipError_t launch_cell_start_generic(const uint2 *kv, int n, uint32_t *cell_start, hipStream_t s);
+
+int rb_num_cus();

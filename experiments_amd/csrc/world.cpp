@@ -1,0 +1,1028 @@
+// world.cpp -- host side of the rigid-body step: the C ABI of mi_rb.h.
+//
+// Mirrors CLPhysicsDemo (opencl/gpu_rigidbody_pipeline2/CLPhysicsDemo.cpp) and the
+// registry of btGpuNarrowphaseAndSolver (opencl/gpu_rigidbody_pipeline/btGpuNarrowPhaseAndSolver.cpp:
+// registerConvexHullShape :423-490, registerRigidBody :566-657): shapes and bodies are
+// registered on the host, uploaded once, and each step is a fixed sequence of
+// kernel launches on one HIP stream with no host synchronisation inside, so it
+// can be captured once in a hipGraph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "hull.h"
+#include "rb_internal.h"
+
+mi_rb_status rb_device_error(hipError_t e, const char *what, const char *file, int line) {
+    if (getenv("MI_RB_VERBOSE"))
+        fprintf(stderr, "mi_rb: %s failed: %s (%s:%d)\n", what, hipGetErrorString(e), file, line);
+    return MI_RB_DEVICE;
+}
+
+int rb_num_cus() {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        hipGetDevice(&dev);
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
+        if (cus <= 0) cus = 256;
+    }
+    return cus;
+}
+
+namespace {
+
+template <typename T>
+hipError_t dalloc(T **p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    return hipMalloc((void **)p, sizeof(T) * n);
+}
+
+int next_pow2(int v) {
+    int p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+int ilog2(uint64_t v) {
+    int b = 0;
+    while ((1ull << b) < v) b++;
+    return b;
+}
+
+__global__ void k_zero_counters(int *c) {
+    int i = threadIdx.x;
+    if (i < RB_CNT_COUNT && i != RB_CNT_OVERFLOW) c[i] = 0;
+}
+
+struct World {
+    mi_rb_config cfg;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool uploaded = false;
+    bool use_graph = false;
+    hipGraphExec_t graph = nullptr;
+    hipEvent_t ev[10] = {};
+    bool timing = true;
+
+    // host shapes
+    std::vector<HullInfo> hulls;
+    std::vector<float4> verts, planes, edges;
+    std::vector<int2> face_index;
+    std::vector<int> indices;
+    std::vector<float> hull_hext_norm;  // |local AABB half extent| per shape
+
+    // host bodies
+    std::vector<float4> pos, quat, linvel, angvel, inv_inertia_local;
+    std::vector<float> inv_mass;
+    std::vector<int> shape, scene;
+
+    // derived
+    StepParams sp{};
+    int key_bits = 0, tile_bits = 0;
+    std::vector<int> is_large, large_list;
+    DeviceWorld d{};
+    SortWork sw{};
+    uint32_t *scan_status = nullptr;
+    long long steps = 0;
+    float last_ms = 0.f;
+    std::vector<void *> allocs;
+
+    template <typename T>
+    hipError_t alloc(T **p, size_t n) {
+        hipError_t e = dalloc(p, n);
+        if (e == hipSuccess) allocs.push_back((void *)*p);
+        return e;
+    }
+    void free_device() {
+        if (graph) { hipGraphExecDestroy(graph); graph = nullptr; }
+        for (void *p : allocs) hipFree(p);
+        allocs.clear();
+        d = DeviceWorld();
+        uploaded = false;
+    }
+};
+
+World *W(void *w) { return static_cast<World *>(w); }
+
+// inertia at registration (btGpuNarrowPhaseAndSolver.cpp:566-657 with the
+// world AABB of CLPhysicsDemo::registerPhysicsInstance, CLPhysicsDemo.cpp:536-552)
+float4 local_inv_inertia(const World &w, float mass, int shape, const float q[4]) {
+    if (mass == 0.f) return make_float4(0, 0, 0, 0);
+    const HullInfo &h = w.hulls[shape];
+    float hx = (h.aabb_max.x - h.aabb_min.x) * 0.5f, hy = (h.aabb_max.y - h.aabb_min.y) * 0.5f,
+          hz = (h.aabb_max.z - h.aabb_min.z) * 0.5f;
+    float lx, ly, lz;
+    if (w.cfg.inertia_mode == MI_RB_INERTIA_BOX) {
+        lx = 2.f * hx; ly = 2.f * hy; lz = 2.f * hz;
+    } else {
+        // btTransformAabb with margin 0.01 and the body's orientation, then
+        // "fake larger inertia": l = 2 * (aabbMax - aabbMin)
+        float margin = 0.01f;
+        float ex = hx + margin, ey = hy + margin, ez = hz + margin;
+        float x = q[0], y = q[1], z = q[2], s = q[3];
+        float m[3][3] = {{1 - 2 * (y * y + z * z), 2 * (x * y - s * z), 2 * (x * z + s * y)},
+                         {2 * (x * y + s * z), 1 - 2 * (x * x + z * z), 2 * (y * z - s * x)},
+                         {2 * (x * z - s * y), 2 * (y * z + s * x), 1 - 2 * (x * x + y * y)}};
+        float e[3];
+        for (int r = 0; r < 3; r++)
+            e[r] = std::fabs(m[r][0]) * ex + std::fabs(m[r][1]) * ey + std::fabs(m[r][2]) * ez;
+        lx = 2.f * (2.f * e[0]); ly = 2.f * (2.f * e[1]); lz = 2.f * (2.f * e[2]);
+    }
+    float ix = (mass / 12.0f) * (ly * ly + lz * lz);
+    float iy = (mass / 12.0f) * (lx * lx + lz * lz);
+    float iz = (mass / 12.0f) * (lx * lx + ly * ly);
+    return make_float4(1.f / ix, 1.f / iy, 1.f / iz, 0.f);
+}
+
+mi_rb_status derive_and_upload(World &w) {
+    const int n = (int)w.pos.size();
+    const int S = (int)w.hulls.size();
+    if (n == 0 || S == 0) return MI_RB_STATE;
+    w.free_device();
+    StepParams &p = w.sp;
+    p = StepParams();
+    p.n_bodies = n;
+    int n_scenes = 0;
+    for (int i = 0; i < n; i++) n_scenes = std::max(n_scenes, w.scene[i] + 1);
+    p.n_scenes = n_scenes;
+    std::vector<int> scene_start(n_scenes, 0), scene_count(n_scenes, 0);
+    for (int i = n - 1; i >= 0; i--) scene_start[w.scene[i]] = i;
+    for (int i = 0; i < n; i++) scene_count[w.scene[i]]++;
+    // cell size: 2 x the largest rotated-AABB half extent of a dynamic body
+    float hmax = 0.f;
+    for (int i = 0; i < n; i++)
+        if (w.inv_mass[i] != 0.f) hmax = std::max(hmax, w.hull_hext_norm[w.shape[i]]);
+    if (hmax == 0.f)
+        for (int i = 0; i < n; i++) hmax = std::max(hmax, w.hull_hext_norm[w.shape[i]]);
+    float cell = w.cfg.cell_size > 0.f ? w.cfg.cell_size : 2.f * hmax;
+    p.inv_cell = 1.0f / cell;
+    // large bodies: rotated AABB may exceed half a cell
+    w.is_large.assign(n, 0);
+    w.large_list.clear();
+    for (int i = 0; i < n; i++)
+        if (w.hull_hext_norm[w.shape[i]] > 0.5f * cell) { w.is_large[i] = 1; w.large_list.push_back(i); }
+    p.n_large = (int)w.large_list.size();
+    // grid: power-of-two dims covering the (per-scene) extent of the small bodies
+    float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+    std::vector<float> slo(3 * (size_t)n_scenes, 1e30f), shi(3 * (size_t)n_scenes, -1e30f);
+    for (int i = 0; i < n; i++) {
+        if (w.is_large[i]) continue;
+        float c[3] = {w.pos[i].x, w.pos[i].y, w.pos[i].z};
+        for (int k = 0; k < 3; k++) {
+            lo[k] = std::min(lo[k], c[k]); hi[k] = std::max(hi[k], c[k]);
+            slo[3 * w.scene[i] + k] = std::min(slo[3 * w.scene[i] + k], c[k]);
+            shi[3 * w.scene[i] + k] = std::max(shi[3 * w.scene[i] + k], c[k]);
+        }
+    }
+    float ext[3] = {0, 0, 0};
+    for (int s = 0; s < n_scenes; s++)
+        for (int k = 0; k < 3; k++)
+            if (shi[3 * s + k] >= slo[3 * s + k]) ext[k] = std::max(ext[k], shi[3 * s + k] - slo[3 * s + k]);
+    int dims[3];
+    for (int k = 0; k < 3; k++) {
+        int want = (int)std::ceil(ext[k] / cell) + 3;
+        dims[k] = w.cfg.grid_dims[k] > 0 ? w.cfg.grid_dims[k] : std::min(std::max(next_pow2(want), 4), 512);
+        if (dims[k] & (dims[k] - 1)) return MI_RB_INVALID_ARG;
+    }
+    int scene_bits = ilog2((uint64_t)n_scenes);
+    while (ilog2((uint64_t)dims[0] * dims[1] * dims[2]) + scene_bits > 30) {
+        int k = dims[0] >= dims[1] && dims[0] >= dims[2] ? 0 : (dims[1] >= dims[2] ? 1 : 2);
+        dims[k] >>= 1;
+    }
+    for (int k = 0; k < 3; k++) p.dims[k] = dims[k];
+    p.cell_bits = ilog2((uint64_t)dims[0] * dims[1] * dims[2]);
+    // one spare key bit: overflow-list bodies carry key 0xffffffff, whose low
+    // key_bits bits then exceed every cell key, so they sort last
+    w.key_bits = p.cell_bits + scene_bits + 1;
+    // parameters
+    const mi_rb_config &c = w.cfg;
+    p.dt = c.dt;
+    p.solver_dt = c.solver_dt > 0.f ? c.solver_dt : c.dt;
+    for (int k = 0; k < 3; k++) p.gravity[k] = c.gravity[k];
+    p.max_pairs = c.max_pairs;
+    p.max_pairs_per_body = c.max_pairs_per_body;
+    p.max_contacts = c.max_contacts;
+    p.friction = c.friction;
+    p.baumgarte = c.baumgarte;
+    p.drift = c.position_drift;
+    p.threshold = c.contact_threshold;
+    p.damping = c.angular_damping;
+    p.max_ang = c.max_angular_motion;
+    p.normal_iters = c.solver_iterations;
+    p.friction_iters = c.solver_iterations;
+    p.tol = c.residual_tolerance;
+    p.warm_start = c.warm_start;
+    // solver tiles
+    if (n_scenes > 1) {
+        p.tile_mode = 1;
+        p.n_tiles = n_scenes;
+        p.tiles_x = n_scenes;
+        p.tiles_z = 1;
+        p.tile_inv = 0.f;
+    } else {
+        p.tile_mode = 0;
+        float T = c.tile_size > 0.f ? c.tile_size : 2.f * cell;
+        p.tile_inv = 1.0f / T;
+        p.tile_origin_x = lo[0] - T;
+        p.tile_origin_z = lo[2] - T;
+        p.tiles_x = std::max(1, (int)std::ceil((hi[0] - lo[0]) / T) + 3);
+        p.tiles_z = std::max(1, (int)std::ceil((hi[2] - lo[2]) / T) + 3);
+        p.n_tiles = p.tiles_x * p.tiles_z;
+    }
+    w.tile_bits = std::max(1, ilog2((uint64_t)p.n_tiles));
+
+    // ---- device buffers ----
+    DeviceWorld &d = w.d;
+    const int maxP = c.max_pairs, maxC = c.max_contacts, K = c.max_pairs_per_body;
+    const size_t ncells = (size_t)1 << (w.key_bits - 1);
+    RB_CHECK(w.alloc(&d.hulls, S));
+    RB_CHECK(w.alloc(&d.verts, w.verts.size()));
+    RB_CHECK(w.alloc(&d.planes, w.planes.size()));
+    RB_CHECK(w.alloc(&d.edges, w.edges.size()));
+    RB_CHECK(w.alloc(&d.face_index, w.face_index.size()));
+    RB_CHECK(w.alloc(&d.indices, w.indices.size()));
+    RB_CHECK(w.alloc(&d.pos, n));
+    RB_CHECK(w.alloc(&d.quat, n));
+    RB_CHECK(w.alloc(&d.linvel, n));
+    RB_CHECK(w.alloc(&d.angvel, n));
+    RB_CHECK(w.alloc(&d.inv_mass, n));
+    RB_CHECK(w.alloc(&d.shape, n));
+    RB_CHECK(w.alloc(&d.scene, n));
+    RB_CHECK(w.alloc(&d.is_large, n));
+    RB_CHECK(w.alloc(&d.inv_inertia_local, n));
+    RB_CHECK(w.alloc(&d.inv_inertia_world, 3 * (size_t)n));
+    RB_CHECK(w.alloc(&d.scene_start, n_scenes));
+    RB_CHECK(w.alloc(&d.scene_count, n_scenes));
+    RB_CHECK(w.alloc(&d.large_list, p.n_large + 1));
+    RB_CHECK(w.alloc(&d.large_off, p.n_large + 1));
+    RB_CHECK(w.alloc(&d.large_first, n_scenes));
+    RB_CHECK(w.alloc(&d.large_num, n_scenes));
+    RB_CHECK(w.alloc(&d.large_index, n));
+    RB_CHECK(w.alloc(&d.aabb_min, n));
+    RB_CHECK(w.alloc(&d.aabb_max, n));
+    RB_CHECK(w.alloc(&d.kv, n));
+    RB_CHECK(w.alloc(&d.cell_start, ncells));
+    RB_CHECK(w.alloc(&d.body_slots, (size_t)n * K));
+    RB_CHECK(w.alloc(&d.pair_count, n));
+    RB_CHECK(w.alloc(&d.pair_off, n));
+    RB_CHECK(w.alloc(&d.pairs, maxP));
+    RB_CHECK(w.alloc(&d.contact_slots, maxP));
+    RB_CHECK(w.alloc(&d.has_contact, maxP));
+    RB_CHECK(w.alloc(&d.contact_off, maxP));
+    RB_CHECK(w.alloc(&d.contacts, maxC));
+    RB_CHECK(w.alloc(&d.cs.linear, maxC));
+    RB_CHECK(w.alloc(&d.cs.pos, 4 * (size_t)maxC));
+    RB_CHECK(w.alloc(&d.cs.center, maxC));
+    RB_CHECK(w.alloc(&d.cs.jac_inv, maxC));
+    RB_CHECK(w.alloc(&d.cs.b, maxC));
+    RB_CHECK(w.alloc(&d.cs.lambda, maxC));
+    RB_CHECK(w.alloc(&d.cs.fric, maxC));
+    RB_CHECK(w.alloc(&d.cs.bodies, maxC));
+    RB_CHECK(w.alloc(&d.cs.pair, maxC));
+    RB_CHECK(w.alloc(&d.tile_kv, maxC));
+    RB_CHECK(w.alloc(&d.tile_first, p.n_tiles));
+    RB_CHECK(w.alloc(&d.tile_cnt, p.n_tiles));
+    RB_CHECK(w.alloc(&d.solve_order, maxC));
+    RB_CHECK(w.alloc(&d.row_color, maxC));
+    RB_CHECK(w.alloc(&d.tile_color_off, (size_t)p.n_tiles * (RB_MAX_LOCAL_COLORS + 1)));
+    RB_CHECK(w.alloc(&d.tile_ncolors, p.n_tiles));
+    RB_CHECK(w.alloc(&d.residual, 2 * 1024));
+    RB_CHECK(w.alloc(&d.counters, RB_CNT_COUNT));
+    // large-body pair lists: each may pair with every earlier body of its scene
+    std::vector<int> large_off(p.n_large + 1, 0), large_index(n, -1);
+    std::vector<int> large_first(n_scenes, 0), large_num(n_scenes, 0);
+    for (int q = 0; q < p.n_large; q++) {
+        int x = w.large_list[q];
+        large_index[x] = q;
+        large_off[q + 1] = large_off[q] + (x - scene_start[w.scene[x]]);
+        large_num[w.scene[x]]++;
+    }
+    for (int s = 0, run = 0; s < n_scenes; s++) { large_first[s] = run; run += large_num[s]; }
+    RB_CHECK(w.alloc(&d.large_slots, (size_t)large_off[p.n_large] + 1));
+    // sort / scan work
+    int sort_cap = std::max(n, maxC);
+    w.sw.max_n = sort_cap;
+    w.sw.max_tiles = (sort_cap + 4095) / 4096;
+    RB_CHECK(w.alloc(&w.sw.tmp, sort_cap));
+    RB_CHECK(w.alloc(&w.sw.hist, 8 * 256));
+    RB_CHECK(w.alloc(&w.sw.status, sort_status_words(sort_cap, 8)));
+    RB_CHECK(w.alloc(&w.sw.tile_ctr, 16));
+    int scan_cap = std::max(n, maxP);
+    RB_CHECK(w.alloc(&w.scan_status, 4 + 2 * (size_t)((scan_cap + 4095) / 4096)));
+
+    // ---- uploads ----
+    hipStream_t s = w.stream;
+    auto up = [&](void *dst, const void *src, size_t bytes) {
+        return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
+    };
+    RB_CHECK(up(d.hulls, w.hulls.data(), sizeof(HullInfo) * S));
+    RB_CHECK(up(d.verts, w.verts.data(), sizeof(float4) * w.verts.size()));
+    RB_CHECK(up(d.planes, w.planes.data(), sizeof(float4) * w.planes.size()));
+    if (!w.edges.empty()) RB_CHECK(up(d.edges, w.edges.data(), sizeof(float4) * w.edges.size()));
+    RB_CHECK(up(d.face_index, w.face_index.data(), sizeof(int2) * w.face_index.size()));
+    RB_CHECK(up(d.indices, w.indices.data(), sizeof(int) * w.indices.size()));
+    RB_CHECK(up(d.pos, w.pos.data(), sizeof(float4) * n));
+    RB_CHECK(up(d.quat, w.quat.data(), sizeof(float4) * n));
+    RB_CHECK(up(d.linvel, w.linvel.data(), sizeof(float4) * n));
+    RB_CHECK(up(d.angvel, w.angvel.data(), sizeof(float4) * n));
+    RB_CHECK(up(d.inv_mass, w.inv_mass.data(), sizeof(float) * n));
+    RB_CHECK(up(d.shape, w.shape.data(), sizeof(int) * n));
+    RB_CHECK(up(d.scene, w.scene.data(), sizeof(int) * n));
+    RB_CHECK(up(d.is_large, w.is_large.data(), sizeof(int) * n));
+    RB_CHECK(up(d.inv_inertia_local, w.inv_inertia_local.data(), sizeof(float4) * n));
+    RB_CHECK(up(d.scene_start, scene_start.data(), sizeof(int) * n_scenes));
+    RB_CHECK(up(d.scene_count, scene_count.data(), sizeof(int) * n_scenes));
+    if (p.n_large) RB_CHECK(up(d.large_list, w.large_list.data(), sizeof(int) * p.n_large));
+    RB_CHECK(up(d.large_off, large_off.data(), sizeof(int) * (p.n_large + 1)));
+    RB_CHECK(up(d.large_first, large_first.data(), sizeof(int) * n_scenes));
+    RB_CHECK(up(d.large_num, large_num.data(), sizeof(int) * n_scenes));
+    RB_CHECK(up(d.large_index, large_index.data(), sizeof(int) * n));
+    RB_CHECK(hipMemsetAsync(d.cell_start, 0xff, sizeof(uint32_t) * ncells, s));
+    RB_CHECK(hipMemsetAsync(d.counters, 0, sizeof(int) * RB_CNT_COUNT, s));
+    RB_CHECK(hipMemsetAsync(d.inv_inertia_world, 0, sizeof(float4) * 3 * n, s));
+    RB_CHECK(hipMemsetAsync(d.pair_count, 0, sizeof(uint32_t) * n, s));
+    RB_CHECK(hipMemsetAsync(d.has_contact, 0, sizeof(uint32_t) * maxP, s));
+    RB_CHECK(hipStreamSynchronize(s));
+    w.uploaded = true;
+    return MI_RB_OK;
+}
+
+// Enqueue one step, stopping after `last_stage`.
+mi_rb_status enqueue_step(World &w, int last_stage) {
+    DeviceWorld &d = w.d;
+    const StepParams &p = w.sp;
+    hipStream_t s = w.stream;
+    const bool ev = w.timing && !w.use_graph;
+    if (ev) RB_CHECK(hipEventRecord(w.ev[0], s));
+    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, d.counters);
+    RB_CHECK(launch_setup(d, p, s));
+    if (ev) RB_CHECK(hipEventRecord(w.ev[1], s));
+    if (last_stage < MI_RB_STAGE_SORT) return MI_RB_OK;
+    RB_CHECK(launch_radix_sort(d.kv, p.n_bodies, w.key_bits, 8, w.sw, s));
+    if (ev) RB_CHECK(hipEventRecord(w.ev[2], s));
+    if (last_stage < MI_RB_STAGE_PAIRS) return MI_RB_OK;
+    RB_CHECK(launch_cell_start(d, p, s));
+    RB_CHECK(launch_find_pairs(d, p, s));
+    RB_CHECK(launch_exclusive_scan(d.pair_count, d.pair_off, p.n_bodies,
+                                   (uint32_t *)(d.counters + RB_CNT_SORT_TILE), w.scan_status, s));
+    RB_CHECK(launch_emit_pairs(d, p, s));
+    if (ev) RB_CHECK(hipEventRecord(w.ev[3], s));
+    if (last_stage < MI_RB_STAGE_NARROWPHASE) return MI_RB_OK;
+    RB_CHECK(launch_narrowphase(d, p, s));
+    if (ev) RB_CHECK(hipEventRecord(w.ev[4], s));
+    if (last_stage < MI_RB_STAGE_CONSTRAINTS) return MI_RB_OK;
+    RB_CHECK(launch_exclusive_scan(d.has_contact, d.contact_off, p.max_pairs,
+                                   (uint32_t *)(d.counters + RB_CNT_CONTACT_TOTAL), w.scan_status,
+                                   s, d.counters + RB_CNT_PAIRS));
+    RB_CHECK(launch_contact_total(d, p, s));
+    RB_CHECK(launch_constraints(d, p, s));
+    if (ev) RB_CHECK(hipEventRecord(w.ev[5], s));
+    if (last_stage < MI_RB_STAGE_COLOR) return MI_RB_OK;
+    RB_CHECK(launch_radix_sort(d.tile_kv, p.max_contacts, w.tile_bits, 8, w.sw, s,
+                               d.counters + RB_CNT_CONTACTS));
+    RB_CHECK(launch_tile_bounds(d, p, s));
+    RB_CHECK(launch_tiles(d, p, s));
+    if (ev) RB_CHECK(hipEventRecord(w.ev[6], s));
+    if (last_stage < MI_RB_STAGE_SOLVE) return MI_RB_OK;
+    RB_CHECK(launch_solve(d, p, s));
+    if (ev) RB_CHECK(hipEventRecord(w.ev[7], s));
+    if (last_stage < MI_RB_STAGE_INTEGRATE) return MI_RB_OK;
+    RB_CHECK(launch_integrate(d, p, s));
+    if (ev) RB_CHECK(hipEventRecord(w.ev[8], s));
+    return MI_RB_OK;
+}
+
+}  // namespace
+
+// ================================================================ C ABI
+extern "C" void mi_rb_config_reference(mi_rb_config *c) {
+    if (!c) return;
+    std::memset(c, 0, sizeof(*c));
+    c->max_bodies = 1 << 17;
+    c->max_shapes = 8192;
+    c->max_hull_vertices = 1 << 20;
+    c->max_pairs = 1 << 21;
+    c->max_contacts = 1 << 20;
+    c->max_pairs_per_body = 31;
+    c->dt = 0.0166666f;           // integrateKernel.cl:114, broadphaseKernel.cl:172
+    c->solver_dt = 1.0f / 60.0f;  // btGpuNarrowPhaseAndSolver.cpp:1082
+    c->gravity[1] = -9.8f;
+    c->solver_iterations = 4;     // btGpuNarrowPhaseAndSolver.cpp:1330
+    c->friction = 0.7f;
+    c->baumgarte = 0.2f;
+    c->position_drift = 0.005f;
+    c->contact_threshold = 0.02f;
+    c->angular_damping = 0.99f;
+    c->max_angular_motion = 0.25f * 3.14159254f;
+    c->warm_start = MI_RB_WARM_ZERO;
+    c->inertia_mode = MI_RB_INERTIA_REFERENCE;
+}
+
+extern "C" mi_rb_status mi_rb_create(const mi_rb_config *cfg, void **out_world) {
+    if (!cfg || !out_world) return MI_RB_INVALID_ARG;
+    if (cfg->max_bodies <= 0 || cfg->max_pairs <= 0 || cfg->max_contacts <= 0 ||
+        cfg->max_pairs_per_body <= 0 || cfg->max_pairs_per_body > 64 || cfg->dt <= 0.f ||
+        cfg->solver_iterations < 0 || cfg->solver_iterations > 256)
+        return MI_RB_INVALID_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= cfg->device) return MI_RB_DEVICE;
+    World *w = new World();
+    w->cfg = *cfg;
+    w->device = cfg->device;
+    if (hipSetDevice(w->device) != hipSuccess || hipStreamCreate(&w->stream) != hipSuccess) {
+        delete w;
+        return MI_RB_DEVICE;
+    }
+    for (auto &e : w->ev) hipEventCreate(&e);
+    *out_world = w;
+    return MI_RB_OK;
+}
+
+extern "C" mi_rb_status mi_rb_destroy(void *world) {
+    World *w = W(world);
+    if (!w) return MI_RB_INVALID_ARG;
+    hipSetDevice(w->device);
+    if (w->stream) hipStreamSynchronize(w->stream);
+    w->free_device();
+    for (auto &e : w->ev) if (e) hipEventDestroy(e);
+    if (w->stream) hipStreamDestroy(w->stream);
+    delete w;
+    return MI_RB_OK;
+}
+
+extern "C" mi_rb_status mi_rb_register_convex_hull(void *world, const float *xyz, int stride_bytes,
+                                                   int n, const float scale[3], int *out_shape) {
+    World *w = W(world);
+    if (!w || !xyz || n < 4 || !out_shape) return MI_RB_INVALID_ARG;
+    if ((int)w->hulls.size() >= w->cfg.max_shapes) return MI_RB_CAPACITY;
+    HullData h;
+    int st = rb_build_hull(xyz, stride_bytes, n, scale, h);
+    if (st != MI_RB_OK) return (mi_rb_status)st;
+    int nv = (int)h.verts.size() / 3, nf = (int)h.planes.size() / 4;
+    int ne = (int)h.edges.size() / 3, ni = (int)h.indices.size();
+    if (nv > RB_MAX_HULL_VERTS || ne > RB_MAX_HULL_EDGES) return MI_RB_CAPACITY;
+    if ((int)w->verts.size() + nv > w->cfg.max_hull_vertices) return MI_RB_CAPACITY;
+    HullInfo hi;
+    hi.face_offset = (int)w->planes.size();
+    hi.num_faces = nf;
+    hi.vertex_offset = (int)w->verts.size();
+    hi.num_vertices = nv;
+    hi.edge_offset = (int)w->edges.size();
+    hi.num_edges = ne;
+    hi.index_offset = (int)w->indices.size();
+    hi.num_indices = ni;
+    hi.center = make_float4(h.center[0], h.center[1], h.center[2], 0.f);
+    hi.aabb_min = make_float4(h.aabb_min[0], h.aabb_min[1], h.aabb_min[2], 0.f);
+    hi.aabb_max = make_float4(h.aabb_max[0], h.aabb_max[1], h.aabb_max[2], 0.f);
+    for (int i = 0; i < nv; i++)
+        w->verts.push_back(make_float4(h.verts[3 * i], h.verts[3 * i + 1], h.verts[3 * i + 2], 0.f));
+    for (int i = 0; i < nf; i++) {
+        w->planes.push_back(make_float4(h.planes[4 * i], h.planes[4 * i + 1], h.planes[4 * i + 2],
+                                        h.planes[4 * i + 3]));
+        w->face_index.push_back(make_int2(hi.index_offset + h.face_index[2 * i], h.face_index[2 * i + 1]));
+    }
+    for (int v : h.indices) w->indices.push_back(v);
+    for (int i = 0; i < ne; i++)
+        w->edges.push_back(make_float4(h.edges[3 * i], h.edges[3 * i + 1], h.edges[3 * i + 2], 0.f));
+    float hx = (hi.aabb_max.x - hi.aabb_min.x) * 0.5f, hy = (hi.aabb_max.y - hi.aabb_min.y) * 0.5f,
+          hz = (hi.aabb_max.z - hi.aabb_min.z) * 0.5f;
+    w->hull_hext_norm.push_back(std::sqrt(hx * hx + hy * hy + hz * hz));
+    w->hulls.push_back(hi);
+    *out_shape = (int)w->hulls.size() - 1;
+    w->uploaded = false;
+    return MI_RB_OK;
+}
+
+extern "C" mi_rb_status mi_rb_register_unsupported_shape(void *world, int kind) {
+    (void)kind;
+    return world ? MI_RB_UNSUPPORTED_SHAPE : MI_RB_INVALID_ARG;
+}
+
+extern "C" mi_rb_status mi_rb_register_body(void *world, float mass, const float pos[3],
+                                            const float quat_xyzw[4], int shape, int scene,
+                                            int *out_body) {
+    World *w = W(world);
+    if (!w || !pos || !quat_xyzw || !out_body || mass < 0.f) return MI_RB_INVALID_ARG;
+    if (shape < 0 || shape >= (int)w->hulls.size() || scene < 0) return MI_RB_INVALID_ARG;
+    if (!w->scene.empty() && scene < w->scene.back()) return MI_RB_INVALID_ARG;
+    if ((int)w->pos.size() >= w->cfg.max_bodies) return MI_RB_CAPACITY;
+    w->pos.push_back(make_float4(pos[0], pos[1], pos[2], 0.f));
+    w->quat.push_back(make_float4(quat_xyzw[0], quat_xyzw[1], quat_xyzw[2], quat_xyzw[3]));
+    w->linvel.push_back(make_float4(0, 0, 0, 0));
+    w->angvel.push_back(make_float4(0, 0, 0, 0));
+    w->inv_mass.push_back(mass != 0.f ? 1.f / mass : 0.f);
+    w->shape.push_back(shape);
+    w->scene.push_back(scene);
+    w->inv_inertia_local.push_back(local_inv_inertia(*w, mass, shape, quat_xyzw));
+    *out_body = (int)w->pos.size() - 1;
+    w->uploaded = false;
+    return MI_RB_OK;
+}
+
+extern "C" mi_rb_status mi_rb_upload(void *world) {
+    World *w = W(world);
+    if (!w) return MI_RB_INVALID_ARG;
+    hipSetDevice(w->device);
+    return derive_and_upload(*w);
+}
+
+extern "C" mi_rb_status mi_rb_set_transform(void *world, int body, const float pos[3],
+                                            const float quat[4]) {
+    World *w = W(world);
+    if (!w || !pos || !quat || body < 0 || body >= (int)w->pos.size()) return MI_RB_INVALID_ARG;
+    w->pos[body] = make_float4(pos[0], pos[1], pos[2], 0.f);
+    w->quat[body] = make_float4(quat[0], quat[1], quat[2], quat[3]);
+    if (w->uploaded) {
+        RB_CHECK(hipMemcpyAsync(w->d.pos + body, &w->pos[body], sizeof(float4), hipMemcpyHostToDevice, w->stream));
+        RB_CHECK(hipMemcpyAsync(w->d.quat + body, &w->quat[body], sizeof(float4), hipMemcpyHostToDevice, w->stream));
+        RB_CHECK(hipStreamSynchronize(w->stream));
+    }
+    return MI_RB_OK;
+}
+
+extern "C" mi_rb_status mi_rb_set_linvel(void *world, int body, const float v[3]) {
+    World *w = W(world);
+    if (!w || !v || body < 0 || body >= (int)w->pos.size()) return MI_RB_INVALID_ARG;
+    w->linvel[body] = make_float4(v[0], v[1], v[2], 0.f);
+    if (w->uploaded) {
+        RB_CHECK(hipMemcpyAsync(w->d.linvel + body, &w->linvel[body], sizeof(float4), hipMemcpyHostToDevice, w->stream));
+        RB_CHECK(hipStreamSynchronize(w->stream));
+    }
+    return MI_RB_OK;
+}
+
+extern "C" mi_rb_status mi_rb_set_body_state(void *world, const float *pos4, const float *quat4,
+                                             const float *linvel4, const float *angvel4) {
+    World *w = W(world);
+    if (!w) return MI_RB_INVALID_ARG;
+    size_t n = w->pos.size();
+    const float *src[4] = {pos4, quat4, linvel4, angvel4};
+    std::vector<float4> *dst[4] = {&w->pos, &w->quat, &w->linvel, &w->angvel};
+    float4 *dev[4] = {w->d.pos, w->d.quat, w->d.linvel, w->d.angvel};
+    for (int f = 0; f < 4; f++) {
+        if (!src[f]) continue;
+        std::memcpy(dst[f]->data(), src[f], sizeof(float4) * n);
+        if (w->uploaded)
+            RB_CHECK(hipMemcpyAsync(dev[f], src[f], sizeof(float4) * n, hipMemcpyHostToDevice, w->stream));
+    }
+    if (w->uploaded) RB_CHECK(hipStreamSynchronize(w->stream));
+    return MI_RB_OK;
+}
+
+extern "C" mi_rb_status mi_rb_get_body_state(void *world, float *pos4, float *quat4,
+                                             float *linvel4, float *angvel4) {
+    World *w = W(world);
+    if (!w) return MI_RB_INVALID_ARG;
+    size_t n = w->pos.size();
+    float *dst[4] = {pos4, quat4, linvel4, angvel4};
+    const std::vector<float4> *host[4] = {&w->pos, &w->quat, &w->linvel, &w->angvel};
+    float4 *dev[4] = {w->d.pos, w->d.quat, w->d.linvel, w->d.angvel};
+    for (int f = 0; f < 4; f++) {
+        if (!dst[f]) continue;
+        if (w->uploaded)
+            RB_CHECK(hipMemcpyAsync(dst[f], dev[f], sizeof(float4) * n, hipMemcpyDeviceToHost, w->stream));
+        else
+            std::memcpy(dst[f], host[f]->data(), sizeof(float4) * n);
+    }
+    if (w->uploaded) RB_CHECK(hipStreamSynchronize(w->stream));
+    return MI_RB_OK;
+}
+
+extern "C" mi_rb_status mi_rb_use_graph(void *world, int enable) {
+    World *w = W(world);
+    if (!w) return MI_RB_INVALID_ARG;
+    if (w->graph) { hipGraphExecDestroy(w->graph); w->graph = nullptr; }
+    w->use_graph = enable != 0;
+    return MI_RB_OK;
+}
+
+extern "C" mi_rb_status mi_rb_step(void *world, int nsteps) {
+    World *w = W(world);
+    if (!w || nsteps < 0) return MI_RB_INVALID_ARG;
+    if (!w->uploaded) return MI_RB_STATE;
+    hipSetDevice(w->device);
+    if (w->use_graph && !w->graph) {
+        hipGraph_t g;
+        RB_CHECK(hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal));
+        mi_rb_status st = enqueue_step(*w, MI_RB_STAGE_INTEGRATE);
+        hipError_t e = hipStreamEndCapture(w->stream, &g);
+        if (st != MI_RB_OK) return st;
+        RB_CHECK(e);
+        RB_CHECK(hipGraphInstantiate(&w->graph, g, nullptr, nullptr, 0));
+        hipGraphDestroy(g);
+    }
+    RB_CHECK(hipEventRecord(w->ev[9], w->stream));
+    for (int k = 0; k < nsteps; k++) {
+        if (w->use_graph) {
+            RB_CHECK(hipGraphLaunch(w->graph, w->stream));
+        } else {
+            mi_rb_status st = enqueue_step(*w, MI_RB_STAGE_INTEGRATE);
+            if (st != MI_RB_OK) return st;
+        }
+        w->steps++;
+    }
+    RB_CHECK(hipEventRecord(w->ev[8], w->stream));
+    RB_CHECK(hipEventSynchronize(w->ev[8]));
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, w->ev[9], w->ev[8]);
+    w->last_ms = ms;
+    return MI_RB_OK;
+}
+
+extern "C" mi_rb_status mi_rb_step_partial(void *world, int stage) {
+    World *w = W(world);
+    if (!w || stage < 0 || stage > MI_RB_STAGE_INTEGRATE) return MI_RB_INVALID_ARG;
+    if (!w->uploaded) return MI_RB_STATE;
+    hipSetDevice(w->device);
+    mi_rb_status st = enqueue_step(*w, stage);
+    if (st != MI_RB_OK) return st;
+    RB_CHECK(hipStreamSynchronize(w->stream));
+    return MI_RB_OK;
+}
+
+extern "C" mi_rb_status mi_rb_last_step_ms(void *world, float *ms) {
+    World *w = W(world);
+    if (!w || !ms) return MI_RB_INVALID_ARG;
+    *ms = w->last_ms;
+    return MI_RB_OK;
+}
+
+extern "C" mi_rb_status mi_rb_readback(void *world) {
+    World *w = W(world);
+    if (!w) return MI_RB_INVALID_ARG;
+    if (!w->uploaded) return MI_RB_STATE;
+    size_t n = w->pos.size();
+    RB_CHECK(hipMemcpyAsync(w->pos.data(), w->d.pos, sizeof(float4) * n, hipMemcpyDeviceToHost, w->stream));
+    RB_CHECK(hipMemcpyAsync(w->quat.data(), w->d.quat, sizeof(float4) * n, hipMemcpyDeviceToHost, w->stream));
+    RB_CHECK(hipMemcpyAsync(w->linvel.data(), w->d.linvel, sizeof(float4) * n, hipMemcpyDeviceToHost, w->stream));
+    RB_CHECK(hipMemcpyAsync(w->angvel.data(), w->d.angvel, sizeof(float4) * n, hipMemcpyDeviceToHost, w->stream));
+    RB_CHECK(hipStreamSynchronize(w->stream));
+    return MI_RB_OK;
+}
+
+extern "C" mi_rb_status mi_rb_get_transform(void *world, int body, float pos[4], float quat[4]) {
+    World *w = W(world);
+    if (!w || !pos || !quat || body < 0 || body >= (int)w->pos.size()) return MI_RB_INVALID_ARG;
+    std::memcpy(pos, &w->pos[body], 16);
+    std::memcpy(quat, &w->quat[body], 16);
+    return MI_RB_OK;
+}
+
+extern "C" mi_rb_status mi_rb_get_stats(void *world, mi_rb_stats *out) {
+    World *w = W(world);
+    if (!w || !out) return MI_RB_INVALID_ARG;
+    std::memset(out, 0, sizeof(*out));
+    out->num_bodies = (int)w->pos.size();
+    out->steps = w->steps;
+    if (!w->uploaded) return MI_RB_OK;
+    int c[RB_CNT_COUNT];
+    RB_CHECK(hipMemcpyAsync(c, w->d.counters, sizeof(c), hipMemcpyDeviceToHost, w->stream));
+    RB_CHECK(hipStreamSynchronize(w->stream));
+    out->num_pairs = c[RB_CNT_PAIRS];
+    out->num_contacts = c[RB_CNT_CONTACTS];
+    out->num_colors = c[RB_CNT_NCOLORS];
+    out->overflow_flags = c[RB_CNT_OVERFLOW];
+    out->normal_iterations = c[RB_CNT_ITERS_N];
+    out->friction_iterations = c[RB_CNT_ITERS_F];
+    std::memcpy(&out->normal_residual, &c[RB_CNT_RES_N], 4);
+    std::memcpy(&out->friction_residual, &c[RB_CNT_RES_F], 4);
+    if (w->timing && !w->use_graph) {
+        for (int k = 0; k < 8; k++) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, w->ev[k], w->ev[k + 1]) == hipSuccess) out->stage_ms[k] = ms;
+        }
+    }
+    return MI_RB_OK;
+}
+
+extern "C" mi_rb_status mi_rb_get_params(void *world, mi_rb_params *out) {
+    World *w = W(world);
+    if (!w || !out) return MI_RB_INVALID_ARG;
+    if (!w->uploaded) return MI_RB_STATE;
+    const StepParams &p = w->sp;
+    out->inv_cell = p.inv_cell;
+    out->cell_size = 1.0f / p.inv_cell;
+    for (int k = 0; k < 3; k++) out->dims[k] = p.dims[k];
+    out->cell_bits = p.cell_bits;
+    out->key_bits = w->key_bits;
+    out->n_scenes = p.n_scenes;
+    out->n_large = p.n_large;
+    out->tile_mode = p.tile_mode;
+    out->n_tiles = p.n_tiles;
+    out->tiles_x = p.tiles_x;
+    out->tiles_z = p.tiles_z;
+    out->tile_inv = p.tile_inv;
+    out->tile_origin_x = p.tile_origin_x;
+    out->tile_origin_z = p.tile_origin_z;
+    return MI_RB_OK;
+}
+
+extern "C" mi_rb_status mi_rb_device_ptrs(void *world, mi_rb_device_view *out) {
+    World *w = W(world);
+    if (!w || !out) return MI_RB_INVALID_ARG;
+    if (!w->uploaded) return MI_RB_STATE;
+    out->pos = w->d.pos;
+    out->quat = w->d.quat;
+    out->linvel = w->d.linvel;
+    out->angvel = w->d.angvel;
+    out->inv_mass = w->d.inv_mass;
+    out->pairs = w->d.pairs;
+    out->contacts = w->d.contacts;
+    out->stream = w->stream;
+    out->num_bodies = (int)w->pos.size();
+    return MI_RB_OK;
+}
+
+extern "C" mi_rb_status mi_rb_read_buffer(void *world, int which, void *host, size_t max_bytes,
+                                          int *count) {
+    World *w = W(world);
+    if (!w || !count) return MI_RB_INVALID_ARG;
+    if (which >= MI_RB_BUF_HULLS) {
+        const void *src = nullptr;
+        size_t elem = 0;
+        int cnt = 0;
+        switch (which) {
+            case MI_RB_BUF_HULLS: src = w->hulls.data(); elem = sizeof(HullInfo); cnt = (int)w->hulls.size(); break;
+            case MI_RB_BUF_VERTS: src = w->verts.data(); elem = 16; cnt = (int)w->verts.size(); break;
+            case MI_RB_BUF_PLANES: src = w->planes.data(); elem = 16; cnt = (int)w->planes.size(); break;
+            case MI_RB_BUF_FACE_INDEX: src = w->face_index.data(); elem = 8; cnt = (int)w->face_index.size(); break;
+            case MI_RB_BUF_INDICES: src = w->indices.data(); elem = 4; cnt = (int)w->indices.size(); break;
+            case MI_RB_BUF_EDGES: src = w->edges.data(); elem = 16; cnt = (int)w->edges.size(); break;
+            case MI_RB_BUF_INV_MASS: src = w->inv_mass.data(); elem = 4; cnt = (int)w->inv_mass.size(); break;
+            case MI_RB_BUF_INV_INERTIA_LOCAL: src = w->inv_inertia_local.data(); elem = 16; cnt = (int)w->inv_inertia_local.size(); break;
+            case MI_RB_BUF_SCENE: src = w->scene.data(); elem = 4; cnt = (int)w->scene.size(); break;
+            case MI_RB_BUF_IS_LARGE: src = w->is_large.data(); elem = 4; cnt = (int)w->is_large.size(); break;
+            case MI_RB_BUF_SHAPE: src = w->shape.data(); elem = 4; cnt = (int)w->shape.size(); break;
+            default: return MI_RB_INVALID_ARG;
+        }
+        *count = cnt;
+        if (host && max_bytes && cnt) std::memcpy(host, src, std::min(max_bytes, elem * (size_t)cnt));
+        return MI_RB_OK;
+    }
+    if (!w->uploaded) return MI_RB_STATE;
+    int c[RB_CNT_COUNT];
+    RB_CHECK(hipMemcpyAsync(c, w->d.counters, sizeof(c), hipMemcpyDeviceToHost, w->stream));
+    RB_CHECK(hipStreamSynchronize(w->stream));
+    const int n = (int)w->pos.size(), np = c[RB_CNT_PAIRS], nc = c[RB_CNT_CONTACTS];
+    const void *src = nullptr;
+    size_t elem = 0;
+    int cnt = 0;
+    std::vector<char> tmp;
+    switch (which) {
+        case MI_RB_BUF_SORTED_KV: src = w->d.kv; elem = 8; cnt = n; break;
+        case MI_RB_BUF_AABB: {
+            cnt = n;
+            elem = 32;
+            tmp.resize((size_t)32 * n);
+            RB_CHECK(hipMemcpy(tmp.data(), w->d.aabb_min, 16 * (size_t)n, hipMemcpyDeviceToHost));
+            RB_CHECK(hipMemcpy(tmp.data() + 16 * (size_t)n, w->d.aabb_max, 16 * (size_t)n, hipMemcpyDeviceToHost));
+            break;
+        }
+        case MI_RB_BUF_PAIRS: src = w->d.pairs; elem = 8; cnt = np; break;
+        case MI_RB_BUF_CONTACTS: src = w->d.contacts; elem = 96; cnt = nc; break;
+        case MI_RB_BUF_CONSTRAINTS: {
+            // pack Constraint4 records (176 bytes) from the SoA
+            cnt = nc;
+            elem = 176;
+            std::vector<float4> lin(nc), cen(nc), jac(nc), bb(nc), lam(nc), fr(nc), ps(4 * (size_t)nc);
+            std::vector<int2> bd(nc);
+            std::vector<int> pr(nc);
+            if (nc) {
+                RB_CHECK(hipMemcpy(lin.data(), w->d.cs.linear, 16 * (size_t)nc, hipMemcpyDeviceToHost));
+                RB_CHECK(hipMemcpy(cen.data(), w->d.cs.center, 16 * (size_t)nc, hipMemcpyDeviceToHost));
+                RB_CHECK(hipMemcpy(jac.data(), w->d.cs.jac_inv, 16 * (size_t)nc, hipMemcpyDeviceToHost));
+                RB_CHECK(hipMemcpy(bb.data(), w->d.cs.b, 16 * (size_t)nc, hipMemcpyDeviceToHost));
+                RB_CHECK(hipMemcpy(lam.data(), w->d.cs.lambda, 16 * (size_t)nc, hipMemcpyDeviceToHost));
+                RB_CHECK(hipMemcpy(fr.data(), w->d.cs.fric, 16 * (size_t)nc, hipMemcpyDeviceToHost));
+                RB_CHECK(hipMemcpy(ps.data(), w->d.cs.pos, 64 * (size_t)nc, hipMemcpyDeviceToHost));
+                RB_CHECK(hipMemcpy(bd.data(), w->d.cs.bodies, 8 * (size_t)nc, hipMemcpyDeviceToHost));
+                RB_CHECK(hipMemcpy(pr.data(), w->d.cs.pair, 4 * (size_t)nc, hipMemcpyDeviceToHost));
+            }
+            tmp.assign((size_t)176 * nc, 0);
+            for (int k = 0; k < nc; k++) {
+                float *r = reinterpret_cast<float *>(tmp.data() + (size_t)176 * k);
+                std::memcpy(r, &lin[k], 16);
+                std::memcpy(r + 4, &ps[4 * (size_t)k], 64);
+                std::memcpy(r + 20, &cen[k], 16);
+                std::memcpy(r + 24, &jac[k], 16);
+                std::memcpy(r + 28, &bb[k], 16);
+                std::memcpy(r + 32, &lam[k], 16);
+                r[36] = fr[k].x; r[37] = fr[k].y; r[38] = fr[k].z; r[39] = fr[k].w;
+                uint32_t *u = reinterpret_cast<uint32_t *>(r + 40);
+                u[0] = (uint32_t)bd[k].x; u[1] = (uint32_t)bd[k].y; u[2] = (uint32_t)pr[k]; u[3] = 0;
+            }
+            break;
+        }
+        case MI_RB_BUF_SOLVE_ORDER: src = w->d.solve_order; elem = 4; cnt = nc; break;
+        case MI_RB_BUF_COLORS: src = w->d.row_color; elem = 4; cnt = nc; break;
+        case MI_RB_BUF_INV_INERTIA: src = w->d.inv_inertia_world; elem = 48; cnt = n; break;
+        case MI_RB_BUF_PHASES: {
+            // per tile: first, end, phase (spatial: (tx%3)+3*(tz%3); scene: 0), ncolors
+            const StepParams &p = w->sp;
+            cnt = p.n_tiles;
+            elem = 16;
+            std::vector<int> tf(p.n_tiles), te(p.n_tiles), tn(p.n_tiles);
+            RB_CHECK(hipMemcpy(tf.data(), w->d.tile_first, 4 * (size_t)p.n_tiles, hipMemcpyDeviceToHost));
+            RB_CHECK(hipMemcpy(te.data(), w->d.tile_cnt, 4 * (size_t)p.n_tiles, hipMemcpyDeviceToHost));
+            RB_CHECK(hipMemcpy(tn.data(), w->d.tile_ncolors, 4 * (size_t)p.n_tiles, hipMemcpyDeviceToHost));
+            tmp.resize((size_t)16 * p.n_tiles);
+            int *o = reinterpret_cast<int *>(tmp.data());
+            for (int t = 0; t < p.n_tiles; t++) {
+                int ph = p.tile_mode == 1 ? 0 : (t % p.tiles_x) % 3 + 3 * ((t / p.tiles_x) % 3);
+                o[4 * t] = tf[t]; o[4 * t + 1] = te[t]; o[4 * t + 2] = ph; o[4 * t + 3] = tn[t];
+            }
+            break;
+        }
+        default: return MI_RB_INVALID_ARG;
+    }
+    *count = cnt;
+    size_t bytes = elem * (size_t)cnt;
+    if (host && max_bytes) {
+        size_t b = std::min(bytes, max_bytes);
+        if (!tmp.empty()) std::memcpy(host, tmp.data(), b);
+        else if (b) RB_CHECK(hipMemcpy(host, src, b, hipMemcpyDeviceToHost));
+    }
+    return MI_RB_OK;
+}
+
+// ---- snapshot: "MIRB" v1, counts, then host arrays ----
+namespace {
+template <typename T>
+void put(std::vector<char> &b, const std::vector<T> &v) {
+    uint64_t n = v.size();
+    b.insert(b.end(), (char *)&n, (char *)&n + 8);
+    if (n) b.insert(b.end(), (const char *)v.data(), (const char *)v.data() + sizeof(T) * n);
+}
+template <typename T>
+bool get(const std::vector<char> &b, size_t &o, std::vector<T> &v) {
+    if (o + 8 > b.size()) return false;
+    uint64_t n;
+    std::memcpy(&n, &b[o], 8);
+    o += 8;
+    if (o + sizeof(T) * n > b.size()) return false;
+    v.resize(n);
+    if (n) std::memcpy(v.data(), &b[o], sizeof(T) * n);
+    o += sizeof(T) * n;
+    return true;
+}
+}  // namespace
+
+extern "C" mi_rb_status mi_rb_save_state(void *world, const char *path) {
+    World *w = W(world);
+    if (!w || !path) return MI_RB_INVALID_ARG;
+    if (w->uploaded) {
+        mi_rb_status st = mi_rb_readback(world);
+        if (st != MI_RB_OK) return st;
+    }
+    std::vector<char> b = {'M', 'I', 'R', 'B', 1, 0, 0, 0};
+    put(b, w->hulls); put(b, w->verts); put(b, w->planes); put(b, w->edges);
+    put(b, w->face_index); put(b, w->indices); put(b, w->hull_hext_norm);
+    put(b, w->pos); put(b, w->quat); put(b, w->linvel); put(b, w->angvel);
+    put(b, w->inv_inertia_local); put(b, w->inv_mass); put(b, w->shape); put(b, w->scene);
+    FILE *f = fopen(path, "wb");
+    if (!f) return MI_RB_INVALID_ARG;
+    size_t wr = fwrite(b.data(), 1, b.size(), f);
+    fclose(f);
+    return wr == b.size() ? MI_RB_OK : MI_RB_INVALID_ARG;
+}
+
+extern "C" mi_rb_status mi_rb_load_state(void *world, const char *path) {
+    World *w = W(world);
+    if (!w || !path) return MI_RB_INVALID_ARG;
+    FILE *f = fopen(path, "rb");
+    if (!f) return MI_RB_INVALID_ARG;
+    std::vector<char> b;
+    char buf[65536];
+    size_t r;
+    while ((r = fread(buf, 1, sizeof(buf), f)) > 0) b.insert(b.end(), buf, buf + r);
+    fclose(f);
+    if (b.size() < 8 || std::memcmp(b.data(), "MIRB", 4) != 0 || b[4] != 1) return MI_RB_INVALID_ARG;
+    size_t o = 8;
+    bool ok = get(b, o, w->hulls) && get(b, o, w->verts) && get(b, o, w->planes) &&
+              get(b, o, w->edges) && get(b, o, w->face_index) && get(b, o, w->indices) &&
+              get(b, o, w->hull_hext_norm) && get(b, o, w->pos) && get(b, o, w->quat) &&
+              get(b, o, w->linvel) && get(b, o, w->angvel) && get(b, o, w->inv_inertia_local) &&
+              get(b, o, w->inv_mass) && get(b, o, w->shape) && get(b, o, w->scene);
+    if (!ok) return MI_RB_INVALID_ARG;
+    w->free_device();
+    return MI_RB_OK;
+}
+
+extern "C" mi_rb_status mi_rb_set_warmstart_weights(void *world, const uint16_t *w_bf16,
+                                                    const float *bias) {
+    World *w = W(world);
+    if (!w || !w_bf16 || !bias) return MI_RB_INVALID_ARG;
+    return MI_RB_UNSUPPORTED_SHAPE;  // learned warm start: see k_mlp
+}
+
+// ================================================================ grid API
+namespace {
+struct Grid {
+    mi_rb_grid_desc desc;
+    int max_n;
+    uint2 *kv;
+    uint32_t *cell_start, *pair_count, *pair_off, *total;
+    int *body_slots, *overflow, *large_list, *large_slots, *large_off, *large_index;
+    SortWork sw;
+    uint32_t *scan_status;
+    int key_bits;
+    hipEvent_t ev[5];
+};
+}  // namespace
+
+extern "C" mi_rb_status mi_rb_grid_create(const mi_rb_grid_desc *desc, int max_aabbs, void **out_grid) {
+    if (!desc || !out_grid || max_aabbs <= 0 || desc->cell_size <= 0.f || desc->max_pairs <= 0 ||
+        desc->max_pairs_per_body <= 0 || desc->max_pairs_per_body > 64)
+        return MI_RB_INVALID_ARG;
+    for (int k = 0; k < 3; k++)
+        if (desc->dims[k] < 4 || (desc->dims[k] & (desc->dims[k] - 1))) return MI_RB_INVALID_ARG;
+    Grid *g = new Grid();
+    g->desc = *desc;
+    g->max_n = max_aabbs;
+    g->key_bits = ilog2((uint64_t)desc->dims[0] * desc->dims[1] * desc->dims[2]);
+    if (g->key_bits > 30) { delete g; return MI_RB_INVALID_ARG; }
+    size_t ncells = (size_t)1 << g->key_bits;
+    int n = max_aabbs, K = desc->max_pairs_per_body;
+    bool ok = dalloc(&g->kv, n) == hipSuccess && dalloc(&g->cell_start, ncells) == hipSuccess &&
+              dalloc(&g->pair_count, n) == hipSuccess && dalloc(&g->pair_off, n) == hipSuccess &&
+              dalloc(&g->total, 4) == hipSuccess && dalloc(&g->body_slots, (size_t)n * K) == hipSuccess &&
+              dalloc(&g->overflow, 4) == hipSuccess && dalloc(&g->sw.tmp, n) == hipSuccess &&
+              dalloc(&g->sw.hist, 8 * 256) == hipSuccess &&
+              dalloc(&g->sw.status, sort_status_words(n, 8)) == hipSuccess &&
+              dalloc(&g->sw.tile_ctr, 16) == hipSuccess &&
+              dalloc(&g->scan_status, 4 + 2 * (size_t)((n + 4095) / 4096)) == hipSuccess;
+    if (!ok) { delete g; return MI_RB_DEVICE; }
+    g->sw.max_n = n;
+    g->sw.max_tiles = (n + 4095) / 4096;
+    g->large_list = g->large_slots = g->large_off = g->large_index = nullptr;
+    hipMemset(g->cell_start, 0xff, sizeof(uint32_t) * ncells);
+    hipMemset(g->overflow, 0, 16);
+    for (auto &e : g->ev) hipEventCreate(&e);
+    *out_grid = g;
+    return MI_RB_OK;
+}
+
+extern "C" mi_rb_status mi_rb_grid_destroy(void *grid) {
+    Grid *g = static_cast<Grid *>(grid);
+    if (!g) return MI_RB_INVALID_ARG;
+    void *ps[] = {g->kv, g->cell_start, g->pair_count, g->pair_off, g->total, g->body_slots,
+                  g->overflow, g->sw.tmp, g->sw.hist, g->sw.status, g->sw.tile_ctr, g->scan_status};
+    for (void *p : ps) hipFree(p);
+    for (auto &e : g->ev) hipEventDestroy(e);
+    delete g;
+    return MI_RB_OK;
+}
+
+hipError_t launch_grid_emit_simple(const uint32_t *pair_count, const uint32_t *pair_off,
+                                   const int *body_slots, int n, int K, int max_pairs, int2 *pairs,
+                                   const uint32_t *total, int *num_pairs, int *overflow,
+                                   hipStream_t s);
+
+extern "C" mi_rb_status mi_rb_grid_find_pairs(void *grid, const void *d_aabb_min,
+                                              const void *d_aabb_max, const void *d_is_large,
+                                              int n, void *d_pairs, void *d_num_pairs,
+                                              void *stream) {
+    Grid *g = static_cast<Grid *>(grid);
+    if (!g || !d_aabb_min || !d_aabb_max || !d_pairs || !d_num_pairs || n < 0 || n > g->max_n)
+        return MI_RB_INVALID_ARG;
+    if (d_is_large) return MI_RB_UNSUPPORTED_SHAPE;  // overflow list only through the world API
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const float4 *mn = static_cast<const float4 *>(d_aabb_min);
+    const float4 *mx = static_cast<const float4 *>(d_aabb_max);
+    float inv_cell = 1.0f / g->desc.cell_size;
+    int K = g->desc.max_pairs_per_body;
+    RB_CHECK(hipEventRecord(g->ev[0], s));
+    RB_CHECK(launch_aabb_hash(mn, mx, nullptr, n, inv_cell, g->desc.dims, g->kv, s));
+    RB_CHECK(hipEventRecord(g->ev[1], s));
+    RB_CHECK(launch_radix_sort(g->kv, n, g->key_bits, 8, g->sw, s));
+    RB_CHECK(hipEventRecord(g->ev[2], s));
+    RB_CHECK(launch_cell_start_generic(g->kv, n, g->cell_start, s));
+    RB_CHECK(launch_grid_pairs(mn, mx, g->kv, g->cell_start, nullptr, 0, n, inv_cell, g->desc.dims,
+                               0, K, g->body_slots, g->pair_count, g->overflow, s));
+    RB_CHECK(hipEventRecord(g->ev[3], s));
+    RB_CHECK(launch_exclusive_scan(g->pair_count, g->pair_off, n, g->total, g->scan_status, s));
+    RB_CHECK(launch_grid_emit_simple(g->pair_count, g->pair_off, g->body_slots, n, K,
+                                     g->desc.max_pairs, static_cast<int2 *>(d_pairs), g->total,
+                                     static_cast<int *>(d_num_pairs), g->overflow, s));
+    RB_CHECK(hipEventRecord(g->ev[4], s));
+    return MI_RB_OK;
+}
+
+extern "C" mi_rb_status mi_rb_grid_last_ms(void *grid, float ms[4]) {
+    Grid *g = static_cast<Grid *>(grid);
+    if (!g || !ms) return MI_RB_INVALID_ARG;
+    RB_CHECK(hipEventSynchronize(g->ev[4]));
+    for (int k = 0; k < 4; k++) {
+        ms[k] = 0.f;
+        hipEventElapsedTime(&ms[k], g->ev[k], g->ev[k + 1]);
+    }
+    return MI_RB_OK;
+}

@@ -1,0 +1,254 @@
+/*
+ * mi_rb.h -- C ABI of the MI355X (gfx950) rigid-body step.
+ *
+ * Drop-in replacement for the per-step GPU path behind the reference's
+ * CLPhysicsDemo host API (opencl/gpu_rigidbody_pipeline2/CLPhysicsDemo.h:48-80)
+ * and its primitive C API (opencl/c_api/btbDeviceCL.h, btcFindPairs.h).
+ * Plain C types only; every call returns mi_rb_status.  One world per host
+ * thread; calls on one world are not thread-safe.  Indices are dense,
+ * 0-based, in registration order (as in the reference).
+ */
+#ifndef MI_RB_H
+#define MI_RB_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MI_RB_ABI_VERSION 1
+
+typedef enum mi_rb_status {
+    MI_RB_OK = 0,
+    MI_RB_INVALID_ARG = 1,       /* bad pointer, index or size */
+    MI_RB_CAPACITY = 2,          /* a configured capacity would be exceeded */
+    MI_RB_DEVICE = 3,            /* HIP runtime error or no usable gfx950 device */
+    MI_RB_UNSUPPORTED_SHAPE = 4, /* non-hull shapes (sphere, plane, compound, mesh) */
+    MI_RB_STATE = 5              /* call out of order (e.g. step before upload) */
+} mi_rb_status;
+
+/* warm start of the contact solver */
+enum { MI_RB_WARM_ZERO = 0, MI_RB_WARM_PREVIOUS = 1, MI_RB_WARM_LEARNED = 2 };
+/* body inertia at registration */
+enum { MI_RB_INERTIA_REFERENCE = 0, MI_RB_INERTIA_BOX = 1 };
+/* sticky overflow bits reported in mi_rb_stats.overflow_flags */
+enum {
+    MI_RB_OVERFLOW_PAIRS = 1,          /* global pair capacity */
+    MI_RB_OVERFLOW_BODY_PAIRS = 2,     /* per-body pair list (max_pairs_per_body) */
+    MI_RB_OVERFLOW_CONTACTS = 4,       /* contact capacity */
+    MI_RB_OVERFLOW_COLORS = 8,         /* solver colouring ran out of colours */
+    MI_RB_OVERFLOW_TILE = 16,          /* solver tile capacity (bodies or rows) */
+    MI_RB_OVERFLOW_POLYGON = 32        /* clipped polygon > 64 vertices */
+};
+
+typedef struct mi_rb_config {
+    int max_bodies;            /* body capacity */
+    int max_shapes;            /* hull capacity */
+    int max_hull_vertices;     /* pooled hull vertices (faces/indices/edges scale with it) */
+    int max_pairs;             /* broadphase pair capacity */
+    int max_contacts;          /* contact manifold capacity */
+    int max_pairs_per_body;    /* per-body pair list length (reference: 32-1) */
+    float dt;                  /* step for gravity and integration */
+    float solver_dt;           /* step in the Baumgarte bias; 0 = dt */
+    float gravity[3];
+    int solver_iterations;     /* normal sweeps, then as many friction sweeps */
+    float friction;            /* Coulomb coefficient (reference: 0.7) */
+    float baumgarte;           /* position constraint coefficient (0.2) */
+    float position_drift;      /* allowed penetration in the bias (0.005) */
+    float contact_threshold;   /* keep clipped points with depth <= threshold (0.02) */
+    float angular_damping;     /* angular velocity scale per step (0.99) */
+    float max_angular_motion;  /* radians per step (0.25*pi) */
+    float cell_size;           /* broadphase grid cell; 0 = 2 x max bounding radius */
+    int grid_dims[3];          /* power of two per axis; 0 = derived from the scene */
+    float residual_tolerance;  /* > 0: sweep until the residual drops below it */
+    int warm_start;            /* MI_RB_WARM_* */
+    int inertia_mode;          /* MI_RB_INERTIA_* */
+    float tile_size;           /* solver tile edge in XZ; 0 = derived */
+    int device;                /* HIP device ordinal */
+} mi_rb_config;
+
+typedef struct mi_rb_stats {
+    int num_bodies;
+    int num_pairs;
+    int num_contacts;
+    int num_colors;            /* solver phases in the last step */
+    int overflow_flags;        /* sticky MI_RB_OVERFLOW_* bits */
+    int normal_iterations;     /* sweeps used in the last step */
+    int friction_iterations;
+    float normal_residual;     /* last sweep's residual */
+    float friction_residual;
+    float stage_ms[8];         /* setup, sort, pairs, narrowphase, constraints, colouring, solve, integrate */
+    long long steps;
+} mi_rb_stats;
+
+/* parameters derived at upload (grid, overflow list, solver tiles) */
+typedef struct mi_rb_params {
+    float cell_size, inv_cell;
+    int dims[3];
+    int cell_bits;      /* bits of the cell hash */
+    int key_bits;       /* bits the cell-key sort orders by */
+    int n_scenes, n_large;
+    int tile_mode;      /* 0 spatial XZ tiles, 1 one tile per scene */
+    int n_tiles, tiles_x, tiles_z;
+    float tile_inv, tile_origin_x, tile_origin_z;
+} mi_rb_params;
+
+typedef struct mi_rb_device_view {
+    void *pos, *quat, *linvel, *angvel; /* float4[num_bodies] */
+    void *inv_mass;                     /* float[num_bodies] */
+    void *pairs;                        /* int2[num_pairs], (x > y) */
+    void *contacts;                     /* Contact4[num_contacts] */
+    void *stream;                       /* hipStream_t of the world */
+    int num_bodies;
+} mi_rb_device_view;
+
+/* debug/parity buffers readable after a step (mi_rb_read_buffer) */
+enum {
+    MI_RB_BUF_SORTED_KV = 0,   /* uint2[num_bodies] (cell key, body) after the radix sort */
+    MI_RB_BUF_AABB = 1,        /* float4[2*num_bodies] min,max */
+    MI_RB_BUF_PAIRS = 2,       /* int2[num_pairs] */
+    MI_RB_BUF_CONTACTS = 3,    /* Contact4[num_contacts], ordered by pair */
+    MI_RB_BUF_CONSTRAINTS = 4, /* Constraint4[num_contacts] after the solve */
+    MI_RB_BUF_SOLVE_ORDER = 5, /* int[num_contacts] constraint visiting order of one sweep */
+    MI_RB_BUF_COLORS = 6,      /* int2[num_contacts] (phase, colour) of each constraint */
+    MI_RB_BUF_INV_INERTIA = 7, /* float4[3*num_bodies] world inverse inertia rows */
+    MI_RB_BUF_PHASES = 8,      /* int4[n_tiles] (first, end, phase, colours) of the solve order */
+    /* registered data (host copies, readable before upload) */
+    MI_RB_BUF_HULLS = 9,       /* 80-byte hull records */
+    MI_RB_BUF_VERTS = 10,      /* float4 */
+    MI_RB_BUF_PLANES = 11,     /* float4 */
+    MI_RB_BUF_FACE_INDEX = 12, /* int2 (first index, count) */
+    MI_RB_BUF_INDICES = 13,    /* int */
+    MI_RB_BUF_EDGES = 14,      /* float4 */
+    MI_RB_BUF_INV_MASS = 15,   /* float per body */
+    MI_RB_BUF_INV_INERTIA_LOCAL = 16, /* float4 per body (diagonal) */
+    MI_RB_BUF_SCENE = 17,      /* int per body */
+    MI_RB_BUF_IS_LARGE = 18,   /* int per body (overflow list), after upload */
+    MI_RB_BUF_SHAPE = 19       /* int per body */
+};
+
+/* stages at which mi_rb_step_partial stops */
+enum {
+    MI_RB_STAGE_SETUP = 0, MI_RB_STAGE_SORT = 1, MI_RB_STAGE_PAIRS = 2,
+    MI_RB_STAGE_NARROWPHASE = 3, MI_RB_STAGE_CONSTRAINTS = 4, MI_RB_STAGE_COLOR = 5,
+    MI_RB_STAGE_SOLVE = 6, MI_RB_STAGE_INTEGRATE = 7
+};
+
+/* ---- world: replaces CLPhysicsDemo (CLPhysicsDemo.h:48-80) ---- */
+
+/* Fills the reference preset: constants hard-coded in the reference kernels
+ * (dt 0.0166666, gravity -9.8, 4 iterations, friction 0.7, Baumgarte 0.2,
+ * drift 0.005, threshold 0.02, damping 0.99, 0.25*pi). */
+void mi_rb_config_reference(mi_rb_config *cfg);
+
+/* CLPhysicsDemo::CLPhysicsDemo + init (CLPhysicsDemo.cpp:603-660) */
+mi_rb_status mi_rb_create(const mi_rb_config *cfg, void **out_world);
+/* CLPhysicsDemo::cleanup (CLPhysicsDemo.cpp:662-730) */
+mi_rb_status mi_rb_destroy(void *world);
+
+/* CLPhysicsDemo::registerConvexPolyhedron(const float*, int strideBytes, int n,
+ * const float* scaling, bool) (CLPhysicsDemo.cpp:491-530): convex hull of the
+ * points (scaled), coplanar faces merged; returns the shape index. */
+mi_rb_status mi_rb_register_convex_hull(void *world, const float *xyz, int stride_bytes, int n,
+                                        const float scale[3], int *out_shape);
+/* registerSphereShape / registerPlaneShape / registerCompoundShape /
+ * registerConcaveMesh (CLPhysicsDemo.h:58-66): not on this path. */
+mi_rb_status mi_rb_register_unsupported_shape(void *world, int kind);
+
+/* CLPhysicsDemo::registerPhysicsInstance(mass, pos, orn xyzw, collidable, user)
+ * (CLPhysicsDemo.cpp:533-600); bodies of one scene must be registered
+ * contiguously, scenes in increasing order. */
+mi_rb_status mi_rb_register_body(void *world, float mass, const float pos[3],
+                                 const float quat_xyzw[4], int shape, int scene, int *out_body);
+
+/* CLPhysicsDemo::writeBodiesToGpu (CLPhysicsDemo.cpp:205-215): uploads shapes
+ * and bodies, allocates the step's device buffers. */
+mi_rb_status mi_rb_upload(void *world);
+/* CLPhysicsDemo::setObjectTransform / setObjectLinearVelocity (CLPhysicsDemo.cpp:218-260) */
+mi_rb_status mi_rb_set_transform(void *world, int body, const float pos[3], const float quat[4]);
+mi_rb_status mi_rb_set_linvel(void *world, int body, const float v[3]);
+/* bulk state transfer (float4 arrays; NULL skips a field) */
+mi_rb_status mi_rb_set_body_state(void *world, const float *pos4, const float *quat4,
+                                  const float *linvel4, const float *angvel4);
+mi_rb_status mi_rb_get_body_state(void *world, float *pos4, float *quat4, float *linvel4,
+                                  float *angvel4);
+
+/* CLPhysicsDemo::stepSimulation (CLPhysicsDemo.cpp:740-943), nsteps times;
+ * enqueued on the world's stream (one captured graph per step when enabled). */
+mi_rb_status mi_rb_step(void *world, int nsteps);
+/* one step that stops after `stage` (MI_RB_STAGE_*), for stage parity tests */
+mi_rb_status mi_rb_step_partial(void *world, int stage);
+/* capture the step in a hipGraph and replay it (1) or launch eagerly (0) */
+mi_rb_status mi_rb_use_graph(void *world, int enable);
+
+/* CLPhysicsDemo::readbackBodiesToCpu + getObjectTransformFromCpu (CLPhysicsDemo.cpp:190-203,262-280) */
+mi_rb_status mi_rb_readback(void *world);
+mi_rb_status mi_rb_get_transform(void *world, int body, float pos[4], float quat[4]);
+
+mi_rb_status mi_rb_get_stats(void *world, mi_rb_stats *out);
+mi_rb_status mi_rb_get_params(void *world, mi_rb_params *out);
+/* CLPhysicsDemo::getBodiesGpu (CLPhysicsDemo.cpp:945): non-owning device view */
+mi_rb_status mi_rb_device_ptrs(void *world, mi_rb_device_view *out);
+/* copy a step buffer (MI_RB_BUF_*) to host; *count = elements available */
+mi_rb_status mi_rb_read_buffer(void *world, int which, void *host, size_t max_bytes, int *count);
+/* versioned binary snapshot of shapes + bodies + velocities */
+mi_rb_status mi_rb_save_state(void *world, const char *path);
+mi_rb_status mi_rb_load_state(void *world, const char *path);
+/* warm-start network weights: bf16 W (16x64+64x64+64x3, out-major) + fp32 bias (131) */
+mi_rb_status mi_rb_set_warmstart_weights(void *world, const uint16_t *w_bf16, const float *bias);
+/* HIP-event time of the last `steps` steps, milliseconds */
+mi_rb_status mi_rb_last_step_ms(void *world, float *ms);
+
+/* ---- host utility (no device work): convex hull + polyhedral features ---- */
+/* Builds the hull record used by registration, for inspection and tests.
+ * Arrays are written up to the given capacities; counts are returned. */
+mi_rb_status mi_rb_hull_build(const float *xyz, int stride_bytes, int n, const float scale[3],
+                              float *verts4, int max_verts, float *planes4, int *face_index2,
+                              int max_faces, int *indices, int max_indices, float *edges4,
+                              int max_edges, int counts[4]);
+
+/* ---- primitives: replace btbDeviceCL.h / btRadixSort32CL / btPrefixScanCL / btBoundSearchCL ---- */
+/* All take device pointers and run on `stream` (hipStream_t, NULL = default). */
+
+/* btbCreateRadixSort (c_api/btbDeviceCL.h:37) */
+mi_rb_status mi_rb_radix_sort_create(int max_elements, void **out_sort);
+mi_rb_status mi_rb_radix_sort_destroy(void *sort);
+/* btbSort / btRadixSort32CL::execute (btRadixSort32CL.cpp:161-535): stable
+ * sort of (uint32 key, uint32 value) pairs (btSortData) by the low sort_bits
+ * key bits, in place. */
+mi_rb_status mi_rb_radix_sort_kv(void *sort, void *d_keyvalue, int n, int sort_bits,
+                                 void *stream);
+/* btPrefixScanCL::execute (btPrefixScanCL.cpp:46-99): exclusive scan of uint32;
+ * the total is written to d_total when non-NULL (device pointer). */
+mi_rb_status mi_rb_exclusive_scan_u32(void *sort, const void *d_in, void *d_out, int n,
+                                      void *d_total, void *stream);
+/* btBoundSearchCL::execute (btBoundSearchCL.cpp:78-136): mode 0 lower, 1 upper,
+ * 2 count over sorted (key,value) pairs into d_dst[n_dst]. */
+mi_rb_status mi_rb_bound_search(const void *d_keyvalue, int n, void *d_dst, int n_dst, int mode,
+                                void *stream);
+
+/* ---- grid pair search: replaces btcFindPairs.h over device AABBs ---- */
+typedef struct mi_rb_grid_desc {
+    float cell_size;
+    int dims[3];       /* power of two per axis */
+    int max_pairs;
+    int max_pairs_per_body;
+} mi_rb_grid_desc;
+mi_rb_status mi_rb_grid_create(const mi_rb_grid_desc *desc, int max_aabbs, void **out_grid);
+mi_rb_status mi_rb_grid_destroy(void *grid);
+/* Overlapping pairs of n AABBs (float4 min[n], max[n], device); bodies flagged
+ * in d_is_large (int[n], may be NULL) go on the brute-force overflow list.
+ * Writes int2 (x > y) pairs sorted by (x, y) to d_pairs and the count to
+ * d_num_pairs (device int). */
+mi_rb_status mi_rb_grid_find_pairs(void *grid, const void *d_aabb_min, const void *d_aabb_max,
+                                   const void *d_is_large, int n, void *d_pairs, void *d_num_pairs,
+                                   void *stream);
+/* time of the last find_pairs per phase (hash, sort, cells+pairs, scan+emit), ms */
+mi_rb_status mi_rb_grid_last_ms(void *grid, float ms[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

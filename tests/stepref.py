@@ -1,0 +1,89 @@
+"""Oracle restatement of one step on a World's exact state (test helper).
+
+Chains the oracle stages in the reference's order (CLPhysicsDemo::stepSimulation,
+opencl/gpu_rigidbody_pipeline2/CLPhysicsDemo.cpp:740-943) on the same inputs the
+GPU sees, and reconstructs the GPU's constraint visiting order so the
+Gauss-Seidel sweeps are compared on one schedule.
+"""
+import numpy as np
+
+import oracle as O
+from experiments_amd import _lib as L
+from experiments_amd.world import shape_arrays
+
+
+class Snapshot:
+    def __init__(self, w):
+        self.state = w.get_state()
+        self.sh = shape_arrays(w)
+        self.params = w.params()
+        self.shapes = O.Shapes(self.sh["hulls"], self.sh["verts"], self.sh["planes"],
+                               self.sh["face_index"], self.sh["indices"], self.sh["edges"])
+        self.is_large = w.read(L.BUF_IS_LARGE, np.int32)
+        self.cfg = w.cfg
+        sc = self.sh["scene"]
+        ns = int(sc.max()) + 1 if len(sc) else 1
+        self.scene_start = np.array([int(np.argmax(sc == s)) for s in range(ns)], np.int32)
+        self.scene_count = np.bincount(sc, minlength=ns).astype(np.int32)
+
+
+def oracle_broadphase(s):
+    st, sh, p = s.state, s.sh, s.params
+    mn, mx = O.world_aabbs(st["pos"], st["quat"], sh["shape"], sh["hulls"])
+    kv = O.calc_hash(mn, mx, np.float32(p["inv_cell"]), p["dims"], p["cell_bits"],
+                     scene=sh["scene"], is_large=s.is_large)
+    k, v = O.radix_sort_kv(kv[:, 0], kv[:, 1], p["key_bits"])
+    skv = np.stack([k, v], 1)
+    pairs = O.find_pairs_grid(skv, mn, mx, np.float32(p["inv_cell"]), p["dims"], p["cell_bits"],
+                              scene=sh["scene"], is_large=s.is_large, scene_start=s.scene_start,
+                              scene_count=s.scene_count)
+    return mn, mx, skv, pairs
+
+
+def oracle_setup(s):
+    st, sh, c = s.state, s.sh, s.cfg
+    lv, invI = O.setup_bodies(st["linvel"], st["angvel"], st["quat"], sh["inv_mass"],
+                              sh["inv_inertia_local"], list(c.gravity), c.dt)
+    av = st["angvel"].copy()
+    av[sh["inv_mass"] == 0] = 0
+    lv[sh["inv_mass"] == 0] = 0
+    return lv, av, invI
+
+
+def oracle_contacts(s, pairs):
+    st, sh, c = s.state, s.sh, s.cfg
+    slots, has = O.narrowphase(pairs, st["pos"], st["quat"], sh["inv_mass"], sh["shape"],
+                               s.shapes, c.friction, c.contact_threshold)
+    return slots[has == 1]
+
+
+def gpu_order(w):
+    """Sequential constraint order equivalent to the GPU's tile schedule."""
+    order = w.read(L.BUF_SOLVE_ORDER, np.int32)
+    tiles = w.read(L.BUF_PHASES, np.int32, (4,))
+    seq = []
+    for ph in range(9):
+        for t in range(len(tiles)):
+            first, end, phase, _ = tiles[t]
+            if phase == ph and end > first:
+                seq.append(order[first:end])
+    return np.concatenate(seq) if seq else np.zeros(0, np.int32)
+
+
+def oracle_step(s, order, contacts=None):
+    """Full step on the oracle from the snapshot state, visiting constraints in
+    `order`.  Returns pos, quat, linvel, angvel and the solved constraints."""
+    st, sh, c = s.state, s.sh, s.cfg
+    if contacts is None:
+        _, _, _, pairs = oracle_broadphase(s)
+        contacts = oracle_contacts(s, pairs)
+    lv, av, invI = oracle_setup(s)
+    cons = O.contact_to_constraint(contacts, st["pos"], lv, av, sh["inv_mass"], invI,
+                                   c.solver_dt if c.solver_dt > 0 else c.dt, c.position_drift,
+                                   c.baumgarte, c.friction)
+    cons, lv2, av2, it, res = O.solve(cons, order, lv, av, st["pos"], sh["inv_mass"], invI,
+                                      c.solver_iterations, c.solver_iterations, c.friction,
+                                      c.residual_tolerance)
+    pos, quat, av3 = O.integrate(st["pos"], st["quat"], lv2, av2, sh["inv_mass"], c.dt,
+                                 c.angular_damping, c.max_angular_motion)
+    return dict(pos=pos, quat=quat, linvel=lv2, angvel=av3, cons=cons, iters=it, res=res)

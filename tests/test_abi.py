@@ -1,0 +1,87 @@
+"""The C-ABI library loads and exports every symbol include/mi_rb.h declares;
+host-only logic (hull building) checked on the CPU."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from experiments_amd import _lib as L
+from experiments_amd import scenes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "mi_rb.h")).read()
+    return sorted(set(re.findall(r"\b(mi_rb_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = L.lib()
+    syms = declared_symbols()
+    assert len(syms) >= 30
+    for s in syms:
+        assert hasattr(lib, s), s
+    for s in syms:
+        assert s in L.SIGNATURES, "binding missing for %s" % s
+
+
+def hull(points):
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    pts = np.ascontiguousarray(points, np.float32)
+    v = np.zeros((64, 4), np.float32)
+    pl = np.zeros((128, 4), np.float32)
+    fi = np.zeros((128, 2), np.int32)
+    ind = np.zeros(512, np.int32)
+    ed = np.zeros((256, 4), np.float32)
+    cnt = np.zeros(4, np.int32)
+    st = L.lib().mi_rb_hull_build(P(pts), 12, len(pts), None, P(v), 64, P(pl), P(fi), 128, P(ind),
+                                  512, P(ed), 256, P(cnt))
+    assert st == 0
+    nv, nf, ni, ne = cnt
+    return v[:nv, :3], pl[:nf], fi[:nf], ind[:ni], ed[:ne, :3]
+
+
+def test_cube_features():
+    v, pl, fi, ind, ed = hull(scenes.cube_points(0.5))
+    assert len(v) == 8 and len(pl) == 6 and len(ed) == 3
+    assert np.all(fi[:, 1] == 4)
+    assert np.allclose(np.abs(pl[:, :3]).sum(1), 1)
+    assert np.allclose(pl[:, 3], -0.5)
+
+
+@pytest.mark.parametrize("nverts", [8, 32])
+def test_random_hull_invariants(nverts):
+    rng = scenes.PCG32(7, 1)
+    for _ in range(20):
+        pts = scenes.random_hull_points(rng, nverts)
+        v, pl, fi, ind, ed = hull(pts)
+        nv, nf = len(v), len(pl)
+        n_edges = fi[:, 1].sum() // 2
+        assert nv - n_edges + nf == 2  # Euler
+        # every input point inside every triangle's plane; faces merged from
+        # near-coplanar triangles (normal dot > 0.999) keep their first
+        # triangle's plane, so points may sit outside it by at most the weld
+        # angle times the hull diameter
+        d = pts @ pl[:, :3].T + pl[:, 3]
+        tri = fi[:, 1] == 3
+        assert d[:, tri].max() < 1e-5
+        assert d.max() < np.sin(np.arccos(0.999)) * 2.0
+        # triangles are planar; every face is wound with the outward normal
+        for f in range(nf):
+            idx = ind[fi[f, 0]:fi[f, 0] + fi[f, 1]]
+            p = v[idx]
+            if len(idx) == 3:
+                assert np.allclose(p @ pl[f, :3] + pl[f, 3], 0, atol=1e-5)
+            c = p.mean(0)
+            nrm = sum(np.cross(p[k] - c, p[(k + 1) % len(p)] - c) for k in range(len(p)))
+            assert np.dot(nrm, pl[f, :3]) > 0
+        assert np.allclose(np.linalg.norm(ed, axis=1), 1, atol=1e-5)
+
+
+def test_status_codes_without_device_work():
+    # invalid arguments are rejected before any device call
+    assert L.lib().mi_rb_destroy(None) == L.INVALID_ARG
+    assert L.lib().mi_rb_radix_sort_kv(None, None, 0, 32, None) == L.INVALID_ARG
