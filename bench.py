@@ -1,0 +1,161 @@
+"""Benchmark of the gfx950 rigid-body step.
+
+Default workload: 65,536 convex hulls (a third unit cubes, a third random
+8-vertex and a third random 32-vertex hulls) falling onto a static ground,
+dt 1/60, 10 solver iterations, FP32 -- body-steps/s of the whole step
+(broadphase, narrowphase, solver, integration) with all state resident in
+HBM, the step replayed from one captured hipGraph.
+
+Prints one JSON line.  Options:
+  --gpus N        only 1 is supported (one GPU)
+  --steps K       timed steps (default 200)
+  --warmup W      untimed steps before timing (default 20)
+  --workload      hulls64k (default) | lattice4k | batched4k
+  --cpu-seconds S budget of the CPU-oracle baseline sample (default 15, 0 = skip)
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+# algorithmic bytes per unit for each stage (documented in DESIGN.md)
+STAGES = ["setup", "sort", "pairs", "narrowphase", "constraints", "colouring", "solve",
+          "integrate"]
+
+
+def stage_bytes(stage, bodies, pairs, contacts, passes, sweeps):
+    if stage == "setup":      # read pos,quat,linvel,invI_local,mass,shape,scene,large; write aabb,kv,linvel,invI
+        return 184 * bodies
+    if stage == "sort":       # histogram read + P passes of read+write (key,value)
+        return (2 * passes + 1) * 8 * bodies
+    if stage == "pairs":      # sorted kv + AABB per body, one int2 per pair, count scan
+        return 40 * bodies + 8 * bodies + 8 * pairs
+    if stage == "narrowphase":  # pair, two body transforms/shape ids, contact slot + flag
+        return (8 + 2 * 40 + 96 + 4) * pairs
+    if stage == "constraints":  # flag scan, slot read, contact + constraint write
+        return 8 * pairs + (96 + 96 + 176) * contacts
+    if stage == "colouring":  # tile key sort + one pass over rows
+        return (3 * 2 + 1) * 8 * contacts + 24 * contacts
+    if stage == "solve":      # per sweep: read the row (168 B) and write its impulses (16 B)
+        return 184 * contacts * sweeps
+    if stage == "integrate":  # read+write pos, quat, linvel, angvel; read inverse mass
+        return 108 * bodies
+    return 0
+
+
+def build_scene(name):
+    from experiments_amd import scenes
+    if name == "hulls64k":
+        return scenes.random_hulls(n=65536, box=120.0, pool=(1024, 1024), steps=500, iterations=10)
+    if name == "lattice4k":
+        return scenes.lattice_cubes(16, 16, 16)
+    if name == "batched4k":
+        return scenes.batched_piles(n_scenes=4096, lo=8, hi=512, iterations=10)
+    raise SystemExit("unknown workload %s" % name)
+
+
+def cpu_baseline(world, scene, budget_s):
+    """The oracle port (single-threaded C restatement) stepping the same scene
+    from the world's current state for about budget_s seconds."""
+    import oracle.step as OS
+    snap = OS.Snapshot(world)
+    n = snap.state["pos"].shape[0]
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        out = OS.oracle_step(snap, None)
+        for k in ("pos", "quat", "linvel", "angvel"):
+            snap.state[k] = out[k]
+        steps += 1
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": n * steps / dt, "unit": "body-steps/s", "cores": 1, "kind": "port",
+            "sample": "%d step(s) of the same %d-body scene from the GPU run's state after "
+                      "warmup, oracle/rb_oracle.c serial C, %.1f s" % (steps, n, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="hulls64k")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--profile-steps", type=int, default=5)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only; --gpus %d is not supported (the multi-GPU "
+              "sharding is a design in DESIGN.md, not built)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from experiments_amd import scenes
+    scene = build_scene(args.workload)
+    n = scene.num_bodies
+    w = scenes.build_world(scene)
+    w.use_graph(True)
+    if args.warmup:
+        w.step(args.warmup)
+    t0 = time.perf_counter()
+    w.step(args.steps)           # synchronises on completion
+    host_s = time.perf_counter() - t0
+    gpu_ms = w.last_step_ms()
+    st = w.stats()
+    value = n * args.steps / (gpu_ms / 1e3)
+    # per-stage breakdown: a few eager steps with HIP events between stages
+    w.use_graph(False)
+    acc = np.zeros(8)
+    counts = dict(pairs=0, contacts=0)
+    for _ in range(args.profile_steps):
+        w.step(1)
+        s2 = w.stats()
+        acc += np.array(s2["stage_ms"])
+        counts["pairs"] += s2["num_pairs"]
+        counts["contacts"] += s2["num_contacts"]
+    k = max(args.profile_steps, 1)
+    stage_ms = acc / k
+    pairs, contacts = counts["pairs"] / k, counts["contacts"] / k
+    prm = w.params()
+    passes = (prm["key_bits"] + 7) // 8
+    sweeps = 2 * scene.iterations
+    dom = int(np.argmax(stage_ms))
+    dname = STAGES[dom]
+    abytes = stage_bytes(dname, n, pairs, contacts, passes, sweeps)
+    achieved = abytes / (stage_ms[dom] / 1e3) / 1e9 if stage_ms[dom] > 0 else 0.0
+    roofline = {"bound": "hbm", "kernel": dname, "achieved": round(achieved, 2),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": None, "stage_ms": round(float(stage_ms[dom]), 4)}
+    breakdown = {STAGES[i]: round(float(stage_ms[i]), 4) for i in range(8)}
+    cpu = None
+    if args.cpu_seconds > 0:
+        cpu = cpu_baseline(w, scene, args.cpu_seconds)
+    out = {
+        "metric": "body-steps/s", "value": round(value, 1), "unit": "body-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(gpu_ms / args.steps, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic: seeded PCG32 scene (random hull shapes, positions, orientations, "
+                "velocities, masses)",
+        "config": {"workload": args.workload, "bodies": n, "solver_iterations": scene.iterations,
+                   "dt": scene.dt, "graph": True, "pairs": int(pairs),
+                   "contacts": int(contacts), "key_bits": prm["key_bits"],
+                   "sort_passes": passes, "host_s": round(host_s, 4)},
+        "stage_ms": breakdown,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "overflow_flags": st["overflow_flags"],
+    }
+    w.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

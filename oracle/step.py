@@ -1,4 +1,4 @@
-"""Oracle restatement of one step on a World's exact state (test helper).
+"""Oracle restatement of one step on a World's exact state (test infrastructure).
 
 Chains the oracle stages in the reference's order (CLPhysicsDemo::stepSimulation,
 opencl/gpu_rigidbody_pipeline2/CLPhysicsDemo.cpp:740-943) on the same inputs the
@@ -8,7 +8,7 @@ Gauss-Seidel sweeps are compared on one schedule.
 import numpy as np
 
 import oracle as O
-from experiments_amd import _lib as L
+from experiments_amd import _lib as L  # registered arrays and buffers of a World
 from experiments_amd.world import shape_arrays
 
 

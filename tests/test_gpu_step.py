@@ -9,7 +9,7 @@ import pytest
 import oracle as O
 from experiments_amd import _lib as L
 from experiments_amd import scenes
-from stepref import Snapshot, gpu_order, oracle_broadphase, oracle_contacts, oracle_step
+from oracle.step import Snapshot, gpu_order, oracle_broadphase, oracle_contacts, oracle_step
 
 pytestmark = pytest.mark.gpu
 
@@ -19,7 +19,7 @@ def lattice():
 
 
 def hulls_dense():
-    return scenes.random_hulls(n=1500, box=14.0, pool=(64, 64), steps=1, iterations=6)
+    return scenes.random_hulls(n=1500, box=18.0, pool=(64, 64), steps=1, iterations=6)
 
 
 def piles():
@@ -31,7 +31,7 @@ SCENES = {"lattice": lattice, "hulls": hulls_dense, "piles": piles}
 
 def settled(name, steps):
     """World advanced `steps` steps so bodies touch and contacts exist."""
-    w = scenes.build_world(SCENES[name]())
+    w = scenes.build_world(SCENES[name](), max_pairs_per_body=64)
     if steps:
         w.step(steps)
     return w
@@ -155,4 +155,19 @@ def test_long_run_statistics():
     c = w.contacts(O.CONTACT_DTYPE)
     depth = np.concatenate([c["pos"][k, :int(c["normal"][k, 3]), 3] for k in range(len(c))])
     assert depth.min() > -0.05
+    w.close()
+
+
+def test_per_body_pair_overflow_is_reported():
+    """A per-body list shorter than the overlaps sets the sticky overflow flag;
+    the emitted pairs stay a subset of the true overlap set."""
+    w = scenes.build_world(hulls_dense(), max_pairs_per_body=2)
+    s = Snapshot(w)
+    w.step_partial(L.STAGE_PAIRS)
+    assert w.stats()["overflow_flags"] & L.OVERFLOW_BODY_PAIRS
+    mn, mx, _, pairs = oracle_broadphase(s)
+    gp = w.pairs()
+    ref = set(map(tuple, pairs.tolist()))
+    assert set(map(tuple, gp.tolist())) <= ref
+    assert np.all(np.bincount(gp[:, 0], minlength=len(mn)) <= 2 + s.is_large.sum())
     w.close()
