@@ -12,6 +12,8 @@ Prints one JSON line.  Options:
   --warmup W      untimed steps before timing (default 20)
   --workload      hulls64k (default) | lattice4k | batched4k
   --cpu-seconds S budget of the CPU-oracle baseline sample (default 15, 0 = skip)
+  --per-scene-baseline  (batched4k) also time the same scenes stepped one
+                  world and one launch set (captured graph) per scene
 """
 import argparse
 import json
@@ -83,6 +85,34 @@ def cpu_baseline(world, scene, budget_s):
                       "warmup, oracle/rb_oracle.c serial C, %.1f s" % (steps, n, dt)}
 
 
+def per_scene_baseline(scene, steps, warmup, max_scenes=None):
+    """The batched scenes stepped one world (own buffers, own captured graph)
+    per scene, one after another: the per-scene launch-set comparison."""
+    from experiments_amd import scenes as S
+    subs = S.split_scenes(scene)
+    if max_scenes:
+        subs = subs[:max_scenes]
+    n_sc = len(subs)
+    worlds, bodies = [], 0
+    for sub in subs:
+        bodies += sub.num_bodies
+        w = S.build_world(sub, max_pairs=max(4096, 16 * sub.num_bodies),
+                          max_contacts=max(2048, 8 * sub.num_bodies), max_hull_vertices=1 << 16)
+        w.use_graph(True)
+        w.step(max(warmup, 1))
+        worlds.append(w)
+    total_ms = 0.0
+    t0 = time.perf_counter()
+    for w in worlds:
+        w.step(steps)
+        total_ms += w.last_step_ms()
+    host_s = time.perf_counter() - t0
+    for w in worlds:
+        w.close()
+    return {"value": bodies * steps / host_s, "scenes": n_sc, "bodies": bodies,
+            "host_s": round(host_s, 4), "gpu_ms_sum": round(total_ms, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +121,8 @@ def main():
     ap.add_argument("--workload", default="hulls64k")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--profile-steps", type=int, default=5)
+    ap.add_argument("--per-scene-baseline", action="store_true")
+    ap.add_argument("--per-scene-max", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only; --gpus %d is not supported (the multi-GPU "
@@ -137,6 +169,13 @@ def main():
     cpu = None
     if args.cpu_seconds > 0:
         cpu = cpu_baseline(w, scene, args.cpu_seconds)
+    per_scene = None
+    if args.per_scene_baseline and args.workload == "batched4k":
+        w.close()
+        w = None
+        per_scene = per_scene_baseline(scene, args.steps, args.warmup, args.per_scene_max or None)
+        per_scene["batched_over_per_scene"] = round(
+            value / (per_scene["value"] * n / per_scene["bodies"]) if per_scene["bodies"] else 0, 3)
     out = {
         "metric": "body-steps/s", "value": round(value, 1), "unit": "body-steps/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -153,7 +192,10 @@ def main():
         "cpu_baseline": cpu,
         "overflow_flags": st["overflow_flags"],
     }
-    w.close()
+    if per_scene is not None:
+        out["per_scene_launch_sets"] = per_scene
+    if w is not None:
+        w.close()
     print(json.dumps(out))
 
 

@@ -207,30 +207,76 @@ __global__ __launch_bounds__(FP_BLOCK) void k_find_pairs(
     pair_count[i] = (uint32_t)c;
 }
 
-// One workgroup per large body x: every body j < x of its scene, in order.
-__global__ __launch_bounds__(256) void k_large_pairs(
+// Overflow list: pairs (x, j) of each large body x with every body j < x of
+// its scene.  The scene range is split into chunks of LARGE_CHUNK bodies, one
+// workgroup per (large body, chunk): a counting pass, then an ordered writing
+// pass that places each chunk after the counts of the chunks before it.
+constexpr int LARGE_CHUNK = 4096;
+
+__device__ __forceinline__ int large_range(const int *large_list, const int *scene,
+                                           const int *scene_start, int q, int c, int &x, int &j0,
+                                           int &j1) {
+    x = large_list[q];
+    int s0 = scene ? scene_start[scene[x]] : 0;
+    j0 = s0 + c * LARGE_CHUNK;
+    j1 = j0 + LARGE_CHUNK < x ? j0 + LARGE_CHUNK : x;
+    return j0 < j1;
+}
+
+__global__ __launch_bounds__(256) void k_large_count(
+    const int *__restrict__ large_list, const int *__restrict__ scene,
+    const int *__restrict__ scene_start, const float4 *__restrict__ aabb_min,
+    const float4 *__restrict__ aabb_max, int nchunks, uint32_t *__restrict__ chunk_cnt) {
+    __shared__ uint32_t s_n;
+    const int q = blockIdx.x, c = blockIdx.y;
+    int x, j0, j1;
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    if (large_range(large_list, scene, scene_start, q, c, x, j0, j1)) {
+        float4 mn0 = aabb_min[x], mx0 = aabb_max[x];
+        uint32_t cnt = 0;
+        for (int j = j0 + threadIdx.x; j < j1; j += 256)
+            cnt += aabb_overlap(mn0, mx0, aabb_min[j], aabb_max[j]) ? 1u : 0u;
+        if (cnt) atomicAdd(&s_n, cnt);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) chunk_cnt[q * nchunks + c] = s_n;
+}
+
+__global__ __launch_bounds__(256) void k_large_write(
     const int *__restrict__ large_list, const int *__restrict__ large_off,
     const int *__restrict__ scene, const int *__restrict__ scene_start,
-    const float4 *__restrict__ aabb_min, const float4 *__restrict__ aabb_max,
-    int *__restrict__ large_slots, uint32_t *__restrict__ pair_count) {
+    const float4 *__restrict__ aabb_min, const float4 *__restrict__ aabb_max, int nchunks,
+    const uint32_t *__restrict__ chunk_cnt, int *__restrict__ large_slots,
+    uint32_t *__restrict__ pair_count) {
     __shared__ uint32_t s_w[4];
     __shared__ uint32_t s_base;
-    int x = large_list[blockIdx.x];
-    int j0 = scene ? scene_start[scene[x]] : 0;
-    int *dst = large_slots + large_off[blockIdx.x];
-    float4 mn0 = aabb_min[x], mx0 = aabb_max[x];
-    if (threadIdx.x == 0) s_base = 0;
+    const int q = blockIdx.x, c = blockIdx.y;
+    int x, j0, j1;
+    int have = large_range(large_list, scene, scene_start, q, c, x, j0, j1);
+    if (threadIdx.x == 0) {
+        uint32_t base = 0, tot = 0;
+        for (int k = 0; k < nchunks; k++) {
+            uint32_t v = chunk_cnt[q * nchunks + k];
+            if (k < c) base += v;
+            tot += v;
+        }
+        s_base = base;
+        if (c == 0) pair_count[large_list[q]] = tot;
+    }
     __syncthreads();
-    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int c0 = j0; c0 < x; c0 += 256) {
-        int j = c0 + threadIdx.x;
-        bool hit = j < x && aabb_overlap(mn0, mx0, aabb_min[j], aabb_max[j]);
-        uint64_t b = __ballot(hit);
-        uint32_t below = (uint32_t)__popcll(b & lanemask_lt());
-        if (lane == 0) s_w[wave] = (uint32_t)__popcll(b);
+    if (!have) return;
+    int *dst = large_slots + large_off[q];
+    float4 mn0 = aabb_min[x], mx0 = aabb_max[x];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int b0 = j0; b0 < j1; b0 += 256) {
+        int j = b0 + threadIdx.x;
+        bool hit = j < j1 && aabb_overlap(mn0, mx0, aabb_min[j], aabb_max[j]);
+        uint64_t bal = __ballot(hit);
+        uint32_t below = (uint32_t)__popcll(bal & lanemask_lt());
+        if (lane == 0) s_w[wave] = (uint32_t)__popcll(bal);
         __syncthreads();
-        uint32_t off = s_base;
-        uint32_t tot = 0;
+        uint32_t off = s_base, tot = 0;
         for (int w = 0; w < 4; w++) {
             if (w < wave) off += s_w[w];
             tot += s_w[w];
@@ -240,7 +286,6 @@ __global__ __launch_bounds__(256) void k_large_pairs(
         if (threadIdx.x == 0) s_base += tot;
         __syncthreads();
     }
-    if (threadIdx.x == 0) pair_count[x] = s_base;
 }
 
 __global__ __launch_bounds__(256) void k_emit_pairs(
@@ -304,15 +349,20 @@ hipError_t launch_find_pairs(const DeviceWorld &d, const StepParams &p, hipStrea
     int n = p.n_bodies;
     if (n == 0) return hipSuccess;
     int K = p.max_pairs_per_body;
-    size_t lds = sizeof(int) * (size_t)FP_BLOCK * (K + 1);
-    hipLaunchKernelGGL(k_find_pairs, dim3((n + FP_BLOCK - 1) / FP_BLOCK), dim3(FP_BLOCK), lds, s, n,
+    int blk = K > 32 ? FP_BLOCK / 2 : FP_BLOCK;  // keep the per-thread lists under 64 KB of LDS
+    size_t lds = sizeof(int) * (size_t)blk * (K + 1);
+    hipLaunchKernelGGL(k_find_pairs, dim3((n + blk - 1) / blk), dim3(blk), lds, s, n,
                        d.kv, d.cell_start, d.aabb_min, d.aabb_max, d.large_list,
                        p.n_large ? d.large_first : nullptr, p.n_large ? d.large_num : nullptr,
                        grid_of(p), K, d.body_slots, d.pair_count, d.counters + RB_CNT_OVERFLOW);
-    if (p.n_large > 0)
-        hipLaunchKernelGGL(k_large_pairs, dim3(p.n_large), dim3(256), 0, s, d.large_list,
-                           d.large_off, d.scene, d.scene_start, d.aabb_min, d.aabb_max,
-                           d.large_slots, d.pair_count);
+    if (p.n_large > 0) {
+        dim3 g(p.n_large, p.large_chunks);
+        hipLaunchKernelGGL(k_large_count, g, dim3(256), 0, s, d.large_list, d.scene,
+                           d.scene_start, d.aabb_min, d.aabb_max, p.large_chunks, d.large_chunk_cnt);
+        hipLaunchKernelGGL(k_large_write, g, dim3(256), 0, s, d.large_list, d.large_off, d.scene,
+                           d.scene_start, d.aabb_min, d.aabb_max, p.large_chunks,
+                           d.large_chunk_cnt, d.large_slots, d.pair_count);
+    }
     return hipGetLastError();
 }
 
@@ -353,9 +403,10 @@ hipError_t launch_grid_pairs(const float4 *mn, const float4 *mx, const uint2 *kv
     g.dims[1] = dims[1];
     g.dims[2] = dims[2];
     g.cell_bits = 31;  // one scene: (key >> 31) is 0 for every cell key
-    size_t lds = sizeof(int) * (size_t)FP_BLOCK * (K + 1);
+    int blk = K > 32 ? FP_BLOCK / 2 : FP_BLOCK;
+    size_t lds = sizeof(int) * (size_t)blk * (K + 1);
     // large bodies: large_list holds [n_large indices][first=0][num=n_large]
-    hipLaunchKernelGGL(k_find_pairs, dim3((n + FP_BLOCK - 1) / FP_BLOCK), dim3(FP_BLOCK), lds, s, n,
+    hipLaunchKernelGGL(k_find_pairs, dim3((n + blk - 1) / blk), dim3(blk), lds, s, n,
                        kv, cell_start, mn, mx, large_list, n_large ? large_list + n_large : nullptr,
                        n_large ? large_list + n_large + 1 : nullptr, g, K, body_slots, pair_count,
                        overflow);

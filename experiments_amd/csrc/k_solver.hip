@@ -59,13 +59,13 @@ __device__ __forceinline__ void plane_space(float4 n, float4 &p, float4 &q) {
     }
 }
 
-__device__ __forceinline__ int tile_of(float4 p, int scene, const StepParams &sp) {
-    if (sp.tile_mode == 1) return scene;
-    int tx = (int)floorf((p.x - sp.tile_origin_x) * sp.tile_inv);
-    int tz = (int)floorf((p.z - sp.tile_origin_z) * sp.tile_inv);
+// Tile coordinates of a point (clamped, so neighbours stay neighbours).
+__device__ __forceinline__ int2 tile_xz(float x, float z, const StepParams &sp) {
+    int tx = (int)floorf((x - sp.tile_origin_x) * sp.tile_inv);
+    int tz = (int)floorf((z - sp.tile_origin_z) * sp.tile_inv);
     tx = tx < 0 ? 0 : (tx >= sp.tiles_x ? sp.tiles_x - 1 : tx);
     tz = tz < 0 ? 0 : (tz >= sp.tiles_z ? sp.tiles_z - 1 : tz);
-    return tz * sp.tiles_x + tx;
+    return make_int2(tx, tz);
 }
 
 // ---- compaction of the pair slots + ContactToConstraintKernel ----
@@ -75,7 +75,8 @@ __global__ __launch_bounds__(256) void k_constraints(
     Contact4 *__restrict__ contacts, const float4 *__restrict__ pos,
     const float4 *__restrict__ linvel, const float4 *__restrict__ angvel,
     const float *__restrict__ inv_mass, const float4 *__restrict__ invI, const int *__restrict__ scene,
-    ConstraintSoA cs, uint2 *__restrict__ tile_kv, int *counters) {
+    const float4 *__restrict__ aabb_min, const float4 *__restrict__ aabb_max, ConstraintSoA cs,
+    uint2 *__restrict__ tile_kv, int *counters) {
     int np = *num_pairs_ptr;
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np || !has_contact[p]) return;
@@ -133,9 +134,27 @@ __global__ __launch_bounds__(256) void k_constraints(
     for (int i = 0; i < 4; i++) cs.pos[4 * k + i] = (float)i < npts ? src.pos[i] : mk4(0, 0, 0, 0);
     cs.bodies[k] = make_int2(a, b);
     cs.pair[k] = p;
-    // owner tile: body A unless static, then body B (SetSortDataKernel, solverSetup2.cl:452-479)
-    int owner = imA != 0.f ? a : b;
-    int t = tile_of(pos[owner], scene[owner], sp);
+    // Owner tile (the reference bins by body A's cell, SetSortDataKernel,
+    // solverSetup2.cl:452-479).  Scene tiles: the scene.  Spatial tiles: the
+    // minimum, per axis, of the tiles of the dynamic bodies' AABB centres; a
+    // row then only touches bodies in its owner tile and the next tile in +x
+    // and +z (AABB centres of touching bodies are at most one cell apart and a
+    // tile is at least a cell), so tiles of equal (x, z) parity share no
+    // dynamic body and 4 phases per sweep suffice.
+    int t;
+    if (sp.tile_mode == 1) {
+        t = scene[imA != 0.f ? a : b];
+    } else {
+        float4 ca = mk4((aabb_min[a].x + aabb_max[a].x) * 0.5f, 0.f,
+                        (aabb_min[a].z + aabb_max[a].z) * 0.5f, 0.f);
+        float4 cb = mk4((aabb_min[b].x + aabb_max[b].x) * 0.5f, 0.f,
+                        (aabb_min[b].z + aabb_max[b].z) * 0.5f, 0.f);
+        int2 ta = tile_xz(ca.x, ca.z, sp), tb = tile_xz(cb.x, cb.z, sp);
+        if (imA == 0.f) ta = tb;
+        if (imB == 0.f) tb = ta;
+        int tx = ta.x < tb.x ? ta.x : tb.x, tz = ta.y < tb.y ? ta.y : tb.y;
+        t = tz * sp.tiles_x + tx;
+    }
     tile_kv[k] = make_uint2((uint32_t)t, (uint32_t)k);
 }
 
@@ -416,11 +435,11 @@ __global__ __launch_bounds__(TB) void k_tile_solve(
     if (sp.tile_mode == 1) {
         t = blockIdx.x;
     } else {
-        // tiles of this phase: (tx % 3, tz % 3) == (phase % 3, phase / 3)
-        int px = phase % 3, pz = phase / 3;
-        int nx = (sp.tiles_x - px + 2) / 3;
+        // tiles of this phase: (tx & 1, tz & 1) == (phase & 1, phase >> 1)
+        int px = phase & 1, pz = phase >> 1;
+        int nx = (sp.tiles_x - px + 1) / 2;
         int bx = blockIdx.x % nx, bz = blockIdx.x / nx;
-        int tx = px + 3 * bx, tz = pz + 3 * bz;
+        int tx = px + 2 * bx, tz = pz + 2 * bz;
         if (tx >= sp.tiles_x || tz >= sp.tiles_z) return;
         t = tz * sp.tiles_x + tx;
     }
@@ -510,6 +529,22 @@ __global__ void k_sweep_done(float *residual, int sweep, int is_friction, float 
 }
 __global__ void k_clear_done(int *counters) { counters[RB_CNT_DONE] = 0; }
 
+// fixed sweep counts: iterations used and the last sweep's residual per phase
+__global__ void k_sweep_summary(const float *residual, int nn, int nf, int *counters) {
+    counters[RB_CNT_ITERS_N] = nn;
+    counters[RB_CNT_ITERS_F] = nf;
+    float *f = (float *)counters;
+    if (nn > 0) {
+        float l = residual[2 * (nn - 1) + 1];
+        f[RB_CNT_RES_N] = residual[2 * (nn - 1)] / (l > 1.f ? l : 1.f);
+    }
+    if (nf > 0) {
+        int sw = nn + nf - 1;
+        float l = residual[2 * sw + 1];
+        f[RB_CNT_RES_F] = residual[2 * sw] / (l > 1.f ? l : 1.f);
+    }
+}
+
 // ---- integration ----
 __global__ __launch_bounds__(256) void k_integrate(int n, float4 *__restrict__ pos,
                                                    float4 *__restrict__ quat,
@@ -561,8 +596,8 @@ hipError_t launch_constraints(const DeviceWorld &d, const StepParams &p, hipStre
     int grid = (p.max_pairs + 255) / 256;
     hipLaunchKernelGGL(k_constraints, dim3(grid), dim3(256), 0, s, p, d.counters + RB_CNT_PAIRS,
                        d.has_contact, d.contact_off, d.contact_slots, d.contacts, d.pos, d.linvel,
-                       d.angvel, d.inv_mass, d.inv_inertia_world, d.scene, d.cs, d.tile_kv,
-                       d.counters);
+                       d.angvel, d.inv_mass, d.inv_inertia_world, d.scene, d.aabb_min, d.aabb_max,
+                       d.cs, d.tile_kv, d.counters);
     return hipGetLastError();
 }
 
@@ -613,27 +648,31 @@ hipError_t launch_solve(const DeviceWorld &d, const StepParams &p, hipStream_t s
                                d.tile_cnt, d.solve_order, d.tile_color_off,
                                d.tile_ncolors, d.cs, d.linvel, d.angvel, d.pos, d.inv_mass,
                                d.inv_inertia_world, d.residual, d.counters);
-            for (int sw = 0; sw < S; sw++)
-                hipLaunchKernelGGL(k_sweep_done, dim3(1), dim3(1), 0, s, d.residual, sw,
-                                   sw >= p.normal_iters, 0.f, d.counters);
+            hipLaunchKernelGGL(k_sweep_summary, dim3(1), dim3(1), 0, s, d.residual, p.normal_iters,
+                               p.friction_iters, d.counters);
         }
     } else {
         for (int sw = 0; sw < S; sw++) {
             int fr = sw >= p.normal_iters;
-            for (int ph = 0; ph < 9; ph++) {
-                int nx = (p.tiles_x - ph % 3 + 2) / 3, nz = (p.tiles_z - ph / 3 + 2) / 3;
+            for (int ph = 0; ph < 4; ph++) {
+                int nx = (p.tiles_x - (ph & 1) + 1) / 2, nz = (p.tiles_z - (ph >> 1) + 1) / 2;
                 if (nx <= 0 || nz <= 0) continue;
                 hipLaunchKernelGGL(k_tile_solve, dim3(nx * nz), dim3(TB), 0, s, p, ph, sw,
-                                   fr ? 0 : 1, fr ? 1 : 0, d.tile_first,
-                                   d.tile_cnt, d.solve_order, d.tile_color_off,
-                                   d.tile_ncolors, d.cs, d.linvel, d.angvel, d.pos, d.inv_mass,
-                                   d.inv_inertia_world, d.residual, d.counters);
+                                   fr ? 0 : 1, fr ? 1 : 0, d.tile_first, d.tile_cnt,
+                                   d.solve_order, d.tile_color_off, d.tile_ncolors, d.cs,
+                                   d.linvel, d.angvel, d.pos, d.inv_mass, d.inv_inertia_world,
+                                   d.residual, d.counters);
             }
-            hipLaunchKernelGGL(k_sweep_done, dim3(1), dim3(1), 0, s, d.residual, sw, fr, p.tol,
-                               d.counters);
-            if (sw == p.normal_iters - 1)
-                hipLaunchKernelGGL(k_clear_done, dim3(1), dim3(1), 0, s, d.counters);
+            if (p.tol > 0.f) {
+                hipLaunchKernelGGL(k_sweep_done, dim3(1), dim3(1), 0, s, d.residual, sw, fr, p.tol,
+                                   d.counters);
+                if (sw == p.normal_iters - 1)
+                    hipLaunchKernelGGL(k_clear_done, dim3(1), dim3(1), 0, s, d.counters);
+            }
         }
+        if (p.tol <= 0.f)
+            hipLaunchKernelGGL(k_sweep_summary, dim3(1), dim3(1), 0, s, d.residual, p.normal_iters,
+                               p.friction_iters, d.counters);
     }
     return hipGetLastError();
 }

@@ -80,6 +80,7 @@ struct DeviceWorld {
     int *large_off;              // [n_large+1] offsets of their pair lists in large_slots
     int *large_first, *large_num;// per scene: first position in large_list, count
     int *large_index;            // per body: position in large_list, or -1
+    uint32_t *large_chunk_cnt;   // [n_large x large_chunks] overlap counts per scene chunk
     // broadphase
     float4 *aabb_min, *aabb_max;
     uint2 *kv, *kv_tmp;
@@ -132,6 +133,7 @@ struct StepParams {
     int n_bodies;
     int n_scenes;
     int n_large;
+    int large_chunks;    // scene chunks of 4096 bodies scanned per large body
     float dt;
     float solver_dt;
     float gravity[3];

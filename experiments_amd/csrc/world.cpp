@@ -229,7 +229,7 @@ mi_rb_status derive_and_upload(World &w) {
         p.tile_inv = 0.f;
     } else {
         p.tile_mode = 0;
-        float T = c.tile_size > 0.f ? c.tile_size : 2.f * cell;
+        float T = c.tile_size > 0.f ? std::max(c.tile_size, cell) : cell;
         p.tile_inv = 1.0f / T;
         p.tile_origin_x = lo[0] - T;
         p.tile_origin_z = lo[2] - T;
@@ -307,6 +307,10 @@ mi_rb_status derive_and_upload(World &w) {
     }
     for (int s = 0, run = 0; s < n_scenes; s++) { large_first[s] = run; run += large_num[s]; }
     RB_CHECK(w.alloc(&d.large_slots, (size_t)large_off[p.n_large] + 1));
+    int max_scene = 0;
+    for (int sc = 0; sc < n_scenes; sc++) max_scene = std::max(max_scene, scene_count[sc]);
+    p.large_chunks = std::max(1, (max_scene + 4095) / 4096);
+    RB_CHECK(w.alloc(&d.large_chunk_cnt, (size_t)std::max(p.n_large, 1) * p.large_chunks));
     // sort / scan work
     int sort_cap = std::max(n, maxC);
     w.sw.max_n = sort_cap;
@@ -836,7 +840,7 @@ extern "C" mi_rb_status mi_rb_read_buffer(void *world, int which, void *host, si
             tmp.resize((size_t)16 * p.n_tiles);
             int *o = reinterpret_cast<int *>(tmp.data());
             for (int t = 0; t < p.n_tiles; t++) {
-                int ph = p.tile_mode == 1 ? 0 : (t % p.tiles_x) % 3 + 3 * ((t / p.tiles_x) % 3);
+                int ph = p.tile_mode == 1 ? 0 : ((t % p.tiles_x) & 1) + 2 * ((t / p.tiles_x) & 1);
                 o[4 * t] = tf[t]; o[4 * t + 1] = te[t]; o[4 * t + 2] = ph; o[4 * t + 3] = tn[t];
             }
             break;

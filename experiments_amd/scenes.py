@@ -181,12 +181,34 @@ def batched_piles(n_scenes=4096, lo=8, hi=512, seed=SEED, steps=1000, iterations
     return s
 
 
+def split_scenes(scene):
+    """The scenes of a batched Scene, each as a Scene of its own holding only
+    the shapes it uses."""
+    by_scene = {}
+    for i in range(scene.num_bodies):
+        by_scene.setdefault(scene.scene[i], []).append(i)
+    out = []
+    for sc in sorted(by_scene):
+        sub = Scene("%s_scene%d" % (scene.name, sc), steps=scene.steps,
+                    iterations=scene.iterations, dt=scene.dt, friction=scene.friction)
+        remap = {}
+        for i in by_scene[sc]:
+            sh = scene.shape[i]
+            if sh not in remap:
+                remap[sh] = len(sub.shapes)
+                sub.shapes.append(scene.shapes[sh])
+            sub.add(scene.mass[i], scene.pos[i], scene.quat[i], remap[sh], 0, scene.linvel[i])
+        out.append(sub)
+    return out
+
+
 def build_world(scene, **cfg):
     """Register the scene's shapes and bodies in a World and upload it."""
     from .world import World, reference_config
 
     n = scene.num_bodies
     defaults = dict(max_bodies=max(n, 16), solver_iterations=scene.iterations,
+                    max_pairs_per_body=64,
                     dt=float(scene.dt), solver_dt=float(scene.dt), friction=scene.friction,
                     max_pairs=max(1 << 16, 8 * n), max_contacts=max(1 << 16, 6 * n),
                     max_hull_vertices=max(1 << 16, 40 * len(scene.shapes)),
