@@ -174,8 +174,7 @@ def main():
         w.close()
         w = None
         per_scene = per_scene_baseline(scene, args.steps, args.warmup, args.per_scene_max or None)
-        per_scene["batched_over_per_scene"] = round(
-            value / (per_scene["value"] * n / per_scene["bodies"]) if per_scene["bodies"] else 0, 3)
+        per_scene["batched_over_per_scene"] = round(value / per_scene["value"], 3)
     out = {
         "metric": "body-steps/s", "value": round(value, 1), "unit": "body-steps/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

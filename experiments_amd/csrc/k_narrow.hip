@@ -55,20 +55,36 @@ __device__ __forceinline__ int wave_max_i32(int v) {
 }
 
 // project(): min/max of the hull's local vertices along dir (world), plus dot(pos, dir)
-__device__ __forceinline__ void project(const float4 *sv, int nv, float4 pos, float4 orn,
+// The vertex list in LDS is padded to a multiple of 8 with copies of its last
+// vertex (which leaves min and max unchanged), so the loop runs 8 independent
+// min/max chains and keeps 8 LDS reads in flight.
+__device__ __forceinline__ void project(const float4 *sv, int nv8, float4 pos, float4 orn,
                                         float4 dir, float &mn, float &mx) {
-    float lo = FLT_MAX, hi = -FLT_MAX;
+    float lo[8], hi[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) { lo[k] = FLT_MAX; hi[k] = -FLT_MAX; }
     float4 ld = qt_inv_rotate(orn, dir);
     float off = dot3(pos, dir);
-    for (int i = 0; i < nv; i++) {
-        float dp = dot3(sv[i], ld);
-        if (dp < lo) lo = dp;
-        if (dp > hi) hi = dp;
+    for (int i = 0; i < nv8; i += 8) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            float dp = dot3(sv[i + k], ld);
+            lo[k] = dp < lo[k] ? dp : lo[k];
+            hi[k] = dp > hi[k] ? dp : hi[k];
+        }
     }
-    if (lo > hi) { float t = lo; lo = hi; hi = t; }
-    mn = lo + off;
-    mx = hi + off;
+#pragma unroll
+    for (int k = 1; k < 8; k++) {
+        lo[0] = lo[k] < lo[0] ? lo[k] : lo[0];
+        hi[0] = hi[k] > hi[0] ? hi[k] : hi[0];
+    }
+    float l = lo[0], h = hi[0];
+    if (l > h) { float t = l; l = h; h = t; }
+    mn = l + off;
+    mx = h + off;
 }
+
+__device__ __forceinline__ int pad8(int n) { return (n + 7) & ~7; }
 
 constexpr int NP_BLOCK = 64;
 
@@ -113,8 +129,10 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrowphase(
         pA.w = 0.f;
         pB.w = 0.f;
         __syncthreads();
-        for (int i = lane; i < hA.num_vertices; i += NP_BLOCK) sVA[i] = verts[hA.vertex_offset + i];
-        for (int i = lane; i < hB.num_vertices; i += NP_BLOCK) sVB[i] = verts[hB.vertex_offset + i];
+        for (int i = lane; i < pad8(hA.num_vertices); i += NP_BLOCK)
+            sVA[i] = verts[hA.vertex_offset + (i < hA.num_vertices ? i : hA.num_vertices - 1)];
+        for (int i = lane; i < pad8(hB.num_vertices); i += NP_BLOCK)
+            sVB[i] = verts[hB.vertex_offset + (i < hB.num_vertices ? i : hB.num_vertices - 1)];
         for (int i = lane; i < hA.num_edges; i += NP_BLOCK)
             sEA[i] = qt_rotate(oA, edges[hA.edge_offset + i]);
         for (int i = lane; i < hB.num_edges; i += NP_BLOCK)
@@ -135,11 +153,11 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrowphase(
             if (dot3(dc, w) < 0) w = scl4(w, -1.f);
             float mn0, mx0, mn1, mx1;
             if (onA) {
-                project(sVA, hA.num_vertices, pA, oA, w, mn0, mx0);
-                project(sVB, hB.num_vertices, pB, oB, w, mn1, mx1);
+                project(sVA, pad8(hA.num_vertices), pA, oA, w, mn0, mx0);
+                project(sVB, pad8(hB.num_vertices), pB, oB, w, mn1, mx1);
             } else {
-                project(sVB, hB.num_vertices, pB, oB, w, mn0, mx0);
-                project(sVA, hA.num_vertices, pA, oA, w, mn1, mx1);
+                project(sVB, pad8(hB.num_vertices), pB, oB, w, mn0, mx0);
+                project(sVA, pad8(hA.num_vertices), pA, oA, w, mn1, mx1);
             }
             if (mx0 < mn1 || mx1 < mn0) {
                 sep_found = 1;
@@ -162,8 +180,8 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrowphase(
                 c = normalize3(c);
                 if (dot3(dc, c) < 0) c = scl4(c, -1.f);
                 float mn0, mx0, mn1, mx1;
-                project(sVA, hA.num_vertices, pA, oA, c, mn0, mx0);
-                project(sVB, hB.num_vertices, pB, oB, c, mn1, mx1);
+                project(sVA, pad8(hA.num_vertices), pA, oA, c, mn0, mx0);
+                project(sVB, pad8(hB.num_vertices), pB, oB, c, mn1, mx1);
                 float d0 = mx0 - mn1, d1 = mx1 - mn0;
                 float dist = d0 < d1 ? d0 : d1;
                 if (dist < best) { best = dist; best_ax = nf + k; }

@@ -71,6 +71,7 @@ struct World {
     hipGraphExec_t graph = nullptr;
     hipEvent_t ev[10] = {};
     bool timing = true;
+    bool events_full = false;  // every stage event of the last eager step was recorded
 
     // host shapes
     std::vector<HullInfo> hulls;
@@ -365,6 +366,8 @@ mi_rb_status enqueue_step(World &w, int last_stage) {
     const StepParams &p = w.sp;
     hipStream_t s = w.stream;
     const bool ev = w.timing && !w.use_graph;
+    (void)hipGetLastError();  // start from a clean error state: launches below are checked one by one
+    w.events_full = ev && last_stage >= MI_RB_STAGE_INTEGRATE;
     if (ev) RB_CHECK(hipEventRecord(w.ev[0], s));
     hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, d.counters);
     RB_CHECK(launch_setup(d, p, s));
@@ -698,11 +701,12 @@ extern "C" mi_rb_status mi_rb_get_stats(void *world, mi_rb_stats *out) {
     out->friction_iterations = c[RB_CNT_ITERS_F];
     std::memcpy(&out->normal_residual, &c[RB_CNT_RES_N], 4);
     std::memcpy(&out->friction_residual, &c[RB_CNT_RES_F], 4);
-    if (w->timing && !w->use_graph) {
+    if (w->timing && !w->use_graph && w->events_full) {
         for (int k = 0; k < 8; k++) {
             float ms = 0.f;
             if (hipEventElapsedTime(&ms, w->ev[k], w->ev[k + 1]) == hipSuccess) out->stage_ms[k] = ms;
         }
+        (void)hipGetLastError();  // an event query must not leave a sticky error behind
     }
     return MI_RB_OK;
 }
