@@ -59,8 +59,16 @@ def build_lib(verbose=False):
     return LIB
 
 
+REFERENCE = "/root/reference"
+
+
 def build_oracle(verbose=False):
+    """The CPU oracle (test infrastructure); where the reference checkout is
+    present, also its own CPU clip emulation into oracle/_ref (the checker's
+    cross-check, never loaded by the product)."""
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+    if os.path.isdir(os.path.join(REFERENCE, "opencl")):
+        _run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref", "REF=" + REFERENCE])
     if verbose:
         print("built oracle")
 
