@@ -19,7 +19,9 @@ OVERFLOW_PAIRS, OVERFLOW_BODY_PAIRS, OVERFLOW_CONTACTS = 1, 2, 4
 OVERFLOW_COLORS, OVERFLOW_TILE, OVERFLOW_POLYGON = 8, 16, 32
 (BUF_SORTED_KV, BUF_AABB, BUF_PAIRS, BUF_CONTACTS, BUF_CONSTRAINTS, BUF_SOLVE_ORDER, BUF_COLORS,
  BUF_INV_INERTIA, BUF_PHASES, BUF_HULLS, BUF_VERTS, BUF_PLANES, BUF_FACE_INDEX, BUF_INDICES,
- BUF_EDGES, BUF_INV_MASS, BUF_INV_INERTIA_LOCAL, BUF_SCENE, BUF_IS_LARGE, BUF_SHAPE) = range(20)
+ BUF_EDGES, BUF_INV_MASS, BUF_INV_INERTIA_LOCAL, BUF_SCENE, BUF_IS_LARGE, BUF_SHAPE,
+ BUF_FEATURES, BUF_PREV_KEYS, BUF_PREV_IMPULSES, BUF_PREDICTIONS) = range(24)
+MLP_NW, MLP_NB = 64 * 16 + 64 * 64 + 3 * 64, 64 + 64 + 3
 (STAGE_SETUP, STAGE_SORT, STAGE_PAIRS, STAGE_NARROWPHASE, STAGE_CONSTRAINTS, STAGE_COLOR,
  STAGE_SOLVE, STAGE_INTEGRATE) = range(8)
 
@@ -104,6 +106,14 @@ SIGNATURES = {
     "mi_rb_save_state": (I, [VP, CP]),
     "mi_rb_load_state": (I, [VP, CP]),
     "mi_rb_set_warmstart_weights": (I, [VP, VP, VP]),
+    "mi_rb_copy_training_batch": (I, [VP, VP, VP, I, PI]),
+    "mi_rb_mlp_create": (I, [ctypes.c_uint64, PVP]),
+    "mi_rb_mlp_destroy": (None, [VP]),
+    "mi_rb_mlp_set_weights": (I, [VP, VP, VP]),
+    "mi_rb_mlp_get_weights": (I, [VP, VP, VP]),
+    "mi_rb_mlp_forward": (I, [VP, VP, I, VP]),
+    "mi_rb_mlp_gradients": (I, [VP, VP, VP, I, VP, ctypes.POINTER(ctypes.c_float)]),
+    "mi_rb_mlp_train": (I, [VP, VP, VP, I, I, I, ctypes.c_float, I, VP]),
     "mi_rb_last_step_ms": (I, [VP, ctypes.POINTER(ctypes.c_float)]),
     "mi_rb_hull_build": (I, [VP, I, I, VP, VP, I, VP, VP, I, VP, I, VP, I, VP]),
     "mi_rb_radix_sort_create": (I, [I, PVP]),

@@ -15,9 +15,10 @@
 // each, or XZ columns of a single large scene), each tile is coloured in LDS so
 // that no dynamic body repeats within a colour, and one workgroup walks its
 // tile's colours with barriers between them, keeping the tile's body velocities
-// in LDS.  Spatial tiles are run in 9 phases (tile x mod 3, z mod 3) per sweep
+// in LDS.  Spatial tiles are run in 4 phases (tile x and z parity) per sweep
 // so that tiles sharing bodies never run together; scene tiles are independent
-// and run every sweep in one launch.
+// and run every sweep in one launch.  An optional warm start applies the
+// previous step's (or a learned prediction of the) impulses before the sweeps.
 #include <float.h>
 #include <hip/hip_runtime.h>
 
@@ -76,7 +77,9 @@ __global__ __launch_bounds__(256) void k_constraints(
     const float4 *__restrict__ linvel, const float4 *__restrict__ angvel,
     const float *__restrict__ inv_mass, const float4 *__restrict__ invI, const int *__restrict__ scene,
     const float4 *__restrict__ aabb_min, const float4 *__restrict__ aabb_max, ConstraintSoA cs,
-    uint2 *__restrict__ tile_kv, int *counters) {
+    uint2 *__restrict__ tile_kv, int *counters, const int2 *__restrict__ prev_keys,
+    const float4 *__restrict__ prev_imp, const int *__restrict__ prev_count,
+    float *__restrict__ features) {
     int np = *num_pairs_ptr;
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np || !has_contact[p]) return;
@@ -129,11 +132,61 @@ __global__ __launch_bounds__(256) void k_constraints(
         fr.x = jac_coeff(neg4(cross3(r0, t0)), cross3(r1, t0), imA, IA, imB, IB);
         fr.y = jac_coeff(neg4(cross3(r0, t1)), cross3(r1, t1), imA, IA, imB, IB);
     }
-    cs.fric[k] = fr;
     cs.center[k] = cen;
     for (int i = 0; i < 4; i++) cs.pos[4 * k + i] = (float)i < npts ? src.pos[i] : mk4(0, 0, 0, 0);
     cs.bodies[k] = make_int2(a, b);
     cs.pair[k] = p;
+    // ---- warm start: previous step's impulses of the same body pair ----
+    int prev_n = *prev_count;
+    float4 prev = mk4(0, 0, 0, 0);
+    int has_prev = 0;
+    {
+        int lo = 0, hi = prev_n;
+        while (lo < hi) {
+            int mid = (lo + hi) >> 1;
+            int2 kk = prev_keys[mid];
+            if (kk.x < a || (kk.x == a && kk.y < b)) lo = mid + 1; else hi = mid;
+        }
+        if (lo < prev_n && prev_keys[lo].x == a && prev_keys[lo].y == b) {
+            prev = prev_imp[lo];
+            has_prev = 1;
+        }
+    }
+    // per-contact features of the learned warm start (16 floats)
+    {
+        float4 t0, t1;
+        plane_space(n, t0, t1);
+        float4 rA = sub4(cen, pA), rB = sub4(cen, pB);
+        float4 va = add4(vA, cross3(wA, rA)), vb = add4(vB, cross3(wB, rB));
+        float4 vr = sub4(va, vb);
+        float dmin = FLT_MAX, jsum = 0.f, bsum = 0.f;
+        for (int i = 0; i < 4; i++)
+            if ((float)i < npts) {
+                dmin = fminf(dmin, src.pos[i].w);
+                jsum += jac[i];
+                bsum += bb[i];
+            }
+        float inv_n = npts > 0 ? 1.f / npts : 0.f;
+        float f[16] = {n.x, n.y, n.z, npts > 0 ? 10.f * dmin : 0.f,
+                       dot3(vr, n), dot3(vr, t0), dot3(vr, t1), imA,
+                       imB, prev.x * RB_MLP_SCALE, prev.y * RB_MLP_SCALE, prev.z * RB_MLP_SCALE,
+                       npts * 0.25f, (float)has_prev, -jsum * inv_n, bsum * inv_n};
+        float4 *fo = reinterpret_cast<float4 *>(features + 16 * (size_t)k);
+        fo[0] = mk4(f[0], f[1], f[2], f[3]);
+        fo[1] = mk4(f[4], f[5], f[6], f[7]);
+        fo[2] = mk4(f[8], f[9], f[10], f[11]);
+        fo[3] = mk4(f[12], f[13], f[14], f[15]);
+    }
+    if (sp.warm_start == MI_RB_WARM_PREVIOUS && has_prev && npts > 0) {
+        // the pair's previous normal impulse, split evenly over the points with a row
+        float cnt = (float)((jac[0] != 0.f) + (jac[1] != 0.f) + (jac[2] != 0.f) + (jac[3] != 0.f));
+        float ln = cnt > 0.f ? prev.x / cnt : 0.f;
+        cs.lambda[k] = mk4(jac[0] != 0.f ? ln : 0.f, jac[1] != 0.f ? ln : 0.f,
+                           jac[2] != 0.f ? ln : 0.f, jac[3] != 0.f ? ln : 0.f);
+        fr.z = prev.y;
+        fr.w = prev.z;
+    }
+    cs.fric[k] = fr;
     // Owner tile (the reference bins by body A's cell, SetSortDataKernel,
     // solverSetup2.cl:452-479).  Scene tiles: the scene.  Spatial tiles: the
     // minimum, per axis, of the tiles of the dynamic bodies' AABB centres; a
@@ -421,10 +474,64 @@ __device__ __forceinline__ void solve_row_friction(int k, const ConstraintSoA &c
     }
 }
 
+// Apply the impulses a constraint already holds (warm start) to its bodies.
+__device__ __forceinline__ void apply_row_warm(int k, const ConstraintSoA &cs, float4 &vA,
+                                               float4 &wA, float4 &vB, float4 &wB, float4 pA,
+                                               float4 pB, float imA, const float4 *IA, float imB,
+                                               const float4 *IB) {
+    float4 lin0 = cs.linear[k];
+    float4 jac = cs.jac_inv[k], lam = cs.lambda[k];
+    float jv[4] = {jac.x, jac.y, jac.z, jac.w};
+    float lv[4] = {lam.x, lam.y, lam.z, lam.w};
+    for (int ic = 0; ic < 4; ic++) {
+        if (jv[ic] == 0.f || lv[ic] == 0.f) continue;
+        float4 wp = cs.pos[4 * k + ic];
+        float4 r0 = sub4(wp, pA), r1 = sub4(wp, pB);
+        float4 n = neg4(lin0);
+        float4 lin = neg4(n);
+        float4 a0 = neg4(cross3(r0, n)), a1 = cross3(r1, n);
+        float rl = lv[ic];
+        vA = add4(vA, scl4(scl4(lin, imA), rl));
+        wA = add4(wA, scl4(mt_mul1(IA, a0), rl));
+        vB = add4(vB, scl4(scl4(neg4(lin), imB), rl));
+        wB = add4(wB, scl4(mt_mul1(IB, a1), rl));
+    }
+    float4 fr = cs.fric[k];
+    if (fr.x == 0.f) return;
+    float4 cen = cs.center[k];
+    float4 n = neg4(lin0);
+    float4 t[2];
+    plane_space(n, t[0], t[1]);
+    float4 r0 = sub4(cen, pA), r1 = sub4(cen, pB);
+    float fl[2] = {fr.z, fr.w};
+    for (int i = 0; i < 2; i++) {
+        float rl = fl[i];
+        if (rl == 0.f) continue;
+        float4 lin = neg4(t[i]);
+        float4 a0 = neg4(cross3(r0, t[i])), a1 = cross3(r1, t[i]);
+        vA = add4(vA, scl4(scl4(lin, imA), rl));
+        wA = add4(wA, scl4(mt_mul1(IA, a0), rl));
+        vB = add4(vB, scl4(scl4(neg4(lin), imB), rl));
+        wB = add4(wB, scl4(mt_mul1(IB, a1), rl));
+    }
+}
+
+// previous-step impulse cache for the warm start, keyed by the body pair
+__global__ void k_save_prev(const int *counters, ConstraintSoA cs, int2 *prev_keys,
+                            float4 *prev_imp, int *prev_count) {
+    int nc = counters[RB_CNT_CONTACTS];
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0) *prev_count = nc;
+    if (k >= nc) return;
+    float4 l = cs.lambda[k], f = cs.fric[k];
+    prev_keys[k] = cs.bodies[k];
+    prev_imp[k] = mk4(((l.x + l.y) + l.z) + l.w, f.z, f.w, 0.f);
+}
+
 // One workgroup per tile.  mode: 0 normal sweeps, 1 friction sweeps.
 // sweeps: how many sweeps to run inside (scene tiles run all of them here).
 __global__ __launch_bounds__(TB) void k_tile_solve(
-    StepParams sp, int phase, int sweep0, int nsweeps_n, int nsweeps_f,
+    StepParams sp, int phase, int sweep0, int nsweeps_n, int nsweeps_f, int warm,
     const int *__restrict__ tile_first, const int *__restrict__ tile_cnt,
     const int *__restrict__ order, const int *__restrict__ tile_color_off,
     const int *__restrict__ tile_ncolors, ConstraintSoA cs, float4 *__restrict__ linvel,
@@ -467,6 +574,35 @@ __global__ __launch_bounds__(TB) void k_tile_solve(
         }
     }
     __syncthreads();
+    if (warm) {
+        // warm start: apply the initial impulses colour by colour, in the
+        // order of a sweep
+        for (int c = 0; c < nc; c++) {
+            int c0 = coff[c], c1 = coff[c + 1];
+            if (c1 > R) c1 = R;
+            for (int r = c0 + threadIdx.x; r < c1; r += TB) {
+                int k = order[first + r];
+                int2 bd = cs.bodies[k];
+                int a = bd.x, b = bd.y;
+                float imA = inv_mass[a], imB = inv_mass[b];
+                int ha = imA != 0.f ? hash_find<SOLVE_SLOTS>(L.key, a) : -1;
+                int hb = imB != 0.f ? hash_find<SOLVE_SLOTS>(L.key, b) : -1;
+                float4 z = mk4(0, 0, 0, 0);
+                float4 vA = ha >= 0 ? L.v[ha] : (imA != 0.f ? linvel[a] : z);
+                float4 wA = ha >= 0 ? L.w[ha] : (imA != 0.f ? angvel[a] : z);
+                float4 vB = hb >= 0 ? L.v[hb] : (imB != 0.f ? linvel[b] : z);
+                float4 wB = hb >= 0 ? L.w[hb] : (imB != 0.f ? angvel[b] : z);
+                float4 IA[3] = {invI[3 * a], invI[3 * a + 1], invI[3 * a + 2]};
+                float4 IB[3] = {invI[3 * b], invI[3 * b + 1], invI[3 * b + 2]};
+                apply_row_warm(k, cs, vA, wA, vB, wB, pos[a], pos[b], imA, IA, imB, IB);
+                if (ha >= 0) { L.v[ha] = vA; L.w[ha] = wA; }
+                else if (imA != 0.f) { linvel[a] = vA; angvel[a] = wA; }
+                if (hb >= 0) { L.v[hb] = vB; L.w[hb] = wB; }
+                else if (imB != 0.f) { linvel[b] = vB; angvel[b] = wB; }
+            }
+            __syncthreads();
+        }
+    }
     const int total = nsweeps_n + nsweeps_f;
     for (int s = 0; s < total; s++) {
         const bool fric = s >= nsweeps_n;
@@ -597,7 +733,8 @@ hipError_t launch_constraints(const DeviceWorld &d, const StepParams &p, hipStre
     hipLaunchKernelGGL(k_constraints, dim3(grid), dim3(256), 0, s, p, d.counters + RB_CNT_PAIRS,
                        d.has_contact, d.contact_off, d.contact_slots, d.contacts, d.pos, d.linvel,
                        d.angvel, d.inv_mass, d.inv_inertia_world, d.scene, d.aabb_min, d.aabb_max,
-                       d.cs, d.tile_kv, d.counters);
+                       d.cs, d.tile_kv, d.counters, d.prev_keys, d.prev_imp, d.prev_count,
+                       d.features);
     return hipGetLastError();
 }
 
@@ -625,45 +762,48 @@ hipError_t launch_tiles(const DeviceWorld &d, const StepParams &p, hipStream_t s
 
 hipError_t launch_solve(const DeviceWorld &d, const StepParams &p, hipStream_t s) {
     const int S = p.normal_iters + p.friction_iters;
+    const int warm = p.warm_start != MI_RB_WARM_ZERO;
     hipLaunchKernelGGL(k_clear_done, dim3(1), dim3(1), 0, s, d.counters);
     launch_fill_u32(reinterpret_cast<uint32_t *>(d.residual), 0u, (size_t)2 * (S + 1), s);
+#define RB_SOLVE_ARGS                                                                         \
+    d.tile_first, d.tile_cnt, d.solve_order, d.tile_color_off, d.tile_ncolors, d.cs, d.linvel, \
+        d.angvel, d.pos, d.inv_mass, d.inv_inertia_world, d.residual, d.counters
     if (p.tile_mode == 1) {
         if (p.tol > 0.f) {
             // scene tiles stop together: one launch per sweep so the residual can gate
+            if (warm)
+                hipLaunchKernelGGL(k_tile_solve, dim3(p.n_tiles), dim3(TB), 0, s, p, 0, 0, 0, 0, 1,
+                                   RB_SOLVE_ARGS);
             for (int sw = 0; sw < S; sw++) {
                 int fr = sw >= p.normal_iters;
                 hipLaunchKernelGGL(k_tile_solve, dim3(p.n_tiles), dim3(TB), 0, s, p, 0, sw,
-                                   fr ? 0 : 1, fr ? 1 : 0, d.tile_first,
-                                   d.tile_cnt, d.solve_order, d.tile_color_off,
-                                   d.tile_ncolors, d.cs, d.linvel, d.angvel, d.pos, d.inv_mass,
-                                   d.inv_inertia_world, d.residual, d.counters);
+                                   fr ? 0 : 1, fr ? 1 : 0, 0, RB_SOLVE_ARGS);
                 hipLaunchKernelGGL(k_sweep_done, dim3(1), dim3(1), 0, s, d.residual, sw, fr, p.tol,
                                    d.counters);
                 if (sw == p.normal_iters - 1)
                     hipLaunchKernelGGL(k_clear_done, dim3(1), dim3(1), 0, s, d.counters);
             }
         } else {
+            // independent scenes: every sweep of a scene in one workgroup, one launch
             hipLaunchKernelGGL(k_tile_solve, dim3(p.n_tiles), dim3(TB), 0, s, p, 0, 0,
-                               p.normal_iters, p.friction_iters, d.tile_first,
-                               d.tile_cnt, d.solve_order, d.tile_color_off,
-                               d.tile_ncolors, d.cs, d.linvel, d.angvel, d.pos, d.inv_mass,
-                               d.inv_inertia_world, d.residual, d.counters);
+                               p.normal_iters, p.friction_iters, warm, RB_SOLVE_ARGS);
             hipLaunchKernelGGL(k_sweep_summary, dim3(1), dim3(1), 0, s, d.residual, p.normal_iters,
                                p.friction_iters, d.counters);
         }
     } else {
-        for (int sw = 0; sw < S; sw++) {
+        for (int sw = warm ? -1 : 0; sw < S; sw++) {
             int fr = sw >= p.normal_iters;
             for (int ph = 0; ph < 4; ph++) {
                 int nx = (p.tiles_x - (ph & 1) + 1) / 2, nz = (p.tiles_z - (ph >> 1) + 1) / 2;
                 if (nx <= 0 || nz <= 0) continue;
-                hipLaunchKernelGGL(k_tile_solve, dim3(nx * nz), dim3(TB), 0, s, p, ph, sw,
-                                   fr ? 0 : 1, fr ? 1 : 0, d.tile_first, d.tile_cnt,
-                                   d.solve_order, d.tile_color_off, d.tile_ncolors, d.cs,
-                                   d.linvel, d.angvel, d.pos, d.inv_mass, d.inv_inertia_world,
-                                   d.residual, d.counters);
+                if (sw < 0)
+                    hipLaunchKernelGGL(k_tile_solve, dim3(nx * nz), dim3(TB), 0, s, p, ph, 0, 0, 0,
+                                       1, RB_SOLVE_ARGS);
+                else
+                    hipLaunchKernelGGL(k_tile_solve, dim3(nx * nz), dim3(TB), 0, s, p, ph, sw,
+                                       fr ? 0 : 1, fr ? 1 : 0, 0, RB_SOLVE_ARGS);
             }
-            if (p.tol > 0.f) {
+            if (sw >= 0 && p.tol > 0.f) {
                 hipLaunchKernelGGL(k_sweep_done, dim3(1), dim3(1), 0, s, d.residual, sw, fr, p.tol,
                                    d.counters);
                 if (sw == p.normal_iters - 1)
@@ -674,6 +814,9 @@ hipError_t launch_solve(const DeviceWorld &d, const StepParams &p, hipStream_t s
             hipLaunchKernelGGL(k_sweep_summary, dim3(1), dim3(1), 0, s, d.residual, p.normal_iters,
                                p.friction_iters, d.counters);
     }
+#undef RB_SOLVE_ARGS
+    hipLaunchKernelGGL(k_save_prev, dim3((p.max_contacts + 255) / 256), dim3(256), 0, s,
+                       d.counters, d.cs, d.prev_keys, d.prev_imp, d.prev_count);
     return hipGetLastError();
 }
 

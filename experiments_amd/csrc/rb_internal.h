@@ -43,6 +43,15 @@ static_assert(sizeof(Contact4) == 96, "Contact4 layout");
 #define RB_MAX_HULL_EDGES 128
 #define RB_MAX_POLY 64
 
+// Learned warm-start network 16-64-64-3: bf16 weights W1[64][16], W2[64][64],
+// W3[3][64] (out-major, in that order), fp32 biases b1[64], b2[64], b3[3].
+#define RB_MLP_IN 16
+#define RB_MLP_H 64
+#define RB_MLP_OUT 3
+#define RB_MLP_NW (RB_MLP_H * RB_MLP_IN + RB_MLP_H * RB_MLP_H + RB_MLP_OUT * RB_MLP_H)
+#define RB_MLP_NB (2 * RB_MLP_H + RB_MLP_OUT)
+#define RB_MLP_SCALE 10.f  // network units per unit of impulse (inputs and outputs)
+
 // Solver tiles
 #define RB_TILE_MAX_ROWS 2048     // constraints per tile held by one workgroup
 #define RB_TILE_MAX_BODIES 1024   // distinct bodies per tile cached in LDS
@@ -103,10 +112,11 @@ struct DeviceWorld {
     int *tile_ncolors;
     float *residual;             // per sweep: max scaled dlambda, max lambda (as uint bits)
     // warm start
-    int2 *prev_pairs;            // previous step's contact body pairs, sorted
-    float4 *prev_lambda;         // previous step's normal lambdas
-    float4 *prev_fric;
-    int *prev_count;
+    int2 *prev_keys;             // previous step's contact body pairs (a, b), sorted
+    float4 *prev_imp;            // their impulses: (sum of normal lambdas, friction 0, friction 1, 0)
+    int *prev_count;             // number of entries (persists across steps)
+    float *features;             // 16 floats per contact, learned warm-start inputs
+    float *mlp_out;              // 3 floats per contact, learned warm-start predictions
     uint16_t *mlp_w;
     float *mlp_b;
     // counters (device scalars)
@@ -154,7 +164,7 @@ struct StepParams {
     int tiles_x, tiles_z;
     float tile_origin_x, tile_origin_z;
     int n_tiles;
-    int tile_mode;       // 0: spatial tiles (9 phases per sweep), 1: one tile per scene
+    int tile_mode;       // 0: spatial tiles (4 phases per sweep), 1: one tile per scene
     int normal_iters, friction_iters;
     float tol;
     int warm_start;
@@ -192,6 +202,9 @@ hipError_t launch_tiles(const DeviceWorld &d, const StepParams &p, hipStream_t s
 hipError_t launch_solve(const DeviceWorld &d, const StepParams &p, hipStream_t s);
 hipError_t launch_integrate(const DeviceWorld &d, const StepParams &p, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
+hipError_t launch_mlp_warm(const DeviceWorld &d, const StepParams &p, hipStream_t s);
+hipError_t launch_mlp_targets(const DeviceWorld &d, int capacity, float *x_out, float *t_out,
+                              hipStream_t s);
 
 // generic kernels shared by the grid API and the world
 hipError_t launch_aabb_hash(const float4 *mn, const float4 *mx, const int *is_large, int n,

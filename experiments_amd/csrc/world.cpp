@@ -72,6 +72,9 @@ struct World {
     hipEvent_t ev[10] = {};
     bool timing = true;
     bool events_full = false;  // every stage event of the last eager step was recorded
+    // learned warm-start weights (host copy; uploaded with the world)
+    std::vector<uint16_t> mlp_w_host;
+    std::vector<float> mlp_b_host;
 
     // host shapes
     std::vector<HullInfo> hulls;
@@ -297,6 +300,17 @@ mi_rb_status derive_and_upload(World &w) {
     RB_CHECK(w.alloc(&d.tile_ncolors, p.n_tiles));
     RB_CHECK(w.alloc(&d.residual, 2 * 1024));
     RB_CHECK(w.alloc(&d.counters, RB_CNT_COUNT));
+    RB_CHECK(w.alloc(&d.prev_keys, maxC));
+    RB_CHECK(w.alloc(&d.prev_imp, maxC));
+    RB_CHECK(w.alloc(&d.prev_count, 1));
+    RB_CHECK(w.alloc(&d.features, 16 * (size_t)maxC));
+    RB_CHECK(w.alloc(&d.mlp_out, 4 * (size_t)maxC));
+    RB_CHECK(w.alloc(&d.mlp_w, RB_MLP_NW));
+    RB_CHECK(w.alloc(&d.mlp_b, RB_MLP_NB));
+    if (!w.mlp_w_host.empty()) {
+        RB_CHECK(hipMemcpy(d.mlp_w, w.mlp_w_host.data(), 2 * (size_t)RB_MLP_NW, hipMemcpyHostToDevice));
+        RB_CHECK(hipMemcpy(d.mlp_b, w.mlp_b_host.data(), 4 * (size_t)RB_MLP_NB, hipMemcpyHostToDevice));
+    }
     // large-body pair lists: each may pair with every earlier body of its scene
     std::vector<int> large_off(p.n_large + 1, 0), large_index(n, -1);
     std::vector<int> large_first(n_scenes, 0), large_num(n_scenes, 0);
@@ -352,6 +366,7 @@ mi_rb_status derive_and_upload(World &w) {
     RB_CHECK(up(d.large_index, large_index.data(), sizeof(int) * n));
     RB_CHECK(hipMemsetAsync(d.cell_start, 0xff, sizeof(uint32_t) * ncells, s));
     RB_CHECK(hipMemsetAsync(d.counters, 0, sizeof(int) * RB_CNT_COUNT, s));
+    RB_CHECK(hipMemsetAsync(d.prev_count, 0, sizeof(int), s));
     RB_CHECK(hipMemsetAsync(d.inv_inertia_world, 0, sizeof(float4) * 3 * n, s));
     RB_CHECK(hipMemsetAsync(d.pair_count, 0, sizeof(uint32_t) * n, s));
     RB_CHECK(hipMemsetAsync(d.has_contact, 0, sizeof(uint32_t) * maxP, s));
@@ -391,6 +406,7 @@ mi_rb_status enqueue_step(World &w, int last_stage) {
                                    s, d.counters + RB_CNT_PAIRS));
     RB_CHECK(launch_contact_total(d, p, s));
     RB_CHECK(launch_constraints(d, p, s));
+    if (p.warm_start == MI_RB_WARM_LEARNED) RB_CHECK(launch_mlp_warm(d, p, s));
     if (ev) RB_CHECK(hipEventRecord(w.ev[5], s));
     if (last_stage < MI_RB_STAGE_COLOR) return MI_RB_OK;
     RB_CHECK(launch_radix_sort(d.tile_kv, p.max_contacts, w.tile_bits, 8, w.sw, s,
@@ -615,6 +631,7 @@ extern "C" mi_rb_status mi_rb_step(void *world, int nsteps) {
     World *w = W(world);
     if (!w || nsteps < 0) return MI_RB_INVALID_ARG;
     if (!w->uploaded) return MI_RB_STATE;
+    if (w->sp.warm_start == MI_RB_WARM_LEARNED && w->mlp_w_host.empty()) return MI_RB_STATE;
     hipSetDevice(w->device);
     if (w->use_graph && !w->graph) {
         hipGraph_t g;
@@ -648,6 +665,7 @@ extern "C" mi_rb_status mi_rb_step_partial(void *world, int stage) {
     World *w = W(world);
     if (!w || stage < 0 || stage > MI_RB_STAGE_INTEGRATE) return MI_RB_INVALID_ARG;
     if (!w->uploaded) return MI_RB_STATE;
+    if (w->sp.warm_start == MI_RB_WARM_LEARNED && w->mlp_w_host.empty()) return MI_RB_STATE;
     hipSetDevice(w->device);
     mi_rb_status st = enqueue_step(*w, stage);
     if (st != MI_RB_OK) return st;
@@ -830,6 +848,17 @@ extern "C" mi_rb_status mi_rb_read_buffer(void *world, int which, void *host, si
             break;
         }
         case MI_RB_BUF_SOLVE_ORDER: src = w->d.solve_order; elem = 4; cnt = nc; break;
+        case MI_RB_BUF_FEATURES: src = w->d.features; elem = 64; cnt = nc; break;
+        case MI_RB_BUF_PREDICTIONS: src = w->d.mlp_out; elem = 12; cnt = nc; break;
+        case MI_RB_BUF_PREV_KEYS:
+        case MI_RB_BUF_PREV_IMPULSES: {
+            int pc = 0;
+            RB_CHECK(hipMemcpy(&pc, w->d.prev_count, sizeof(int), hipMemcpyDeviceToHost));
+            cnt = pc;
+            if (which == MI_RB_BUF_PREV_KEYS) { src = w->d.prev_keys; elem = 8; }
+            else { src = w->d.prev_imp; elem = 16; }
+            break;
+        }
         case MI_RB_BUF_COLORS: src = w->d.row_color; elem = 4; cnt = nc; break;
         case MI_RB_BUF_INV_INERTIA: src = w->d.inv_inertia_world; elem = 48; cnt = n; break;
         case MI_RB_BUF_PHASES: {
@@ -928,7 +957,33 @@ extern "C" mi_rb_status mi_rb_set_warmstart_weights(void *world, const uint16_t 
                                                     const float *bias) {
     World *w = W(world);
     if (!w || !w_bf16 || !bias) return MI_RB_INVALID_ARG;
-    return MI_RB_UNSUPPORTED_SHAPE;  // learned warm start: see k_mlp
+    w->mlp_w_host.assign(w_bf16, w_bf16 + RB_MLP_NW);
+    w->mlp_b_host.assign(bias, bias + RB_MLP_NB);
+    if (w->uploaded) {
+        hipSetDevice(w->device);
+        // ordered with the world's stream: steps already queued keep their weights
+        RB_CHECK(hipStreamSynchronize(w->stream));
+        RB_CHECK(hipMemcpy(w->d.mlp_w, w->mlp_w_host.data(), 2 * (size_t)RB_MLP_NW, hipMemcpyHostToDevice));
+        RB_CHECK(hipMemcpy(w->d.mlp_b, w->mlp_b_host.data(), 4 * (size_t)RB_MLP_NB, hipMemcpyHostToDevice));
+    }
+    return MI_RB_OK;
+}
+
+extern "C" mi_rb_status mi_rb_copy_training_batch(void *world, float *x_dev, float *t_dev,
+                                                  int capacity, int *count) {
+    World *w = W(world);
+    if (!w || !x_dev || !t_dev || capacity < 0 || !count) return MI_RB_INVALID_ARG;
+    if (!w->uploaded) return MI_RB_STATE;
+    hipSetDevice(w->device);
+    *count = 0;
+    if (capacity == 0) return MI_RB_OK;
+    RB_CHECK(launch_mlp_targets(w->d, capacity, x_dev, t_dev, w->stream));
+    int nc = 0;
+    RB_CHECK(hipMemcpyAsync(&nc, w->d.counters + RB_CNT_CONTACTS, sizeof(int), hipMemcpyDeviceToHost,
+                            w->stream));
+    RB_CHECK(hipStreamSynchronize(w->stream));
+    *count = nc < capacity ? nc : capacity;
+    return MI_RB_OK;
 }
 
 // ================================================================ grid API

@@ -1,0 +1,121 @@
+"""Learned warm start of the contact solver: the network handle, logging of
+training pairs from the plain solver and the A/B evaluation of iterations to
+a residual tolerance.
+
+Every forward, backward and Adam step runs in libmi_rb.so's kernels (BF16
+MFMA forward, FP32 MFMA backward); torch only provides device buffers.  The
+reference has no counterpart: its solver starts every step from zero impulses
+(dynamics/basic_demo/Stubs/solverSetup.cl:529-531).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+
+FEATURES = ["n.x", "n.y", "n.z", "10*min depth", "vr.n", "vr.t0", "vr.t1", "1/mA", "1/mB",
+            "10*prev lambda_n", "10*prev friction0", "10*prev friction1", "npts/4", "has prev",
+            "-mean jac_inv", "mean b"]
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("the learned warm start trains on the GPU; no device is visible")
+    return torch
+
+
+def _vp(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class Mlp:
+    """16-64-64-3 network: bf16 weights, fp32 biases, Adam state on device."""
+
+    def __init__(self, seed=1):
+        self.h = ctypes.c_void_p()
+        L.call("mi_rb_mlp_create", int(seed), ctypes.byref(self.h))
+
+    def close(self):
+        if self.h:
+            L.lib().mi_rb_mlp_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_weights(self):
+        w = np.zeros(L.MLP_NW, np.uint16)
+        b = np.zeros(L.MLP_NB, np.float32)
+        L.call("mi_rb_mlp_get_weights", self.h, w.ctypes.data_as(ctypes.c_void_p),
+               b.ctypes.data_as(ctypes.c_void_p))
+        return w, b
+
+    def set_weights(self, w_bf16, bias):
+        w = np.ascontiguousarray(w_bf16, np.uint16).reshape(-1)
+        b = np.ascontiguousarray(bias, np.float32).reshape(-1)
+        if w.size != L.MLP_NW or b.size != L.MLP_NB:
+            raise ValueError("expected %d bf16 weights and %d biases" % (L.MLP_NW, L.MLP_NB))
+        L.call("mi_rb_mlp_set_weights", self.h, w.ctypes.data_as(ctypes.c_void_p),
+               b.ctypes.data_as(ctypes.c_void_p))
+
+    def forward(self, x):
+        """x: float32 device tensor (n, 16) -> device tensor (n, 3)."""
+        torch = _torch()
+        x = x.contiguous()
+        y = torch.empty((x.shape[0], 3), dtype=torch.float32, device=x.device)
+        L.call("mi_rb_mlp_forward", self.h, _vp(x), int(x.shape[0]), _vp(y))
+        return y
+
+    def gradients(self, x, t):
+        """Gradient of the MSE loss over all rows: (device tensor [5312+131], loss)."""
+        torch = _torch()
+        x, t = x.contiguous(), t.contiguous()
+        g = torch.empty(L.MLP_NW + L.MLP_NB, dtype=torch.float32, device=x.device)
+        loss = ctypes.c_float()
+        L.call("mi_rb_mlp_gradients", self.h, _vp(x), _vp(t), int(x.shape[0]), _vp(g),
+               ctypes.byref(loss))
+        return g, loss.value
+
+    def train(self, x, t, batch, steps, lr, seed=0):
+        """Adam steps on minibatches drawn from (x, t); returns the loss history."""
+        hist = np.zeros(max(steps, 1), np.float32)
+        L.call("mi_rb_mlp_train", self.h, _vp(x), _vp(t), int(x.shape[0]), int(batch), int(steps),
+               float(lr), int(seed), hist.ctypes.data_as(ctypes.c_void_p))
+        return hist[:steps]
+
+
+def log_plain_solver(world, steps, every=1):
+    """Step `world` (configured with the zero warm start and a residual
+    tolerance) and collect each logged step's features and solved impulses.
+    Returns device tensors X (N, 16), T (N, 3)."""
+    torch = _torch()
+    cap = int(world.cfg.max_contacts)
+    xs, ts = [], []
+    bx = torch.empty((cap, 16), dtype=torch.float32, device="cuda")
+    bt = torch.empty((cap, 3), dtype=torch.float32, device="cuda")
+    for k in range(steps):
+        world.step(1)
+        if k % every:
+            continue
+        n = world.copy_training_batch(bx.data_ptr(), bt.data_ptr(), cap)
+        xs.append(bx[:n].clone())
+        ts.append(bt[:n].clone())
+    return torch.cat(xs), torch.cat(ts)
+
+
+def run_iterations(world, steps):
+    """Step and record per-step solver sweeps and final residuals."""
+    rows = []
+    for _ in range(steps):
+        world.step(1)
+        st = world.stats()
+        rows.append((st["normal_iterations"], st["friction_iterations"], st["normal_residual"],
+                     st["friction_residual"]))
+    a = np.array(rows, np.float64).reshape(-1, 4)
+    return dict(mean_iters_normal=float(a[:, 0].mean()), mean_iters_friction=float(a[:, 1].mean()),
+                max_residual_normal=float(a[:, 2].max()), max_residual_friction=float(a[:, 3].max()),
+                per_step=a)

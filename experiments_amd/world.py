@@ -156,6 +156,21 @@ class World:
     def contacts(self, dtype):
         return self.read(L.BUF_CONTACTS, dtype)
 
+    def set_warmstart_weights(self, w_bf16, bias):
+        w = np.ascontiguousarray(w_bf16, np.uint16).reshape(-1)
+        b = np.ascontiguousarray(bias, np.float32).reshape(-1)
+        if w.size != L.MLP_NW or b.size != L.MLP_NB:
+            raise ValueError("expected %d bf16 weights and %d biases" % (L.MLP_NW, L.MLP_NB))
+        L.call("mi_rb_set_warmstart_weights", self.h, _ptr(w), _ptr(b))
+
+    def copy_training_batch(self, x_dev_ptr, t_dev_ptr, capacity):
+        """Copy the last step's features and solved impulses (network units)
+        into device buffers; returns the number of contacts written."""
+        cnt = ctypes.c_int()
+        L.call("mi_rb_copy_training_batch", self.h, ctypes.c_void_p(x_dev_ptr),
+               ctypes.c_void_p(t_dev_ptr), int(capacity), ctypes.byref(cnt))
+        return cnt.value
+
     def save(self, path):
         L.call("mi_rb_save_state", self.h, path.encode())
 

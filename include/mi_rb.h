@@ -126,7 +126,11 @@ enum {
     MI_RB_BUF_INV_INERTIA_LOCAL = 16, /* float4 per body (diagonal) */
     MI_RB_BUF_SCENE = 17,      /* int per body */
     MI_RB_BUF_IS_LARGE = 18,   /* int per body (overflow list), after upload */
-    MI_RB_BUF_SHAPE = 19       /* int per body */
+    MI_RB_BUF_SHAPE = 19,      /* int per body */
+    MI_RB_BUF_FEATURES = 20,   /* float[16] per contact: warm-start network inputs */
+    MI_RB_BUF_PREV_KEYS = 21,  /* int2 (a, b) per cached pair: impulse cache of the last solve */
+    MI_RB_BUF_PREV_IMPULSES = 22, /* float4 per cached pair: sum lambda_n, friction 0, 1, pad */
+    MI_RB_BUF_PREDICTIONS = 23 /* float[3] per contact: network output (learned warm start) */
 };
 
 /* stages at which mi_rb_step_partial stops */
@@ -196,8 +200,18 @@ mi_rb_status mi_rb_read_buffer(void *world, int which, void *host, size_t max_by
 /* versioned binary snapshot of shapes + bodies + velocities */
 mi_rb_status mi_rb_save_state(void *world, const char *path);
 mi_rb_status mi_rb_load_state(void *world, const char *path);
-/* warm-start network weights: bf16 W (16x64+64x64+64x3, out-major) + fp32 bias (131) */
+/* Learned warm start (no reference counterpart: the reference zeroes lambda
+ * every step, dynamics/basic_demo/Stubs/solverSetup.cl:529-531).  Weights of
+ * the 16-64-64-3 network used when warm_start == MI_RB_WARM_LEARNED: bf16
+ * W1[64][16], W2[64][64], W3[3][64] (out-major, 5312 values) and fp32 biases
+ * b1[64], b2[64], b3[3] (131 values), host pointers. */
 mi_rb_status mi_rb_set_warmstart_weights(void *world, const uint16_t *w_bf16, const float *bias);
+/* Training pairs of the last step, into caller device buffers: per contact the
+ * 16 features (x_dev, float[16]) and the solved impulses in network units
+ * (t_dev, float[3]: 10 x (sum of normal lambdas, friction 0, friction 1)).
+ * *count = contacts written (at most capacity). */
+mi_rb_status mi_rb_copy_training_batch(void *world, float *x_dev, float *t_dev, int capacity,
+                                       int *count);
 /* HIP-event time of the last `steps` steps, milliseconds */
 mi_rb_status mi_rb_last_step_ms(void *world, float *ms);
 
@@ -247,6 +261,27 @@ mi_rb_status mi_rb_grid_find_pairs(void *grid, const void *d_aabb_min, const voi
                                    void *stream);
 /* time of the last find_pairs per phase (hash, sort, cells+pairs, scan+emit), ms */
 mi_rb_status mi_rb_grid_last_ms(void *grid, float ms[4]);
+
+/* ---- learned warm-start network: training and inference (no reference counterpart) ---- */
+/* Network handle with He-uniform weights from `seed`, Adam state and scratch. */
+mi_rb_status mi_rb_mlp_create(uint64_t seed, void **out_mlp);
+void mi_rb_mlp_destroy(void *mlp);
+/* host copies of the bf16 weights / fp32 biases (layout as mi_rb_set_warmstart_weights);
+ * set also resets the Adam moments */
+mi_rb_status mi_rb_mlp_set_weights(void *mlp, const uint16_t *w_bf16, const float *bias);
+mi_rb_status mi_rb_mlp_get_weights(void *mlp, uint16_t *w_bf16, float *bias);
+/* y_dev[n][3] = network(x_dev[n][16]) (device pointers, synchronous) */
+mi_rb_status mi_rb_mlp_forward(void *mlp, const float *x_dev, int n, float *y_dev);
+/* gradients of loss = mean_n sum_o (y_o - t_o)^2 / 3 with respect to the
+ * weights and biases (grad_dev: float[5312 + 131], device) and the loss (host) */
+mi_rb_status mi_rb_mlp_gradients(void *mlp, const float *x_dev, const float *t_dev, int n,
+                                 float *grad_dev, float *loss);
+/* `steps` Adam steps (beta 0.9/0.999, eps 1e-8) on minibatches of `batch`
+ * rows drawn with replacement from n_samples (device x[16], t[3]); seed < 0
+ * takes rows 0..batch-1 in order instead (batch <= n_samples).  The loss of
+ * each step goes to loss_history[steps] (host, may be NULL). */
+mi_rb_status mi_rb_mlp_train(void *mlp, const float *x_dev, const float *t_dev, int n_samples,
+                             int batch, int steps, float lr, int seed, float *loss_history);
 
 #ifdef __cplusplus
 }

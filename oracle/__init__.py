@@ -225,6 +225,30 @@ def contact_to_constraint(contacts, pos, linvel, angvel, inv_mass, inv_inertia_w
     return out[:len(contacts)]
 
 
+def contact_features(contacts, cons, pos, linvel, angvel, inv_mass, prev_keys=None,
+                     prev_imp=None, apply_previous=False):
+    """16 warm-start features per contact; with apply_previous the previous
+    impulses found for the pair are written into a copy of cons."""
+    contacts = np.ascontiguousarray(contacts, CONTACT_DTYPE)
+    cons = np.array(cons, CONSTRAINT_DTYPE, copy=True)
+    nc = len(contacts)
+    feats = np.zeros((max(nc, 1), 16), np.float32)
+    pk = np.ascontiguousarray(prev_keys if prev_keys is not None else np.zeros((0, 2)), np.int32)
+    pi = np.ascontiguousarray(prev_imp if prev_imp is not None else np.zeros((0, 4)), np.float32)
+    lib().orc_contact_features(
+        ctypes.c_int(nc), _p(contacts), _p(cons), _p(f4(pos)), _p(f4(linvel)), _p(f4(angvel)),
+        _p(np.ascontiguousarray(inv_mass, np.float32)), ctypes.c_int(len(pk)), _p(pk), _p(pi),
+        ctypes.c_int(1 if apply_previous else 0), _p(feats))
+    return feats[:nc], cons
+
+
+def warm_from_prediction(cons, y, friction):
+    cons = np.array(cons, CONSTRAINT_DTYPE, copy=True)
+    y = np.ascontiguousarray(y, np.float32).reshape(-1, 3)
+    lib().orc_warm_from_prediction(ctypes.c_int(len(cons)), _p(cons), _p(y), _f(friction))
+    return cons
+
+
 def color_contacts(cons, inv_mass):
     cons = np.ascontiguousarray(cons, CONSTRAINT_DTYPE)
     col = np.zeros(max(len(cons), 1), np.int32)

@@ -752,6 +752,92 @@ void orc_contact_to_constraint(int nc, const orc_contact *contacts, const float 
     }
 }
 
+/* Warm-start inputs (no reference counterpart: the reference zeroes lambda
+ * every step, solverSetup.cl:529-531).  For contact k with constraint cons[k]
+ * (orc_contact_to_constraint on the same inputs) the body pair is looked up
+ * in the previous step's (a, b)-sorted impulse cache (prev_imp: sum of normal
+ * lambdas, friction 0, friction 1, pad) and 16 features are written:
+ *   n.xyz, 10 * min depth, vr.n, vr.t0, vr.t1, 1/mA, 1/mB,
+ *   10 * previous impulses (3), npts / 4, found flag, -mean jac_inv, mean b
+ * with vr the relative velocity of the manifold centre and t0, t1 from
+ * plane_space(n).  With apply_previous the found impulses become the initial
+ * lambdas: the normal impulse split evenly over the points with a row. */
+void orc_contact_features(int nc, const orc_contact *contacts, orc_constraint *cons,
+                          const float *pos, const float *linvel, const float *angvel,
+                          const float *inv_mass, int nprev, const int *prev_keys,
+                          const float *prev_imp, int apply_previous, float *features)
+{
+    for (int k = 0; k < nc; k++) {
+        const orc_contact *src = &contacts[k];
+        orc_constraint *c = &cons[k];
+        int a = abs(src->body_a), b = abs(src->body_b);
+        f4 pA = ld4(pos + 4 * a), pB = ld4(pos + 4 * b);
+        f4 z = mk4(0, 0, 0, 0);
+        float imA = inv_mass[a], imB = inv_mass[b];
+        f4 vA = imA != 0.f ? ld4(linvel + 4 * a) : z, wA = imA != 0.f ? ld4(angvel + 4 * a) : z;
+        f4 vB = imB != 0.f ? ld4(linvel + 4 * b) : z, wB = imB != 0.f ? ld4(angvel + 4 * b) : z;
+        f4 n = ld4(src->normal);
+        float npts = src->normal[3];
+        /* linear search: the cache is small in tests, order does not matter */
+        f4 prev = z;
+        int found = 0;
+        for (int i = 0; i < nprev; i++)
+            if (prev_keys[2 * i] == a && prev_keys[2 * i + 1] == b) {
+                prev = ld4(prev_imp + 4 * i);
+                found = 1;
+                break;
+            }
+        f4 t0, t1;
+        plane_space(n, &t0, &t1);
+        f4 cen = npts > 0 ? ld4(c->center) : z;
+        f4 rA = sub4(cen, pA), rB = sub4(cen, pB);
+        f4 vr = sub4(add4(vA, cross3(wA, rA)), add4(vB, cross3(wB, rB)));
+        float dmin = FLT_MAX, jsum = 0.f, bsum = 0.f;
+        for (int i = 0; i < 4; i++)
+            if ((float)i < npts) {
+                dmin = fminf(dmin, src->pos[i][3]);
+                jsum += c->jac_inv[i];
+                bsum += c->b[i];
+            }
+        float inv_n = npts > 0 ? 1.f / npts : 0.f;
+        float *f = features + 16 * (size_t)k;
+        f[0] = n.x; f[1] = n.y; f[2] = n.z; f[3] = npts > 0 ? 10.f * dmin : 0.f;
+        f[4] = dot3(vr, n); f[5] = dot3(vr, t0); f[6] = dot3(vr, t1); f[7] = imA;
+        f[8] = imB; f[9] = prev.x * 10.f; f[10] = prev.y * 10.f; f[11] = prev.z * 10.f;
+        f[12] = npts * 0.25f; f[13] = (float)found; f[14] = -jsum * inv_n; f[15] = bsum * inv_n;
+        if (apply_previous && found && npts > 0) {
+            int cnt = 0;
+            for (int i = 0; i < 4; i++) cnt += c->jac_inv[i] != 0.f;
+            float ln = cnt > 0 ? prev.x / (float)cnt : 0.f;
+            for (int i = 0; i < 4; i++) c->lambda[i] = c->jac_inv[i] != 0.f ? ln : 0.f;
+            c->f_lambda[0] = prev.y;
+            c->f_lambda[1] = prev.z;
+        }
+    }
+}
+
+/* Initial lambdas from network predictions y[k][3] (10 x impulse): normal
+ * impulse clamped at zero and split evenly over the points with a row,
+ * friction clamped to friction x that normal impulse, none without friction rows. */
+void orc_warm_from_prediction(int nc, orc_constraint *cons, const float *y, float friction)
+{
+    for (int k = 0; k < nc; k++) {
+        orc_constraint *c = &cons[k];
+        int cnt = 0;
+        for (int i = 0; i < 4; i++) cnt += c->jac_inv[i] != 0.f;
+        float ln = y[3 * k] / 10.f;
+        ln = ln > 0.f ? ln : 0.f;
+        float per = cnt > 0 ? ln / (float)cnt : 0.f;
+        for (int i = 0; i < 4; i++) c->lambda[i] = c->jac_inv[i] != 0.f ? per : 0.f;
+        float lim = friction * ln;
+        float f0 = fminf(fmaxf(y[3 * k + 1] / 10.f, -lim), lim);
+        float f1 = fminf(fmaxf(y[3 * k + 2] / 10.f, -lim), lim);
+        if (c->f_jac_inv[0] == 0.f) { f0 = 0.f; f1 = 0.f; }
+        c->f_lambda[0] = f0;
+        c->f_lambda[1] = f1;
+    }
+}
+
 /* Colour priority: a bijective 32-bit mix of the constraint index, so that the
  * colouring is a deterministic function of the constraint list. */
 static uint32_t color_prio(uint32_t k)

@@ -101,6 +101,11 @@ void orc_contact_to_constraint(int nc, const orc_contact *contacts, const float 
                                float coeff, float friction, orc_constraint *out);
 int orc_color_contacts(int nc, const orc_constraint *cs, const float *inv_mass, int n_bodies,
                        int *color);
+void orc_contact_features(int nc, const orc_contact *contacts, orc_constraint *cons,
+                          const float *pos, const float *linvel, const float *angvel,
+                          const float *inv_mass, int nprev, const int *prev_keys,
+                          const float *prev_imp, int apply_previous, float *features);
+void orc_warm_from_prediction(int nc, orc_constraint *cons, const float *y, float friction);
 void orc_solve(int nc, orc_constraint *cs, const int *order, float *linvel, float *angvel,
                const float *pos, const float *inv_mass, const float *inv_inertia_world,
                int normal_iters, int friction_iters, float friction, float tol,

@@ -62,7 +62,7 @@ def gpu_order(w):
     order = w.read(L.BUF_SOLVE_ORDER, np.int32)
     tiles = w.read(L.BUF_PHASES, np.int32, (4,))
     seq = []
-    for ph in range(9):
+    for ph in range(4):
         for t in range(len(tiles)):
             first, end, phase, _ = tiles[t]
             if phase == ph and end > first:
@@ -70,9 +70,12 @@ def gpu_order(w):
     return np.concatenate(seq) if seq else np.zeros(0, np.int32)
 
 
-def oracle_step(s, order, contacts=None):
+def oracle_step(s, order, contacts=None, prev=None, warm=0, prediction=None):
     """Full step on the oracle from the snapshot state, visiting constraints in
-    `order`.  Returns pos, quat, linvel, angvel and the solved constraints."""
+    `order`.  prev: (keys int2[m], impulses float4[m]) of the previous solve;
+    warm: 0 zero, 1 previous impulses, 2 impulses from `prediction` (n, 3).
+    Returns pos, quat, linvel, angvel, the solved constraints and the
+    warm-start features."""
     st, sh, c = s.state, s.sh, s.cfg
     if contacts is None:
         _, _, _, pairs = oracle_broadphase(s)
@@ -81,9 +84,15 @@ def oracle_step(s, order, contacts=None):
     cons = O.contact_to_constraint(contacts, st["pos"], lv, av, sh["inv_mass"], invI,
                                    c.solver_dt if c.solver_dt > 0 else c.dt, c.position_drift,
                                    c.baumgarte, c.friction)
+    keys, imp = prev if prev is not None else (None, None)
+    feats, cons = O.contact_features(contacts, cons, st["pos"], lv, av, sh["inv_mass"], keys, imp,
+                                     apply_previous=(warm == 1))
+    if warm == 2:
+        cons = O.warm_from_prediction(cons, prediction, c.friction)
     cons, lv2, av2, it, res = O.solve(cons, order, lv, av, st["pos"], sh["inv_mass"], invI,
                                       c.solver_iterations, c.solver_iterations, c.friction,
                                       c.residual_tolerance)
     pos, quat, av3 = O.integrate(st["pos"], st["quat"], lv2, av2, sh["inv_mass"], c.dt,
                                  c.angular_damping, c.max_angular_motion)
-    return dict(pos=pos, quat=quat, linvel=lv2, angvel=av3, cons=cons, iters=it, res=res)
+    return dict(pos=pos, quat=quat, linvel=lv2, angvel=av3, cons=cons, iters=it, res=res,
+                features=feats)
