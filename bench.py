@@ -123,19 +123,46 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--per-scene-baseline", action="store_true")
     ap.add_argument("--per-scene-max", type=int, default=0)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shapes instead of timing the candidates first")
+    ap.add_argument("--eager", action="store_true",
+                    help="launch the timed steps stage by stage instead of replaying the "
+                         "captured graph (for kernel tracers that do not follow graph launches)")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB 5 times (the HBM "
+                         "counter calibration for rocprofv3 --pmc) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only; --gpus %d is not supported (the multi-GPU "
               "sharding is a design in DESIGN.md, not built)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_probe_mib:
+        import torch
+        src = torch.ones(args.copy_probe_mib << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_each_way": src.numel() * 4, "launches": 5}))
+        return
+
     from experiments_amd import scenes
     scene = build_scene(args.workload)
     n = scene.num_bodies
     w = scenes.build_world(scene)
-    w.use_graph(True)
+    w.use_graph(not args.eager)
     if args.warmup:
         w.step(args.warmup)
+    tuning, tune_ms = w.get_tuning(), None
+    if not args.no_autotune:
+        # launch shapes timed on a state from the middle of the timed window
+        # (where the contacts are), then back to the post-warmup state
+        snap = w.get_state()
+        w.step(max(args.steps // 2, 1))
+        tuning, tune_ms = w.autotune(steps=5)
+        w.set_state(**snap)
+        w.use_graph(not args.eager)
     t0 = time.perf_counter()
     w.step(args.steps)           # synchronises on completion
     host_s = time.perf_counter() - t0
@@ -183,9 +210,10 @@ def main():
         "data": "synthetic: seeded PCG32 scene (random hull shapes, positions, orientations, "
                 "velocities, masses)",
         "config": {"workload": args.workload, "bodies": n, "solver_iterations": scene.iterations,
-                   "dt": scene.dt, "graph": True, "pairs": int(pairs),
+                   "dt": scene.dt, "graph": not args.eager, "pairs": int(pairs),
                    "contacts": int(contacts), "key_bits": prm["key_bits"],
-                   "sort_passes": passes, "host_s": round(host_s, 4)},
+                   "sort_passes": passes, "host_s": round(host_s, 4), "tuning": tuning},
+        "autotune_ms_per_step": tune_ms,
         "stage_ms": breakdown,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -26,6 +26,10 @@ MLP_NW, MLP_NB = 64 * 16 + 64 * 64 + 3 * 64, 64 + 64 + 3
  STAGE_SOLVE, STAGE_INTEGRATE) = range(8)
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("solve_block", ctypes.c_int), ("narrow_waves", ctypes.c_int)]
+
+
 class Config(ctypes.Structure):
     _fields_ = [
         ("max_bodies", ctypes.c_int), ("max_shapes", ctypes.c_int),
@@ -47,7 +51,8 @@ class Stats(ctypes.Structure):
         ("num_colors", ctypes.c_int), ("overflow_flags", ctypes.c_int),
         ("normal_iterations", ctypes.c_int), ("friction_iterations", ctypes.c_int),
         ("normal_residual", ctypes.c_float), ("friction_residual", ctypes.c_float),
-        ("stage_ms", ctypes.c_float * 8), ("steps", ctypes.c_longlong)]
+        ("stage_ms", ctypes.c_float * 8), ("steps", ctypes.c_longlong),
+        ("mean_normal_iterations", ctypes.c_float), ("mean_friction_iterations", ctypes.c_float)]
 
 
 class Params(ctypes.Structure):
@@ -107,6 +112,9 @@ SIGNATURES = {
     "mi_rb_load_state": (I, [VP, CP]),
     "mi_rb_set_warmstart_weights": (I, [VP, VP, VP]),
     "mi_rb_copy_training_batch": (I, [VP, VP, VP, I, PI]),
+    "mi_rb_set_tuning": (I, [VP, ctypes.POINTER(Tuning)]),
+    "mi_rb_get_tuning": (I, [VP, ctypes.POINTER(Tuning)]),
+    "mi_rb_autotune": (I, [VP, I, ctypes.POINTER(Tuning), VP]),
     "mi_rb_mlp_create": (I, [ctypes.c_uint64, PVP]),
     "mi_rb_mlp_destroy": (None, [VP]),
     "mi_rb_mlp_set_weights": (I, [VP, VP, VP]),

@@ -304,10 +304,26 @@ __global__ __launch_bounds__(256) void k_emit_pairs(
         *num_pairs = (int)tt;
     }
     if (x >= n) return;
+    if (is_large && is_large[x]) return;  // long lists: k_emit_large
     uint32_t c = pair_count[x], off = pair_off[x];
-    const int *src = (is_large && is_large[x]) ? large_slots + large_off[large_index[x]]
-                                               : body_slots + (size_t)x * K;
+    const int *src = body_slots + (size_t)x * K;
     for (uint32_t k = 0; k < c; k++) {
+        uint32_t o = off + k;
+        if (o < (uint32_t)max_pairs) pairs[o] = make_int2(x, src[k]);
+    }
+}
+
+// Pair lists of the overflow-list bodies can hold a whole scene's worth of
+// partners: spread each one over a grid row of workgroups instead of one thread.
+__global__ __launch_bounds__(256) void k_emit_large(
+    const int *__restrict__ large_list, const uint32_t *__restrict__ pair_count,
+    const uint32_t *__restrict__ pair_off, const int *__restrict__ large_off,
+    const int *__restrict__ large_slots, int max_pairs, int2 *__restrict__ pairs) {
+    const int q = blockIdx.x;
+    const int x = large_list[q];
+    const uint32_t c = pair_count[x], off = pair_off[x];
+    const int *src = large_slots + large_off[q];
+    for (uint32_t k = blockIdx.y * blockDim.x + threadIdx.x; k < c; k += gridDim.y * blockDim.x) {
         uint32_t o = off + k;
         if (o < (uint32_t)max_pairs) pairs[o] = make_int2(x, src[k]);
     }
@@ -374,6 +390,10 @@ hipError_t launch_emit_pairs(const DeviceWorld &d, const StepParams &p, hipStrea
                        d.large_off, d.large_slots, p.max_pairs, d.pairs,
                        (const uint32_t *)(d.counters + RB_CNT_SORT_TILE),
                        d.counters + RB_CNT_PAIRS, d.counters + RB_CNT_OVERFLOW);
+    if (p.n_large > 0)
+        hipLaunchKernelGGL(k_emit_large, dim3(p.n_large, 32), dim3(256), 0, s, d.large_list,
+                           d.pair_count, d.pair_off, d.large_off, d.large_slots, p.max_pairs,
+                           d.pairs);
     return hipGetLastError();
 }
 

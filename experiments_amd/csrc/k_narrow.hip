@@ -54,27 +54,52 @@ __device__ __forceinline__ int wave_max_i32(int v) {
     return v;
 }
 
-// project(): min/max of the hull's local vertices along dir (world), plus dot(pos, dir)
-// The vertex list in LDS is padded to a multiple of 8 with copies of its last
-// vertex (which leaves min and max unchanged), so the loop runs 8 independent
-// min/max chains and keeps 8 LDS reads in flight.
-__device__ __forceinline__ void project(const float4 *sv, int nv8, float4 pos, float4 orn,
+// Hull vertices are read through the scalar data cache: every lane needs
+// every vertex for its own axis and the vertex address is the same on all
+// lanes, so a scalar load hands the vertex to the whole wave as SGPR operands.
+// (An LDS broadcast of a float4 still moves 1 KB per wave and made the axis
+// loops LDS-bound.)  The vertex arrays are read-only for the whole step.
+struct Vtx { float x, y, z, w; };
+struct Vtx4 { Vtx v[4]; };  // four consecutive vertices: one 64-byte scalar load
+typedef __attribute__((address_space(4))) const Vtx cfloat4;
+typedef __attribute__((address_space(4))) const Vtx4 cVtx4;
+
+// project(): min/max of the hull's local vertices along dir (world), plus
+// dot(pos, dir) (sat.cl:227-250).  Four independent min/max chains over
+// groups of four vertices; the last group repeats the last vertex, which
+// leaves min and max unchanged.  fminf/fmaxf equal the reference's compare
+// and select for every non-NaN input and ignore a NaN projection the same
+// way; only the sign of an exact zero extreme may differ, which no overlap
+// comparison can see.
+__device__ __forceinline__ void project(const cfloat4 *sv, int nv, float4 pos, float4 orn,
                                         float4 dir, float &mn, float &mx) {
-    float lo[8], hi[8];
+    float lo[4], hi[4];
 #pragma unroll
-    for (int k = 0; k < 8; k++) { lo[k] = FLT_MAX; hi[k] = -FLT_MAX; }
+    for (int k = 0; k < 4; k++) { lo[k] = FLT_MAX; hi[k] = -FLT_MAX; }
     float4 ld = qt_inv_rotate(orn, dir);
     float off = dot3(pos, dir);
-    for (int i = 0; i < nv8; i += 8) {
+    int i = 0;
+    for (; i + 4 <= nv; i += 4) {
+        const cVtx4 *q = (const cVtx4 *)(sv + i);
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-            float dp = dot3(sv[i + k], ld);
-            lo[k] = dp < lo[k] ? dp : lo[k];
-            hi[k] = dp > hi[k] ? dp : hi[k];
+        for (int k = 0; k < 4; k++) {
+            float dp = (q->v[k].x * ld.x + q->v[k].y * ld.y) + q->v[k].z * ld.z;
+            lo[k] = fminf(lo[k], dp);
+            hi[k] = fmaxf(hi[k], dp);
+        }
+    }
+    if (i < nv) {
+        const int last = nv - 1;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const cfloat4 *v = sv + (i + k < last ? i + k : last);
+            float dp = (v->x * ld.x + v->y * ld.y) + v->z * ld.z;
+            lo[k] = fminf(lo[k], dp);
+            hi[k] = fmaxf(hi[k], dp);
         }
     }
 #pragma unroll
-    for (int k = 1; k < 8; k++) {
+    for (int k = 1; k < 4; k++) {
         lo[0] = lo[k] < lo[0] ? lo[k] : lo[0];
         hi[0] = hi[k] > hi[0] ? hi[k] : hi[0];
     }
@@ -84,43 +109,42 @@ __device__ __forceinline__ void project(const float4 *sv, int nv8, float4 pos, f
     mx = h + off;
 }
 
-__device__ __forceinline__ int pad8(int n) { return (n + 7) & ~7; }
-
 constexpr int NP_BLOCK = 64;
 
-__global__ __launch_bounds__(NP_BLOCK) void k_narrowphase(
+__device__ __forceinline__ bool hull_limits_ok(const HullInfo &hA, const HullInfo &hB) {
+    return hA.num_vertices <= RB_MAX_HULL_VERTS && hB.num_vertices <= RB_MAX_HULL_VERTS &&
+           hA.num_edges <= RB_MAX_HULL_EDGES && hB.num_edges <= RB_MAX_HULL_EDGES &&
+           hA.num_faces <= 2 * NP_BLOCK && hB.num_faces <= 2 * NP_BLOCK;
+}
+
+// Pass 1, every pair: face axes of A and B.  A pair separated by a face axis
+// (or static-static, or beyond the hull limits) has no contact; the others
+// become candidates for pass 2 with their face minimum (depth, axis) and the
+// number of edge-edge axes still to test, the cost key of pass 2's order.
+__global__ __launch_bounds__(NP_BLOCK) void k_narrow_face(
     int max_pairs, const int *__restrict__ num_pairs_ptr, const int2 *__restrict__ pairs,
     const float4 *__restrict__ pos, const float4 *__restrict__ quat,
     const float *__restrict__ inv_mass, const int *__restrict__ shape,
     const HullInfo *__restrict__ hulls, const float4 *__restrict__ verts,
-    const float4 *__restrict__ planes, const int2 *__restrict__ face_index,
-    const int *__restrict__ indices, const float4 *__restrict__ edges, float friction,
-    float max_dist, Contact4 *__restrict__ slots, uint32_t *__restrict__ has_contact,
-    int *overflow) {
-    __shared__ float4 sVA[RB_MAX_HULL_VERTS], sVB[RB_MAX_HULL_VERTS];
-    __shared__ float4 sEA[RB_MAX_HULL_EDGES], sEB[RB_MAX_HULL_EDGES];
-    __shared__ float4 sPoly[2][RB_MAX_POLY];
-    __shared__ float4 sPts[RB_MAX_POLY];
-
+    const float4 *__restrict__ planes, uint32_t *__restrict__ has_contact,
+    uint32_t *__restrict__ cand, int4 *__restrict__ face_best, int *overflow) {
     const int lane = threadIdx.x;
     int npairs = *num_pairs_ptr;
     if (npairs > max_pairs) npairs = max_pairs;
-
     for (int p = blockIdx.x; p < npairs; p += gridDim.x) {
-        int2 pr = pairs[p];
+        const int2 pr = pairs[p];
         int a = pr.x, b = pr.y;
         float imA = inv_mass[a], imB = inv_mass[b];
         if (imA == 0.f && imB == 0.f) {
-            if (lane == 0) has_contact[p] = 0;
+            if (lane == 0) { has_contact[p] = 0; cand[p] = 0; }
             continue;
         }
         const HullInfo hA = hulls[shape[a]];
         const HullInfo hB = hulls[shape[b]];
-        if (hA.num_vertices > RB_MAX_HULL_VERTS || hB.num_vertices > RB_MAX_HULL_VERTS ||
-            hA.num_edges > RB_MAX_HULL_EDGES || hB.num_edges > RB_MAX_HULL_EDGES ||
-            hA.num_faces > 2 * NP_BLOCK || hB.num_faces > 2 * NP_BLOCK) {
+        if (!hull_limits_ok(hA, hB)) {
             if (lane == 0) {
                 has_contact[p] = 0;
+                cand[p] = 0;
                 atomicOr(overflow, (int)MI_RB_OVERFLOW_POLYGON);
             }
             continue;
@@ -128,17 +152,11 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrowphase(
         float4 pA = pos[a], pB = pos[b], oA = quat[a], oB = quat[b];
         pA.w = 0.f;
         pB.w = 0.f;
-        __syncthreads();
-        for (int i = lane; i < pad8(hA.num_vertices); i += NP_BLOCK)
-            sVA[i] = verts[hA.vertex_offset + (i < hA.num_vertices ? i : hA.num_vertices - 1)];
-        for (int i = lane; i < pad8(hB.num_vertices); i += NP_BLOCK)
-            sVB[i] = verts[hB.vertex_offset + (i < hB.num_vertices ? i : hB.num_vertices - 1)];
-        for (int i = lane; i < hA.num_edges; i += NP_BLOCK)
-            sEA[i] = qt_rotate(oA, edges[hA.edge_offset + i]);
-        for (int i = lane; i < hB.num_edges; i += NP_BLOCK)
-            sEB[i] = qt_rotate(oB, edges[hB.edge_offset + i]);
-        __syncthreads();
-
+        // counts and offsets are the same on every lane: make them scalar
+        const int nvA = __builtin_amdgcn_readfirstlane(hA.num_vertices);
+        const int nvB = __builtin_amdgcn_readfirstlane(hB.num_vertices);
+        const cfloat4 *sVA = (const cfloat4 *)(verts + __builtin_amdgcn_readfirstlane(hA.vertex_offset));
+        const cfloat4 *sVB = (const cfloat4 *)(verts + __builtin_amdgcn_readfirstlane(hB.vertex_offset));
         float4 dc = sub4(xform(hA.center, pA, oA), xform(hB.center, pB, oB));
 
         // ---- SAT, face axes ----
@@ -153,11 +171,11 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrowphase(
             if (dot3(dc, w) < 0) w = scl4(w, -1.f);
             float mn0, mx0, mn1, mx1;
             if (onA) {
-                project(sVA, pad8(hA.num_vertices), pA, oA, w, mn0, mx0);
-                project(sVB, pad8(hB.num_vertices), pB, oB, w, mn1, mx1);
+                project(sVA, nvA, pA, oA, w, mn0, mx0);
+                project(sVB, nvB, pB, oB, w, mn1, mx1);
             } else {
-                project(sVB, pad8(hB.num_vertices), pB, oB, w, mn0, mx0);
-                project(sVA, pad8(hA.num_vertices), pA, oA, w, mn1, mx1);
+                project(sVB, nvB, pB, oB, w, mn0, mx0);
+                project(sVA, nvA, pA, oA, w, mn1, mx1);
             }
             if (mx0 < mn1 || mx1 < mn0) {
                 sep_found = 1;
@@ -168,10 +186,85 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrowphase(
             }
         }
         if (__ballot(sep_found)) {
-            if (lane == 0) has_contact[p] = 0;
+            if (lane == 0) { has_contact[p] = 0; cand[p] = 0; }
             continue;
         }
-        // ---- SAT, edge x edge axes (never reject) ----
+        wave_argmin(best, best_ax);
+        if (lane == 0) {
+            cand[p] = 1;
+            face_best[p] = make_int4(__float_as_int(best), best_ax, hA.num_edges * hB.num_edges, 0);
+        }
+    }
+}
+
+// Candidates in descending order of edge-edge axis count (stable).
+__global__ __launch_bounds__(256) void k_narrow_keys(int max_pairs, const int *__restrict__ num_pairs_ptr,
+                                                     const uint32_t *__restrict__ cand,
+                                                     const uint32_t *__restrict__ cand_off,
+                                                     const int4 *__restrict__ face_best,
+                                                     uint2 *__restrict__ kv) {
+    int npairs = *num_pairs_ptr;
+    if (npairs > max_pairs) npairs = max_pairs;
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npairs || !cand[p]) return;
+    int ne = face_best[p].z;
+    kv[cand_off[p]] = make_uint2(0xffffu - (uint32_t)(ne < 0xffff ? ne : 0xffff), (uint32_t)p);
+}
+
+// Pass 2, the candidates: edge x edge axes (never reject), then clipping of
+// B's incident face against A's reference face and the reduction to <= 4
+// points.  Workgroups take candidates in boustrophedon order over the
+// cost-sorted list (round r: workgroup b takes r*G + b, or r*G + G-1-b on odd
+// rounds), which spreads the expensive pairs over all workgroups.
+__global__ __launch_bounds__(NP_BLOCK) void k_narrow_clip(
+    const uint32_t *__restrict__ n_cand_ptr, const uint2 *__restrict__ order,
+    const int4 *__restrict__ face_best, const int2 *__restrict__ pairs,
+    const float4 *__restrict__ pos, const float4 *__restrict__ quat,
+    const float *__restrict__ inv_mass, const int *__restrict__ shape,
+    const HullInfo *__restrict__ hulls, const float4 *__restrict__ verts,
+    const float4 *__restrict__ planes, const int2 *__restrict__ face_index,
+    const int *__restrict__ indices, const float4 *__restrict__ edges, float friction,
+    float max_dist, Contact4 *__restrict__ slots, uint32_t *__restrict__ has_contact,
+    int *overflow) {
+    __shared__ float4 sEA[RB_MAX_HULL_EDGES], sEB[RB_MAX_HULL_EDGES];
+    __shared__ float4 sPoly[2][RB_MAX_POLY];
+    __shared__ float4 sPts[RB_MAX_POLY];
+
+    const int lane = threadIdx.x;
+    const int n_cand = (int)*n_cand_ptr;
+    const int G = gridDim.x;
+    for (int r = 0;; r++) {
+        const int i = r * G + ((r & 1) ? (G - 1 - (int)blockIdx.x) : (int)blockIdx.x);
+        if (i >= n_cand) break;
+        const int p = (int)order[i].y;
+        const int2 pr = pairs[p];
+        int a = pr.x, b = pr.y;
+        float imA = inv_mass[a], imB = inv_mass[b];
+        const HullInfo hA = hulls[shape[a]];
+        const HullInfo hB = hulls[shape[b]];
+        float4 pA = pos[a], pB = pos[b], oA = quat[a], oB = quat[b];
+        pA.w = 0.f;
+        pB.w = 0.f;
+        __syncthreads();
+        // counts and offsets are the same on every lane: make them scalar
+        const int nvA = __builtin_amdgcn_readfirstlane(hA.num_vertices);
+        const int nvB = __builtin_amdgcn_readfirstlane(hB.num_vertices);
+        const cfloat4 *sVA = (const cfloat4 *)(verts + __builtin_amdgcn_readfirstlane(hA.vertex_offset));
+        const cfloat4 *sVB = (const cfloat4 *)(verts + __builtin_amdgcn_readfirstlane(hB.vertex_offset));
+        for (int i = lane; i < hA.num_edges; i += NP_BLOCK)
+            sEA[i] = qt_rotate(oA, edges[hA.edge_offset + i]);
+        for (int i = lane; i < hB.num_edges; i += NP_BLOCK)
+            sEB[i] = qt_rotate(oB, edges[hB.edge_offset + i]);
+        __syncthreads();
+
+        float4 dc = sub4(xform(hA.center, pA, oA), xform(hB.center, pB, oB));
+
+        // lane-local minimum over the edge x edge axes, then the wave's,
+        // then against the face minimum of the first pass: face axes come
+        // first in the axis order, so a tie keeps the face axis
+        float best = FLT_MAX;
+        int best_ax = 0x7fffffff;
+        const int nf = hA.num_faces + hB.num_faces;
         const int ne = hA.num_edges * hB.num_edges;
         for (int k = lane; k < ne; k += NP_BLOCK) {
             int e0 = k / hB.num_edges, e1 = k - e0 * hB.num_edges;
@@ -180,14 +273,19 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrowphase(
                 c = normalize3(c);
                 if (dot3(dc, c) < 0) c = scl4(c, -1.f);
                 float mn0, mx0, mn1, mx1;
-                project(sVA, pad8(hA.num_vertices), pA, oA, c, mn0, mx0);
-                project(sVB, pad8(hB.num_vertices), pB, oB, c, mn1, mx1);
+                project(sVA, nvA, pA, oA, c, mn0, mx0);
+                project(sVB, nvB, pB, oB, c, mn1, mx1);
                 float d0 = mx0 - mn1, d1 = mx1 - mn0;
                 float dist = d0 < d1 ? d0 : d1;
                 if (dist < best) { best = dist; best_ax = nf + k; }
             }
         }
         wave_argmin(best, best_ax);
+        {
+            const int4 fb = face_best[p];
+            const float fd = __int_as_float(fb.x);
+            if (fd < best || (fd == best && fb.y < best_ax)) { best = fd; best_ax = fb.y; }
+        }
         if (best_ax == 0x7fffffff) {
             if (lane == 0) has_contact[p] = 0;
             continue;
@@ -344,11 +442,26 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrowphase(
 
 }  // namespace
 
-hipError_t launch_narrowphase(const DeviceWorld &d, const StepParams &p, hipStream_t s) {
-    int grid = rb_num_cus() * 16;
-    hipLaunchKernelGGL(k_narrowphase, dim3(grid), dim3(NP_BLOCK), 0, s, p.max_pairs,
+hipError_t launch_narrowphase(const DeviceWorld &d, const StepParams &p, hipStream_t s,
+                              const SortWork &sw, uint32_t *scan_status) {
+    const int grid = rb_num_cus() * 4 * (p.narrow_waves > 0 ? p.narrow_waves : 4);
+    hipLaunchKernelGGL(k_narrow_face, dim3(grid), dim3(NP_BLOCK), 0, s, p.max_pairs,
                        d.counters + RB_CNT_PAIRS, d.pairs, d.pos, d.quat, d.inv_mass, d.shape,
-                       d.hulls, d.verts, d.planes, d.face_index, d.indices, d.edges, p.friction,
-                       p.threshold, d.contact_slots, d.has_contact, d.counters + RB_CNT_OVERFLOW);
+                       d.hulls, d.verts, d.planes, d.has_contact, d.narrow_cand, d.face_best,
+                       d.counters + RB_CNT_OVERFLOW);
+    hipError_t e = launch_exclusive_scan(d.narrow_cand, d.cand_off, p.max_pairs,
+                                         (uint32_t *)(d.counters + RB_CNT_NARROW_CAND), scan_status,
+                                         s, d.counters + RB_CNT_PAIRS);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_narrow_keys, dim3((p.max_pairs + 255) / 256), dim3(256), 0, s,
+                       p.max_pairs, d.counters + RB_CNT_PAIRS, d.narrow_cand, d.cand_off,
+                       d.face_best, d.cand_kv);
+    e = launch_radix_sort(d.cand_kv, p.max_pairs, 16, 8, sw, s, d.counters + RB_CNT_NARROW_CAND);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_narrow_clip, dim3(grid), dim3(NP_BLOCK), 0, s,
+                       (const uint32_t *)(d.counters + RB_CNT_NARROW_CAND), d.cand_kv, d.face_best,
+                       d.pairs, d.pos, d.quat, d.inv_mass, d.shape, d.hulls, d.verts, d.planes,
+                       d.face_index, d.indices, d.edges, p.friction, p.threshold, d.contact_slots,
+                       d.has_contact, d.counters + RB_CNT_OVERFLOW);
     return hipGetLastError();
 }

@@ -29,7 +29,8 @@ using namespace rb;
 
 namespace {
 
-constexpr int TB = 256;  // threads per tile workgroup
+constexpr int TB = 256;        // threads per tile workgroup (colouring, spatial solve tiles)
+constexpr int TB_SCENE = 512;  // threads per scene tile in the solve
 constexpr int HASH_SLOTS = 2 * RB_TILE_MAX_BODIES;
 
 __device__ __forceinline__ uint32_t color_prio(uint32_t k) {
@@ -386,19 +387,59 @@ struct SolveLDS {
     float4 w[SOLVE_SLOTS];
 };
 
-__device__ __forceinline__ void solve_row_normal(int k, const ConstraintSoA &cs, float4 &vA,
-                                                 float4 &wA, float4 &vB, float4 &wB, float4 pA,
-                                                 float4 pB, float imA, const float4 *IA, float imB,
-                                                 const float4 *IB, float &rmax, float &lmax) {
-    float4 lin0 = cs.linear[k];
-    float4 jac = cs.jac_inv[k], bb = cs.b[k], lam = cs.lambda[k];
-    float jv[4] = {jac.x, jac.y, jac.z, jac.w};
-    float bv[4] = {bb.x, bb.y, bb.z, bb.w};
-    float lv[4] = {lam.x, lam.y, lam.z, lam.w};
+// Constraint fields one row needs (Constraint4 minus the body indices).
+struct RowData {
+    float4 lin0, jac, bb, lam, fr, cen;
+    float4 p[4];
+};
+// The row's two bodies: inverse mass, world inverse inertia, position and the
+// LDS cache slot of the velocity (-1: static, or kept in global memory).
+struct RowBodies {
+    int a, b, ha, hb;
+    float imA, imB;
+    float4 IA[3], IB[3];
+    float4 pA, pB;
+};
+
+__device__ __forceinline__ void load_row(int k, const ConstraintSoA &cs, RowData &d) {
+    d.lin0 = cs.linear[k];
+    d.jac = cs.jac_inv[k];
+    d.bb = cs.b[k];
+    d.lam = cs.lambda[k];
+    d.fr = cs.fric[k];
+    d.cen = cs.center[k];
+    for (int i = 0; i < 4; i++) d.p[i] = cs.pos[4 * k + i];
+}
+
+__device__ __forceinline__ void load_bodies(int2 bd, const float *inv_mass, const float4 *invI,
+                                            const float4 *pos, const int *key, RowBodies &rb) {
+    rb.a = bd.x;
+    rb.b = bd.y;
+    rb.imA = inv_mass[bd.x];
+    rb.imB = inv_mass[bd.y];
+    for (int i = 0; i < 3; i++) {
+        rb.IA[i] = invI[3 * bd.x + i];
+        rb.IB[i] = invI[3 * bd.y + i];
+    }
+    rb.pA = pos[bd.x];
+    rb.pB = pos[bd.y];
+    rb.ha = rb.imA != 0.f ? hash_find<SOLVE_SLOTS>(key, bd.x) : -1;
+    rb.hb = rb.imB != 0.f ? hash_find<SOLVE_SLOTS>(key, bd.y) : -1;
+}
+
+// solveContact (solveContact.cl:272-312): sequential impulses on the normal
+// rows, lambda >= 0.
+__device__ __forceinline__ void row_normal(RowData &d, float4 &vA, float4 &wA, float4 &vB,
+                                           float4 &wB, const RowBodies &rb, float &rmax,
+                                           float &lmax) {
+    const float4 lin0 = d.lin0;
+    float jv[4] = {d.jac.x, d.jac.y, d.jac.z, d.jac.w};
+    float bv[4] = {d.bb.x, d.bb.y, d.bb.z, d.bb.w};
+    float lv[4] = {d.lam.x, d.lam.y, d.lam.z, d.lam.w};
     for (int ic = 0; ic < 4; ic++) {
         if (jv[ic] == 0.f) continue;
-        float4 wp = cs.pos[4 * k + ic];
-        float4 r0 = sub4(wp, pA), r1 = sub4(wp, pB);
+        float4 wp = d.p[ic];
+        float4 r0 = sub4(wp, rb.pA), r1 = sub4(wp, rb.pB);
         float4 n = neg4(lin0);
         float4 lin = neg4(n);
         float4 a0 = neg4(cross3(r0, n));
@@ -411,35 +452,35 @@ __device__ __forceinline__ void solve_row_normal(int k, const ConstraintSoA &cs,
         upd = fminf(upd, FLT_MAX);
         rl = upd - prev;
         lv[ic] = upd;
-        float4 li0 = scl4(scl4(lin, imA), rl);
-        float4 li1 = scl4(scl4(neg4(lin), imB), rl);
-        float4 ai0 = scl4(mt_mul1(IA, a0), rl);
-        float4 ai1 = scl4(mt_mul1(IB, a1), rl);
+        float4 li0 = scl4(scl4(lin, rb.imA), rl);
+        float4 li1 = scl4(scl4(neg4(lin), rb.imB), rl);
+        float4 ai0 = scl4(mt_mul1(rb.IA, a0), rl);
+        float4 ai1 = scl4(mt_mul1(rb.IB, a1), rl);
         vA = add4(vA, li0); wA = add4(wA, ai0);
         vB = add4(vB, li1); wB = add4(wB, ai1);
         float sc = fabsf(rl) / fabsf(jv[ic]);
         rmax = fmaxf(rmax, sc);
         lmax = fmaxf(lmax, fabsf(upd));
     }
-    cs.lambda[k] = mk4(lv[0], lv[1], lv[2], lv[3]);
+    d.lam = mk4(lv[0], lv[1], lv[2], lv[3]);
 }
 
-__device__ __forceinline__ void solve_row_friction(int k, const ConstraintSoA &cs, float4 &vA,
-                                                   float4 &wA, float4 &vB, float4 &wB, float4 pA,
-                                                   float4 pB, float imA, const float4 *IA,
-                                                   float imB, const float4 *IB, float friction,
-                                                   float &rmax, float &lmax) {
-    float4 lam = cs.lambda[k];
+// solveFriction (solveFriction.cl:297-430): two tangents at the manifold
+// centre, |lambda| <= mu * sum lambda_n, then the angular damping about n.
+__device__ __forceinline__ void row_friction(RowData &d, float4 &vA, float4 &wA, float4 &vB,
+                                             float4 &wB, const RowBodies &rb, float friction,
+                                             float &rmax, float &lmax) {
+    const float4 lam = d.lam;
     float sum = 0;
     sum += lam.x; sum += lam.y; sum += lam.z; sum += lam.w;
     float mx = friction * sum, mn = -mx;
-    float4 fr = cs.fric[k];
+    const float4 fr = d.fr;
     if (fr.x == 0 && fr.x == 0) return;
-    float4 cen = cs.center[k];
-    float4 n = neg4(cs.linear[k]);
+    const float4 cen = d.cen;
+    float4 n = neg4(d.lin0);
     float4 t[2];
     plane_space(n, t[0], t[1]);
-    float4 r0 = sub4(cen, pA), r1 = sub4(cen, pB);
+    float4 r0 = sub4(cen, rb.pA), r1 = sub4(cen, rb.pB);
     float fj[2] = {fr.x, fr.y};
     float fl[2] = {fr.z, fr.w};
     for (int i = 0; i < 2; i++) {
@@ -454,65 +495,61 @@ __device__ __forceinline__ void solve_row_friction(int k, const ConstraintSoA &c
         upd = fminf(upd, mx);
         rl = upd - prev;
         fl[i] = upd;
-        float4 li0 = scl4(scl4(lin, imA), rl);
-        float4 li1 = scl4(scl4(neg4(lin), imB), rl);
-        float4 ai0 = scl4(mt_mul1(IA, a0), rl);
-        float4 ai1 = scl4(mt_mul1(IB, a1), rl);
+        float4 li0 = scl4(scl4(lin, rb.imA), rl);
+        float4 li1 = scl4(scl4(neg4(lin), rb.imB), rl);
+        float4 ai0 = scl4(mt_mul1(rb.IA, a0), rl);
+        float4 ai1 = scl4(mt_mul1(rb.IB, a1), rl);
         vA = add4(vA, li0); wA = add4(wA, ai0);
         vB = add4(vB, li1); wB = add4(wB, ai1);
         float sc = fabsf(rl) / fabsf(fj[i]);
         rmax = fmaxf(rmax, sc);
         lmax = fmaxf(lmax, fabsf(upd));
     }
-    cs.fric[k] = mk4(fr.x, fr.y, fl[0], fl[1]);
-    float4 ab = normalize3(sub4(pB, pA));
-    float4 ac = normalize3(sub4(cen, pA));
-    if (dot3(ab, ac) > 0.95f || (imA == 0.f || imB == 0.f)) {
+    d.fr = mk4(fr.x, fr.y, fl[0], fl[1]);
+    float4 ab = normalize3(sub4(rb.pB, rb.pA));
+    float4 ac = normalize3(sub4(cen, rb.pA));
+    if (dot3(ab, ac) > 0.95f || (rb.imA == 0.f || rb.imB == 0.f)) {
         float na = dot3(n, wA), nb = dot3(n, wB);
         wA = sub4(wA, scl4(n, na * 0.1f));
         wB = sub4(wB, scl4(n, nb * 0.1f));
     }
 }
 
-// Apply the impulses a constraint already holds (warm start) to its bodies.
-__device__ __forceinline__ void apply_row_warm(int k, const ConstraintSoA &cs, float4 &vA,
-                                               float4 &wA, float4 &vB, float4 &wB, float4 pA,
-                                               float4 pB, float imA, const float4 *IA, float imB,
-                                               const float4 *IB) {
-    float4 lin0 = cs.linear[k];
-    float4 jac = cs.jac_inv[k], lam = cs.lambda[k];
-    float jv[4] = {jac.x, jac.y, jac.z, jac.w};
-    float lv[4] = {lam.x, lam.y, lam.z, lam.w};
+// Apply the impulses a row already holds (warm start) to its bodies.
+__device__ __forceinline__ void row_warm(const RowData &d, float4 &vA, float4 &wA, float4 &vB,
+                                         float4 &wB, const RowBodies &rb) {
+    const float4 lin0 = d.lin0;
+    float jv[4] = {d.jac.x, d.jac.y, d.jac.z, d.jac.w};
+    float lv[4] = {d.lam.x, d.lam.y, d.lam.z, d.lam.w};
     for (int ic = 0; ic < 4; ic++) {
         if (jv[ic] == 0.f || lv[ic] == 0.f) continue;
-        float4 wp = cs.pos[4 * k + ic];
-        float4 r0 = sub4(wp, pA), r1 = sub4(wp, pB);
+        float4 wp = d.p[ic];
+        float4 r0 = sub4(wp, rb.pA), r1 = sub4(wp, rb.pB);
         float4 n = neg4(lin0);
         float4 lin = neg4(n);
         float4 a0 = neg4(cross3(r0, n)), a1 = cross3(r1, n);
         float rl = lv[ic];
-        vA = add4(vA, scl4(scl4(lin, imA), rl));
-        wA = add4(wA, scl4(mt_mul1(IA, a0), rl));
-        vB = add4(vB, scl4(scl4(neg4(lin), imB), rl));
-        wB = add4(wB, scl4(mt_mul1(IB, a1), rl));
+        vA = add4(vA, scl4(scl4(lin, rb.imA), rl));
+        wA = add4(wA, scl4(mt_mul1(rb.IA, a0), rl));
+        vB = add4(vB, scl4(scl4(neg4(lin), rb.imB), rl));
+        wB = add4(wB, scl4(mt_mul1(rb.IB, a1), rl));
     }
-    float4 fr = cs.fric[k];
+    const float4 fr = d.fr;
     if (fr.x == 0.f) return;
-    float4 cen = cs.center[k];
     float4 n = neg4(lin0);
     float4 t[2];
     plane_space(n, t[0], t[1]);
-    float4 r0 = sub4(cen, pA), r1 = sub4(cen, pB);
+    float4 r0 = sub4(d.cen, rb.pA), r1 = sub4(d.cen, rb.pB);
     float fl[2] = {fr.z, fr.w};
     for (int i = 0; i < 2; i++) {
         float rl = fl[i];
         if (rl == 0.f) continue;
         float4 lin = neg4(t[i]);
         float4 a0 = neg4(cross3(r0, t[i])), a1 = cross3(r1, t[i]);
-        vA = add4(vA, scl4(scl4(lin, imA), rl));
-        wA = add4(wA, scl4(mt_mul1(IA, a0), rl));
-        vB = add4(vB, scl4(scl4(neg4(lin), imB), rl));
-        wB = add4(wB, scl4(mt_mul1(IB, a1), rl));
+        vA = add4(vA, scl4(scl4(lin, rb.imA), rl));
+        wA = add4(wA, scl4(mt_mul1(rb.IA, a0), rl));
+        vB = add4(vB, scl4(scl4(neg4(lin), rb.imB), rl));
+        wB = add4(wB, scl4(mt_mul1(rb.IB, a1), rl));
     }
 }
 
@@ -528,16 +565,130 @@ __global__ void k_save_prev(const int *counters, ConstraintSoA cs, int2 *prev_ke
     prev_imp[k] = mk4(((l.x + l.y) + l.z) + l.w, f.z, f.w, 0.f);
 }
 
-// One workgroup per tile.  mode: 0 normal sweeps, 1 friction sweeps.
-// sweeps: how many sweeps to run inside (scene tiles run all of them here).
-__global__ __launch_bounds__(TB) void k_tile_solve(
+// One row of a sweep (mode 0 normal, 1 friction, 2 warm start), velocities
+// from the LDS cache or, for bodies that did not fit it, global memory.
+__device__ __forceinline__ void run_row(int mode, RowData &d, const RowBodies &rb, SolveLDS &L,
+                                        float4 *linvel, float4 *angvel, float friction,
+                                        float &rmax, float &lmax) {
+    const float4 z = mk4(0, 0, 0, 0);
+    float4 vA = rb.ha >= 0 ? L.v[rb.ha] : (rb.imA != 0.f ? linvel[rb.a] : z);
+    float4 wA = rb.ha >= 0 ? L.w[rb.ha] : (rb.imA != 0.f ? angvel[rb.a] : z);
+    float4 vB = rb.hb >= 0 ? L.v[rb.hb] : (rb.imB != 0.f ? linvel[rb.b] : z);
+    float4 wB = rb.hb >= 0 ? L.w[rb.hb] : (rb.imB != 0.f ? angvel[rb.b] : z);
+    if (mode == 0) row_normal(d, vA, wA, vB, wB, rb, rmax, lmax);
+    else if (mode == 1) row_friction(d, vA, wA, vB, wB, rb, friction, rmax, lmax);
+    else row_warm(d, vA, wA, vB, wB, rb);
+    if (rb.ha >= 0) { L.v[rb.ha] = vA; L.w[rb.ha] = wA; }
+    else if (rb.imA != 0.f) { linvel[rb.a] = vA; angvel[rb.a] = wA; }
+    if (rb.hb >= 0) { L.v[rb.hb] = vB; L.w[rb.hb] = wB; }
+    else if (rb.imB != 0.f) { linvel[rb.b] = vB; angvel[rb.b] = wB; }
+}
+
+// Shared state of one tile's solve: the LDS body cache and the tile-wide
+// residual reduction of the current sweep.
+struct TileSolveLDS {
+    SolveLDS c;
+    unsigned int red[2];
+};
+
+// The sweeps of one tile: the optional warm-start pass, then normal sweeps,
+// then friction sweeps.  REG: the tile has at most TBK rows and row r lives in
+// the registers of thread r for the whole launch (load once, store once), so
+// a colour pass waits on no global load; otherwise rows are read from global
+// memory on every pass.  Within a colour rows share no dynamic body, so the
+// thread that runs a row does not change the result.  local_tol: a scene
+// tile ends each phase as soon as its own residual is below sp.tol.
+template <int TBK, bool REG>
+__device__ __forceinline__ void tile_sweeps(const StepParams &sp, int sweep0, int nsweeps_n,
+                                            int nsweeps_f, int warm, bool local_tol, int first,
+                                            int R, int nc, const int *coff, const int *order,
+                                            ConstraintSoA &cs, float4 *linvel, float4 *angvel,
+                                            const float4 *pos, const float *inv_mass,
+                                            const float4 *invI, float *residual,
+                                            TileSolveLDS &L, int used[2], float rfin[2]) {
+    const int tid = threadIdx.x;
+    RowData rd;
+    RowBodies rb;
+    int k = -1, rc = -1;
+    if (REG) {
+        k = tid < R ? order[first + tid] : -1;
+        if (k >= 0) {
+            load_row(k, cs, rd);
+            load_bodies(cs.bodies[k], inv_mass, invI, pos, L.c.key, rb);
+            // colour of this thread's row: the colour range holding row tid
+            for (int c = 0; c < nc; c++)
+                if (coff[c] <= tid && tid < coff[c + 1]) rc = c;
+        }
+    }
+    bool swept = false;
+    for (int phase = warm ? -1 : 0; phase < 2; phase++) {
+        const int nsw = phase < 0 ? 1 : (phase == 0 ? nsweeps_n : nsweeps_f);
+        const int mode = phase < 0 ? 2 : phase;
+        for (int it = 0; it < nsw; it++) {
+            float rmax = 0.f, lmax = 0.f;
+            if (local_tol) {
+                if (tid == 0) { L.red[0] = 0u; L.red[1] = 0u; }
+                __syncthreads();
+            }
+            for (int c = 0; c < nc; c++) {
+                if (REG) {
+                    if (rc == c)
+                        run_row(mode, rd, rb, L.c, linvel, angvel, sp.friction, rmax, lmax);
+                } else {
+                    int c0 = coff[c], c1 = coff[c + 1];
+                    if (c1 > R) c1 = R;
+                    for (int r = c0 + tid; r < c1; r += TBK) {
+                        int kk = order[first + r];
+                        RowData d;
+                        RowBodies b;
+                        load_row(kk, cs, d);
+                        load_bodies(cs.bodies[kk], inv_mass, invI, pos, L.c.key, b);
+                        run_row(mode, d, b, L.c, linvel, angvel, sp.friction, rmax, lmax);
+                        if (mode == 0) cs.lambda[kk] = d.lam;
+                        else if (mode == 1) cs.fric[kk] = d.fr;
+                    }
+                }
+                __syncthreads();
+            }
+            if (phase < 0) continue;
+            swept = true;
+            const int sweep = sweep0 + (phase == 0 ? it : nsweeps_n + it);
+            if (rmax > 0.f || lmax > 0.f) {
+                atomicMax((unsigned int *)&residual[2 * sweep], __float_as_uint(rmax));
+                atomicMax((unsigned int *)&residual[2 * sweep + 1], __float_as_uint(lmax));
+            }
+            if (local_tol) {
+                // rmax, lmax >= 0: their float bits order like the values
+                if (rmax > 0.f) atomicMax(&L.red[0], __float_as_uint(rmax));
+                if (lmax > 0.f) atomicMax(&L.red[1], __float_as_uint(lmax));
+                __syncthreads();
+                float rm = __uint_as_float(L.red[0]), lm = __uint_as_float(L.red[1]);
+                float r = rm / (lm > 1.f ? lm : 1.f);
+                used[phase] = it + 1;
+                rfin[phase] = r;
+                __syncthreads();  // every thread has read red before the next reset
+                if (r < sp.tol) break;
+            }
+        }
+    }
+    if (REG && k >= 0 && swept) {
+        cs.lambda[k] = rd.lam;
+        cs.fric[k] = rd.fr;
+    }
+}
+
+// One workgroup per tile (spatial: the tiles of one parity phase; scene: one
+// scene); the tile's body velocities live in LDS for the launch.
+template <int TBK>
+__global__ __launch_bounds__(TBK) void k_tile_solve(
     StepParams sp, int phase, int sweep0, int nsweeps_n, int nsweeps_f, int warm,
     const int *__restrict__ tile_first, const int *__restrict__ tile_cnt,
     const int *__restrict__ order, const int *__restrict__ tile_color_off,
     const int *__restrict__ tile_ncolors, ConstraintSoA cs, float4 *__restrict__ linvel,
     float4 *__restrict__ angvel, const float4 *__restrict__ pos, const float *__restrict__ inv_mass,
-    const float4 *__restrict__ invI, float *__restrict__ residual, int *counters) {
-    __shared__ SolveLDS L;
+    const float4 *__restrict__ invI, float *__restrict__ residual, int *counters,
+    int4 *__restrict__ tile_stats) {
+    __shared__ TileSolveLDS L;
     int t;
     if (sp.tile_mode == 1) {
         t = blockIdx.x;
@@ -550,105 +701,86 @@ __global__ __launch_bounds__(TB) void k_tile_solve(
         if (tx >= sp.tiles_x || tz >= sp.tiles_z) return;
         t = tz * sp.tiles_x + tx;
     }
+    const bool local_tol = sp.tile_mode == 1 && sp.tol > 0.f;
+    const int tid = threadIdx.x;
     if (counters[RB_CNT_DONE] != 0) return;  // converged flag set by a previous sweep
     const int first = tile_first[t];
     int R = tile_cnt[t] - first;
-    if (R <= 0) return;
+    if (R <= 0) {
+        if (local_tol && tid == 0) tile_stats[t] = make_int4(-1, -1, 0, 0);
+        return;
+    }
     if (R > RB_TILE_MAX_ROWS) R = RB_TILE_MAX_ROWS;
     const int nc = tile_ncolors[t];
     const int *coff = tile_color_off + t * (RB_MAX_LOCAL_COLORS + 1);
-    for (int i = threadIdx.x; i < SOLVE_SLOTS; i += TB) L.key[i] = -1;
+    for (int i = tid; i < SOLVE_SLOTS; i += TBK) L.c.key[i] = -1;
     __syncthreads();
-    for (int r = threadIdx.x; r < R; r += TB) {
-        int k = order[first + r];
-        int2 bd = cs.bodies[k];
+    // body velocities into the LDS cache
+    for (int r = tid; r < R; r += TBK) {
+        int2 bd = cs.bodies[order[first + r]];
         if (inv_mass[bd.x] != 0.f) {
-            int h = hash_insert<SOLVE_SLOTS>(L.key, bd.x);
-            if (h >= 0) { L.v[h] = linvel[bd.x]; L.w[h] = angvel[bd.x]; }
+            int h = hash_insert<SOLVE_SLOTS>(L.c.key, bd.x);
+            if (h >= 0) { L.c.v[h] = linvel[bd.x]; L.c.w[h] = angvel[bd.x]; }
             else atomicOr(counters + RB_CNT_OVERFLOW, (int)MI_RB_OVERFLOW_TILE);
         }
         if (inv_mass[bd.y] != 0.f) {
-            int h = hash_insert<SOLVE_SLOTS>(L.key, bd.y);
-            if (h >= 0) { L.v[h] = linvel[bd.y]; L.w[h] = angvel[bd.y]; }
+            int h = hash_insert<SOLVE_SLOTS>(L.c.key, bd.y);
+            if (h >= 0) { L.c.v[h] = linvel[bd.y]; L.c.w[h] = angvel[bd.y]; }
             else atomicOr(counters + RB_CNT_OVERFLOW, (int)MI_RB_OVERFLOW_TILE);
         }
     }
     __syncthreads();
-    if (warm) {
-        // warm start: apply the initial impulses colour by colour, in the
-        // order of a sweep
-        for (int c = 0; c < nc; c++) {
-            int c0 = coff[c], c1 = coff[c + 1];
-            if (c1 > R) c1 = R;
-            for (int r = c0 + threadIdx.x; r < c1; r += TB) {
-                int k = order[first + r];
-                int2 bd = cs.bodies[k];
-                int a = bd.x, b = bd.y;
-                float imA = inv_mass[a], imB = inv_mass[b];
-                int ha = imA != 0.f ? hash_find<SOLVE_SLOTS>(L.key, a) : -1;
-                int hb = imB != 0.f ? hash_find<SOLVE_SLOTS>(L.key, b) : -1;
-                float4 z = mk4(0, 0, 0, 0);
-                float4 vA = ha >= 0 ? L.v[ha] : (imA != 0.f ? linvel[a] : z);
-                float4 wA = ha >= 0 ? L.w[ha] : (imA != 0.f ? angvel[a] : z);
-                float4 vB = hb >= 0 ? L.v[hb] : (imB != 0.f ? linvel[b] : z);
-                float4 wB = hb >= 0 ? L.w[hb] : (imB != 0.f ? angvel[b] : z);
-                float4 IA[3] = {invI[3 * a], invI[3 * a + 1], invI[3 * a + 2]};
-                float4 IB[3] = {invI[3 * b], invI[3 * b + 1], invI[3 * b + 2]};
-                apply_row_warm(k, cs, vA, wA, vB, wB, pos[a], pos[b], imA, IA, imB, IB);
-                if (ha >= 0) { L.v[ha] = vA; L.w[ha] = wA; }
-                else if (imA != 0.f) { linvel[a] = vA; angvel[a] = wA; }
-                if (hb >= 0) { L.v[hb] = vB; L.w[hb] = wB; }
-                else if (imB != 0.f) { linvel[b] = vB; angvel[b] = wB; }
-            }
-            __syncthreads();
-        }
+    int used[2] = {0, 0};
+    float rfin[2] = {0.f, 0.f};
+    if (R <= TBK)
+        tile_sweeps<TBK, true>(sp, sweep0, nsweeps_n, nsweeps_f, warm, local_tol, first, R, nc,
+                               coff, order, cs, linvel, angvel, pos, inv_mass, invI, residual, L,
+                               used, rfin);
+    else
+        tile_sweeps<TBK, false>(sp, sweep0, nsweeps_n, nsweeps_f, warm, local_tol, first, R, nc,
+                                coff, order, cs, linvel, angvel, pos, inv_mass, invI, residual, L,
+                                used, rfin);
+    if (local_tol && tid == 0)
+        tile_stats[t] = make_int4(used[0], used[1], (int)__float_as_uint(rfin[0]),
+                                  (int)__float_as_uint(rfin[1]));
+    __syncthreads();
+    for (int h = tid; h < SOLVE_SLOTS; h += TBK) {
+        int b = L.c.key[h];
+        if (b >= 0) { linvel[b] = L.c.v[h]; angvel[b] = L.c.w[h]; }
     }
-    const int total = nsweeps_n + nsweeps_f;
-    for (int s = 0; s < total; s++) {
-        const bool fric = s >= nsweeps_n;
-        float rmax = 0.f, lmax = 0.f;
-        for (int c = 0; c < nc; c++) {
-            int c0 = coff[c], c1 = coff[c + 1];
-            if (c1 > R) c1 = R;
-            for (int r = c0 + threadIdx.x; r < c1; r += TB) {
-                int k = order[first + r];
-                int2 bd = cs.bodies[k];
-                int a = bd.x, b = bd.y;
-                float imA = inv_mass[a], imB = inv_mass[b];
-                int ha = imA != 0.f ? hash_find<SOLVE_SLOTS>(L.key, a) : -1;
-                int hb = imB != 0.f ? hash_find<SOLVE_SLOTS>(L.key, b) : -1;
-                float4 z = mk4(0, 0, 0, 0);
-                // dynamic bodies that did not fit the LDS cache live in global memory
-                float4 vA = ha >= 0 ? L.v[ha] : (imA != 0.f ? linvel[a] : z);
-                float4 wA = ha >= 0 ? L.w[ha] : (imA != 0.f ? angvel[a] : z);
-                float4 vB = hb >= 0 ? L.v[hb] : (imB != 0.f ? linvel[b] : z);
-                float4 wB = hb >= 0 ? L.w[hb] : (imB != 0.f ? angvel[b] : z);
-                float4 IA[3] = {invI[3 * a], invI[3 * a + 1], invI[3 * a + 2]};
-                float4 IB[3] = {invI[3 * b], invI[3 * b + 1], invI[3 * b + 2]};
-                float4 pA = pos[a], pB = pos[b];
-                if (!fric)
-                    solve_row_normal(k, cs, vA, wA, vB, wB, pA, pB, imA, IA, imB, IB, rmax, lmax);
-                else
-                    solve_row_friction(k, cs, vA, wA, vB, wB, pA, pB, imA, IA, imB, IB, sp.friction,
-                                       rmax, lmax);
-                if (ha >= 0) { L.v[ha] = vA; L.w[ha] = wA; }
-                else if (imA != 0.f) { linvel[a] = vA; angvel[a] = wA; }
-                if (hb >= 0) { L.v[hb] = vB; L.w[hb] = wB; }
-                else if (imB != 0.f) { linvel[b] = vB; angvel[b] = wB; }
-            }
-            __syncthreads();
-        }
-        // sweep residual of this tile
-        int sweep = sweep0 + s;
-        if (rmax > 0.f || lmax > 0.f) {
-            atomicMax((unsigned int *)&residual[2 * sweep], __float_as_uint(rmax));
-            atomicMax((unsigned int *)&residual[2 * sweep + 1], __float_as_uint(lmax));
-        }
+}
+
+// Scene tiles with a tolerance: sweeps used and final residuals over the
+// tiles that had rows (maximum, and sums for the mean).
+__global__ __launch_bounds__(256) void k_tile_summary(int n_tiles, const int4 *tile_stats,
+                                                      int *counters) {
+    __shared__ int smax[2], ssum[2], scnt;
+    __shared__ unsigned int sres[2];
+    if (threadIdx.x == 0) {
+        smax[0] = smax[1] = 0; ssum[0] = ssum[1] = 0; scnt = 0; sres[0] = sres[1] = 0u;
     }
     __syncthreads();
-    for (int h = threadIdx.x; h < SOLVE_SLOTS; h += TB) {
-        int b = L.key[h];
-        if (b >= 0) { linvel[b] = L.v[h]; angvel[b] = L.w[h]; }
+    int mx0 = 0, mx1 = 0, sm0 = 0, sm1 = 0, cnt = 0;
+    unsigned int r0 = 0u, r1 = 0u;
+    for (int t = threadIdx.x; t < n_tiles; t += blockDim.x) {
+        int4 st = tile_stats[t];
+        if (st.x < 0) continue;
+        mx0 = max(mx0, st.x); mx1 = max(mx1, st.y);
+        sm0 += st.x; sm1 += st.y; cnt++;
+        r0 = max(r0, (unsigned int)st.z); r1 = max(r1, (unsigned int)st.w);
+    }
+    atomicMax(&smax[0], mx0); atomicMax(&smax[1], mx1);
+    atomicAdd(&ssum[0], sm0); atomicAdd(&ssum[1], sm1); atomicAdd(&scnt, cnt);
+    atomicMax(&sres[0], r0); atomicMax(&sres[1], r1);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        counters[RB_CNT_ITERS_N] = smax[0];
+        counters[RB_CNT_ITERS_F] = smax[1];
+        counters[RB_CNT_SUM_ITERS_N] = ssum[0];
+        counters[RB_CNT_SUM_ITERS_F] = ssum[1];
+        counters[RB_CNT_TILES_SOLVED] = scnt;
+        counters[RB_CNT_RES_N] = (int)sres[0];
+        counters[RB_CNT_RES_F] = (int)sres[1];
     }
 }
 
@@ -765,31 +897,25 @@ hipError_t launch_solve(const DeviceWorld &d, const StepParams &p, hipStream_t s
     const int warm = p.warm_start != MI_RB_WARM_ZERO;
     hipLaunchKernelGGL(k_clear_done, dim3(1), dim3(1), 0, s, d.counters);
     launch_fill_u32(reinterpret_cast<uint32_t *>(d.residual), 0u, (size_t)2 * (S + 1), s);
+    // scene tiles: one workgroup per scene, 256 or 512 threads
 #define RB_SOLVE_ARGS                                                                         \
     d.tile_first, d.tile_cnt, d.solve_order, d.tile_color_off, d.tile_ncolors, d.cs, d.linvel, \
-        d.angvel, d.pos, d.inv_mass, d.inv_inertia_world, d.residual, d.counters
+        d.angvel, d.pos, d.inv_mass, d.inv_inertia_world, d.residual, d.counters, d.tile_stats
     if (p.tile_mode == 1) {
-        if (p.tol > 0.f) {
-            // scene tiles stop together: one launch per sweep so the residual can gate
-            if (warm)
-                hipLaunchKernelGGL(k_tile_solve, dim3(p.n_tiles), dim3(TB), 0, s, p, 0, 0, 0, 0, 1,
-                                   RB_SOLVE_ARGS);
-            for (int sw = 0; sw < S; sw++) {
-                int fr = sw >= p.normal_iters;
-                hipLaunchKernelGGL(k_tile_solve, dim3(p.n_tiles), dim3(TB), 0, s, p, 0, sw,
-                                   fr ? 0 : 1, fr ? 1 : 0, 0, RB_SOLVE_ARGS);
-                hipLaunchKernelGGL(k_sweep_done, dim3(1), dim3(1), 0, s, d.residual, sw, fr, p.tol,
-                                   d.counters);
-                if (sw == p.normal_iters - 1)
-                    hipLaunchKernelGGL(k_clear_done, dim3(1), dim3(1), 0, s, d.counters);
-            }
-        } else {
-            // independent scenes: every sweep of a scene in one workgroup, one launch
-            hipLaunchKernelGGL(k_tile_solve, dim3(p.n_tiles), dim3(TB), 0, s, p, 0, 0,
+        // independent scenes: every sweep of a scene in one workgroup, one
+        // launch; with a tolerance each scene stops on its own residual
+        if (p.solve_block == 256)
+            hipLaunchKernelGGL(k_tile_solve<256>, dim3(p.n_tiles), dim3(256), 0, s, p, 0, 0,
                                p.normal_iters, p.friction_iters, warm, RB_SOLVE_ARGS);
+        else
+            hipLaunchKernelGGL(k_tile_solve<TB_SCENE>, dim3(p.n_tiles), dim3(TB_SCENE), 0, s, p,
+                               0, 0, p.normal_iters, p.friction_iters, warm, RB_SOLVE_ARGS);
+        if (p.tol > 0.f)
+            hipLaunchKernelGGL(k_tile_summary, dim3(1), dim3(256), 0, s, p.n_tiles, d.tile_stats,
+                               d.counters);
+        else
             hipLaunchKernelGGL(k_sweep_summary, dim3(1), dim3(1), 0, s, d.residual, p.normal_iters,
                                p.friction_iters, d.counters);
-        }
     } else {
         for (int sw = warm ? -1 : 0; sw < S; sw++) {
             int fr = sw >= p.normal_iters;
@@ -797,10 +923,10 @@ hipError_t launch_solve(const DeviceWorld &d, const StepParams &p, hipStream_t s
                 int nx = (p.tiles_x - (ph & 1) + 1) / 2, nz = (p.tiles_z - (ph >> 1) + 1) / 2;
                 if (nx <= 0 || nz <= 0) continue;
                 if (sw < 0)
-                    hipLaunchKernelGGL(k_tile_solve, dim3(nx * nz), dim3(TB), 0, s, p, ph, 0, 0, 0,
+                    hipLaunchKernelGGL(k_tile_solve<TB>, dim3(nx * nz), dim3(TB), 0, s, p, ph, 0, 0, 0,
                                        1, RB_SOLVE_ARGS);
                 else
-                    hipLaunchKernelGGL(k_tile_solve, dim3(nx * nz), dim3(TB), 0, s, p, ph, sw,
+                    hipLaunchKernelGGL(k_tile_solve<TB>, dim3(nx * nz), dim3(TB), 0, s, p, ph, sw,
                                        fr ? 0 : 1, fr ? 1 : 0, 0, RB_SOLVE_ARGS);
             }
             if (sw >= 0 && p.tol > 0.f) {

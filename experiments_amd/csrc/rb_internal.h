@@ -110,6 +110,12 @@ struct DeviceWorld {
     int *row_color;              // local colour per constraint
     int *tile_color_off;         // per tile RB_MAX_LOCAL_COLORS+1 offsets (relative)
     int *tile_ncolors;
+    int4 *tile_stats;            // per tile: normal / friction sweeps used, final residuals (bits)
+    // narrowphase pass 1 -> pass 2
+    uint32_t *narrow_cand;       // per pair: 1 if not separated by a face axis
+    uint32_t *cand_off;          // its exclusive scan
+    int4 *face_best;             // per pair: face minimum (depth bits, axis), edge-axis count
+    uint2 *cand_kv;              // (cost key, pair) of the candidates, sorted
     float *residual;             // per sweep: max scaled dlambda, max lambda (as uint bits)
     // warm start
     int2 *prev_keys;             // previous step's contact body pairs (a, b), sorted
@@ -134,7 +140,11 @@ enum {
     RB_CNT_RES_F = 7,
     RB_CNT_SORT_TILE = 8,     // uint32 total of the pair-count scan
     RB_CNT_CONTACT_TOTAL = 9, // uint32 total of the contact scan
+    RB_CNT_NARROW_CAND = 10,  // uint32 number of narrowphase pass-2 candidates
     RB_CNT_DONE = 12,         // solver converged flag for the remaining sweeps
+    RB_CNT_SUM_ITERS_N = 13,  // scene tiles with a tolerance: sweeps summed over tiles
+    RB_CNT_SUM_ITERS_F = 14,
+    RB_CNT_TILES_SOLVED = 15, // ... and the number of tiles that had rows
     RB_CNT_COUNT = 16
 };
 
@@ -168,6 +178,9 @@ struct StepParams {
     int normal_iters, friction_iters;
     float tol;
     int warm_start;
+    // launch shapes (mi_rb_tuning)
+    int solve_block;     // threads per scene-tile solve workgroup: 256 or 512
+    int narrow_waves;    // narrowphase waves per SIMD (grid = CUs * 4 * narrow_waves)
 };
 
 // ---- launch interfaces (implemented in the .hip files) ----
@@ -194,7 +207,8 @@ hipError_t launch_setup(const DeviceWorld &d, const StepParams &p, hipStream_t s
 hipError_t launch_cell_start(const DeviceWorld &d, const StepParams &p, hipStream_t s);
 hipError_t launch_find_pairs(const DeviceWorld &d, const StepParams &p, hipStream_t s);
 hipError_t launch_emit_pairs(const DeviceWorld &d, const StepParams &p, hipStream_t s);
-hipError_t launch_narrowphase(const DeviceWorld &d, const StepParams &p, hipStream_t s);
+hipError_t launch_narrowphase(const DeviceWorld &d, const StepParams &p, hipStream_t s,
+                              const SortWork &sw, uint32_t *scan_status);
 hipError_t launch_constraints(const DeviceWorld &d, const StepParams &p, hipStream_t s);
 hipError_t launch_contact_total(const DeviceWorld &d, const StepParams &p, hipStream_t s);
 hipError_t launch_tile_bounds(const DeviceWorld &d, const StepParams &p, hipStream_t s);

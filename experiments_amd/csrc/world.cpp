@@ -97,6 +97,7 @@ struct World {
     uint32_t *scan_status = nullptr;
     long long steps = 0;
     float last_ms = 0.f;
+    mi_rb_tuning tuning{0, 0};  // launch shapes; 0 = default
     std::vector<void *> allocs;
 
     template <typename T>
@@ -224,6 +225,8 @@ mi_rb_status derive_and_upload(World &w) {
     p.friction_iters = c.solver_iterations;
     p.tol = c.residual_tolerance;
     p.warm_start = c.warm_start;
+    p.solve_block = w.tuning.solve_block ? w.tuning.solve_block : 512;
+    p.narrow_waves = w.tuning.narrow_waves ? w.tuning.narrow_waves : 6;
     // solver tiles
     if (n_scenes > 1) {
         p.tile_mode = 1;
@@ -298,6 +301,11 @@ mi_rb_status derive_and_upload(World &w) {
     RB_CHECK(w.alloc(&d.row_color, maxC));
     RB_CHECK(w.alloc(&d.tile_color_off, (size_t)p.n_tiles * (RB_MAX_LOCAL_COLORS + 1)));
     RB_CHECK(w.alloc(&d.tile_ncolors, p.n_tiles));
+    RB_CHECK(w.alloc(&d.tile_stats, p.n_tiles));
+    RB_CHECK(w.alloc(&d.narrow_cand, maxP));
+    RB_CHECK(w.alloc(&d.cand_off, maxP));
+    RB_CHECK(w.alloc(&d.face_best, maxP));
+    RB_CHECK(w.alloc(&d.cand_kv, maxP));
     RB_CHECK(w.alloc(&d.residual, 2 * 1024));
     RB_CHECK(w.alloc(&d.counters, RB_CNT_COUNT));
     RB_CHECK(w.alloc(&d.prev_keys, maxC));
@@ -327,7 +335,7 @@ mi_rb_status derive_and_upload(World &w) {
     p.large_chunks = std::max(1, (max_scene + 4095) / 4096);
     RB_CHECK(w.alloc(&d.large_chunk_cnt, (size_t)std::max(p.n_large, 1) * p.large_chunks));
     // sort / scan work
-    int sort_cap = std::max(n, maxC);
+    int sort_cap = std::max(std::max(n, maxC), maxP);
     w.sw.max_n = sort_cap;
     w.sw.max_tiles = (sort_cap + 4095) / 4096;
     RB_CHECK(w.alloc(&w.sw.tmp, sort_cap));
@@ -398,7 +406,7 @@ mi_rb_status enqueue_step(World &w, int last_stage) {
     RB_CHECK(launch_emit_pairs(d, p, s));
     if (ev) RB_CHECK(hipEventRecord(w.ev[3], s));
     if (last_stage < MI_RB_STAGE_NARROWPHASE) return MI_RB_OK;
-    RB_CHECK(launch_narrowphase(d, p, s));
+    RB_CHECK(launch_narrowphase(d, p, s, w.sw, w.scan_status));
     if (ev) RB_CHECK(hipEventRecord(w.ev[4], s));
     if (last_stage < MI_RB_STAGE_CONSTRAINTS) return MI_RB_OK;
     RB_CHECK(launch_exclusive_scan(d.has_contact, d.contact_off, p.max_pairs,
@@ -453,7 +461,7 @@ extern "C" mi_rb_status mi_rb_create(const mi_rb_config *cfg, void **out_world) 
     if (!cfg || !out_world) return MI_RB_INVALID_ARG;
     if (cfg->max_bodies <= 0 || cfg->max_pairs <= 0 || cfg->max_contacts <= 0 ||
         cfg->max_pairs_per_body <= 0 || cfg->max_pairs_per_body > 64 || cfg->dt <= 0.f ||
-        cfg->solver_iterations < 0 || cfg->solver_iterations > 256)
+        cfg->solver_iterations < 0 || cfg->solver_iterations > 500)
         return MI_RB_INVALID_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= cfg->device) return MI_RB_DEVICE;
@@ -673,6 +681,86 @@ extern "C" mi_rb_status mi_rb_step_partial(void *world, int stage) {
     return MI_RB_OK;
 }
 
+extern "C" mi_rb_status mi_rb_set_tuning(void *world, const mi_rb_tuning *t) {
+    World *w = W(world);
+    if (!w || !t) return MI_RB_INVALID_ARG;
+    if (!(t->solve_block == 0 || t->solve_block == 256 || t->solve_block == 512)) return MI_RB_INVALID_ARG;
+    if (t->narrow_waves < 0 || t->narrow_waves > 8) return MI_RB_INVALID_ARG;
+    w->tuning = *t;
+    w->sp.solve_block = t->solve_block ? t->solve_block : 512;
+    w->sp.narrow_waves = t->narrow_waves ? t->narrow_waves : 6;
+    if (w->graph) { hipGraphExecDestroy(w->graph); w->graph = nullptr; }
+    return MI_RB_OK;
+}
+
+extern "C" mi_rb_status mi_rb_get_tuning(void *world, mi_rb_tuning *t) {
+    World *w = W(world);
+    if (!w || !t) return MI_RB_INVALID_ARG;
+    t->solve_block = w->tuning.solve_block ? w->tuning.solve_block : 512;
+    t->narrow_waves = w->tuning.narrow_waves ? w->tuning.narrow_waves : 6;
+    return MI_RB_OK;
+}
+
+extern "C" mi_rb_status mi_rb_autotune(void *world, int steps, mi_rb_tuning *best, float ms_out[8]) {
+    World *w = W(world);
+    if (!w || steps <= 0 || !best) return MI_RB_INVALID_ARG;
+    if (!w->uploaded) return MI_RB_STATE;
+    hipSetDevice(w->device);
+    DeviceWorld &d = w->d;
+    const size_t n = w->pos.size(), maxC = (size_t)w->cfg.max_contacts;
+    // snapshot of everything a step reads that a step changes
+    std::vector<float4> st(4 * n), pk(maxC), pi(maxC);
+    int pc = 0;
+    float4 *dev[4] = {d.pos, d.quat, d.linvel, d.angvel};
+    RB_CHECK(hipStreamSynchronize(w->stream));
+    for (int f = 0; f < 4; f++)
+        RB_CHECK(hipMemcpy(st.data() + f * n, dev[f], 16 * n, hipMemcpyDeviceToHost));
+    RB_CHECK(hipMemcpy(pk.data(), d.prev_keys, 8 * maxC, hipMemcpyDeviceToHost));
+    RB_CHECK(hipMemcpy(pi.data(), d.prev_imp, 16 * maxC, hipMemcpyDeviceToHost));
+    RB_CHECK(hipMemcpy(&pc, d.prev_count, sizeof(int), hipMemcpyDeviceToHost));
+    auto restore = [&]() -> mi_rb_status {
+        for (int f = 0; f < 4; f++)
+            RB_CHECK(hipMemcpy(dev[f], st.data() + f * n, 16 * n, hipMemcpyHostToDevice));
+        RB_CHECK(hipMemcpy(d.prev_keys, pk.data(), 8 * maxC, hipMemcpyHostToDevice));
+        RB_CHECK(hipMemcpy(d.prev_imp, pi.data(), 16 * maxC, hipMemcpyHostToDevice));
+        RB_CHECK(hipMemcpy(d.prev_count, &pc, sizeof(int), hipMemcpyHostToDevice));
+        return MI_RB_OK;
+    };
+    const mi_rb_tuning keep = w->tuning;
+    const bool keep_graph = w->use_graph;
+    const long long keep_steps = w->steps;
+    const int blocks[2] = {256, 512}, waves[4] = {2, 4, 6, 8};
+    float best_ms = 0.f;
+    mi_rb_tuning bt = keep;
+    int idx = 0;
+    for (int bi = 0; bi < 2; bi++)
+        for (int wi = 0; wi < 4; wi++, idx++) {
+            mi_rb_tuning t{blocks[bi], waves[wi]};
+            mi_rb_status stt = mi_rb_set_tuning(world, &t);
+            if (stt == MI_RB_OK) stt = restore();
+            w->use_graph = true;
+            if (stt == MI_RB_OK) stt = mi_rb_step(world, 1);  // capture + warm
+            if (stt == MI_RB_OK) stt = restore();
+            if (stt == MI_RB_OK) stt = mi_rb_step(world, steps);
+            if (stt != MI_RB_OK) {
+                mi_rb_set_tuning(world, &keep);
+                w->use_graph = keep_graph;
+                return stt;
+            }
+            float ms = w->last_ms / (float)steps;
+            if (ms_out) ms_out[idx] = ms;
+            if (idx == 0 || ms < best_ms) { best_ms = ms; bt = t; }
+        }
+    mi_rb_status stt = mi_rb_set_tuning(world, &bt);
+    if (stt != MI_RB_OK) return stt;
+    w->use_graph = keep_graph;
+    w->steps = keep_steps;
+    stt = restore();
+    if (stt != MI_RB_OK) return stt;
+    *best = bt;
+    return MI_RB_OK;
+}
+
 extern "C" mi_rb_status mi_rb_last_step_ms(void *world, float *ms) {
     World *w = W(world);
     if (!w || !ms) return MI_RB_INVALID_ARG;
@@ -719,6 +807,14 @@ extern "C" mi_rb_status mi_rb_get_stats(void *world, mi_rb_stats *out) {
     out->friction_iterations = c[RB_CNT_ITERS_F];
     std::memcpy(&out->normal_residual, &c[RB_CNT_RES_N], 4);
     std::memcpy(&out->friction_residual, &c[RB_CNT_RES_F], 4);
+    if (w->sp.tile_mode == 1 && w->sp.tol > 0.f) {
+        int tiles = c[RB_CNT_TILES_SOLVED] > 0 ? c[RB_CNT_TILES_SOLVED] : 1;
+        out->mean_normal_iterations = (float)c[RB_CNT_SUM_ITERS_N] / (float)tiles;
+        out->mean_friction_iterations = (float)c[RB_CNT_SUM_ITERS_F] / (float)tiles;
+    } else {
+        out->mean_normal_iterations = (float)out->normal_iterations;
+        out->mean_friction_iterations = (float)out->friction_iterations;
+    }
     if (w->timing && !w->use_graph && w->events_full) {
         for (int k = 0; k < 8; k++) {
             float ms = 0.f;
@@ -771,7 +867,7 @@ extern "C" mi_rb_status mi_rb_read_buffer(void *world, int which, void *host, si
                                           int *count) {
     World *w = W(world);
     if (!w || !count) return MI_RB_INVALID_ARG;
-    if (which >= MI_RB_BUF_HULLS) {
+    if (which >= MI_RB_BUF_HULLS && which <= MI_RB_BUF_SHAPE) {  // host-side registration arrays
         const void *src = nullptr;
         size_t elem = 0;
         int cnt = 0;

@@ -108,14 +108,17 @@ def log_plain_solver(world, steps, every=1):
 
 
 def run_iterations(world, steps):
-    """Step and record per-step solver sweeps and final residuals."""
+    """Step and record per-step solver sweeps (mean over scenes and maximum)
+    and final residuals."""
     rows = []
     for _ in range(steps):
         world.step(1)
         st = world.stats()
-        rows.append((st["normal_iterations"], st["friction_iterations"], st["normal_residual"],
+        rows.append((st["mean_normal_iterations"], st["mean_friction_iterations"],
+                     st["normal_iterations"], st["friction_iterations"], st["normal_residual"],
                      st["friction_residual"]))
-    a = np.array(rows, np.float64).reshape(-1, 4)
+    a = np.array(rows, np.float64).reshape(-1, 6)
     return dict(mean_iters_normal=float(a[:, 0].mean()), mean_iters_friction=float(a[:, 1].mean()),
-                max_residual_normal=float(a[:, 2].max()), max_residual_friction=float(a[:, 3].max()),
+                max_iters_normal=int(a[:, 2].max()), max_iters_friction=int(a[:, 3].max()),
+                max_residual_normal=float(a[:, 4].max()), max_residual_friction=float(a[:, 5].max()),
                 per_step=a)

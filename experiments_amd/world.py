@@ -156,6 +156,25 @@ class World:
     def contacts(self, dtype):
         return self.read(L.BUF_CONTACTS, dtype)
 
+    def set_tuning(self, solve_block=0, narrow_waves=0):
+        t = L.Tuning(solve_block, narrow_waves)
+        L.call("mi_rb_set_tuning", self.h, ctypes.byref(t))
+
+    def get_tuning(self):
+        t = L.Tuning()
+        L.call("mi_rb_get_tuning", self.h, ctypes.byref(t))
+        return {"solve_block": t.solve_block, "narrow_waves": t.narrow_waves}
+
+    def autotune(self, steps=5):
+        """Time every launch shape from the current state (restored after
+        each trial), apply the fastest; returns (best, {shape: ms/step})."""
+        t = L.Tuning()
+        ms = np.zeros(8, np.float32)
+        L.call("mi_rb_autotune", self.h, int(steps), ctypes.byref(t), _ptr(ms))
+        shapes = [(b, wv) for b in (256, 512) for wv in (2, 4, 6, 8)]
+        return ({"solve_block": t.solve_block, "narrow_waves": t.narrow_waves},
+                {"%d/%d" % sh: float(m) for sh, m in zip(shapes, ms)})
+
     def set_warmstart_weights(self, w_bf16, bias):
         w = np.ascontiguousarray(w_bf16, np.uint16).reshape(-1)
         b = np.ascontiguousarray(bias, np.float32).reshape(-1)

@@ -81,6 +81,10 @@ typedef struct mi_rb_stats {
     float friction_residual;
     float stage_ms[8];         /* setup, sort, pairs, narrowphase, constraints, colouring, solve, integrate */
     long long steps;
+    /* scene tiles with a residual tolerance stop one by one: the sweeps above
+     * are the maximum over scenes, these the mean over scenes with contacts */
+    float mean_normal_iterations;
+    float mean_friction_iterations;
 } mi_rb_stats;
 
 /* parameters derived at upload (grid, overflow list, solver tiles) */
@@ -147,9 +151,9 @@ enum {
  * drift 0.005, threshold 0.02, damping 0.99, 0.25*pi). */
 void mi_rb_config_reference(mi_rb_config *cfg);
 
-/* CLPhysicsDemo::CLPhysicsDemo + init (CLPhysicsDemo.cpp:603-660) */
+/* CLPhysicsDemo::CLPhysicsDemo + init (CLPhysicsDemo.cpp:174-182,603-655) */
 mi_rb_status mi_rb_create(const mi_rb_config *cfg, void **out_world);
-/* CLPhysicsDemo::cleanup (CLPhysicsDemo.cpp:662-730) */
+/* CLPhysicsDemo::cleanup (CLPhysicsDemo.cpp:668-690) */
 mi_rb_status mi_rb_destroy(void *world);
 
 /* CLPhysicsDemo::registerConvexPolyhedron(const float*, int strideBytes, int n,
@@ -170,7 +174,7 @@ mi_rb_status mi_rb_register_body(void *world, float mass, const float pos[3],
 /* CLPhysicsDemo::writeBodiesToGpu (CLPhysicsDemo.cpp:205-215): uploads shapes
  * and bodies, allocates the step's device buffers. */
 mi_rb_status mi_rb_upload(void *world);
-/* CLPhysicsDemo::setObjectTransform / setObjectLinearVelocity (CLPhysicsDemo.cpp:218-260) */
+/* CLPhysicsDemo::setObjectTransform / setObjectLinearVelocity (CLPhysicsDemo.cpp:692-701) */
 mi_rb_status mi_rb_set_transform(void *world, int body, const float pos[3], const float quat[4]);
 mi_rb_status mi_rb_set_linvel(void *world, int body, const float v[3]);
 /* bulk state transfer (float4 arrays; NULL skips a field) */
@@ -187,7 +191,7 @@ mi_rb_status mi_rb_step_partial(void *world, int stage);
 /* capture the step in a hipGraph and replay it (1) or launch eagerly (0) */
 mi_rb_status mi_rb_use_graph(void *world, int enable);
 
-/* CLPhysicsDemo::readbackBodiesToCpu + getObjectTransformFromCpu (CLPhysicsDemo.cpp:190-203,262-280) */
+/* CLPhysicsDemo::readbackBodiesToCpu + getObjectTransformFromCpu (CLPhysicsDemo.cpp:190-203) */
 mi_rb_status mi_rb_readback(void *world);
 mi_rb_status mi_rb_get_transform(void *world, int body, float pos[4], float quat[4]);
 
@@ -214,6 +218,22 @@ mi_rb_status mi_rb_copy_training_batch(void *world, float *x_dev, float *t_dev, 
                                        int *count);
 /* HIP-event time of the last `steps` steps, milliseconds */
 mi_rb_status mi_rb_last_step_ms(void *world, float *ms);
+
+/* Launch shapes of the step (no reference counterpart: the reference fixes
+ * its work-group sizes in the host code).  0 = default.  Changing them drops
+ * the captured graph; results do not depend on them. */
+typedef struct mi_rb_tuning {
+    int solve_block;   /* threads per scene-tile solver workgroup: 256 or 512 */
+    int narrow_waves;  /* narrowphase waves per SIMD: 1..8 */
+} mi_rb_tuning;
+mi_rb_status mi_rb_set_tuning(void *world, const mi_rb_tuning *t);
+mi_rb_status mi_rb_get_tuning(void *world, mi_rb_tuning *t);
+/* Autotuner: times every candidate launch shape over `steps` graph steps from
+ * the world's current state (restored after each trial, so the world ends
+ * where it started), applies the fastest and returns it.  ms_out (may be
+ * NULL) receives the per-candidate step times in the order solve_block in
+ * {256, 512} x narrow_waves in {2, 4, 6, 8}. */
+mi_rb_status mi_rb_autotune(void *world, int steps, mi_rb_tuning *best, float ms_out[8]);
 
 /* ---- host utility (no device work): convex hull + polyhedral features ---- */
 /* Builds the hull record used by registration, for inspection and tests.

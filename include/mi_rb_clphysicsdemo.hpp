@@ -1,0 +1,133 @@
+/* mi_rb_clphysicsdemo.hpp -- header-only C++ class with the method names,
+ * argument meaning and call order of the reference's CLPhysicsDemo
+ * (opencl/gpu_rigidbody_pipeline2/CLPhysicsDemo.h:31-78), forwarding to the
+ * mi_rb C ABI.  A demo that includes this header instead of CLPhysicsDemo.h
+ * and links libmi_rb.so runs its step on the gfx950 kernels.
+ *
+ * Differences a caller can see:
+ *   - getBodiesGpu() returns a non-owning mi_rb_device_view pointer instead of
+ *     a cl_mem (the bodies are structure-of-arrays in device memory);
+ *   - non-hull shapes (sphere, plane, concave mesh, compound) return -1, as
+ *     the step handles convex hulls only;
+ *   - errors are reported by last_status() (mi_rb_status of the last call)
+ *     instead of asserts and printf.
+ */
+#ifndef MI_RB_CLPHYSICSDEMO_HPP
+#define MI_RB_CLPHYSICSDEMO_HPP
+
+#include "mi_rb.h"
+
+struct CLPhysicsDemo {
+    int m_numPhysicsInstances = 0;
+    int m_numDynamicPhysicsInstances = 0;
+    int m_maxShapeBufferCapacityInBytes = 0;
+    int m_maxNumPhysicsInstances = 0;
+
+    /* CLPhysicsDemo(int maxShapeBufferCapacity, int maxNumObjects) */
+    CLPhysicsDemo(int maxShapeBufferCapacity, int maxNumObjects)
+        : m_maxShapeBufferCapacityInBytes(maxShapeBufferCapacity),
+          m_maxNumPhysicsInstances(maxNumObjects) {
+        mi_rb_config_reference(&m_cfg);
+        m_cfg.max_bodies = maxNumObjects;
+    }
+    virtual ~CLPhysicsDemo() { cleanup(); }
+
+    /* init(preferredDevice, preferredPlatform, useInterop): the device index
+     * selects the GPU; platform and interop have no meaning here */
+    void init(int preferredDevice, int preferredPlatform, bool useInterop) {
+        (void)preferredPlatform;
+        (void)useInterop;
+        if (preferredDevice >= 0) m_cfg.device = preferredDevice;
+        m_status = mi_rb_create(&m_cfg, &m_world);
+    }
+
+    /* the bodies in device memory (replaces the cl_mem of the body array) */
+    const mi_rb_device_view *getBodiesGpu() {
+        m_status = mi_rb_device_ptrs(m_world, &m_view);
+        return m_status == MI_RB_OK ? &m_view : nullptr;
+    }
+
+    int registerConvexPolyhedron(const float *vertices, int strideInBytes, int numVertices,
+                                 const float *scaling, bool noHeightField) {
+        (void)noHeightField;
+        int shape = -1;
+        m_status = mi_rb_register_convex_hull(m_world, vertices, strideInBytes, numVertices,
+                                              scaling, &shape);
+        return m_status == MI_RB_OK ? shape : -1;
+    }
+    int registerSphereShape(float radius) {
+        (void)radius;
+        m_status = mi_rb_register_unsupported_shape(m_world, 1);
+        return -1;
+    }
+    int registerPlaneShape(const float planeNormal[3], float planeConstant) {
+        (void)planeNormal;
+        (void)planeConstant;
+        m_status = mi_rb_register_unsupported_shape(m_world, 2);
+        return -1;
+    }
+    template <class Vertices, class Indices>
+    int registerConcaveMesh(Vertices *vertices, Indices *indices, const float *scaling) {
+        (void)vertices;
+        (void)indices;
+        (void)scaling;
+        m_status = mi_rb_register_unsupported_shape(m_world, 3);
+        return -1;
+    }
+    template <class Children>
+    int registerCompoundShape(Children *childShapes) {
+        (void)childShapes;
+        m_status = mi_rb_register_unsupported_shape(m_world, 4);
+        return -1;
+    }
+
+    /* position: float[3] (or [4]), orientation: quaternion xyzw */
+    int registerPhysicsInstance(float mass, const float *position, const float *orientation,
+                                int collisionShapeIndex, int userPointer) {
+        (void)userPointer;
+        int body = -1;
+        m_status = mi_rb_register_body(m_world, mass, position, orientation, collisionShapeIndex,
+                                       0, &body);
+        if (m_status != MI_RB_OK) return -1;
+        m_numPhysicsInstances++;
+        if (mass != 0.f) m_numDynamicPhysicsInstances++;
+        return body;
+    }
+
+    void setObjectTransform(const float *position, const float *orientation, int objectIndex) {
+        m_status = mi_rb_set_transform(m_world, objectIndex, position, orientation);
+    }
+    void setObjectLinearVelocity(const float *velocity, int objectIndex) {
+        m_status = mi_rb_set_linvel(m_world, objectIndex, velocity);
+    }
+
+    /* velocities set above are written through at once */
+    void writeVelocitiesToGpu() { m_status = MI_RB_OK; }
+    /* shapes, bodies and every derived table go to the device */
+    void writeBodiesToGpu() { m_status = mi_rb_upload(m_world); }
+
+    void readbackBodiesToCpu() { m_status = mi_rb_readback(m_world); }
+    void getObjectTransformFromCpu(float *position, float *orientation, int objectIndex) {
+        m_status = mi_rb_get_transform(m_world, objectIndex, position, orientation);
+    }
+
+    void cleanup() {
+        if (m_world) mi_rb_destroy(m_world);
+        m_world = nullptr;
+    }
+
+    void stepSimulation() { m_status = mi_rb_step(m_world, 1); }
+
+    /* additions: the status of the last call, and the world handle for the
+     * rest of the C ABI (statistics, graph replay, warm start) */
+    mi_rb_status last_status() const { return m_status; }
+    void *world() const { return m_world; }
+
+  private:
+    mi_rb_config m_cfg{};
+    void *m_world = nullptr;
+    mi_rb_device_view m_view{};
+    mi_rb_status m_status = MI_RB_OK;
+};
+
+#endif

@@ -265,7 +265,9 @@ def solve(cons, order, linvel, angvel, pos, inv_mass, inv_inertia_world, normal_
     iters = np.zeros(2, np.int32)
     res = np.zeros(2, np.float32)
     order = None if order is None else np.ascontiguousarray(order, np.int32)
-    lib().orc_solve(ctypes.c_int(len(cons)), _p(cons), _p(order), _p(lv), _p(av), _p(f4(pos)),
+    # with an order, only the rows it lists are visited (a tile's segment)
+    n = len(cons) if order is None else len(order)
+    lib().orc_solve(ctypes.c_int(n), _p(cons), _p(order), _p(lv), _p(av), _p(f4(pos)),
                     _p(np.ascontiguousarray(inv_mass, np.float32)),
                     _p(np.ascontiguousarray(inv_inertia_world, np.float32)),
                     ctypes.c_int(normal_iters), ctypes.c_int(friction_iters), _f(friction),

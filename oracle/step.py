@@ -70,9 +70,19 @@ def gpu_order(w):
     return np.concatenate(seq) if seq else np.zeros(0, np.int32)
 
 
-def oracle_step(s, order, contacts=None, prev=None, warm=0, prediction=None):
+def gpu_tile_orders(w):
+    """The GPU's constraint order split by solver tile (scene tiles solve
+    independently), in tile order."""
+    order = w.read(L.BUF_SOLVE_ORDER, np.int32)
+    tiles = w.read(L.BUF_PHASES, np.int32, (4,))
+    return [order[first:end] for first, end, _, _ in tiles if end > first]
+
+
+def oracle_step(s, order, contacts=None, prev=None, warm=0, prediction=None, segments=None):
     """Full step on the oracle from the snapshot state, visiting constraints in
-    `order`.  prev: (keys int2[m], impulses float4[m]) of the previous solve;
+    `order`.  With `segments` (a list of orders, one per independent tile) each
+    segment is solved on its own, stopping at the residual tolerance of its
+    own rows.  prev: (keys int2[m], impulses float4[m]) of the previous solve;
     warm: 0 zero, 1 previous impulses, 2 impulses from `prediction` (n, 3).
     Returns pos, quat, linvel, angvel, the solved constraints and the
     warm-start features."""
@@ -89,9 +99,18 @@ def oracle_step(s, order, contacts=None, prev=None, warm=0, prediction=None):
                                      apply_previous=(warm == 1))
     if warm == 2:
         cons = O.warm_from_prediction(cons, prediction, c.friction)
-    cons, lv2, av2, it, res = O.solve(cons, order, lv, av, st["pos"], sh["inv_mass"], invI,
-                                      c.solver_iterations, c.solver_iterations, c.friction,
-                                      c.residual_tolerance)
+    if segments is None:
+        cons, lv2, av2, it, res = O.solve(cons, order, lv, av, st["pos"], sh["inv_mass"], invI,
+                                          c.solver_iterations, c.solver_iterations, c.friction,
+                                          c.residual_tolerance)
+    else:
+        lv2, av2, it, res = lv, av, [], []
+        for seg in segments:
+            cons, lv2, av2, i_s, r_s = O.solve(cons, seg, lv2, av2, st["pos"], sh["inv_mass"],
+                                               invI, c.solver_iterations, c.solver_iterations,
+                                               c.friction, c.residual_tolerance)
+            it.append(i_s)
+            res.append(r_s)
     pos, quat, av3 = O.integrate(st["pos"], st["quat"], lv2, av2, sh["inv_mass"], c.dt,
                                  c.angular_damping, c.max_angular_motion)
     return dict(pos=pos, quat=quat, linvel=lv2, angvel=av3, cons=cons, iters=it, res=res,

@@ -28,6 +28,14 @@ def sorter():
     L.lib().mi_rb_radix_sort_destroy(h)
 
 
+def test_radix_sort_c_api_known_answer(sorter):
+    # opencl/c_api/main.c:44-98: four keys in reverse order come back sorted
+    kv = _dev(np.array([[3, 0], [2, 1], [1, 2], [0, 3]], np.int32))
+    L.call("mi_rb_radix_sort_kv", sorter, _p(kv), 4, 32, None)
+    torch.cuda.synchronize()
+    assert kv.cpu().numpy().tolist() == [[0, 3], [1, 2], [2, 1], [3, 0]]
+
+
 @pytest.mark.parametrize("n", [1, 2, 255, 4096, 4097, 65537, 1 << 20])
 @pytest.mark.parametrize("bits", [8, 19, 32])
 def test_radix_sort_kv(sorter, n, bits):

@@ -9,7 +9,7 @@ import oracle as O
 from experiments_amd import _lib as L
 from experiments_amd import scenes
 from experiments_amd.warmstart import Mlp
-from oracle.step import Snapshot, gpu_order, oracle_step
+from oracle.step import Snapshot, gpu_order, gpu_tile_orders, oracle_step
 
 torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
@@ -196,4 +196,34 @@ def test_learned_without_weights_is_refused():
     w = pile_world(L.WARM_LEARNED)
     with pytest.raises(L.MiRbError):
         w.step(1)
+    w.close()
+
+
+@pytest.mark.parametrize("warm", [L.WARM_ZERO, L.WARM_PREVIOUS])
+def test_scene_tolerance_matches_oracle(warm):
+    """Scene tiles with a residual tolerance: every scene stops on its own
+    residual inside one launch; the oracle solves each scene's rows alone
+    with the same tolerance."""
+    sc = scenes.batched_piles(n_scenes=24, lo=8, hi=64, pool8=32, iterations=60)
+    w = scenes.build_world(sc, max_pairs_per_body=64, warm_start=warm, residual_tolerance=1e-3)
+    w.step(40)
+    s = Snapshot(w)
+    keys = w.read(L.BUF_PREV_KEYS, np.int32, (2,))
+    imp = w.read(L.BUF_PREV_IMPULSES, np.float32, (4,))
+    w.step_partial(L.STAGE_COLOR)
+    segs = gpu_tile_orders(w)
+    contacts = w.contacts(O.CONTACT_DTYPE)
+    w.set_state(**{k: s.state[k] for k in ("pos", "quat", "linvel", "angvel")})
+    w.step(1)
+    g = w.get_state()
+    st = w.stats()
+    ref = oracle_step(s, None, contacts, prev=(keys, imp), warm=warm, segments=segs)
+    for k in ("linvel", "angvel", "pos", "quat"):
+        c = 3 if k != "quat" else 4
+        assert np.allclose(g[k][:, :c], ref[k][:, :c], rtol=1e-4, atol=1e-4), k
+    its = np.array(ref["iters"])
+    assert st["normal_iterations"] == its[:, 0].max()
+    assert st["friction_iterations"] == its[:, 1].max()
+    assert abs(st["mean_normal_iterations"] - its[:, 0].mean()) < 1e-3
+    assert its[:, 0].min() < its[:, 0].max()  # scenes really stop at different sweeps
     w.close()
