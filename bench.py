@@ -113,6 +113,63 @@ def per_scene_baseline(scene, steps, warmup, max_scenes=None):
             "host_s": round(host_s, 4), "gpu_ms_sum": round(total_ms, 3)}
 
 
+def broadphase_1m(steps=100, warmup=5):
+    """Broadphase stress (1,048,576 spheres and boxes, radii log-uniform in
+    [0.25, 2], 400^3 domain, level-0 cell = 2 x the median radius, 4 grid
+    levels): the hierarchical grid API on AABBs that move with their velocity
+    every step.  Per-phase times from the grid's HIP events; hash + sort +
+    scan bytes against the HBM peak."""
+    import ctypes
+    import torch
+    from experiments_amd import _lib as L
+    from experiments_amd import scenes
+    mn, mx, v, r = scenes.broadphase_aabbs()
+    n = len(mn)
+    cell = float(2 * np.median(r))
+    max_pairs = 1 << 23
+    desc = L.GridDesc(cell, (ctypes.c_int * 3)(256, 256, 256), max_pairs, 1, 4)
+    g = ctypes.c_void_p()
+    L.call("mi_rb_grid_create", ctypes.byref(desc), n, ctypes.byref(g))
+    dmn = torch.from_numpy(mn).cuda()
+    dmx = torch.from_numpy(mx).cuda()
+    dv = torch.zeros((n, 4), dtype=torch.float32, device="cuda")
+    dv[:, :3] = torch.from_numpy(v).cuda() * (1.0 / 60.0)
+    pairs = torch.zeros((max_pairs, 2), dtype=torch.int32, device="cuda")
+    num = torch.zeros(1, dtype=torch.int32, device="cuda")
+    P = lambda t: ctypes.c_void_p(t.data_ptr())
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    acc = np.zeros(4)
+    ms = np.zeros(4, np.float32)
+    npairs = 0
+    for k in range(warmup + steps):
+        dmn += dv
+        dmx += dv
+        L.call("mi_rb_grid_find_pairs", g, P(dmn), P(dmx), None, n, P(pairs), P(num), stream)
+        L.call("mi_rb_grid_last_ms", g, ms.ctypes.data_as(ctypes.c_void_p))
+        if k >= warmup:
+            acc += ms
+            npairs += int(num.cpu()[0])
+    flags = ctypes.c_int()
+    L.call("mi_rb_grid_overflow", g, ctypes.byref(flags))
+    L.lib().mi_rb_grid_destroy(g)
+    acc /= steps
+    key_bits = 24 + 2
+    passes = (key_bits + 7) // 8
+    hss_bytes = (40 + (2 * passes + 1) * 8 + 8) * n      # hash + sort + count scan
+    hss_ms = acc[0] + acc[1]                              # hash, sort (the scan is inside "pairs")
+    total_ms = float(acc.sum())
+    return {"bodies": n, "steps": steps, "pairs_per_step": npairs // steps,
+            "phase_ms": {"hash": round(acc[0], 4), "sort": round(acc[1], 4),
+                         "cells_count_scan_write": round(acc[2], 4), "pair_order": round(acc[3], 4)},
+            "broadphase_ms_per_step": round(total_ms, 4),
+            "body_steps_per_s": round(n / (total_ms / 1e3), 1),
+            "key_bits": key_bits, "sort_passes": passes,
+            "hash_sort_hbm": {"bytes_per_body": (40 + (2 * passes + 1) * 8), "ms": round(hss_ms, 4),
+                              "achieved_GBs": round((40 + (2 * passes + 1) * 8) * n / (hss_ms / 1e3) / 1e9, 1),
+                              "frac_of_8TBs": round((40 + (2 * passes + 1) * 8) * n / (hss_ms / 1e3) / 8e12, 4)},
+            "overflow_flags": flags.value}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +180,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--per-scene-baseline", action="store_true")
     ap.add_argument("--per-scene-max", type=int, default=0)
+    ap.add_argument("--no-broad1m", action="store_true",
+                    help="skip the 1M-body broadphase stress figures")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes instead of timing the candidates first")
     ap.add_argument("--eager", action="store_true",
@@ -221,6 +280,11 @@ def main():
     }
     if per_scene is not None:
         out["per_scene_launch_sets"] = per_scene
+    if not args.no_broad1m and args.workload == "hulls64k":
+        if w is not None:
+            w.close()
+            w = None
+        out["broadphase_1m"] = broadphase_1m()
     if w is not None:
         w.close()
     print(json.dumps(out))

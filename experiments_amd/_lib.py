@@ -16,11 +16,11 @@ STATUS_NAMES = {0: "OK", 1: "INVALID_ARG", 2: "CAPACITY", 3: "DEVICE", 4: "UNSUP
 WARM_ZERO, WARM_PREVIOUS, WARM_LEARNED = 0, 1, 2
 INERTIA_REFERENCE, INERTIA_BOX = 0, 1
 OVERFLOW_PAIRS, OVERFLOW_BODY_PAIRS, OVERFLOW_CONTACTS = 1, 2, 4
-OVERFLOW_COLORS, OVERFLOW_TILE, OVERFLOW_POLYGON = 8, 16, 32
+OVERFLOW_COLORS, OVERFLOW_TILE, OVERFLOW_POLYGON, OVERFLOW_LEVEL = 8, 16, 32, 64
 (BUF_SORTED_KV, BUF_AABB, BUF_PAIRS, BUF_CONTACTS, BUF_CONSTRAINTS, BUF_SOLVE_ORDER, BUF_COLORS,
  BUF_INV_INERTIA, BUF_PHASES, BUF_HULLS, BUF_VERTS, BUF_PLANES, BUF_FACE_INDEX, BUF_INDICES,
  BUF_EDGES, BUF_INV_MASS, BUF_INV_INERTIA_LOCAL, BUF_SCENE, BUF_IS_LARGE, BUF_SHAPE,
- BUF_FEATURES, BUF_PREV_KEYS, BUF_PREV_IMPULSES, BUF_PREDICTIONS) = range(24)
+ BUF_FEATURES, BUF_PREV_KEYS, BUF_PREV_IMPULSES, BUF_PREDICTIONS, BUF_PREV_POINTS) = range(25)
 MLP_NW, MLP_NB = 64 * 16 + 64 * 64 + 3 * 64, 64 + 64 + 3
 (STAGE_SETUP, STAGE_SORT, STAGE_PAIRS, STAGE_NARROWPHASE, STAGE_CONSTRAINTS, STAGE_COLOR,
  STAGE_SOLVE, STAGE_INTEGRATE) = range(8)
@@ -73,7 +73,8 @@ class DeviceView(ctypes.Structure):
 
 class GridDesc(ctypes.Structure):
     _fields_ = [("cell_size", ctypes.c_float), ("dims", ctypes.c_int * 3),
-                ("max_pairs", ctypes.c_int), ("max_pairs_per_body", ctypes.c_int)]
+                ("max_pairs", ctypes.c_int), ("max_pairs_per_body", ctypes.c_int),
+                ("levels", ctypes.c_int)]
 
 
 class MiRbError(RuntimeError):
@@ -113,6 +114,7 @@ SIGNATURES = {
     "mi_rb_set_warmstart_weights": (I, [VP, VP, VP]),
     "mi_rb_copy_training_batch": (I, [VP, VP, VP, I, PI]),
     "mi_rb_set_tuning": (I, [VP, ctypes.POINTER(Tuning)]),
+    "mi_rb_set_solver": (I, [VP, I, ctypes.c_float, I]),
     "mi_rb_get_tuning": (I, [VP, ctypes.POINTER(Tuning)]),
     "mi_rb_autotune": (I, [VP, I, ctypes.POINTER(Tuning), VP]),
     "mi_rb_mlp_create": (I, [ctypes.c_uint64, PVP]),
@@ -133,6 +135,7 @@ SIGNATURES = {
     "mi_rb_grid_destroy": (I, [VP]),
     "mi_rb_grid_find_pairs": (I, [VP, VP, VP, VP, I, VP, VP, VP]),
     "mi_rb_grid_last_ms": (I, [VP, VP]),
+    "mi_rb_grid_overflow": (I, [VP, PI]),
 }
 
 

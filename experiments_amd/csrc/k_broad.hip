@@ -119,20 +119,6 @@ __global__ __launch_bounds__(256) void k_setup(
     }
 }
 
-// Cell key of each AABB for the stand-alone grid API (no scenes).
-__global__ void k_aabb_hash(const float4 *mn, const float4 *mx, const int *is_large, int n,
-                            GridP g, uint2 *kv) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint32_t key = KEY_NONE;
-    if (!(is_large && is_large[i])) {
-        int gp[3];
-        grid_pos(mn[i], mx[i], g.inv_cell, g.dims, gp);
-        key = pos_hash(gp[0], gp[1], gp[2], g.dims);
-    }
-    kv[i] = make_uint2(key, (uint32_t)i);
-}
-
 __global__ void k_cell_start(const uint2 *__restrict__ kv, int n, uint32_t *cell_start) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
@@ -329,6 +315,121 @@ __global__ __launch_bounds__(256) void k_emit_large(
     }
 }
 
+
+// ---- hierarchical grid (stand-alone grid API) ----
+// Level l has cells of cell0 * 2^l; an AABB lives on the lowest level whose
+// cell is at least its largest extent, keyed (level << cell_bits) | cell of
+// its centre.  Two overlapping AABBs on levels lx <= ly have centres within
+// one level-ly cell per axis, so each pair is found exactly once, by the
+// lower-level body: its 27 own-level neighbour cells (partners of lower index)
+// and the 27 cells around its centre on every higher level (all partners).
+struct HGridP {
+    float inv_cell0;
+    int dims[3];
+    int cell_bits;
+    int levels;
+};
+
+__device__ __forceinline__ int hg_level(float4 mn, float4 mx, float inv_cell0, int levels) {
+    float ext = fmaxf(fmaxf(mx.x - mn.x, mx.y - mn.y), mx.z - mn.z) * inv_cell0;
+    float c = 1.f;
+    for (int l = 0; l < levels; l++, c *= 2.f)
+        if (ext <= c) return l;
+    return -1;
+}
+__device__ __forceinline__ void hg_cell(float4 mn, float4 mx, float inv_l, const int *dims,
+                                        int gp[3]) {
+    grid_pos(mn, mx, inv_l, dims, gp);
+}
+
+__global__ __launch_bounds__(256) void k_hg_hash(const float4 *__restrict__ mn,
+                                                 const float4 *__restrict__ mx, int n, HGridP g,
+                                                 uint2 *__restrict__ kv, int *overflow) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 a = mn[i], b = mx[i];
+    int l = hg_level(a, b, g.inv_cell0, g.levels);
+    uint32_t key = KEY_NONE;
+    if (l < 0) {
+        atomicOr(overflow, (int)MI_RB_OVERFLOW_LEVEL);
+    } else {
+        int gp[3];
+        hg_cell(a, b, g.inv_cell0 * exp2f(-(float)l), g.dims, gp);
+        key = ((uint32_t)l << g.cell_bits) | pos_hash(gp[0], gp[1], gp[2], g.dims);
+    }
+    kv[i] = make_uint2(key, (uint32_t)i);
+}
+
+// One thread per sorted entry.  WRITE = 0: count the entry's pairs;
+// WRITE = 1: write them as (max, min) at the scanned offset.
+template <int WRITE>
+__global__ __launch_bounds__(256) void k_hg_pairs(int n, const uint2 *__restrict__ kv,
+                                                  const uint32_t *__restrict__ cell_start,
+                                                  const float4 *__restrict__ aabb_min,
+                                                  const float4 *__restrict__ aabb_max, HGridP g,
+                                                  uint32_t *__restrict__ counts,
+                                                  const uint32_t *__restrict__ offsets,
+                                                  int max_pairs, uint2 *__restrict__ out) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    uint2 e = kv[t];
+    if (e.x == KEY_NONE) {
+        if (!WRITE) counts[t] = 0;
+        return;
+    }
+    const int x = (int)e.y;
+    const int lx = (int)(e.x >> g.cell_bits);
+    const float4 mn0 = aabb_min[x], mx0 = aabb_max[x];
+    uint32_t c = 0, o = WRITE ? offsets[t] : 0;
+    for (int l = lx; l < g.levels; l++) {
+        int gp[3];
+        hg_cell(mn0, mx0, g.inv_cell0 * exp2f(-(float)l), g.dims, gp);
+        for (int dz = -1; dz <= 1; dz++)
+            for (int dy = -1; dy <= 1; dy++)
+                for (int dx = -1; dx <= 1; dx++) {
+                    uint32_t nk = ((uint32_t)l << g.cell_bits) |
+                                  pos_hash(gp[0] + dx, gp[1] + dy, gp[2] + dz, g.dims);
+                    uint32_t s = cell_start[nk];
+                    if (s >= (uint32_t)n || kv[s].x != nk || (s > 0 && kv[s - 1].x == nk)) continue;
+                    for (uint32_t u = s; u < (uint32_t)n; u++) {
+                        uint2 f = kv[u];
+                        if (f.x != nk) break;
+                        int y = (int)f.y;
+                        if (l == lx && y >= x) continue;
+                        if (!aabb_overlap(mn0, mx0, aabb_min[y], aabb_max[y])) continue;
+                        if (WRITE && o + c < (uint32_t)max_pairs)
+                            out[o + c] = make_uint2((uint32_t)(x > y ? x : y), (uint32_t)(x > y ? y : x));
+                        c++;
+                    }
+                }
+    }
+    if (!WRITE) counts[t] = c;
+}
+
+// pair list <-> sort records: (minor, major) for the first sort, then
+// (major, minor) for the second, then int2 (x, y)
+__global__ void k_pairs_to_kv(const uint32_t *__restrict__ total, int max_pairs,
+                              uint2 *__restrict__ kv, int swap) {
+    uint32_t np = *total < (uint32_t)max_pairs ? *total : (uint32_t)max_pairs;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    uint2 v = kv[i];
+    if (swap) kv[i] = make_uint2(v.y, v.x);
+}
+__global__ void k_pairs_out(const uint32_t *__restrict__ total, int max_pairs,
+                            const uint2 *__restrict__ kv, int2 *__restrict__ pairs,
+                            int *num_pairs, int *overflow) {
+    uint32_t tt = *total;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        if (tt > (uint32_t)max_pairs) atomicOr(overflow, (int)MI_RB_OVERFLOW_PAIRS);
+        *num_pairs = (int)(tt < (uint32_t)max_pairs ? tt : (uint32_t)max_pairs);
+    }
+    uint32_t np = tt < (uint32_t)max_pairs ? tt : (uint32_t)max_pairs;
+    if (i >= np) return;
+    uint2 v = kv[i];
+    pairs[i] = make_int2((int)v.x, (int)v.y);
+}
 }  // namespace
 
 // ---------------------------------------------------------------- launchers
@@ -397,49 +498,47 @@ hipError_t launch_emit_pairs(const DeviceWorld &d, const StepParams &p, hipStrea
     return hipGetLastError();
 }
 
-hipError_t launch_aabb_hash(const float4 *mn, const float4 *mx, const int *is_large, int n,
-                            float inv_cell, const int dims[3], uint2 *kv, hipStream_t s) {
-    if (n == 0) return hipSuccess;
-    GridP g;
-    g.inv_cell = inv_cell;
-    g.dims[0] = dims[0];
-    g.dims[1] = dims[1];
-    g.dims[2] = dims[2];
-    g.cell_bits = 0;
-    hipLaunchKernelGGL(k_aabb_hash, dim3((n + 255) / 256), dim3(256), 0, s, mn, mx, is_large, n, g,
-                       kv);
-    return hipGetLastError();
-}
-
-hipError_t launch_grid_pairs(const float4 *mn, const float4 *mx, const uint2 *kv,
-                             uint32_t *cell_start, const int *large_list, int n_large, int n,
-                             float inv_cell, const int dims[3], int cell_bits, int K,
-                             int *body_slots, uint32_t *pair_count, int *overflow, hipStream_t s) {
-    (void)cell_bits;
-    if (n == 0) return hipSuccess;
-    GridP g;
-    g.inv_cell = inv_cell;
-    g.dims[0] = dims[0];
-    g.dims[1] = dims[1];
-    g.dims[2] = dims[2];
-    g.cell_bits = 31;  // one scene: (key >> 31) is 0 for every cell key
-    int blk = K > 32 ? FP_BLOCK / 2 : FP_BLOCK;
-    size_t lds = sizeof(int) * (size_t)blk * (K + 1);
-    // large bodies: large_list holds [n_large indices][first=0][num=n_large]
-    hipLaunchKernelGGL(k_find_pairs, dim3((n + blk - 1) / blk), dim3(blk), lds, s, n,
-                       kv, cell_start, mn, mx, large_list, n_large ? large_list + n_large : nullptr,
-                       n_large ? large_list + n_large + 1 : nullptr, g, K, body_slots, pair_count,
-                       overflow);
-    return hipGetLastError();
-}
-
-hipError_t launch_grid_emit_simple(const uint32_t *pair_count, const uint32_t *pair_off,
-                                   const int *body_slots, int n, int K, int max_pairs, int2 *pairs,
-                                   const uint32_t *total, int *num_pairs, int *overflow,
-                                   hipStream_t s) {
-    hipLaunchKernelGGL(k_emit_pairs, dim3((n + 255) / 256 > 0 ? (n + 255) / 256 : 1), dim3(256), 0,
-                       s, n, pair_count, pair_off, body_slots, K, (const int *)nullptr,
-                       (const int *)nullptr, (const int *)nullptr, (const int *)nullptr, max_pairs,
-                       pairs, total, num_pairs, overflow);
+// Hierarchical-grid pair search of n AABBs: hash, sort, cell starts, count,
+// scan, write, then the (x, y) order by two stable sorts (y, then x).
+hipError_t launch_hgrid_pairs(const float4 *mn, const float4 *mx, int n, float inv_cell0,
+                              const int dims[3], int cell_bits, int levels, uint2 *kv,
+                              uint32_t *cell_start, uint32_t *counts, uint32_t *offsets,
+                              uint32_t *total, uint2 *pair_kv, int max_pairs, int pair_bits,
+                              int2 *pairs, int *num_pairs, int *overflow, const SortWork &sw,
+                              uint32_t *scan_status, hipStream_t s, hipEvent_t *ev) {
+    HGridP g;
+    g.inv_cell0 = inv_cell0;
+    g.dims[0] = dims[0]; g.dims[1] = dims[1]; g.dims[2] = dims[2];
+    g.cell_bits = cell_bits;
+    g.levels = levels;
+    const int blocks = n > 0 ? (n + 255) / 256 : 1;
+    if (ev) hipEventRecord(ev[0], s);
+    if (n > 0) hipLaunchKernelGGL(k_hg_hash, dim3(blocks), dim3(256), 0, s, mn, mx, n, g, kv, overflow);
+    if (ev) hipEventRecord(ev[1], s);
+    hipError_t e = launch_radix_sort(kv, n, cell_bits + 2, 8, sw, s);
+    if (e != hipSuccess) return e;
+    if (ev) hipEventRecord(ev[2], s);
+    if (n > 0) {
+        hipLaunchKernelGGL(k_cell_start, dim3(blocks), dim3(256), 0, s, kv, n, cell_start);
+        hipLaunchKernelGGL(k_hg_pairs<0>, dim3(blocks), dim3(256), 0, s, n, kv, cell_start, mn, mx,
+                           g, counts, (const uint32_t *)nullptr, max_pairs, pair_kv);
+    }
+    e = launch_exclusive_scan(counts, offsets, n, total, scan_status, s);
+    if (e != hipSuccess) return e;
+    if (n > 0)
+        hipLaunchKernelGGL(k_hg_pairs<1>, dim3(blocks), dim3(256), 0, s, n, kv, cell_start, mn, mx,
+                           g, counts, offsets, max_pairs, pair_kv);
+    if (ev) hipEventRecord(ev[3], s);
+    // pair_kv holds (x, y): sort by y (swap to key y), then by x (swap back)
+    const int pblocks = (max_pairs + 255) / 256;
+    hipLaunchKernelGGL(k_pairs_to_kv, dim3(pblocks), dim3(256), 0, s, total, max_pairs, pair_kv, 1);
+    e = launch_radix_sort(pair_kv, max_pairs, pair_bits, 8, sw, s, (const int *)total);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_pairs_to_kv, dim3(pblocks), dim3(256), 0, s, total, max_pairs, pair_kv, 1);
+    e = launch_radix_sort(pair_kv, max_pairs, pair_bits, 8, sw, s, (const int *)total);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_pairs_out, dim3(pblocks), dim3(256), 0, s, total, max_pairs, pair_kv, pairs,
+                       num_pairs, overflow);
+    if (ev) hipEventRecord(ev[4], s);
     return hipGetLastError();
 }

@@ -124,17 +124,25 @@ __device__ __forceinline__ void mlp_tile(const MlpRegs &m, float4 x, f32x4 a1[4]
 }
 
 // Impulses from a prediction (network units are RB_MLP_SCALE x impulse):
-// the normal impulse is clamped at zero and split evenly over the points with
-// a row, friction is clamped to the cone of that normal impulse.
+// the predicted normal impulse, clamped at zero, is shared over the points in
+// proportion to the impulses the constraint setup matched from the previous
+// step (evenly over the points with a row when there are none); friction is
+// clamped to the cone of the predicted normal impulse.
 __device__ __forceinline__ void apply_prediction(int k, float y0, float y1, float y2,
                                                  float friction, ConstraintSoA cs) {
-    float4 jac = cs.jac_inv[k];
-    float cnt = (float)((jac.x != 0.f) + (jac.y != 0.f) + (jac.z != 0.f) + (jac.w != 0.f));
+    const float4 jac = cs.jac_inv[k], lp = cs.lambda[k];
+    const float cnt = (float)((jac.x != 0.f) + (jac.y != 0.f) + (jac.z != 0.f) + (jac.w != 0.f));
+    const float sum = ((lp.x + lp.y) + lp.z) + lp.w;
     float ln = y0 / RB_MLP_SCALE;
     ln = ln > 0.f ? ln : 0.f;
-    float per = cnt > 0.f ? ln / cnt : 0.f;
-    cs.lambda[k] = make_float4(jac.x != 0.f ? per : 0.f, jac.y != 0.f ? per : 0.f,
-                               jac.z != 0.f ? per : 0.f, jac.w != 0.f ? per : 0.f);
+    const float jv[4] = {jac.x, jac.y, jac.z, jac.w}, pv[4] = {lp.x, lp.y, lp.z, lp.w};
+    float lv[4];
+    for (int i = 0; i < 4; i++) {
+        if (jv[i] == 0.f) lv[i] = 0.f;
+        else if (sum > 0.f) lv[i] = ln * (pv[i] / sum);
+        else lv[i] = cnt > 0.f ? ln / cnt : 0.f;
+    }
+    cs.lambda[k] = make_float4(lv[0], lv[1], lv[2], lv[3]);
     float4 fr = cs.fric[k];
     float lim = friction * ln;
     float f0 = y1 / RB_MLP_SCALE, f1 = y2 / RB_MLP_SCALE;

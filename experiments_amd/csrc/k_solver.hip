@@ -79,8 +79,8 @@ __global__ __launch_bounds__(256) void k_constraints(
     const float *__restrict__ inv_mass, const float4 *__restrict__ invI, const int *__restrict__ scene,
     const float4 *__restrict__ aabb_min, const float4 *__restrict__ aabb_max, ConstraintSoA cs,
     uint2 *__restrict__ tile_kv, int *counters, const int2 *__restrict__ prev_keys,
-    const float4 *__restrict__ prev_imp, const int *__restrict__ prev_count,
-    float *__restrict__ features) {
+    const float4 *__restrict__ prev_imp, const float4 *__restrict__ prev_pts,
+    const int *__restrict__ prev_count, float *__restrict__ features) {
     int np = *num_pairs_ptr;
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np || !has_contact[p]) return;
@@ -140,7 +140,7 @@ __global__ __launch_bounds__(256) void k_constraints(
     // ---- warm start: previous step's impulses of the same body pair ----
     int prev_n = *prev_count;
     float4 prev = mk4(0, 0, 0, 0);
-    int has_prev = 0;
+    int has_prev = 0, lo_match = 0;
     {
         int lo = 0, hi = prev_n;
         while (lo < hi) {
@@ -151,6 +151,7 @@ __global__ __launch_bounds__(256) void k_constraints(
         if (lo < prev_n && prev_keys[lo].x == a && prev_keys[lo].y == b) {
             prev = prev_imp[lo];
             has_prev = 1;
+            lo_match = lo;
         }
     }
     // per-contact features of the learned warm start (16 floats)
@@ -178,12 +179,25 @@ __global__ __launch_bounds__(256) void k_constraints(
         fo[2] = mk4(f[8], f[9], f[10], f[11]);
         fo[3] = mk4(f[12], f[13], f[14], f[15]);
     }
-    if (sp.warm_start == MI_RB_WARM_PREVIOUS && has_prev && npts > 0) {
-        // the pair's previous normal impulse, split evenly over the points with a row
-        float cnt = (float)((jac[0] != 0.f) + (jac[1] != 0.f) + (jac[2] != 0.f) + (jac[3] != 0.f));
-        float ln = cnt > 0.f ? prev.x / cnt : 0.f;
-        cs.lambda[k] = mk4(jac[0] != 0.f ? ln : 0.f, jac[1] != 0.f ? ln : 0.f,
-                           jac[2] != 0.f ? ln : 0.f, jac[3] != 0.f ? ln : 0.f);
+    if (sp.warm_start != MI_RB_WARM_ZERO && has_prev && npts > 0) {
+        // each point takes the normal impulse of the nearest point of the
+        // pair's previous manifold (first of equals); the learned start
+        // rescales these to its predicted total
+        const int pn = (int)prev.w;
+        float lv[4] = {0.f, 0.f, 0.f, 0.f};
+        for (int i = 0; i < 4; i++) {
+            if ((float)i >= npts || jac[i] == 0.f) continue;
+            float4 q = src.pos[i];
+            float dbest = FLT_MAX, lbest = 0.f;
+            for (int j = 0; j < pn; j++) {
+                float4 pp = prev_pts[4 * (size_t)lo_match + j];
+                float dx = q.x - pp.x, dy = q.y - pp.y, dz = q.z - pp.z;
+                float d2 = (dx * dx + dy * dy) + dz * dz;
+                if (d2 < dbest) { dbest = d2; lbest = pp.w; }
+            }
+            lv[i] = lbest;
+        }
+        cs.lambda[k] = mk4(lv[0], lv[1], lv[2], lv[3]);
         fr.z = prev.y;
         fr.w = prev.z;
     }
@@ -553,16 +567,25 @@ __device__ __forceinline__ void row_warm(const RowData &d, float4 &vA, float4 &w
     }
 }
 
-// previous-step impulse cache for the warm start, keyed by the body pair
+// Previous-step impulse cache for the warm start, keyed by the body pair
+// (constraints are ordered by pair, so the keys are sorted): the totals
+// (sum of normal impulses, friction 0, friction 1, number of points) and each
+// point's position with its normal impulse.
 __global__ void k_save_prev(const int *counters, ConstraintSoA cs, int2 *prev_keys,
-                            float4 *prev_imp, int *prev_count) {
+                            float4 *prev_imp, float4 *prev_pts, int *prev_count) {
     int nc = counters[RB_CNT_CONTACTS];
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k == 0) *prev_count = nc;
     if (k >= nc) return;
-    float4 l = cs.lambda[k], f = cs.fric[k];
+    float4 l = cs.lambda[k], f = cs.fric[k], jac = cs.jac_inv[k];
+    int np = (jac.x != 0.f) + (jac.y != 0.f) + (jac.z != 0.f) + (jac.w != 0.f);
     prev_keys[k] = cs.bodies[k];
-    prev_imp[k] = mk4(((l.x + l.y) + l.z) + l.w, f.z, f.w, 0.f);
+    prev_imp[k] = mk4(((l.x + l.y) + l.z) + l.w, f.z, f.w, (float)np);
+    float lv[4] = {l.x, l.y, l.z, l.w};
+    for (int i = 0; i < 4; i++) {
+        float4 p = cs.pos[4 * (size_t)k + i];
+        prev_pts[4 * (size_t)k + i] = mk4(p.x, p.y, p.z, lv[i]);
+    }
 }
 
 // One row of a sweep (mode 0 normal, 1 friction, 2 warm start), velocities
@@ -865,7 +888,7 @@ hipError_t launch_constraints(const DeviceWorld &d, const StepParams &p, hipStre
     hipLaunchKernelGGL(k_constraints, dim3(grid), dim3(256), 0, s, p, d.counters + RB_CNT_PAIRS,
                        d.has_contact, d.contact_off, d.contact_slots, d.contacts, d.pos, d.linvel,
                        d.angvel, d.inv_mass, d.inv_inertia_world, d.scene, d.aabb_min, d.aabb_max,
-                       d.cs, d.tile_kv, d.counters, d.prev_keys, d.prev_imp, d.prev_count,
+                       d.cs, d.tile_kv, d.counters, d.prev_keys, d.prev_imp, d.prev_pts, d.prev_count,
                        d.features);
     return hipGetLastError();
 }
@@ -942,7 +965,7 @@ hipError_t launch_solve(const DeviceWorld &d, const StepParams &p, hipStream_t s
     }
 #undef RB_SOLVE_ARGS
     hipLaunchKernelGGL(k_save_prev, dim3((p.max_contacts + 255) / 256), dim3(256), 0, s,
-                       d.counters, d.cs, d.prev_keys, d.prev_imp, d.prev_count);
+                       d.counters, d.cs, d.prev_keys, d.prev_imp, d.prev_pts, d.prev_count);
     return hipGetLastError();
 }
 

@@ -119,7 +119,8 @@ struct DeviceWorld {
     float *residual;             // per sweep: max scaled dlambda, max lambda (as uint bits)
     // warm start
     int2 *prev_keys;             // previous step's contact body pairs (a, b), sorted
-    float4 *prev_imp;            // their impulses: (sum of normal lambdas, friction 0, friction 1, 0)
+    float4 *prev_imp;            // their impulses: (sum of normal lambdas, friction 0, friction 1, points)
+    float4 *prev_pts;            // 4 per entry: point position (xyz) and its normal lambda (w)
     int *prev_count;             // number of entries (persists across steps)
     float *features;             // 16 floats per contact, learned warm-start inputs
     float *mlp_out;              // 3 floats per contact, learned warm-start predictions
@@ -220,13 +221,13 @@ hipError_t launch_mlp_warm(const DeviceWorld &d, const StepParams &p, hipStream_
 hipError_t launch_mlp_targets(const DeviceWorld &d, int capacity, float *x_out, float *t_out,
                               hipStream_t s);
 
-// generic kernels shared by the grid API and the world
-hipError_t launch_aabb_hash(const float4 *mn, const float4 *mx, const int *is_large, int n,
-                            float inv_cell, const int dims[3], uint2 *kv, hipStream_t s);
-hipError_t launch_grid_pairs(const float4 *mn, const float4 *mx, const uint2 *kv,
-                             uint32_t *cell_start, const int *large_list, int n_large, int n,
-                             float inv_cell, const int dims[3], int cell_bits, int K,
-                             int *body_slots, uint32_t *pair_count, int *overflow, hipStream_t s);
+// the stand-alone grid API
 hipError_t launch_cell_start_generic(const uint2 *kv, int n, uint32_t *cell_start, hipStream_t s);
+hipError_t launch_hgrid_pairs(const float4 *mn, const float4 *mx, int n, float inv_cell0,
+                              const int dims[3], int cell_bits, int levels, uint2 *kv,
+                              uint32_t *cell_start, uint32_t *counts, uint32_t *offsets,
+                              uint32_t *total, uint2 *pair_kv, int max_pairs, int pair_bits,
+                              int2 *pairs, int *num_pairs, int *overflow, const SortWork &sw,
+                              uint32_t *scan_status, hipStream_t s, hipEvent_t *ev);
 
 int rb_num_cus();

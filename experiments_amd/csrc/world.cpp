@@ -310,6 +310,7 @@ mi_rb_status derive_and_upload(World &w) {
     RB_CHECK(w.alloc(&d.counters, RB_CNT_COUNT));
     RB_CHECK(w.alloc(&d.prev_keys, maxC));
     RB_CHECK(w.alloc(&d.prev_imp, maxC));
+    RB_CHECK(w.alloc(&d.prev_pts, 4 * (size_t)maxC));
     RB_CHECK(w.alloc(&d.prev_count, 1));
     RB_CHECK(w.alloc(&d.features, 16 * (size_t)maxC));
     RB_CHECK(w.alloc(&d.mlp_out, 4 * (size_t)maxC));
@@ -693,6 +694,22 @@ extern "C" mi_rb_status mi_rb_set_tuning(void *world, const mi_rb_tuning *t) {
     return MI_RB_OK;
 }
 
+extern "C" mi_rb_status mi_rb_set_solver(void *world, int iterations, float tolerance,
+                                         int warm_start) {
+    World *w = W(world);
+    if (!w || iterations < 0 || iterations > 500 || !(tolerance >= 0.f)) return MI_RB_INVALID_ARG;
+    if (warm_start < MI_RB_WARM_ZERO || warm_start > MI_RB_WARM_LEARNED) return MI_RB_INVALID_ARG;
+    w->cfg.solver_iterations = iterations;
+    w->cfg.residual_tolerance = tolerance;
+    w->cfg.warm_start = warm_start;
+    w->sp.normal_iters = iterations;
+    w->sp.friction_iters = iterations;
+    w->sp.tol = tolerance;
+    w->sp.warm_start = warm_start;
+    if (w->graph) { hipGraphExecDestroy(w->graph); w->graph = nullptr; }
+    return MI_RB_OK;
+}
+
 extern "C" mi_rb_status mi_rb_get_tuning(void *world, mi_rb_tuning *t) {
     World *w = W(world);
     if (!w || !t) return MI_RB_INVALID_ARG;
@@ -709,7 +726,7 @@ extern "C" mi_rb_status mi_rb_autotune(void *world, int steps, mi_rb_tuning *bes
     DeviceWorld &d = w->d;
     const size_t n = w->pos.size(), maxC = (size_t)w->cfg.max_contacts;
     // snapshot of everything a step reads that a step changes
-    std::vector<float4> st(4 * n), pk(maxC), pi(maxC);
+    std::vector<float4> st(4 * n), pk(maxC), pi(maxC), pp(4 * maxC);
     int pc = 0;
     float4 *dev[4] = {d.pos, d.quat, d.linvel, d.angvel};
     RB_CHECK(hipStreamSynchronize(w->stream));
@@ -717,12 +734,14 @@ extern "C" mi_rb_status mi_rb_autotune(void *world, int steps, mi_rb_tuning *bes
         RB_CHECK(hipMemcpy(st.data() + f * n, dev[f], 16 * n, hipMemcpyDeviceToHost));
     RB_CHECK(hipMemcpy(pk.data(), d.prev_keys, 8 * maxC, hipMemcpyDeviceToHost));
     RB_CHECK(hipMemcpy(pi.data(), d.prev_imp, 16 * maxC, hipMemcpyDeviceToHost));
+    RB_CHECK(hipMemcpy(pp.data(), d.prev_pts, 64 * maxC, hipMemcpyDeviceToHost));
     RB_CHECK(hipMemcpy(&pc, d.prev_count, sizeof(int), hipMemcpyDeviceToHost));
     auto restore = [&]() -> mi_rb_status {
         for (int f = 0; f < 4; f++)
             RB_CHECK(hipMemcpy(dev[f], st.data() + f * n, 16 * n, hipMemcpyHostToDevice));
         RB_CHECK(hipMemcpy(d.prev_keys, pk.data(), 8 * maxC, hipMemcpyHostToDevice));
         RB_CHECK(hipMemcpy(d.prev_imp, pi.data(), 16 * maxC, hipMemcpyHostToDevice));
+        RB_CHECK(hipMemcpy(d.prev_pts, pp.data(), 64 * maxC, hipMemcpyHostToDevice));
         RB_CHECK(hipMemcpy(d.prev_count, &pc, sizeof(int), hipMemcpyHostToDevice));
         return MI_RB_OK;
     };
@@ -947,12 +966,14 @@ extern "C" mi_rb_status mi_rb_read_buffer(void *world, int which, void *host, si
         case MI_RB_BUF_FEATURES: src = w->d.features; elem = 64; cnt = nc; break;
         case MI_RB_BUF_PREDICTIONS: src = w->d.mlp_out; elem = 12; cnt = nc; break;
         case MI_RB_BUF_PREV_KEYS:
-        case MI_RB_BUF_PREV_IMPULSES: {
+        case MI_RB_BUF_PREV_IMPULSES:
+        case MI_RB_BUF_PREV_POINTS: {
             int pc = 0;
             RB_CHECK(hipMemcpy(&pc, w->d.prev_count, sizeof(int), hipMemcpyDeviceToHost));
             cnt = pc;
             if (which == MI_RB_BUF_PREV_KEYS) { src = w->d.prev_keys; elem = 8; }
-            else { src = w->d.prev_imp; elem = 16; }
+            else if (which == MI_RB_BUF_PREV_IMPULSES) { src = w->d.prev_imp; elem = 16; }
+            else { src = w->d.prev_pts; elem = 64; }
             break;
         }
         case MI_RB_BUF_COLORS: src = w->d.row_color; elem = 4; cnt = nc; break;
@@ -1087,44 +1108,52 @@ namespace {
 struct Grid {
     mi_rb_grid_desc desc;
     int max_n;
-    uint2 *kv;
-    uint32_t *cell_start, *pair_count, *pair_off, *total;
-    int *body_slots, *overflow, *large_list, *large_slots, *large_off, *large_index;
-    SortWork sw;
-    uint32_t *scan_status;
-    int key_bits;
-    hipEvent_t ev[5];
+    uint2 *kv = nullptr, *pair_kv = nullptr;
+    uint32_t *cell_start = nullptr, *counts = nullptr, *offsets = nullptr, *total = nullptr;
+    int *overflow = nullptr;
+    SortWork sw{};
+    uint32_t *scan_status = nullptr;
+    int cell_bits = 0, pair_bits = 0;
+    hipEvent_t ev[5] = {};
+    void release() {
+        void *ps[] = {kv, pair_kv, cell_start, counts, offsets, total, overflow, sw.tmp, sw.hist,
+                      sw.status, sw.tile_ctr, scan_status};
+        for (void *p : ps) if (p) hipFree(p);
+        for (auto &e : ev) if (e) hipEventDestroy(e);
+    }
 };
 }  // namespace
 
 extern "C" mi_rb_status mi_rb_grid_create(const mi_rb_grid_desc *desc, int max_aabbs, void **out_grid) {
     if (!desc || !out_grid || max_aabbs <= 0 || desc->cell_size <= 0.f || desc->max_pairs <= 0 ||
-        desc->max_pairs_per_body <= 0 || desc->max_pairs_per_body > 64)
+        desc->levels < 0 || desc->levels > 4)
         return MI_RB_INVALID_ARG;
     for (int k = 0; k < 3; k++)
         if (desc->dims[k] < 4 || (desc->dims[k] & (desc->dims[k] - 1))) return MI_RB_INVALID_ARG;
     Grid *g = new Grid();
     g->desc = *desc;
+    if (g->desc.levels == 0) g->desc.levels = 1;
     g->max_n = max_aabbs;
-    g->key_bits = ilog2((uint64_t)desc->dims[0] * desc->dims[1] * desc->dims[2]);
-    if (g->key_bits > 30) { delete g; return MI_RB_INVALID_ARG; }
-    size_t ncells = (size_t)1 << g->key_bits;
-    int n = max_aabbs, K = desc->max_pairs_per_body;
-    bool ok = dalloc(&g->kv, n) == hipSuccess && dalloc(&g->cell_start, ncells) == hipSuccess &&
-              dalloc(&g->pair_count, n) == hipSuccess && dalloc(&g->pair_off, n) == hipSuccess &&
-              dalloc(&g->total, 4) == hipSuccess && dalloc(&g->body_slots, (size_t)n * K) == hipSuccess &&
-              dalloc(&g->overflow, 4) == hipSuccess && dalloc(&g->sw.tmp, n) == hipSuccess &&
+    g->cell_bits = ilog2((uint64_t)desc->dims[0] * desc->dims[1] * desc->dims[2]);
+    if (g->cell_bits > 28) { delete g; return MI_RB_INVALID_ARG; }
+    g->pair_bits = std::max(1, ilog2(next_pow2(max_aabbs)));
+    size_t ncells = (size_t)1 << (g->cell_bits + 2);
+    int n = max_aabbs, P = desc->max_pairs, cap = std::max(n, P);
+    bool ok = dalloc(&g->kv, n) == hipSuccess && dalloc(&g->pair_kv, P) == hipSuccess &&
+              dalloc(&g->cell_start, ncells) == hipSuccess && dalloc(&g->counts, n) == hipSuccess &&
+              dalloc(&g->offsets, n) == hipSuccess && dalloc(&g->total, 4) == hipSuccess &&
+              dalloc(&g->overflow, 4) == hipSuccess && dalloc(&g->sw.tmp, cap) == hipSuccess &&
               dalloc(&g->sw.hist, 8 * 256) == hipSuccess &&
-              dalloc(&g->sw.status, sort_status_words(n, 8)) == hipSuccess &&
+              dalloc(&g->sw.status, sort_status_words(cap, 8)) == hipSuccess &&
               dalloc(&g->sw.tile_ctr, 16) == hipSuccess &&
               dalloc(&g->scan_status, 4 + 2 * (size_t)((n + 4095) / 4096)) == hipSuccess;
-    if (!ok) { delete g; return MI_RB_DEVICE; }
-    g->sw.max_n = n;
-    g->sw.max_tiles = (n + 4095) / 4096;
-    g->large_list = g->large_slots = g->large_off = g->large_index = nullptr;
+    if (!ok) { g->release(); delete g; return MI_RB_DEVICE; }
+    g->sw.max_n = cap;
+    g->sw.max_tiles = (cap + 4095) / 4096;
     hipMemset(g->cell_start, 0xff, sizeof(uint32_t) * ncells);
     hipMemset(g->overflow, 0, 16);
     for (auto &e : g->ev) hipEventCreate(&e);
+    if (hipDeviceSynchronize() != hipSuccess) { g->release(); delete g; return MI_RB_DEVICE; }
     *out_grid = g;
     return MI_RB_OK;
 }
@@ -1132,18 +1161,11 @@ extern "C" mi_rb_status mi_rb_grid_create(const mi_rb_grid_desc *desc, int max_a
 extern "C" mi_rb_status mi_rb_grid_destroy(void *grid) {
     Grid *g = static_cast<Grid *>(grid);
     if (!g) return MI_RB_INVALID_ARG;
-    void *ps[] = {g->kv, g->cell_start, g->pair_count, g->pair_off, g->total, g->body_slots,
-                  g->overflow, g->sw.tmp, g->sw.hist, g->sw.status, g->sw.tile_ctr, g->scan_status};
-    for (void *p : ps) hipFree(p);
-    for (auto &e : g->ev) hipEventDestroy(e);
+    hipDeviceSynchronize();
+    g->release();
     delete g;
     return MI_RB_OK;
 }
-
-hipError_t launch_grid_emit_simple(const uint32_t *pair_count, const uint32_t *pair_off,
-                                   const int *body_slots, int n, int K, int max_pairs, int2 *pairs,
-                                   const uint32_t *total, int *num_pairs, int *overflow,
-                                   hipStream_t s);
 
 extern "C" mi_rb_status mi_rb_grid_find_pairs(void *grid, const void *d_aabb_min,
                                               const void *d_aabb_max, const void *d_is_large,
@@ -1152,26 +1174,25 @@ extern "C" mi_rb_status mi_rb_grid_find_pairs(void *grid, const void *d_aabb_min
     Grid *g = static_cast<Grid *>(grid);
     if (!g || !d_aabb_min || !d_aabb_max || !d_pairs || !d_num_pairs || n < 0 || n > g->max_n)
         return MI_RB_INVALID_ARG;
-    if (d_is_large) return MI_RB_UNSUPPORTED_SHAPE;  // overflow list only through the world API
+    if (d_is_large) return MI_RB_INVALID_ARG;  // the levels replace the overflow list
     hipStream_t s = static_cast<hipStream_t>(stream);
-    const float4 *mn = static_cast<const float4 *>(d_aabb_min);
-    const float4 *mx = static_cast<const float4 *>(d_aabb_max);
-    float inv_cell = 1.0f / g->desc.cell_size;
-    int K = g->desc.max_pairs_per_body;
-    RB_CHECK(hipEventRecord(g->ev[0], s));
-    RB_CHECK(launch_aabb_hash(mn, mx, nullptr, n, inv_cell, g->desc.dims, g->kv, s));
-    RB_CHECK(hipEventRecord(g->ev[1], s));
-    RB_CHECK(launch_radix_sort(g->kv, n, g->key_bits, 8, g->sw, s));
-    RB_CHECK(hipEventRecord(g->ev[2], s));
-    RB_CHECK(launch_cell_start_generic(g->kv, n, g->cell_start, s));
-    RB_CHECK(launch_grid_pairs(mn, mx, g->kv, g->cell_start, nullptr, 0, n, inv_cell, g->desc.dims,
-                               0, K, g->body_slots, g->pair_count, g->overflow, s));
-    RB_CHECK(hipEventRecord(g->ev[3], s));
-    RB_CHECK(launch_exclusive_scan(g->pair_count, g->pair_off, n, g->total, g->scan_status, s));
-    RB_CHECK(launch_grid_emit_simple(g->pair_count, g->pair_off, g->body_slots, n, K,
-                                     g->desc.max_pairs, static_cast<int2 *>(d_pairs), g->total,
-                                     static_cast<int *>(d_num_pairs), g->overflow, s));
-    RB_CHECK(hipEventRecord(g->ev[4], s));
+    (void)hipGetLastError();
+    RB_CHECK(launch_hgrid_pairs(static_cast<const float4 *>(d_aabb_min),
+                                static_cast<const float4 *>(d_aabb_max), n,
+                                1.0f / g->desc.cell_size, g->desc.dims, g->cell_bits,
+                                g->desc.levels, g->kv, g->cell_start, g->counts, g->offsets,
+                                g->total, g->pair_kv, g->desc.max_pairs, g->pair_bits,
+                                static_cast<int2 *>(d_pairs), static_cast<int *>(d_num_pairs),
+                                g->overflow, g->sw, g->scan_status, s, g->ev));
+    return MI_RB_OK;
+}
+
+extern "C" mi_rb_status mi_rb_grid_overflow(void *grid, int *flags) {
+    Grid *g = static_cast<Grid *>(grid);
+    if (!g || !flags) return MI_RB_INVALID_ARG;
+    RB_CHECK(hipDeviceSynchronize());
+    RB_CHECK(hipMemcpy(flags, g->overflow, sizeof(int), hipMemcpyDeviceToHost));
+    RB_CHECK(hipMemset(g->overflow, 0, sizeof(int)));
     return MI_RB_OK;
 }
 

@@ -225,3 +225,28 @@ def build_world(scene, **cfg):
     if np.any(lv):
         w.set_state(linvel=lv)
     return w
+
+
+def broadphase_aabbs(n=1 << 20, box=400.0, rmin=0.25, rmax=2.0, seed=SEED):
+    """Broadphase stress input: n spheres and boxes (half each), radius (box
+    half extent) log-uniform in [rmin, rmax], centres uniform in a box^3
+    domain, uniform box orientations, N(0, 1) velocity per axis.  Numpy's
+    PCG64 seeded with `seed`.  Returns aabb_min, aabb_max (float32 [n, 4]),
+    velocity [n, 3] and radius [n]."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    r = np.exp(rng.uniform(np.log(rmin), np.log(rmax), n))
+    c = rng.uniform(-0.5 * box, 0.5 * box, (n, 3))
+    is_box = rng.integers(0, 2, n).astype(bool)
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    x, y, z, w = q.T
+    R = np.stack([np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)], -1),
+                  np.stack([2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)], -1),
+                  np.stack([2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], -1)], 1)
+    ext = np.where(is_box[:, None], np.abs(R).sum(2) * r[:, None], r[:, None])
+    mn = np.zeros((n, 4), np.float32)
+    mx = np.zeros((n, 4), np.float32)
+    mn[:, :3] = c - ext
+    mx[:, :3] = c + ext
+    v = rng.normal(size=(n, 3)).astype(np.float32)
+    return mn, mx, v, r.astype(np.float32)

@@ -2,16 +2,22 @@
 iterations to a residual tolerance for the zero, previous-step and learned
 warm starts.
 
-    python -m experiments_amd.train_warmstart [--scenes 4096] [--log-steps 200]
-        [--train-steps 20000] [--batch 65536] [--eval-steps 100] [--out w.npz]
+    python -m experiments_amd.train_warmstart [--scenes 4096] [--settle 240]
+        [--log-steps 200] [--train-steps 20000] [--batch 65536]
+        [--eval-steps 100] [--out w.npz]
 
-Protocol: the plain solver (zero warm start) runs to the tolerance for
-`log-steps` steps on the training scenes and every step's features and solved
-impulses are logged on the device; the network is trained with the library's
-own forward/backward/Adam kernels; then held-out scenes (another seed) are
-stepped `eval-steps` times from the same initial state with each warm start,
-every solve running to the same tolerance, and the sweeps used and final
-residuals are reported.  Prints one JSON line.
+Protocol:
+  1. Training scenes settle for `settle` steps with the fixed-sweep plain
+     solver.  The plain solver (zero warm start) then runs to the tolerance
+     for `log-steps` steps, and every step's features and solved impulses are
+     logged on the device.
+  2. The network is trained with the library's own forward/backward/Adam
+     kernels.
+  3. Held-out scenes (another seed) settle the same way, once per warm start.
+     Each is then stepped `eval-steps` times, every solve running to the same
+     tolerance.  The sweeps used (mean over scenes, per phase) and the final
+     residuals are reported.
+Prints one JSON line.
 """
 import argparse
 import json
@@ -25,10 +31,14 @@ from . import scenes
 from .warmstart import Mlp, log_plain_solver, run_iterations
 
 
-def make_world(scene, warm, tol, max_sweeps, graph=True):
-    w = scenes.build_world(scene, warm_start=warm, residual_tolerance=tol,
-                           solver_iterations=max_sweeps)
-    w.use_graph(graph)
+def settled_world(scene, settle, warm, tol, max_sweeps, settle_sweeps=10):
+    """The scene after `settle` fixed-sweep plain steps, then switched to the
+    given warm start and tolerance (same state for every warm start)."""
+    w = scenes.build_world(scene, solver_iterations=settle_sweeps)
+    w.use_graph(True)
+    if settle:
+        w.step(settle)
+    w.set_solver(max_sweeps, tol, warm)
     return w
 
 
@@ -46,27 +56,31 @@ def train(X, T, steps, batch, lr=2e-3, lr_min=2e-5, chunk=1000, seed=1):
     return net, np.concatenate(hist) if hist else np.zeros(0, np.float32)
 
 
-def evaluate(scene, weights, steps, tol, max_sweeps):
+def evaluate(scene, weights, settle, steps, tol, max_sweeps):
     out = {}
     for name, mode in (("zero", L.WARM_ZERO), ("previous", L.WARM_PREVIOUS),
                        ("learned", L.WARM_LEARNED)):
-        w = make_world(scene, mode, tol, max_sweeps)
+        w = settled_world(scene, settle, L.WARM_ZERO, tol, max_sweeps)
         if mode == L.WARM_LEARNED:
             w.set_warmstart_weights(*weights)
+        w.set_solver(max_sweeps, tol, mode)
         r = run_iterations(w, steps)
         r.pop("per_step")
         r["mean_sweeps"] = r["mean_iters_normal"] + r["mean_iters_friction"]
         out[name] = r
         w.close()
     z = out["zero"]["mean_sweeps"]
+    zn = out["zero"]["mean_iters_normal"]
     for name in ("previous", "learned"):
         out[name]["fewer_sweeps_than_zero"] = z / max(out[name]["mean_sweeps"], 1e-9)
+        out[name]["fewer_normal_sweeps_than_zero"] = zn / max(out[name]["mean_iters_normal"], 1e-9)
     return out
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--scenes", type=int, default=4096)
+    ap.add_argument("--settle", type=int, default=240)
     ap.add_argument("--log-steps", type=int, default=200)
     ap.add_argument("--train-steps", type=int, default=20000)
     ap.add_argument("--batch", type=int, default=65536)
@@ -82,7 +96,7 @@ def main(argv=None):
 
     t0 = time.perf_counter()
     train_scene = scenes.batched_piles(n_scenes=args.scenes, iterations=args.max_sweeps)
-    w = make_world(train_scene, L.WARM_ZERO, args.tol, args.max_sweeps)
+    w = settled_world(train_scene, args.settle, L.WARM_ZERO, args.tol, args.max_sweeps)
     X, T = log_plain_solver(w, args.log_steps)
     w.close()
     t_log = time.perf_counter() - t0
@@ -96,14 +110,15 @@ def main(argv=None):
         np.savez(args.out, w_bf16=weights[0], bias=weights[1], loss=hist)
     eval_scene = scenes.batched_piles(n_scenes=args.scenes, seed=scenes.SEED + 1,
                                       iterations=args.max_sweeps)
-    res = evaluate(eval_scene, weights, args.eval_steps, args.tol, args.max_sweeps)
+    res = evaluate(eval_scene, weights, args.settle, args.eval_steps, args.tol, args.max_sweeps)
     print(json.dumps({
         "samples": int(X.shape[0]), "log_s": round(t_log, 2), "train_s": round(t_train, 2),
         "train_steps": args.train_steps, "batch": args.batch,
         "loss_first": float(hist[:10].mean()) if len(hist) else None,
         "loss_last": float(hist[-100:].mean()) if len(hist) else None,
         "target_var": float(T.var(0).mean()),
-        "tol": args.tol, "max_sweeps": args.max_sweeps, "eval_steps": args.eval_steps,
+        "tol": args.tol, "max_sweeps": args.max_sweeps, "settle": args.settle,
+        "eval_steps": args.eval_steps,
         "eval": res}))
 
 

@@ -156,6 +156,12 @@ class World:
     def contacts(self, dtype):
         return self.read(L.BUF_CONTACTS, dtype)
 
+    def set_solver(self, iterations, tolerance=0.0, warm_start=L.WARM_ZERO):
+        L.call("mi_rb_set_solver", self.h, int(iterations), float(tolerance), int(warm_start))
+        self.cfg.solver_iterations = int(iterations)
+        self.cfg.residual_tolerance = float(tolerance)
+        self.cfg.warm_start = int(warm_start)
+
     def set_tuning(self, solve_block=0, narrow_waves=0):
         t = L.Tuning(solve_block, narrow_waves)
         L.call("mi_rb_set_tuning", self.h, ctypes.byref(t))

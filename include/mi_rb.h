@@ -40,7 +40,8 @@ enum {
     MI_RB_OVERFLOW_CONTACTS = 4,       /* contact capacity */
     MI_RB_OVERFLOW_COLORS = 8,         /* solver colouring ran out of colours */
     MI_RB_OVERFLOW_TILE = 16,          /* solver tile capacity (bodies or rows) */
-    MI_RB_OVERFLOW_POLYGON = 32        /* clipped polygon > 64 vertices */
+    MI_RB_OVERFLOW_POLYGON = 32,       /* clipped polygon > 64 vertices */
+    MI_RB_OVERFLOW_LEVEL = 64          /* grid API: an AABB larger than the top level's cell */
 };
 
 typedef struct mi_rb_config {
@@ -133,8 +134,9 @@ enum {
     MI_RB_BUF_SHAPE = 19,      /* int per body */
     MI_RB_BUF_FEATURES = 20,   /* float[16] per contact: warm-start network inputs */
     MI_RB_BUF_PREV_KEYS = 21,  /* int2 (a, b) per cached pair: impulse cache of the last solve */
-    MI_RB_BUF_PREV_IMPULSES = 22, /* float4 per cached pair: sum lambda_n, friction 0, 1, pad */
-    MI_RB_BUF_PREDICTIONS = 23 /* float[3] per contact: network output (learned warm start) */
+    MI_RB_BUF_PREV_IMPULSES = 22, /* float4 per cached pair: sum lambda_n, friction 0, 1, points */
+    MI_RB_BUF_PREDICTIONS = 23, /* float[3] per contact: network output (learned warm start) */
+    MI_RB_BUF_PREV_POINTS = 24 /* float4[4] per cached pair: point xyz, its lambda_n in w */
 };
 
 /* stages at which mi_rb_step_partial stops */
@@ -216,6 +218,10 @@ mi_rb_status mi_rb_set_warmstart_weights(void *world, const uint16_t *w_bf16, co
  * *count = contacts written (at most capacity). */
 mi_rb_status mi_rb_copy_training_batch(void *world, float *x_dev, float *t_dev, int capacity,
                                        int *count);
+/* Solver settings after creation (for comparing warm starts on one state):
+ * sweeps per phase (max with a tolerance), residual tolerance (0 = fixed
+ * sweeps), MI_RB_WARM_*.  Drops the captured graph. */
+mi_rb_status mi_rb_set_solver(void *world, int iterations, float tolerance, int warm_start);
 /* HIP-event time of the last `steps` steps, milliseconds */
 mi_rb_status mi_rb_last_step_ms(void *world, float *ms);
 
@@ -264,21 +270,27 @@ mi_rb_status mi_rb_bound_search(const void *d_keyvalue, int n, void *d_dst, int 
                                 void *stream);
 
 /* ---- grid pair search: replaces btcFindPairs.h over device AABBs ---- */
+/* Hierarchical uniform grid: level l has cells of cell_size * 2^l (levels
+ * 1..4); an AABB lives on the lowest level whose cell is at least its largest
+ * extent, so bodies of very different sizes need no overflow list. */
 typedef struct mi_rb_grid_desc {
-    float cell_size;
-    int dims[3];       /* power of two per axis */
+    float cell_size;   /* level-0 cell edge */
+    int dims[3];       /* cells per axis (power of two; positions wrap) */
     int max_pairs;
-    int max_pairs_per_body;
+    int max_pairs_per_body; /* unused by the hierarchical search (no per-body cap) */
+    int levels;        /* 1..4 (0 = 1) */
 } mi_rb_grid_desc;
 mi_rb_status mi_rb_grid_create(const mi_rb_grid_desc *desc, int max_aabbs, void **out_grid);
 mi_rb_status mi_rb_grid_destroy(void *grid);
-/* Overlapping pairs of n AABBs (float4 min[n], max[n], device); bodies flagged
- * in d_is_large (int[n], may be NULL) go on the brute-force overflow list.
- * Writes int2 (x > y) pairs sorted by (x, y) to d_pairs and the count to
- * d_num_pairs (device int). */
+/* Overlapping pairs of n AABBs (float4 min[n], max[n], device): int2 (x > y)
+ * sorted by (x, y) to d_pairs, the count to d_num_pairs (device int).
+ * d_is_large must be NULL (the levels replace the overflow list).  Overflow
+ * bits (pairs beyond max_pairs, AABBs beyond the top level) accumulate in the
+ * grid; mi_rb_grid_overflow reads and clears them. */
 mi_rb_status mi_rb_grid_find_pairs(void *grid, const void *d_aabb_min, const void *d_aabb_max,
                                    const void *d_is_large, int n, void *d_pairs, void *d_num_pairs,
                                    void *stream);
+mi_rb_status mi_rb_grid_overflow(void *grid, int *flags);
 /* time of the last find_pairs per phase (hash, sort, cells+pairs, scan+emit), ms */
 mi_rb_status mi_rb_grid_last_ms(void *grid, float ms[4]);
 

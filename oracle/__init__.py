@@ -226,19 +226,24 @@ def contact_to_constraint(contacts, pos, linvel, angvel, inv_mass, inv_inertia_w
 
 
 def contact_features(contacts, cons, pos, linvel, angvel, inv_mass, prev_keys=None,
-                     prev_imp=None, apply_previous=False):
+                     prev_imp=None, prev_pts=None, apply_previous=False):
     """16 warm-start features per contact; with apply_previous the previous
-    impulses found for the pair are written into a copy of cons."""
+    impulses found for the pair (per point: the nearest previous point's) are
+    written into a copy of cons."""
     contacts = np.ascontiguousarray(contacts, CONTACT_DTYPE)
     cons = np.array(cons, CONSTRAINT_DTYPE, copy=True)
     nc = len(contacts)
     feats = np.zeros((max(nc, 1), 16), np.float32)
     pk = np.ascontiguousarray(prev_keys if prev_keys is not None else np.zeros((0, 2)), np.int32)
     pi = np.ascontiguousarray(prev_imp if prev_imp is not None else np.zeros((0, 4)), np.float32)
+    pp = np.ascontiguousarray(prev_pts if prev_pts is not None else np.zeros((len(pk), 4, 4)),
+                              np.float32)
+    if len(pk):
+        pp = pp.reshape(len(pk), 4, 4)
     lib().orc_contact_features(
         ctypes.c_int(nc), _p(contacts), _p(cons), _p(f4(pos)), _p(f4(linvel)), _p(f4(angvel)),
         _p(np.ascontiguousarray(inv_mass, np.float32)), ctypes.c_int(len(pk)), _p(pk), _p(pi),
-        ctypes.c_int(1 if apply_previous else 0), _p(feats))
+        _p(pp), ctypes.c_int(1 if apply_previous else 0), _p(feats))
     return feats[:nc], cons
 
 

@@ -755,17 +755,21 @@ void orc_contact_to_constraint(int nc, const orc_contact *contacts, const float 
 /* Warm-start inputs (no reference counterpart: the reference zeroes lambda
  * every step, solverSetup.cl:529-531).  For contact k with constraint cons[k]
  * (orc_contact_to_constraint on the same inputs) the body pair is looked up
- * in the previous step's (a, b)-sorted impulse cache (prev_imp: sum of normal
- * lambdas, friction 0, friction 1, pad) and 16 features are written:
+ * in the previous step's (a, b)-sorted impulse cache: prev_imp holds the sum
+ * of normal impulses, friction 0, friction 1 and the number of points;
+ * prev_pts four (x, y, z, normal impulse) per entry.  16 features are written:
  *   n.xyz, 10 * min depth, vr.n, vr.t0, vr.t1, 1/mA, 1/mB,
  *   10 * previous impulses (3), npts / 4, found flag, -mean jac_inv, mean b
  * with vr the relative velocity of the manifold centre and t0, t1 from
- * plane_space(n).  With apply_previous the found impulses become the initial
- * lambdas: the normal impulse split evenly over the points with a row. */
+ * plane_space(n).  With apply_previous a found pair's impulses become the
+ * initial lambdas: each point with a row takes the normal impulse of the
+ * nearest previous point (first of equal distances), friction the previous
+ * friction impulses. */
 void orc_contact_features(int nc, const orc_contact *contacts, orc_constraint *cons,
                           const float *pos, const float *linvel, const float *angvel,
                           const float *inv_mass, int nprev, const int *prev_keys,
-                          const float *prev_imp, int apply_previous, float *features)
+                          const float *prev_imp, const float *prev_pts, int apply_previous,
+                          float *features)
 {
     for (int k = 0; k < nc; k++) {
         const orc_contact *src = &contacts[k];
@@ -780,11 +784,11 @@ void orc_contact_features(int nc, const orc_contact *contacts, orc_constraint *c
         float npts = src->normal[3];
         /* linear search: the cache is small in tests, order does not matter */
         f4 prev = z;
-        int found = 0;
+        int found = -1;
         for (int i = 0; i < nprev; i++)
             if (prev_keys[2 * i] == a && prev_keys[2 * i + 1] == b) {
                 prev = ld4(prev_imp + 4 * i);
-                found = 1;
+                found = i;
                 break;
             }
         f4 t0, t1;
@@ -804,31 +808,51 @@ void orc_contact_features(int nc, const orc_contact *contacts, orc_constraint *c
         f[0] = n.x; f[1] = n.y; f[2] = n.z; f[3] = npts > 0 ? 10.f * dmin : 0.f;
         f[4] = dot3(vr, n); f[5] = dot3(vr, t0); f[6] = dot3(vr, t1); f[7] = imA;
         f[8] = imB; f[9] = prev.x * 10.f; f[10] = prev.y * 10.f; f[11] = prev.z * 10.f;
-        f[12] = npts * 0.25f; f[13] = (float)found; f[14] = -jsum * inv_n; f[15] = bsum * inv_n;
-        if (apply_previous && found && npts > 0) {
-            int cnt = 0;
-            for (int i = 0; i < 4; i++) cnt += c->jac_inv[i] != 0.f;
-            float ln = cnt > 0 ? prev.x / (float)cnt : 0.f;
-            for (int i = 0; i < 4; i++) c->lambda[i] = c->jac_inv[i] != 0.f ? ln : 0.f;
+        f[12] = npts * 0.25f; f[13] = found >= 0 ? 1.f : 0.f; f[14] = -jsum * inv_n;
+        f[15] = bsum * inv_n;
+        if (apply_previous && found >= 0 && npts > 0) {
+            const float *pp = prev_pts + 16 * (size_t)found;
+            int pn = (int)prev.w;
+            for (int i = 0; i < 4; i++) {
+                float lbest = 0.f;
+                if ((float)i < npts && c->jac_inv[i] != 0.f) {
+                    float dbest = FLT_MAX;
+                    for (int j = 0; j < pn; j++) {
+                        float dx = src->pos[i][0] - pp[4 * j], dy = src->pos[i][1] - pp[4 * j + 1];
+                        float dz = src->pos[i][2] - pp[4 * j + 2];
+                        float d2 = (dx * dx + dy * dy) + dz * dz;
+                        if (d2 < dbest) { dbest = d2; lbest = pp[4 * j + 3]; }
+                    }
+                }
+                c->lambda[i] = lbest;
+            }
             c->f_lambda[0] = prev.y;
             c->f_lambda[1] = prev.z;
         }
     }
 }
 
-/* Initial lambdas from network predictions y[k][3] (10 x impulse): normal
- * impulse clamped at zero and split evenly over the points with a row,
- * friction clamped to friction x that normal impulse, none without friction rows. */
+/* Initial lambdas from network predictions y[k][3] (10 x impulse): the
+ * normal impulse, clamped at zero, is shared over the points in proportion
+ * to the lambdas cons already holds (the matched previous impulses), or
+ * evenly over the points with a row when they sum to zero; friction is
+ * clamped to friction x that normal impulse, none without friction rows. */
 void orc_warm_from_prediction(int nc, orc_constraint *cons, const float *y, float friction)
 {
     for (int k = 0; k < nc; k++) {
         orc_constraint *c = &cons[k];
         int cnt = 0;
         for (int i = 0; i < 4; i++) cnt += c->jac_inv[i] != 0.f;
+        float sum = ((c->lambda[0] + c->lambda[1]) + c->lambda[2]) + c->lambda[3];
         float ln = y[3 * k] / 10.f;
         ln = ln > 0.f ? ln : 0.f;
-        float per = cnt > 0 ? ln / (float)cnt : 0.f;
-        for (int i = 0; i < 4; i++) c->lambda[i] = c->jac_inv[i] != 0.f ? per : 0.f;
+        float lv[4];
+        for (int i = 0; i < 4; i++) {
+            if (c->jac_inv[i] == 0.f) lv[i] = 0.f;
+            else if (sum > 0.f) lv[i] = ln * (c->lambda[i] / sum);
+            else lv[i] = cnt > 0 ? ln / (float)cnt : 0.f;
+        }
+        for (int i = 0; i < 4; i++) c->lambda[i] = lv[i];
         float lim = friction * ln;
         float f0 = fminf(fmaxf(y[3 * k + 1] / 10.f, -lim), lim);
         float f1 = fminf(fmaxf(y[3 * k + 2] / 10.f, -lim), lim);

@@ -104,7 +104,8 @@ int orc_color_contacts(int nc, const orc_constraint *cs, const float *inv_mass, 
 void orc_contact_features(int nc, const orc_contact *contacts, orc_constraint *cons,
                           const float *pos, const float *linvel, const float *angvel,
                           const float *inv_mass, int nprev, const int *prev_keys,
-                          const float *prev_imp, int apply_previous, float *features);
+                          const float *prev_imp, const float *prev_pts, int apply_previous,
+                          float *features);
 void orc_warm_from_prediction(int nc, orc_constraint *cons, const float *y, float friction);
 void orc_solve(int nc, orc_constraint *cs, const int *order, float *linvel, float *angvel,
                const float *pos, const float *inv_mass, const float *inv_inertia_world,

@@ -82,7 +82,8 @@ def oracle_step(s, order, contacts=None, prev=None, warm=0, prediction=None, seg
     """Full step on the oracle from the snapshot state, visiting constraints in
     `order`.  With `segments` (a list of orders, one per independent tile) each
     segment is solved on its own, stopping at the residual tolerance of its
-    own rows.  prev: (keys int2[m], impulses float4[m]) of the previous solve;
+    own rows.  prev: (keys int2[m], totals float4[m], points float4[m][4]) of
+    the previous solve;
     warm: 0 zero, 1 previous impulses, 2 impulses from `prediction` (n, 3).
     Returns pos, quat, linvel, angvel, the solved constraints and the
     warm-start features."""
@@ -94,9 +95,9 @@ def oracle_step(s, order, contacts=None, prev=None, warm=0, prediction=None, seg
     cons = O.contact_to_constraint(contacts, st["pos"], lv, av, sh["inv_mass"], invI,
                                    c.solver_dt if c.solver_dt > 0 else c.dt, c.position_drift,
                                    c.baumgarte, c.friction)
-    keys, imp = prev if prev is not None else (None, None)
+    keys, imp, pts = prev if prev is not None else (None, None, None)
     feats, cons = O.contact_features(contacts, cons, st["pos"], lv, av, sh["inv_mass"], keys, imp,
-                                     apply_previous=(warm == 1))
+                                     pts, apply_previous=(warm != 0))
     if warm == 2:
         cons = O.warm_from_prediction(cons, prediction, c.friction)
     if segments is None:

@@ -145,7 +145,8 @@ def test_warm_step_matches_oracle(warm, kind):
     s = Snapshot(w)
     keys = w.read(L.BUF_PREV_KEYS, np.int32, (2,))
     imp = w.read(L.BUF_PREV_IMPULSES, np.float32, (4,))
-    assert len(keys) > 0
+    pts = w.read(L.BUF_PREV_POINTS, np.float32, (4, 4))
+    assert len(keys) > 0 and len(pts) == len(keys)
     # cache is sorted by pair (the lookup is a binary search)
     kk = keys[:, 0].astype(np.int64) * (1 << 32) + keys[:, 1]
     assert np.all(np.diff(kk) > 0)
@@ -157,7 +158,7 @@ def test_warm_step_matches_oracle(warm, kind):
     w.set_state(**{k: s.state[k] for k in ("pos", "quat", "linvel", "angvel")})
     w.step(1)
     g = w.get_state()
-    ref = oracle_step(s, order, contacts, prev=(keys, imp), warm=warm, prediction=pred)
+    ref = oracle_step(s, order, contacts, prev=(keys, imp, pts), warm=warm, prediction=pred)
     assert np.allclose(feats, ref["features"], rtol=1e-5, atol=1e-6)
     assert np.mean(ref["features"][:, 13]) > 0.5  # most pairs were in contact last step
     if warm == L.WARM_LEARNED:
@@ -210,6 +211,7 @@ def test_scene_tolerance_matches_oracle(warm):
     s = Snapshot(w)
     keys = w.read(L.BUF_PREV_KEYS, np.int32, (2,))
     imp = w.read(L.BUF_PREV_IMPULSES, np.float32, (4,))
+    pts = w.read(L.BUF_PREV_POINTS, np.float32, (4, 4))
     w.step_partial(L.STAGE_COLOR)
     segs = gpu_tile_orders(w)
     contacts = w.contacts(O.CONTACT_DTYPE)
@@ -217,7 +219,7 @@ def test_scene_tolerance_matches_oracle(warm):
     w.step(1)
     g = w.get_state()
     st = w.stats()
-    ref = oracle_step(s, None, contacts, prev=(keys, imp), warm=warm, segments=segs)
+    ref = oracle_step(s, None, contacts, prev=(keys, imp, pts), warm=warm, segments=segs)
     for k in ("linvel", "angvel", "pos", "quat"):
         c = 3 if k != "quat" else 4
         assert np.allclose(g[k][:, :c], ref[k][:, :c], rtol=1e-4, atol=1e-4), k

@@ -167,8 +167,15 @@ def test_contact_features_closed_form(npts):
     pos, lin, ang, im, invI = _bodies()
     cons = O.contact_to_constraint(c, pos, lin, ang, im, invI, 1 / 60.0)
     keys = np.array([[1, 0], [2, 1]], np.int32)
-    imp = np.array([[0.8, 0.1, -0.2, 0], [5, 5, 5, 0]], np.float32)
-    f, cw = O.contact_features(c, cons, pos, lin, ang, im, keys, imp, apply_previous=True)
+    imp = np.array([[0.8, 0.1, -0.2, 3], [5, 5, 5, 4]], np.float32)
+    # the previous manifold: 3 points, the nearest to current point i is
+    # previous point (i % 3), moved slightly; its impulse is 0.1 * (i % 3 + 1)
+    pts = np.zeros((2, 4, 4), np.float32)
+    for j in range(3):
+        pts[0, j, :3] = c["pos"][0, j, :3] + 0.01
+        pts[0, j, 3] = 0.1 * (j + 1)
+    pts[0, 0, :3] = c["pos"][0, 0, :3] - 0.01
+    f, cw = O.contact_features(c, cons, pos, lin, ang, im, keys, imp, pts, apply_previous=True)
     f = f[0]
     cen = c["pos"][0, :npts, :3].mean(0)
     r = cen - pos[1, :3]
@@ -187,7 +194,9 @@ def test_contact_features_closed_form(npts):
     assert np.isclose(f[14], -jac.mean(), rtol=1e-6)
     assert np.isclose(f[15], cons["b"][0, :npts].mean(), rtol=1e-6)
     lam = cw["lambda"][0]
-    assert np.allclose(lam[:npts], 0.8 / npts) and np.all(lam[npts:] == 0)
+    # point 3 (if present) is nearest to previous point 0 (the corners are 1 apart)
+    want = [0.1, 0.2, 0.3, 0.1][:npts]
+    assert np.allclose(lam[:npts], want) and np.all(lam[npts:] == 0)
     assert np.allclose(cw["f_lambda"][0], [0.1, -0.2])
 
 
@@ -196,19 +205,22 @@ def test_contact_features_without_previous():
     pos, lin, ang, im, invI = _bodies()
     cons = O.contact_to_constraint(c, pos, lin, ang, im, invI, 1 / 60.0)
     keys = np.array([[1, 2]], np.int32)   # another pair
-    imp = np.array([[9, 9, 9, 0]], np.float32)
-    f, cw = O.contact_features(c, cons, pos, lin, ang, im, keys, imp, apply_previous=True)
+    imp = np.array([[9, 9, 9, 4]], np.float32)
+    pts = np.ones((1, 4, 4), np.float32)
+    f, cw = O.contact_features(c, cons, pos, lin, ang, im, keys, imp, pts, apply_previous=True)
     assert np.all(f[0, 9:12] == 0) and f[0, 13] == 0
     assert np.all(cw["lambda"] == 0) and np.all(cw["f_lambda"] == 0)
 
 
 def test_warm_from_prediction_clamps():
-    c = np.concatenate([_box_contact(4), _box_contact(2)])
+    c = np.concatenate([_box_contact(4), _box_contact(2), _box_contact(4)])
     pos, lin, ang, im, invI = _bodies()
     cons = O.contact_to_constraint(c, pos, lin, ang, im, invI, 1 / 60.0)
-    y = np.array([[20.0, 30.0, -1.0], [-5.0, 1.0, 1.0]], np.float32)
+    cons["lambda"][2] = [0.1, 0.3, 0.0, 0.4]                 # matched previous impulses
+    y = np.array([[20.0, 30.0, -1.0], [-5.0, 1.0, 1.0], [16.0, 0.0, 0.0]], np.float32)
     out = O.warm_from_prediction(cons, y, 0.7)
-    assert np.allclose(out["lambda"][0], 0.5)               # 2.0 split over 4 points
+    assert np.allclose(out["lambda"][2], [0.2, 0.6, 0.0, 0.8])  # 1.6 in proportion 1:3:0:4
+    assert np.allclose(out["lambda"][0], 0.5)               # no previous: 2.0 split over 4 points
     assert np.isclose(out["f_lambda"][0, 0], 0.7 * 2.0)     # clamped to the cone
     assert np.isclose(out["f_lambda"][0, 1], -0.1)
     assert np.all(out["lambda"][1] == 0)                    # negative normal -> 0
