@@ -80,9 +80,12 @@ def cpu_baseline(world, scene, budget_s):
         if time.perf_counter() - t0 >= budget_s:
             break
     dt = time.perf_counter() - t0
-    return {"value": n * steps / dt, "unit": "body-steps/s", "cores": 1, "kind": "port",
+    import oracle as O
+    threads = int(O.lib().orc_num_threads())
+    return {"value": n * steps / dt, "unit": "body-steps/s", "cores": threads, "kind": "port",
             "sample": "%d step(s) of the same %d-body scene from the GPU run's state after "
-                      "warmup, oracle/rb_oracle.c serial C, %.1f s" % (steps, n, dt)}
+                      "warmup, oracle/rb_oracle.c (C; narrowphase on %d OpenMP threads, the "
+                      "other stages serial), %.1f s" % (steps, n, threads, dt)}
 
 
 def per_scene_baseline(scene, steps, warmup, max_scenes=None):
@@ -205,6 +208,14 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_each_way": src.numel() * 4, "launches": 5}))
         return
+
+    try:
+        # torch provides device buffers for the 1M broadphase figures; its HIP
+        # runtime must initialise before the library's first HIP call
+        import torch
+        torch.cuda.init()
+    except Exception:
+        pass
 
     from experiments_amd import scenes
     scene = build_scene(args.workload)

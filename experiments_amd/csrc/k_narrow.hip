@@ -266,12 +266,23 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_clip(
         int best_ax = 0x7fffffff;
         const int nf = hA.num_faces + hB.num_faces;
         const int ne = hA.num_edges * hB.num_edges;
+        // Each hull holds a ball of radius rho (center.w) around its centre,
+        // so along a unit axis c oriented with dot(dc, c) >= 0 the overlap is
+        // at least rhoA + rhoB - dot(dc, c).  An axis whose bound exceeds the
+        // face minimum by a rounding margin cannot be the minimum and is not
+        // projected; the minimum found is the one of the full search.
+        const int4 fb0 = face_best[p];
+        const float face_d = __int_as_float(fb0.x);
+        const float margin = 1e-3f + 4e-6f * ((fabsf(pA.x) + fabsf(pA.y) + fabsf(pA.z)) +
+                                              (fabsf(pB.x) + fabsf(pB.y) + fabsf(pB.z)));
+        const float lb_base = (hA.center.w + hB.center.w) - margin;
         for (int k = lane; k < ne; k += NP_BLOCK) {
             int e0 = k / hB.num_edges, e1 = k - e0 * hB.num_edges;
             float4 c = cross3(sEA[e0], sEB[e1]);
             if (fabsf(c.x) > 1e-6f || fabsf(c.y) > 1e-6f || fabsf(c.z) > 1e-6f) {
                 c = normalize3(c);
                 if (dot3(dc, c) < 0) c = scl4(c, -1.f);
+                if (lb_base - dot3(dc, c) > face_d) continue;
                 float mn0, mx0, mn1, mx1;
                 project(sVA, nvA, pA, oA, c, mn0, mx0);
                 project(sVB, nvB, pB, oB, c, mn1, mx1);
@@ -281,11 +292,7 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_clip(
             }
         }
         wave_argmin(best, best_ax);
-        {
-            const int4 fb = face_best[p];
-            const float fd = __int_as_float(fb.x);
-            if (fd < best || (fd == best && fb.y < best_ax)) { best = fd; best_ax = fb.y; }
-        }
+        if (face_d < best || (face_d == best && fb0.y < best_ax)) { best = face_d; best_ax = fb0.y; }
         if (best_ax == 0x7fffffff) {
             if (lane == 0) has_contact[p] = 0;
             continue;

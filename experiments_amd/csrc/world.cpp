@@ -511,7 +511,18 @@ extern "C" mi_rb_status mi_rb_register_convex_hull(void *world, const float *xyz
     hi.num_edges = ne;
     hi.index_offset = (int)w->indices.size();
     hi.num_indices = ni;
-    hi.center = make_float4(h.center[0], h.center[1], h.center[2], 0.f);
+    // w: radius of a ball around the centre inside the hull (the smallest
+    // centre-to-face distance, shrunk by 1e-4 relative), a lower bound the
+    // narrowphase uses to skip axes
+    double rho = 1e30;
+    for (int i = 0; i < nf; i++) {
+        const float *pl = &h.planes[4 * i];
+        double d = -((double)pl[0] * h.center[0] + (double)pl[1] * h.center[1] +
+                     (double)pl[2] * h.center[2] + (double)pl[3]);
+        rho = std::min(rho, d);
+    }
+    rho = nf > 0 && rho > 0.0 ? rho * (1.0 - 1e-4) : 0.0;
+    hi.center = make_float4(h.center[0], h.center[1], h.center[2], (float)rho);
     hi.aabb_min = make_float4(h.aabb_min[0], h.aabb_min[1], h.aabb_min[2], 0.f);
     hi.aabb_max = make_float4(h.aabb_max[0], h.aabb_max[1], h.aabb_max[2], 0.f);
     for (int i = 0; i < nv; i++)

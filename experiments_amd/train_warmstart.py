@@ -22,6 +22,7 @@ Prints one JSON line.
 import argparse
 import json
 import math
+import sys
 import time
 
 import numpy as np
@@ -66,6 +67,8 @@ def evaluate(scene, weights, settle, steps, tol, max_sweeps):
         w.set_solver(max_sweeps, tol, mode)
         r = run_iterations(w, steps)
         r.pop("per_step")
+        print("evaluated %s: %.1f normal / %.1f friction sweeps per scene" %
+              (name, r["mean_iters_normal"], r["mean_iters_friction"]), file=sys.stderr, flush=True)
         r["mean_sweeps"] = r["mean_iters_normal"] + r["mean_iters_friction"]
         out[name] = r
         w.close()
@@ -99,9 +102,12 @@ def main(argv=None):
     w = settled_world(train_scene, args.settle, L.WARM_ZERO, args.tol, args.max_sweeps)
     X, T = log_plain_solver(w, args.log_steps)
     w.close()
+    print("logged %d contacts" % X.shape[0], file=sys.stderr, flush=True)
     t_log = time.perf_counter() - t0
     t1 = time.perf_counter()
     net, hist = train(X, T, args.train_steps, args.batch)
+    print("trained: loss %.4f -> %.4f" % (float(hist[:10].mean()), float(hist[-100:].mean())),
+          file=sys.stderr, flush=True)
     import torch
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t1

@@ -8,6 +8,7 @@
  * links it.  Build: see oracle/Makefile (gcc -O2 -ffp-contract=off).
  */
 #include "rb_oracle.h"
+#include <omp.h>
 #include <float.h>
 #include <math.h>
 #include <stdlib.h>
@@ -616,6 +617,9 @@ int orc_narrowphase(int num_pairs, const int32_t *pairs, const float *pos, const
                     float max_dist, orc_contact *slots, int *has_contact)
 {
     int total = 0;
+    /* pairs are independent: spread them over the host cores (each writes
+     * only its own slot, so the result does not depend on the thread count) */
+#pragma omp parallel for reduction(+ : total) schedule(dynamic, 64)
     for (int p = 0; p < num_pairs; p++) {
         int a = pairs[2 * p], b = pairs[2 * p + 1];
         has_contact[p] = 0;
@@ -1247,3 +1251,6 @@ void orc_mlp_backward(int n, const float *x, const uint16_t *w, const float *bia
     if (loss) *loss = (float)(L / (3.0 * (double)(n ? n : 1)));
     free(gw);
 }
+
+/* threads the parallel oracle loops use (OpenMP) */
+int orc_num_threads(void) { return omp_get_max_threads(); }

@@ -57,6 +57,7 @@ typedef struct {
 } orc_constraint;
 
 /* ---- primitives ---------------------------------------------------------- */
+int orc_num_threads(void);
 void orc_radix_sort_kv(uint32_t *kv, int n, int sort_bits);
 uint32_t orc_exclusive_scan(const uint32_t *in, uint32_t *out, int n);
 void orc_bound_search(const uint32_t *kv, int n, uint32_t *dst, int n_dst, int mode);
