@@ -80,7 +80,7 @@ __global__ __launch_bounds__(256) void k_constraints(
     const float4 *__restrict__ aabb_min, const float4 *__restrict__ aabb_max, ConstraintSoA cs,
     uint2 *__restrict__ tile_kv, int *counters, const int2 *__restrict__ prev_keys,
     const float4 *__restrict__ prev_imp, const float4 *__restrict__ prev_pts,
-    const int *__restrict__ prev_count, float *__restrict__ features) {
+    const int *__restrict__ prev_count, float *__restrict__ features, int *__restrict__ solve_pos) {
     int np = *num_pairs_ptr;
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np || !has_contact[p]) return;
@@ -137,6 +137,7 @@ __global__ __launch_bounds__(256) void k_constraints(
     for (int i = 0; i < 4; i++) cs.pos[4 * k + i] = (float)i < npts ? src.pos[i] : mk4(0, 0, 0, 0);
     cs.bodies[k] = make_int2(a, b);
     cs.pair[k] = p;
+    solve_pos[k] = -1;  // set by the tile packing when the row is solved
     // ---- warm start: previous step's impulses of the same body pair ----
     int prev_n = *prev_count;
     float4 prev = mk4(0, 0, 0, 0);
@@ -394,12 +395,13 @@ __global__ __launch_bounds__(TB) void k_tile_color(
 }
 
 // ---- solve ----
-constexpr int SOLVE_SLOTS = 1024;  // LDS velocity cache of one tile (36 KB)
+constexpr int SOLVE_SLOTS = 1024;  // LDS velocity cache of one tile (32 KB)
 struct SolveLDS {
-    int key[SOLVE_SLOTS];
     float4 v[SOLVE_SLOTS];
     float4 w[SOLVE_SLOTS];
 };
+constexpr int SLOT_STATIC = -1;    // static body: zero velocity, no writes
+constexpr int SLOT_GLOBAL = -2;    // dynamic body beyond the LDS cache: global memory
 
 // Constraint fields one row needs (Constraint4 minus the body indices).
 struct RowData {
@@ -415,30 +417,29 @@ struct RowBodies {
     float4 pA, pB;
 };
 
-__device__ __forceinline__ void load_row(int k, const ConstraintSoA &cs, RowData &d) {
-    d.lin0 = cs.linear[k];
-    d.jac = cs.jac_inv[k];
-    d.bb = cs.b[k];
-    d.lam = cs.lambda[k];
-    d.fr = cs.fric[k];
-    d.cen = cs.center[k];
-    for (int i = 0; i < 4; i++) d.p[i] = cs.pos[4 * k + i];
-}
-
-__device__ __forceinline__ void load_bodies(int2 bd, const float *inv_mass, const float4 *invI,
-                                            const float4 *pos, const int *key, RowBodies &rb) {
-    rb.a = bd.x;
-    rb.b = bd.y;
-    rb.imA = inv_mass[bd.x];
-    rb.imB = inv_mass[bd.y];
-    for (int i = 0; i < 3; i++) {
-        rb.IA[i] = invI[3 * bd.x + i];
-        rb.IB[i] = invI[3 * bd.y + i];
+// A row of the packed solve arrays (position `i` in solve order).
+__device__ __forceinline__ void load_packed(int i, const PackedRows &pk, RowData &d, RowBodies &rb) {
+    d.lin0 = pk.lin0[i];
+    d.jac = pk.jac[i];
+    d.bb = pk.b[i];
+    d.lam = pk.lam[i];
+    d.fr = pk.fric[i];
+    d.cen = pk.cen[i];
+    for (int q = 0; q < 4; q++) d.p[q] = pk.p[4 * (size_t)i + q];
+    const int4 m = pk.meta[i];
+    rb.a = m.x;
+    rb.b = m.y;
+    rb.ha = m.z;
+    rb.hb = m.w;
+    const float2 im = pk.im[i];
+    rb.imA = im.x;
+    rb.imB = im.y;
+    for (int q = 0; q < 3; q++) {
+        rb.IA[q] = pk.IA[3 * (size_t)i + q];
+        rb.IB[q] = pk.IB[3 * (size_t)i + q];
     }
-    rb.pA = pos[bd.x];
-    rb.pB = pos[bd.y];
-    rb.ha = rb.imA != 0.f ? hash_find<SOLVE_SLOTS>(key, bd.x) : -1;
-    rb.hb = rb.imB != 0.f ? hash_find<SOLVE_SLOTS>(key, bd.y) : -1;
+    rb.pA = pk.pA[i];
+    rb.pB = pk.pB[i];
 }
 
 // solveContact (solveContact.cl:272-312): sequential impulses on the normal
@@ -567,17 +568,30 @@ __device__ __forceinline__ void row_warm(const RowData &d, float4 &vA, float4 &w
     }
 }
 
-// Previous-step impulse cache for the warm start, keyed by the body pair
-// (constraints are ordered by pair, so the keys are sorted): the totals
-// (sum of normal impulses, friction 0, friction 1, number of points) and each
-// point's position with its normal impulse.
-__global__ void k_save_prev(const int *counters, ConstraintSoA cs, int2 *prev_keys,
+// After the solve: the impulses go back from the packed solve arrays to the
+// constraints, and into the previous-step impulse cache for the warm start,
+// keyed by the body pair (constraints are ordered by pair, so the keys are
+// sorted): the totals (sum of normal impulses, friction 0, friction 1, number
+// of points) and each point's position with its normal impulse.
+__global__ void k_save_prev(const int *counters, ConstraintSoA cs, const int *solve_pos,
+                            const float4 *pk_lam, const float4 *pk_fric, int2 *prev_keys,
                             float4 *prev_imp, float4 *prev_pts, int *prev_count) {
     int nc = counters[RB_CNT_CONTACTS];
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k == 0) *prev_count = nc;
     if (k >= nc) return;
-    float4 l = cs.lambda[k], f = cs.fric[k], jac = cs.jac_inv[k];
+    float4 l, f;
+    const int pos = solve_pos[k];
+    if (pos >= 0) {
+        l = pk_lam[pos];
+        f = pk_fric[pos];
+        cs.lambda[k] = l;
+        cs.fric[k] = f;
+    } else {  // not solved (tile capacity exceeded): the initial impulses stay
+        l = cs.lambda[k];
+        f = cs.fric[k];
+    }
+    float4 jac = cs.jac_inv[k];
     int np = (jac.x != 0.f) + (jac.y != 0.f) + (jac.z != 0.f) + (jac.w != 0.f);
     prev_keys[k] = cs.bodies[k];
     prev_imp[k] = mk4(((l.x + l.y) + l.z) + l.w, f.z, f.w, (float)np);
@@ -588,60 +602,132 @@ __global__ void k_save_prev(const int *counters, ConstraintSoA cs, int2 *prev_ke
     }
 }
 
+// Per step, after the colouring: each tile's rows in solve order as packed
+// arrays (constraint fields, both bodies' inverse mass, world inverse inertia
+// and position, and their slots in a dense list of the tile's dynamic
+// bodies), so that a solve launch reads them contiguously, with no hashing
+// and no gathers through body indices.
+struct PackLDS {
+    int key[HASH_SLOTS];
+    int dense[HASH_SLOTS];
+    int count;
+};
+__global__ __launch_bounds__(TB) void k_tile_pack(
+    const int *__restrict__ tile_first, const int *__restrict__ tile_cnt,
+    const int *__restrict__ order, ConstraintSoA cs, const float4 *__restrict__ pos,
+    const float *__restrict__ inv_mass, const float4 *__restrict__ invI, PackedRows pk,
+    int *__restrict__ solve_pos, int *__restrict__ tile_bodies, int *__restrict__ tile_nb,
+    int *counters) {
+    __shared__ PackLDS L;
+    const int t = blockIdx.x, tid = threadIdx.x;
+    const int first = tile_first[t];
+    int R = tile_cnt[t] - first;
+    if (R > RB_TILE_MAX_ROWS) R = RB_TILE_MAX_ROWS;
+    if (R <= 0) {
+        if (tid == 0) tile_nb[t] = 0;
+        return;
+    }
+    for (int i = tid; i < HASH_SLOTS; i += TB) { L.key[i] = -1; L.dense[i] = -1; }
+    if (tid == 0) L.count = 0;
+    __syncthreads();
+    for (int r = tid; r < R; r += TB) {
+        int2 bd = cs.bodies[order[first + r]];
+        bool fail = false;
+        if (inv_mass[bd.x] != 0.f) fail |= hash_insert<HASH_SLOTS>(L.key, bd.x) < 0;
+        if (inv_mass[bd.y] != 0.f) fail |= hash_insert<HASH_SLOTS>(L.key, bd.y) < 0;
+        if (fail) atomicOr(counters + RB_CNT_OVERFLOW, (int)MI_RB_OVERFLOW_TILE);
+    }
+    __syncthreads();
+    // dense slots: the occupied hash slots numbered in slot order
+    for (int i = tid; i < HASH_SLOTS; i += TB)
+        if (L.key[i] >= 0) L.dense[i] = atomicAdd(&L.count, 1);
+    __syncthreads();
+    const int nb = L.count;
+    for (int i = tid; i < HASH_SLOTS; i += TB)
+        if (L.key[i] >= 0) tile_bodies[2 * (size_t)first + L.dense[i]] = L.key[i];
+    if (tid == 0) tile_nb[t] = nb;
+    for (int r = tid; r < R; r += TB) {
+        const int i = first + r;
+        const int k = order[i];
+        const int2 bd = cs.bodies[k];
+        const float imA = inv_mass[bd.x], imB = inv_mass[bd.y];
+        int sa = SLOT_STATIC, sb = SLOT_STATIC;
+        if (imA != 0.f) {
+            int h = hash_find<HASH_SLOTS>(L.key, bd.x);
+            sa = h >= 0 && L.dense[h] < SOLVE_SLOTS ? L.dense[h] : SLOT_GLOBAL;
+        }
+        if (imB != 0.f) {
+            int h = hash_find<HASH_SLOTS>(L.key, bd.y);
+            sb = h >= 0 && L.dense[h] < SOLVE_SLOTS ? L.dense[h] : SLOT_GLOBAL;
+        }
+        pk.lin0[i] = cs.linear[k];
+        pk.jac[i] = cs.jac_inv[k];
+        pk.b[i] = cs.b[k];
+        pk.lam[i] = cs.lambda[k];
+        pk.fric[i] = cs.fric[k];
+        pk.cen[i] = cs.center[k];
+        for (int q = 0; q < 4; q++) pk.p[4 * (size_t)i + q] = cs.pos[4 * (size_t)k + q];
+        pk.meta[i] = make_int4(bd.x, bd.y, sa, sb);
+        pk.im[i] = make_float2(imA, imB);
+        for (int q = 0; q < 3; q++) {
+            pk.IA[3 * (size_t)i + q] = invI[3 * (size_t)bd.x + q];
+            pk.IB[3 * (size_t)i + q] = invI[3 * (size_t)bd.y + q];
+        }
+        pk.pA[i] = pos[bd.x];
+        pk.pB[i] = pos[bd.y];
+        solve_pos[k] = i;
+    }
+}
+
 // One row of a sweep (mode 0 normal, 1 friction, 2 warm start), velocities
 // from the LDS cache or, for bodies that did not fit it, global memory.
 __device__ __forceinline__ void run_row(int mode, RowData &d, const RowBodies &rb, SolveLDS &L,
                                         float4 *linvel, float4 *angvel, float friction,
                                         float &rmax, float &lmax) {
     const float4 z = mk4(0, 0, 0, 0);
-    float4 vA = rb.ha >= 0 ? L.v[rb.ha] : (rb.imA != 0.f ? linvel[rb.a] : z);
-    float4 wA = rb.ha >= 0 ? L.w[rb.ha] : (rb.imA != 0.f ? angvel[rb.a] : z);
-    float4 vB = rb.hb >= 0 ? L.v[rb.hb] : (rb.imB != 0.f ? linvel[rb.b] : z);
-    float4 wB = rb.hb >= 0 ? L.w[rb.hb] : (rb.imB != 0.f ? angvel[rb.b] : z);
+    float4 vA = rb.ha >= 0 ? L.v[rb.ha] : (rb.ha == SLOT_GLOBAL ? linvel[rb.a] : z);
+    float4 wA = rb.ha >= 0 ? L.w[rb.ha] : (rb.ha == SLOT_GLOBAL ? angvel[rb.a] : z);
+    float4 vB = rb.hb >= 0 ? L.v[rb.hb] : (rb.hb == SLOT_GLOBAL ? linvel[rb.b] : z);
+    float4 wB = rb.hb >= 0 ? L.w[rb.hb] : (rb.hb == SLOT_GLOBAL ? angvel[rb.b] : z);
     if (mode == 0) row_normal(d, vA, wA, vB, wB, rb, rmax, lmax);
     else if (mode == 1) row_friction(d, vA, wA, vB, wB, rb, friction, rmax, lmax);
     else row_warm(d, vA, wA, vB, wB, rb);
     if (rb.ha >= 0) { L.v[rb.ha] = vA; L.w[rb.ha] = wA; }
-    else if (rb.imA != 0.f) { linvel[rb.a] = vA; angvel[rb.a] = wA; }
+    else if (rb.ha == SLOT_GLOBAL) { linvel[rb.a] = vA; angvel[rb.a] = wA; }
     if (rb.hb >= 0) { L.v[rb.hb] = vB; L.w[rb.hb] = wB; }
-    else if (rb.imB != 0.f) { linvel[rb.b] = vB; angvel[rb.b] = wB; }
+    else if (rb.hb == SLOT_GLOBAL) { linvel[rb.b] = vB; angvel[rb.b] = wB; }
 }
 
-// Shared state of one tile's solve: the LDS body cache and the tile-wide
-// residual reduction of the current sweep.
+// Shared state of one tile's solve: the LDS velocity cache and the
+// tile-wide residual reduction of the current sweep.
 struct TileSolveLDS {
     SolveLDS c;
     unsigned int red[2];
 };
 
 // The sweeps of one tile: the optional warm-start pass, then normal sweeps,
-// then friction sweeps.  REG: the tile has at most TBK rows and row r lives in
-// the registers of thread r for the whole launch (load once, store once), so
-// a colour pass waits on no global load; otherwise rows are read from global
-// memory on every pass.  Within a colour rows share no dynamic body, so the
-// thread that runs a row does not change the result.  local_tol: a scene
-// tile ends each phase as soon as its own residual is below sp.tol.
+// then friction sweeps, over the tile's packed rows (positions first..first+R
+// in solve order).  REG: the tile has at most TBK rows and row r lives in the
+// registers of thread r for the whole launch (loaded once, stored once);
+// otherwise rows are read from the packed arrays on every pass.  Within a
+// colour rows share no dynamic body, so the thread that runs a row does not
+// change the result.  local_tol: a scene tile ends each phase as soon as its
+// own residual is below sp.tol.
 template <int TBK, bool REG>
 __device__ __forceinline__ void tile_sweeps(const StepParams &sp, int sweep0, int nsweeps_n,
                                             int nsweeps_f, int warm, bool local_tol, int first,
-                                            int R, int nc, const int *coff, const int *order,
-                                            ConstraintSoA &cs, float4 *linvel, float4 *angvel,
-                                            const float4 *pos, const float *inv_mass,
-                                            const float4 *invI, float *residual,
+                                            int R, int nc, const int *coff, const PackedRows &pk,
+                                            float4 *linvel, float4 *angvel, float *residual,
                                             TileSolveLDS &L, int used[2], float rfin[2]) {
     const int tid = threadIdx.x;
     RowData rd;
     RowBodies rb;
-    int k = -1, rc = -1;
-    if (REG) {
-        k = tid < R ? order[first + tid] : -1;
-        if (k >= 0) {
-            load_row(k, cs, rd);
-            load_bodies(cs.bodies[k], inv_mass, invI, pos, L.c.key, rb);
-            // colour of this thread's row: the colour range holding row tid
-            for (int c = 0; c < nc; c++)
-                if (coff[c] <= tid && tid < coff[c + 1]) rc = c;
-        }
+    int rc = -1;
+    if (REG && tid < R) {
+        load_packed(first + tid, pk, rd, rb);
+        // colour of this thread's row: the colour range holding row tid
+        for (int c = 0; c < nc; c++)
+            if (coff[c] <= tid && tid < coff[c + 1]) rc = c;
     }
     bool swept = false;
     for (int phase = warm ? -1 : 0; phase < 2; phase++) {
@@ -661,14 +747,12 @@ __device__ __forceinline__ void tile_sweeps(const StepParams &sp, int sweep0, in
                     int c0 = coff[c], c1 = coff[c + 1];
                     if (c1 > R) c1 = R;
                     for (int r = c0 + tid; r < c1; r += TBK) {
-                        int kk = order[first + r];
                         RowData d;
                         RowBodies b;
-                        load_row(kk, cs, d);
-                        load_bodies(cs.bodies[kk], inv_mass, invI, pos, L.c.key, b);
+                        load_packed(first + r, pk, d, b);
                         run_row(mode, d, b, L.c, linvel, angvel, sp.friction, rmax, lmax);
-                        if (mode == 0) cs.lambda[kk] = d.lam;
-                        else if (mode == 1) cs.fric[kk] = d.fr;
+                        if (mode == 0) pk.lam[first + r] = d.lam;
+                        else if (mode == 1) pk.fric[first + r] = d.fr;
                     }
                 }
                 __syncthreads();
@@ -694,23 +778,22 @@ __device__ __forceinline__ void tile_sweeps(const StepParams &sp, int sweep0, in
             }
         }
     }
-    if (REG && k >= 0 && swept) {
-        cs.lambda[k] = rd.lam;
-        cs.fric[k] = rd.fr;
+    if (REG && tid < R && swept) {
+        pk.lam[first + tid] = rd.lam;
+        pk.fric[first + tid] = rd.fr;
     }
 }
 
 // One workgroup per tile (spatial: the tiles of one parity phase; scene: one
-// scene); the tile's body velocities live in LDS for the launch.
+// scene); the tile's dynamic body velocities live in LDS for the launch.
 template <int TBK>
 __global__ __launch_bounds__(TBK) void k_tile_solve(
     StepParams sp, int phase, int sweep0, int nsweeps_n, int nsweeps_f, int warm,
     const int *__restrict__ tile_first, const int *__restrict__ tile_cnt,
-    const int *__restrict__ order, const int *__restrict__ tile_color_off,
-    const int *__restrict__ tile_ncolors, ConstraintSoA cs, float4 *__restrict__ linvel,
-    float4 *__restrict__ angvel, const float4 *__restrict__ pos, const float *__restrict__ inv_mass,
-    const float4 *__restrict__ invI, float *__restrict__ residual, int *counters,
-    int4 *__restrict__ tile_stats) {
+    const int *__restrict__ tile_color_off, const int *__restrict__ tile_ncolors,
+    const int *__restrict__ tile_bodies, const int *__restrict__ tile_nb, PackedRows pk,
+    float4 *__restrict__ linvel, float4 *__restrict__ angvel, float *__restrict__ residual,
+    int *counters, int4 *__restrict__ tile_stats) {
     __shared__ TileSolveLDS L;
     int t;
     if (sp.tile_mode == 1) {
@@ -736,40 +819,31 @@ __global__ __launch_bounds__(TBK) void k_tile_solve(
     if (R > RB_TILE_MAX_ROWS) R = RB_TILE_MAX_ROWS;
     const int nc = tile_ncolors[t];
     const int *coff = tile_color_off + t * (RB_MAX_LOCAL_COLORS + 1);
-    for (int i = tid; i < SOLVE_SLOTS; i += TBK) L.c.key[i] = -1;
-    __syncthreads();
-    // body velocities into the LDS cache
-    for (int r = tid; r < R; r += TBK) {
-        int2 bd = cs.bodies[order[first + r]];
-        if (inv_mass[bd.x] != 0.f) {
-            int h = hash_insert<SOLVE_SLOTS>(L.c.key, bd.x);
-            if (h >= 0) { L.c.v[h] = linvel[bd.x]; L.c.w[h] = angvel[bd.x]; }
-            else atomicOr(counters + RB_CNT_OVERFLOW, (int)MI_RB_OVERFLOW_TILE);
-        }
-        if (inv_mass[bd.y] != 0.f) {
-            int h = hash_insert<SOLVE_SLOTS>(L.c.key, bd.y);
-            if (h >= 0) { L.c.v[h] = linvel[bd.y]; L.c.w[h] = angvel[bd.y]; }
-            else atomicOr(counters + RB_CNT_OVERFLOW, (int)MI_RB_OVERFLOW_TILE);
-        }
+    int nb = tile_nb[t];
+    if (nb > SOLVE_SLOTS) nb = SOLVE_SLOTS;
+    const int *bodies = tile_bodies + 2 * (size_t)first;
+    for (int s = tid; s < nb; s += TBK) {
+        int b = bodies[s];
+        L.c.v[s] = linvel[b];
+        L.c.w[s] = angvel[b];
     }
     __syncthreads();
     int used[2] = {0, 0};
     float rfin[2] = {0.f, 0.f};
     if (R <= TBK)
         tile_sweeps<TBK, true>(sp, sweep0, nsweeps_n, nsweeps_f, warm, local_tol, first, R, nc,
-                               coff, order, cs, linvel, angvel, pos, inv_mass, invI, residual, L,
-                               used, rfin);
+                               coff, pk, linvel, angvel, residual, L, used, rfin);
     else
         tile_sweeps<TBK, false>(sp, sweep0, nsweeps_n, nsweeps_f, warm, local_tol, first, R, nc,
-                                coff, order, cs, linvel, angvel, pos, inv_mass, invI, residual, L,
-                                used, rfin);
+                                coff, pk, linvel, angvel, residual, L, used, rfin);
     if (local_tol && tid == 0)
         tile_stats[t] = make_int4(used[0], used[1], (int)__float_as_uint(rfin[0]),
                                   (int)__float_as_uint(rfin[1]));
     __syncthreads();
-    for (int h = tid; h < SOLVE_SLOTS; h += TBK) {
-        int b = L.c.key[h];
-        if (b >= 0) { linvel[b] = L.c.v[h]; angvel[b] = L.c.w[h]; }
+    for (int s = tid; s < nb; s += TBK) {
+        int b = bodies[s];
+        linvel[b] = L.c.v[s];
+        angvel[b] = L.c.w[s];
     }
 }
 
@@ -889,7 +963,7 @@ hipError_t launch_constraints(const DeviceWorld &d, const StepParams &p, hipStre
                        d.has_contact, d.contact_off, d.contact_slots, d.contacts, d.pos, d.linvel,
                        d.angvel, d.inv_mass, d.inv_inertia_world, d.scene, d.aabb_min, d.aabb_max,
                        d.cs, d.tile_kv, d.counters, d.prev_keys, d.prev_imp, d.prev_pts, d.prev_count,
-                       d.features);
+                       d.features, d.solve_pos);
     return hipGetLastError();
 }
 
@@ -912,6 +986,9 @@ hipError_t launch_tiles(const DeviceWorld &d, const StepParams &p, hipStream_t s
     hipLaunchKernelGGL(k_tile_color, dim3(p.n_tiles), dim3(TB), 0, s, d.tile_kv, d.tile_first,
                        d.tile_cnt, d.cs.bodies, d.inv_mass, d.solve_order, d.row_color,
                        d.tile_color_off, d.tile_ncolors, d.counters);
+    hipLaunchKernelGGL(k_tile_pack, dim3(p.n_tiles), dim3(TB), 0, s, d.tile_first, d.tile_cnt,
+                       d.solve_order, d.cs, d.pos, d.inv_mass, d.inv_inertia_world, d.pk,
+                       d.solve_pos, d.tile_bodies, d.tile_nb, d.counters);
     return hipGetLastError();
 }
 
@@ -922,8 +999,8 @@ hipError_t launch_solve(const DeviceWorld &d, const StepParams &p, hipStream_t s
     launch_fill_u32(reinterpret_cast<uint32_t *>(d.residual), 0u, (size_t)2 * (S + 1), s);
     // scene tiles: one workgroup per scene, 256 or 512 threads
 #define RB_SOLVE_ARGS                                                                         \
-    d.tile_first, d.tile_cnt, d.solve_order, d.tile_color_off, d.tile_ncolors, d.cs, d.linvel, \
-        d.angvel, d.pos, d.inv_mass, d.inv_inertia_world, d.residual, d.counters, d.tile_stats
+    d.tile_first, d.tile_cnt, d.tile_color_off, d.tile_ncolors, d.tile_bodies, d.tile_nb, d.pk, \
+        d.linvel, d.angvel, d.residual, d.counters, d.tile_stats
     if (p.tile_mode == 1) {
         // independent scenes: every sweep of a scene in one workgroup, one
         // launch; with a tolerance each scene stops on its own residual
@@ -965,7 +1042,8 @@ hipError_t launch_solve(const DeviceWorld &d, const StepParams &p, hipStream_t s
     }
 #undef RB_SOLVE_ARGS
     hipLaunchKernelGGL(k_save_prev, dim3((p.max_contacts + 255) / 256), dim3(256), 0, s,
-                       d.counters, d.cs, d.prev_keys, d.prev_imp, d.prev_pts, d.prev_count);
+                       d.counters, d.cs, d.solve_pos, d.pk.lam, d.pk.fric, d.prev_keys, d.prev_imp,
+                       d.prev_pts, d.prev_count);
     return hipGetLastError();
 }
 

@@ -72,6 +72,15 @@ struct ConstraintSoA {
 };
 
 // Device state of one world.
+// A tile's rows in solve order, packed for the solve launches (position i).
+struct PackedRows {
+    float4 *lin0, *p /* 4 per row */, *cen, *jac, *b, *lam, *fric;
+    float4 *IA, *IB;  // 3 rows each: world inverse inertia of body A, B
+    float4 *pA, *pB;
+    int4 *meta;       // body A, body B, their LDS slots (-1 static, -2 global)
+    float2 *im;       // inverse masses
+};
+
 struct DeviceWorld {
     // shapes
     HullInfo *hulls;
@@ -111,6 +120,10 @@ struct DeviceWorld {
     int *tile_color_off;         // per tile RB_MAX_LOCAL_COLORS+1 offsets (relative)
     int *tile_ncolors;
     int4 *tile_stats;            // per tile: normal / friction sweeps used, final residuals (bits)
+    PackedRows pk;               // rows in solve order (built per step after the colouring)
+    int *solve_pos;              // per constraint: its position in the packed rows, -1 if unsolved
+    int *tile_bodies;            // per tile at 2 * first: its dynamic bodies (dense slots)
+    int *tile_nb;                // per tile: number of those bodies
     // narrowphase pass 1 -> pass 2
     uint32_t *narrow_cand;       // per pair: 1 if not separated by a face axis
     uint32_t *cand_off;          // its exclusive scan

@@ -302,6 +302,22 @@ mi_rb_status derive_and_upload(World &w) {
     RB_CHECK(w.alloc(&d.tile_color_off, (size_t)p.n_tiles * (RB_MAX_LOCAL_COLORS + 1)));
     RB_CHECK(w.alloc(&d.tile_ncolors, p.n_tiles));
     RB_CHECK(w.alloc(&d.tile_stats, p.n_tiles));
+    RB_CHECK(w.alloc(&d.pk.lin0, maxC));
+    RB_CHECK(w.alloc(&d.pk.p, 4 * (size_t)maxC));
+    RB_CHECK(w.alloc(&d.pk.cen, maxC));
+    RB_CHECK(w.alloc(&d.pk.jac, maxC));
+    RB_CHECK(w.alloc(&d.pk.b, maxC));
+    RB_CHECK(w.alloc(&d.pk.lam, maxC));
+    RB_CHECK(w.alloc(&d.pk.fric, maxC));
+    RB_CHECK(w.alloc(&d.pk.IA, 3 * (size_t)maxC));
+    RB_CHECK(w.alloc(&d.pk.IB, 3 * (size_t)maxC));
+    RB_CHECK(w.alloc(&d.pk.pA, maxC));
+    RB_CHECK(w.alloc(&d.pk.pB, maxC));
+    RB_CHECK(w.alloc(&d.pk.meta, maxC));
+    RB_CHECK(w.alloc(&d.pk.im, maxC));
+    RB_CHECK(w.alloc(&d.solve_pos, maxC));
+    RB_CHECK(w.alloc(&d.tile_bodies, 2 * (size_t)maxC));
+    RB_CHECK(w.alloc(&d.tile_nb, p.n_tiles));
     RB_CHECK(w.alloc(&d.narrow_cand, maxP));
     RB_CHECK(w.alloc(&d.cand_off, maxP));
     RB_CHECK(w.alloc(&d.face_best, maxP));
