@@ -229,6 +229,7 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_clip(
     __shared__ float4 sEA[RB_MAX_HULL_EDGES], sEB[RB_MAX_HULL_EDGES];
     __shared__ float4 sPoly[2][RB_MAX_POLY];
     __shared__ float4 sPts[RB_MAX_POLY];
+    __shared__ int sQ[2 * NP_BLOCK];  // edge axes that survived the bound test
 
     const int lane = threadIdx.x;
     const int n_cand = (int)*n_cand_ptr;
@@ -276,21 +277,56 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_clip(
         const float margin = 1e-3f + 4e-6f * ((fabsf(pA.x) + fabsf(pA.y) + fabsf(pA.z)) +
                                               (fabsf(pB.x) + fabsf(pB.y) + fabsf(pB.z)));
         const float lb_base = (hA.center.w + hB.center.w) - margin;
-        for (int k = lane; k < ne; k += NP_BLOCK) {
-            int e0 = k / hB.num_edges, e1 = k - e0 * hB.num_edges;
+        // The bound test runs on all lanes, one axis each; the surviving
+        // axes are compacted into an LDS queue and projected 64 at a time, so
+        // the projection loops run with full waves instead of with the few
+        // lanes whose axis survived.  Lanes see axes out of index order, so
+        // the lane-local minimum breaks ties by the axis index.
+        const int nEB = hB.num_edges;
+        auto axis_of = [&](int k) {
+            int e0 = k / nEB, e1 = k - e0 * nEB;
             float4 c = cross3(sEA[e0], sEB[e1]);
-            if (fabsf(c.x) > 1e-6f || fabsf(c.y) > 1e-6f || fabsf(c.z) > 1e-6f) {
+            bool ok = fabsf(c.x) > 1e-6f || fabsf(c.y) > 1e-6f || fabsf(c.z) > 1e-6f;
+            if (ok) {
                 c = normalize3(c);
                 if (dot3(dc, c) < 0) c = scl4(c, -1.f);
-                if (lb_base - dot3(dc, c) > face_d) continue;
-                float mn0, mx0, mn1, mx1;
-                project(sVA, nvA, pA, oA, c, mn0, mx0);
-                project(sVB, nvB, pB, oB, c, mn1, mx1);
-                float d0 = mx0 - mn1, d1 = mx1 - mn0;
-                float dist = d0 < d1 ? d0 : d1;
-                if (dist < best) { best = dist; best_ax = nf + k; }
+            }
+            return ok ? c : mk4(0.f, 0.f, 0.f, -1.f);  // w = -1: degenerate axis
+        };
+        auto project_axis = [&](int k) {
+            float4 c = axis_of(k);
+            float mn0, mx0, mn1, mx1;
+            project(sVA, nvA, pA, oA, c, mn0, mx0);
+            project(sVB, nvB, pB, oB, c, mn1, mx1);
+            float d0 = mx0 - mn1, d1 = mx1 - mn0;
+            float dist = d0 < d1 ? d0 : d1;
+            int ax = nf + k;
+            if (dist < best || (dist == best && ax < best_ax)) { best = dist; best_ax = ax; }
+        };
+        int qn = 0;
+        for (int base = 0; base < ne; base += NP_BLOCK) {
+            const int k = base + lane;
+            bool keep = false;
+            if (k < ne) {
+                float4 c = axis_of(k);
+                keep = c.w == 0.f && !(lb_base - dot3(dc, c) > face_d);
+            }
+            const uint64_t m = __ballot(keep);
+            if (keep) sQ[qn + __popcll(m & lanemask_lt())] = k;
+            qn += __popcll(m);
+            __syncthreads();
+            if (qn >= NP_BLOCK) {
+                project_axis(sQ[lane]);
+                int rest = qn - NP_BLOCK;
+                int moved = lane < rest ? sQ[NP_BLOCK + lane] : 0;
+                __syncthreads();
+                if (lane < rest) sQ[lane] = moved;
+                qn = rest;
+                __syncthreads();
             }
         }
+        if (lane < qn) project_axis(sQ[lane]);
+        __syncthreads();
         wave_argmin(best, best_ax);
         if (face_d < best || (face_d == best && fb0.y < best_ax)) { best = face_d; best_ax = fb0.y; }
         if (best_ax == 0x7fffffff) {
