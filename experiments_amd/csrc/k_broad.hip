@@ -57,8 +57,10 @@ __global__ __launch_bounds__(256) void k_setup(
     const HullInfo *__restrict__ hulls, const float4 *__restrict__ inv_inertia_local,
     float4 *__restrict__ inv_inertia_world, float4 *__restrict__ aabb_min,
     float4 *__restrict__ aabb_max, uint2 *__restrict__ kv, GridP g, float gx, float gy, float gz,
-    float dt) {
+    float dt, uint32_t *__restrict__ occ, int occ_words) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
+    // the cell occupancy bits of the previous step
+    for (int w = i; w < occ_words; w += gridDim.x * blockDim.x) occ[w] = 0u;
     if (i >= n) return;
     float4 p = pos[i], q = quat[i];
     const HullInfo *h = hulls + shape[i];
@@ -119,78 +121,134 @@ __global__ __launch_bounds__(256) void k_setup(
     }
 }
 
-__global__ void k_cell_start(const uint2 *__restrict__ kv, int n, uint32_t *cell_start) {
+// First sorted index and occupancy bit of every occupied cell, and the AABBs
+// in sorted order (the pair search then reads a cell member's key and box at
+// one index).  occ[] was cleared by k_setup.
+__global__ void k_cell_start(const uint2 *__restrict__ kv, int n,
+                             const float4 *__restrict__ aabb_min,
+                             const float4 *__restrict__ aabb_max, uint32_t *__restrict__ cell_start,
+                             uint32_t *__restrict__ occ, float4 *__restrict__ smn,
+                             float4 *__restrict__ smx) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
-    uint32_t key = kv[s].x;
-    if (key == KEY_NONE) return;
-    if (s == 0 || kv[s - 1].x != key) cell_start[key] = (uint32_t)s;
+    uint2 e = kv[s];
+    if (e.x == KEY_NONE) return;
+    smn[s] = aabb_min[e.y];
+    smx[s] = aabb_max[e.y];
+    if (s == 0 || kv[s - 1].x != e.x) {
+        cell_start[e.x] = (uint32_t)s;
+        atomicOr(&occ[e.x >> 5], 1u << (e.x & 31u));
+    }
 }
 
 constexpr int FP_BLOCK = 256;
 
-// One thread per sorted entry.  lists: per-thread sorted list in LDS,
-// stride K+1 words to spread banks.
+// LANES lanes per sorted entry; lane q walks the stencil cells
+// [q*C, q*C + C), C = ceil(27 / LANES), cells numbered z, then y, then x
+// from outermost to innermost, so a body's cells are searched in parallel
+// (32 lanes for few dense bodies, 8 for many).  The overlaps are numbered in
+// the order a serial search visits them (cells in stencil order, members in
+// sorted order, then the scene's large bodies in index order): a count pass,
+// a prefix over the lanes, and a store pass that keeps the first K in LDS.
+// The kept indices are distinct (a body sits in one cell), so the list is
+// ordered by rank.
+template <int LANES>
 __global__ __launch_bounds__(FP_BLOCK) void k_find_pairs(
     int n, const uint2 *__restrict__ kv, const uint32_t *__restrict__ cell_start,
-    const float4 *__restrict__ aabb_min, const float4 *__restrict__ aabb_max,
-    const int *__restrict__ large_list, const int *__restrict__ large_first,
+    const uint32_t *__restrict__ occ, const float4 *__restrict__ smn,
+    const float4 *__restrict__ smx, const float4 *__restrict__ aabb_min,
+    const float4 *__restrict__ aabb_max, const int *__restrict__ large_list,
+    const int *__restrict__ large_first,
     const int *__restrict__ large_num, GridP g, int K, int *__restrict__ body_slots,
     uint32_t *__restrict__ pair_count, int *overflow) {
+    constexpr int ENTRIES = FP_BLOCK / LANES;
+    constexpr int C = (27 + LANES - 1) / LANES;
     extern __shared__ int s_lists[];
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & (LANES - 1);
+    const int slot = threadIdx.x / LANES;
+    const int seg = slot & (64 / LANES - 1);  // which LANES-lane segment of the wave
+    const int t = blockIdx.x * ENTRIES + slot;
+    // every exit below is uniform over the entry's lanes
     if (t >= n) return;
-    uint2 e = kv[t];
-    if (e.x == KEY_NONE) return;  // large body: handled by k_large_pairs
-    int *lst = s_lists + threadIdx.x * (K + 1);
-    int i = (int)e.y;
-    int sc = (int)(e.x >> g.cell_bits);
-    float4 mn0 = aabb_min[i], mx0 = aabb_max[i];
+    const uint2 e = kv[t];
+    if (e.x == KEY_NONE) return;  // large body: k_large_count / k_large_write
+    int *lst = s_lists + slot * K;
+    const int i = (int)e.y;
+    const int sc = (int)(e.x >> g.cell_bits);
+    const float4 mn0 = smn[t], mx0 = smx[t];
     int gp[3];
     grid_pos(mn0, mx0, g.inv_cell, g.dims, gp);
-    int c = 0;
-    for (int dz = -1; dz <= 1; dz++)
-        for (int dy = -1; dy <= 1; dy++)
-            for (int dx = -1; dx <= 1; dx++) {
-                uint32_t nk = pos_hash(gp[0] + dx, gp[1] + dy, gp[2] + dz, g.dims) |
-                              ((uint32_t)sc << g.cell_bits);
-                uint32_t s = cell_start[nk];
-                if (s >= (uint32_t)n || kv[s].x != nk || (s > 0 && kv[s - 1].x == nk)) continue;
-                for (uint32_t u = s; u < (uint32_t)n; u++) {
-                    uint2 f = kv[u];
-                    if (f.x != nk) break;
-                    int j = (int)f.y;
-                    if (j >= i) continue;
-                    if (!aabb_overlap(mn0, mx0, aabb_min[j], aabb_max[j])) continue;
-                    if (c < K) {
-                        int k = c;
-                        while (k > 0 && lst[k - 1] > j) { lst[k] = lst[k - 1]; k--; }
-                        lst[k] = j;
-                    }
-                    c++;
-                }
-            }
-    if (large_num) {
-        int l0 = large_first[sc], nl = large_num[sc];
-        for (int q = 0; q < nl; q++) {
-            int L = large_list[l0 + q];
-            if (L >= i) break;  // list sorted by index
-            if (!aabb_overlap(mn0, mx0, aabb_min[L], aabb_max[L])) continue;
-            if (c < K) {
-                int k = c;
-                while (k > 0 && lst[k - 1] > L) { lst[k] = lst[k - 1]; k--; }
-                lst[k] = L;
-            }
-            c++;
+    const uint32_t sk = (uint32_t)sc << g.cell_bits;
+    uint32_t nk[C], u0[C];
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+        const int q = lane * C + k;
+        nk[k] = 0;
+        u0[k] = (uint32_t)n;
+        if (q < 27) {
+            const int dx = q % 3 - 1, dy = (q / 3) % 3 - 1, dz = q / 9 - 1;
+            nk[k] = pos_hash(gp[0] + dx, gp[1] + dy, gp[2] + dz, g.dims) | sk;
+            if ((occ[nk[k] >> 5] >> (nk[k] & 31u)) & 1u) u0[k] = cell_start[nk[k]];
         }
     }
-    if (c > K) {
-        atomicOr(overflow, (int)MI_RB_OVERFLOW_BODY_PAIRS);
-        c = K;
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int k = 0; k < C; k++)
+        for (uint32_t u = u0[k]; u < (uint32_t)n; u++) {
+            const uint2 f = kv[u];
+            if (f.x != nk[k]) break;
+            if ((int)f.y < i && aabb_overlap(mn0, mx0, smn[u], smx[u])) cnt++;
+        }
+    uint32_t incl = cnt;
+#pragma unroll
+    for (int o = 1; o < LANES; o <<= 1) {
+        uint32_t v = __shfl_up(incl, o, LANES);
+        if (lane >= o) incl += v;
     }
+    uint32_t c = __shfl(incl, LANES - 1, LANES);
+    uint32_t v = incl - cnt;
+#pragma unroll
+    for (int k = 0; k < C; k++)
+        for (uint32_t u = u0[k]; u < (uint32_t)n && v < (uint32_t)K; u++) {
+            const uint2 f = kv[u];
+            if (f.x != nk[k]) break;
+            if ((int)f.y < i && aabb_overlap(mn0, mx0, smn[u], smx[u])) lst[v++] = (int)f.y;
+        }
+    if (large_num) {
+        const int l0 = large_first[sc], nl = large_num[sc];
+        const uint64_t segmask = LANES == 64 ? ~0ull : ((1ull << LANES) - 1ull);
+        for (int q0 = 0; q0 < nl; q0 += LANES) {
+            const int q = q0 + lane;
+            bool hit = false, stop = false;
+            int L = 0;
+            if (q < nl) {
+                L = large_list[l0 + q];
+                stop = L >= i;  // list sorted by index
+                hit = !stop && aabb_overlap(mn0, mx0, aabb_min[L], aabb_max[L]);
+            }
+            const uint64_t hm = (__ballot(hit) >> (LANES * seg)) & segmask;
+            const uint64_t sm = (__ballot(stop) >> (LANES * seg)) & segmask;
+            if (hit) {
+                uint32_t w = c + (uint32_t)__popcll(hm & ((1ull << lane) - 1ull));
+                if (w < (uint32_t)K) lst[w] = L;
+            }
+            c += (uint32_t)__popcll(hm);
+            if (sm) break;
+        }
+    }
+    if (c > (uint32_t)K) {
+        if (lane == 0) atomicOr(overflow, (int)MI_RB_OVERFLOW_BODY_PAIRS);
+        c = (uint32_t)K;
+    }
+    wave_sync();  // the list's LDS stores come before the other lanes' loads
     int *dst = body_slots + (size_t)i * K;
-    for (int k = 0; k < c; k++) dst[k] = lst[k];
-    pair_count[i] = (uint32_t)c;
+    for (uint32_t w = lane; w < c; w += LANES) {
+        const int j = lst[w];
+        uint32_t r = 0;
+        for (uint32_t k = 0; k < c; k++) r += lst[k] < j ? 1u : 0u;
+        dst[r] = j;
+    }
+    if (lane == 0) pair_count[i] = c;
 }
 
 // Overflow list: pairs (x, j) of each large body x with every body j < x of
@@ -344,9 +402,14 @@ __device__ __forceinline__ void hg_cell(float4 mn, float4 mx, float inv_l, const
 
 __global__ __launch_bounds__(256) void k_hg_hash(const float4 *__restrict__ mn,
                                                  const float4 *__restrict__ mx, int n, HGridP g,
-                                                 uint2 *__restrict__ kv, int *overflow) {
+                                                 uint2 *__restrict__ kv, uint32_t *__restrict__ counts,
+                                                 uint32_t *__restrict__ cursor,
+                                                 uint32_t *__restrict__ big_n, int *overflow) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *big_n = 0u;
     if (i >= n) return;
+    counts[i] = 0u;
+    cursor[i] = 0u;
     float4 a = mn[i], b = mx[i];
     int l = hg_level(a, b, g.inv_cell0, g.levels);
     uint32_t key = KEY_NONE;
@@ -360,75 +423,179 @@ __global__ __launch_bounds__(256) void k_hg_hash(const float4 *__restrict__ mn,
     kv[i] = make_uint2(key, (uint32_t)i);
 }
 
-// One thread per sorted entry.  WRITE = 0: count the entry's pairs;
-// WRITE = 1: write them as (max, min) at the scanned offset.
+// Cell starts of the sorted keys, the occupancy bit of every occupied cell,
+// a flag per occupied level, and the AABBs in sorted order (so that the pair
+// search reads a cell member's key and box at one index).  occ[] and the
+// level flags after it are cleared before this kernel.
+__global__ __launch_bounds__(256) void k_hg_cells(const uint2 *__restrict__ kv, int n,
+                                                  const float4 *__restrict__ aabb_min,
+                                                  const float4 *__restrict__ aabb_max, HGridP g,
+                                                  uint32_t *__restrict__ cell_start,
+                                                  uint32_t *__restrict__ occ,
+                                                  uint32_t *__restrict__ level_used,
+                                                  float4 *__restrict__ smn,
+                                                  float4 *__restrict__ smx) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    uint2 e = kv[s];
+    smn[s] = aabb_min[e.y];
+    smx[s] = aabb_max[e.y];
+    if (e.x == KEY_NONE) return;
+    if (s == 0 || kv[s - 1].x != e.x) {
+        cell_start[e.x] = (uint32_t)s;
+        atomicOr(&occ[e.x >> 5], 1u << (e.x & 31u));
+        level_used[e.x >> g.cell_bits] = 1u;
+    }
+}
+
+// HG_LANES lanes per sorted entry, lane q taking stencil cells q, q +
+// HG_LANES, ...  A pair (x, y), x > y, belongs to body x.  WRITE = 0: count
+// every found pair for its body; WRITE = 1: store y in the body's range of
+// the scanned counts (slot order within a range is that of the atomics;
+// k_pairs_group orders it).  The entry visits the 27 cells around it on its
+// own level and on every coarser level that holds bodies, and walks only the
+// occupied ones.
+constexpr int HG_LANES = 8;
 template <int WRITE>
 __global__ __launch_bounds__(256) void k_hg_pairs(int n, const uint2 *__restrict__ kv,
                                                   const uint32_t *__restrict__ cell_start,
-                                                  const float4 *__restrict__ aabb_min,
-                                                  const float4 *__restrict__ aabb_max, HGridP g,
+                                                  const uint32_t *__restrict__ occ,
+                                                  const uint32_t *__restrict__ level_used,
+                                                  const float4 *__restrict__ smn,
+                                                  const float4 *__restrict__ smx, HGridP g,
                                                   uint32_t *__restrict__ counts,
                                                   const uint32_t *__restrict__ offsets,
-                                                  int max_pairs, uint2 *__restrict__ out) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+                                                  uint32_t *__restrict__ cursor, int max_pairs,
+                                                  uint32_t *__restrict__ ys) {
+    constexpr int C = (27 + HG_LANES - 1) / HG_LANES;
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = gid / HG_LANES, lane = gid % HG_LANES;
     if (t >= n) return;
     uint2 e = kv[t];
-    if (e.x == KEY_NONE) {
-        if (!WRITE) counts[t] = 0;
-        return;
-    }
+    if (e.x == KEY_NONE) return;
     const int x = (int)e.y;
     const int lx = (int)(e.x >> g.cell_bits);
-    const float4 mn0 = aabb_min[x], mx0 = aabb_max[x];
-    uint32_t c = 0, o = WRITE ? offsets[t] : 0;
+    const float4 mn0 = smn[t], mx0 = smx[t];
     for (int l = lx; l < g.levels; l++) {
+        if (l != lx && !level_used[l]) continue;
         int gp[3];
         hg_cell(mn0, mx0, g.inv_cell0 * exp2f(-(float)l), g.dims, gp);
-        for (int dz = -1; dz <= 1; dz++)
-            for (int dy = -1; dy <= 1; dy++)
-                for (int dx = -1; dx <= 1; dx++) {
-                    uint32_t nk = ((uint32_t)l << g.cell_bits) |
-                                  pos_hash(gp[0] + dx, gp[1] + dy, gp[2] + dz, g.dims);
-                    uint32_t s = cell_start[nk];
-                    if (s >= (uint32_t)n || kv[s].x != nk || (s > 0 && kv[s - 1].x == nk)) continue;
-                    for (uint32_t u = s; u < (uint32_t)n; u++) {
-                        uint2 f = kv[u];
-                        if (f.x != nk) break;
-                        int y = (int)f.y;
-                        if (l == lx && y >= x) continue;
-                        if (!aabb_overlap(mn0, mx0, aabb_min[y], aabb_max[y])) continue;
-                        if (WRITE && o + c < (uint32_t)max_pairs)
-                            out[o + c] = make_uint2((uint32_t)(x > y ? x : y), (uint32_t)(x > y ? y : x));
-                        c++;
-                    }
+        const uint32_t lk = (uint32_t)l << g.cell_bits;
+        uint32_t nk[C], u0[C];
+#pragma unroll
+        for (int k = 0; k < C; k++) {
+            const int q = lane + k * HG_LANES;
+            nk[k] = 0;
+            u0[k] = (uint32_t)n;
+            if (q < 27) {
+                const int dx = q % 3 - 1, dy = (q / 3) % 3 - 1, dz = q / 9 - 1;
+                nk[k] = lk | pos_hash(gp[0] + dx, gp[1] + dy, gp[2] + dz, g.dims);
+                if ((occ[nk[k] >> 5] >> (nk[k] & 31u)) & 1u) u0[k] = cell_start[nk[k]];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < C; k++)
+            for (uint32_t u = u0[k]; u < (uint32_t)n; u++) {
+                uint2 f = kv[u];
+                if (f.x != nk[k]) break;
+                int y = (int)f.y;
+                if (l == lx && y >= x) continue;
+                if (!aabb_overlap(mn0, mx0, smn[u], smx[u])) continue;
+                const uint32_t hi = (uint32_t)(x > y ? x : y), lo = (uint32_t)(x > y ? y : x);
+                if (!WRITE) {
+                    atomicAdd(&counts[hi], 1u);
+                } else {
+                    uint32_t pos = offsets[hi] + atomicAdd(&cursor[hi], 1u);
+                    if (pos < (uint32_t)max_pairs) ys[pos] = lo;
                 }
+            }
     }
-    if (!WRITE) counts[t] = c;
 }
 
-// pair list <-> sort records: (minor, major) for the first sort, then
-// (major, minor) for the second, then int2 (x, y)
-__global__ void k_pairs_to_kv(const uint32_t *__restrict__ total, int max_pairs,
-                              uint2 *__restrict__ kv, int swap) {
-    uint32_t np = *total < (uint32_t)max_pairs ? *total : (uint32_t)max_pairs;
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= np) return;
-    uint2 v = kv[i];
-    if (swap) kv[i] = make_uint2(v.y, v.x);
+// The pair range of body x: [offsets[x], offsets[x] + counts[x]) clipped to
+// the capacity.
+__device__ __forceinline__ uint32_t pair_range(const uint32_t *counts, const uint32_t *offsets,
+                                               int max_pairs, int x, uint32_t &off) {
+    off = offsets[x];
+    uint32_t len = counts[x];
+    if (off >= (uint32_t)max_pairs) return 0u;
+    if (off + len > (uint32_t)max_pairs) len = (uint32_t)max_pairs - off;
+    return len;
 }
-__global__ void k_pairs_out(const uint32_t *__restrict__ total, int max_pairs,
-                            const uint2 *__restrict__ kv, int2 *__restrict__ pairs,
-                            int *num_pairs, int *overflow) {
-    uint32_t tt = *total;
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) {
+
+constexpr int GROUP_SMALL = 8;
+
+// One thread per body: its range sorted by y (an odd-even transposition
+// network over GROUP_SMALL registers) and written as int2 (x, y).  Longer
+// ranges go to a list for k_pairs_group_big.  Also the pair count and the
+// capacity flag.
+__global__ __launch_bounds__(256) void k_pairs_group(int n, const uint32_t *__restrict__ counts,
+                                                     const uint32_t *__restrict__ offsets,
+                                                     const uint32_t *__restrict__ total,
+                                                     int max_pairs, const uint32_t *__restrict__ ys,
+                                                     int2 *__restrict__ pairs, int *num_pairs,
+                                                     int *overflow, uint32_t *__restrict__ big,
+                                                     uint32_t *__restrict__ big_n) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x == 0) {
+        uint32_t tt = *total;
         if (tt > (uint32_t)max_pairs) atomicOr(overflow, (int)MI_RB_OVERFLOW_PAIRS);
         *num_pairs = (int)(tt < (uint32_t)max_pairs ? tt : (uint32_t)max_pairs);
     }
-    uint32_t np = tt < (uint32_t)max_pairs ? tt : (uint32_t)max_pairs;
-    if (i >= np) return;
-    uint2 v = kv[i];
-    pairs[i] = make_int2((int)v.x, (int)v.y);
+    if (x >= n) return;
+    uint32_t off;
+    const uint32_t len = pair_range(counts, offsets, max_pairs, x, off);
+    if (len == 0) return;
+    if (len > (uint32_t)GROUP_SMALL) {
+        big[atomicAdd(big_n, 1u)] = (uint32_t)x;
+        return;
+    }
+    uint32_t v[GROUP_SMALL];
+#pragma unroll
+    for (int k = 0; k < GROUP_SMALL; k++) v[k] = (uint32_t)k < len ? ys[off + k] : 0xffffffffu;
+#pragma unroll
+    for (int r = 0; r < GROUP_SMALL; r++)
+#pragma unroll
+        for (int k = r & 1; k + 1 < GROUP_SMALL; k += 2) {
+            uint32_t a = v[k], b = v[k + 1];
+            v[k] = a < b ? a : b;
+            v[k + 1] = a < b ? b : a;
+        }
+#pragma unroll
+    for (int k = 0; k < GROUP_SMALL; k++)
+        if ((uint32_t)k < len) pairs[off + k] = make_int2(x, (int)v[k]);
+}
+
+// Bodies with more than GROUP_SMALL pairs, one workgroup each: the rank of
+// every y among its range (the y of one body are distinct) in LDS chunks.
+constexpr int GROUP_CHUNK = 1024;
+__global__ __launch_bounds__(256) void k_pairs_group_big(const uint32_t *__restrict__ counts,
+                                                         const uint32_t *__restrict__ offsets,
+                                                         int max_pairs,
+                                                         const uint32_t *__restrict__ ys,
+                                                         int2 *__restrict__ pairs,
+                                                         const uint32_t *__restrict__ big,
+                                                         const uint32_t *__restrict__ big_n) {
+    __shared__ uint32_t sy[GROUP_CHUNK];
+    const uint32_t nb = *big_n;
+    for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
+        const int x = (int)big[b];
+        uint32_t off;
+        const uint32_t len = pair_range(counts, offsets, max_pairs, x, off);
+        for (uint32_t i0 = 0; i0 < len; i0 += 256) {
+            const uint32_t i = i0 + threadIdx.x;
+            const uint32_t yi = i < len ? ys[off + i] : 0u;
+            uint32_t rank = 0;
+            for (uint32_t j0 = 0; j0 < len; j0 += GROUP_CHUNK) {
+                const uint32_t m = len - j0 < (uint32_t)GROUP_CHUNK ? len - j0 : GROUP_CHUNK;
+                __syncthreads();
+                for (uint32_t j = threadIdx.x; j < m; j += 256) sy[j] = ys[off + j0 + j];
+                __syncthreads();
+                for (uint32_t j = 0; j < m; j++) rank += sy[j] < yi ? 1u : 0u;
+            }
+            if (i < len) pairs[off + rank] = make_int2(x, (int)yi);
+        }
+    }
 }
 }  // namespace
 
@@ -448,17 +615,16 @@ hipError_t launch_setup(const DeviceWorld &d, const StepParams &p, hipStream_t s
     hipLaunchKernelGGL(k_setup, dim3((p.n_bodies + 255) / 256), dim3(256), 0, s, p.n_bodies,
                        d.pos, d.quat, d.linvel, d.angvel, d.inv_mass, d.shape, d.scene, d.is_large,
                        d.hulls, d.inv_inertia_local, d.inv_inertia_world, d.aabb_min, d.aabb_max,
-                       d.kv, grid_of(p), p.gravity[0], p.gravity[1], p.gravity[2], p.dt);
+                       d.kv, grid_of(p), p.gravity[0], p.gravity[1], p.gravity[2], p.dt,
+                       d.occ, d.occ_words);
     return hipGetLastError();
 }
 
 hipError_t launch_cell_start(const DeviceWorld &d, const StepParams &p, hipStream_t s) {
-    return launch_cell_start_generic(d.kv, p.n_bodies, d.cell_start, s);
-}
-
-hipError_t launch_cell_start_generic(const uint2 *kv, int n, uint32_t *cell_start, hipStream_t s) {
+    const int n = p.n_bodies;
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_cell_start, dim3((n + 255) / 256), dim3(256), 0, s, kv, n, cell_start);
+    hipLaunchKernelGGL(k_cell_start, dim3((n + 255) / 256), dim3(256), 0, s, d.kv, n, d.aabb_min,
+                       d.aabb_max, d.cell_start, d.occ, d.smn, d.smx);
     return hipGetLastError();
 }
 
@@ -466,10 +632,14 @@ hipError_t launch_find_pairs(const DeviceWorld &d, const StepParams &p, hipStrea
     int n = p.n_bodies;
     if (n == 0) return hipSuccess;
     int K = p.max_pairs_per_body;
-    int blk = K > 32 ? FP_BLOCK / 2 : FP_BLOCK;  // keep the per-thread lists under 64 KB of LDS
-    size_t lds = sizeof(int) * (size_t)blk * (K + 1);
-    hipLaunchKernelGGL(k_find_pairs, dim3((n + blk - 1) / blk), dim3(blk), lds, s, n,
-                       d.kv, d.cell_start, d.aabb_min, d.aabb_max, d.large_list,
+    // 32 lanes per body while the bodies alone do not fill the chip, else 8
+    const int lanes = n < (1 << 18) ? 32 : 8;
+    const int entries = FP_BLOCK / lanes;
+    size_t lds = sizeof(int) * (size_t)entries * K;
+    auto kern = lanes == 32 ? k_find_pairs<32> : k_find_pairs<8>;
+    hipLaunchKernelGGL(kern, dim3((n + entries - 1) / entries), dim3(FP_BLOCK), lds, s, n,
+                       d.kv, d.cell_start, d.occ, d.smn, d.smx, d.aabb_min, d.aabb_max,
+                       d.large_list,
                        p.n_large ? d.large_first : nullptr, p.n_large ? d.large_num : nullptr,
                        grid_of(p), K, d.body_slots, d.pair_count, d.counters + RB_CNT_OVERFLOW);
     if (p.n_large > 0) {
@@ -498,12 +668,13 @@ hipError_t launch_emit_pairs(const DeviceWorld &d, const StepParams &p, hipStrea
     return hipGetLastError();
 }
 
-// Hierarchical-grid pair search of n AABBs: hash, sort, cell starts, count,
-// scan, write, then the (x, y) order by two stable sorts (y, then x).
+// Hierarchical-grid pair search of n AABBs: hash, sort, cell starts, count
+// per body, scan, write, then each body's pairs ordered by y.
 hipError_t launch_hgrid_pairs(const float4 *mn, const float4 *mx, int n, float inv_cell0,
                               const int dims[3], int cell_bits, int levels, uint2 *kv,
-                              uint32_t *cell_start, uint32_t *counts, uint32_t *offsets,
-                              uint32_t *total, uint2 *pair_kv, int max_pairs, int pair_bits,
+                              uint32_t *cell_start, uint32_t *occ, float4 *smn, float4 *smx,
+                              uint32_t *counts, uint32_t *offsets, uint32_t *cursor,
+                              uint32_t *big, uint32_t *total, uint32_t *ys, int max_pairs,
                               int2 *pairs, int *num_pairs, int *overflow, const SortWork &sw,
                               uint32_t *scan_status, hipStream_t s, hipEvent_t *ev) {
     HGridP g;
@@ -512,33 +683,38 @@ hipError_t launch_hgrid_pairs(const float4 *mn, const float4 *mx, int n, float i
     g.cell_bits = cell_bits;
     g.levels = levels;
     const int blocks = n > 0 ? (n + 255) / 256 : 1;
+    const int hg_blocks = (int)(((int64_t)n * HG_LANES + 255) / 256);
+    uint32_t *big_n = big + (n > 0 ? n : 0);
     if (ev) hipEventRecord(ev[0], s);
-    if (n > 0) hipLaunchKernelGGL(k_hg_hash, dim3(blocks), dim3(256), 0, s, mn, mx, n, g, kv, overflow);
+    hipLaunchKernelGGL(k_hg_hash, dim3(blocks), dim3(256), 0, s, mn, mx, n, g, kv, counts, cursor,
+                       big_n, overflow);
     if (ev) hipEventRecord(ev[1], s);
     hipError_t e = launch_radix_sort(kv, n, cell_bits + 2, 8, sw, s);
     if (e != hipSuccess) return e;
     if (ev) hipEventRecord(ev[2], s);
+    // occupancy words of every level's cells, then one flag per level
+    const size_t occ_words = ((size_t)1 << (cell_bits + 2)) / 32;
+    uint32_t *level_used = occ + occ_words;
     if (n > 0) {
-        hipLaunchKernelGGL(k_cell_start, dim3(blocks), dim3(256), 0, s, kv, n, cell_start);
-        hipLaunchKernelGGL(k_hg_pairs<0>, dim3(blocks), dim3(256), 0, s, n, kv, cell_start, mn, mx,
-                           g, counts, (const uint32_t *)nullptr, max_pairs, pair_kv);
+        e = launch_fill_u32(occ, 0u, occ_words + 4, s);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_hg_cells, dim3(blocks), dim3(256), 0, s, kv, n, mn, mx, g, cell_start,
+                           occ, level_used, smn, smx);
+        hipLaunchKernelGGL(k_hg_pairs<0>, dim3(hg_blocks), dim3(256), 0, s, n, kv, cell_start, occ,
+                           level_used, smn, smx, g, counts, (const uint32_t *)nullptr, cursor,
+                           max_pairs, ys);
     }
     e = launch_exclusive_scan(counts, offsets, n, total, scan_status, s);
     if (e != hipSuccess) return e;
     if (n > 0)
-        hipLaunchKernelGGL(k_hg_pairs<1>, dim3(blocks), dim3(256), 0, s, n, kv, cell_start, mn, mx,
-                           g, counts, offsets, max_pairs, pair_kv);
+        hipLaunchKernelGGL(k_hg_pairs<1>, dim3(hg_blocks), dim3(256), 0, s, n, kv, cell_start, occ,
+                           level_used, smn, smx, g, counts, offsets, cursor, max_pairs, ys);
     if (ev) hipEventRecord(ev[3], s);
-    // pair_kv holds (x, y): sort by y (swap to key y), then by x (swap back)
-    const int pblocks = (max_pairs + 255) / 256;
-    hipLaunchKernelGGL(k_pairs_to_kv, dim3(pblocks), dim3(256), 0, s, total, max_pairs, pair_kv, 1);
-    e = launch_radix_sort(pair_kv, max_pairs, pair_bits, 8, sw, s, (const int *)total);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_pairs_to_kv, dim3(pblocks), dim3(256), 0, s, total, max_pairs, pair_kv, 1);
-    e = launch_radix_sort(pair_kv, max_pairs, pair_bits, 8, sw, s, (const int *)total);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_pairs_out, dim3(pblocks), dim3(256), 0, s, total, max_pairs, pair_kv, pairs,
-                       num_pairs, overflow);
+    hipLaunchKernelGGL(k_pairs_group, dim3(blocks), dim3(256), 0, s, n, counts, offsets, total,
+                       max_pairs, ys, pairs, num_pairs, overflow, big, big_n);
+    if (n > 0)
+        hipLaunchKernelGGL(k_pairs_group_big, dim3(1024), dim3(256), 0, s, counts, offsets,
+                           max_pairs, ys, pairs, big, big_n);
     if (ev) hipEventRecord(ev[4], s);
     return hipGetLastError();
 }

@@ -55,10 +55,17 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_histogram(const uint2 *kv, int n
     for (int i = threadIdx.x; i < passes * RADIX; i += SORT_BLOCK) h[i] = 0;
     __syncthreads();
     int base = blockIdx.x * SORT_TILE;
+    uint32_t keys[SORT_ITEMS];
+#pragma unroll
+    for (int k = 0; k < SORT_ITEMS; k++) {
+        int i = base + k * SORT_BLOCK + threadIdx.x;
+        keys[k] = i < n ? kv[i].x : 0u;
+    }
+#pragma unroll
     for (int k = 0; k < SORT_ITEMS; k++) {
         int i = base + k * SORT_BLOCK + threadIdx.x;
         if (i < n) {
-            uint32_t key = kv[i].x;
+            uint32_t key = keys[k];
             for (int p = 0; p < passes; p++) {
                 int shift = p * RBITS;
                 int bits = sort_bits - shift < RBITS ? sort_bits - shift : RBITS;
@@ -100,10 +107,16 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_onesweep(const uint2 *__restrict
 
     uint2 item[SORT_ITEMS];
     uint32_t rank[SORT_ITEMS];
+    // every load of the tile is in flight before the first ranking step
+#pragma unroll
+    for (int r = 0; r < SORT_ITEMS; r++) {
+        int i = base + r * 64 + lane;
+        item[r] = i < n ? src[i] : make_uint2(0xffffffffu, 0xffffffffu);
+    }
+#pragma unroll
     for (int r = 0; r < SORT_ITEMS; r++) {
         int i = base + r * 64 + lane;
         bool valid = i < n;
-        item[r] = valid ? src[i] : make_uint2(0xffffffffu, 0xffffffffu);
         uint32_t d = (item[r].x >> shift) & dmask;
         uint64_t peers = __ballot(valid);
 #pragma unroll

@@ -103,6 +103,9 @@ struct DeviceWorld {
     float4 *aabb_min, *aabb_max;
     uint2 *kv, *kv_tmp;
     uint32_t *cell_start;
+    uint32_t *occ;               // occupancy bit per cell key, cleared by k_setup
+    int occ_words;
+    float4 *smn, *smx;           // AABBs in sorted key order
     int *body_slots;             // max_pairs_per_body per body
     int *large_slots;
     uint32_t *pair_count, *pair_off;
@@ -235,11 +238,11 @@ hipError_t launch_mlp_targets(const DeviceWorld &d, int capacity, float *x_out, 
                               hipStream_t s);
 
 // the stand-alone grid API
-hipError_t launch_cell_start_generic(const uint2 *kv, int n, uint32_t *cell_start, hipStream_t s);
 hipError_t launch_hgrid_pairs(const float4 *mn, const float4 *mx, int n, float inv_cell0,
                               const int dims[3], int cell_bits, int levels, uint2 *kv,
-                              uint32_t *cell_start, uint32_t *counts, uint32_t *offsets,
-                              uint32_t *total, uint2 *pair_kv, int max_pairs, int pair_bits,
+                              uint32_t *cell_start, uint32_t *occ, float4 *smn, float4 *smx,
+                              uint32_t *counts, uint32_t *offsets, uint32_t *cursor,
+                              uint32_t *big, uint32_t *total, uint32_t *ys, int max_pairs,
                               int2 *pairs, int *num_pairs, int *overflow, const SortWork &sw,
                               uint32_t *scan_status, hipStream_t s, hipEvent_t *ev);
 

@@ -277,6 +277,10 @@ mi_rb_status derive_and_upload(World &w) {
     RB_CHECK(w.alloc(&d.aabb_max, n));
     RB_CHECK(w.alloc(&d.kv, n));
     RB_CHECK(w.alloc(&d.cell_start, ncells));
+    d.occ_words = (int)(ncells / 32 + 1);
+    RB_CHECK(w.alloc(&d.occ, (size_t)d.occ_words));
+    RB_CHECK(w.alloc(&d.smn, n));
+    RB_CHECK(w.alloc(&d.smx, n));
     RB_CHECK(w.alloc(&d.body_slots, (size_t)n * K));
     RB_CHECK(w.alloc(&d.pair_count, n));
     RB_CHECK(w.alloc(&d.pair_off, n));
@@ -390,6 +394,7 @@ mi_rb_status derive_and_upload(World &w) {
     RB_CHECK(up(d.large_num, large_num.data(), sizeof(int) * n_scenes));
     RB_CHECK(up(d.large_index, large_index.data(), sizeof(int) * n));
     RB_CHECK(hipMemsetAsync(d.cell_start, 0xff, sizeof(uint32_t) * ncells, s));
+    RB_CHECK(hipMemsetAsync(d.occ, 0, sizeof(uint32_t) * d.occ_words, s));
     RB_CHECK(hipMemsetAsync(d.counters, 0, sizeof(int) * RB_CNT_COUNT, s));
     RB_CHECK(hipMemsetAsync(d.prev_count, 0, sizeof(int), s));
     RB_CHECK(hipMemsetAsync(d.inv_inertia_world, 0, sizeof(float4) * 3 * n, s));
@@ -1135,16 +1140,20 @@ namespace {
 struct Grid {
     mi_rb_grid_desc desc;
     int max_n;
-    uint2 *kv = nullptr, *pair_kv = nullptr;
+    uint2 *kv = nullptr;
+    uint32_t *ys = nullptr;              // per pair slot: the smaller index, grouped by the larger
+    uint32_t *cursor = nullptr, *big = nullptr;
     uint32_t *cell_start = nullptr, *counts = nullptr, *offsets = nullptr, *total = nullptr;
+    uint32_t *occ = nullptr;             // occupancy bit per cell, then one flag per level
+    float4 *smn = nullptr, *smx = nullptr;  // AABBs in sorted order
     int *overflow = nullptr;
     SortWork sw{};
     uint32_t *scan_status = nullptr;
-    int cell_bits = 0, pair_bits = 0;
+    int cell_bits = 0;
     hipEvent_t ev[5] = {};
     void release() {
-        void *ps[] = {kv, pair_kv, cell_start, counts, offsets, total, overflow, sw.tmp, sw.hist,
-                      sw.status, sw.tile_ctr, scan_status};
+        void *ps[] = {kv, ys, cursor, big, cell_start, counts, offsets, total, overflow, sw.tmp,
+                      sw.hist, sw.status, sw.tile_ctr, scan_status, occ, smn, smx};
         for (void *p : ps) if (p) hipFree(p);
         for (auto &e : ev) if (e) hipEventDestroy(e);
     }
@@ -1163,20 +1172,22 @@ extern "C" mi_rb_status mi_rb_grid_create(const mi_rb_grid_desc *desc, int max_a
     g->max_n = max_aabbs;
     g->cell_bits = ilog2((uint64_t)desc->dims[0] * desc->dims[1] * desc->dims[2]);
     if (g->cell_bits > 28) { delete g; return MI_RB_INVALID_ARG; }
-    g->pair_bits = std::max(1, ilog2(next_pow2(max_aabbs)));
     size_t ncells = (size_t)1 << (g->cell_bits + 2);
-    int n = max_aabbs, P = desc->max_pairs, cap = std::max(n, P);
-    bool ok = dalloc(&g->kv, n) == hipSuccess && dalloc(&g->pair_kv, P) == hipSuccess &&
-              dalloc(&g->cell_start, ncells) == hipSuccess && dalloc(&g->counts, n) == hipSuccess &&
+    int n = max_aabbs, P = desc->max_pairs;
+    bool ok = dalloc(&g->kv, n) == hipSuccess && dalloc(&g->ys, P) == hipSuccess &&
+              dalloc(&g->cursor, n) == hipSuccess && dalloc(&g->big, (size_t)n + 1) == hipSuccess &&
+              dalloc(&g->cell_start, ncells) == hipSuccess &&
+              dalloc(&g->occ, ncells / 32 + 4) == hipSuccess && dalloc(&g->smn, n) == hipSuccess &&
+              dalloc(&g->smx, n) == hipSuccess && dalloc(&g->counts, n) == hipSuccess &&
               dalloc(&g->offsets, n) == hipSuccess && dalloc(&g->total, 4) == hipSuccess &&
-              dalloc(&g->overflow, 4) == hipSuccess && dalloc(&g->sw.tmp, cap) == hipSuccess &&
+              dalloc(&g->overflow, 4) == hipSuccess && dalloc(&g->sw.tmp, n) == hipSuccess &&
               dalloc(&g->sw.hist, 8 * 256) == hipSuccess &&
-              dalloc(&g->sw.status, sort_status_words(cap, 8)) == hipSuccess &&
+              dalloc(&g->sw.status, sort_status_words(n, 8)) == hipSuccess &&
               dalloc(&g->sw.tile_ctr, 16) == hipSuccess &&
               dalloc(&g->scan_status, 4 + 2 * (size_t)((n + 4095) / 4096)) == hipSuccess;
     if (!ok) { g->release(); delete g; return MI_RB_DEVICE; }
-    g->sw.max_n = cap;
-    g->sw.max_tiles = (cap + 4095) / 4096;
+    g->sw.max_n = n;
+    g->sw.max_tiles = (n + 4095) / 4096;
     hipMemset(g->cell_start, 0xff, sizeof(uint32_t) * ncells);
     hipMemset(g->overflow, 0, 16);
     for (auto &e : g->ev) hipEventCreate(&e);
@@ -1207,8 +1218,9 @@ extern "C" mi_rb_status mi_rb_grid_find_pairs(void *grid, const void *d_aabb_min
     RB_CHECK(launch_hgrid_pairs(static_cast<const float4 *>(d_aabb_min),
                                 static_cast<const float4 *>(d_aabb_max), n,
                                 1.0f / g->desc.cell_size, g->desc.dims, g->cell_bits,
-                                g->desc.levels, g->kv, g->cell_start, g->counts, g->offsets,
-                                g->total, g->pair_kv, g->desc.max_pairs, g->pair_bits,
+                                g->desc.levels, g->kv, g->cell_start, g->occ, g->smn, g->smx,
+                                g->counts, g->offsets, g->cursor, g->big, g->total, g->ys,
+                                g->desc.max_pairs,
                                 static_cast<int2 *>(d_pairs), static_cast<int *>(d_num_pairs),
                                 g->overflow, g->sw, g->scan_status, s, g->ev));
     return MI_RB_OK;
