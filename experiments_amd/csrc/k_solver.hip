@@ -253,6 +253,7 @@ __global__ void k_tile_bounds2(const uint2 *__restrict__ tile_kv, const int *cou
 struct TileLDS {
     int key[HASH_SLOTS];                 // body index per hash slot, -1 empty
     unsigned long long mark[HASH_SLOTS]; // (round << 32) | priority
+    unsigned long long used[HASH_SLOTS]; // colours already taken by the body's rows
     short la[RB_TILE_MAX_ROWS], lb[RB_TILE_MAX_ROWS];  // local slots (-1 static)
     unsigned char color[RB_TILE_MAX_ROWS];
     int ccount[RB_MAX_LOCAL_COLORS + 1];
@@ -296,7 +297,11 @@ __global__ __launch_bounds__(TB) void k_tile_color(
         if (threadIdx.x == 0) atomicOr(counters + RB_CNT_OVERFLOW, (int)MI_RB_OVERFLOW_TILE);
         R = RB_TILE_MAX_ROWS;
     }
-    for (int i = threadIdx.x; i < HASH_SLOTS; i += TB) { L.key[i] = -1; L.mark[i] = 0ull; }
+    for (int i = threadIdx.x; i < HASH_SLOTS; i += TB) {
+        L.key[i] = -1;
+        L.mark[i] = 0ull;
+        L.used[i] = 0ull;
+    }
     for (int i = threadIdx.x; i <= RB_MAX_LOCAL_COLORS; i += TB) L.ccount[i] = 0;
     if (threadIdx.x == 0) L.remaining = 0;
     __syncthreads();
@@ -310,6 +315,10 @@ __global__ __launch_bounds__(TB) void k_tile_color(
         L.color[r] = 0xff;
     }
     __syncthreads();
+    // Jones-Plassmann rounds: a row whose priority is the highest among the
+    // uncoloured rows at both its dynamic bodies takes the lowest colour that
+    // none of those bodies' coloured rows holds.  The rows taking colours in
+    // one round share no body, so each body's colour mask has one writer.
     int ncol = 0;
     for (int round = 0; round < RB_MAX_LOCAL_COLORS; round++) {
         unsigned long long tag = ((unsigned long long)(round + 1) << 32);
@@ -324,16 +333,26 @@ __global__ __launch_bounds__(TB) void k_tile_color(
         for (int r = threadIdx.x; r < R; r += TB) {
             if (L.color[r] != 0xff) continue;
             unsigned long long v = tag | color_prio(tile_kv[first + r].y);
-            bool win = (L.la[r] < 0 || L.mark[L.la[r]] == v) && (L.lb[r] < 0 || L.mark[L.lb[r]] == v);
-            if (win) L.color[r] = (unsigned char)round;
-            else left++;
+            const int la = L.la[r], lb = L.lb[r];
+            bool win = (la < 0 || L.mark[la] == v) && (lb < 0 || L.mark[lb] == v);
+            if (win) {
+                const unsigned long long u = (la >= 0 ? L.used[la] : 0ull) |
+                                             (lb >= 0 ? L.used[lb] : 0ull);
+                int c = RB_MAX_LOCAL_COLORS - 1;
+                if (~u) c = __builtin_ctzll(~u);
+                else atomicOr(counters + RB_CNT_OVERFLOW, (int)MI_RB_OVERFLOW_COLORS);
+                L.color[r] = (unsigned char)c;
+                if (la >= 0) L.used[la] |= 1ull << c;
+                if (lb >= 0) L.used[lb] |= 1ull << c;
+            } else {
+                left++;
+            }
         }
         if (left) atomicAdd(&L.remaining, left);
         __syncthreads();
         int rem = L.remaining;
         __syncthreads();
         if (threadIdx.x == 0) L.remaining = 0;
-        ncol = round + 1;
         if (rem == 0) break;
         __syncthreads();
     }
