@@ -109,6 +109,37 @@ __device__ __forceinline__ void project(const cfloat4 *sv, int nv, float4 pos, f
     mx = h + off;
 }
 
+// project() over a part of the vertices: this lane takes vertices part,
+// part + 2^lg, ... (vector loads), and the 2^lg lanes of an aligned group
+// combine their minima and maxima.  Min and max are exact, so the result
+// equals project()'s for any split.  Every lane of the wave must call it.
+__device__ __forceinline__ void project_part(const float4 *__restrict__ v, int nv, int part,
+                                             int lg, float4 pos, float4 orn, float4 dir,
+                                             float &mn, float &mx) {
+    float4 ld = qt_inv_rotate(orn, dir);
+    float off = dot3(pos, dir);
+    float lo = FLT_MAX, hi = -FLT_MAX;
+    for (int i = part; i < nv; i += 1 << lg) {
+        const float4 q = v[i];
+        float dp = (q.x * ld.x + q.y * ld.y) + q.z * ld.z;
+        lo = fminf(lo, dp);
+        hi = fmaxf(hi, dp);
+    }
+    for (int o = 1; o < (1 << lg); o <<= 1) {
+        lo = fminf(lo, __shfl_xor(lo, o, 64));
+        hi = fmaxf(hi, __shfl_xor(hi, o, 64));
+    }
+    if (lo > hi) { float t = lo; lo = hi; hi = t; }
+    mn = lo + off;
+    mx = hi + off;
+}
+
+// log2 of the lanes per axis for a batch of cnt <= 64 axes: 64 / the next
+// power of two of cnt
+__device__ __forceinline__ int lanes_log2(int cnt) {
+    return cnt > 32 ? 0 : cnt > 16 ? 1 : cnt > 8 ? 2 : cnt > 4 ? 3 : cnt > 2 ? 4 : cnt > 1 ? 5 : 6;
+}
+
 constexpr int NP_BLOCK = 64;
 
 __device__ __forceinline__ bool hull_limits_ok(const HullInfo &hA, const HullInfo &hB) {
@@ -160,30 +191,48 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_face(
         float4 dc = sub4(xform(hA.center, pA, oA), xform(hB.center, pB, oB));
 
         // ---- SAT, face axes ----
+        // Batches of up to 64 axes; a batch of fewer axes gives each axis
+        // 64 / next_pow2(count) lanes that split the vertices.
         float best = FLT_MAX;
         int best_ax = 0x7fffffff;
         int sep_found = 0;
         const int nf = hA.num_faces + hB.num_faces;
-        for (int ax = lane; ax < nf; ax += NP_BLOCK) {
+        const float4 *gVA = verts + hA.vertex_offset, *gVB = verts + hB.vertex_offset;
+        for (int base = 0; base < nf;) {
+            const int cnt = nf - base < NP_BLOCK ? nf - base : NP_BLOCK;
+            const int lg = lanes_log2(cnt);
+            const int slot = lane >> lg, part = lane & ((1 << lg) - 1);
+            const bool valid = slot < cnt;
+            const int ax = base + (valid ? slot : 0);  // spare lanes repeat an axis
             bool onA = ax < hA.num_faces;
             float4 nrm = onA ? planes[hA.face_offset + ax] : planes[hB.face_offset + ax - hA.num_faces];
             float4 w = qt_rotate(onA ? oA : oB, nrm);
             if (dot3(dc, w) < 0) w = scl4(w, -1.f);
             float mn0, mx0, mn1, mx1;
-            if (onA) {
-                project(sVA, nvA, pA, oA, w, mn0, mx0);
-                project(sVB, nvB, pB, oB, w, mn1, mx1);
+            if (lg == 0) {
+                if (onA) {
+                    project(sVA, nvA, pA, oA, w, mn0, mx0);
+                    project(sVB, nvB, pB, oB, w, mn1, mx1);
+                } else {
+                    project(sVB, nvB, pB, oB, w, mn0, mx0);
+                    project(sVA, nvA, pA, oA, w, mn1, mx1);
+                }
             } else {
-                project(sVB, nvB, pB, oB, w, mn0, mx0);
-                project(sVA, nvA, pA, oA, w, mn1, mx1);
+                project_part(onA ? gVA : gVB, onA ? nvA : nvB, part, lg, onA ? pA : pB,
+                             onA ? oA : oB, w, mn0, mx0);
+                project_part(onA ? gVB : gVA, onA ? nvB : nvA, part, lg, onA ? pB : pA,
+                             onA ? oB : oA, w, mn1, mx1);
             }
-            if (mx0 < mn1 || mx1 < mn0) {
-                sep_found = 1;
-            } else {
-                float d0 = mx0 - mn1, d1 = mx1 - mn0;
-                float d = d0 < d1 ? d0 : d1;
-                if (d < best) { best = d; best_ax = ax; }
+            if (valid && part == 0) {
+                if (mx0 < mn1 || mx1 < mn0) {
+                    sep_found = 1;
+                } else {
+                    float d0 = mx0 - mn1, d1 = mx1 - mn0;
+                    float d = d0 < d1 ? d0 : d1;
+                    if (d < best) { best = d; best_ax = ax; }
+                }
             }
+            base += cnt;
         }
         if (__ballot(sep_found)) {
             if (lane == 0) { has_contact[p] = 0; cand[p] = 0; }
@@ -268,15 +317,43 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_clip(
         const int nf = hA.num_faces + hB.num_faces;
         const int ne = hA.num_edges * hB.num_edges;
         // Each hull holds a ball of radius rho (center.w) around its centre,
-        // so along a unit axis c oriented with dot(dc, c) >= 0 the overlap is
-        // at least rhoA + rhoB - dot(dc, c).  An axis whose bound exceeds the
-        // face minimum by a rounding margin cannot be the minimum and is not
-        // projected; the minimum found is the one of the full search.
+        // so along a unit axis c oriented with dot(dc, c) >= 0 the overlap
+        // max_A(c) - min_B(c) is at least rhoA + rhoB + dot(dc, c) and the
+        // overlap max_B(c) - min_A(c) at least rhoA + rhoB - dot(dc, c).  An
+        // axis whose bound exceeds the face minimum by a rounding margin
+        // cannot be the minimum and is not projected; the minimum found is
+        // the one of the full search.
         const int4 fb0 = face_best[p];
         const float face_d = __int_as_float(fb0.x);
         const float margin = 1e-3f + 4e-6f * ((fabsf(pA.x) + fabsf(pA.y) + fabsf(pA.z)) +
                                               (fabsf(pB.x) + fabsf(pB.y) + fabsf(pB.z)));
-        const float lb_base = (hA.center.w + hB.center.w) - margin;
+        const float rho = hA.center.w + hB.center.w;
+        // A second bound for the overlap max_B(c) - min_A(c): it is at least
+        // dot(sB - sA, c) for any vertices sB of B and sA of A.  Taking the
+        // vertices furthest along dc (B) and -dc (A) makes the bound close to
+        // the overlap for axes near dc, where the first bound is weakest.
+        // (only worth its cost for pairs with many edge axes)
+        const bool use_dv = ne > 512;
+        float4 dv = mk4(0.f, 0.f, 0.f, 0.f);
+        if (use_dv) {
+            const float4 lb = qt_inv_rotate(oB, dc), la = qt_inv_rotate(oA, neg4(dc));
+            float eb = -FLT_MAX, ea = -FLT_MAX;
+            int ib = 0, ia = 0;
+            for (int v = lane; v < nvB; v += NP_BLOCK) {
+                const float4 q = verts[hB.vertex_offset + v];
+                const float e = (q.x * lb.x + q.y * lb.y) + q.z * lb.z;
+                if (e > eb) { eb = e; ib = v; }
+            }
+            for (int v = lane; v < nvA; v += NP_BLOCK) {
+                const float4 q = verts[hA.vertex_offset + v];
+                const float e = (q.x * la.x + q.y * la.y) + q.z * la.z;
+                if (e > ea) { ea = e; ia = v; }
+            }
+            wave_argmax(eb, ib);
+            wave_argmax(ea, ia);
+            dv = sub4(xform(verts[hB.vertex_offset + ib], pB, oB),
+                      xform(verts[hA.vertex_offset + ia], pA, oA));
+        }
         // The bound test runs on all lanes, one axis each; the surviving
         // axes are compacted into an LDS queue and projected 64 at a time, so
         // the projection loops run with full waves instead of with the few
@@ -309,7 +386,10 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_clip(
             bool keep = false;
             if (k < ne) {
                 float4 c = axis_of(k);
-                keep = c.w == 0.f && !(lb_base - dot3(dc, c) > face_d);
+                const float dd = dot3(dc, c);
+                const float lb1 = use_dv ? fmaxf(rho - dd, dot3(dv, c)) : rho - dd;
+                const float lb = fminf(rho + dd, lb1) - margin;
+                keep = c.w == 0.f && !(lb > face_d);
             }
             const uint64_t m = __ballot(keep);
             if (keep) sQ[qn + __popcll(m & lanemask_lt())] = k;
@@ -325,7 +405,28 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_clip(
                 __syncthreads();
             }
         }
-        if (lane < qn) project_axis(sQ[lane]);
+        if (qn > 0) {
+            // the last partial batch: 64 / next_pow2(qn) lanes per axis
+            const int lg = lanes_log2(qn);
+            const int slot = lane >> lg, part = lane & ((1 << lg) - 1);
+            const int k = sQ[slot < qn ? slot : 0];
+            if (lg == 0) {
+                if (slot < qn) project_axis(k);
+            } else {
+                float4 c = axis_of(k);
+                float mn0, mx0, mn1, mx1;
+                project_part(verts + hA.vertex_offset, nvA, part, lg, pA, oA, c, mn0, mx0);
+                project_part(verts + hB.vertex_offset, nvB, part, lg, pB, oB, c, mn1, mx1);
+                float d0 = mx0 - mn1, d1 = mx1 - mn0;
+                float dist = d0 < d1 ? d0 : d1;
+                int ax = nf + k;
+                if (slot < qn && part == 0 &&
+                    (dist < best || (dist == best && ax < best_ax))) {
+                    best = dist;
+                    best_ax = ax;
+                }
+            }
+        }
         __syncthreads();
         wave_argmin(best, best_ax);
         if (face_d < best || (face_d == best && fb0.y < best_ax)) { best = face_d; best_ax = fb0.y; }

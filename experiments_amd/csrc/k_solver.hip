@@ -636,14 +636,17 @@ __global__ __launch_bounds__(TB) void k_tile_pack(
     const int *__restrict__ order, ConstraintSoA cs, const float4 *__restrict__ pos,
     const float *__restrict__ inv_mass, const float4 *__restrict__ invI, PackedRows pk,
     int *__restrict__ solve_pos, int *__restrict__ tile_bodies, int *__restrict__ tile_nb,
-    int *counters) {
+    const int *__restrict__ tile_ncolors, int4 *__restrict__ tile_hdr, int *counters) {
     __shared__ PackLDS L;
     const int t = blockIdx.x, tid = threadIdx.x;
     const int first = tile_first[t];
     int R = tile_cnt[t] - first;
     if (R > RB_TILE_MAX_ROWS) R = RB_TILE_MAX_ROWS;
     if (R <= 0) {
-        if (tid == 0) tile_nb[t] = 0;
+        if (tid == 0) {
+            tile_nb[t] = 0;
+            tile_hdr[t] = make_int4(first, 0, 0, 0);
+        }
         return;
     }
     for (int i = tid; i < HASH_SLOTS; i += TB) { L.key[i] = -1; L.dense[i] = -1; }
@@ -664,7 +667,10 @@ __global__ __launch_bounds__(TB) void k_tile_pack(
     const int nb = L.count;
     for (int i = tid; i < HASH_SLOTS; i += TB)
         if (L.key[i] >= 0) tile_bodies[2 * (size_t)first + L.dense[i]] = L.key[i];
-    if (tid == 0) tile_nb[t] = nb;
+    if (tid == 0) {
+        tile_nb[t] = nb;
+        tile_hdr[t] = make_int4(first, R, tile_ncolors[t], nb < SOLVE_SLOTS ? nb : SOLVE_SLOTS);
+    }
     for (int r = tid; r < R; r += TB) {
         const int i = first + r;
         const int k = order[i];
@@ -727,7 +733,8 @@ struct TileSolveLDS {
 // The sweeps of one tile: the optional warm-start pass, then normal sweeps,
 // then friction sweeps, over the tile's packed rows (positions first..first+R
 // in solve order).  REG: the tile has at most TBK rows and row r lives in the
-// registers of thread r for the whole launch (loaded once, stored once);
+// registers of thread r for the whole launch (rd, rb and its colour rc,
+// loaded by the caller; stored once at the end);
 // otherwise rows are read from the packed arrays on every pass.  Within a
 // colour rows share no dynamic body, so the thread that runs a row does not
 // change the result.  local_tol: a scene tile ends each phase as soon as its
@@ -737,17 +744,9 @@ __device__ __forceinline__ void tile_sweeps(const StepParams &sp, int sweep0, in
                                             int nsweeps_f, int warm, bool local_tol, int first,
                                             int R, int nc, const int *coff, const PackedRows &pk,
                                             float4 *linvel, float4 *angvel, float *residual,
-                                            TileSolveLDS &L, int used[2], float rfin[2]) {
+                                            TileSolveLDS &L, int used[2], float rfin[2],
+                                            RowData &rd, RowBodies &rb, int rc) {
     const int tid = threadIdx.x;
-    RowData rd;
-    RowBodies rb;
-    int rc = -1;
-    if (REG && tid < R) {
-        load_packed(first + tid, pk, rd, rb);
-        // colour of this thread's row: the colour range holding row tid
-        for (int c = 0; c < nc; c++)
-            if (coff[c] <= tid && tid < coff[c + 1]) rc = c;
-    }
     bool swept = false;
     for (int phase = warm ? -1 : 0; phase < 2; phase++) {
         const int nsw = phase < 0 ? 1 : (phase == 0 ? nsweeps_n : nsweeps_f);
@@ -808,9 +807,8 @@ __device__ __forceinline__ void tile_sweeps(const StepParams &sp, int sweep0, in
 template <int TBK>
 __global__ __launch_bounds__(TBK) void k_tile_solve(
     StepParams sp, int phase, int sweep0, int nsweeps_n, int nsweeps_f, int warm,
-    const int *__restrict__ tile_first, const int *__restrict__ tile_cnt,
-    const int *__restrict__ tile_color_off, const int *__restrict__ tile_ncolors,
-    const int *__restrict__ tile_bodies, const int *__restrict__ tile_nb, PackedRows pk,
+    const int4 *__restrict__ tile_hdr, const int *__restrict__ tile_color_off,
+    const int *__restrict__ tile_bodies, PackedRows pk,
     float4 *__restrict__ linvel, float4 *__restrict__ angvel, float *__restrict__ residual,
     int *counters, int4 *__restrict__ tile_stats) {
     __shared__ TileSolveLDS L;
@@ -828,19 +826,26 @@ __global__ __launch_bounds__(TBK) void k_tile_solve(
     }
     const bool local_tol = sp.tile_mode == 1 && sp.tol > 0.f;
     const int tid = threadIdx.x;
-    if (counters[RB_CNT_DONE] != 0) return;  // converged flag set by a previous sweep
-    const int first = tile_first[t];
-    int R = tile_cnt[t] - first;
+    // (first, rows, colours, cached bodies) of the tile, written by k_tile_pack
+    const int4 hdr = tile_hdr[t];
+    if (sp.tol > 0.f && counters[RB_CNT_DONE] != 0) return;  // converged in an earlier sweep
+    const int first = hdr.x, R = hdr.y, nc = hdr.z, nb = hdr.w;
     if (R <= 0) {
         if (local_tol && tid == 0) tile_stats[t] = make_int4(-1, -1, 0, 0);
         return;
     }
-    if (R > RB_TILE_MAX_ROWS) R = RB_TILE_MAX_ROWS;
-    const int nc = tile_ncolors[t];
     const int *coff = tile_color_off + t * (RB_MAX_LOCAL_COLORS + 1);
-    int nb = tile_nb[t];
-    if (nb > SOLVE_SLOTS) nb = SOLVE_SLOTS;
     const int *bodies = tile_bodies + 2 * (size_t)first;
+    // a tile of at most TBK rows keeps row tid in thread tid's registers;
+    // its loads go out together with the velocity gather below
+    RowData rd;
+    RowBodies rb;
+    int rc = -1;
+    if (R <= TBK && tid < R) {
+        load_packed(first + tid, pk, rd, rb);
+        for (int c = 0; c < nc; c++)
+            if (coff[c] <= tid && tid < coff[c + 1]) rc = c;
+    }
     for (int s = tid; s < nb; s += TBK) {
         int b = bodies[s];
         L.c.v[s] = linvel[b];
@@ -851,10 +856,10 @@ __global__ __launch_bounds__(TBK) void k_tile_solve(
     float rfin[2] = {0.f, 0.f};
     if (R <= TBK)
         tile_sweeps<TBK, true>(sp, sweep0, nsweeps_n, nsweeps_f, warm, local_tol, first, R, nc,
-                               coff, pk, linvel, angvel, residual, L, used, rfin);
+                               coff, pk, linvel, angvel, residual, L, used, rfin, rd, rb, rc);
     else
         tile_sweeps<TBK, false>(sp, sweep0, nsweeps_n, nsweeps_f, warm, local_tol, first, R, nc,
-                                coff, pk, linvel, angvel, residual, L, used, rfin);
+                                coff, pk, linvel, angvel, residual, L, used, rfin, rd, rb, rc);
     if (local_tol && tid == 0)
         tile_stats[t] = make_int4(used[0], used[1], (int)__float_as_uint(rfin[0]),
                                   (int)__float_as_uint(rfin[1]));
@@ -1007,7 +1012,8 @@ hipError_t launch_tiles(const DeviceWorld &d, const StepParams &p, hipStream_t s
                        d.tile_color_off, d.tile_ncolors, d.counters);
     hipLaunchKernelGGL(k_tile_pack, dim3(p.n_tiles), dim3(TB), 0, s, d.tile_first, d.tile_cnt,
                        d.solve_order, d.cs, d.pos, d.inv_mass, d.inv_inertia_world, d.pk,
-                       d.solve_pos, d.tile_bodies, d.tile_nb, d.counters);
+                       d.solve_pos, d.tile_bodies, d.tile_nb, d.tile_ncolors, d.tile_hdr,
+                       d.counters);
     return hipGetLastError();
 }
 
@@ -1018,8 +1024,8 @@ hipError_t launch_solve(const DeviceWorld &d, const StepParams &p, hipStream_t s
     launch_fill_u32(reinterpret_cast<uint32_t *>(d.residual), 0u, (size_t)2 * (S + 1), s);
     // scene tiles: one workgroup per scene, 256 or 512 threads
 #define RB_SOLVE_ARGS                                                                         \
-    d.tile_first, d.tile_cnt, d.tile_color_off, d.tile_ncolors, d.tile_bodies, d.tile_nb, d.pk, \
-        d.linvel, d.angvel, d.residual, d.counters, d.tile_stats
+    d.tile_hdr, d.tile_color_off, d.tile_bodies, d.pk, d.linvel, d.angvel, d.residual,        \
+        d.counters, d.tile_stats
     if (p.tile_mode == 1) {
         // independent scenes: every sweep of a scene in one workgroup, one
         // launch; with a tolerance each scene stops on its own residual

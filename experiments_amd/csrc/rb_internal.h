@@ -127,6 +127,7 @@ struct DeviceWorld {
     int *solve_pos;              // per constraint: its position in the packed rows, -1 if unsolved
     int *tile_bodies;            // per tile at 2 * first: its dynamic bodies (dense slots)
     int *tile_nb;                // per tile: number of those bodies
+    int4 *tile_hdr;              // per tile: first row, rows, colours, cached bodies
     // narrowphase pass 1 -> pass 2
     uint32_t *narrow_cand;       // per pair: 1 if not separated by a face axis
     uint32_t *cand_off;          // its exclusive scan

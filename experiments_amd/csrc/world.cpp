@@ -322,6 +322,7 @@ mi_rb_status derive_and_upload(World &w) {
     RB_CHECK(w.alloc(&d.solve_pos, maxC));
     RB_CHECK(w.alloc(&d.tile_bodies, 2 * (size_t)maxC));
     RB_CHECK(w.alloc(&d.tile_nb, p.n_tiles));
+    RB_CHECK(w.alloc(&d.tile_hdr, p.n_tiles));
     RB_CHECK(w.alloc(&d.narrow_cand, maxP));
     RB_CHECK(w.alloc(&d.cand_off, maxP));
     RB_CHECK(w.alloc(&d.face_best, maxP));
