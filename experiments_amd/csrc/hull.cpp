@@ -281,6 +281,25 @@ int rb_build_hull(const float *xyz, int stride_bytes, int n, const float scale[3
         for (int k = 0; k < 3; k++) c[k] += V[3 * i + k];
     float inv = 1.f / (float)nv;
     for (int k = 0; k < 3; k++) out.center[k] = c[k] * inv;
+    // inscribed-ball radius about the centre from the hull's own triangles
+    // (merged faces can lean up to ~2.6 degrees off their triangles, so their
+    // planes may lie outside part of the hull)
+    {
+        const V3 cc = {out.center[0], out.center[1], out.center[2]};
+        double rin = 1e300;
+        for (auto &t : tris) {
+            V3 a, b, d;
+            const float *pa = &V[3 * remap[t.v[0]]], *pb = &V[3 * remap[t.v[1]]],
+                        *pc = &V[3 * remap[t.v[2]]];
+            a = {pa[0], pa[1], pa[2]}; b = {pb[0], pb[1], pb[2]}; d = {pc[0], pc[1], pc[2]};
+            V3 nn = cross(sub(b, a), sub(d, a));
+            const double l = len(nn);
+            if (!(l > 0.0)) continue;
+            double dist = dot(nn, sub(a, cc)) / l;
+            rin = std::min(rin, std::fabs(dist));
+        }
+        out.inradius = rin < 1e300 ? rin : 0.0;
+    }
     for (int k = 0; k < 3; k++) { out.aabb_min[k] = 1e30f; out.aabb_max[k] = -1e30f; }
     for (int i = 0; i < nv; i++)
         for (int k = 0; k < 3; k++) {

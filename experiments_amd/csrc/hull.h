@@ -9,6 +9,7 @@ struct HullData {
     std::vector<int> indices;      // vertex indices per face, relative to this hull
     std::vector<float> edges;      // unique unit edge directions, xyz
     float center[3] = {0, 0, 0};   // vertex average
+    double inradius = 0.0;         // smallest centre-to-facet distance over the hull triangles
     float aabb_min[3] = {0, 0, 0};
     float aabb_max[3] = {0, 0, 0};
 };

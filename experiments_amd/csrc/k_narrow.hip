@@ -140,6 +140,49 @@ __device__ __forceinline__ int lanes_log2(int cnt) {
     return cnt > 32 ? 0 : cnt > 16 ? 1 : cnt > 8 ? 2 : cnt > 4 ? 3 : cnt > 2 ? 4 : cnt > 1 ? 5 : 6;
 }
 
+// Lower bounds of the SAT overlap along a unit axis c oriented with
+// dot(dc, c) >= 0, dc = centre A - centre B.  Each hull holds a ball of
+// radius rho (HullInfo center.w) around its centre, so max_A(c) - min_B(c)
+// >= rhoA + rhoB + dot(dc, c) and max_B(c) - min_A(c) >= rhoA + rhoB -
+// dot(dc, c); also max_B(c) - min_A(c) >= dot(sB - sA, c) for any vertex sB of
+// B and sA of A, which with the vertices furthest along dc (B) and -dc (A)
+// is close to the overlap for axes near dc.  An axis whose bound exceeds a
+// known overlap by the rounding margin can neither separate (when that
+// overlap is >= 0) nor be the minimum.
+__device__ __forceinline__ float sat_margin(float4 pA, float4 pB) {
+    return 1e-3f + 4e-6f * ((fabsf(pA.x) + fabsf(pA.y) + fabsf(pA.z)) +
+                            (fabsf(pB.x) + fabsf(pB.y) + fabsf(pB.z)));
+}
+__device__ __forceinline__ float sat_bound(float rho, float4 dc, bool use_dv, float4 dv,
+                                           float4 c) {
+    const float dd = dot3(dc, c);
+    const float lb1 = use_dv ? fmaxf(rho - dd, dot3(dv, c)) : rho - dd;
+    return fminf(rho + dd, lb1);
+}
+// sB - sA in world space (every lane of the wave must call it)
+__device__ __forceinline__ float4 support_diff(const float4 *__restrict__ verts,
+                                               const HullInfo &hA, const HullInfo &hB,
+                                               float4 pA, float4 oA, float4 pB, float4 oB,
+                                               float4 dc, int lane) {
+    const float4 lb = qt_inv_rotate(oB, dc), la = qt_inv_rotate(oA, neg4(dc));
+    float eb = -FLT_MAX, ea = -FLT_MAX;
+    int ib = 0, ia = 0;
+    for (int v = lane; v < hB.num_vertices; v += 64) {
+        const float4 q = verts[hB.vertex_offset + v];
+        const float e = (q.x * lb.x + q.y * lb.y) + q.z * lb.z;
+        if (e > eb) { eb = e; ib = v; }
+    }
+    for (int v = lane; v < hA.num_vertices; v += 64) {
+        const float4 q = verts[hA.vertex_offset + v];
+        const float e = (q.x * la.x + q.y * la.y) + q.z * la.z;
+        if (e > ea) { ea = e; ia = v; }
+    }
+    wave_argmax(eb, ib);
+    wave_argmax(ea, ia);
+    return sub4(xform(verts[hB.vertex_offset + ib], pB, oB),
+                xform(verts[hA.vertex_offset + ia], pA, oA));
+}
+
 constexpr int NP_BLOCK = 64;
 
 __device__ __forceinline__ bool hull_limits_ok(const HullInfo &hA, const HullInfo &hB) {
@@ -159,6 +202,10 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_face(
     const HullInfo *__restrict__ hulls, const float4 *__restrict__ verts,
     const float4 *__restrict__ planes, uint32_t *__restrict__ has_contact,
     uint32_t *__restrict__ cand, int4 *__restrict__ face_best, int *overflow) {
+    // (hull_limits_ok: at most 2 * NP_BLOCK faces per hull)
+    __shared__ float4 sW[4 * NP_BLOCK];  // face axes in world space
+    __shared__ float sLB[4 * NP_BLOCK];  // their overlap bounds
+    __shared__ int sQ[4 * NP_BLOCK];     // axes left to project
     const int lane = threadIdx.x;
     int npairs = *num_pairs_ptr;
     if (npairs > max_pairs) npairs = max_pairs;
@@ -191,48 +238,95 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_face(
         float4 dc = sub4(xform(hA.center, pA, oA), xform(hB.center, pB, oB));
 
         // ---- SAT, face axes ----
-        // Batches of up to 64 axes; a batch of fewer axes gives each axis
-        // 64 / next_pow2(count) lanes that split the vertices.
-        float best = FLT_MAX;
-        int best_ax = 0x7fffffff;
-        int sep_found = 0;
+        // 1. every face axis in world space (oriented along dc) and its
+        //    overlap bound; 2. the axis with the smallest bound projected on
+        //    all lanes: a separating axis ends the pair, else its overlap is
+        //    the first minimum; 3. the axes whose bound does not exceed it,
+        //    compacted and projected in batches of up to 64 (fewer axes get
+        //    several lanes each).  Axes left out cannot separate or be the
+        //    minimum, so the result is that of projecting every axis.
         const int nf = hA.num_faces + hB.num_faces;
+        const int nfA = hA.num_faces;
         const float4 *gVA = verts + hA.vertex_offset, *gVB = verts + hB.vertex_offset;
-        for (int base = 0; base < nf;) {
-            const int cnt = nf - base < NP_BLOCK ? nf - base : NP_BLOCK;
-            const int lg = lanes_log2(cnt);
-            const int slot = lane >> lg, part = lane & ((1 << lg) - 1);
-            const bool valid = slot < cnt;
-            const int ax = base + (valid ? slot : 0);  // spare lanes repeat an axis
-            bool onA = ax < hA.num_faces;
-            float4 nrm = onA ? planes[hA.face_offset + ax] : planes[hB.face_offset + ax - hA.num_faces];
+        const float margin = sat_margin(pA, pB);
+        const float rho = hA.center.w + hB.center.w;
+        const bool use_dv = nvA + nvB > 16;
+        const float4 dv = use_dv ? support_diff(verts, hA, hB, pA, oA, pB, oB, dc, lane)
+                                 : mk4(0.f, 0.f, 0.f, 0.f);
+        float lbmin = FLT_MAX;
+        int lbax = 0x7fffffff;
+        __syncthreads();  // the previous pair's reads of sW, sLB and sQ are done
+        for (int ax = lane; ax < nf; ax += NP_BLOCK) {
+            const bool onA = ax < nfA;
+            const float4 nrm = onA ? planes[hA.face_offset + ax] : planes[hB.face_offset + ax - nfA];
             float4 w = qt_rotate(onA ? oA : oB, nrm);
             if (dot3(dc, w) < 0) w = scl4(w, -1.f);
+            sW[ax] = w;
+            const float lb = sat_bound(rho, dc, use_dv, dv, w) - margin;
+            sLB[ax] = lb;
+            if (lb < lbmin) { lbmin = lb; lbax = ax; }
+        }
+        wave_argmin(lbmin, lbax);
+        __syncthreads();
+        int sep_found = 0;
+        float best;
+        int best_ax = lbax;
+        {
+            const float4 w = sW[lbax];
+            const bool onA = lbax < nfA;
             float mn0, mx0, mn1, mx1;
-            if (lg == 0) {
-                if (onA) {
-                    project(sVA, nvA, pA, oA, w, mn0, mx0);
-                    project(sVB, nvB, pB, oB, w, mn1, mx1);
-                } else {
-                    project(sVB, nvB, pB, oB, w, mn0, mx0);
-                    project(sVA, nvA, pA, oA, w, mn1, mx1);
-                }
-            } else {
-                project_part(onA ? gVA : gVB, onA ? nvA : nvB, part, lg, onA ? pA : pB,
-                             onA ? oA : oB, w, mn0, mx0);
-                project_part(onA ? gVB : gVA, onA ? nvB : nvA, part, lg, onA ? pB : pA,
-                             onA ? oB : oA, w, mn1, mx1);
+            project_part(onA ? gVA : gVB, onA ? nvA : nvB, lane, 6, onA ? pA : pB, onA ? oA : oB,
+                         w, mn0, mx0);
+            project_part(onA ? gVB : gVA, onA ? nvB : nvA, lane, 6, onA ? pB : pA, onA ? oB : oA,
+                         w, mn1, mx1);
+            sep_found = mx0 < mn1 || mx1 < mn0;
+            const float d0 = mx0 - mn1, d1 = mx1 - mn0;
+            best = d0 < d1 ? d0 : d1;
+        }
+        if (!sep_found) {
+            const float ref = best;
+            int qn = 0;
+            for (int base = 0; base < nf; base += NP_BLOCK) {
+                const int k = base + lane;
+                const bool keep = k < nf && k != lbax && !(sLB[k] > ref);
+                const uint64_t m = __ballot(keep);
+                if (keep) sQ[qn + __popcll(m & lanemask_lt())] = k;
+                qn += __popcll(m);
             }
-            if (valid && part == 0) {
-                if (mx0 < mn1 || mx1 < mn0) {
-                    sep_found = 1;
+            __syncthreads();
+            for (int b0 = 0; b0 < qn; b0 += NP_BLOCK) {
+                const int cnt = qn - b0 < NP_BLOCK ? qn - b0 : NP_BLOCK;
+                const int lg = lanes_log2(cnt);
+                const int slot = lane >> lg, part = lane & ((1 << lg) - 1);
+                const bool valid = slot < cnt;
+                const int ax = sQ[b0 + (valid ? slot : 0)];  // spare lanes repeat an axis
+                const bool onA = ax < nfA;
+                const float4 w = sW[ax];
+                float mn0, mx0, mn1, mx1;
+                if (lg == 0) {
+                    if (onA) {
+                        project(sVA, nvA, pA, oA, w, mn0, mx0);
+                        project(sVB, nvB, pB, oB, w, mn1, mx1);
+                    } else {
+                        project(sVB, nvB, pB, oB, w, mn0, mx0);
+                        project(sVA, nvA, pA, oA, w, mn1, mx1);
+                    }
                 } else {
-                    float d0 = mx0 - mn1, d1 = mx1 - mn0;
-                    float d = d0 < d1 ? d0 : d1;
-                    if (d < best) { best = d; best_ax = ax; }
+                    project_part(onA ? gVA : gVB, onA ? nvA : nvB, part, lg, onA ? pA : pB,
+                                 onA ? oA : oB, w, mn0, mx0);
+                    project_part(onA ? gVB : gVA, onA ? nvB : nvA, part, lg, onA ? pB : pA,
+                                 onA ? oB : oA, w, mn1, mx1);
+                }
+                if (valid && part == 0) {
+                    if (mx0 < mn1 || mx1 < mn0) {
+                        sep_found = 1;
+                    } else {
+                        const float d0 = mx0 - mn1, d1 = mx1 - mn0;
+                        const float d = d0 < d1 ? d0 : d1;
+                        if (d < best || (d == best && ax < best_ax)) { best = d; best_ax = ax; }
+                    }
                 }
             }
-            base += cnt;
         }
         if (__ballot(sep_found)) {
             if (lane == 0) { has_contact[p] = 0; cand[p] = 0; }
@@ -316,44 +410,17 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_clip(
         int best_ax = 0x7fffffff;
         const int nf = hA.num_faces + hB.num_faces;
         const int ne = hA.num_edges * hB.num_edges;
-        // Each hull holds a ball of radius rho (center.w) around its centre,
-        // so along a unit axis c oriented with dot(dc, c) >= 0 the overlap
-        // max_A(c) - min_B(c) is at least rhoA + rhoB + dot(dc, c) and the
-        // overlap max_B(c) - min_A(c) at least rhoA + rhoB - dot(dc, c).  An
-        // axis whose bound exceeds the face minimum by a rounding margin
-        // cannot be the minimum and is not projected; the minimum found is
-        // the one of the full search.
+        // An edge axis whose overlap bound (sat_bound) exceeds the face
+        // minimum by the rounding margin cannot be the minimum and is not
+        // projected; the minimum found is the one of the full search.
         const int4 fb0 = face_best[p];
         const float face_d = __int_as_float(fb0.x);
-        const float margin = 1e-3f + 4e-6f * ((fabsf(pA.x) + fabsf(pA.y) + fabsf(pA.z)) +
-                                              (fabsf(pB.x) + fabsf(pB.y) + fabsf(pB.z)));
+        const float margin = sat_margin(pA, pB);
         const float rho = hA.center.w + hB.center.w;
-        // A second bound for the overlap max_B(c) - min_A(c): it is at least
-        // dot(sB - sA, c) for any vertices sB of B and sA of A.  Taking the
-        // vertices furthest along dc (B) and -dc (A) makes the bound close to
-        // the overlap for axes near dc, where the first bound is weakest.
-        // (only worth its cost for pairs with many edge axes)
+        // (the vertex bound only pays for pairs with many edge axes)
         const bool use_dv = ne > 512;
-        float4 dv = mk4(0.f, 0.f, 0.f, 0.f);
-        if (use_dv) {
-            const float4 lb = qt_inv_rotate(oB, dc), la = qt_inv_rotate(oA, neg4(dc));
-            float eb = -FLT_MAX, ea = -FLT_MAX;
-            int ib = 0, ia = 0;
-            for (int v = lane; v < nvB; v += NP_BLOCK) {
-                const float4 q = verts[hB.vertex_offset + v];
-                const float e = (q.x * lb.x + q.y * lb.y) + q.z * lb.z;
-                if (e > eb) { eb = e; ib = v; }
-            }
-            for (int v = lane; v < nvA; v += NP_BLOCK) {
-                const float4 q = verts[hA.vertex_offset + v];
-                const float e = (q.x * la.x + q.y * la.y) + q.z * la.z;
-                if (e > ea) { ea = e; ia = v; }
-            }
-            wave_argmax(eb, ib);
-            wave_argmax(ea, ia);
-            dv = sub4(xform(verts[hB.vertex_offset + ib], pB, oB),
-                      xform(verts[hA.vertex_offset + ia], pA, oA));
-        }
+        const float4 dv = use_dv ? support_diff(verts, hA, hB, pA, oA, pB, oB, dc, lane)
+                                 : mk4(0.f, 0.f, 0.f, 0.f);
         // The bound test runs on all lanes, one axis each; the surviving
         // axes are compacted into an LDS queue and projected 64 at a time, so
         // the projection loops run with full waves instead of with the few
@@ -386,10 +453,7 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_clip(
             bool keep = false;
             if (k < ne) {
                 float4 c = axis_of(k);
-                const float dd = dot3(dc, c);
-                const float lb1 = use_dv ? fmaxf(rho - dd, dot3(dv, c)) : rho - dd;
-                const float lb = fminf(rho + dd, lb1) - margin;
-                keep = c.w == 0.f && !(lb > face_d);
+                keep = c.w == 0.f && !(sat_bound(rho, dc, use_dv, dv, c) - margin > face_d);
             }
             const uint64_t m = __ballot(keep);
             if (keep) sQ[qn + __popcll(m & lanemask_lt())] = k;

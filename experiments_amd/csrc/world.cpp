@@ -534,9 +534,10 @@ extern "C" mi_rb_status mi_rb_register_convex_hull(void *world, const float *xyz
     hi.index_offset = (int)w->indices.size();
     hi.num_indices = ni;
     // w: radius of a ball around the centre inside the hull (the smallest
-    // centre-to-face distance, shrunk by 1e-4 relative), a lower bound the
-    // narrowphase uses to skip axes
-    double rho = 1e30;
+    // centre-to-plane distance over the hull triangles and the merged faces,
+    // shrunk by 1e-4 relative), a lower bound the narrowphase uses to skip
+    // axes
+    double rho = h.inradius;
     for (int i = 0; i < nf; i++) {
         const float *pl = &h.planes[4 * i];
         double d = -((double)pl[0] * h.center[0] + (double)pl[1] * h.center[1] +

@@ -171,3 +171,50 @@ def test_per_body_pair_overflow_is_reported():
     assert set(map(tuple, gp.tolist())) <= ref
     assert np.all(np.bincount(gp[:, 0], minlength=len(mn)) <= 2 + s.is_large.sum())
     w.close()
+
+
+def test_narrowphase_contacts_settled_piles():
+    """Per-pair contact counts, pair order and points of the narrowphase on
+    1024 batched piles after 200 steps (cubes and 8-vertex hulls resting on
+    each other), against the oracle on the same state."""
+    w = scenes.build_world(scenes.batched_piles(n_scenes=1024), max_pairs_per_body=64)
+    w.use_graph(True)
+    w.step(200)
+    w.use_graph(False)
+    s = Snapshot(w)
+    w.step_partial(L.STAGE_CONSTRAINTS)
+    _, _, _, pairs = oracle_broadphase(s)
+    ref = oracle_contacts(s, pairs)
+    got = w.contacts(O.CONTACT_DTYPE)
+    w.close()
+    gk = dict(zip(got["pair_index"].tolist(), range(len(got))))
+    rk = dict(zip(ref["pair_index"].tolist(), range(len(ref))))
+    only_gpu = sorted(set(gk) - set(rk))
+    only_ref = sorted(set(rk) - set(gk))
+    assert not only_gpu and not only_ref, (len(only_gpu), len(only_ref), only_gpu[:5], only_ref[:5])
+    n_got, n_ref = got["normal"][:, 3], ref["normal"][:, 3]
+    bad = np.nonzero(n_got != n_ref)[0]
+    assert len(bad) == 0, [(int(got["pair_index"][k]), got["normal"][k].tolist(),
+                            ref["normal"][k].tolist()) for k in bad[:5]]
+    assert np.allclose(got["normal"][:, :3], ref["normal"][:, :3], rtol=1e-5, atol=1e-6)
+    n = n_ref.astype(int)
+    for k in range(len(ref)):
+        assert np.allclose(got["pos"][k, :n[k]], ref["pos"][k, :n[k]], rtol=1e-5, atol=1e-6), k
+
+
+def test_hull_inscribed_radius_is_inside():
+    """The ball the narrowphase uses to skip SAT axes (HullInfo center.w about
+    the vertex-average centre) lies inside every registered hull: its radius
+    is at most the distance from the centre to every facet plane of the
+    hull's vertices (scipy's Qhull facets)."""
+    from scipy.spatial import ConvexHull
+    w = scenes.build_world(scenes.batched_piles(n_scenes=256))
+    hulls = w.read(L.BUF_HULLS, np.uint8, (80,)).view(O.HULL_DTYPE).reshape(-1)
+    verts = w.read(L.BUF_VERTS, np.float32, (4,))
+    w.close()
+    assert len(hulls) > 100
+    for h in hulls:
+        v = verts[h["vertex_offset"]:h["vertex_offset"] + h["num_vertices"], :3].astype(np.float64)
+        c = h["center"][:3].astype(np.float64)
+        facet = min(-(eq[:3] @ c + eq[3]) for eq in ConvexHull(v).equations)
+        assert 0.0 < h["center"][3] <= facet, (h["center"][3], facet)
