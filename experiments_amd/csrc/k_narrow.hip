@@ -184,6 +184,9 @@ __device__ __forceinline__ float4 support_diff(const float4 *__restrict__ verts,
 }
 
 constexpr int NP_BLOCK = 64;
+// face axes up to which pass 1 projects every axis instead of ordering them
+// by an overlap bound first
+constexpr int FACE_PLAIN_MAX = 32;
 
 __device__ __forceinline__ bool hull_limits_ok(const HullInfo &hA, const HullInfo &hB) {
     return hA.num_vertices <= RB_MAX_HULL_VERTS && hB.num_vertices <= RB_MAX_HULL_VERTS &&
@@ -238,94 +241,104 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_face(
         float4 dc = sub4(xform(hA.center, pA, oA), xform(hB.center, pB, oB));
 
         // ---- SAT, face axes ----
-        // 1. every face axis in world space (oriented along dc) and its
-        //    overlap bound; 2. the axis with the smallest bound projected on
-        //    all lanes: a separating axis ends the pair, else its overlap is
-        //    the first minimum; 3. the axes whose bound does not exceed it,
-        //    compacted and projected in batches of up to 64 (fewer axes get
-        //    several lanes each).  Axes left out cannot separate or be the
-        //    minimum, so the result is that of projecting every axis.
         const int nf = hA.num_faces + hB.num_faces;
         const int nfA = hA.num_faces;
         const float4 *gVA = verts + hA.vertex_offset, *gVB = verts + hB.vertex_offset;
-        const float margin = sat_margin(pA, pB);
-        const float rho = hA.center.w + hB.center.w;
-        const bool use_dv = nvA + nvB > 16;
-        const float4 dv = use_dv ? support_diff(verts, hA, hB, pA, oA, pB, oB, dc, lane)
-                                 : mk4(0.f, 0.f, 0.f, 0.f);
-        float lbmin = FLT_MAX;
-        int lbax = 0x7fffffff;
-        __syncthreads();  // the previous pair's reads of sW, sLB and sQ are done
-        for (int ax = lane; ax < nf; ax += NP_BLOCK) {
-            const bool onA = ax < nfA;
-            const float4 nrm = onA ? planes[hA.face_offset + ax] : planes[hB.face_offset + ax - nfA];
-            float4 w = qt_rotate(onA ? oA : oB, nrm);
-            if (dot3(dc, w) < 0) w = scl4(w, -1.f);
-            sW[ax] = w;
-            const float lb = sat_bound(rho, dc, use_dv, dv, w) - margin;
-            sLB[ax] = lb;
-            if (lb < lbmin) { lbmin = lb; lbax = ax; }
-        }
-        wave_argmin(lbmin, lbax);
-        __syncthreads();
         int sep_found = 0;
-        float best;
-        int best_ax = lbax;
-        {
-            const float4 w = sW[lbax];
-            const bool onA = lbax < nfA;
+        float best = FLT_MAX;
+        int best_ax = 0x7fffffff;
+        // Project every axis of a batch of up to 64; a batch of fewer axes
+        // gives each axis 64 / next_pow2(count) lanes that split the vertices.
+        // Lane results: separation flag, lexicographic (overlap, axis) minimum.
+        auto project_batch = [&](int cnt, auto axis_at) {
+            const int lg = lanes_log2(cnt);
+            const int slot = lane >> lg, part = lane & ((1 << lg) - 1);
+            const bool valid = slot < cnt;
+            const int ax = axis_at(valid ? slot : 0);  // spare lanes repeat an axis
+            const bool onA = ax < nfA;
+            const float4 w = sW[ax];
             float mn0, mx0, mn1, mx1;
-            project_part(onA ? gVA : gVB, onA ? nvA : nvB, lane, 6, onA ? pA : pB, onA ? oA : oB,
-                         w, mn0, mx0);
-            project_part(onA ? gVB : gVA, onA ? nvB : nvA, lane, 6, onA ? pB : pA, onA ? oB : oA,
-                         w, mn1, mx1);
-            sep_found = mx0 < mn1 || mx1 < mn0;
-            const float d0 = mx0 - mn1, d1 = mx1 - mn0;
-            best = d0 < d1 ? d0 : d1;
-        }
-        if (!sep_found) {
-            const float ref = best;
-            int qn = 0;
-            for (int base = 0; base < nf; base += NP_BLOCK) {
-                const int k = base + lane;
-                const bool keep = k < nf && k != lbax && !(sLB[k] > ref);
-                const uint64_t m = __ballot(keep);
-                if (keep) sQ[qn + __popcll(m & lanemask_lt())] = k;
-                qn += __popcll(m);
+            if (lg == 0) {
+                if (onA) {
+                    project(sVA, nvA, pA, oA, w, mn0, mx0);
+                    project(sVB, nvB, pB, oB, w, mn1, mx1);
+                } else {
+                    project(sVB, nvB, pB, oB, w, mn0, mx0);
+                    project(sVA, nvA, pA, oA, w, mn1, mx1);
+                }
+            } else {
+                project_part(onA ? gVA : gVB, onA ? nvA : nvB, part, lg, onA ? pA : pB,
+                             onA ? oA : oB, w, mn0, mx0);
+                project_part(onA ? gVB : gVA, onA ? nvB : nvA, part, lg, onA ? pB : pA,
+                             onA ? oB : oA, w, mn1, mx1);
+            }
+            if (valid && part == 0) {
+                if (mx0 < mn1 || mx1 < mn0) {
+                    sep_found = 1;
+                } else {
+                    const float d0 = mx0 - mn1, d1 = mx1 - mn0;
+                    const float d = d0 < d1 ? d0 : d1;
+                    if (d < best || (d == best && ax < best_ax)) { best = d; best_ax = ax; }
+                }
+            }
+        };
+        __syncthreads();  // the previous pair's reads of sW, sLB and sQ are done
+        if (nf <= FACE_PLAIN_MAX) {
+            // few axes: all of them in one or two batches
+            for (int ax = lane; ax < nf; ax += NP_BLOCK) {
+                const bool onA = ax < nfA;
+                const float4 nrm = onA ? planes[hA.face_offset + ax] : planes[hB.face_offset + ax - nfA];
+                float4 w = qt_rotate(onA ? oA : oB, nrm);
+                if (dot3(dc, w) < 0) w = scl4(w, -1.f);
+                sW[ax] = w;
             }
             __syncthreads();
-            for (int b0 = 0; b0 < qn; b0 += NP_BLOCK) {
-                const int cnt = qn - b0 < NP_BLOCK ? qn - b0 : NP_BLOCK;
-                const int lg = lanes_log2(cnt);
-                const int slot = lane >> lg, part = lane & ((1 << lg) - 1);
-                const bool valid = slot < cnt;
-                const int ax = sQ[b0 + (valid ? slot : 0)];  // spare lanes repeat an axis
+            for (int base = 0; base < nf; base += NP_BLOCK)
+                project_batch(nf - base < NP_BLOCK ? nf - base : NP_BLOCK,
+                              [&](int k) { return base + k; });
+        } else {
+            // many axes: 1. every face axis in world space (oriented along
+            // dc) and its overlap bound; 2. the axis with the smallest bound
+            // projected on all lanes: a separating axis ends the pair, else
+            // its overlap is a known overlap; 3. the axes whose bound does
+            // not exceed it, compacted and projected in batches.  Axes left
+            // out cannot separate or be the minimum, so the result is that
+            // of projecting every axis.
+            const float margin = sat_margin(pA, pB);
+            const float rho = hA.center.w + hB.center.w;
+            const float4 dv = support_diff(verts, hA, hB, pA, oA, pB, oB, dc, lane);
+            float lbmin = FLT_MAX;
+            int lbax = 0x7fffffff;
+            for (int ax = lane; ax < nf; ax += NP_BLOCK) {
                 const bool onA = ax < nfA;
-                const float4 w = sW[ax];
-                float mn0, mx0, mn1, mx1;
-                if (lg == 0) {
-                    if (onA) {
-                        project(sVA, nvA, pA, oA, w, mn0, mx0);
-                        project(sVB, nvB, pB, oB, w, mn1, mx1);
-                    } else {
-                        project(sVB, nvB, pB, oB, w, mn0, mx0);
-                        project(sVA, nvA, pA, oA, w, mn1, mx1);
-                    }
-                } else {
-                    project_part(onA ? gVA : gVB, onA ? nvA : nvB, part, lg, onA ? pA : pB,
-                                 onA ? oA : oB, w, mn0, mx0);
-                    project_part(onA ? gVB : gVA, onA ? nvB : nvA, part, lg, onA ? pB : pA,
-                                 onA ? oB : oA, w, mn1, mx1);
+                const float4 nrm = onA ? planes[hA.face_offset + ax] : planes[hB.face_offset + ax - nfA];
+                float4 w = qt_rotate(onA ? oA : oB, nrm);
+                if (dot3(dc, w) < 0) w = scl4(w, -1.f);
+                sW[ax] = w;
+                const float lb = sat_bound(rho, dc, true, dv, w) - margin;
+                sLB[ax] = lb;
+                if (lb < lbmin) { lbmin = lb; lbax = ax; }
+            }
+            wave_argmin(lbmin, lbax);
+            __syncthreads();
+            project_batch(1, [&](int) { return lbax; });
+            best = __shfl(best, 0, 64);
+            best_ax = __shfl(best_ax, 0, 64);
+            sep_found = __shfl(sep_found, 0, 64);
+            if (!sep_found) {
+                const float ref = best;
+                int qn = 0;
+                for (int base = 0; base < nf; base += NP_BLOCK) {
+                    const int k = base + lane;
+                    const bool keep = k < nf && k != lbax && !(sLB[k] > ref);
+                    const uint64_t m = __ballot(keep);
+                    if (keep) sQ[qn + __popcll(m & lanemask_lt())] = k;
+                    qn += __popcll(m);
                 }
-                if (valid && part == 0) {
-                    if (mx0 < mn1 || mx1 < mn0) {
-                        sep_found = 1;
-                    } else {
-                        const float d0 = mx0 - mn1, d1 = mx1 - mn0;
-                        const float d = d0 < d1 ? d0 : d1;
-                        if (d < best || (d == best && ax < best_ax)) { best = d; best_ax = ax; }
-                    }
-                }
+                __syncthreads();
+                for (int b0 = 0; b0 < qn; b0 += NP_BLOCK)
+                    project_batch(qn - b0 < NP_BLOCK ? qn - b0 : NP_BLOCK,
+                                  [&](int k) { return sQ[b0 + k]; });
             }
         }
         if (__ballot(sep_found)) {
