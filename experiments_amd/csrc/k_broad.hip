@@ -146,7 +146,7 @@ constexpr int FP_BLOCK = 256;
 // LANES lanes per sorted entry; lane q walks the stencil cells
 // [q*C, q*C + C), C = ceil(27 / LANES), cells numbered z, then y, then x
 // from outermost to innermost, so a body's cells are searched in parallel
-// (32 lanes for few dense bodies, 8 for many).  The overlaps are numbered in
+// (32 lanes for few dense bodies, 1 for many).  The overlaps are numbered in
 // the order a serial search visits them (cells in stencil order, members in
 // sorted order, then the scene's large bodies in index order): a count pass,
 // a prefix over the lanes, and a store pass that keeps the first K in LDS.
@@ -172,48 +172,75 @@ __global__ __launch_bounds__(FP_BLOCK) void k_find_pairs(
     if (t >= n) return;
     const uint2 e = kv[t];
     if (e.x == KEY_NONE) return;  // large body: k_large_count / k_large_write
-    int *lst = s_lists + slot * K;
+    int *lst = s_lists + slot * (K | 1);  // odd stride: the lists start in distinct banks
     const int i = (int)e.y;
     const int sc = (int)(e.x >> g.cell_bits);
     const float4 mn0 = smn[t], mx0 = smx[t];
     int gp[3];
     grid_pos(mn0, mx0, g.inv_cell, g.dims, gp);
     const uint32_t sk = (uint32_t)sc << g.cell_bits;
-    uint32_t nk[C], u0[C];
+    uint32_t c = 0;
+    if constexpr (LANES == 1) {
+        // one lane: the occupancy bits of the 27 cells fetched together,
+        // then the occupied cells walked in order
+        uint32_t bits = 0;
 #pragma unroll
-    for (int k = 0; k < C; k++) {
-        const int q = lane * C + k;
-        nk[k] = 0;
-        u0[k] = (uint32_t)n;
-        if (q < 27) {
+        for (int q = 0; q < 27; q++) {
             const int dx = q % 3 - 1, dy = (q / 3) % 3 - 1, dz = q / 9 - 1;
-            nk[k] = pos_hash(gp[0] + dx, gp[1] + dy, gp[2] + dz, g.dims) | sk;
-            if ((occ[nk[k] >> 5] >> (nk[k] & 31u)) & 1u) u0[k] = cell_start[nk[k]];
+            const uint32_t nkq = pos_hash(gp[0] + dx, gp[1] + dy, gp[2] + dz, g.dims) | sk;
+            bits |= ((occ[nkq >> 5] >> (nkq & 31u)) & 1u) << q;
         }
+        while (bits) {
+            const int q = __builtin_ctz(bits);
+            bits &= bits - 1u;
+            const int dx = q % 3 - 1, dy = (q / 3) % 3 - 1, dz = q / 9 - 1;
+            const uint32_t nkq = pos_hash(gp[0] + dx, gp[1] + dy, gp[2] + dz, g.dims) | sk;
+            for (uint32_t u = cell_start[nkq]; u < (uint32_t)n; u++) {
+                const uint2 f = kv[u];
+                if (f.x != nkq) break;
+                if ((int)f.y < i && aabb_overlap(mn0, mx0, smn[u], smx[u])) {
+                    if (c < (uint32_t)K) lst[c] = (int)f.y;
+                    c++;
+                }
+            }
+        }
+    } else {
+        uint32_t nk[C], u0[C];
+#pragma unroll
+        for (int k = 0; k < C; k++) {
+            const int q = lane * C + k;
+            nk[k] = 0;
+            u0[k] = (uint32_t)n;
+            if (q < 27) {
+                const int dx = q % 3 - 1, dy = (q / 3) % 3 - 1, dz = q / 9 - 1;
+                nk[k] = pos_hash(gp[0] + dx, gp[1] + dy, gp[2] + dz, g.dims) | sk;
+                if ((occ[nk[k] >> 5] >> (nk[k] & 31u)) & 1u) u0[k] = cell_start[nk[k]];
+            }
+        }
+        uint32_t cnt = 0;
+#pragma unroll
+        for (int k = 0; k < C; k++)
+            for (uint32_t u = u0[k]; u < (uint32_t)n; u++) {
+                const uint2 f = kv[u];
+                if (f.x != nk[k]) break;
+                if ((int)f.y < i && aabb_overlap(mn0, mx0, smn[u], smx[u])) cnt++;
+            }
+        uint32_t incl = cnt;
+#pragma unroll
+        for (int o = 1; o < LANES; o <<= 1) {
+            uint32_t v = __shfl_up(incl, o, LANES);
+            if (lane >= o) incl += v;
+        }
+        c = __shfl(incl, LANES - 1, LANES);
+        uint32_t v = incl - cnt;
+#pragma unroll
+        for (int k = 0; k < C; k++)
+            for (uint32_t u = u0[k]; u < (uint32_t)n && v < (uint32_t)K; u++) {
+                const uint2 f = kv[u];
+                if (f.x != nk[k]) break;
+                if ((int)f.y < i && aabb_overlap(mn0, mx0, smn[u], smx[u])) lst[v++] = (int)f.y;
+            }
     }
-    uint32_t cnt = 0;
-#pragma unroll
-    for (int k = 0; k < C; k++)
-        for (uint32_t u = u0[k]; u < (uint32_t)n; u++) {
-            const uint2 f = kv[u];
-            if (f.x != nk[k]) break;
-            if ((int)f.y < i && aabb_overlap(mn0, mx0, smn[u], smx[u])) cnt++;
-        }
-    uint32_t incl = cnt;
-#pragma unroll
-    for (int o = 1; o < LANES; o <<= 1) {
-        uint32_t v = __shfl_up(incl, o, LANES);
-        if (lane >= o) incl += v;
-    }
-    uint32_t c = __shfl(incl, LANES - 1, LANES);
-    uint32_t v = incl - cnt;
-#pragma unroll
-    for (int k = 0; k < C; k++)
-        for (uint32_t u = u0[k]; u < (uint32_t)n && v < (uint32_t)K; u++) {
-            const uint2 f = kv[u];
-            if (f.x != nk[k]) break;
-            if ((int)f.y < i && aabb_overlap(mn0, mx0, smn[u], smx[u])) lst[v++] = (int)f.y;
-        }
     if (large_num) {
         const int l0 = large_first[sc], nl = large_num[sc];
         const uint64_t segmask = LANES == 64 ? ~0ull : ((1ull << LANES) - 1ull);
@@ -632,11 +659,12 @@ hipError_t launch_find_pairs(const DeviceWorld &d, const StepParams &p, hipStrea
     int n = p.n_bodies;
     if (n == 0) return hipSuccess;
     int K = p.max_pairs_per_body;
-    // 32 lanes per body while the bodies alone do not fill the chip, else 8
-    const int lanes = n < (1 << 18) ? 32 : 8;
+    // 32 lanes per body while the bodies alone do not fill the chip, else
+    // one lane per body (its cells walked in turn)
+    const int lanes = n < (1 << 18) ? 32 : 1;
     const int entries = FP_BLOCK / lanes;
-    size_t lds = sizeof(int) * (size_t)entries * K;
-    auto kern = lanes == 32 ? k_find_pairs<32> : k_find_pairs<8>;
+    size_t lds = sizeof(int) * (size_t)entries * (K | 1);
+    auto kern = lanes == 32 ? k_find_pairs<32> : k_find_pairs<1>;
     hipLaunchKernelGGL(kern, dim3((n + entries - 1) / entries), dim3(FP_BLOCK), lds, s, n,
                        d.kv, d.cell_start, d.occ, d.smn, d.smx, d.aabb_min, d.aabb_max,
                        d.large_list,
