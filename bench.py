@@ -28,6 +28,13 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per k_tile_solve dispatch, 2.0 x FETCH_SIZE + 1.0 x WRITE_SIZE from
+# rocprofv3 --pmc runs at the state the stage timing uses (copy-kernel
+# calibration and commands in profiles/hbm_traffic_hulls64k.md and
+# profiles/hbm_traffic_batched4k.md)
+SOLVE_TRAFFIC_BYTES = {"hulls64k": (4997 * 2.0 + 1793) * 1024,
+                       "batched4k": (96647 * 2.0 + 50051) * 1024}
+
 # algorithmic bytes per unit for each stage (documented in DESIGN.md)
 STAGES = ["setup", "sort", "pairs", "narrowphase", "constraints", "colouring", "solve",
           "integrate"]
@@ -46,8 +53,11 @@ def stage_bytes(stage, bodies, pairs, contacts, passes, sweeps):
         return 8 * pairs + (96 + 96 + 176) * contacts
     if stage == "colouring":  # tile key sort + one pass over rows
         return (3 * 2 + 1) * 8 * contacts + 24 * contacts
-    if stage == "solve":      # per sweep: read the row (168 B) and write its impulses (16 B)
-        return 184 * contacts * sweeps
+    if stage == "solve":      # per launch: read each row once (312 B) and write its impulses
+        # (16 B normal + 16 B friction); read and write each body's velocities
+        # (64 B).  Spatial tiles launch once per sweep and parity phase, scene
+        # tiles run every sweep in one launch.
+        return (328 * contacts + 64 * bodies) * sweeps
     if stage == "integrate":  # read+write pos, quat, linvel, angvel; read inverse mass
         return 108 * bodies
     return 0
@@ -255,13 +265,27 @@ def main():
     prm = w.params()
     passes = (prm["key_bits"] + 7) // 8
     sweeps = 2 * scene.iterations
-    dom = int(np.argmax(stage_ms))
-    dname = STAGES[dom]
-    abytes = stage_bytes(dname, n, pairs, contacts, passes, sweeps)
-    achieved = abytes / (stage_ms[dom] / 1e3) / 1e9 if stage_ms[dom] > 0 else 0.0
-    roofline = {"bound": "hbm", "kernel": dname, "achieved": round(achieved, 2),
+    # the dominant kernel is the tile solve (spatial tiles: one launch per
+    # sweep and parity phase; scene tiles: one launch per step)
+    scene_tiles = prm["tile_mode"] == 1
+    launches = 1 if scene_tiles else 4 * sweeps  # (the zero warm start has no warm pass)
+    solve_ms = float(stage_ms[STAGES.index("solve")])
+    per_launch_bytes = stage_bytes("solve", n, pairs, contacts, passes,
+                                   1 if scene_tiles else sweeps) / launches
+    launch_s = solve_ms / 1e3 / launches
+    achieved = per_launch_bytes / launch_s / 1e9 if launch_s > 0 else 0.0
+    traffic = SOLVE_TRAFFIC_BYTES.get(args.workload)
+    roofline = {"bound": "hbm", "kernel": "k_tile_solve", "achieved": round(achieved, 2),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": None, "stage_ms": round(float(stage_ms[dom]), 4)}
+                "traffic": round(traffic) if traffic else None,
+                "algorithmic_bytes_per_launch": round(per_launch_bytes),
+                "launches_per_step": launches, "launch_us": round(launch_s * 1e6, 2)}
+    stage_hbm = {}
+    for i, name in enumerate(STAGES):
+        sb = stage_bytes(name, n, pairs, contacts, passes,
+                         1 if (name == "solve" and scene_tiles) else sweeps)
+        if stage_ms[i] > 0:
+            stage_hbm[name] = round(sb / (stage_ms[i] / 1e3) / 1e9, 1)
     breakdown = {STAGES[i]: round(float(stage_ms[i]), 4) for i in range(8)}
     cpu = None
     if args.cpu_seconds > 0:
@@ -285,6 +309,7 @@ def main():
                    "sort_passes": passes, "host_s": round(host_s, 4), "tuning": tuning},
         "autotune_ms_per_step": tune_ms,
         "stage_ms": breakdown,
+        "stage_hbm_GBs": stage_hbm,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "overflow_flags": st["overflow_flags"],
