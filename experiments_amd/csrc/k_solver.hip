@@ -698,9 +698,31 @@ __global__ __launch_bounds__(TB) void k_tile_pack(
             pk.IA[3 * (size_t)i + q] = invI[3 * (size_t)bd.x + q];
             pk.IB[3 * (size_t)i + q] = invI[3 * (size_t)bd.y + q];
         }
-        pk.pA[i] = pos[bd.x];
-        pk.pB[i] = pos[bd.y];
+        const float4 pA = pos[bd.x], pB = pos[bd.y];
+        pk.pA[i] = pA;
+        pk.pB[i] = pB;
         solve_pos[k] = i;
+        // velocity-independent terms of the normal row (as row_normal forms them)
+        {
+            const float4 lin0 = cs.linear[k];
+            const float4 n = neg4(lin0), lin = neg4(n);
+            float4 IA[3], IB[3];
+            for (int q = 0; q < 3; q++) {
+                IA[q] = invI[3 * (size_t)bd.x + q];
+                IB[q] = invI[3 * (size_t)bd.y + q];
+            }
+            for (int q = 0; q < 4; q++) {
+                const float4 wp = cs.pos[4 * (size_t)k + q];
+                const float4 r0 = sub4(wp, pA), r1 = sub4(wp, pB);
+                const float4 a0 = neg4(cross3(r0, n)), a1 = cross3(r1, n);
+                pk.na0[4 * (size_t)i + q] = a0;
+                pk.na1[4 * (size_t)i + q] = a1;
+                pk.nia0[4 * (size_t)i + q] = mt_mul1(IA, a0);
+                pk.nia1[4 * (size_t)i + q] = mt_mul1(IB, a1);
+            }
+            pk.nl[2 * (size_t)i] = scl4(lin, imA);
+            pk.nl[2 * (size_t)i + 1] = scl4(neg4(lin), imB);
+        }
     }
 }
 
@@ -721,6 +743,74 @@ __device__ __forceinline__ void run_row(int mode, RowData &d, const RowBodies &r
     else if (rb.ha == SLOT_GLOBAL) { linvel[rb.a] = vA; angvel[rb.a] = wA; }
     if (rb.hb >= 0) { L.v[rb.hb] = vB; L.w[rb.hb] = wB; }
     else if (rb.hb == SLOT_GLOBAL) { linvel[rb.b] = vB; angvel[rb.b] = wB; }
+}
+
+// A normal row with its velocity-independent terms precomputed by
+// k_tile_pack (the same operations as row_normal performs, so the results
+// are identical): what a launch of normal sweeps needs.
+struct NormalRow {
+    float4 lin0, jac, bb, lam;
+    float4 a0[4], a1[4], ia0[4], ia1[4];
+    float4 l0, l1;  // lin0 * imA, -lin0 * imB
+    int a, b, ha, hb;
+};
+__device__ __forceinline__ void load_normal(int i, const PackedRows &pk, NormalRow &r) {
+    r.lin0 = pk.lin0[i];
+    r.jac = pk.jac[i];
+    r.bb = pk.b[i];
+    r.lam = pk.lam[i];
+    for (int q = 0; q < 4; q++) {
+        r.a0[q] = pk.na0[4 * (size_t)i + q];
+        r.a1[q] = pk.na1[4 * (size_t)i + q];
+        r.ia0[q] = pk.nia0[4 * (size_t)i + q];
+        r.ia1[q] = pk.nia1[4 * (size_t)i + q];
+    }
+    r.l0 = pk.nl[2 * (size_t)i];
+    r.l1 = pk.nl[2 * (size_t)i + 1];
+    const int4 m = pk.meta[i];
+    r.a = m.x;
+    r.b = m.y;
+    r.ha = m.z;
+    r.hb = m.w;
+}
+// row_normal on a NormalRow
+__device__ __forceinline__ void row_normal_pre(NormalRow &d, float4 &vA, float4 &wA, float4 &vB,
+                                               float4 &wB, float &rmax, float &lmax) {
+    const float4 lin0 = d.lin0;
+    float jv[4] = {d.jac.x, d.jac.y, d.jac.z, d.jac.w};
+    float bv[4] = {d.bb.x, d.bb.y, d.bb.z, d.bb.w};
+    float lv[4] = {d.lam.x, d.lam.y, d.lam.z, d.lam.w};
+#pragma unroll
+    for (int ic = 0; ic < 4; ic++) {
+        if (jv[ic] == 0.f) continue;
+        float rl = rel_vel(lin0, neg4(lin0), d.a0[ic], d.a1[ic], vA, wA, vB, wB) + bv[ic];
+        rl *= jv[ic];
+        float prev = lv[ic];
+        float upd = prev + rl;
+        upd = fmaxf(upd, 0.f);
+        upd = fminf(upd, FLT_MAX);
+        rl = upd - prev;
+        lv[ic] = upd;
+        vA = add4(vA, scl4(d.l0, rl)); wA = add4(wA, scl4(d.ia0[ic], rl));
+        vB = add4(vB, scl4(d.l1, rl)); wB = add4(wB, scl4(d.ia1[ic], rl));
+        float sc = fabsf(rl) / fabsf(jv[ic]);
+        rmax = fmaxf(rmax, sc);
+        lmax = fmaxf(lmax, fabsf(upd));
+    }
+    d.lam = mk4(lv[0], lv[1], lv[2], lv[3]);
+}
+__device__ __forceinline__ void run_row_normal(NormalRow &d, SolveLDS &L, float4 *linvel,
+                                               float4 *angvel, float &rmax, float &lmax) {
+    const float4 z = mk4(0, 0, 0, 0);
+    float4 vA = d.ha >= 0 ? L.v[d.ha] : (d.ha == SLOT_GLOBAL ? linvel[d.a] : z);
+    float4 wA = d.ha >= 0 ? L.w[d.ha] : (d.ha == SLOT_GLOBAL ? angvel[d.a] : z);
+    float4 vB = d.hb >= 0 ? L.v[d.hb] : (d.hb == SLOT_GLOBAL ? linvel[d.b] : z);
+    float4 wB = d.hb >= 0 ? L.w[d.hb] : (d.hb == SLOT_GLOBAL ? angvel[d.b] : z);
+    row_normal_pre(d, vA, wA, vB, wB, rmax, lmax);
+    if (d.ha >= 0) { L.v[d.ha] = vA; L.w[d.ha] = wA; }
+    else if (d.ha == SLOT_GLOBAL) { linvel[d.a] = vA; angvel[d.a] = wA; }
+    if (d.hb >= 0) { L.v[d.hb] = vB; L.w[d.hb] = wB; }
+    else if (d.hb == SLOT_GLOBAL) { linvel[d.b] = vB; angvel[d.b] = wB; }
 }
 
 // Shared state of one tile's solve: the LDS velocity cache and the
@@ -802,28 +892,50 @@ __device__ __forceinline__ void tile_sweeps(const StepParams &sp, int sweep0, in
     }
 }
 
+// One normal sweep of a tile on precomputed rows (spatial tiles: a launch
+// per sweep and parity phase).  REG: row tid is nr, colour rc.
+template <int TBK, bool REG>
+__device__ __forceinline__ void tile_normal_sweep(int sweep, int first, int R, int nc,
+                                                  const int *coff, const PackedRows &pk,
+                                                  float4 *linvel, float4 *angvel,
+                                                  float *residual, TileSolveLDS &L,
+                                                  NormalRow &nr, int rc) {
+    const int tid = threadIdx.x;
+    float rmax = 0.f, lmax = 0.f;
+    for (int c = 0; c < nc; c++) {
+        if (REG) {
+            if (rc == c) run_row_normal(nr, L.c, linvel, angvel, rmax, lmax);
+        } else {
+            int c0 = coff[c], c1 = coff[c + 1];
+            if (c1 > R) c1 = R;
+            for (int r = c0 + tid; r < c1; r += TBK) {
+                NormalRow d;
+                load_normal(first + r, pk, d);
+                run_row_normal(d, L.c, linvel, angvel, rmax, lmax);
+                pk.lam[first + r] = d.lam;
+            }
+        }
+        __syncthreads();
+    }
+    if (rmax > 0.f || lmax > 0.f) {
+        atomicMax((unsigned int *)&residual[2 * sweep], __float_as_uint(rmax));
+        atomicMax((unsigned int *)&residual[2 * sweep + 1], __float_as_uint(lmax));
+    }
+    if (REG && tid < R) pk.lam[first + tid] = nr.lam;
+}
+
 // One workgroup per tile (spatial: the tiles of one parity phase; scene: one
 // scene); the tile's dynamic body velocities live in LDS for the launch.
-template <int TBK>
-__global__ __launch_bounds__(TBK) void k_tile_solve(
-    StepParams sp, int phase, int sweep0, int nsweeps_n, int nsweeps_f, int warm,
+// The solve of one tile (rows first..first+R of the packed arrays): its
+// dynamic bodies' velocities gathered into LDS, the sweeps, the velocities
+// stored back.  NORMAL: one normal sweep on the precomputed rows.
+template <int TBK, bool NORMAL>
+__device__ __forceinline__ void solve_tile(
+    const StepParams &sp, int t, int sweep0, int nsweeps_n, int nsweeps_f, int warm,
     const int4 *__restrict__ tile_hdr, const int *__restrict__ tile_color_off,
-    const int *__restrict__ tile_bodies, PackedRows pk,
-    float4 *__restrict__ linvel, float4 *__restrict__ angvel, float *__restrict__ residual,
-    int *counters, int4 *__restrict__ tile_stats) {
-    __shared__ TileSolveLDS L;
-    int t;
-    if (sp.tile_mode == 1) {
-        t = blockIdx.x;
-    } else {
-        // tiles of this phase: (tx & 1, tz & 1) == (phase & 1, phase >> 1)
-        int px = phase & 1, pz = phase >> 1;
-        int nx = (sp.tiles_x - px + 1) / 2;
-        int bx = blockIdx.x % nx, bz = blockIdx.x / nx;
-        int tx = px + 2 * bx, tz = pz + 2 * bz;
-        if (tx >= sp.tiles_x || tz >= sp.tiles_z) return;
-        t = tz * sp.tiles_x + tx;
-    }
+    const int *__restrict__ tile_bodies, const PackedRows &pk, float4 *__restrict__ linvel,
+    float4 *__restrict__ angvel, float *__restrict__ residual, int *counters,
+    int4 *__restrict__ tile_stats, TileSolveLDS &L) {
     const bool local_tol = sp.tile_mode == 1 && sp.tol > 0.f;
     const int tid = threadIdx.x;
     // (first, rows, colours, cached bodies) of the tile, written by k_tile_pack
@@ -836,6 +948,35 @@ __global__ __launch_bounds__(TBK) void k_tile_solve(
     }
     const int *coff = tile_color_off + t * (RB_MAX_LOCAL_COLORS + 1);
     const int *bodies = tile_bodies + 2 * (size_t)first;
+    if constexpr (NORMAL) {
+        // one normal sweep (nsweeps_n == 1) on the precomputed rows
+        NormalRow nr;
+        int rc = -1;
+        if (R <= TBK && tid < R) {
+            load_normal(first + tid, pk, nr);
+            for (int c = 0; c < nc; c++)
+                if (coff[c] <= tid && tid < coff[c + 1]) rc = c;
+        }
+        for (int s = tid; s < nb; s += TBK) {
+            int b = bodies[s];
+            L.c.v[s] = linvel[b];
+            L.c.w[s] = angvel[b];
+        }
+        __syncthreads();
+        if (R <= TBK)
+            tile_normal_sweep<TBK, true>(sweep0, first, R, nc, coff, pk, linvel, angvel, residual,
+                                         L, nr, rc);
+        else
+            tile_normal_sweep<TBK, false>(sweep0, first, R, nc, coff, pk, linvel, angvel, residual,
+                                          L, nr, rc);
+        __syncthreads();
+        for (int s = tid; s < nb; s += TBK) {
+            int b = bodies[s];
+            linvel[b] = L.c.v[s];
+            angvel[b] = L.c.w[s];
+        }
+        return;
+    }
     // a tile of at most TBK rows keeps row tid in thread tid's registers;
     // its loads go out together with the velocity gather below
     RowData rd;
@@ -869,6 +1010,31 @@ __global__ __launch_bounds__(TBK) void k_tile_solve(
         linvel[b] = L.c.v[s];
         angvel[b] = L.c.w[s];
     }
+}
+
+
+template <int TBK, bool NORMAL = false>
+__global__ __launch_bounds__(TBK) void k_tile_solve(
+    StepParams sp, int phase, int sweep0, int nsweeps_n, int nsweeps_f, int warm,
+    const int4 *__restrict__ tile_hdr, const int *__restrict__ tile_color_off,
+    const int *__restrict__ tile_bodies, PackedRows pk,
+    float4 *__restrict__ linvel, float4 *__restrict__ angvel, float *__restrict__ residual,
+    int *counters, int4 *__restrict__ tile_stats) {
+    __shared__ TileSolveLDS L;
+    int t;
+    if (sp.tile_mode == 1) {
+        t = blockIdx.x;
+    } else {
+        // tiles of this phase: (tx & 1, tz & 1) == (phase & 1, phase >> 1)
+        int px = phase & 1, pz = phase >> 1;
+        int nx = (sp.tiles_x - px + 1) / 2;
+        int bx = blockIdx.x % nx, bz = blockIdx.x / nx;
+        int tx = px + 2 * bx, tz = pz + 2 * bz;
+        if (tx >= sp.tiles_x || tz >= sp.tiles_z) return;
+        t = tz * sp.tiles_x + tx;
+    }
+    solve_tile<TBK, NORMAL>(sp, t, sweep0, nsweeps_n, nsweeps_f, warm, tile_hdr, tile_color_off,
+                            tile_bodies, pk, linvel, angvel, residual, counters, tile_stats, L);
 }
 
 // Scene tiles with a tolerance: sweeps used and final residuals over the
@@ -1050,9 +1216,12 @@ hipError_t launch_solve(const DeviceWorld &d, const StepParams &p, hipStream_t s
                 if (sw < 0)
                     hipLaunchKernelGGL(k_tile_solve<TB>, dim3(nx * nz), dim3(TB), 0, s, p, ph, 0, 0, 0,
                                        1, RB_SOLVE_ARGS);
+                else if (!fr)
+                    hipLaunchKernelGGL((k_tile_solve<TB, true>), dim3(nx * nz), dim3(TB), 0, s, p,
+                                       ph, sw, 1, 0, 0, RB_SOLVE_ARGS);
                 else
                     hipLaunchKernelGGL(k_tile_solve<TB>, dim3(nx * nz), dim3(TB), 0, s, p, ph, sw,
-                                       fr ? 0 : 1, fr ? 1 : 0, 0, RB_SOLVE_ARGS);
+                                       0, 1, 0, RB_SOLVE_ARGS);
             }
             if (sw >= 0 && p.tol > 0.f) {
                 hipLaunchKernelGGL(k_sweep_done, dim3(1), dim3(1), 0, s, d.residual, sw, fr, p.tol,

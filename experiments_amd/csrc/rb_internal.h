@@ -79,6 +79,10 @@ struct PackedRows {
     float4 *pA, *pB;
     int4 *meta;       // body A, body B, their LDS slots (-1 static, -2 global)
     float2 *im;       // inverse masses
+    // the velocity-independent terms of the normal rows, 4 per row (one per
+    // point): angular Jacobians a0, a1 and their images under the inverse
+    // inertias; 2 per row: the linear impulse directions scaled by 1/m
+    float4 *na0, *na1, *nia0, *nia1, *nl;
 };
 
 struct DeviceWorld {

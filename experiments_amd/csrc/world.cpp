@@ -307,6 +307,11 @@ mi_rb_status derive_and_upload(World &w) {
     RB_CHECK(w.alloc(&d.tile_ncolors, p.n_tiles));
     RB_CHECK(w.alloc(&d.tile_stats, p.n_tiles));
     RB_CHECK(w.alloc(&d.pk.lin0, maxC));
+    RB_CHECK(w.alloc(&d.pk.na0, 4 * (size_t)maxC));
+    RB_CHECK(w.alloc(&d.pk.na1, 4 * (size_t)maxC));
+    RB_CHECK(w.alloc(&d.pk.nia0, 4 * (size_t)maxC));
+    RB_CHECK(w.alloc(&d.pk.nia1, 4 * (size_t)maxC));
+    RB_CHECK(w.alloc(&d.pk.nl, 2 * (size_t)maxC));
     RB_CHECK(w.alloc(&d.pk.p, 4 * (size_t)maxC));
     RB_CHECK(w.alloc(&d.pk.cen, maxC));
     RB_CHECK(w.alloc(&d.pk.jac, maxC));
