@@ -386,6 +386,7 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_clip(
     __shared__ float4 sPoly[2][RB_MAX_POLY];
     __shared__ float4 sPts[RB_MAX_POLY];
     __shared__ int sQ[2 * NP_BLOCK];  // edge axes that survived the bound test
+    __shared__ float4 sSide[RB_MAX_HULL_VERTS];  // side planes of the reference face (n, w)
 
     const int lane = threadIdx.x;
     const int n_cand = (int)*n_cand_ptr;
@@ -554,16 +555,23 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_clip(
         int2 fiA = face_index[hA.face_offset + fA];
         float4 plA = planes[hA.face_offset + fA];
         float4 wpn = qt_rotate(oA, mk4(plA.x, plA.y, plA.z, 0.f));
+        // the side planes of A's reference face, one lane per edge (a face has
+        // at most RB_MAX_HULL_VERTS vertices)
+        for (int e = lane; e < fiA.y; e += NP_BLOCK) {
+            float4 va = verts[hA.vertex_offset + indices[fiA.x + e]];
+            float4 vb = verts[hA.vertex_offset + indices[fiA.x + (e + 1) % fiA.y]];
+            float4 we = qt_rotate(oA, sub4(va, vb));
+            float4 pn = neg4(cross3(we, wpn));
+            float4 wa = xform(va, pA, oA);
+            sSide[e] = mk4(pn.x, pn.y, pn.z, -dot3(wa, pn));
+        }
         int cur = 0;
         bool poly_over = false;
         __syncthreads();
         for (int e0 = 0; e0 < fiA.y; e0++) {
-            float4 va = verts[hA.vertex_offset + indices[fiA.x + e0]];
-            float4 vb = verts[hA.vertex_offset + indices[fiA.x + (e0 + 1) % fiA.y]];
-            float4 we = qt_rotate(oA, sub4(va, vb));
-            float4 pn = neg4(cross3(we, wpn));
-            float4 wa = xform(va, pA, oA);
-            float eq = -dot3(wa, pn);
+            const float4 side = sSide[e0];
+            const float4 pn = mk4(side.x, side.y, side.z, 0.f);
+            const float eq = side.w;
             const float4 *in = sPoly[cur];
             float4 *outp = sPoly[cur ^ 1];
             // one lane per input vertex (nin <= 64)
