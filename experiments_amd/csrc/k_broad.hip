@@ -44,6 +44,36 @@ __device__ __forceinline__ uint32_t pos_hash(int x, int y, int z, const int *dim
     return (uint32_t)(z * dims[1] * dims[0] + y * dims[0] + x);
 }
 
+// Stencil cells that can hold a member overlapping the AABB [mn, mx]: a
+// member of cell k (unwrapped, along one axis) has its centre in
+// [k c, (k+1) c) and a half-extent of at most c/2 (the grid's cell covers
+// every member), so its AABB lies in [k c - c/2, (k+1) c + c/2].  Bit q of the
+// result (q = 9 (dz+1) + 3 (dy+1) + (dx+1)) is set unless that cell is
+// provably out of reach; wrapped aliases of a cell only hold members farther
+// away, which the AABB test rejects anyway.
+__device__ __forceinline__ uint32_t reachable_cells(float4 mn, float4 mx, float inv_cell) {
+    const float c = 1.0f / inv_cell, h = 0.5f * c, eps = 1e-4f * c;
+    const float lo[3] = {mn.x, mn.y, mn.z}, hi[3] = {mx.x, mx.y, mx.z};
+    uint32_t ok[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const float ctr = (lo[k] + hi[k]) * 0.5f;
+        const float g = floorf(ctr * inv_cell);
+        ok[k] = 0;
+#pragma unroll
+        for (int d = -1; d <= 1; d++) {
+            const float c0 = (g + (float)d) * c - h - eps, c1 = (g + (float)(d + 1)) * c + h + eps;
+            if (hi[k] >= c0 && lo[k] <= c1) ok[k] |= 1u << (d + 1);
+        }
+    }
+    uint32_t m = 0;
+#pragma unroll
+    for (int q = 0; q < 27; q++)
+        if (((ok[0] >> (q % 3)) & 1u) && ((ok[1] >> ((q / 3) % 3)) & 1u) && ((ok[2] >> (q / 9)) & 1u))
+            m |= 1u << q;
+    return m;
+}
+
 struct GridP {
     float inv_cell;
     int dims[3];
@@ -184,12 +214,14 @@ __global__ __launch_bounds__(FP_BLOCK) void k_find_pairs(
         // one lane: the occupancy bits of the 27 cells fetched together,
         // then the occupied cells walked in order
         uint32_t bits = 0;
+        const uint32_t reach = reachable_cells(mn0, mx0, g.inv_cell);
 #pragma unroll
         for (int q = 0; q < 27; q++) {
             const int dx = q % 3 - 1, dy = (q / 3) % 3 - 1, dz = q / 9 - 1;
             const uint32_t nkq = pos_hash(gp[0] + dx, gp[1] + dy, gp[2] + dz, g.dims) | sk;
             bits |= ((occ[nkq >> 5] >> (nkq & 31u)) & 1u) << q;
         }
+        bits &= reach;
         while (bits) {
             const int q = __builtin_ctz(bits);
             bits &= bits - 1u;
@@ -206,12 +238,13 @@ __global__ __launch_bounds__(FP_BLOCK) void k_find_pairs(
         }
     } else {
         uint32_t nk[C], u0[C];
+        const uint32_t reach = reachable_cells(mn0, mx0, g.inv_cell);
 #pragma unroll
         for (int k = 0; k < C; k++) {
             const int q = lane * C + k;
             nk[k] = 0;
             u0[k] = (uint32_t)n;
-            if (q < 27) {
+            if (q < 27 && ((reach >> q) & 1u)) {
                 const int dx = q % 3 - 1, dy = (q / 3) % 3 - 1, dz = q / 9 - 1;
                 nk[k] = pos_hash(gp[0] + dx, gp[1] + dy, gp[2] + dz, g.dims) | sk;
                 if ((occ[nk[k] >> 5] >> (nk[k] & 31u)) & 1u) u0[k] = cell_start[nk[k]];
@@ -506,7 +539,9 @@ __global__ __launch_bounds__(256) void k_hg_pairs(int n, const uint2 *__restrict
     for (int l = lx; l < g.levels; l++) {
         if (l != lx && !level_used[l]) continue;
         int gp[3];
-        hg_cell(mn0, mx0, g.inv_cell0 * exp2f(-(float)l), g.dims, gp);
+        const float inv_l = g.inv_cell0 * exp2f(-(float)l);
+        hg_cell(mn0, mx0, inv_l, g.dims, gp);
+        const uint32_t reach = reachable_cells(mn0, mx0, inv_l);
         const uint32_t lk = (uint32_t)l << g.cell_bits;
         uint32_t nk[C], u0[C];
 #pragma unroll
@@ -514,7 +549,7 @@ __global__ __launch_bounds__(256) void k_hg_pairs(int n, const uint2 *__restrict
             const int q = lane + k * HG_LANES;
             nk[k] = 0;
             u0[k] = (uint32_t)n;
-            if (q < 27) {
+            if (q < 27 && ((reach >> q) & 1u)) {
                 const int dx = q % 3 - 1, dy = (q / 3) % 3 - 1, dz = q / 9 - 1;
                 nk[k] = lk | pos_hash(gp[0] + dx, gp[1] + dy, gp[2] + dz, g.dims);
                 if ((occ[nk[k] >> 5] >> (nk[k] & 31u)) & 1u) u0[k] = cell_start[nk[k]];
