@@ -184,6 +184,7 @@ __device__ __forceinline__ float4 support_diff(const float4 *__restrict__ verts,
 }
 
 constexpr int NP_BLOCK = 64;
+static_assert(RB_MAX_HULL_VERTS <= RB_MAX_POLY, "a face's side planes fit the point buffer");
 // face axes up to which pass 1 projects every axis instead of ordering them
 // by an overlap bound first
 constexpr int FACE_PLAIN_MAX = 32;
@@ -212,16 +213,42 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_face(
     const int lane = threadIdx.x;
     int npairs = *num_pairs_ptr;
     if (npairs > max_pairs) npairs = max_pairs;
-    for (int p = blockIdx.x; p < npairs; p += gridDim.x) {
-        const int2 pr = pairs[p];
-        int a = pr.x, b = pr.y;
-        float imA = inv_mass[a], imB = inv_mass[b];
+    // The pair and body loads of the next pair go out at the top of each
+    // iteration (and the pair index of the one after), so their latency
+    // overlaps the current pair's work.
+    struct Head {
+        int a, b, sA, sB;
+        float imA, imB;
+        float4 pA, pB, oA, oB;
+    };
+    auto load_head = [&](int2 pr, Head &h) {
+        h.a = pr.x;
+        h.b = pr.y;
+        h.imA = inv_mass[pr.x];
+        h.imB = inv_mass[pr.y];
+        h.sA = shape[pr.x];
+        h.sB = shape[pr.y];
+        h.pA = pos[pr.x];
+        h.pB = pos[pr.y];
+        h.oA = quat[pr.x];
+        h.oB = quat[pr.y];
+    };
+    const int G = gridDim.x;
+    Head cur, nxt;
+    int2 pr_next = (int)blockIdx.x + G < npairs ? pairs[blockIdx.x + G] : make_int2(0, 0);
+    int2 pr_after = make_int2(0, 0);
+    if ((int)blockIdx.x < npairs) load_head(pairs[blockIdx.x], cur);
+    for (int p = blockIdx.x; p < npairs; p += G, cur = nxt, pr_next = pr_after) {
+        if (p + G < npairs) load_head(pr_next, nxt);
+        pr_after = p + 2 * G < npairs ? pairs[p + 2 * G] : make_int2(0, 0);
+        const int a = cur.a, b = cur.b;
+        const float imA = cur.imA, imB = cur.imB;
         if (imA == 0.f && imB == 0.f) {
             if (lane == 0) { has_contact[p] = 0; cand[p] = 0; }
             continue;
         }
-        const HullInfo hA = hulls[shape[a]];
-        const HullInfo hB = hulls[shape[b]];
+        const HullInfo hA = hulls[cur.sA];
+        const HullInfo hB = hulls[cur.sB];
         if (!hull_limits_ok(hA, hB)) {
             if (lane == 0) {
                 has_contact[p] = 0;
@@ -230,7 +257,7 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_face(
             }
             continue;
         }
-        float4 pA = pos[a], pB = pos[b], oA = quat[a], oB = quat[b];
+        float4 pA = cur.pA, pB = cur.pB, oA = cur.oA, oB = cur.oB;
         pA.w = 0.f;
         pB.w = 0.f;
         // counts and offsets are the same on every lane: make them scalar
@@ -386,7 +413,6 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_clip(
     __shared__ float4 sPoly[2][RB_MAX_POLY];
     __shared__ float4 sPts[RB_MAX_POLY];
     __shared__ int sQ[2 * NP_BLOCK];  // edge axes that survived the bound test
-    __shared__ float4 sSide[RB_MAX_HULL_VERTS];  // side planes of the reference face (n, w)
 
     const int lane = threadIdx.x;
     const int n_cand = (int)*n_cand_ptr;
@@ -555,8 +581,10 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_clip(
         int2 fiA = face_index[hA.face_offset + fA];
         float4 plA = planes[hA.face_offset + fA];
         float4 wpn = qt_rotate(oA, mk4(plA.x, plA.y, plA.z, 0.f));
-        // the side planes of A's reference face, one lane per edge (a face has
-        // at most RB_MAX_HULL_VERTS vertices)
+        // the side planes of A's reference face (n, w), one lane per edge (a
+        // face has at most RB_MAX_HULL_VERTS <= RB_MAX_POLY vertices), kept in
+        // sPts until the clipped points replace them
+        float4 *sSide = sPts;
         for (int e = lane; e < fiA.y; e += NP_BLOCK) {
             float4 va = verts[hA.vertex_offset + indices[fiA.x + e]];
             float4 vb = verts[hA.vertex_offset + indices[fiA.x + (e + 1) % fiA.y]];
