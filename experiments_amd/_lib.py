@@ -107,6 +107,7 @@ SIGNATURES = {
     "mi_rb_get_transform": (I, [VP, I, VP, VP]),
     "mi_rb_get_stats": (I, [VP, ctypes.POINTER(Stats)]),
     "mi_rb_device_ptrs": (I, [VP, ctypes.POINTER(DeviceView)]),
+    "mi_rb_copy_transforms": (I, [VP, VP, I, I, VP]),
     "mi_rb_get_params": (I, [VP, ctypes.POINTER(Params)]),
     "mi_rb_read_buffer": (I, [VP, I, VP, ctypes.c_size_t, PI]),
     "mi_rb_save_state": (I, [VP, CP]),

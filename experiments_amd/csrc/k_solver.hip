@@ -1241,6 +1241,26 @@ hipError_t launch_solve(const DeviceWorld &d, const StepParams &p, hipStream_t s
     return hipGetLastError();
 }
 
+// Body transforms into a renderer's vertex buffer (copyTransformsToVBOKernel,
+// opencl/broadphase_benchmark/broadphaseKernel.cl:236-247): float4 (x, y, z,
+// 1) of body i at base + i, its quaternion at base + n + i.
+__global__ __launch_bounds__(256) void k_copy_transforms(int n, const float4 *__restrict__ pos,
+                                                         const float4 *__restrict__ quat,
+                                                         float4 *__restrict__ dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = pos[i];
+    dst[i] = make_float4(p.x, p.y, p.z, 1.f);
+    dst[n + i] = quat[i];
+}
+
+hipError_t launch_copy_transforms(const DeviceWorld &d, int n, float4 *dst, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_copy_transforms, dim3((n + 255) / 256), dim3(256), 0, s, n, d.pos, d.quat,
+                       dst);
+    return hipGetLastError();
+}
+
 hipError_t launch_integrate(const DeviceWorld &d, const StepParams &p, hipStream_t s) {
     if (p.n_bodies == 0) return hipSuccess;
     hipLaunchKernelGGL(k_integrate, dim3((p.n_bodies + 255) / 256), dim3(256), 0, s, p.n_bodies,

@@ -236,6 +236,7 @@ hipError_t launch_contact_total(const DeviceWorld &d, const StepParams &p, hipSt
 hipError_t launch_tile_bounds(const DeviceWorld &d, const StepParams &p, hipStream_t s);
 hipError_t launch_tiles(const DeviceWorld &d, const StepParams &p, hipStream_t s);
 hipError_t launch_solve(const DeviceWorld &d, const StepParams &p, hipStream_t s);
+hipError_t launch_copy_transforms(const DeviceWorld &d, int n, float4 *dst, hipStream_t s);
 hipError_t launch_integrate(const DeviceWorld &d, const StepParams &p, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
 hipError_t launch_mlp_warm(const DeviceWorld &d, const StepParams &p, hipStream_t s);

@@ -70,6 +70,7 @@ struct World {
     bool use_graph = false;
     hipGraphExec_t graph = nullptr;
     hipEvent_t ev[10] = {};
+    hipEvent_t ev_order = nullptr;  // orders work on a caller's stream after the world's steps
     bool timing = true;
     bool events_full = false;  // every stage event of the last eager step was recorded
     // learned warm-start weights (host copy; uploaded with the world)
@@ -501,6 +502,7 @@ extern "C" mi_rb_status mi_rb_create(const mi_rb_config *cfg, void **out_world) 
         return MI_RB_DEVICE;
     }
     for (auto &e : w->ev) hipEventCreate(&e);
+    hipEventCreateWithFlags(&w->ev_order, hipEventDisableTiming);
     *out_world = w;
     return MI_RB_OK;
 }
@@ -512,6 +514,7 @@ extern "C" mi_rb_status mi_rb_destroy(void *world) {
     if (w->stream) hipStreamSynchronize(w->stream);
     w->free_device();
     for (auto &e : w->ev) if (e) hipEventDestroy(e);
+    if (w->ev_order) hipEventDestroy(w->ev_order);
     if (w->stream) hipStreamDestroy(w->stream);
     delete w;
     return MI_RB_OK;
@@ -918,6 +921,28 @@ extern "C" mi_rb_status mi_rb_device_ptrs(void *world, mi_rb_device_view *out) {
     out->contacts = w->d.contacts;
     out->stream = w->stream;
     out->num_bodies = (int)w->pos.size();
+    return MI_RB_OK;
+}
+
+extern "C" mi_rb_status mi_rb_copy_transforms(void *world, void *d_vertex_buffer,
+                                              int start_offset, int capacity_float4,
+                                              void *stream) {
+    World *w = W(world);
+    if (!w || !d_vertex_buffer || start_offset < 0 || (start_offset & 3) || capacity_float4 < 0)
+        return MI_RB_INVALID_ARG;
+    if (!w->uploaded) return MI_RB_STATE;
+    const int n = (int)w->pos.size();
+    if ((int64_t)start_offset / 4 + 2 * (int64_t)n > capacity_float4) return MI_RB_CAPACITY;
+    hipSetDevice(w->device);
+    (void)hipGetLastError();
+    // the step's stream unless the caller names one (then ordered after the
+    // world's pending steps)
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : w->stream;
+    if (stream) {
+        RB_CHECK(hipEventRecord(w->ev_order, w->stream));
+        RB_CHECK(hipStreamWaitEvent(s, w->ev_order, 0));
+    }
+    RB_CHECK(launch_copy_transforms(w->d, n, static_cast<float4 *>(d_vertex_buffer) + start_offset / 4, s));
     return MI_RB_OK;
 }
 

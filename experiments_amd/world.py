@@ -99,6 +99,14 @@ class World:
                 arrs.append(np.ascontiguousarray(a))
         L.call("mi_rb_set_body_state", self.h, *[None if a is None else _ptr(a) for a in arrs])
 
+    def copy_transforms(self, dst, start_offset=0, stream=None):
+        """Body transforms into a device float4 buffer `dst` (a torch tensor
+        or a device pointer with `capacity` float4): (x, y, z, 1) of body i at
+        start_offset/4 + i, its quaternion at start_offset/4 + n + i."""
+        ptr, cap = (dst.data_ptr(), dst.numel() // 4) if hasattr(dst, "data_ptr") else dst
+        L.call("mi_rb_copy_transforms", self.h, ctypes.c_void_p(ptr), int(start_offset), int(cap),
+               ctypes.c_void_p(stream) if stream else None)
+
     def get_state(self):
         n = self.num_bodies
         out = [np.zeros((n, 4), np.float32) for _ in range(4)]

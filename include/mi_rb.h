@@ -201,6 +201,16 @@ mi_rb_status mi_rb_get_stats(void *world, mi_rb_stats *out);
 mi_rb_status mi_rb_get_params(void *world, mi_rb_params *out);
 /* CLPhysicsDemo::getBodiesGpu (CLPhysicsDemo.cpp:945): non-owning device view */
 mi_rb_status mi_rb_device_ptrs(void *world, mi_rb_device_view *out);
+/* copyTransformsToBVO (opencl/broadphase_benchmark/findPairsOpenCL.cpp:140-160,
+ * kernel broadphaseKernel.cl:236-247; called by gpu_rigidbody_pipeline2/main.cpp:274-278):
+ * writes the bodies' transforms into a device-visible vertex buffer of float4
+ * (a renderer's interop buffer): (x, y, z, 1) of body i at float4 index
+ * start_offset/4 + i and its quaternion (x, y, z, w) at start_offset/4 + n + i.
+ * start_offset counts floats (a multiple of 4); capacity_float4 is the
+ * buffer's size.  Enqueued on `stream` (NULL: the world's stream) after the
+ * world's pending steps; MI_RB_CAPACITY if the buffer is too small. */
+mi_rb_status mi_rb_copy_transforms(void *world, void *d_vertex_buffer, int start_offset,
+                                   int capacity_float4, void *stream);
 /* copy a step buffer (MI_RB_BUF_*) to host; *count = elements available */
 mi_rb_status mi_rb_read_buffer(void *world, int which, void *host, size_t max_bytes, int *count);
 /* versioned binary snapshot of shapes + bodies + velocities */

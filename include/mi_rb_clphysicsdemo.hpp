@@ -124,10 +124,23 @@ struct CLPhysicsDemo {
     void *world() const { return m_world; }
 
   private:
+    friend void copyTransformsToBVO(CLPhysicsDemo &demo, void *vbo, int positionOffset,
+                                    int vboFloat4Capacity);
     mi_rb_config m_cfg{};
     void *m_world = nullptr;
     mi_rb_device_view m_view{};
     mi_rb_status m_status = MI_RB_OK;
 };
+
+/* copyTransformsToBVO (opencl/broadphase_benchmark/findPairsOpenCL.cpp:140-160), the
+ * demo's render interop step (gpu_rigidbody_pipeline2/main.cpp:274-278): the
+ * reference takes its btFindPairsIO and the body cl_mem; here the demo and
+ * the device pointer of the vertex buffer (mapped for HIP by the renderer),
+ * the positions' offset in floats and the buffer's float4 capacity. */
+inline void copyTransformsToBVO(CLPhysicsDemo &demo, void *vbo, int positionOffset,
+                                int vboFloat4Capacity) {
+    demo.m_status = mi_rb_copy_transforms(demo.m_world, vbo, positionOffset, vboFloat4Capacity,
+                                          nullptr);
+}
 
 #endif

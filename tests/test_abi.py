@@ -89,6 +89,7 @@ def test_status_codes_without_device_work():
 
 SHIM_PROGRAM = r"""
 #include <cstdio>
+#include <hip/hip_runtime_api.h>
 #include "mi_rb_clphysicsdemo.hpp"
 int main() {
     // the reference demo's call sequence (gpu_rigidbody_pipeline2/main.cpp:173-189)
@@ -106,6 +107,17 @@ int main() {
     demo.registerPhysicsInstance(1.f, pos, orn, shape, 0);
     demo.writeBodiesToGpu();
     demo.stepSimulation();
+    // the render interop step (main.cpp:274-278): transforms into a device buffer
+    void *vbo = nullptr;
+    const mi_rb_device_view *view = demo.getBodiesGpu();
+    if (!view) return 1;
+    if (hipMalloc(&vbo, 2 * sizeof(float) * 4) != hipSuccess) return 1;
+    copyTransformsToBVO(demo, vbo, 0, 2);
+    if (demo.last_status() != MI_RB_OK) return 1;
+    float host[8];
+    if (hipMemcpy(host, vbo, sizeof(host), hipMemcpyDeviceToHost) != hipSuccess) return 1;
+    if (hipFree(vbo) != hipSuccess) return 1;
+    if (host[3] != 1.f || host[7] != 1.f) return 1;  // (x, y, z, 1), then quat w = 1
     demo.readbackBodiesToCpu();
     float p[4], q[4];
     demo.getObjectTransformFromCpu(p, q, 0);
@@ -124,8 +136,10 @@ def test_clphysicsdemo_shim_compiles_links_and_runs(tmp_path):
     src.write_text(SHIM_PROGRAM)
     exe = tmp_path / "demo"
     libdir = os.path.dirname(L.LIB_PATH)
-    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", str(src), "-I" + os.path.join(ROOT, "include"),
-                        "-L" + libdir, "-lmi_rb", "-Wl,-rpath," + libdir, "-o", str(exe)],
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-D__HIP_PLATFORM_AMD__", str(src),
+                        "-I" + os.path.join(ROOT, "include"), "-I/opt/rocm/include",
+                        "-L" + libdir, "-lmi_rb", "-Wl,-rpath," + libdir,
+                        "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib", "-o", str(exe)],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
