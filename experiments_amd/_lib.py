@@ -71,6 +71,10 @@ class DeviceView(ctypes.Structure):
                 ("stream", ctypes.c_void_p), ("num_bodies", ctypes.c_int)]
 
 
+class SapDesc(ctypes.Structure):
+    _fields_ = [("axis", ctypes.c_int), ("max_pairs", ctypes.c_int)]
+
+
 class GridDesc(ctypes.Structure):
     _fields_ = [("cell_size", ctypes.c_float), ("dims", ctypes.c_int * 3),
                 ("max_pairs", ctypes.c_int), ("max_pairs_per_body", ctypes.c_int),
@@ -137,6 +141,11 @@ SIGNATURES = {
     "mi_rb_grid_find_pairs": (I, [VP, VP, VP, VP, I, VP, VP, VP]),
     "mi_rb_grid_last_ms": (I, [VP, VP]),
     "mi_rb_grid_overflow": (I, [VP, PI]),
+    "mi_rb_sap_create": (I, [ctypes.POINTER(SapDesc), I, PVP]),
+    "mi_rb_sap_destroy": (I, [VP]),
+    "mi_rb_sap_find_pairs": (I, [VP, VP, VP, I, VP, VP, VP]),
+    "mi_rb_sap_last_ms": (I, [VP, VP]),
+    "mi_rb_sap_overflow": (I, [VP, PI]),
 }
 
 

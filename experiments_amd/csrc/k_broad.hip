@@ -659,6 +659,80 @@ __global__ __launch_bounds__(256) void k_pairs_group_big(const uint32_t *__restr
         }
     }
 }
+// ---- sweep and prune (btGpuSapBroadphase::calculateOverlappingPairs,
+// opencl/gpu_rigidbody_pipeline2/btGpuSapBroadphase.cpp:91-434; kernels
+// opencl/broadphase_benchmark/sap.cl:90-116, 290-320) ----
+
+// order-preserving map of a float to uint32 (sap.cl FloatFlip: negative
+// values have every bit flipped, others only the sign bit)
+__device__ __forceinline__ uint32_t float_flip(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return u ^ ((uint32_t)(-(int32_t)(u >> 31)) | 0x80000000u);
+}
+__device__ __forceinline__ float axis_of4(float4 v, int axis) {
+    return axis == 0 ? v.x : (axis == 1 ? v.y : v.z);
+}
+
+// (FloatFlip(min[axis]), index) per AABB; zeroes the per-body pair counts
+__global__ __launch_bounds__(256) void k_sap_keys(const float4 *__restrict__ mn, int n, int axis,
+                                                  uint2 *__restrict__ kv,
+                                                  uint32_t *__restrict__ counts,
+                                                  uint32_t *__restrict__ cursor,
+                                                  uint32_t *__restrict__ big_n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *big_n = 0u;
+    if (i >= n) return;
+    kv[i] = make_uint2(float_flip(axis_of4(mn[i], axis)), (uint32_t)i);
+    counts[i] = 0u;
+    cursor[i] = 0u;
+}
+
+// AABBs in sorted order, the original index in min.w (as int bits)
+__global__ __launch_bounds__(256) void k_sap_scatter(const uint2 *__restrict__ kv, int n,
+                                                     const float4 *__restrict__ mn,
+                                                     const float4 *__restrict__ mx,
+                                                     float4 *__restrict__ smn,
+                                                     float4 *__restrict__ smx) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t k = kv[i].y;
+    float4 a = mn[k];
+    a.w = __int_as_float((int)k);
+    smn[i] = a;
+    smx[i] = mx[k];
+}
+
+// One thread per sorted AABB i: the AABBs after it whose min along the axis
+// does not pass its max (computePairsKernelOriginal), each tested on all
+// three axes.  A pair (x, y), x > y, belongs to body x: WRITE = 0 counts it,
+// WRITE = 1 stores y in x's range of the scanned counts.
+template <int WRITE>
+__global__ __launch_bounds__(256) void k_sap_pairs(int n, int axis, const float4 *__restrict__ smn,
+                                                   const float4 *__restrict__ smx,
+                                                   uint32_t *__restrict__ counts,
+                                                   const uint32_t *__restrict__ offsets,
+                                                   uint32_t *__restrict__ cursor, int max_pairs,
+                                                   uint32_t *__restrict__ ys) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 amn = smn[i], amx = smx[i];
+    const int x = __float_as_int(amn.w);
+    const float lim = axis_of4(amx, axis);
+    for (int j = i + 1; j < n; j++) {
+        const float4 bmn = smn[j];
+        if (lim < axis_of4(bmn, axis)) break;
+        if (!aabb_overlap(amn, amx, bmn, smx[j])) continue;
+        const int y = __float_as_int(bmn.w);
+        const uint32_t hi = (uint32_t)(x > y ? x : y), lo = (uint32_t)(x > y ? y : x);
+        if (!WRITE) {
+            atomicAdd(&counts[hi], 1u);
+        } else {
+            const uint32_t pos = offsets[hi] + atomicAdd(&cursor[hi], 1u);
+            if (pos < (uint32_t)max_pairs) ys[pos] = lo;
+        }
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- launchers
@@ -772,6 +846,43 @@ hipError_t launch_hgrid_pairs(const float4 *mn, const float4 *mx, int n, float i
     if (n > 0)
         hipLaunchKernelGGL(k_hg_pairs<1>, dim3(hg_blocks), dim3(256), 0, s, n, kv, cell_start, occ,
                            level_used, smn, smx, g, counts, offsets, cursor, max_pairs, ys);
+    if (ev) hipEventRecord(ev[3], s);
+    hipLaunchKernelGGL(k_pairs_group, dim3(blocks), dim3(256), 0, s, n, counts, offsets, total,
+                       max_pairs, ys, pairs, num_pairs, overflow, big, big_n);
+    if (n > 0)
+        hipLaunchKernelGGL(k_pairs_group_big, dim3(1024), dim3(256), 0, s, counts, offsets,
+                           max_pairs, ys, pairs, big, big_n);
+    if (ev) hipEventRecord(ev[4], s);
+    return hipGetLastError();
+}
+
+// Sweep and prune of n AABBs along `axis`: keys, 32-bit sort, scatter, count
+// per body, scan, write, then each body's pairs ordered by y (as the grid).
+hipError_t launch_sap_pairs(const float4 *mn, const float4 *mx, int n, int axis, uint2 *kv,
+                            float4 *smn, float4 *smx, uint32_t *counts, uint32_t *offsets,
+                            uint32_t *cursor, uint32_t *big, uint32_t *total, uint32_t *ys,
+                            int max_pairs, int2 *pairs, int *num_pairs, int *overflow,
+                            const SortWork &sw, uint32_t *scan_status, hipStream_t s,
+                            hipEvent_t *ev) {
+    const int blocks = n > 0 ? (n + 255) / 256 : 1;
+    uint32_t *big_n = big + (n > 0 ? n : 0);
+    if (ev) hipEventRecord(ev[0], s);
+    hipLaunchKernelGGL(k_sap_keys, dim3(blocks), dim3(256), 0, s, mn, n, axis, kv, counts, cursor,
+                       big_n);
+    if (ev) hipEventRecord(ev[1], s);
+    hipError_t e = launch_radix_sort(kv, n, 32, 8, sw, s);
+    if (e != hipSuccess) return e;
+    if (ev) hipEventRecord(ev[2], s);
+    if (n > 0) {
+        hipLaunchKernelGGL(k_sap_scatter, dim3(blocks), dim3(256), 0, s, kv, n, mn, mx, smn, smx);
+        hipLaunchKernelGGL(k_sap_pairs<0>, dim3(blocks), dim3(256), 0, s, n, axis, smn, smx, counts,
+                           (const uint32_t *)nullptr, cursor, max_pairs, ys);
+    }
+    e = launch_exclusive_scan(counts, offsets, n, total, scan_status, s);
+    if (e != hipSuccess) return e;
+    if (n > 0)
+        hipLaunchKernelGGL(k_sap_pairs<1>, dim3(blocks), dim3(256), 0, s, n, axis, smn, smx, counts,
+                           offsets, cursor, max_pairs, ys);
     if (ev) hipEventRecord(ev[3], s);
     hipLaunchKernelGGL(k_pairs_group, dim3(blocks), dim3(256), 0, s, n, counts, offsets, total,
                        max_pairs, ys, pairs, num_pairs, overflow, big, big_n);

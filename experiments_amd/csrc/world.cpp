@@ -1258,6 +1258,100 @@ extern "C" mi_rb_status mi_rb_grid_find_pairs(void *grid, const void *d_aabb_min
     return MI_RB_OK;
 }
 
+namespace {
+struct Sap {
+    mi_rb_sap_desc desc;
+    int max_n;
+    uint2 *kv = nullptr;
+    float4 *smn = nullptr, *smx = nullptr;
+    uint32_t *ys = nullptr, *counts = nullptr, *offsets = nullptr, *cursor = nullptr,
+             *big = nullptr, *total = nullptr;
+    int *overflow = nullptr;
+    SortWork sw{};
+    uint32_t *scan_status = nullptr;
+    hipEvent_t ev[5] = {};
+    void release() {
+        void *ps[] = {kv, smn, smx, ys, counts, offsets, cursor, big, total, overflow, sw.tmp,
+                      sw.hist, sw.status, sw.tile_ctr, scan_status};
+        for (void *p : ps) if (p) hipFree(p);
+        for (auto &e : ev) if (e) hipEventDestroy(e);
+    }
+};
+}  // namespace
+
+extern "C" mi_rb_status mi_rb_sap_create(const mi_rb_sap_desc *desc, int max_aabbs, void **out_sap) {
+    if (!desc || !out_sap || max_aabbs <= 0 || desc->axis < 0 || desc->axis > 2 ||
+        desc->max_pairs <= 0)
+        return MI_RB_INVALID_ARG;
+    Sap *g = new Sap();
+    g->desc = *desc;
+    g->max_n = max_aabbs;
+    const int n = max_aabbs, P = desc->max_pairs;
+    bool ok = dalloc(&g->kv, n) == hipSuccess && dalloc(&g->smn, n) == hipSuccess &&
+              dalloc(&g->smx, n) == hipSuccess && dalloc(&g->ys, P) == hipSuccess &&
+              dalloc(&g->counts, n) == hipSuccess && dalloc(&g->offsets, n) == hipSuccess &&
+              dalloc(&g->cursor, n) == hipSuccess && dalloc(&g->big, (size_t)n + 1) == hipSuccess &&
+              dalloc(&g->total, 4) == hipSuccess && dalloc(&g->overflow, 4) == hipSuccess &&
+              dalloc(&g->sw.tmp, n) == hipSuccess && dalloc(&g->sw.hist, 8 * 256) == hipSuccess &&
+              dalloc(&g->sw.status, sort_status_words(n, 8)) == hipSuccess &&
+              dalloc(&g->sw.tile_ctr, 16) == hipSuccess &&
+              dalloc(&g->scan_status, 4 + 2 * (size_t)((n + 4095) / 4096)) == hipSuccess;
+    if (!ok) { g->release(); delete g; return MI_RB_DEVICE; }
+    g->sw.max_n = n;
+    g->sw.max_tiles = (n + 4095) / 4096;
+    hipMemset(g->overflow, 0, 16);
+    for (auto &e : g->ev) hipEventCreate(&e);
+    if (hipDeviceSynchronize() != hipSuccess) { g->release(); delete g; return MI_RB_DEVICE; }
+    *out_sap = g;
+    return MI_RB_OK;
+}
+
+extern "C" mi_rb_status mi_rb_sap_destroy(void *sap) {
+    Sap *g = static_cast<Sap *>(sap);
+    if (!g) return MI_RB_INVALID_ARG;
+    hipDeviceSynchronize();
+    g->release();
+    delete g;
+    return MI_RB_OK;
+}
+
+extern "C" mi_rb_status mi_rb_sap_find_pairs(void *sap, const void *d_aabb_min,
+                                             const void *d_aabb_max, int n, void *d_pairs,
+                                             void *d_num_pairs, void *stream) {
+    Sap *g = static_cast<Sap *>(sap);
+    if (!g || !d_aabb_min || !d_aabb_max || !d_pairs || !d_num_pairs || n < 0 || n > g->max_n)
+        return MI_RB_INVALID_ARG;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    (void)hipGetLastError();
+    RB_CHECK(launch_sap_pairs(static_cast<const float4 *>(d_aabb_min),
+                              static_cast<const float4 *>(d_aabb_max), n, g->desc.axis, g->kv,
+                              g->smn, g->smx, g->counts, g->offsets, g->cursor, g->big, g->total,
+                              g->ys, g->desc.max_pairs, static_cast<int2 *>(d_pairs),
+                              static_cast<int *>(d_num_pairs), g->overflow, g->sw,
+                              g->scan_status, s, g->ev));
+    return MI_RB_OK;
+}
+
+extern "C" mi_rb_status mi_rb_sap_overflow(void *sap, int *flags) {
+    Sap *g = static_cast<Sap *>(sap);
+    if (!g || !flags) return MI_RB_INVALID_ARG;
+    RB_CHECK(hipDeviceSynchronize());
+    RB_CHECK(hipMemcpy(flags, g->overflow, sizeof(int), hipMemcpyDeviceToHost));
+    RB_CHECK(hipMemset(g->overflow, 0, sizeof(int)));
+    return MI_RB_OK;
+}
+
+extern "C" mi_rb_status mi_rb_sap_last_ms(void *sap, float ms[4]) {
+    Sap *g = static_cast<Sap *>(sap);
+    if (!g || !ms) return MI_RB_INVALID_ARG;
+    RB_CHECK(hipEventSynchronize(g->ev[4]));
+    for (int k = 0; k < 4; k++) {
+        ms[k] = 0.f;
+        hipEventElapsedTime(&ms[k], g->ev[k], g->ev[k + 1]);
+    }
+    return MI_RB_OK;
+}
+
 extern "C" mi_rb_status mi_rb_grid_overflow(void *grid, int *flags) {
     Grid *g = static_cast<Grid *>(grid);
     if (!g || !flags) return MI_RB_INVALID_ARG;

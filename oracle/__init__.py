@@ -44,7 +44,7 @@ def lib():
             raise RuntimeError("oracle library not built: run `make -C oracle`")
         _lib = ctypes.CDLL(LIB_PATH)
         _lib.orc_exclusive_scan.restype = ctypes.c_uint32
-        for name in ("orc_find_pairs_grid", "orc_find_pairs_brute", "orc_sat",
+        for name in ("orc_find_pairs_grid", "orc_find_pairs_brute", "orc_sap_pairs", "orc_sat",
                      "orc_clip_hull_hull", "orc_clip_face", "orc_reduce_manifold",
                      "orc_narrowphase", "orc_color_contacts"):
             getattr(_lib, name).restype = ctypes.c_int
@@ -145,6 +145,15 @@ def find_pairs_brute(aabb_min, aabb_max, scene=None, max_pairs=1 << 22):
     sc = None if scene is None else np.ascontiguousarray(scene, np.int32)
     n = lib().orc_find_pairs_brute(ctypes.c_int(len(mn)), _p(mn), _p(mx), _p(sc),
                                    ctypes.c_int(max_pairs), _p(out))
+    return out[:min(n, max_pairs)].copy()
+
+
+def sap_pairs(aabb_min, aabb_max, axis=0, max_pairs=1 << 22):
+    """Sweep-and-prune pair set (orc_sap_pairs): (x > y) sorted by (x, y)."""
+    mn, mx = f4(aabb_min), f4(aabb_max)
+    out = np.zeros((max_pairs, 2), np.int32)
+    n = lib().orc_sap_pairs(ctypes.c_int(len(mn)), _p(mn), _p(mx), ctypes.c_int(axis),
+                            ctypes.c_int(max_pairs), _p(out))
     return out[:min(n, max_pairs)].copy()
 
 

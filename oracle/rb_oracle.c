@@ -322,6 +322,41 @@ int orc_find_pairs_brute(int n, const float *aabb_min, const float *aabb_max, co
     return finish_pairs(&b, max_pairs, pairs);
 }
 
+/* Sweep and prune (btGpuSapBroadphase::calculateOverlappingPairs,
+ * opencl/gpu_rigidbody_pipeline2/btGpuSapBroadphase.cpp:91-434, with the
+ * kernels flipFloatKernel / scatterKernel / computePairsKernelOriginal of
+ * opencl/broadphase_benchmark/sap.cl:90-116, 290-320): the AABB mins along
+ * `axis` as FloatFlip keys (negative floats: all bits flipped, others: the
+ * sign bit), a stable radix sort, then each AABB in sorted order tests the
+ * following ones until their min passes its max.  Every AABB takes part in
+ * the sweep (the reference pairs its large AABBs by brute force instead).
+ * Output as the grid: (x > y) sorted by (x, y).  Returns the pair count. */
+int orc_sap_pairs(int n, const float *aabb_min, const float *aabb_max, int axis, int max_pairs,
+                  int32_t *pairs)
+{
+    pairbuf b = {0, 0, 0, 0};
+    uint32_t *kv = (uint32_t *)malloc(sizeof(uint32_t) * 2 * (size_t)(n > 0 ? n : 1));
+    for (int i = 0; i < n; i++) {
+        uint32_t u;
+        memcpy(&u, aabb_min + 4 * i + axis, 4);
+        kv[2 * i] = u ^ ((uint32_t)(-(int32_t)(u >> 31)) | 0x80000000u);
+        kv[2 * i + 1] = (uint32_t)i;
+    }
+    orc_radix_sort_kv(kv, n, 32);
+    for (int s = 0; s < n; s++) {
+        const int i = (int)kv[2 * s + 1];
+        const float lim = aabb_max[4 * i + axis];
+        for (int t = s + 1; t < n; t++) {
+            const int j = (int)kv[2 * t + 1];
+            if (lim < aabb_min[4 * j + axis]) break;
+            if (aabb_overlap(aabb_min + 4 * i, aabb_max + 4 * i, aabb_min + 4 * j, aabb_max + 4 * j))
+                push_pair(&b, i > j ? i : j, i > j ? j : i);
+        }
+    }
+    free(kv);
+    return finish_pairs(&b, max_pairs, pairs);
+}
+
 /* mtMul (broadphaseKernel.cl:130-148): ans[i][j] = dot3(a.row[i], b.col[j]) */
 static void mt_mul(const f4 a[3], const f4 b[3], f4 out[3])
 {

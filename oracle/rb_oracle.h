@@ -74,6 +74,8 @@ int orc_find_pairs_grid(int n, const uint32_t *sorted_kv, const float *aabb_min,
                         const int dims[3], int cell_bits, int max_pairs, int32_t *pairs);
 int orc_find_pairs_brute(int n, const float *aabb_min, const float *aabb_max, const int *scene,
                          int max_pairs, int32_t *pairs);
+int orc_sap_pairs(int n, const float *aabb_min, const float *aabb_max, int axis, int max_pairs,
+                  int32_t *pairs);
 void orc_setup_bodies(int n, float *linvel, const float *angvel, const float *quat,
                       const float *inv_mass, const float *inv_inertia_local,
                       float *inv_inertia_world, const float gravity[3], float dt);
