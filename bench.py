@@ -126,6 +126,44 @@ def per_scene_baseline(scene, steps, warmup, max_scenes=None):
             "host_s": round(host_s, 4), "gpu_ms_sum": round(total_ms, 3)}
 
 
+def sap_on_world(w, steps=20):
+    """The sweep-and-prune pair search (mi_rb_sap_*) on the world's current
+    AABBs (after the stage-timing step), timed with its HIP events; checked
+    against the world's grid pair list of the same AABBs."""
+    import ctypes
+    import torch
+    from experiments_amd import _lib as L
+    mn, mx = w.aabbs()
+    n = len(mn)
+    dmn = torch.from_numpy(np.ascontiguousarray(mn)).cuda()
+    dmx = torch.from_numpy(np.ascontiguousarray(mx)).cuda()
+    # the grid's overflow list (the ground) as the large AABBs
+    dl = torch.from_numpy(w.read(L.BUF_IS_LARGE, np.int32)).cuda()
+    max_pairs = 1 << 23
+    desc = L.SapDesc(0, max_pairs)
+    h = ctypes.c_void_p()
+    L.call("mi_rb_sap_create", ctypes.byref(desc), n, ctypes.byref(h))
+    pairs = torch.zeros((max_pairs, 2), dtype=torch.int32, device="cuda")
+    num = torch.zeros(1, dtype=torch.int32, device="cuda")
+    P = lambda t: ctypes.c_void_p(t.data_ptr())
+    ms = np.zeros(4, np.float32)
+    acc = np.zeros(4)
+    for k in range(steps + 2):
+        L.call("mi_rb_sap_find_pairs", h, P(dmn), P(dmx), P(dl), n, P(pairs), P(num), None)
+        L.call("mi_rb_sap_last_ms", h, ms.ctypes.data_as(ctypes.c_void_p))
+        if k >= 2:
+            acc += ms
+    acc /= steps
+    k = int(num.cpu()[0])
+    same = bool(np.array_equal(pairs[:k].cpu().numpy(), w.pairs()))
+    L.lib().mi_rb_sap_destroy(h)
+    return {"aabbs": n, "axis": "x", "large": int(dl.sum()), "pairs": k,
+            "equal_to_grid_pairs": same,
+            "phase_ms": {"keys": round(acc[0], 4), "sort": round(acc[1], 4),
+                         "sweep_count_scan_write": round(acc[2], 4), "pair_order": round(acc[3], 4)},
+            "ms": round(float(acc.sum()), 4)}
+
+
 def broadphase_1m(steps=100, warmup=5):
     """Broadphase stress (1,048,576 spheres and boxes, radii log-uniform in
     [0.25, 2], 400^3 domain, level-0 cell = 2 x the median radius, 4 grid
@@ -318,6 +356,7 @@ def main():
         out["per_scene_launch_sets"] = per_scene
     if not args.no_broad1m and args.workload == "hulls64k":
         if w is not None:
+            out["sap_broadphase"] = sap_on_world(w)
             w.close()
             w = None
         out["broadphase_1m"] = broadphase_1m()

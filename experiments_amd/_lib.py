@@ -143,7 +143,7 @@ SIGNATURES = {
     "mi_rb_grid_overflow": (I, [VP, PI]),
     "mi_rb_sap_create": (I, [ctypes.POINTER(SapDesc), I, PVP]),
     "mi_rb_sap_destroy": (I, [VP]),
-    "mi_rb_sap_find_pairs": (I, [VP, VP, VP, I, VP, VP, VP]),
+    "mi_rb_sap_find_pairs": (I, [VP, VP, VP, VP, I, VP, VP, VP]),
     "mi_rb_sap_last_ms": (I, [VP, VP]),
     "mi_rb_sap_overflow": (I, [VP, PI]),
 }

@@ -466,7 +466,7 @@ __global__ __launch_bounds__(256) void k_hg_hash(const float4 *__restrict__ mn,
                                                  uint32_t *__restrict__ cursor,
                                                  uint32_t *__restrict__ big_n, int *overflow) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) *big_n = 0u;
+    if (i == 0) { big_n[0] = 0u; big_n[1] = 0u; }
     if (i >= n) return;
     counts[i] = 0u;
     cursor[i] = 0u;
@@ -589,15 +589,16 @@ constexpr int GROUP_SMALL = 8;
 
 // One thread per body: its range sorted by y (an odd-even transposition
 // network over GROUP_SMALL registers) and written as int2 (x, y).  Longer
-// ranges go to a list for k_pairs_group_big.  Also the pair count and the
-// capacity flag.
+// ranges go to a list for k_pairs_group_big (from the front of big[]), those
+// longer than huge_min to one for k_pairs_group_huge (from the back).  Also
+// the pair count and the capacity flag.
 __global__ __launch_bounds__(256) void k_pairs_group(int n, const uint32_t *__restrict__ counts,
                                                      const uint32_t *__restrict__ offsets,
                                                      const uint32_t *__restrict__ total,
                                                      int max_pairs, const uint32_t *__restrict__ ys,
                                                      int2 *__restrict__ pairs, int *num_pairs,
                                                      int *overflow, uint32_t *__restrict__ big,
-                                                     uint32_t *__restrict__ big_n) {
+                                                     uint32_t *__restrict__ big_n, int huge_min) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x == 0) {
         uint32_t tt = *total;
@@ -608,6 +609,10 @@ __global__ __launch_bounds__(256) void k_pairs_group(int n, const uint32_t *__re
     uint32_t off;
     const uint32_t len = pair_range(counts, offsets, max_pairs, x, off);
     if (len == 0) return;
+    if (len > (uint32_t)huge_min) {
+        big[n - 1 - (int)atomicAdd(big_n + 1, 1u)] = (uint32_t)x;  // from the back
+        return;
+    }
     if (len > (uint32_t)GROUP_SMALL) {
         big[atomicAdd(big_n, 1u)] = (uint32_t)x;
         return;
@@ -659,6 +664,60 @@ __global__ __launch_bounds__(256) void k_pairs_group_big(const uint32_t *__restr
         }
     }
 }
+
+// Bodies with more than huge_min pairs (a ground touching thousands of
+// bodies), one workgroup each: the y of the range are distinct indices below
+// x, so one bit per index in LDS (cleared, set), per-thread chunk counts and
+// their prefix give every y its rank.  Dynamic LDS: (n + 31) / 32 words.
+constexpr int HUGE_BLOCK = 1024;
+__global__ __launch_bounds__(HUGE_BLOCK) void k_pairs_group_huge(
+    int n, const uint32_t *__restrict__ counts, const uint32_t *__restrict__ offsets,
+    int max_pairs, const uint32_t *__restrict__ ys, int2 *__restrict__ pairs,
+    const uint32_t *__restrict__ big, const uint32_t *__restrict__ big_n) {
+    extern __shared__ uint32_t bm[];
+    __shared__ uint32_t cbase[HUGE_BLOCK];
+    const int tid = threadIdx.x;
+    const uint32_t nh = big_n[1];
+    for (uint32_t b = blockIdx.x; b < nh; b += gridDim.x) {
+        const int x = (int)big[n - 1 - (int)b];
+        uint32_t off;
+        const uint32_t len = pair_range(counts, offsets, max_pairs, x, off);
+        const int W = (x + 31) >> 5;
+        const int chunk = (W + HUGE_BLOCK - 1) / HUGE_BLOCK;
+        __syncthreads();
+        for (int w = tid; w < W; w += HUGE_BLOCK) bm[w] = 0u;
+        __syncthreads();
+        for (uint32_t i = tid; i < len; i += HUGE_BLOCK) {
+            const uint32_t y = ys[off + i];
+            atomicOr(&bm[y >> 5], 1u << (y & 31u));
+        }
+        __syncthreads();
+        uint32_t c = 0;
+        for (int w = tid * chunk; w < (tid + 1) * chunk && w < W; w++) c += __popc(bm[w]);
+        cbase[tid] = c;
+        __syncthreads();
+        // exclusive prefix of the chunk counts (Hillis-Steele in place)
+        for (int o = 1; o < HUGE_BLOCK; o <<= 1) {
+            const uint32_t v = tid >= o ? cbase[tid - o] : 0u;
+            __syncthreads();
+            cbase[tid] += v;
+            __syncthreads();
+        }
+        const uint32_t excl = cbase[tid] - c;
+        __syncthreads();
+        cbase[tid] = excl;
+        __syncthreads();
+        for (uint32_t i = tid; i < len; i += HUGE_BLOCK) {
+            const uint32_t y = ys[off + i];
+            const int w = (int)(y >> 5), ck = w / chunk;
+            uint32_t r = cbase[ck];
+            for (int k = ck * chunk; k < w; k++) r += __popc(bm[k]);
+            r += __popc(bm[w] & ((1u << (y & 31u)) - 1u));
+            pairs[off + r] = make_int2(x, (int)y);
+        }
+    }
+}
+
 // ---- sweep and prune (btGpuSapBroadphase::calculateOverlappingPairs,
 // opencl/gpu_rigidbody_pipeline2/btGpuSapBroadphase.cpp:91-434; kernels
 // opencl/broadphase_benchmark/sap.cl:90-116, 290-320) ----
@@ -673,18 +732,60 @@ __device__ __forceinline__ float axis_of4(float4 v, int axis) {
     return axis == 0 ? v.x : (axis == 1 ? v.y : v.z);
 }
 
-// (FloatFlip(min[axis]), index) per AABB; zeroes the per-body pair counts
-__global__ __launch_bounds__(256) void k_sap_keys(const float4 *__restrict__ mn, int n, int axis,
+// (FloatFlip(min[axis]), index) per AABB; large AABBs (is_large[i] != 0)
+// sort last and go to a list of their own (lg, count lg_n, zero at launch).
+// Zeroes the per-body pair counts.
+__global__ __launch_bounds__(256) void k_sap_keys(const float4 *__restrict__ mn,
+                                                  const int *__restrict__ is_large, int n, int axis,
                                                   uint2 *__restrict__ kv,
                                                   uint32_t *__restrict__ counts,
                                                   uint32_t *__restrict__ cursor,
-                                                  uint32_t *__restrict__ big_n) {
+                                                  uint32_t *__restrict__ big_n,
+                                                  uint32_t *__restrict__ lg,
+                                                  uint32_t *__restrict__ lg_n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) *big_n = 0u;
+    if (i == 0) { big_n[0] = 0u; big_n[1] = 0u; }
     if (i >= n) return;
-    kv[i] = make_uint2(float_flip(axis_of4(mn[i], axis)), (uint32_t)i);
+    const bool large = is_large && is_large[i] != 0;
+    kv[i] = make_uint2(large ? 0xffffffffu : float_flip(axis_of4(mn[i], axis)), (uint32_t)i);
+    if (large) lg[atomicAdd(lg_n, 1u)] = (uint32_t)i;
     counts[i] = 0u;
     cursor[i] = 0u;
+}
+
+// Pairs of the large AABBs (sap2Kernel's role, btGpuSapBroadphase.cpp:311-333):
+// one thread per sorted AABB j tests it against every large AABB L; a pair
+// of two large AABBs is taken by the one of lower index only.
+template <int WRITE>
+__global__ __launch_bounds__(256) void k_sap_large(int n, const float4 *__restrict__ mn,
+                                                   const float4 *__restrict__ mx,
+                                                   const float4 *__restrict__ smn,
+                                                   const float4 *__restrict__ smx,
+                                                   const uint32_t *__restrict__ lg,
+                                                   const uint32_t *__restrict__ lg_n,
+                                                   uint32_t *__restrict__ counts,
+                                                   const uint32_t *__restrict__ offsets,
+                                                   uint32_t *__restrict__ cursor, int max_pairs,
+                                                   uint32_t *__restrict__ ys) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int nl = (int)*lg_n;
+    if (nl == 0) return;
+    const int ns = n - nl;
+    const float4 bmn = smn[j], bmx = smx[j];
+    const int y = __float_as_int(bmn.w);
+    for (int q = 0; q < nl; q++) {
+        const int L = (int)lg[q];
+        if (L == y || (j >= ns && y > L)) continue;
+        if (!aabb_overlap(mn[L], mx[L], bmn, bmx)) continue;
+        const uint32_t hi = (uint32_t)(L > y ? L : y), lo = (uint32_t)(L > y ? y : L);
+        if (!WRITE) {
+            atomicAdd(&counts[hi], 1u);
+        } else {
+            const uint32_t pos = offsets[hi] + atomicAdd(&cursor[hi], 1u);
+            if (pos < (uint32_t)max_pairs) ys[pos] = lo;
+        }
+    }
 }
 
 // AABBs in sorted order, the original index in min.w (as int bits)
@@ -702,17 +803,19 @@ __global__ __launch_bounds__(256) void k_sap_scatter(const uint2 *__restrict__ k
     smx[i] = mx[k];
 }
 
-// One thread per sorted AABB i: the AABBs after it whose min along the axis
-// does not pass its max (computePairsKernelOriginal), each tested on all
-// three axes.  A pair (x, y), x > y, belongs to body x: WRITE = 0 counts it,
+// One thread per sorted small AABB i: the small AABBs after it whose min
+// along the axis does not pass its max (computePairsKernelOriginal), each
+// tested on all three axes.  A pair (x, y), x > y, belongs to body x: WRITE = 0 counts it,
 // WRITE = 1 stores y in x's range of the scanned counts.
 template <int WRITE>
 __global__ __launch_bounds__(256) void k_sap_pairs(int n, int axis, const float4 *__restrict__ smn,
                                                    const float4 *__restrict__ smx,
+                                                   const uint32_t *__restrict__ lg_n,
                                                    uint32_t *__restrict__ counts,
                                                    const uint32_t *__restrict__ offsets,
                                                    uint32_t *__restrict__ cursor, int max_pairs,
                                                    uint32_t *__restrict__ ys) {
+    n -= (int)*lg_n;  // the large AABBs sorted last do not sweep
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 amn = smn[i], amx = smx[i];
@@ -805,6 +908,29 @@ hipError_t launch_emit_pairs(const DeviceWorld &d, const StepParams &p, hipStrea
     return hipGetLastError();
 }
 
+// Each body's pair range ordered by y: ranges of up to GROUP_SMALL pairs in
+// registers, up to HUGE_MIN by rank in LDS chunks, longer ones (a ground)
+// through an LDS bitmap of the indices below the body when it fits.
+constexpr int HUGE_MIN = 4096;
+constexpr size_t HUGE_LDS_MAX = 60 * 1024;
+static void launch_pairs_group(int n, uint32_t *counts, uint32_t *offsets, uint32_t *total,
+                               int max_pairs, uint32_t *ys, int2 *pairs, int *num_pairs,
+                               int *overflow, uint32_t *big, uint32_t *big_n, int blocks,
+                               hipStream_t s) {
+    const size_t bm_bytes = sizeof(uint32_t) * (((size_t)n + 31) / 32 + 1);
+    const bool huge = bm_bytes <= HUGE_LDS_MAX;
+    hipLaunchKernelGGL(k_pairs_group, dim3(blocks), dim3(256), 0, s, n, counts, offsets, total,
+                       max_pairs, ys, pairs, num_pairs, overflow, big, big_n,
+                       huge ? HUGE_MIN : INT_MAX);
+    if (n > 0) {
+        hipLaunchKernelGGL(k_pairs_group_big, dim3(1024), dim3(256), 0, s, counts, offsets,
+                           max_pairs, ys, pairs, big, big_n);
+        if (huge)
+            hipLaunchKernelGGL(k_pairs_group_huge, dim3(64), dim3(HUGE_BLOCK), bm_bytes, s, n,
+                               counts, offsets, max_pairs, ys, pairs, big, big_n);
+    }
+}
+
 // Hierarchical-grid pair search of n AABBs: hash, sort, cell starts, count
 // per body, scan, write, then each body's pairs ordered by y.
 hipError_t launch_hgrid_pairs(const float4 *mn, const float4 *mx, int n, float inv_cell0,
@@ -847,48 +973,50 @@ hipError_t launch_hgrid_pairs(const float4 *mn, const float4 *mx, int n, float i
         hipLaunchKernelGGL(k_hg_pairs<1>, dim3(hg_blocks), dim3(256), 0, s, n, kv, cell_start, occ,
                            level_used, smn, smx, g, counts, offsets, cursor, max_pairs, ys);
     if (ev) hipEventRecord(ev[3], s);
-    hipLaunchKernelGGL(k_pairs_group, dim3(blocks), dim3(256), 0, s, n, counts, offsets, total,
-                       max_pairs, ys, pairs, num_pairs, overflow, big, big_n);
-    if (n > 0)
-        hipLaunchKernelGGL(k_pairs_group_big, dim3(1024), dim3(256), 0, s, counts, offsets,
-                           max_pairs, ys, pairs, big, big_n);
+    launch_pairs_group(n, counts, offsets, total, max_pairs, ys, pairs, num_pairs, overflow, big,
+                       big_n, blocks, s);
     if (ev) hipEventRecord(ev[4], s);
     return hipGetLastError();
 }
 
 // Sweep and prune of n AABBs along `axis`: keys, 32-bit sort, scatter, count
 // per body, scan, write, then each body's pairs ordered by y (as the grid).
-hipError_t launch_sap_pairs(const float4 *mn, const float4 *mx, int n, int axis, uint2 *kv,
-                            float4 *smn, float4 *smx, uint32_t *counts, uint32_t *offsets,
-                            uint32_t *cursor, uint32_t *big, uint32_t *total, uint32_t *ys,
-                            int max_pairs, int2 *pairs, int *num_pairs, int *overflow,
-                            const SortWork &sw, uint32_t *scan_status, hipStream_t s,
-                            hipEvent_t *ev) {
+hipError_t launch_sap_pairs(const float4 *mn, const float4 *mx, const int *is_large, int n,
+                            int axis, uint2 *kv, float4 *smn, float4 *smx, uint32_t *counts,
+                            uint32_t *offsets, uint32_t *cursor, uint32_t *big, uint32_t *lg,
+                            uint32_t *total, uint32_t *ys, int max_pairs, int2 *pairs,
+                            int *num_pairs, int *overflow, const SortWork &sw,
+                            uint32_t *scan_status, hipStream_t s, hipEvent_t *ev) {
     const int blocks = n > 0 ? (n + 255) / 256 : 1;
     uint32_t *big_n = big + (n > 0 ? n : 0);
+    uint32_t *lg_n = lg + (n > 0 ? n : 0);
     if (ev) hipEventRecord(ev[0], s);
-    hipLaunchKernelGGL(k_sap_keys, dim3(blocks), dim3(256), 0, s, mn, n, axis, kv, counts, cursor,
-                       big_n);
+    hipError_t e = launch_fill_u32(lg_n, 0u, 1, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_sap_keys, dim3(blocks), dim3(256), 0, s, mn, is_large, n, axis, kv, counts,
+                       cursor, big_n, lg, lg_n);
     if (ev) hipEventRecord(ev[1], s);
-    hipError_t e = launch_radix_sort(kv, n, 32, 8, sw, s);
+    e = launch_radix_sort(kv, n, 32, 8, sw, s);
     if (e != hipSuccess) return e;
     if (ev) hipEventRecord(ev[2], s);
     if (n > 0) {
         hipLaunchKernelGGL(k_sap_scatter, dim3(blocks), dim3(256), 0, s, kv, n, mn, mx, smn, smx);
-        hipLaunchKernelGGL(k_sap_pairs<0>, dim3(blocks), dim3(256), 0, s, n, axis, smn, smx, counts,
-                           (const uint32_t *)nullptr, cursor, max_pairs, ys);
+        hipLaunchKernelGGL(k_sap_pairs<0>, dim3(blocks), dim3(256), 0, s, n, axis, smn, smx, lg_n,
+                           counts, (const uint32_t *)nullptr, cursor, max_pairs, ys);
+        hipLaunchKernelGGL(k_sap_large<0>, dim3(blocks), dim3(256), 0, s, n, mn, mx, smn, smx, lg,
+                           lg_n, counts, (const uint32_t *)nullptr, cursor, max_pairs, ys);
     }
     e = launch_exclusive_scan(counts, offsets, n, total, scan_status, s);
     if (e != hipSuccess) return e;
-    if (n > 0)
-        hipLaunchKernelGGL(k_sap_pairs<1>, dim3(blocks), dim3(256), 0, s, n, axis, smn, smx, counts,
-                           offsets, cursor, max_pairs, ys);
+    if (n > 0) {
+        hipLaunchKernelGGL(k_sap_pairs<1>, dim3(blocks), dim3(256), 0, s, n, axis, smn, smx, lg_n,
+                           counts, offsets, cursor, max_pairs, ys);
+        hipLaunchKernelGGL(k_sap_large<1>, dim3(blocks), dim3(256), 0, s, n, mn, mx, smn, smx, lg,
+                           lg_n, counts, offsets, cursor, max_pairs, ys);
+    }
     if (ev) hipEventRecord(ev[3], s);
-    hipLaunchKernelGGL(k_pairs_group, dim3(blocks), dim3(256), 0, s, n, counts, offsets, total,
-                       max_pairs, ys, pairs, num_pairs, overflow, big, big_n);
-    if (n > 0)
-        hipLaunchKernelGGL(k_pairs_group_big, dim3(1024), dim3(256), 0, s, counts, offsets,
-                           max_pairs, ys, pairs, big, big_n);
+    launch_pairs_group(n, counts, offsets, total, max_pairs, ys, pairs, num_pairs, overflow, big,
+                       big_n, blocks, s);
     if (ev) hipEventRecord(ev[4], s);
     return hipGetLastError();
 }

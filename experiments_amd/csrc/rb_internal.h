@@ -251,11 +251,11 @@ hipError_t launch_hgrid_pairs(const float4 *mn, const float4 *mx, int n, float i
                               uint32_t *big, uint32_t *total, uint32_t *ys, int max_pairs,
                               int2 *pairs, int *num_pairs, int *overflow, const SortWork &sw,
                               uint32_t *scan_status, hipStream_t s, hipEvent_t *ev);
-hipError_t launch_sap_pairs(const float4 *mn, const float4 *mx, int n, int axis, uint2 *kv,
-                            float4 *smn, float4 *smx, uint32_t *counts, uint32_t *offsets,
-                            uint32_t *cursor, uint32_t *big, uint32_t *total, uint32_t *ys,
-                            int max_pairs, int2 *pairs, int *num_pairs, int *overflow,
-                            const SortWork &sw, uint32_t *scan_status, hipStream_t s,
-                            hipEvent_t *ev);
+hipError_t launch_sap_pairs(const float4 *mn, const float4 *mx, const int *is_large, int n,
+                            int axis, uint2 *kv, float4 *smn, float4 *smx, uint32_t *counts,
+                            uint32_t *offsets, uint32_t *cursor, uint32_t *big, uint32_t *lg,
+                            uint32_t *total, uint32_t *ys, int max_pairs, int2 *pairs,
+                            int *num_pairs, int *overflow, const SortWork &sw,
+                            uint32_t *scan_status, hipStream_t s, hipEvent_t *ev);
 
 int rb_num_cus();

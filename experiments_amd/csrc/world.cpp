@@ -1207,7 +1207,7 @@ extern "C" mi_rb_status mi_rb_grid_create(const mi_rb_grid_desc *desc, int max_a
     size_t ncells = (size_t)1 << (g->cell_bits + 2);
     int n = max_aabbs, P = desc->max_pairs;
     bool ok = dalloc(&g->kv, n) == hipSuccess && dalloc(&g->ys, P) == hipSuccess &&
-              dalloc(&g->cursor, n) == hipSuccess && dalloc(&g->big, (size_t)n + 1) == hipSuccess &&
+              dalloc(&g->cursor, n) == hipSuccess && dalloc(&g->big, (size_t)n + 2) == hipSuccess &&
               dalloc(&g->cell_start, ncells) == hipSuccess &&
               dalloc(&g->occ, ncells / 32 + 4) == hipSuccess && dalloc(&g->smn, n) == hipSuccess &&
               dalloc(&g->smx, n) == hipSuccess && dalloc(&g->counts, n) == hipSuccess &&
@@ -1265,13 +1265,13 @@ struct Sap {
     uint2 *kv = nullptr;
     float4 *smn = nullptr, *smx = nullptr;
     uint32_t *ys = nullptr, *counts = nullptr, *offsets = nullptr, *cursor = nullptr,
-             *big = nullptr, *total = nullptr;
+             *big = nullptr, *lg = nullptr, *total = nullptr;
     int *overflow = nullptr;
     SortWork sw{};
     uint32_t *scan_status = nullptr;
     hipEvent_t ev[5] = {};
     void release() {
-        void *ps[] = {kv, smn, smx, ys, counts, offsets, cursor, big, total, overflow, sw.tmp,
+        void *ps[] = {kv, smn, smx, ys, counts, offsets, cursor, big, lg, total, overflow, sw.tmp,
                       sw.hist, sw.status, sw.tile_ctr, scan_status};
         for (void *p : ps) if (p) hipFree(p);
         for (auto &e : ev) if (e) hipEventDestroy(e);
@@ -1290,7 +1290,8 @@ extern "C" mi_rb_status mi_rb_sap_create(const mi_rb_sap_desc *desc, int max_aab
     bool ok = dalloc(&g->kv, n) == hipSuccess && dalloc(&g->smn, n) == hipSuccess &&
               dalloc(&g->smx, n) == hipSuccess && dalloc(&g->ys, P) == hipSuccess &&
               dalloc(&g->counts, n) == hipSuccess && dalloc(&g->offsets, n) == hipSuccess &&
-              dalloc(&g->cursor, n) == hipSuccess && dalloc(&g->big, (size_t)n + 1) == hipSuccess &&
+              dalloc(&g->cursor, n) == hipSuccess && dalloc(&g->big, (size_t)n + 2) == hipSuccess &&
+              dalloc(&g->lg, (size_t)n + 1) == hipSuccess &&
               dalloc(&g->total, 4) == hipSuccess && dalloc(&g->overflow, 4) == hipSuccess &&
               dalloc(&g->sw.tmp, n) == hipSuccess && dalloc(&g->sw.hist, 8 * 256) == hipSuccess &&
               dalloc(&g->sw.status, sort_status_words(n, 8)) == hipSuccess &&
@@ -1316,17 +1317,19 @@ extern "C" mi_rb_status mi_rb_sap_destroy(void *sap) {
 }
 
 extern "C" mi_rb_status mi_rb_sap_find_pairs(void *sap, const void *d_aabb_min,
-                                             const void *d_aabb_max, int n, void *d_pairs,
-                                             void *d_num_pairs, void *stream) {
+                                             const void *d_aabb_max, const void *d_is_large,
+                                             int n, void *d_pairs, void *d_num_pairs,
+                                             void *stream) {
     Sap *g = static_cast<Sap *>(sap);
     if (!g || !d_aabb_min || !d_aabb_max || !d_pairs || !d_num_pairs || n < 0 || n > g->max_n)
         return MI_RB_INVALID_ARG;
     hipStream_t s = static_cast<hipStream_t>(stream);
     (void)hipGetLastError();
     RB_CHECK(launch_sap_pairs(static_cast<const float4 *>(d_aabb_min),
-                              static_cast<const float4 *>(d_aabb_max), n, g->desc.axis, g->kv,
-                              g->smn, g->smx, g->counts, g->offsets, g->cursor, g->big, g->total,
-                              g->ys, g->desc.max_pairs, static_cast<int2 *>(d_pairs),
+                              static_cast<const float4 *>(d_aabb_max),
+                              static_cast<const int *>(d_is_large), n, g->desc.axis, g->kv,
+                              g->smn, g->smx, g->counts, g->offsets, g->cursor, g->big, g->lg,
+                              g->total, g->ys, g->desc.max_pairs, static_cast<int2 *>(d_pairs),
                               static_cast<int *>(d_num_pairs), g->overflow, g->sw,
                               g->scan_status, s, g->ev));
     return MI_RB_OK;

@@ -310,9 +310,9 @@ mi_rb_status mi_rb_grid_last_ms(void *grid, float ms[4]);
  * opencl/broadphase_benchmark/sap.cl:90-116, 290-320): AABB mins along `axis`
  * as FloatFlip keys, a stable 32-bit radix sort, then each AABB tests the
  * following ones until their min passes its max.  Unlike the reference
- * (atomic append, large AABBs on a separate brute-force list) every AABB
- * takes part in the sweep and the output is the same as the grid's: every
- * overlapping pair once as int2 (x, y), x > y, sorted by (x, y).  Sticky
+ * (atomic append) the output is the same as the grid's: every overlapping
+ * pair once as int2 (x, y), x > y, sorted by (x, y).  As in the reference,
+ * AABBs flagged large are tested against all others instead of sweeping.  Sticky
  * MI_RB_OVERFLOW_PAIRS when max_pairs is exceeded. */
 typedef struct mi_rb_sap_desc {
     int axis;       /* 0, 1 or 2 (the reference sweeps along x) */
@@ -320,10 +320,13 @@ typedef struct mi_rb_sap_desc {
 } mi_rb_sap_desc;
 mi_rb_status mi_rb_sap_create(const mi_rb_sap_desc *desc, int max_aabbs, void **out_sap);
 mi_rb_status mi_rb_sap_destroy(void *sap);
-/* d_aabb_min/max: float4[n] device arrays; d_pairs int2[max_pairs];
+/* d_aabb_min/max: float4[n] device arrays; d_is_large: int[n] device array or
+ * NULL, nonzero for AABBs that do not sweep but are tested against every AABB
+ * (the reference's large proxies, e.g. a ground); d_pairs int2[max_pairs];
  * d_num_pairs int[1]; enqueued on `stream` (NULL: the null stream) */
 mi_rb_status mi_rb_sap_find_pairs(void *sap, const void *d_aabb_min, const void *d_aabb_max,
-                                  int n, void *d_pairs, void *d_num_pairs, void *stream);
+                                  const void *d_is_large, int n, void *d_pairs,
+                                  void *d_num_pairs, void *stream);
 mi_rb_status mi_rb_sap_overflow(void *sap, int *flags);
 /* milliseconds of the last call: keys, sort, sweep (count, scan, write), pair order */
 mi_rb_status mi_rb_sap_last_ms(void *sap, float ms[4]);
