@@ -28,12 +28,13 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
-# HBM bytes per k_tile_solve dispatch, 2.0 x FETCH_SIZE + 1.0 x WRITE_SIZE from
-# rocprofv3 --pmc runs at the state the stage timing uses (copy-kernel
-# calibration and commands in profiles/hbm_traffic_hulls64k.md and
-# profiles/hbm_traffic_batched4k.md)
-SOLVE_TRAFFIC_BYTES = {"hulls64k": (4997 * 2.0 + 1793) * 1024,
-                       "batched4k": (96647 * 2.0 + 50051) * 1024}
+# HBM bytes per k_tile_solve dispatch, 2.0 x FETCH_SIZE + 1.0 x WRITE_SIZE (KiB)
+# from rocprofv3 --pmc runs over the stage-timing steps of the default command
+# (copy-kernel calibration and commands in profiles/hbm_traffic_hulls64k.md and
+# profiles/hbm_traffic_batched4k.md).  Spatial tiles: the mean of the normal-
+# and the friction-sweep dispatches (40 of each per step).
+SOLVE_TRAFFIC_BYTES = {"hulls64k": ((5371 + 5006) * 2.0 + 1478 + 1769) / 2 * 1024,
+                       "batched4k": (1165580 * 2.0 + 233047) * 1024}
 
 # algorithmic bytes per unit for each stage (documented in DESIGN.md)
 STAGES = ["setup", "sort", "pairs", "narrowphase", "constraints", "colouring", "solve",
@@ -53,11 +54,16 @@ def stage_bytes(stage, bodies, pairs, contacts, passes, sweeps):
         return 8 * pairs + (96 + 96 + 176) * contacts
     if stage == "colouring":  # tile key sort + one pass over rows
         return (3 * 2 + 1) * 8 * contacts + 24 * contacts
-    if stage == "solve":      # per launch: read each row once (312 B) and write its impulses
-        # (16 B normal + 16 B friction); read and write each body's velocities
-        # (64 B).  Spatial tiles launch once per sweep and parity phase, scene
-        # tiles run every sweep in one launch.
-        return (328 * contacts + 64 * bodies) * sweeps
+    if stage == "solve":      # per launch: read each row once and write its impulses, read
+        # and write each body's velocities (64 B).  Scene tiles run every sweep
+        # in one launch (rows: 312 B read, 16 B normal + 16 B friction impulses
+        # written); spatial tiles launch once per sweep and parity phase, the
+        # normal sweeps on precomputed rows (368 B read, 16 B written), the
+        # friction sweeps on the full rows (312 B read, 16 B written).
+        if sweeps == 1:
+            return 328 * contacts + 64 * bodies
+        half = sweeps // 2
+        return (384 * contacts + 64 * bodies) * half + (328 * contacts + 64 * bodies) * half
     if stage == "integrate":  # read+write pos, quat, linvel, angvel; read inverse mass
         return 108 * bodies
     return 0

@@ -813,6 +813,30 @@ __device__ __forceinline__ void run_row_normal(NormalRow &d, SolveLDS &L, float4
     else if (d.hb == SLOT_GLOBAL) { linvel[d.b] = vB; angvel[d.b] = wB; }
 }
 
+// The sweep residual terms (largest scaled impulse change, largest impulse)
+// go into per-sweep maxima, one atomic per wave after a max over the wave's
+// lanes (on the float bits: the terms are >= 0, so the bits order like the
+// values and the result equals per-lane atomics).  They are needed after
+// every sweep with a tolerance, otherwise only for the last normal and the
+// last friction sweep, the residuals a step reports.
+__device__ __forceinline__ bool sweep_residual_needed(const StepParams &sp, int sweep) {
+    return sp.tol > 0.f || sweep == sp.normal_iters - 1 ||
+           sweep == sp.normal_iters + sp.friction_iters - 1;
+}
+__device__ __forceinline__ void add_sweep_residual(float *residual, int sweep, float rmax,
+                                                   float lmax) {
+    unsigned r = __float_as_uint(rmax), l = __float_as_uint(lmax);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        r = max(r, (unsigned)__shfl_xor((int)r, o));
+        l = max(l, (unsigned)__shfl_xor((int)l, o));
+    }
+    if ((threadIdx.x & 63) == 0 && (r != 0u || l != 0u)) {
+        atomicMax((unsigned int *)&residual[2 * sweep], r);
+        atomicMax((unsigned int *)&residual[2 * sweep + 1], l);
+    }
+}
+
 // Shared state of one tile's solve: the LDS velocity cache and the
 // tile-wide residual reduction of the current sweep.
 struct TileSolveLDS {
@@ -868,10 +892,7 @@ __device__ __forceinline__ void tile_sweeps(const StepParams &sp, int sweep0, in
             if (phase < 0) continue;
             swept = true;
             const int sweep = sweep0 + (phase == 0 ? it : nsweeps_n + it);
-            if (rmax > 0.f || lmax > 0.f) {
-                atomicMax((unsigned int *)&residual[2 * sweep], __float_as_uint(rmax));
-                atomicMax((unsigned int *)&residual[2 * sweep + 1], __float_as_uint(lmax));
-            }
+            if (sweep_residual_needed(sp, sweep)) add_sweep_residual(residual, sweep, rmax, lmax);
             if (local_tol) {
                 // rmax, lmax >= 0: their float bits order like the values
                 if (rmax > 0.f) atomicMax(&L.red[0], __float_as_uint(rmax));
@@ -895,8 +916,9 @@ __device__ __forceinline__ void tile_sweeps(const StepParams &sp, int sweep0, in
 // One normal sweep of a tile on precomputed rows (spatial tiles: a launch
 // per sweep and parity phase).  REG: row tid is nr, colour rc.
 template <int TBK, bool REG>
-__device__ __forceinline__ void tile_normal_sweep(int sweep, int first, int R, int nc,
-                                                  const int *coff, const PackedRows &pk,
+__device__ __forceinline__ void tile_normal_sweep(const StepParams &sp, int sweep, int first,
+                                                  int R, int nc, const int *coff,
+                                                  const PackedRows &pk,
                                                   float4 *linvel, float4 *angvel,
                                                   float *residual, TileSolveLDS &L,
                                                   NormalRow &nr, int rc) {
@@ -917,10 +939,7 @@ __device__ __forceinline__ void tile_normal_sweep(int sweep, int first, int R, i
         }
         __syncthreads();
     }
-    if (rmax > 0.f || lmax > 0.f) {
-        atomicMax((unsigned int *)&residual[2 * sweep], __float_as_uint(rmax));
-        atomicMax((unsigned int *)&residual[2 * sweep + 1], __float_as_uint(lmax));
-    }
+    if (sweep_residual_needed(sp, sweep)) add_sweep_residual(residual, sweep, rmax, lmax);
     if (REG && tid < R) pk.lam[first + tid] = nr.lam;
 }
 
@@ -964,11 +983,11 @@ __device__ __forceinline__ void solve_tile(
         }
         __syncthreads();
         if (R <= TBK)
-            tile_normal_sweep<TBK, true>(sweep0, first, R, nc, coff, pk, linvel, angvel, residual,
-                                         L, nr, rc);
+            tile_normal_sweep<TBK, true>(sp, sweep0, first, R, nc, coff, pk, linvel, angvel,
+                                         residual, L, nr, rc);
         else
-            tile_normal_sweep<TBK, false>(sweep0, first, R, nc, coff, pk, linvel, angvel, residual,
-                                          L, nr, rc);
+            tile_normal_sweep<TBK, false>(sp, sweep0, first, R, nc, coff, pk, linvel, angvel,
+                                          residual, L, nr, rc);
         __syncthreads();
         for (int s = tid; s < nb; s += TBK) {
             int b = bodies[s];

@@ -62,6 +62,11 @@ __global__ void k_zero_counters(int *c) {
     if (i < RB_CNT_COUNT && i != RB_CNT_OVERFLOW) c[i] = 0;
 }
 
+// Launch shapes used when mi_rb_tuning leaves a field 0 (the autotuner's
+// choice on the 65,537-hull scene and on the 4,096 batched scenes).
+constexpr int DEFAULT_SOLVE_BLOCK = 256;
+constexpr int DEFAULT_NARROW_WAVES = 8;
+
 struct World {
     mi_rb_config cfg;
     int device = 0;
@@ -226,8 +231,8 @@ mi_rb_status derive_and_upload(World &w) {
     p.friction_iters = c.solver_iterations;
     p.tol = c.residual_tolerance;
     p.warm_start = c.warm_start;
-    p.solve_block = w.tuning.solve_block ? w.tuning.solve_block : 512;
-    p.narrow_waves = w.tuning.narrow_waves ? w.tuning.narrow_waves : 6;
+    p.solve_block = w.tuning.solve_block ? w.tuning.solve_block : DEFAULT_SOLVE_BLOCK;
+    p.narrow_waves = w.tuning.narrow_waves ? w.tuning.narrow_waves : DEFAULT_NARROW_WAVES;
     // solver tiles
     if (n_scenes > 1) {
         p.tile_mode = 1;
@@ -730,8 +735,8 @@ extern "C" mi_rb_status mi_rb_set_tuning(void *world, const mi_rb_tuning *t) {
     if (!(t->solve_block == 0 || t->solve_block == 256 || t->solve_block == 512)) return MI_RB_INVALID_ARG;
     if (t->narrow_waves < 0 || t->narrow_waves > 8) return MI_RB_INVALID_ARG;
     w->tuning = *t;
-    w->sp.solve_block = t->solve_block ? t->solve_block : 512;
-    w->sp.narrow_waves = t->narrow_waves ? t->narrow_waves : 6;
+    w->sp.solve_block = t->solve_block ? t->solve_block : DEFAULT_SOLVE_BLOCK;
+    w->sp.narrow_waves = t->narrow_waves ? t->narrow_waves : DEFAULT_NARROW_WAVES;
     if (w->graph) { hipGraphExecDestroy(w->graph); w->graph = nullptr; }
     return MI_RB_OK;
 }
@@ -755,8 +760,8 @@ extern "C" mi_rb_status mi_rb_set_solver(void *world, int iterations, float tole
 extern "C" mi_rb_status mi_rb_get_tuning(void *world, mi_rb_tuning *t) {
     World *w = W(world);
     if (!w || !t) return MI_RB_INVALID_ARG;
-    t->solve_block = w->tuning.solve_block ? w->tuning.solve_block : 512;
-    t->narrow_waves = w->tuning.narrow_waves ? w->tuning.narrow_waves : 6;
+    t->solve_block = w->tuning.solve_block ? w->tuning.solve_block : DEFAULT_SOLVE_BLOCK;
+    t->narrow_waves = w->tuning.narrow_waves ? w->tuning.narrow_waves : DEFAULT_NARROW_WAVES;
     return MI_RB_OK;
 }
 
