@@ -11,7 +11,7 @@ LIB_PATH = os.path.join(HERE, "libmi_rb.so")
 
 OK, INVALID_ARG, CAPACITY, DEVICE, UNSUPPORTED_SHAPE, STATE = range(6)
 STATUS_NAMES = {0: "OK", 1: "INVALID_ARG", 2: "CAPACITY", 3: "DEVICE", 4: "UNSUPPORTED_SHAPE",
-                5: "STATE"}
+                5: "STATE", 6: "FORMAT"}
 
 WARM_ZERO, WARM_PREVIOUS, WARM_LEARNED = 0, 1, 2
 INERTIA_REFERENCE, INERTIA_BOX = 0, 1
@@ -81,6 +81,27 @@ class GridDesc(ctypes.Structure):
                 ("levels", ctypes.c_int)]
 
 
+class BulletShape(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int), ("num_points", ctypes.c_int), ("first_point", ctypes.c_int),
+                ("local_scaling", ctypes.c_float * 3), ("implicit_dims", ctypes.c_float * 3),
+                ("margin", ctypes.c_float), ("plane_normal", ctypes.c_float * 3),
+                ("plane_constant", ctypes.c_float)]
+
+
+class BulletBody(ctypes.Structure):
+    _fields_ = [("shape", ctypes.c_int), ("inverse_mass", ctypes.c_float),
+                ("basis", ctypes.c_float * 9), ("origin", ctypes.c_float * 3),
+                ("quat", ctypes.c_float * 4), ("linvel", ctypes.c_float * 3),
+                ("angvel", ctypes.c_float * 3), ("friction", ctypes.c_float),
+                ("restitution", ctypes.c_float)]
+
+
+class BulletSummary(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in ("shapes", "shapes_registered", "unsupported_shapes",
+                                           "bodies", "bodies_registered", "ground_bodies",
+                                           "pointer_size", "big_endian")]
+
+
 class MiRbError(RuntimeError):
     def __init__(self, fn, status):
         super().__init__("%s failed: %s" % (fn, STATUS_NAMES.get(status, status)))
@@ -146,6 +167,8 @@ SIGNATURES = {
     "mi_rb_sap_find_pairs": (I, [VP, VP, VP, VP, I, VP, VP, VP]),
     "mi_rb_sap_last_ms": (I, [VP, VP]),
     "mi_rb_sap_overflow": (I, [VP, PI]),
+    "mi_rb_bullet_read": (I, [CP, VP, I, VP, I, VP, I, PI]),
+    "mi_rb_load_bullet": (I, [VP, CP, I, ctypes.POINTER(BulletSummary)]),
 }
 
 

@@ -87,11 +87,14 @@ class World:
     # ---- state ----
     def set_state(self, pos=None, quat=None, linvel=None, angvel=None):
         arrs = []
+        n = self.body_count()
         for a in (pos, quat, linvel, angvel):
             if a is None:
                 arrs.append(None)
             else:
                 a = _f32(a)
+                if len(a) != n:
+                    raise ValueError("state arrays need one row per body (%d)" % n)
                 if a.shape[-1] == 3:
                     b = np.zeros((len(a), 4), np.float32)
                     b[:, :3] = a
@@ -107,8 +110,12 @@ class World:
         L.call("mi_rb_copy_transforms", self.h, ctypes.c_void_p(ptr), int(start_offset), int(cap),
                ctypes.c_void_p(stream) if stream else None)
 
+    def body_count(self):
+        """Bodies registered in the library (however they were registered)."""
+        return self.stats()["num_bodies"]
+
     def get_state(self):
-        n = self.num_bodies
+        n = self.body_count()
         out = [np.zeros((n, 4), np.float32) for _ in range(4)]
         L.call("mi_rb_get_body_state", self.h, *[_ptr(a) for a in out])
         return dict(pos=out[0], quat=out[1], linvel=out[2], angvel=out[3])
@@ -204,6 +211,16 @@ class World:
                ctypes.c_void_p(t_dev_ptr), int(capacity), ctypes.byref(cnt))
         return cnt.value
 
+    def load_bullet(self, path, keep_static=False):
+        """Register a .bullet file's boxes, convex hulls and bodies (before
+        upload), as the reference's btBulletDataExtractor does; returns the
+        summary counts."""
+        from .bullet import load_bullet
+        summary = load_bullet(self, path, keep_static)
+        self.num_bodies += summary["bodies_registered"] + summary["ground_bodies"]
+        self.num_shapes += summary["shapes_registered"] + (1 if summary["ground_bodies"] else 0)
+        return summary
+
     def save(self, path):
         L.call("mi_rb_save_state", self.h, path.encode())
 
@@ -247,6 +264,14 @@ class CLPhysicsDemo:
 
     def writeBodiesToGpu(self):
         self.world.upload()
+
+    def createSceneFromBulletFile(self, fileName, keepStaticObjects=False):
+        """createScene + btBulletDataExtractor::convertAllObjects
+        (BulletDataExtractor.cpp:454-520, 594-700), then writeBodiesToGpu."""
+        summary = self.world.load_bullet(fileName, keepStaticObjects)
+        self.m_numPhysicsInstances += summary["bodies_registered"] + summary["ground_bodies"]
+        self.writeBodiesToGpu()
+        return summary
 
     def writeVelocitiesToGpu(self):
         return None

@@ -26,7 +26,8 @@ typedef enum mi_rb_status {
     MI_RB_CAPACITY = 2,          /* a configured capacity would be exceeded */
     MI_RB_DEVICE = 3,            /* HIP runtime error or no usable gfx950 device */
     MI_RB_UNSUPPORTED_SHAPE = 4, /* non-hull shapes (sphere, plane, compound, mesh) */
-    MI_RB_STATE = 5              /* call out of order (e.g. step before upload) */
+    MI_RB_STATE = 5,             /* call out of order (e.g. step before upload) */
+    MI_RB_FORMAT = 6             /* a file that is malformed or of an unsupported kind */
 } mi_rb_status;
 
 /* warm start of the contact solver */
@@ -216,6 +217,57 @@ mi_rb_status mi_rb_read_buffer(void *world, int which, void *host, size_t max_by
 /* versioned binary snapshot of shapes + bodies + velocities */
 mi_rb_status mi_rb_save_state(void *world, const char *path);
 mi_rb_status mi_rb_load_state(void *world, const char *path);
+
+/* ---- .bullet scene files ----
+ * The reference reads Bullet's serialized scenes with bParse::btBulletFile and
+ * converts them with btBulletDataExtractor (gpu_rigidbody_pipeline2/
+ * BulletDataExtractor.cpp:454-520 createScene, :594-700 convertAllObjects,
+ * :703-1010 convertCollisionShape, createBoxShape).  Float files with 4- or
+ * 8-byte pointers in either byte order; every struct layout is taken from
+ * the file's own DNA.  Double-precision files give MI_RB_FORMAT, as the
+ * reference's demo refuses them.  Host code: no device work. */
+enum {
+    MI_RB_SHAPE_BOX = 0, MI_RB_SHAPE_CONVEX_HULL = 4, MI_RB_SHAPE_SPHERE = 8,
+    MI_RB_SHAPE_TRIANGLE_MESH = 21, MI_RB_SHAPE_STATIC_PLANE = 28, MI_RB_SHAPE_COMPOUND = 31
+};
+typedef struct mi_rb_bullet_shape {
+    int type;                /* Bullet's shape type number (MI_RB_SHAPE_* and others) */
+    int num_points;          /* convex hull: unscaled points, from first_point of the points */
+    int first_point;
+    float local_scaling[3];
+    float implicit_dims[3];  /* box: half extents (without margin); sphere: radius in [0] */
+    float margin;
+    float plane_normal[3];   /* static plane */
+    float plane_constant;
+} mi_rb_bullet_shape;
+typedef struct mi_rb_bullet_body {
+    int shape;               /* index into the file's shapes, -1 if none */
+    float inverse_mass;
+    float basis[9];          /* world rotation, rows */
+    float origin[3];
+    float quat[4];           /* the rotation as x, y, z, w */
+    float linvel[3], angvel[3];
+    float friction, restitution;
+} mi_rb_bullet_body;
+typedef struct mi_rb_bullet_summary {
+    int shapes, shapes_registered, unsupported_shapes;
+    int bodies, bodies_registered, ground_bodies;
+    int pointer_size, big_endian;
+} mi_rb_bullet_summary;
+/* Parse only: up to max_* shapes, bodies and points (xyz); counts = {shapes,
+ * bodies, points, pointer size | 0x100 if big-endian}.  Any array may be NULL. */
+mi_rb_status mi_rb_bullet_read(const char *path, mi_rb_bullet_shape *shapes, int max_shapes,
+                               mi_rb_bullet_body *bodies, int max_bodies, float *points,
+                               int max_points, int counts[4]);
+/* btBulletDataExtractor::convertAllObjects into a world before mi_rb_upload:
+ * boxes (the unit cube scaled by half extent * scaling + margin) and convex
+ * hulls (points * scaling) are registered in file order, other shape types
+ * are counted in unsupported_shapes; bodies are registered shape by shape
+ * (mass 1/inverse mass, rotation from the basis), static ones only with
+ * keep_static; without keep_static a 50 x 50 grid of static pyramids is
+ * added as the ground, as the reference does. */
+mi_rb_status mi_rb_load_bullet(void *world, const char *path, int keep_static,
+                               mi_rb_bullet_summary *out);
 /* Learned warm start (no reference counterpart: the reference zeroes lambda
  * every step, dynamics/basic_demo/Stubs/solverSetup.cl:529-531).  Weights of
  * the 16-64-64-3 network used when warm_start == MI_RB_WARM_LEARNED: bf16

@@ -126,6 +126,8 @@ struct CLPhysicsDemo {
   private:
     friend void copyTransformsToBVO(CLPhysicsDemo &demo, void *vbo, int positionOffset,
                                     int vboFloat4Capacity);
+    friend bool createSceneFromBulletFile(CLPhysicsDemo &demo, const char *fileName,
+                                          bool keepStaticObjects);
     mi_rb_config m_cfg{};
     void *m_world = nullptr;
     mi_rb_device_view m_view{};
@@ -141,6 +143,23 @@ inline void copyTransformsToBVO(CLPhysicsDemo &demo, void *vbo, int positionOffs
                                 int vboFloat4Capacity) {
     demo.m_status = mi_rb_copy_transforms(demo.m_world, vbo, positionOffset, vboFloat4Capacity,
                                           nullptr);
+}
+
+/* createScene(renderer, physicsSim, useConvexHeightfield, fileName)
+ * (gpu_rigidbody_pipeline2/BulletDataExtractor.cpp:454-520) with
+ * btBulletDataExtractor::convertAllObjects (:594-700): registers the .bullet
+ * file's boxes, convex hulls and bodies (static ones only with
+ * keepStaticObjects, the reference's global of that name; otherwise its
+ * pyramid ground grid), then writeBodiesToGpu.  Returns false when the file
+ * cannot be read or is not a float .bullet file (the reference then builds
+ * its programmatic scene); last_status() tells which. */
+inline bool createSceneFromBulletFile(CLPhysicsDemo &demo, const char *fileName,
+                                      bool keepStaticObjects = false) {
+    mi_rb_bullet_summary summary;
+    demo.m_status = mi_rb_load_bullet(demo.m_world, fileName, keepStaticObjects ? 1 : 0, &summary);
+    if (demo.m_status != MI_RB_OK) return false;
+    demo.writeBodiesToGpu();
+    return demo.m_status == MI_RB_OK;
 }
 
 #endif

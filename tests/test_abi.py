@@ -91,7 +91,7 @@ SHIM_PROGRAM = r"""
 #include <cstdio>
 #include <hip/hip_runtime_api.h>
 #include "mi_rb_clphysicsdemo.hpp"
-int main() {
+int main(int argc, char **argv) {
     // the reference demo's call sequence (gpu_rigidbody_pipeline2/main.cpp:173-189)
     CLPhysicsDemo demo(1 << 20, 1 << 10);
     demo.init(-1, -1, false);
@@ -122,6 +122,24 @@ int main() {
     float p[4], q[4];
     demo.getObjectTransformFromCpu(p, q, 0);
     std::printf("y %f\n", p[1]);
+    // a scene from a .bullet file (createScene, BulletDataExtractor.cpp:454-520)
+    if (argc > 1) {
+        CLPhysicsDemo fromFile(1 << 20, 1 << 13);
+        fromFile.init(-1, -1, false);
+        if (!createSceneFromBulletFile(fromFile, argv[1], false)) return 1;
+        for (int i = 0; i < 10; i++) fromFile.stepSimulation();
+        fromFile.readbackBodiesToCpu();
+        fromFile.getObjectTransformFromCpu(p, q, 0);
+        if (fromFile.last_status() != MI_RB_OK || !(p[1] > 0.f && p[1] < 1.f)) return 1;
+        // a double-precision file is refused with MI_RB_FORMAT
+        if (argc > 2) {
+            CLPhysicsDemo refused(1 << 20, 1 << 13);
+            refused.init(-1, -1, false);
+            if (createSceneFromBulletFile(refused, argv[2], false)) return 1;
+            if (refused.last_status() != MI_RB_FORMAT) return 1;
+        }
+        std::printf("bullet y %f\n", p[1]);
+    }
     return 0;
 }
 """
@@ -142,5 +160,8 @@ def test_clphysicsdemo_shim_compiles_links_and_runs(tmp_path):
                         "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib", "-o", str(exe)],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
-    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    gold = os.path.join(ROOT, "tests", "golden")
+    r = subprocess.run([str(exe), os.path.join(gold, "stack_le32.bullet"),
+                        os.path.join(gold, "stack_double.bullet")],
+                       capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stdout + r.stderr
