@@ -119,6 +119,8 @@ SIGNATURES = {
     "mi_rb_destroy": (I, [VP]),
     "mi_rb_register_convex_hull": (I, [VP, VP, I, I, VP, PI]),
     "mi_rb_register_unsupported_shape": (I, [VP, I]),
+    "mi_rb_register_sphere": (I, [VP, F, PI]),
+    "mi_rb_register_plane": (I, [VP, VP, F, PI]),
     "mi_rb_register_body": (I, [VP, F, VP, VP, I, I, PI]),
     "mi_rb_upload": (I, [VP]),
     "mi_rb_set_transform": (I, [VP, I, VP, VP]),

@@ -195,6 +195,104 @@ __device__ __forceinline__ bool hull_limits_ok(const HullInfo &hA, const HullInf
            hA.num_faces <= 2 * NP_BLOCK && hB.num_faces <= 2 * NP_BLOCK;
 }
 
+// Shapes without vertices: a sphere (no faces; radius in center.w) or a
+// plane (one face: normal and constant, n . x = constant).
+__device__ __forceinline__ int shape_kind(const HullInfo &h) {
+    return h.num_vertices > 0 ? SHAPE_KIND_HULL
+                              : (h.num_faces > 0 ? SHAPE_KIND_PLANE : SHAPE_KIND_SPHERE);
+}
+
+// The contact of a pair with a sphere or a plane: clipHullHullKernel's first
+// branches (satClipHullContacts.cl:1055-1118).  Sphere-sphere and
+// plane-sphere give one point (the plane body first in a plane-sphere
+// contact, as computeContactPlaneConvex is called); a hull against a sphere
+// or a plane and a plane against a plane give none, as in the reference.
+__device__ bool special_contact(int p, int a, int b, float imA, float imB, const HullInfo &hA,
+                                const HullInfo &hB, float4 pA, float4 oA, float4 pB, float4 oB,
+                                const float4 *__restrict__ planes, float friction, Contact4 &ct) {
+    const int kA = shape_kind(hA), kB = shape_kind(hB);
+    ct.coeffs = (uint32_t)(0.f * 0xffff) | ((uint32_t)(friction * 0xffff) << 16);
+    ct.pair_index = p;
+    for (int i = 1; i < 4; i++) ct.pos[i] = mk4(0, 0, 0, 0);
+    if (kA == SHAPE_KIND_SPHERE && kB == SHAPE_KIND_SPHERE) {
+        const float rA = hA.center.w, rB = hB.center.w;
+        const float4 diff = sub4(pA, pB);
+        const float len = sqrtf(dot3(diff, diff));
+        if (!(len <= rA + rB)) return false;
+        const float dist = len - (rA + rB);
+        float4 n = mk4(1.f, 0.f, 0.f, 0.f);
+        if (len > 0.00001f) n = mk4(diff.x / len, diff.y / len, diff.z / len, diff.w / len);
+        float4 c = add4(pB, scl4(n, rB));
+        c.w = dist;
+        const float4 nn = neg4(n);
+        ct.normal = mk4(nn.x, nn.y, nn.z, 1.f);
+        ct.pos[0] = c;
+        ct.body_a = imA == 0.f ? -a : a;
+        ct.body_b = imB == 0.f ? -b : b;
+        return true;
+    }
+    const bool plane_sphere = kA == SHAPE_KIND_PLANE && kB == SHAPE_KIND_SPHERE;
+    const bool sphere_plane = kA == SHAPE_KIND_SPHERE && kB == SHAPE_KIND_PLANE;
+    if (!plane_sphere && !sphere_plane) return false;
+    // computeContactPlaneConvex(plane body, sphere body)
+    const int ip = plane_sphere ? a : b, is = plane_sphere ? b : a;
+    const float imP = plane_sphere ? imA : imB, imS = plane_sphere ? imB : imA;
+    const HullInfo &hP = plane_sphere ? hA : hB;
+    const float radius = (plane_sphere ? hB : hA).center.w;
+    const float4 posA1 = plane_sphere ? pA : pB, ornA1 = plane_sphere ? oA : oB;
+    const float4 posB1 = plane_sphere ? pB : pA, ornB1 = plane_sphere ? oB : oA;
+    const float4 eq = planes[hP.face_offset];
+    const float4 n = mk4(eq.x, eq.y, eq.z, 0.f);
+    const float pc = eq.w;
+    // trInverse / trMul: the sphere in the plane's frame and the plane in the sphere's
+    const float4 invOrnA = qt_inv(ornA1), invPosA = qt_rotate(invOrnA, neg4(posA1));
+    const float4 cOrn = qt_mul(invOrnA, ornB1), cPos = xform(posB1, invPosA, invOrnA);
+    const float4 pOrn = qt_mul(qt_inv(ornB1), ornA1);  // (only the rotation is used)
+    const float4 vtx1 = scl4(qt_rotate(pOrn, neg4(n)), radius);
+    const float4 vtxInPlane = xform(vtx1, cPos, cOrn);
+    const float distance = dot3(n, vtxInPlane) - pc;
+    if (!(distance < 0.f)) return false;
+    const float4 proj = sub4(vtxInPlane, scl4(n, distance));
+    const float4 world = xform(proj, posA1, ornA1);
+    const float4 nB = qt_rotate(ornA1, n);
+    float4 pOnB = add4(world, scl4(nB, distance));
+    pOnB.w = distance;
+    ct.normal = mk4(nB.x, nB.y, nB.z, 1.f);
+    ct.pos[0] = pOnB;
+    ct.body_a = imP == 0.f ? -ip : ip;
+    ct.body_b = imS == 0.f ? -is : is;
+    return true;
+}
+
+// Pairs with a sphere or a plane (launched only for worlds that have such
+// shapes, after the face pass, which leaves these pairs without contact):
+// one thread per pair.
+__global__ __launch_bounds__(256) void k_narrow_special(
+    int max_pairs, const int *__restrict__ num_pairs_ptr, const int2 *__restrict__ pairs,
+    const float4 *__restrict__ pos, const float4 *__restrict__ quat,
+    const float *__restrict__ inv_mass, const int *__restrict__ shape,
+    const HullInfo *__restrict__ hulls, const float4 *__restrict__ planes, float friction,
+    Contact4 *__restrict__ slots, uint32_t *__restrict__ has_contact) {
+    int npairs = *num_pairs_ptr;
+    if (npairs > max_pairs) npairs = max_pairs;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += gridDim.x * blockDim.x) {
+        const int2 pr = pairs[p];
+        const HullInfo &hA = hulls[shape[pr.x]], &hB = hulls[shape[pr.y]];
+        if (hA.num_vertices != 0 && hB.num_vertices != 0) continue;
+        const float imA = inv_mass[pr.x], imB = inv_mass[pr.y];
+        if (imA == 0.f && imB == 0.f) continue;
+        float4 pA = pos[pr.x], pB = pos[pr.y];
+        pA.w = 0.f;
+        pB.w = 0.f;
+        Contact4 ct;
+        if (special_contact(p, pr.x, pr.y, imA, imB, hA, hB, pA, quat[pr.x], pB, quat[pr.y], planes,
+                            friction, ct)) {
+            slots[p] = ct;
+            has_contact[p] = 1;
+        }
+    }
+}
+
 // Pass 1, every pair: face axes of A and B.  A pair separated by a face axis
 // (or static-static, or beyond the hull limits) has no contact; the others
 // become candidates for pass 2 with their face minimum (depth, axis) and the
@@ -249,6 +347,10 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_face(
         }
         const HullInfo hA = hulls[cur.sA];
         const HullInfo hB = hulls[cur.sB];
+        if (hA.num_vertices == 0 || hB.num_vertices == 0) {  // a sphere or a plane: k_narrow_special
+            if (lane == 0) { has_contact[p] = 0; cand[p] = 0; }
+            continue;
+        }
         if (!hull_limits_ok(hA, hB)) {
             if (lane == 0) {
                 has_contact[p] = 0;
@@ -706,6 +808,10 @@ hipError_t launch_narrowphase(const DeviceWorld &d, const StepParams &p, hipStre
                        d.counters + RB_CNT_PAIRS, d.pairs, d.pos, d.quat, d.inv_mass, d.shape,
                        d.hulls, d.verts, d.planes, d.has_contact, d.narrow_cand, d.face_best,
                        d.counters + RB_CNT_OVERFLOW);
+    if (p.has_vertexless)
+        hipLaunchKernelGGL(k_narrow_special, dim3(rb_num_cus() * 4), dim3(256), 0, s, p.max_pairs,
+                           d.counters + RB_CNT_PAIRS, d.pairs, d.pos, d.quat, d.inv_mass, d.shape,
+                           d.hulls, d.planes, p.friction, d.contact_slots, d.has_contact);
     hipError_t e = launch_exclusive_scan(d.narrow_cand, d.cand_off, p.max_pairs,
                                          (uint32_t *)(d.counters + RB_CNT_NARROW_CAND), scan_status,
                                          s, d.counters + RB_CNT_PAIRS);

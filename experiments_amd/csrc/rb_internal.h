@@ -24,6 +24,9 @@ struct HullInfo {
     float4 aabb_max;
 };
 static_assert(sizeof(HullInfo) == 80, "HullInfo layout");
+// A shape without vertices is a sphere (no faces, radius in center.w) or a
+// plane (one face at face_offset: normal xyz and constant w, n . x = w).
+enum { SHAPE_KIND_HULL = 0, SHAPE_KIND_SPHERE = 1, SHAPE_KIND_PLANE = 2 };
 
 // Contact manifold in the Contact4 layout (dynamics/basic_demo/Stubs/AdlContact4.h:26-59).
 struct Contact4 {
@@ -203,6 +206,7 @@ struct StepParams {
     // launch shapes (mi_rb_tuning)
     int solve_block;     // threads per scene-tile solve workgroup: 256 or 512
     int narrow_waves;    // narrowphase waves per SIMD (grid = CUs * 4 * narrow_waves)
+    int has_vertexless;  // some shape is a sphere or a plane
 };
 
 // ---- launch interfaces (implemented in the .hip files) ----

@@ -161,6 +161,8 @@ mi_rb_status derive_and_upload(World &w) {
     StepParams &p = w.sp;
     p = StepParams();
     p.n_bodies = n;
+    p.has_vertexless = 0;
+    for (const HullInfo &h : w.hulls) p.has_vertexless |= h.num_vertices == 0;
     int n_scenes = 0;
     for (int i = 0; i < n; i++) n_scenes = std::max(n_scenes, w.scene[i] + 1);
     p.n_scenes = n_scenes;
@@ -580,6 +582,49 @@ extern "C" mi_rb_status mi_rb_register_convex_hull(void *world, const float *xyz
     return MI_RB_OK;
 }
 
+// A shape without vertices (a sphere or a plane): no faces for a sphere, the
+// plane's equation as the one face of a plane; the local AABB is the
+// reference's (CLPhysicsDemo.cpp:420-489).
+static mi_rb_status add_vertexless_shape(World *w, float radius, const float *plane,
+                                         int *out_shape) {
+    if ((int)w->hulls.size() >= w->cfg.max_shapes) return MI_RB_CAPACITY;
+    HullInfo hi{};
+    hi.face_offset = (int)w->planes.size();
+    hi.num_faces = plane ? 1 : 0;
+    hi.vertex_offset = (int)w->verts.size();
+    hi.edge_offset = (int)w->edges.size();
+    hi.index_offset = (int)w->indices.size();
+    const float e = plane ? 1e30f : radius;
+    hi.center = make_float4(0.f, 0.f, 0.f, plane ? 0.f : radius);
+    hi.aabb_min = make_float4(-e, -e, -e, 0.f);
+    hi.aabb_max = make_float4(e, e, e, 0.f);
+    if (plane) {
+        w->planes.push_back(make_float4(plane[0], plane[1], plane[2], plane[3]));
+        w->face_index.push_back(make_int2(hi.index_offset, 0));
+    }
+    w->hull_hext_norm.push_back(std::sqrt(3.f) * e);
+    w->hulls.push_back(hi);
+    *out_shape = (int)w->hulls.size() - 1;
+    w->uploaded = false;
+    return MI_RB_OK;
+}
+
+extern "C" mi_rb_status mi_rb_register_sphere(void *world, float radius, int *out_shape) {
+    World *w = W(world);
+    if (!w || !out_shape || !(radius > 0.f) || !(radius < 1e18f)) return MI_RB_INVALID_ARG;
+    return add_vertexless_shape(w, radius, nullptr, out_shape);
+}
+
+extern "C" mi_rb_status mi_rb_register_plane(void *world, const float normal[3], float constant,
+                                             int *out_shape) {
+    World *w = W(world);
+    if (!w || !normal || !out_shape || !std::isfinite(constant)) return MI_RB_INVALID_ARG;
+    const float len2 = normal[0] * normal[0] + normal[1] * normal[1] + normal[2] * normal[2];
+    if (!(len2 > 0.f) || !std::isfinite(len2)) return MI_RB_INVALID_ARG;
+    const float eq[4] = {normal[0], normal[1], normal[2], constant};
+    return add_vertexless_shape(w, 0.f, eq, out_shape);
+}
+
 extern "C" mi_rb_status mi_rb_register_unsupported_shape(void *world, int kind) {
     (void)kind;
     return world ? MI_RB_UNSUPPORTED_SHAPE : MI_RB_INVALID_ARG;
@@ -591,6 +636,9 @@ extern "C" mi_rb_status mi_rb_register_body(void *world, float mass, const float
     World *w = W(world);
     if (!w || !pos || !quat_xyzw || !out_body || mass < 0.f) return MI_RB_INVALID_ARG;
     if (shape < 0 || shape >= (int)w->hulls.size() || scene < 0) return MI_RB_INVALID_ARG;
+    // a plane is infinite: it can only be static
+    if (mass != 0.f && w->hulls[shape].num_vertices == 0 && w->hulls[shape].num_faces > 0)
+        return MI_RB_INVALID_ARG;
     if (!w->scene.empty() && scene < w->scene.back()) return MI_RB_INVALID_ARG;
     if ((int)w->pos.size() >= w->cfg.max_bodies) return MI_RB_CAPACITY;
     w->pos.push_back(make_float4(pos[0], pos[1], pos[2], 0.f));

@@ -72,6 +72,20 @@ class World:
         self.num_shapes += 1
         return out.value
 
+    def add_sphere(self, radius):
+        out = ctypes.c_int()
+        L.call("mi_rb_register_sphere", self.h, ctypes.c_float(radius), ctypes.byref(out))
+        self.num_shapes += 1
+        return out.value
+
+    def add_plane(self, normal, constant):
+        n = _f32(normal)[:3].copy()
+        out = ctypes.c_int()
+        L.call("mi_rb_register_plane", self.h, _ptr(n), ctypes.c_float(constant),
+               ctypes.byref(out))
+        self.num_shapes += 1
+        return out.value
+
     def add_body(self, mass, pos, quat_xyzw, shape, scene=0):
         p = _f32(pos)
         q = _f32(quat_xyzw)
@@ -248,12 +262,10 @@ class CLPhysicsDemo:
         return self.world.add_hull(pts, scaling[:3])
 
     def registerSphereShape(self, radius):
-        st = L.lib().mi_rb_register_unsupported_shape(self.world.h, 1)
-        return -1 if st else -1
+        return self.world.add_sphere(radius)
 
     def registerPlaneShape(self, planeNormal, planeConstant):
-        L.lib().mi_rb_register_unsupported_shape(self.world.h, 2)
-        return -1
+        return self.world.add_plane(planeNormal, planeConstant)
 
     def registerPhysicsInstance(self, mass, position, orientation, collidableShape, userIndex):
         idx = self.world.add_body(mass, position[:3], orientation[:4], collidableShape, 0)

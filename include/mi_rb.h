@@ -164,8 +164,18 @@ mi_rb_status mi_rb_destroy(void *world);
  * points (scaled), coplanar faces merged; returns the shape index. */
 mi_rb_status mi_rb_register_convex_hull(void *world, const float *xyz, int stride_bytes, int n,
                                         const float scale[3], int *out_shape);
-/* registerSphereShape / registerPlaneShape / registerCompoundShape /
- * registerConcaveMesh (CLPhysicsDemo.h:58-66): not on this path. */
+/* CLPhysicsDemo::registerSphereShape(radius) (CLPhysicsDemo.cpp:420-456): a
+ * sphere; local AABB +-radius.  Contacts: sphere-sphere and sphere-plane,
+ * one point each (satClipHullContacts.cl:1055-1118); a sphere against a hull
+ * gets none, as in the reference. */
+mi_rb_status mi_rb_register_sphere(void *world, float radius, int *out_shape);
+/* CLPhysicsDemo::registerPlaneShape(normal, constant) (CLPhysicsDemo.cpp:459-489):
+ * the plane n . x = constant, for static bodies only (a body with mass on it
+ * gives MI_RB_INVALID_ARG); local AABB +-1e30.  Contacts with spheres only. */
+mi_rb_status mi_rb_register_plane(void *world, const float normal[3], float constant,
+                                  int *out_shape);
+/* registerCompoundShape / registerConcaveMesh (CLPhysicsDemo.h:58-66): not on
+ * this path. */
 mi_rb_status mi_rb_register_unsupported_shape(void *world, int kind);
 
 /* CLPhysicsDemo::registerPhysicsInstance(mass, pos, orn xyzw, collidable, user)

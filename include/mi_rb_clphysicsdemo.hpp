@@ -56,15 +56,14 @@ struct CLPhysicsDemo {
         return m_status == MI_RB_OK ? shape : -1;
     }
     int registerSphereShape(float radius) {
-        (void)radius;
-        m_status = mi_rb_register_unsupported_shape(m_world, 1);
-        return -1;
+        int shape = -1;
+        m_status = mi_rb_register_sphere(m_world, radius, &shape);
+        return m_status == MI_RB_OK ? shape : -1;
     }
     int registerPlaneShape(const float planeNormal[3], float planeConstant) {
-        (void)planeNormal;
-        (void)planeConstant;
-        m_status = mi_rb_register_unsupported_shape(m_world, 2);
-        return -1;
+        int shape = -1;
+        m_status = mi_rb_register_plane(m_world, planeNormal, planeConstant, &shape);
+        return m_status == MI_RB_OK ? shape : -1;
     }
     template <class Vertices, class Indices>
     int registerConcaveMesh(Vertices *vertices, Indices *indices, const float *scaling) {

@@ -641,6 +641,60 @@ int orc_reduce_manifold(const float *pts, int n, const float normal[4], int idx[
     return 4;
 }
 
+/* Shapes without vertices: a sphere (no faces, radius in center[3]) or a
+ * plane (one face: normal and constant, n . x = constant).  clipHullHullKernel's
+ * first branches (satClipHullContacts.cl:1055-1118): sphere-sphere gives one
+ * point on B's surface with the normal from A to B; plane-sphere gives the
+ * sphere's deepest point with the plane's world normal, the plane body first.
+ * The plane-sphere case is written here in closed form (the signed distance
+ * of the sphere's centre from the plane in world space); the reference forms
+ * the same point through relative transforms.  A hull against a sphere or a
+ * plane, and a plane against a plane, give no contact, as in the reference. */
+static int special_contact(int p, int a, int b, const orc_hull *hA, const orc_hull *hB,
+                           const float *pos, const float *quat, const float *planes,
+                           const float *inv_mass, float friction, orc_contact *c)
+{
+    int kA = hA->num_vertices > 0 ? 0 : (hA->num_faces > 0 ? 2 : 1);
+    int kB = hB->num_vertices > 0 ? 0 : (hB->num_faces > 0 ? 2 : 1);
+    memset(c, 0, sizeof(*c));
+    c->coeffs = (uint32_t)(0.f * 0xffff) | ((uint32_t)(friction * 0xffff) << 16);
+    c->pair_index = p;
+    c->normal[3] = 1.f;
+    if (kA == 1 && kB == 1) {
+        f4 pa = mk4(pos[4 * a], pos[4 * a + 1], pos[4 * a + 2], 0.f);
+        f4 pb = mk4(pos[4 * b], pos[4 * b + 1], pos[4 * b + 2], 0.f);
+        float rA = hA->center[3], rB = hB->center[3];
+        f4 d = sub4(pa, pb);
+        float len = sqrtf(dot3(d, d));
+        if (!(len <= rA + rB)) return 0;
+        f4 n = len > 0.00001f ? mk4(d.x / len, d.y / len, d.z / len, 0.f) : mk4(1.f, 0.f, 0.f, 0.f);
+        f4 q = add4(pb, scl4(n, rB));
+        q.w = len - (rA + rB);
+        st4(c->pos[0], q);
+        c->normal[0] = -n.x; c->normal[1] = -n.y; c->normal[2] = -n.z;
+        c->body_a = inv_mass[a] == 0 ? -a : a;
+        c->body_b = inv_mass[b] == 0 ? -b : b;
+        return 1;
+    }
+    if (!((kA == 2 && kB == 1) || (kA == 1 && kB == 2))) return 0;
+    int ip = kA == 2 ? a : b, is = kA == 2 ? b : a;
+    const orc_hull *hP = kA == 2 ? hA : hB, *hS = kA == 2 ? hB : hA;
+    const float *eq = planes + 4 * hP->face_offset;
+    f4 nw = qt_rotate(ld4(quat + 4 * ip), mk4(eq[0], eq[1], eq[2], 0.f));
+    f4 pp = mk4(pos[4 * ip], pos[4 * ip + 1], pos[4 * ip + 2], 0.f);
+    f4 ps = mk4(pos[4 * is], pos[4 * is + 1], pos[4 * is + 2], 0.f);
+    float r = hS->center[3];
+    float dist = dot3(nw, ps) - (eq[3] + dot3(nw, pp)) - r;
+    if (!(dist < 0.f)) return 0;
+    f4 q = sub4(ps, scl4(nw, r));
+    q.w = dist;
+    st4(c->pos[0], q);
+    c->normal[0] = nw.x; c->normal[1] = nw.y; c->normal[2] = nw.z;
+    c->body_a = inv_mass[ip] == 0 ? -ip : ip;
+    c->body_b = inv_mass[is] == 0 ? -is : is;
+    return 1;
+}
+
 /* clipHullHullKernel convex-convex branch (satClipHullContacts.cl:1019-1181)
  * after findSeparatingAxisKernel (sat.cl:928-1261), one pair at a time.  The
  * manifold of pair p goes to slots[p]; has_contact[p] says whether it exists.
@@ -661,6 +715,13 @@ int orc_narrowphase(int num_pairs, const int32_t *pairs, const float *pos, const
         memset(&slots[p], 0, sizeof(orc_contact));
         if (inv_mass[a] == 0 && inv_mass[b] == 0) continue;
         const orc_hull *hA = &hulls[shape[a]], *hB = &hulls[shape[b]];
+        if (hA->num_vertices == 0 || hB->num_vertices == 0) {
+            if (special_contact(p, a, b, hA, hB, pos, quat, planes, inv_mass, friction, &slots[p])) {
+                has_contact[p] = 1;
+                total++;
+            }
+            continue;
+        }
         float sep[4];
         float pa[4], pb[4];
         memcpy(pa, pos + 4 * a, 16); memcpy(pb, pos + 4 * b, 16);
