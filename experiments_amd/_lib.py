@@ -20,7 +20,8 @@ OVERFLOW_COLORS, OVERFLOW_TILE, OVERFLOW_POLYGON, OVERFLOW_LEVEL = 8, 16, 32, 64
 (BUF_SORTED_KV, BUF_AABB, BUF_PAIRS, BUF_CONTACTS, BUF_CONSTRAINTS, BUF_SOLVE_ORDER, BUF_COLORS,
  BUF_INV_INERTIA, BUF_PHASES, BUF_HULLS, BUF_VERTS, BUF_PLANES, BUF_FACE_INDEX, BUF_INDICES,
  BUF_EDGES, BUF_INV_MASS, BUF_INV_INERTIA_LOCAL, BUF_SCENE, BUF_IS_LARGE, BUF_SHAPE,
- BUF_FEATURES, BUF_PREV_KEYS, BUF_PREV_IMPULSES, BUF_PREDICTIONS, BUF_PREV_POINTS) = range(25)
+ BUF_FEATURES, BUF_PREV_KEYS, BUF_PREV_IMPULSES, BUF_PREDICTIONS, BUF_PREV_POINTS,
+ BUF_CHILDREN) = range(26)
 MLP_NW, MLP_NB = 64 * 16 + 64 * 64 + 3 * 64, 64 + 64 + 3
 (STAGE_SETUP, STAGE_SORT, STAGE_PAIRS, STAGE_NARROWPHASE, STAGE_CONSTRAINTS, STAGE_COLOR,
  STAGE_SOLVE, STAGE_INTEGRATE) = range(8)
@@ -121,6 +122,7 @@ SIGNATURES = {
     "mi_rb_register_unsupported_shape": (I, [VP, I]),
     "mi_rb_register_sphere": (I, [VP, F, PI]),
     "mi_rb_register_plane": (I, [VP, VP, F, PI]),
+    "mi_rb_register_compound": (I, [VP, I, VP, VP, VP, PI]),
     "mi_rb_register_body": (I, [VP, F, VP, VP, I, I, PI]),
     "mi_rb_upload": (I, [VP]),
     "mi_rb_set_transform": (I, [VP, I, VP, VP]),

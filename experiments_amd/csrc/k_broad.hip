@@ -315,7 +315,7 @@ __global__ __launch_bounds__(FP_BLOCK) void k_find_pairs(
 // its scene.  The scene range is split into chunks of LARGE_CHUNK bodies, one
 // workgroup per (large body, chunk): a counting pass, then an ordered writing
 // pass that places each chunk after the counts of the chunks before it.
-constexpr int LARGE_CHUNK = 4096;
+constexpr int LARGE_CHUNK = RB_LARGE_CHUNK;
 
 __device__ __forceinline__ int large_range(const int *large_list, const int *scene,
                                            const int *scene_start, int q, int c, int &x, int &j0,

@@ -197,10 +197,7 @@ __device__ __forceinline__ bool hull_limits_ok(const HullInfo &hA, const HullInf
 
 // Shapes without vertices: a sphere (no faces; radius in center.w) or a
 // plane (one face: normal and constant, n . x = constant).
-__device__ __forceinline__ int shape_kind(const HullInfo &h) {
-    return h.num_vertices > 0 ? SHAPE_KIND_HULL
-                              : (h.num_faces > 0 ? SHAPE_KIND_PLANE : SHAPE_KIND_SPHERE);
-}
+__device__ __forceinline__ int shape_kind(const HullInfo &h) { return shape_kind_of(h); }
 
 // The contact of a pair with a sphere or a plane: clipHullHullKernel's first
 // branches (satClipHullContacts.cl:1055-1118).  Sphere-sphere and
@@ -510,7 +507,7 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_clip(
     const float4 *__restrict__ planes, const int2 *__restrict__ face_index,
     const int *__restrict__ indices, const float4 *__restrict__ edges, float friction,
     float max_dist, Contact4 *__restrict__ slots, uint32_t *__restrict__ has_contact,
-    int *overflow) {
+    const int *__restrict__ owner, int *overflow) {
     __shared__ float4 sEA[RB_MAX_HULL_EDGES], sEB[RB_MAX_HULL_EDGES];
     __shared__ float4 sPoly[2][RB_MAX_POLY];
     __shared__ float4 sPts[RB_MAX_POLY];
@@ -791,12 +788,90 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_clip(
             for (int i = 0; i < 4; i++) ct.pos[i] = i < nr ? sPts[idx[i]] : mk4(0, 0, 0, 0);
             ct.coeffs = (uint32_t)(0.f * 0xffff) | ((uint32_t)(friction * 0xffff) << 16);
             ct.pair_index = p;
-            ct.body_a = imA == 0.f ? -a : a;
-            ct.body_b = imB == 0.f ? -b : b;
+            // a compound child (proxy body) names its compound
+            const int oa = owner[a], ob = owner[b];
+            ct.body_a = imA == 0.f ? -oa : oa;
+            ct.body_b = imB == 0.f ? -ob : ob;
             slots[p] = ct;
             has_contact[p] = 1;
         }
     }
+}
+
+// ---- compound shapes (findCompoundPairsKernel, sat.cl:763-925) ----
+// Proxy body n + k is child proxy[k].y of body proxy[k].x: its world
+// transform is the body's times the child's (qtRotate(orn, childPos) + pos,
+// qtMul(orn, childOrn)).
+__global__ __launch_bounds__(256) void k_proxy_transforms(int P, int n, const int2 *__restrict__ proxy,
+                                                          const ChildShape *__restrict__ children,
+                                                          float4 *__restrict__ pos,
+                                                          float4 *__restrict__ quat) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= P) return;
+    const int2 pr = proxy[k];
+    const ChildShape c = children[pr.y];
+    const float4 pb = pos[pr.x], qb = quat[pr.x];
+    float4 np = add4(qt_rotate(qb, c.pos), pb);
+    np.w = 0.f;
+    pos[n + k] = np;
+    quat[n + k] = qt_mul(qb, c.orn);
+}
+
+// Child pairs of pair p: (children of A or A) x (children of B or B) when A
+// or B is a compound and not both are static; 0 otherwise.
+__device__ __forceinline__ int compound_children(const HullInfo &h) {
+    return shape_kind_of(h) == SHAPE_KIND_COMPOUND ? h.num_edges : 0;
+}
+__global__ __launch_bounds__(256) void k_compound_count(int max_pairs, const int *__restrict__ num_pairs_ptr,
+                                                        const int2 *__restrict__ pairs,
+                                                        const int *__restrict__ shape,
+                                                        const float *__restrict__ inv_mass,
+                                                        const HullInfo *__restrict__ hulls,
+                                                        uint32_t *__restrict__ cnt) {
+    int npairs = *num_pairs_ptr;
+    if (npairs > max_pairs) npairs = max_pairs;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += gridDim.x * blockDim.x) {
+        const int2 pr = pairs[p];
+        const int ca = compound_children(hulls[shape[pr.x]]);
+        const int cb = compound_children(hulls[shape[pr.y]]);
+        const bool both_static = inv_mass[pr.x] == 0.f && inv_mass[pr.y] == 0.f;
+        cnt[p] = (ca || cb) && !both_static ? (uint32_t)((ca ? ca : 1) * (cb ? cb : 1)) : 0u;
+    }
+}
+// Child pairs in pair order, child of A outer, child of B inner, after the
+// npairs pairs (up to max_pairs; k_compound_total flags the rest).
+__global__ __launch_bounds__(256) void k_compound_write(int max_pairs, const int *__restrict__ num_pairs_ptr,
+                                                        int2 *__restrict__ pairs,
+                                                        const int *__restrict__ shape,
+                                                        const HullInfo *__restrict__ hulls,
+                                                        const int *__restrict__ proxy_first, int n,
+                                                        const uint32_t *__restrict__ cnt,
+                                                        const uint32_t *__restrict__ off) {
+    int npairs = *num_pairs_ptr;
+    if (npairs > max_pairs) npairs = max_pairs;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += gridDim.x * blockDim.x) {
+        const uint32_t c = cnt[p];
+        if (c == 0) continue;
+        const int2 pr = pairs[p];
+        const int cb = compound_children(hulls[shape[pr.y]]);
+        const int nb = cb ? cb : 1;
+        const int fa = proxy_first[pr.x], fb = proxy_first[pr.y];
+        for (uint32_t k = 0; k < c; k++) {
+            const long long dst = (long long)npairs + off[p] + k;
+            if (dst >= max_pairs) break;
+            const int i = (int)k / nb, j = (int)k % nb;
+            pairs[dst] = make_int2(fa >= 0 ? n + fa + i : pr.x, fb >= 0 ? n + fb + j : pr.y);
+        }
+    }
+}
+__global__ void k_compound_total(int max_pairs, int *num_pairs, const uint32_t *total,
+                                 int *overflow) {
+    long long np = (long long)min(*num_pairs, max_pairs) + *total;
+    if (np > max_pairs) {
+        np = max_pairs;
+        atomicOr(overflow, (int)MI_RB_OVERFLOW_PAIRS);
+    }
+    *num_pairs = (int)np;
 }
 
 }  // namespace
@@ -825,6 +900,27 @@ hipError_t launch_narrowphase(const DeviceWorld &d, const StepParams &p, hipStre
                        (const uint32_t *)(d.counters + RB_CNT_NARROW_CAND), d.cand_kv, d.face_best,
                        d.pairs, d.pos, d.quat, d.inv_mass, d.shape, d.hulls, d.verts, d.planes,
                        d.face_index, d.indices, d.edges, p.friction, p.threshold, d.contact_slots,
-                       d.has_contact, d.counters + RB_CNT_OVERFLOW);
+                       d.has_contact, d.owner, d.counters + RB_CNT_OVERFLOW);
+    return hipGetLastError();
+}
+
+hipError_t launch_compound_pairs(const DeviceWorld &d, const StepParams &p, hipStream_t s,
+                                 uint32_t *scan_status) {
+    hipLaunchKernelGGL(k_proxy_transforms, dim3((p.n_proxies + 255) / 256), dim3(256), 0, s,
+                       p.n_proxies, p.n_bodies, d.proxy, d.children, d.pos, d.quat);
+    const int grid = rb_num_cus() * 4;
+    hipLaunchKernelGGL(k_compound_count, dim3(grid), dim3(256), 0, s, p.max_pairs,
+                       d.counters + RB_CNT_PAIRS, d.pairs, d.shape, d.inv_mass, d.hulls,
+                       d.narrow_cand);
+    hipError_t e = launch_exclusive_scan(d.narrow_cand, d.cand_off, p.max_pairs,
+                                         (uint32_t *)(d.counters + RB_CNT_NARROW_CAND), scan_status,
+                                         s, d.counters + RB_CNT_PAIRS);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_compound_write, dim3(grid), dim3(256), 0, s, p.max_pairs,
+                       d.counters + RB_CNT_PAIRS, d.pairs, d.shape, d.hulls, d.proxy_first,
+                       p.n_bodies, d.narrow_cand, d.cand_off);
+    hipLaunchKernelGGL(k_compound_total, dim3(1), dim3(1), 0, s, p.max_pairs,
+                       d.counters + RB_CNT_PAIRS, (const uint32_t *)(d.counters + RB_CNT_NARROW_CAND),
+                       d.counters + RB_CNT_OVERFLOW);
     return hipGetLastError();
 }

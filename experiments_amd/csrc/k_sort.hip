@@ -22,6 +22,7 @@ constexpr int SORT_BLOCK = 256;            // 4 waves
 constexpr int SORT_ITEMS = 16;             // keys per lane
 constexpr int SORT_TILE = SORT_BLOCK * SORT_ITEMS;  // 4096
 constexpr int WAVES = SORT_BLOCK / 64;
+static_assert(SORT_TILE == RB_SORT_TILE, "host-side tile count");
 constexpr int MAX_PASSES = 8;
 
 constexpr uint32_t FLAG_AGG = 1u << 30;
@@ -209,6 +210,7 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_onesweep(const uint2 *__restrict
 constexpr int SCAN_BLOCK = 256;
 constexpr int SCAN_ITEMS = 16;
 constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
+static_assert(SCAN_TILE == RB_SCAN_TILE, "host-side status size");
 constexpr uint64_t SFLAG_AGG = 1ull << 62;
 constexpr uint64_t SFLAG_INC = 2ull << 62;
 constexpr uint64_t SVAL_MASK = (1ull << 62) - 1ull;

@@ -24,9 +24,25 @@ struct HullInfo {
     float4 aabb_max;
 };
 static_assert(sizeof(HullInfo) == 80, "HullInfo layout");
-// A shape without vertices is a sphere (no faces, radius in center.w) or a
-// plane (one face at face_offset: normal xyz and constant w, n . x = w).
-enum { SHAPE_KIND_HULL = 0, SHAPE_KIND_SPHERE = 1, SHAPE_KIND_PLANE = 2 };
+// A shape without vertices is a compound (num_edges children from
+// edge_offset in the child table), a plane (one face at face_offset: normal
+// xyz and constant w, n . x = w) or else a sphere (radius in center.w).
+enum { SHAPE_KIND_HULL = 0, SHAPE_KIND_SPHERE = 1, SHAPE_KIND_PLANE = 2, SHAPE_KIND_COMPOUND = 3 };
+inline __host__ __device__ int shape_kind_of(const HullInfo &h) {
+    return h.num_vertices > 0 ? SHAPE_KIND_HULL
+                              : (h.num_edges > 0 ? SHAPE_KIND_COMPOUND
+                                                 : (h.num_faces > 0 ? SHAPE_KIND_PLANE
+                                                                    : SHAPE_KIND_SPHERE));
+}
+// A child of a compound shape: a hull shape at a position and orientation
+// in the compound's frame (btGpuChildShape, 48 bytes).
+struct ChildShape {
+    float4 pos;   // w unused
+    float4 orn;   // quaternion xyzw
+    int shape;    // hull shape index
+    int pad[3];
+};
+static_assert(sizeof(ChildShape) == 48, "ChildShape layout");
 
 // Contact manifold in the Contact4 layout (dynamics/basic_demo/Stubs/AdlContact4.h:26-59).
 struct Contact4 {
@@ -98,6 +114,11 @@ struct DeviceWorld {
     float4 *pos, *quat, *linvel, *angvel;
     float *inv_mass;
     int *shape, *scene, *is_large;
+    // compound children as proxy bodies n .. n+P-1 of pos/quat/inv_mass/shape
+    ChildShape *children;
+    int2 *proxy;          // per proxy: (parent body, child index)
+    int *proxy_first;     // per body: its first proxy (-1: not a compound)
+    int *owner;           // per body and proxy: the body a contact names
     float4 *inv_inertia_local;   // diagonal
     float4 *inv_inertia_world;   // 3 rows per body
     int *scene_start, *scene_count;
@@ -206,7 +227,8 @@ struct StepParams {
     // launch shapes (mi_rb_tuning)
     int solve_block;     // threads per scene-tile solve workgroup: 256 or 512
     int narrow_waves;    // narrowphase waves per SIMD (grid = CUs * 4 * narrow_waves)
-    int has_vertexless;  // some shape is a sphere or a plane
+    int has_vertexless;  // some shape is a sphere, a plane or a compound
+    int n_proxies;       // compound children bodies (after the n bodies)
 };
 
 // ---- launch interfaces (implemented in the .hip files) ----
@@ -233,6 +255,10 @@ hipError_t launch_setup(const DeviceWorld &d, const StepParams &p, hipStream_t s
 hipError_t launch_cell_start(const DeviceWorld &d, const StepParams &p, hipStream_t s);
 hipError_t launch_find_pairs(const DeviceWorld &d, const StepParams &p, hipStream_t s);
 hipError_t launch_emit_pairs(const DeviceWorld &d, const StepParams &p, hipStream_t s);
+// compound pairs: proxy transforms, then one child pair per (child of A,
+// child of B) appended after the pairs (the compound pair itself gets none)
+hipError_t launch_compound_pairs(const DeviceWorld &d, const StepParams &p, hipStream_t s,
+                                 uint32_t *scan_status);
 hipError_t launch_narrowphase(const DeviceWorld &d, const StepParams &p, hipStream_t s,
                               const SortWork &sw, uint32_t *scan_status);
 hipError_t launch_constraints(const DeviceWorld &d, const StepParams &p, hipStream_t s);
@@ -243,6 +269,13 @@ hipError_t launch_solve(const DeviceWorld &d, const StepParams &p, hipStream_t s
 hipError_t launch_copy_transforms(const DeviceWorld &d, int n, float4 *dst, hipStream_t s);
 hipError_t launch_integrate(const DeviceWorld &d, const StepParams &p, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
+// elements per tile of the radix sort and of the scan (their look-back
+// status arrays hold one entry per tile)
+constexpr int RB_SORT_TILE = 4096;
+constexpr int RB_SCAN_TILE = 4096;
+constexpr int RB_LARGE_CHUNK = 4096;  // bodies per chunk of the large-body pair search
+inline int rb_sort_tiles(int n) { return (n + RB_SORT_TILE - 1) / RB_SORT_TILE; }
+inline size_t rb_scan_status_words(int n) { return 4 + 2 * (size_t)((n + RB_SCAN_TILE - 1) / RB_SCAN_TILE); }
 hipError_t launch_mlp_warm(const DeviceWorld &d, const StepParams &p, hipStream_t s);
 hipError_t launch_mlp_targets(const DeviceWorld &d, int capacity, float *x_out, float *t_out,
                               hipStream_t s);

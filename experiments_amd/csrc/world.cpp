@@ -84,6 +84,7 @@ struct World {
 
     // host shapes
     std::vector<HullInfo> hulls;
+    std::vector<ChildShape> children;  // compound children (HullInfo edge_offset/num_edges)
     std::vector<float4> verts, planes, edges;
     std::vector<int2> face_index;
     std::vector<int> indices;
@@ -264,12 +265,29 @@ mi_rb_status derive_and_upload(World &w) {
     RB_CHECK(w.alloc(&d.edges, w.edges.size()));
     RB_CHECK(w.alloc(&d.face_index, w.face_index.size()));
     RB_CHECK(w.alloc(&d.indices, w.indices.size()));
-    RB_CHECK(w.alloc(&d.pos, n));
-    RB_CHECK(w.alloc(&d.quat, n));
+    // compound children become proxy bodies n .. n+P-1 of the narrowphase
+    std::vector<int> proxy_first(n, -1), owner;
+    std::vector<int2> proxies;
+    for (int i = 0; i < n; i++) {
+        const HullInfo &h = w.hulls[w.shape[i]];
+        if (shape_kind_of(h) != SHAPE_KIND_COMPOUND) continue;
+        proxy_first[i] = (int)proxies.size();
+        for (int c = 0; c < h.num_edges; c++) proxies.push_back(make_int2(i, h.edge_offset + c));
+    }
+    const int P = (int)proxies.size();
+    p.n_proxies = P;
+    for (int i = 0; i < n; i++) owner.push_back(i);
+    for (const int2 &pr : proxies) owner.push_back(pr.x);
+    RB_CHECK(w.alloc(&d.pos, (size_t)n + P));
+    RB_CHECK(w.alloc(&d.quat, (size_t)n + P));
     RB_CHECK(w.alloc(&d.linvel, n));
     RB_CHECK(w.alloc(&d.angvel, n));
-    RB_CHECK(w.alloc(&d.inv_mass, n));
-    RB_CHECK(w.alloc(&d.shape, n));
+    RB_CHECK(w.alloc(&d.inv_mass, (size_t)n + P));
+    RB_CHECK(w.alloc(&d.shape, (size_t)n + P));
+    RB_CHECK(w.alloc(&d.owner, (size_t)n + P));
+    RB_CHECK(w.alloc(&d.proxy_first, n));
+    RB_CHECK(w.alloc(&d.proxy, std::max(P, 1)));
+    RB_CHECK(w.alloc(&d.children, std::max((int)w.children.size(), 1)));
     RB_CHECK(w.alloc(&d.scene, n));
     RB_CHECK(w.alloc(&d.is_large, n));
     RB_CHECK(w.alloc(&d.inv_inertia_local, n));
@@ -367,18 +385,18 @@ mi_rb_status derive_and_upload(World &w) {
     RB_CHECK(w.alloc(&d.large_slots, (size_t)large_off[p.n_large] + 1));
     int max_scene = 0;
     for (int sc = 0; sc < n_scenes; sc++) max_scene = std::max(max_scene, scene_count[sc]);
-    p.large_chunks = std::max(1, (max_scene + 4095) / 4096);
+    p.large_chunks = std::max(1, (max_scene + RB_LARGE_CHUNK - 1) / RB_LARGE_CHUNK);
     RB_CHECK(w.alloc(&d.large_chunk_cnt, (size_t)std::max(p.n_large, 1) * p.large_chunks));
     // sort / scan work
     int sort_cap = std::max(std::max(n, maxC), maxP);
     w.sw.max_n = sort_cap;
-    w.sw.max_tiles = (sort_cap + 4095) / 4096;
+    w.sw.max_tiles = rb_sort_tiles(sort_cap);
     RB_CHECK(w.alloc(&w.sw.tmp, sort_cap));
     RB_CHECK(w.alloc(&w.sw.hist, 8 * 256));
     RB_CHECK(w.alloc(&w.sw.status, sort_status_words(sort_cap, 8)));
     RB_CHECK(w.alloc(&w.sw.tile_ctr, 16));
     int scan_cap = std::max(n, maxP);
-    RB_CHECK(w.alloc(&w.scan_status, 4 + 2 * (size_t)((scan_cap + 4095) / 4096)));
+    RB_CHECK(w.alloc(&w.scan_status, rb_scan_status_words(scan_cap)));
 
     // ---- uploads ----
     hipStream_t s = w.stream;
@@ -397,6 +415,20 @@ mi_rb_status derive_and_upload(World &w) {
     RB_CHECK(up(d.angvel, w.angvel.data(), sizeof(float4) * n));
     RB_CHECK(up(d.inv_mass, w.inv_mass.data(), sizeof(float) * n));
     RB_CHECK(up(d.shape, w.shape.data(), sizeof(int) * n));
+    RB_CHECK(up(d.owner, owner.data(), sizeof(int) * owner.size()));
+    RB_CHECK(up(d.proxy_first, proxy_first.data(), sizeof(int) * n));
+    if (P > 0) {
+        std::vector<float> pim(P);
+        std::vector<int> psh(P);
+        for (int k = 0; k < P; k++) {
+            pim[k] = w.inv_mass[proxies[k].x];
+            psh[k] = w.children[proxies[k].y].shape;
+        }
+        RB_CHECK(up(d.inv_mass + n, pim.data(), sizeof(float) * P));
+        RB_CHECK(up(d.shape + n, psh.data(), sizeof(int) * P));
+        RB_CHECK(up(d.proxy, proxies.data(), sizeof(int2) * P));
+        RB_CHECK(up(d.children, w.children.data(), sizeof(ChildShape) * w.children.size()));
+    }
     RB_CHECK(up(d.scene, w.scene.data(), sizeof(int) * n));
     RB_CHECK(up(d.is_large, w.is_large.data(), sizeof(int) * n));
     RB_CHECK(up(d.inv_inertia_local, w.inv_inertia_local.data(), sizeof(float4) * n));
@@ -440,6 +472,7 @@ mi_rb_status enqueue_step(World &w, int last_stage) {
     RB_CHECK(launch_exclusive_scan(d.pair_count, d.pair_off, p.n_bodies,
                                    (uint32_t *)(d.counters + RB_CNT_SORT_TILE), w.scan_status, s));
     RB_CHECK(launch_emit_pairs(d, p, s));
+    if (p.n_proxies > 0) RB_CHECK(launch_compound_pairs(d, p, s, w.scan_status));
     if (ev) RB_CHECK(hipEventRecord(w.ev[3], s));
     if (last_stage < MI_RB_STAGE_NARROWPHASE) return MI_RB_OK;
     RB_CHECK(launch_narrowphase(d, p, s, w.sw, w.scan_status));
@@ -604,6 +637,62 @@ static mi_rb_status add_vertexless_shape(World *w, float radius, const float *pl
     }
     w->hull_hext_norm.push_back(std::sqrt(3.f) * e);
     w->hulls.push_back(hi);
+    *out_shape = (int)w->hulls.size() - 1;
+    w->uploaded = false;
+    return MI_RB_OK;
+}
+
+extern "C" mi_rb_status mi_rb_register_compound(void *world, int n, const int *child_shapes,
+                                                const float *child_pos, const float *child_orn,
+                                                int *out_shape) {
+    World *w = W(world);
+    if (!w || n < 1 || n > 64 || !child_shapes || !child_pos || !child_orn || !out_shape)
+        return MI_RB_INVALID_ARG;
+    if ((int)w->hulls.size() >= w->cfg.max_shapes) return MI_RB_CAPACITY;
+    float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+    std::vector<ChildShape> kids(n);
+    for (int i = 0; i < n; i++) {
+        const int cs = child_shapes[i];
+        if (cs < 0 || cs >= (int)w->hulls.size() || shape_kind_of(w->hulls[cs]) != SHAPE_KIND_HULL)
+            return MI_RB_INVALID_ARG;
+        const float *q = child_orn + 4 * i;
+        const float qn = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+        if (!(qn > 0.5f && qn < 2.f)) return MI_RB_INVALID_ARG;
+        ChildShape &k = kids[i];
+        k.pos = make_float4(child_pos[3 * i], child_pos[3 * i + 1], child_pos[3 * i + 2], 0.f);
+        k.orn = make_float4(q[0], q[1], q[2], q[3]);
+        k.shape = cs;
+        k.pad[0] = k.pad[1] = k.pad[2] = 0;
+        // the child's local AABB, rotated into the compound frame
+        const HullInfo &h = w->hulls[cs];
+        const float c[3] = {(h.aabb_max.x + h.aabb_min.x) * 0.5f, (h.aabb_max.y + h.aabb_min.y) * 0.5f,
+                            (h.aabb_max.z + h.aabb_min.z) * 0.5f};
+        const float e[3] = {(h.aabb_max.x - h.aabb_min.x) * 0.5f, (h.aabb_max.y - h.aabb_min.y) * 0.5f,
+                            (h.aabb_max.z - h.aabb_min.z) * 0.5f};
+        const float x = q[0], y = q[1], z = q[2], s0 = q[3];
+        const float m[3][3] = {{1 - 2 * (y * y + z * z), 2 * (x * y - s0 * z), 2 * (x * z + s0 * y)},
+                               {2 * (x * y + s0 * z), 1 - 2 * (x * x + z * z), 2 * (y * z - s0 * x)},
+                               {2 * (x * z - s0 * y), 2 * (y * z + s0 * x), 1 - 2 * (x * x + y * y)}};
+        for (int r = 0; r < 3; r++) {
+            const float cc = m[r][0] * c[0] + m[r][1] * c[1] + m[r][2] * c[2] + child_pos[3 * i + r];
+            const float ee = std::fabs(m[r][0]) * e[0] + std::fabs(m[r][1]) * e[1] +
+                             std::fabs(m[r][2]) * e[2];
+            lo[r] = std::min(lo[r], cc - ee);
+            hi[r] = std::max(hi[r], cc + ee);
+        }
+    }
+    HullInfo hi_{};
+    hi_.face_offset = (int)w->planes.size();
+    hi_.vertex_offset = (int)w->verts.size();
+    hi_.index_offset = (int)w->indices.size();
+    hi_.edge_offset = (int)w->children.size();
+    hi_.num_edges = n;
+    hi_.aabb_min = make_float4(lo[0], lo[1], lo[2], 0.f);
+    hi_.aabb_max = make_float4(hi[0], hi[1], hi[2], 0.f);
+    for (const ChildShape &k : kids) w->children.push_back(k);
+    const float hx = (hi[0] - lo[0]) * 0.5f, hy = (hi[1] - lo[1]) * 0.5f, hz = (hi[2] - lo[2]) * 0.5f;
+    w->hull_hext_norm.push_back(std::sqrt(hx * hx + hy * hy + hz * hz));
+    w->hulls.push_back(hi_);
     *out_shape = (int)w->hulls.size() - 1;
     w->uploaded = false;
     return MI_RB_OK;
@@ -1003,6 +1092,13 @@ extern "C" mi_rb_status mi_rb_read_buffer(void *world, int which, void *host, si
                                           int *count) {
     World *w = W(world);
     if (!w || !count) return MI_RB_INVALID_ARG;
+    if (which == MI_RB_BUF_CHILDREN) {  // host-side: compound children
+        *count = (int)w->children.size();
+        if (host && max_bytes && *count)
+            std::memcpy(host, w->children.data(),
+                        std::min(max_bytes, sizeof(ChildShape) * w->children.size()));
+        return MI_RB_OK;
+    }
     if (which >= MI_RB_BUF_HULLS && which <= MI_RB_BUF_SHAPE) {  // host-side registration arrays
         const void *src = nullptr;
         size_t elem = 0;
@@ -1153,11 +1249,12 @@ extern "C" mi_rb_status mi_rb_save_state(void *world, const char *path) {
         mi_rb_status st = mi_rb_readback(world);
         if (st != MI_RB_OK) return st;
     }
-    std::vector<char> b = {'M', 'I', 'R', 'B', 1, 0, 0, 0};
+    std::vector<char> b = {'M', 'I', 'R', 'B', 2, 0, 0, 0};
     put(b, w->hulls); put(b, w->verts); put(b, w->planes); put(b, w->edges);
     put(b, w->face_index); put(b, w->indices); put(b, w->hull_hext_norm);
     put(b, w->pos); put(b, w->quat); put(b, w->linvel); put(b, w->angvel);
     put(b, w->inv_inertia_local); put(b, w->inv_mass); put(b, w->shape); put(b, w->scene);
+    put(b, w->children);
     FILE *f = fopen(path, "wb");
     if (!f) return MI_RB_INVALID_ARG;
     size_t wr = fwrite(b.data(), 1, b.size(), f);
@@ -1175,13 +1272,17 @@ extern "C" mi_rb_status mi_rb_load_state(void *world, const char *path) {
     size_t r;
     while ((r = fread(buf, 1, sizeof(buf), f)) > 0) b.insert(b.end(), buf, buf + r);
     fclose(f);
-    if (b.size() < 8 || std::memcmp(b.data(), "MIRB", 4) != 0 || b[4] != 1) return MI_RB_INVALID_ARG;
+    // version 1: no compound children; version 2 adds them
+    if (b.size() < 8 || std::memcmp(b.data(), "MIRB", 4) != 0 || (b[4] != 1 && b[4] != 2))
+        return MI_RB_INVALID_ARG;
     size_t o = 8;
     bool ok = get(b, o, w->hulls) && get(b, o, w->verts) && get(b, o, w->planes) &&
               get(b, o, w->edges) && get(b, o, w->face_index) && get(b, o, w->indices) &&
               get(b, o, w->hull_hext_norm) && get(b, o, w->pos) && get(b, o, w->quat) &&
               get(b, o, w->linvel) && get(b, o, w->angvel) && get(b, o, w->inv_inertia_local) &&
               get(b, o, w->inv_mass) && get(b, o, w->shape) && get(b, o, w->scene);
+    if (ok && b[4] == 2) ok = get(b, o, w->children);
+    else if (ok) w->children.clear();
     if (!ok) return MI_RB_INVALID_ARG;
     w->free_device();
     return MI_RB_OK;
@@ -1269,10 +1370,10 @@ extern "C" mi_rb_status mi_rb_grid_create(const mi_rb_grid_desc *desc, int max_a
               dalloc(&g->sw.hist, 8 * 256) == hipSuccess &&
               dalloc(&g->sw.status, sort_status_words(n, 8)) == hipSuccess &&
               dalloc(&g->sw.tile_ctr, 16) == hipSuccess &&
-              dalloc(&g->scan_status, 4 + 2 * (size_t)((n + 4095) / 4096)) == hipSuccess;
+              dalloc(&g->scan_status, rb_scan_status_words(n)) == hipSuccess;
     if (!ok) { g->release(); delete g; return MI_RB_DEVICE; }
     g->sw.max_n = n;
-    g->sw.max_tiles = (n + 4095) / 4096;
+    g->sw.max_tiles = rb_sort_tiles(n);
     hipMemset(g->cell_start, 0xff, sizeof(uint32_t) * ncells);
     hipMemset(g->overflow, 0, 16);
     for (auto &e : g->ev) hipEventCreate(&e);
@@ -1349,10 +1450,10 @@ extern "C" mi_rb_status mi_rb_sap_create(const mi_rb_sap_desc *desc, int max_aab
               dalloc(&g->sw.tmp, n) == hipSuccess && dalloc(&g->sw.hist, 8 * 256) == hipSuccess &&
               dalloc(&g->sw.status, sort_status_words(n, 8)) == hipSuccess &&
               dalloc(&g->sw.tile_ctr, 16) == hipSuccess &&
-              dalloc(&g->scan_status, 4 + 2 * (size_t)((n + 4095) / 4096)) == hipSuccess;
+              dalloc(&g->scan_status, rb_scan_status_words(n)) == hipSuccess;
     if (!ok) { g->release(); delete g; return MI_RB_DEVICE; }
     g->sw.max_n = n;
-    g->sw.max_tiles = (n + 4095) / 4096;
+    g->sw.max_tiles = rb_sort_tiles(n);
     hipMemset(g->overflow, 0, 16);
     for (auto &e : g->ev) hipEventCreate(&e);
     if (hipDeviceSynchronize() != hipSuccess) { g->release(); delete g; return MI_RB_DEVICE; }

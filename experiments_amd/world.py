@@ -86,6 +86,20 @@ class World:
         self.num_shapes += 1
         return out.value
 
+    def add_compound(self, child_shapes, child_pos, child_orn):
+        """A rigid union of hull shapes: child i at child_pos[i] with
+        orientation child_orn[i] (xyzw) in the compound's frame."""
+        sh = np.ascontiguousarray(child_shapes, np.int32)
+        cp = _f32(child_pos).reshape(-1, 3).copy()
+        co = _f32(child_orn).reshape(-1, 4).copy()
+        if not (len(sh) == len(cp) == len(co)):
+            raise ValueError("one position and one orientation per child")
+        out = ctypes.c_int()
+        L.call("mi_rb_register_compound", self.h, len(sh), _ptr(sh), _ptr(cp), _ptr(co),
+               ctypes.byref(out))
+        self.num_shapes += 1
+        return out.value
+
     def add_body(self, mass, pos, quat_xyzw, shape, scene=0):
         p = _f32(pos)
         q = _f32(quat_xyzw)
@@ -266,6 +280,20 @@ class CLPhysicsDemo:
 
     def registerPlaneShape(self, planeNormal, planeConstant):
         return self.world.add_plane(planeNormal, planeConstant)
+
+    def registerCompoundShape(self, childShapes):
+        """childShapes: records with m_childPosition, m_childOrientation and
+        m_shapeIndex (btGpuChildShape), or (shape, position, orientation)."""
+        sh, cp, co = [], [], []
+        for c in childShapes:
+            if isinstance(c, (tuple, list)):
+                s_, p_, o_ = c
+            else:
+                s_, p_, o_ = c.m_shapeIndex, c.m_childPosition, c.m_childOrientation
+            sh.append(int(s_))
+            cp.append(list(p_)[:3])
+            co.append(list(o_)[:4])
+        return self.world.add_compound(sh, cp, co)
 
     def registerPhysicsInstance(self, mass, position, orientation, collidableShape, userIndex):
         idx = self.world.add_body(mass, position[:3], orientation[:4], collidableShape, 0)

@@ -137,7 +137,8 @@ enum {
     MI_RB_BUF_PREV_KEYS = 21,  /* int2 (a, b) per cached pair: impulse cache of the last solve */
     MI_RB_BUF_PREV_IMPULSES = 22, /* float4 per cached pair: sum lambda_n, friction 0, 1, points */
     MI_RB_BUF_PREDICTIONS = 23, /* float[3] per contact: network output (learned warm start) */
-    MI_RB_BUF_PREV_POINTS = 24 /* float4[4] per cached pair: point xyz, its lambda_n in w */
+    MI_RB_BUF_PREV_POINTS = 24, /* float4[4] per cached pair: point xyz, its lambda_n in w */
+    MI_RB_BUF_CHILDREN = 25    /* 48 B per compound child: float4 pos, float4 orn, int shape, pad */
 };
 
 /* stages at which mi_rb_step_partial stops */
@@ -174,8 +175,17 @@ mi_rb_status mi_rb_register_sphere(void *world, float radius, int *out_shape);
  * gives MI_RB_INVALID_ARG); local AABB +-1e30.  Contacts with spheres only. */
 mi_rb_status mi_rb_register_plane(void *world, const float normal[3], float constant,
                                   int *out_shape);
-/* registerCompoundShape / registerConcaveMesh (CLPhysicsDemo.h:58-66): not on
- * this path. */
+/* CLPhysicsDemo::registerCompoundShape(childShapes) (CLPhysicsDemo.h:66;
+ * btGpuNarrowPhaseAndSolver::registerCompoundShape): a rigid union of n hull
+ * shapes, child i at child_pos[3i..] with orientation child_orn[4i..] (xyzw)
+ * in the compound's frame.  Each child pair is found and clipped like a hull
+ * pair with the child's world transform (sat.cl:639-925,
+ * satClipHullContacts.cl:1184-1310); the contact names the compound body.
+ * At most 64 children, each a hull shape. */
+mi_rb_status mi_rb_register_compound(void *world, int n, const int *child_shapes,
+                                     const float *child_pos, const float *child_orn,
+                                     int *out_shape);
+/* registerConcaveMesh (CLPhysicsDemo.h:58-66): not on this path. */
 mi_rb_status mi_rb_register_unsupported_shape(void *world, int kind);
 
 /* CLPhysicsDemo::registerPhysicsInstance(mass, pos, orn xyzw, collidable, user)

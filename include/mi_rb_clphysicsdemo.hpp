@@ -15,6 +15,8 @@
 #ifndef MI_RB_CLPHYSICSDEMO_HPP
 #define MI_RB_CLPHYSICSDEMO_HPP
 
+#include <vector>
+
 #include "mi_rb.h"
 
 struct CLPhysicsDemo {
@@ -73,11 +75,22 @@ struct CLPhysicsDemo {
         m_status = mi_rb_register_unsupported_shape(m_world, 3);
         return -1;
     }
+    /* childShapes: a container (size(), operator[]) of btGpuChildShape-like
+     * records: float m_childPosition[4], m_childOrientation[4], int m_shapeIndex */
     template <class Children>
     int registerCompoundShape(Children *childShapes) {
-        (void)childShapes;
-        m_status = mi_rb_register_unsupported_shape(m_world, 4);
-        return -1;
+        const int n = (int)childShapes->size();
+        std::vector<int> shapes(n);
+        std::vector<float> pos(3 * (size_t)n), orn(4 * (size_t)n);
+        for (int i = 0; i < n; i++) {
+            const auto &c = (*childShapes)[i];
+            shapes[i] = c.m_shapeIndex;
+            for (int k = 0; k < 3; k++) pos[3 * i + k] = c.m_childPosition[k];
+            for (int k = 0; k < 4; k++) orn[4 * i + k] = c.m_childOrientation[k];
+        }
+        int shape = -1;
+        m_status = mi_rb_register_compound(m_world, n, shapes.data(), pos.data(), orn.data(), &shape);
+        return m_status == MI_RB_OK ? shape : -1;
     }
 
     /* position: float[3] (or [4]), orientation: quaternion xyzw */

@@ -206,6 +206,25 @@ def reduce_manifold(points, normal):
     return idx[:n].copy()
 
 
+CHILD_DTYPE = np.dtype([("pos", "<f4", 4), ("orn", "<f4", 4), ("shape", "<i4"), ("pad", "<i4", 3)])
+assert CHILD_DTYPE.itemsize == 48
+
+
+def compound_proxies(proxy, children, pos, quat):
+    """pos/quat of the n bodies extended by the world transforms of the
+    compound children proxy[k] = (body, child index)."""
+    n, P = len(pos), len(proxy)
+    pos4, quat4 = np.zeros((n + P, 4), np.float32), np.zeros((n + P, 4), np.float32)
+    pos4[:n] = f4(pos)
+    quat4[:n] = np.ascontiguousarray(quat, np.float32)
+    if P:
+        pr = np.ascontiguousarray(proxy, np.int32).reshape(-1, 2)
+        ch = np.ascontiguousarray(children, CHILD_DTYPE)
+        lib().orc_compound_proxies(ctypes.c_int(P), ctypes.c_int(n), _p(pr), _p(ch), _p(pos4),
+                                   _p(quat4))
+    return pos4, quat4
+
+
 def narrowphase(pairs, pos, quat, inv_mass, shape, shapes, friction=0.7, max_dist=0.02):
     pairs = np.ascontiguousarray(pairs, np.int32).reshape(-1, 2)
     P = len(pairs)

@@ -654,8 +654,9 @@ static int special_contact(int p, int a, int b, const orc_hull *hA, const orc_hu
                            const float *pos, const float *quat, const float *planes,
                            const float *inv_mass, float friction, orc_contact *c)
 {
-    int kA = hA->num_vertices > 0 ? 0 : (hA->num_faces > 0 ? 2 : 1);
-    int kB = hB->num_vertices > 0 ? 0 : (hB->num_faces > 0 ? 2 : 1);
+    /* 0 hull, 1 sphere, 2 plane, 3 compound (its pair is expanded into child pairs) */
+    int kA = hA->num_vertices > 0 ? 0 : (hA->num_edges > 0 ? 3 : (hA->num_faces > 0 ? 2 : 1));
+    int kB = hB->num_vertices > 0 ? 0 : (hB->num_edges > 0 ? 3 : (hB->num_faces > 0 ? 2 : 1));
     memset(c, 0, sizeof(*c));
     c->coeffs = (uint32_t)(0.f * 0xffff) | ((uint32_t)(friction * 0xffff) << 16);
     c->pair_index = p;
@@ -693,6 +694,26 @@ static int special_contact(int p, int a, int b, const orc_hull *hA, const orc_hu
     c->body_a = inv_mass[ip] == 0 ? -ip : ip;
     c->body_b = inv_mass[is] == 0 ? -is : is;
     return 1;
+}
+
+/* World transforms of compound children (findCompoundPairsKernel and
+ * clipCompoundsHullHullKernel, sat.cl:787-797, satClipHullContacts.cl:1228-1248):
+ * proxy body n + k is child proxy[2k+1] of body proxy[2k]; position
+ * qtRotate(orn, childPos) + pos (w = 0), orientation qtMul(orn, childOrn).
+ * children: 12 floats per child (position xyzw, orientation xyzw, shape and
+ * padding). */
+void orc_compound_proxies(int P, int n, const int32_t *proxy, const float *children, float *pos,
+                          float *quat)
+{
+    for (int k = 0; k < P; k++) {
+        int b = proxy[2 * k], c = proxy[2 * k + 1];
+        const float *ch = children + 12 * c;
+        f4 pb = ld4(pos + 4 * b), qb = ld4(quat + 4 * b);
+        f4 np = add4(qt_rotate(qb, mk4(ch[0], ch[1], ch[2], 0.f)), pb);
+        np.w = 0.f;
+        st4(pos + 4 * (n + k), np);
+        st4(quat + 4 * (n + k), qt_mul(qb, ld4(ch + 4)));
+    }
 }
 
 /* clipHullHullKernel convex-convex branch (satClipHullContacts.cl:1019-1181)

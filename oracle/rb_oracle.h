@@ -91,6 +91,8 @@ int orc_clip_hull_hull(const float sep[4], const orc_hull *hA, const orc_hull *h
                        float *out /* 64 float4 */);
 int orc_clip_face(const float *in, int n_in, const float nrm[4], float eq, float *out);
 int orc_reduce_manifold(const float *pts, int n, const float normal[4], int idx[4]);
+void orc_compound_proxies(int P, int n, const int32_t *proxy, const float *children, float *pos,
+                          float *quat);
 int orc_narrowphase(int num_pairs, const int32_t *pairs, const float *pos, const float *quat,
                     const float *inv_mass, const int *shape, const orc_hull *hulls,
                     const float *verts, const float *planes, const int *face_index,

@@ -21,6 +21,18 @@ class Snapshot:
                                self.sh["face_index"], self.sh["indices"], self.sh["edges"])
         self.is_large = w.read(L.BUF_IS_LARGE, np.int32)
         self.cfg = w.cfg
+        # compound children as proxy bodies after the n bodies (in body order,
+        # children in order), as the library numbers them
+        self.children = w.read(L.BUF_CHILDREN, np.uint8, (48,)).view(O.CHILD_DTYPE).reshape(-1)
+        hulls = self.sh["hulls"]
+        self.proxy_first = np.full(len(self.sh["shape"]), -1, np.int64)
+        proxies = []
+        for i, sid in enumerate(self.sh["shape"]):
+            h = hulls[sid]
+            if h["num_vertices"] == 0 and h["num_edges"] > 0:
+                self.proxy_first[i] = len(proxies)
+                proxies += [(i, int(h["edge_offset"]) + c) for c in range(int(h["num_edges"]))]
+        self.proxy = np.array(proxies, np.int32).reshape(-1, 2)
         sc = self.sh["scene"]
         ns = int(sc.max()) + 1 if len(sc) else 1
         self.scene_start = np.array([int(np.argmax(sc == s)) for s in range(ns)], np.int32)
@@ -50,11 +62,52 @@ def oracle_setup(s):
     return lv, av, invI
 
 
+def expand_compound_pairs(s, pairs):
+    """The pair list the narrowphase sees: the broadphase pairs, then for
+    each pair with a compound (not both static) one pair per (child of A or
+    A, child of B or B), child of A outer (findCompoundPairsKernel,
+    sat.cl:763-925), in pair order."""
+    if len(s.proxy) == 0:
+        return pairs
+    hulls, shape, im = s.sh["hulls"], s.sh["shape"], s.sh["inv_mass"]
+    n = len(shape)
+
+    def kids(b):
+        h = hulls[shape[b]]
+        return int(h["num_edges"]) if h["num_vertices"] == 0 and h["num_edges"] > 0 else 0
+
+    extra = []
+    for a, b in np.asarray(pairs).reshape(-1, 2):
+        ca, cb = kids(a), kids(b)
+        if not (ca or cb) or (im[a] == 0 and im[b] == 0):
+            continue
+        for i in range(ca or 1):
+            for j in range(cb or 1):
+                extra.append((n + s.proxy_first[a] + i if ca else a,
+                              n + s.proxy_first[b] + j if cb else b))
+    return np.concatenate([np.asarray(pairs).reshape(-1, 2),
+                           np.array(extra, np.int32).reshape(-1, 2)]).astype(np.int32)
+
+
 def oracle_contacts(s, pairs):
     st, sh, c = s.state, s.sh, s.cfg
-    slots, has = O.narrowphase(pairs, st["pos"], st["quat"], sh["inv_mass"], sh["shape"],
-                               s.shapes, c.friction, c.contact_threshold)
-    return slots[has == 1]
+    pos, quat, im, shape = st["pos"], st["quat"], sh["inv_mass"], sh["shape"]
+    owner = np.arange(len(shape))
+    if len(s.proxy):
+        pairs = expand_compound_pairs(s, pairs)
+        pos, quat = O.compound_proxies(s.proxy, s.children, pos, quat)
+        im = np.concatenate([im, im[s.proxy[:, 0]]]).astype(np.float32)
+        shape = np.concatenate([shape, s.children["shape"][s.proxy[:, 1]]]).astype(np.int32)
+        owner = np.concatenate([owner, s.proxy[:, 0]])
+    slots, has = O.narrowphase(pairs, pos, quat, im, shape, s.shapes, c.friction,
+                               c.contact_threshold)
+    out = slots[has == 1].copy()
+    if len(s.proxy):
+        # a compound child names its compound (sign: static)
+        for k in ("body_a", "body_b"):
+            b = owner[np.abs(out[k])]
+            out[k] = np.where(im[b] == 0, -b, b)
+    return out
 
 
 def gpu_order(w):
