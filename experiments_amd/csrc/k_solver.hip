@@ -1205,8 +1205,7 @@ hipError_t launch_tiles(const DeviceWorld &d, const StepParams &p, hipStream_t s
 hipError_t launch_solve(const DeviceWorld &d, const StepParams &p, hipStream_t s) {
     const int S = p.normal_iters + p.friction_iters;
     const int warm = p.warm_start != MI_RB_WARM_ZERO;
-    hipLaunchKernelGGL(k_clear_done, dim3(1), dim3(1), 0, s, d.counters);
-    launch_fill_u32(reinterpret_cast<uint32_t *>(d.residual), 0u, (size_t)2 * (S + 1), s);
+    // (the step's first kernel has cleared the converged flag and the residuals)
     // scene tiles: one workgroup per scene, 256 or 512 threads
 #define RB_SOLVE_ARGS                                                                         \
     d.tile_hdr, d.tile_color_off, d.tile_bodies, d.pk, d.linvel, d.angvel, d.residual,        \

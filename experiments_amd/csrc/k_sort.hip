@@ -35,6 +35,17 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n) {
     for (; i < n; i += stride) p[i] = v;
 }
 
+// The three zero fills a sort starts from (digit counts, look-back status
+// words, tile counters) in one launch.
+__global__ void k_sort_clear(uint32_t *hist, size_t n_hist, uint32_t *status, size_t n_status,
+                             uint32_t *tile_ctr, size_t n_ctr) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (size_t i = i0; i < n_status; i += stride) status[i] = 0u;
+    for (size_t i = i0; i < n_hist; i += stride) hist[i] = 0u;
+    for (size_t i = i0; i < n_ctr; i += stride) tile_ctr[i] = 0u;
+}
+
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
     int lane = __lane_id();
 #pragma unroll
@@ -321,9 +332,9 @@ hipError_t radix_sort_impl(uint2 *kv, int n, int sort_bits, const SortWork &w, h
     if (passes > MAX_PASSES || tiles > w.max_tiles) return hipErrorInvalidValue;
     size_t zero_words = (size_t)passes * RADIX;
     size_t status_words = (size_t)passes * tiles * RADIX;
-    hipLaunchKernelGGL(k_fill_u32, dim3(64), dim3(256), 0, s, w.hist, 0u, zero_words);
-    hipLaunchKernelGGL(k_fill_u32, dim3(256), dim3(256), 0, s, w.status, 0u, status_words);
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, w.tile_ctr, 0u, (size_t)MAX_PASSES);  // the error word after them is sticky
+    // (the error word after the tile counters is sticky: not cleared)
+    hipLaunchKernelGGL(k_sort_clear, dim3(256), dim3(256), 0, s, w.hist, zero_words, w.status,
+                       status_words, w.tile_ctr, (size_t)MAX_PASSES);
     hipLaunchKernelGGL(k_histogram<RBITS>, dim3(tiles), dim3(SORT_BLOCK), 0, s, kv, n, passes,
                        sort_bits, w.hist, n_dev);
     uint2 *src = kv, *dst = w.tmp;

@@ -57,9 +57,12 @@ int ilog2(uint64_t v) {
     return b;
 }
 
-__global__ void k_zero_counters(int *c) {
+// Start of a step: the counters (the overflow bits are sticky) and the
+// solver's per-sweep residual maxima.
+__global__ void k_zero_counters(int *c, float *residual, int n_residual) {
     int i = threadIdx.x;
     if (i < RB_CNT_COUNT && i != RB_CNT_OVERFLOW) c[i] = 0;
+    for (int k = i; k < n_residual; k += blockDim.x) residual[k] = 0.f;
 }
 
 // Launch shapes used when mi_rb_tuning leaves a field 0 (the autotuner's
@@ -460,7 +463,8 @@ mi_rb_status enqueue_step(World &w, int last_stage) {
     (void)hipGetLastError();  // start from a clean error state: launches below are checked one by one
     w.events_full = ev && last_stage >= MI_RB_STAGE_INTEGRATE;
     if (ev) RB_CHECK(hipEventRecord(w.ev[0], s));
-    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, d.counters);
+    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(256), 0, s, d.counters, d.residual,
+                       2 * (p.normal_iters + p.friction_iters + 1));
     RB_CHECK(launch_setup(d, p, s));
     if (ev) RB_CHECK(hipEventRecord(w.ev[1], s));
     if (last_stage < MI_RB_STAGE_SORT) return MI_RB_OK;
