@@ -822,19 +822,65 @@ __global__ __launch_bounds__(256) void k_proxy_transforms(int P, int n, const in
 __device__ __forceinline__ int compound_children(const HullInfo &h) {
     return shape_kind_of(h) == SHAPE_KIND_COMPOUND ? h.num_edges : 0;
 }
+// Triangle tri (a child hull of mesh body m at the identity) against body
+// b's world AABB: sat.cl:985-1023 tests the triangle's mesh-frame AABB
+// against b's world AABB; here its vertices are taken to world space first,
+// which is the same test for a mesh body at the identity.
+__device__ __forceinline__ bool mesh_tri_overlaps(const HullInfo &tri, const float4 *__restrict__ verts,
+                                                  float4 pm, float4 qm, float4 bmin, float4 bmax) {
+    float4 lo = mk4(1e30f, 1e30f, 1e30f, 0.f), hi = mk4(-1e30f, -1e30f, -1e30f, 0.f);
+    for (int k = 0; k < 3; k++) {
+        const float4 v = add4(qt_rotate(qm, verts[tri.vertex_offset + k]), pm);
+        lo = mk4(fminf(lo.x, v.x), fminf(lo.y, v.y), fminf(lo.z, v.z), 0.f);
+        hi = mk4(fmaxf(hi.x, v.x), fmaxf(hi.y, v.y), fmaxf(hi.z, v.z), 0.f);
+    }
+    return !(lo.x > bmax.x || hi.x < bmin.x || lo.z > bmax.z || hi.z < bmin.z || lo.y > bmax.y ||
+             hi.y < bmin.y);
+}
+// What the triangle cull reads: transforms, the bodies' world AABBs from
+// the broadphase, hull vertices and the child table.
+struct MeshView {
+    const float4 *pos, *quat, *aabb_min, *aabb_max, *verts;
+    const ChildShape *children;
+};
+// A mesh pairs with convex hull bodies only: 0 when A is the mesh, 1 when B
+// is, 2 for a mesh with anything else (no child pairs), -1 without a mesh.
+__device__ __forceinline__ int mesh_side(const HullInfo &ha, const HullInfo &hb) {
+    const bool ma = compound_is_mesh(ha), mb = compound_is_mesh(hb);
+    if (!ma && !mb) return -1;
+    if (ma && mb) return 2;  // mesh against mesh: nothing
+    const HullInfo &other = ma ? hb : ha;
+    return shape_kind_of(other) == SHAPE_KIND_HULL ? (ma ? 0 : 1) : 2;
+}
 __global__ __launch_bounds__(256) void k_compound_count(int max_pairs, const int *__restrict__ num_pairs_ptr,
                                                         const int2 *__restrict__ pairs,
                                                         const int *__restrict__ shape,
                                                         const float *__restrict__ inv_mass,
-                                                        const HullInfo *__restrict__ hulls,
+                                                        const HullInfo *__restrict__ hulls, MeshView mv,
                                                         uint32_t *__restrict__ cnt) {
     int npairs = *num_pairs_ptr;
     if (npairs > max_pairs) npairs = max_pairs;
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += gridDim.x * blockDim.x) {
         const int2 pr = pairs[p];
-        const int ca = compound_children(hulls[shape[pr.x]]);
-        const int cb = compound_children(hulls[shape[pr.y]]);
+        const HullInfo ha = hulls[shape[pr.x]], hb = hulls[shape[pr.y]];
         const bool both_static = inv_mass[pr.x] == 0.f && inv_mass[pr.y] == 0.f;
+        const int side = mesh_side(ha, hb);
+        if (side >= 0) {
+            uint32_t c = 0;
+            if (side < 2 && !both_static) {
+                const int m = side == 0 ? pr.x : pr.y, b = side == 0 ? pr.y : pr.x;
+                const HullInfo &hm = side == 0 ? ha : hb;
+                const float4 pm = mv.pos[m], qm = mv.quat[m], bmin = mv.aabb_min[b],
+                             bmax = mv.aabb_max[b];
+                for (int t = 0; t < hm.num_edges; t++)
+                    c += mesh_tri_overlaps(hulls[mv.children[hm.edge_offset + t].shape], mv.verts, pm,
+                                           qm, bmin, bmax);
+            }
+            cnt[p] = c;
+            continue;
+        }
+        const int ca = compound_children(ha);
+        const int cb = compound_children(hb);
         cnt[p] = (ca || cb) && !both_static ? (uint32_t)((ca ? ca : 1) * (cb ? cb : 1)) : 0u;
     }
 }
@@ -843,7 +889,7 @@ __global__ __launch_bounds__(256) void k_compound_count(int max_pairs, const int
 __global__ __launch_bounds__(256) void k_compound_write(int max_pairs, const int *__restrict__ num_pairs_ptr,
                                                         int2 *__restrict__ pairs,
                                                         const int *__restrict__ shape,
-                                                        const HullInfo *__restrict__ hulls,
+                                                        const HullInfo *__restrict__ hulls, MeshView mv,
                                                         const int *__restrict__ proxy_first, int n,
                                                         const uint32_t *__restrict__ cnt,
                                                         const uint32_t *__restrict__ off) {
@@ -853,9 +899,26 @@ __global__ __launch_bounds__(256) void k_compound_write(int max_pairs, const int
         const uint32_t c = cnt[p];
         if (c == 0) continue;
         const int2 pr = pairs[p];
-        const int cb = compound_children(hulls[shape[pr.y]]);
-        const int nb = cb ? cb : 1;
+        const HullInfo ha = hulls[shape[pr.x]], hb = hulls[shape[pr.y]];
         const int fa = proxy_first[pr.x], fb = proxy_first[pr.y];
+        const int side = mesh_side(ha, hb);
+        if (side >= 0) {  // overlapping triangles in triangle order (c > 0: side is 0 or 1)
+            const int m = side == 0 ? pr.x : pr.y, b = side == 0 ? pr.y : pr.x;
+            const HullInfo &hm = side == 0 ? ha : hb;
+            const float4 pm = mv.pos[m], qm = mv.quat[m], bmin = mv.aabb_min[b], bmax = mv.aabb_max[b];
+            uint32_t k = 0;
+            for (int t = 0; t < hm.num_edges && k < c; t++) {
+                if (!mesh_tri_overlaps(hulls[mv.children[hm.edge_offset + t].shape], mv.verts, pm, qm,
+                                       bmin, bmax))
+                    continue;
+                const long long dst = (long long)npairs + off[p] + k++;
+                if (dst >= max_pairs) break;
+                pairs[dst] = side == 0 ? make_int2(n + fa + t, pr.y) : make_int2(pr.x, n + fb + t);
+            }
+            continue;
+        }
+        const int cb = compound_children(hb);
+        const int nb = cb ? cb : 1;
         for (uint32_t k = 0; k < c; k++) {
             const long long dst = (long long)npairs + off[p] + k;
             if (dst >= max_pairs) break;
@@ -909,15 +972,16 @@ hipError_t launch_compound_pairs(const DeviceWorld &d, const StepParams &p, hipS
     hipLaunchKernelGGL(k_proxy_transforms, dim3((p.n_proxies + 255) / 256), dim3(256), 0, s,
                        p.n_proxies, p.n_bodies, d.proxy, d.children, d.pos, d.quat);
     const int grid = rb_num_cus() * 4;
+    const MeshView mv{d.pos, d.quat, d.aabb_min, d.aabb_max, d.verts, d.children};
     hipLaunchKernelGGL(k_compound_count, dim3(grid), dim3(256), 0, s, p.max_pairs,
-                       d.counters + RB_CNT_PAIRS, d.pairs, d.shape, d.inv_mass, d.hulls,
+                       d.counters + RB_CNT_PAIRS, d.pairs, d.shape, d.inv_mass, d.hulls, mv,
                        d.narrow_cand);
     hipError_t e = launch_exclusive_scan(d.narrow_cand, d.cand_off, p.max_pairs,
                                          (uint32_t *)(d.counters + RB_CNT_NARROW_CAND), scan_status,
                                          s, d.counters + RB_CNT_PAIRS);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_compound_write, dim3(grid), dim3(256), 0, s, p.max_pairs,
-                       d.counters + RB_CNT_PAIRS, d.pairs, d.shape, d.hulls, d.proxy_first,
+                       d.counters + RB_CNT_PAIRS, d.pairs, d.shape, d.hulls, mv, d.proxy_first,
                        p.n_bodies, d.narrow_cand, d.cand_off);
     hipLaunchKernelGGL(k_compound_total, dim3(1), dim3(1), 0, s, p.max_pairs,
                        d.counters + RB_CNT_PAIRS, (const uint32_t *)(d.counters + RB_CNT_NARROW_CAND),

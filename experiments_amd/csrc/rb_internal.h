@@ -34,6 +34,12 @@ inline __host__ __device__ int shape_kind_of(const HullInfo &h) {
                                                  : (h.num_faces > 0 ? SHAPE_KIND_PLANE
                                                                     : SHAPE_KIND_SPHERE));
 }
+// A compound with a nonzero num_indices is a concave triangle mesh: its
+// children are triangle hulls at the identity, and a pair with a hull gets
+// one child pair per triangle whose world AABB overlaps the hull's.
+inline __host__ __device__ bool compound_is_mesh(const HullInfo &h) {
+    return h.num_vertices == 0 && h.num_edges > 0 && h.num_indices != 0;
+}
 // A child of a compound shape: a hull shape at a position and orientation
 // in the compound's frame (btGpuChildShape, 48 bytes).
 struct ChildShape {

@@ -646,12 +646,10 @@ static mi_rb_status add_vertexless_shape(World *w, float radius, const float *pl
     return MI_RB_OK;
 }
 
-extern "C" mi_rb_status mi_rb_register_compound(void *world, int n, const int *child_shapes,
-                                                const float *child_pos, const float *child_orn,
-                                                int *out_shape) {
-    World *w = W(world);
-    if (!w || n < 1 || n > 64 || !child_shapes || !child_pos || !child_orn || !out_shape)
-        return MI_RB_INVALID_ARG;
+// A compound of n hull children; a mesh compound (num_indices 1) marks its
+// children as triangles to be culled per pair by their world AABBs.
+static mi_rb_status add_compound(World *w, int n, const int *child_shapes, const float *child_pos,
+                                 const float *child_orn, bool mesh, int *out_shape) {
     if ((int)w->hulls.size() >= w->cfg.max_shapes) return MI_RB_CAPACITY;
     float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
     std::vector<ChildShape> kids(n);
@@ -691,6 +689,7 @@ extern "C" mi_rb_status mi_rb_register_compound(void *world, int n, const int *c
     hi_.index_offset = (int)w->indices.size();
     hi_.edge_offset = (int)w->children.size();
     hi_.num_edges = n;
+    hi_.num_indices = mesh ? 1 : 0;
     hi_.aabb_min = make_float4(lo[0], lo[1], lo[2], 0.f);
     hi_.aabb_max = make_float4(hi[0], hi[1], hi[2], 0.f);
     for (const ChildShape &k : kids) w->children.push_back(k);
@@ -700,6 +699,126 @@ extern "C" mi_rb_status mi_rb_register_compound(void *world, int n, const int *c
     *out_shape = (int)w->hulls.size() - 1;
     w->uploaded = false;
     return MI_RB_OK;
+}
+
+extern "C" mi_rb_status mi_rb_register_compound(void *world, int n, const int *child_shapes,
+                                                const float *child_pos, const float *child_orn,
+                                                int *out_shape) {
+    World *w = W(world);
+    if (!w || n < 1 || n > 64 || !child_shapes || !child_pos || !child_orn || !out_shape)
+        return MI_RB_INVALID_ARG;
+    return add_compound(w, n, child_shapes, child_pos, child_orn, false, out_shape);
+}
+
+// One triangle of a concave mesh as a hull of 3 vertices, 5 faces and 3
+// unique edges, the polyhedron sat.cl:1040-1116 builds per triangle: the
+// front face (the triangle's plane, n = (v1-v0) x (v2-v0) normalised,
+// c = -n.v0), the back face (-n, n.v0), and per edge (v_i, v_prev) a face of
+// 2 indices whose plane holds the edge and is perpendicular to the triangle
+// (normal (n x (v_prev - v_i)) normalised, c = -normal.v_i); unique edges
+// v1-v0, v2-v1, v0-v2.  Returns false for a triangle without area.
+static bool add_triangle_hull(World *w, const float v[3][3]) {
+    auto sub = [](const float *a, const float *b, float *r) {
+        r[0] = a[0] - b[0], r[1] = a[1] - b[1], r[2] = a[2] - b[2];
+    };
+    auto cross = [](const float *a, const float *b, float *r) {
+        r[0] = a[1] * b[2] - a[2] * b[1], r[1] = a[2] * b[0] - a[0] * b[2], r[2] = a[0] * b[1] - a[1] * b[0];
+    };
+    auto dot = [](const float *a, const float *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+    auto normalize = [&](float *a) {
+        const float l = std::sqrt(dot(a, a));
+        if (!(l > 0.f) || !std::isfinite(l)) return false;
+        a[0] /= l, a[1] /= l, a[2] /= l;
+        return true;
+    };
+    float e01[3], e02[3], nrm[3];
+    sub(v[1], v[0], e01);
+    sub(v[2], v[0], e02);
+    cross(e01, e02, nrm);
+    if (!normalize(nrm)) return false;
+    float side[3][4];
+    for (int i = 0, prev = 2; i < 3; prev = i++) {
+        float d[3];
+        sub(v[prev], v[i], d);
+        cross(nrm, d, side[i]);
+        if (!normalize(side[i])) return false;
+        side[i][3] = -dot(side[i], v[i]);
+    }
+    HullInfo hi{};
+    hi.face_offset = (int)w->planes.size();
+    hi.num_faces = 5;
+    hi.vertex_offset = (int)w->verts.size();
+    hi.num_vertices = 3;
+    hi.edge_offset = (int)w->edges.size();
+    hi.num_edges = 3;
+    hi.index_offset = (int)w->indices.size();
+    hi.num_indices = 3 + 3 + 2 * 3;
+    float lo[3], hi3[3];
+    for (int k = 0; k < 3; k++) {
+        lo[k] = std::min(v[0][k], std::min(v[1][k], v[2][k]));
+        hi3[k] = std::max(v[0][k], std::max(v[1][k], v[2][k]));
+    }
+    // zero thickness: no inner ball (w = 0)
+    hi.center = make_float4((v[0][0] + v[1][0] + v[2][0]) * (1.f / 3.f),
+                            (v[0][1] + v[1][1] + v[2][1]) * (1.f / 3.f),
+                            (v[0][2] + v[1][2] + v[2][2]) * (1.f / 3.f), 0.f);
+    hi.aabb_min = make_float4(lo[0], lo[1], lo[2], 0.f);
+    hi.aabb_max = make_float4(hi3[0], hi3[1], hi3[2], 0.f);
+    for (int k = 0; k < 3; k++) w->verts.push_back(make_float4(v[k][0], v[k][1], v[k][2], 0.f));
+    const int io = hi.index_offset;
+    w->planes.push_back(make_float4(nrm[0], nrm[1], nrm[2], -dot(nrm, v[0])));
+    w->face_index.push_back(make_int2(io, 3));
+    w->planes.push_back(make_float4(-nrm[0], -nrm[1], -nrm[2], dot(nrm, v[0])));
+    w->face_index.push_back(make_int2(io + 3, 3));
+    for (int x : {0, 1, 2, 2, 1, 0}) w->indices.push_back(x);
+    for (int i = 0, prev = 2; i < 3; prev = i++) {
+        w->planes.push_back(make_float4(side[i][0], side[i][1], side[i][2], side[i][3]));
+        w->face_index.push_back(make_int2(io + 6 + 2 * i, 2));
+        w->indices.push_back(i);
+        w->indices.push_back(prev);
+    }
+    for (int i = 0; i < 3; i++) {
+        float e[3];
+        sub(v[(i + 1) % 3], v[i], e);
+        w->edges.push_back(make_float4(e[0], e[1], e[2], 0.f));
+    }
+    const float hx = (hi3[0] - lo[0]) * 0.5f, hy = (hi3[1] - lo[1]) * 0.5f, hz = (hi3[2] - lo[2]) * 0.5f;
+    w->hull_hext_norm.push_back(std::sqrt(hx * hx + hy * hy + hz * hz));
+    w->hulls.push_back(hi);
+    return true;
+}
+
+extern "C" mi_rb_status mi_rb_register_concave_mesh(void *world, const float *xyz, int stride_bytes,
+                                                    int num_vertices, const int *indices,
+                                                    int num_triangles, const float scale[3],
+                                                    int *out_shape) {
+    World *w = W(world);
+    if (!w || !xyz || !indices || !out_shape || num_vertices < 3 || num_triangles < 1 ||
+        num_triangles > MI_RB_MAX_MESH_TRIANGLES || stride_bytes < 3 * (int)sizeof(float))
+        return MI_RB_INVALID_ARG;
+    for (int i = 0; i < 3 * num_triangles; i++)
+        if (indices[i] < 0 || indices[i] >= num_vertices) return MI_RB_INVALID_ARG;
+    // room for every triangle and the mesh shape, checked before anything is added
+    if ((long long)w->hulls.size() + num_triangles + 1 > w->cfg.max_shapes ||
+        (long long)w->verts.size() + 3LL * num_triangles > w->cfg.max_hull_vertices)
+        return MI_RB_CAPACITY;
+    const float sx = scale ? scale[0] : 1.f, sy = scale ? scale[1] : 1.f, sz = scale ? scale[2] : 1.f;
+    auto vert = [&](int i, float *r) {
+        const float *p = (const float *)((const char *)xyz + (size_t)i * stride_bytes);
+        r[0] = p[0] * sx, r[1] = p[1] * sy, r[2] = p[2] * sz;
+    };
+    std::vector<int> tri_shapes;
+    for (int t = 0; t < num_triangles; t++) {
+        float v[3][3];
+        for (int k = 0; k < 3; k++) vert(indices[3 * t + k], v[k]);
+        // a triangle without area has no plane and is left out
+        if (add_triangle_hull(w, v)) tri_shapes.push_back((int)w->hulls.size() - 1);
+    }
+    if (tri_shapes.empty()) return MI_RB_INVALID_ARG;
+    const int m = (int)tri_shapes.size();
+    std::vector<float> zero(3 * (size_t)m, 0.f), ident(4 * (size_t)m, 0.f);
+    for (int i = 0; i < m; i++) ident[4 * i + 3] = 1.f;
+    return add_compound(w, m, tri_shapes.data(), zero.data(), ident.data(), true, out_shape);
 }
 
 extern "C" mi_rb_status mi_rb_register_sphere(void *world, float radius, int *out_shape) {
@@ -732,6 +851,8 @@ extern "C" mi_rb_status mi_rb_register_body(void *world, float mass, const float
     // a plane is infinite: it can only be static
     if (mass != 0.f && w->hulls[shape].num_vertices == 0 && w->hulls[shape].num_faces > 0)
         return MI_RB_INVALID_ARG;
+    // so is a concave mesh (the reference gives meshes no inertia)
+    if (mass != 0.f && compound_is_mesh(w->hulls[shape])) return MI_RB_INVALID_ARG;
     if (!w->scene.empty() && scene < w->scene.back()) return MI_RB_INVALID_ARG;
     if ((int)w->pos.size() >= w->cfg.max_bodies) return MI_RB_CAPACITY;
     w->pos.push_back(make_float4(pos[0], pos[1], pos[2], 0.f));

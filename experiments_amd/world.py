@@ -100,6 +100,20 @@ class World:
         self.num_shapes += 1
         return out.value
 
+    def add_concave_mesh(self, vertices, indices, scaling=(1.0, 1.0, 1.0)):
+        """A triangle mesh for static bodies: vertices (V, 3), triangles as
+        index triples; each triangle takes a shape slot, the mesh one more."""
+        v = _f32(vertices).reshape(-1, 3).copy()
+        ix = np.ascontiguousarray(indices, np.int32).reshape(-1)
+        if len(ix) % 3:
+            raise ValueError("three indices per triangle")
+        sc = _f32(scaling)[:3].copy()
+        out = ctypes.c_int()
+        L.call("mi_rb_register_concave_mesh", self.h, _ptr(v), 12, len(v), _ptr(ix), len(ix) // 3,
+               _ptr(sc), ctypes.byref(out))
+        self.num_shapes = out.value + 1
+        return out.value
+
     def add_body(self, mass, pos, quat_xyzw, shape, scene=0):
         p = _f32(pos)
         q = _f32(quat_xyzw)
@@ -280,6 +294,9 @@ class CLPhysicsDemo:
 
     def registerPlaneShape(self, planeNormal, planeConstant):
         return self.world.add_plane(planeNormal, planeConstant)
+
+    def registerConcaveMesh(self, vertices, indices, scaling=(1.0, 1.0, 1.0)):
+        return self.world.add_concave_mesh(vertices, indices, scaling)
 
     def registerCompoundShape(self, childShapes):
         """childShapes: records with m_childPosition, m_childOrientation and

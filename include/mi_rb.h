@@ -25,7 +25,7 @@ typedef enum mi_rb_status {
     MI_RB_INVALID_ARG = 1,       /* bad pointer, index or size */
     MI_RB_CAPACITY = 2,          /* a configured capacity would be exceeded */
     MI_RB_DEVICE = 3,            /* HIP runtime error or no usable gfx950 device */
-    MI_RB_UNSUPPORTED_SHAPE = 4, /* non-hull shapes (sphere, plane, compound, mesh) */
+    MI_RB_UNSUPPORTED_SHAPE = 4, /* shape kinds this library does not build */
     MI_RB_STATE = 5,             /* call out of order (e.g. step before upload) */
     MI_RB_FORMAT = 6             /* a file that is malformed or of an unsupported kind */
 } mi_rb_status;
@@ -185,7 +185,23 @@ mi_rb_status mi_rb_register_plane(void *world, const float normal[3], float cons
 mi_rb_status mi_rb_register_compound(void *world, int n, const int *child_shapes,
                                      const float *child_pos, const float *child_orn,
                                      int *out_shape);
-/* registerConcaveMesh (CLPhysicsDemo.h:58-66): not on this path. */
+/* CLPhysicsDemo::registerConcaveMesh(vertices, indices, scaling)
+ * (CLPhysicsDemo.cpp:287-300, btGpuNarrowPhaseAndSolver.cpp:264-333): a
+ * triangle mesh for static bodies only (a body with mass on it gives
+ * MI_RB_INVALID_ARG).  num_vertices points (scaled) at stride_bytes apart,
+ * triangle t = indices[3t..3t+2].  Each triangle becomes the polyhedron
+ * sat.cl:1040-1116 builds (3 vertices, front and back faces, 3 edge faces);
+ * per pair with a convex hull body, each triangle whose world AABB overlaps
+ * the hull's gives a triangle-hull pair, found and clipped like a hull pair
+ * (sat.cl:970-1191, satClipHullContacts.cl:1387); contacts name the mesh
+ * body.  Triangles without area are left out; a mesh against a sphere, a
+ * plane, a compound or a mesh gets no contacts.  Uses one shape slot and 3
+ * hull vertices per triangle plus one slot for the mesh. */
+#define MI_RB_MAX_MESH_TRIANGLES 65536
+mi_rb_status mi_rb_register_concave_mesh(void *world, const float *xyz, int stride_bytes,
+                                         int num_vertices, const int *indices, int num_triangles,
+                                         const float scale[3], int *out_shape);
+/* Shape kinds this library does not build: returns MI_RB_UNSUPPORTED_SHAPE. */
 mi_rb_status mi_rb_register_unsupported_shape(void *world, int kind);
 
 /* CLPhysicsDemo::registerPhysicsInstance(mass, pos, orn xyzw, collidable, user)

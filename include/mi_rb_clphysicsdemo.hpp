@@ -7,8 +7,8 @@
  * Differences a caller can see:
  *   - getBodiesGpu() returns a non-owning mi_rb_device_view pointer instead of
  *     a cl_mem (the bodies are structure-of-arrays in device memory);
- *   - non-hull shapes (sphere, plane, concave mesh, compound) return -1, as
- *     the step handles convex hulls only;
+ *   - a concave mesh takes a shape slot per triangle (plus one for the
+ *     mesh) and collides with convex hull bodies only;
  *   - errors are reported by last_status() (mi_rb_status of the last call)
  *     instead of asserts and printf.
  */
@@ -67,14 +67,6 @@ struct CLPhysicsDemo {
         m_status = mi_rb_register_plane(m_world, planeNormal, planeConstant, &shape);
         return m_status == MI_RB_OK ? shape : -1;
     }
-    template <class Vertices, class Indices>
-    int registerConcaveMesh(Vertices *vertices, Indices *indices, const float *scaling) {
-        (void)vertices;
-        (void)indices;
-        (void)scaling;
-        m_status = mi_rb_register_unsupported_shape(m_world, 3);
-        return -1;
-    }
     /* childShapes: a container (size(), operator[]) of btGpuChildShape-like
      * records: float m_childPosition[4], m_childOrientation[4], int m_shapeIndex */
     template <class Children>
@@ -90,6 +82,23 @@ struct CLPhysicsDemo {
         }
         int shape = -1;
         m_status = mi_rb_register_compound(m_world, n, shapes.data(), pos.data(), orn.data(), &shape);
+        return m_status == MI_RB_OK ? shape : -1;
+    }
+
+    /* registerConcaveMesh(vertices, indices, scaling) (CLPhysicsDemo.cpp:287-300):
+     * Vertices: an array of points with operator[] giving x, y, z (e.g.
+     * btAlignedObjectArray<btVector3>); Indices: ints, three per triangle */
+    template <class Vertices, class Indices>
+    int registerConcaveMesh(Vertices *vertices, Indices *indices, const float *scaling) {
+        const int nv = (int)vertices->size(), ni = (int)indices->size();
+        std::vector<float> xyz(3 * (size_t)nv);
+        std::vector<int> ix(ni);
+        for (int i = 0; i < nv; i++)
+            for (int k = 0; k < 3; k++) xyz[3 * i + k] = (*vertices)[i][k];
+        for (int i = 0; i < ni; i++) ix[i] = (*indices)[i];
+        int shape = -1;
+        m_status = mi_rb_register_concave_mesh(m_world, xyz.data(), 3 * (int)sizeof(float), nv,
+                                               ix.data(), ni / 3, scaling, &shape);
         return m_status == MI_RB_OK ? shape : -1;
     }
 

@@ -62,15 +62,43 @@ def oracle_setup(s):
     return lv, av, invI
 
 
+def is_mesh(h):
+    """A compound whose num_indices is nonzero is a concave triangle mesh."""
+    return h["num_vertices"] == 0 and h["num_edges"] > 0 and h["num_indices"] != 0
+
+
+def mesh_triangles_overlapping(s, m, b, mn, mx):
+    """Triangles (child indices) of mesh body m whose world AABB overlaps
+    body b's (sat.cl:985-1023, the triangle taken to world space by m's
+    transform), in triangle order."""
+    h = s.sh["hulls"][s.sh["shape"][m]]
+    T, e0 = int(h["num_edges"]), int(h["edge_offset"])
+    tri = s.shapes.hulls[s.children["shape"][e0:e0 + T]]
+    vo = tri["vertex_offset"].astype(np.int64)
+    local = s.shapes.verts[(vo[:, None] + np.arange(3)).reshape(-1)]
+    pseudo = np.zeros(3 * T, O.CHILD_DTYPE)      # the vertices as children at the identity
+    pseudo["pos"][:, :3] = local[:, :3]
+    pseudo["orn"][:, 3] = 1.0
+    proxy = np.stack([np.zeros(3 * T, np.int32), np.arange(3 * T, dtype=np.int32)], 1)
+    wp, _ = O.compound_proxies(proxy, pseudo, s.state["pos"][m:m + 1], s.state["quat"][m:m + 1])
+    wv = wp[1:, :3].reshape(T, 3, 3)
+    lo, hi = wv.min(axis=1), wv.max(axis=1)
+    bmn, bmx = mn[b, :3], mx[b, :3]
+    sep = (lo > bmx) | (hi < bmn)
+    return np.nonzero(~sep.any(axis=1))[0]
+
+
 def expand_compound_pairs(s, pairs):
     """The pair list the narrowphase sees: the broadphase pairs, then for
     each pair with a compound (not both static) one pair per (child of A or
     A, child of B or B), child of A outer (findCompoundPairsKernel,
-    sat.cl:763-925), in pair order."""
+    sat.cl:763-925), in pair order.  A concave mesh pairs with hull bodies
+    only, one pair per triangle whose world AABB overlaps the hull's."""
     if len(s.proxy) == 0:
         return pairs
     hulls, shape, im = s.sh["hulls"], s.sh["shape"], s.sh["inv_mass"]
     n = len(shape)
+    mn, mx = O.world_aabbs(s.state["pos"], s.state["quat"], shape, hulls)
 
     def kids(b):
         h = hulls[shape[b]]
@@ -78,6 +106,17 @@ def expand_compound_pairs(s, pairs):
 
     extra = []
     for a, b in np.asarray(pairs).reshape(-1, 2):
+        ha, hb = hulls[shape[a]], hulls[shape[b]]
+        if is_mesh(ha) or is_mesh(hb):
+            other = hb if is_mesh(ha) else ha
+            if (is_mesh(ha) and is_mesh(hb)) or other["num_vertices"] == 0 or \
+                    (im[a] == 0 and im[b] == 0):
+                continue
+            m, o = (a, b) if is_mesh(ha) else (b, a)
+            for t in mesh_triangles_overlapping(s, m, o, mn, mx):
+                p = n + s.proxy_first[m] + int(t)
+                extra.append((p, b) if m == a else (a, p))
+            continue
         ca, cb = kids(a), kids(b)
         if not (ca or cb) or (im[a] == 0 and im[b] == 0):
             continue

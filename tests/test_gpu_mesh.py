@@ -1,0 +1,154 @@
+"""Concave triangle meshes on the GPU path (registerConcaveMesh,
+CLPhysicsDemo.cpp:287-300): the triangle hulls the library builds hold the
+polyhedron of sat.cl:1040-1116, the expanded pair list (broadphase pairs,
+then one pair per triangle whose world AABB overlaps a hull body's) is
+bit-exact, contacts match the oracle per pair, one step matches within
+1e-4, and cubes dropped on a bumpy terrain settle on it."""
+import numpy as np
+import pytest
+
+import oracle as O
+from experiments_amd import _lib as L
+from experiments_amd import scenes
+from experiments_amd.world import World, reference_config
+from oracle.step import (Snapshot, expand_compound_pairs, gpu_order, oracle_broadphase,
+                         oracle_contacts, oracle_step)
+
+pytestmark = pytest.mark.gpu
+
+QUADS, HALF = 10, 10.0
+
+
+def height(x, z):
+    return 0.15 * np.sin(0.6 * x) * np.cos(0.5 * z)
+
+
+def terrain():
+    g = np.linspace(-HALF, HALF, QUADS + 1)
+    verts = np.array([(x, height(x, z), z) for x in g for z in g], np.float32)
+    tris = []
+    for i in range(QUADS):
+        for j in range(QUADS):
+            a, b = i * (QUADS + 1) + j, i * (QUADS + 1) + j + 1
+            c, d = (i + 1) * (QUADS + 1) + j + 1, (i + 1) * (QUADS + 1) + j
+            tris += [(a, b, c), (a, c, d)]   # counter-clockwise seen from +y
+    return verts, np.array(tris, np.int32)
+
+
+def quat_axis(axis, ang):
+    a = np.asarray(axis, np.float64) / np.linalg.norm(axis)
+    return np.array([*(a * np.sin(ang / 2)), np.cos(ang / 2)], np.float32)
+
+
+def mesh_world(seed=3, layers=2):
+    rng = np.random.default_rng(seed)
+    w = World(reference_config(max_bodies=256, max_shapes=512, max_pairs=1 << 15,
+                               max_contacts=1 << 15, max_pairs_per_body=64, solver_iterations=8))
+    cube = w.add_hull(scenes.cube_points(0.5))
+    verts, tris = terrain()
+    mesh = w.add_concave_mesh(verts, tris)
+    w.add_body(0.0, (0, 0, 0), (0, 0, 0, 1), mesh)
+    for ly in range(layers):
+        for i in range(4):
+            for j in range(4):
+                pos = (-4.5 + 3.0 * i, 1.2 + 1.6 * ly, -4.5 + 3.0 * j)
+                w.add_body(1.0, pos, quat_axis(rng.normal(size=3), rng.uniform(0, 0.6)), cube)
+    w.upload()
+    return w, mesh, verts, tris
+
+
+def test_triangle_hulls_hold_the_reference_polyhedron():
+    w, mesh, verts, tris = mesh_world(layers=1)
+    H = w.read(L.BUF_HULLS, O.HULL_DTYPE)
+    V = w.read(L.BUF_VERTS, np.float32, (4,))
+    P = w.read(L.BUF_PLANES, np.float32, (4,))
+    FI = w.read(L.BUF_FACE_INDEX, np.int32, (2,))
+    IX = w.read(L.BUF_INDICES, np.int32)
+    CH = w.read(L.BUF_CHILDREN, O.CHILD_DTYPE)
+    m = H[mesh]
+    assert m["num_vertices"] == 0 and m["num_indices"] == 1 and m["num_edges"] == len(tris)
+    kids = CH[m["edge_offset"]:m["edge_offset"] + m["num_edges"]]
+    assert np.all(kids["pos"] == 0) and np.all(kids["orn"] == [0, 0, 0, 1])
+    for t, k in enumerate(kids):
+        h = H[k["shape"]]
+        assert (h["num_vertices"], h["num_faces"], h["num_edges"]) == (3, 5, 3)
+        v = V[h["vertex_offset"]:h["vertex_offset"] + 3, :3].astype(np.float64)
+        assert np.array_equal(v, verts[tris[t]])
+        pl = P[h["face_offset"]:h["face_offset"] + 5].astype(np.float64)
+        assert pl[0, 1] > 0.9 and np.array_equal(pl[1], -pl[0])
+        assert np.allclose(v @ pl[0, :3] + pl[0, 3], 0, atol=1e-5)
+        cen = v.mean(0)
+        for f in range(2, 5):
+            a, cnt = FI[h["face_offset"] + f]
+            assert cnt == 2
+            ends = v[IX[a:a + 2]]
+            assert np.allclose(ends @ pl[f, :3] + pl[f, 3], 0, atol=1e-5)
+            assert abs(pl[f, :3] @ pl[0, :3]) < 1e-5 and cen @ pl[f, :3] + pl[f, 3] < 0
+    with pytest.raises(L.MiRbError):
+        w.add_body(1.0, (0, 5, 0), (0, 0, 0, 1), mesh)   # a mesh is for static bodies
+    w.close()
+
+
+@pytest.fixture(scope="module")
+def settled():
+    w = mesh_world()[0]
+    w.step(40)
+    yield w
+    w.close()
+
+
+def test_pairs_contacts_match_oracle(settled):
+    w = settled
+    s = Snapshot(w)
+    w.step_partial(L.STAGE_CONSTRAINTS)
+    _, _, _, pairs = oracle_broadphase(s)
+    expanded = expand_compound_pairs(s, pairs)
+    assert len(expanded) > len(pairs)
+    assert np.array_equal(w.pairs(), expanded)
+    ref = oracle_contacts(s, pairs)
+    got = w.contacts(O.CONTACT_DTYPE)
+    assert len(got) == len(ref) >= 16
+    for k in ("pair_index", "body_a", "body_b"):
+        assert np.array_equal(got[k], ref[k]), k
+    assert np.array_equal(got["normal"][:, 3], ref["normal"][:, 3])
+    n = ref["normal"][:, 3].astype(int)
+    for k in range(len(ref)):
+        assert np.allclose(got["pos"][k, :n[k]], ref["pos"][k, :n[k]], rtol=1e-5, atol=1e-6), k
+    assert np.allclose(got["normal"][:, :3], ref["normal"][:, :3], rtol=1e-5, atol=1e-6)
+    # contacts from triangle pairs name the mesh body (0)
+    tri = got["pair_index"] >= len(pairs)
+    assert tri.any() and np.all((got["body_a"][tri] == 0) | (got["body_b"][tri] == 0))
+
+
+def test_one_step_matches_oracle(settled):
+    w = settled
+    s = Snapshot(w)
+    w.step_partial(L.STAGE_COLOR)
+    order = gpu_order(w)
+    contacts = w.contacts(O.CONTACT_DTYPE)
+    w.set_state(**{k: s.state[k] for k in ("pos", "quat", "linvel", "angvel")})
+    w.step(1)
+    g = w.get_state()
+    ref = oracle_step(s, order, contacts)
+    for k in ("linvel", "angvel", "pos", "quat"):
+        n = 4 if k == "quat" else 3
+        assert np.allclose(g[k][:, :n], ref[k][:, :n], rtol=1e-4, atol=1e-4), k
+
+
+def test_cubes_settle_on_the_terrain():
+    w = mesh_world(seed=8)[0]
+    w.use_graph(True)
+    ke = []
+    for _ in range(8):
+        w.step(30)
+        v = w.get_state()["linvel"][1:, :3]
+        ke.append(0.5 * float((v ** 2).sum()))
+    st = w.get_state()
+    assert w.stats()["overflow_flags"] == 0
+    assert np.all(np.isfinite(st["pos"])) and np.all(np.isfinite(st["quat"]))
+    p = st["pos"][1:, :3]
+    # every cube rests on the surface: centre above the terrain by about half a side
+    above = p[:, 1] - height(p[:, 0], p[:, 2])
+    assert above.min() > 0.3, above
+    assert ke[-1] < 0.05 * max(ke), ke
+    w.close()
