@@ -124,6 +124,8 @@ SIGNATURES = {
     "mi_rb_register_plane": (I, [VP, VP, F, PI]),
     "mi_rb_register_compound": (I, [VP, I, VP, VP, VP, PI]),
     "mi_rb_register_concave_mesh": (I, [VP, VP, I, I, VP, I, VP, PI]),
+    "mi_rb_obj_read": (I, [ctypes.c_char_p, VP, I, VP, I, PI, PI]),
+    "mi_rb_register_concave_mesh_obj": (I, [VP, ctypes.c_char_p, VP, PI]),
     "mi_rb_register_body": (I, [VP, F, VP, VP, I, I, PI]),
     "mi_rb_upload": (I, [VP]),
     "mi_rb_set_transform": (I, [VP, I, VP, VP]),

@@ -24,7 +24,7 @@ HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-ffp-contr
              "-Wno-unused-result", "-I" + os.path.join(ROOT, "include")]
 HIP_SOURCES = ["k_sort.hip", "k_broad.hip", "k_narrow.hip", "k_solver.hip", "k_mlp.hip",
                "world.cpp"]
-HOST_SOURCES = ["hull.cpp", "bullet_file.cpp"]
+HOST_SOURCES = ["hull.cpp", "bullet_file.cpp", "obj_file.cpp"]
 
 
 def _run(cmd):

@@ -114,6 +114,15 @@ class World:
         self.num_shapes = out.value + 1
         return out.value
 
+    def add_concave_mesh_obj(self, path, scaling=(1.0, 1.0, 1.0)):
+        """A triangle mesh read from a Wavefront .obj file."""
+        sc = _f32(scaling)[:3].copy()
+        out = ctypes.c_int()
+        L.call("mi_rb_register_concave_mesh_obj", self.h, str(path).encode(), _ptr(sc),
+               ctypes.byref(out))
+        self.num_shapes = out.value + 1
+        return out.value
+
     def add_body(self, mass, pos, quat_xyzw, shape, scene=0):
         p = _f32(pos)
         q = _f32(quat_xyzw)

@@ -201,6 +201,22 @@ mi_rb_status mi_rb_register_compound(void *world, int n, const int *child_shapes
 mi_rb_status mi_rb_register_concave_mesh(void *world, const float *xyz, int stride_bytes,
                                          int num_vertices, const int *indices, int num_triangles,
                                          const float scale[3], int *out_shape);
+/* Wavefront .obj meshes (the reference's objLoader, rendering/
+ * WavefrontObjLoader/obj_parser.cpp:19-105, as registerConcaveMeshShape(
+ * objLoader*, ...) uses it, btGpuNarrowPhaseAndSolver.cpp:335-400): "v x y z"
+ * lines and "f" lines of "v", "v/t", "v//n" or "v/t/n" tokens (1-based,
+ * negative = counted back); each face gives the triangle of its first three
+ * vertices, as in the reference.  mi_rb_obj_read returns the counts, and the
+ * vertices (xyz) and triangles (3 indices each, 0-based) when both arrays
+ * are given and large enough (MI_RB_CAPACITY otherwise); a malformed or
+ * missing file gives MI_RB_INVALID_ARG.  Host only. */
+mi_rb_status mi_rb_obj_read(const char *path, float *xyz, int max_vertices, int *indices,
+                            int max_triangles, int *num_vertices, int *num_triangles);
+/* CLPhysicsDemo::registerConcaveMesh of an .obj file (the demo's ground mesh,
+ * BulletDataExtractor.cpp:262-310): mi_rb_register_concave_mesh of what
+ * mi_rb_obj_read gives. */
+mi_rb_status mi_rb_register_concave_mesh_obj(void *world, const char *path, const float scale[3],
+                                             int *out_shape);
 /* Shape kinds this library does not build: returns MI_RB_UNSUPPORTED_SHAPE. */
 mi_rb_status mi_rb_register_unsupported_shape(void *world, int kind);
 

@@ -152,3 +152,34 @@ def test_cubes_settle_on_the_terrain():
     assert above.min() > 0.3, above
     assert ke[-1] < 0.05 * max(ke), ke
     w.close()
+
+
+def test_obj_ground_mesh():
+    """The demo's ground mesh read from its .obj file (plane.obj, 800
+    triangles): the pair list is bit-exact with the oracle's expansion and
+    dropped cubes come to rest on it (y = 0)."""
+    from pathlib import Path
+    w = World(reference_config(max_bodies=64, max_shapes=1024, max_pairs=1 << 14,
+                               max_contacts=1 << 14, max_pairs_per_body=64, solver_iterations=8))
+    cube = w.add_hull(scenes.cube_points(0.5))
+    mesh = w.add_concave_mesh_obj(Path(__file__).parent / "golden" / "plane.obj")
+    assert mesh == 801
+    w.add_body(0.0, (0, 0, 0), (0, 0, 0, 1), mesh)
+    rng = np.random.default_rng(5)
+    for i in range(4):
+        for j in range(4):
+            w.add_body(1.0, (-4.5 + 3.0 * i, 1.0, -4.5 + 3.0 * j),
+                       quat_axis(rng.normal(size=3), rng.uniform(0, 0.5)), cube)
+    w.upload()
+    w.step(20)
+    s = Snapshot(w)
+    w.step_partial(L.STAGE_CONSTRAINTS)
+    _, _, _, pairs = oracle_broadphase(s)
+    assert np.array_equal(w.pairs(), expand_compound_pairs(s, pairs))
+    w.use_graph(True)
+    w.step(200)
+    st = w.get_state()
+    assert w.stats()["overflow_flags"] == 0
+    assert np.allclose(st["pos"][1:, 1], 0.5, atol=0.05), st["pos"][1:, 1]
+    assert np.abs(st["linvel"][1:, :3]).max() < 0.05
+    w.close()
