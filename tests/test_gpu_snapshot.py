@@ -2,8 +2,8 @@
 a world saved mid-run and loaded into a fresh world continues bit for bit
 like the original uploaded again from the same state (upload derives the
 grid and the solver tiles from the bodies' positions, so the comparison is
-with the original re-uploaded at the save), for hulls, spheres, planes and
-compounds."""
+with the original re-uploaded at the save), for hulls, spheres, planes,
+compounds and concave meshes."""
 import numpy as np
 import pytest
 
@@ -36,6 +36,11 @@ def mixed_world():
         w.add_body(1.0, pos, q / np.linalg.norm(q), [cube, dumbbell][k % 2])
     for k in range(4):
         w.add_body(1.0, (12.0 + k, 2.0, 0.0), ident, ball)
+    # a two-triangle mesh away from the rest, a cube dropping on it
+    quad = [(-2, 0, -2), (-2, 0, 2), (2, 0, 2), (2, 0, -2)]
+    mesh = w.add_concave_mesh(quad, [(0, 1, 2), (0, 2, 3)])
+    w.add_body(0.0, (30.0, 0.0, 0.0), ident, mesh)
+    w.add_body(1.0, (30.0, 0.8, 0.0), q / np.linalg.norm(q), cube)
     w.upload()
     return w
 
