@@ -852,6 +852,12 @@ __device__ __forceinline__ int mesh_side(const HullInfo &ha, const HullInfo &hb)
     const HullInfo &other = ma ? hb : ha;
     return shape_kind_of(other) == SHAPE_KIND_HULL ? (ma ? 0 : 1) : 2;
 }
+// One wave per pair (the pair's type is uniform across the wave): a mesh
+// pair tests 64 triangles at a time, counting with a ballot, and writes the
+// overlapping ones in triangle order through the ballot's prefix counts.
+__device__ __forceinline__ unsigned long long lanes_below(int lane) {
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
 __global__ __launch_bounds__(256) void k_compound_count(int max_pairs, const int *__restrict__ num_pairs_ptr,
                                                         const int2 *__restrict__ pairs,
                                                         const int *__restrict__ shape,
@@ -860,28 +866,34 @@ __global__ __launch_bounds__(256) void k_compound_count(int max_pairs, const int
                                                         uint32_t *__restrict__ cnt) {
     int npairs = *num_pairs_ptr;
     if (npairs > max_pairs) npairs = max_pairs;
-    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += gridDim.x * blockDim.x) {
+    const int lane = threadIdx.x & 63;
+    const int waves = gridDim.x * (blockDim.x >> 6);
+    for (int p = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); p < npairs; p += waves) {
         const int2 pr = pairs[p];
         const HullInfo ha = hulls[shape[pr.x]], hb = hulls[shape[pr.y]];
         const bool both_static = inv_mass[pr.x] == 0.f && inv_mass[pr.y] == 0.f;
         const int side = mesh_side(ha, hb);
+        uint32_t c = 0;
         if (side >= 0) {
-            uint32_t c = 0;
             if (side < 2 && !both_static) {
                 const int m = side == 0 ? pr.x : pr.y, b = side == 0 ? pr.y : pr.x;
                 const HullInfo &hm = side == 0 ? ha : hb;
                 const float4 pm = mv.pos[m], qm = mv.quat[m], bmin = mv.aabb_min[b],
                              bmax = mv.aabb_max[b];
-                for (int t = 0; t < hm.num_edges; t++)
-                    c += mesh_tri_overlaps(hulls[mv.children[hm.edge_offset + t].shape], mv.verts, pm,
-                                           qm, bmin, bmax);
+                for (int t0 = 0; t0 < hm.num_edges; t0 += 64) {
+                    const int t = t0 + lane;
+                    const bool ov = t < hm.num_edges &&
+                                    mesh_tri_overlaps(hulls[mv.children[hm.edge_offset + t].shape],
+                                                      mv.verts, pm, qm, bmin, bmax);
+                    c += (uint32_t)__popcll(__ballot(ov));
+                }
             }
-            cnt[p] = c;
-            continue;
+        } else {
+            const int ca = compound_children(ha);
+            const int cb = compound_children(hb);
+            c = (ca || cb) && !both_static ? (uint32_t)((ca ? ca : 1) * (cb ? cb : 1)) : 0u;
         }
-        const int ca = compound_children(ha);
-        const int cb = compound_children(hb);
-        cnt[p] = (ca || cb) && !both_static ? (uint32_t)((ca ? ca : 1) * (cb ? cb : 1)) : 0u;
+        if (lane == 0) cnt[p] = c;
     }
 }
 // Child pairs in pair order, child of A outer, child of B inner, after the
@@ -895,32 +907,38 @@ __global__ __launch_bounds__(256) void k_compound_write(int max_pairs, const int
                                                         const uint32_t *__restrict__ off) {
     int npairs = *num_pairs_ptr;
     if (npairs > max_pairs) npairs = max_pairs;
-    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += gridDim.x * blockDim.x) {
+    const int lane = threadIdx.x & 63;
+    const int waves = gridDim.x * (blockDim.x >> 6);
+    for (int p = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); p < npairs; p += waves) {
         const uint32_t c = cnt[p];
         if (c == 0) continue;
         const int2 pr = pairs[p];
         const HullInfo ha = hulls[shape[pr.x]], hb = hulls[shape[pr.y]];
         const int fa = proxy_first[pr.x], fb = proxy_first[pr.y];
+        const long long base = (long long)npairs + off[p];
         const int side = mesh_side(ha, hb);
         if (side >= 0) {  // overlapping triangles in triangle order (c > 0: side is 0 or 1)
             const int m = side == 0 ? pr.x : pr.y, b = side == 0 ? pr.y : pr.x;
             const HullInfo &hm = side == 0 ? ha : hb;
             const float4 pm = mv.pos[m], qm = mv.quat[m], bmin = mv.aabb_min[b], bmax = mv.aabb_max[b];
-            uint32_t k = 0;
-            for (int t = 0; t < hm.num_edges && k < c; t++) {
-                if (!mesh_tri_overlaps(hulls[mv.children[hm.edge_offset + t].shape], mv.verts, pm, qm,
-                                       bmin, bmax))
-                    continue;
-                const long long dst = (long long)npairs + off[p] + k++;
-                if (dst >= max_pairs) break;
-                pairs[dst] = side == 0 ? make_int2(n + fa + t, pr.y) : make_int2(pr.x, n + fb + t);
+            uint32_t k0 = 0;
+            for (int t0 = 0; t0 < hm.num_edges && k0 < c; t0 += 64) {
+                const int t = t0 + lane;
+                const bool ov = t < hm.num_edges &&
+                                mesh_tri_overlaps(hulls[mv.children[hm.edge_offset + t].shape], mv.verts,
+                                                  pm, qm, bmin, bmax);
+                const unsigned long long mask = __ballot(ov);
+                const uint32_t k = k0 + (uint32_t)__popcll(mask & lanes_below(lane));
+                if (ov && k < c && base + k < max_pairs)
+                    pairs[base + k] = side == 0 ? make_int2(n + fa + t, pr.y) : make_int2(pr.x, n + fb + t);
+                k0 += (uint32_t)__popcll(mask);
             }
             continue;
         }
         const int cb = compound_children(hb);
         const int nb = cb ? cb : 1;
-        for (uint32_t k = 0; k < c; k++) {
-            const long long dst = (long long)npairs + off[p] + k;
+        for (uint32_t k = lane; k < c; k += 64) {
+            const long long dst = base + k;
             if (dst >= max_pairs) break;
             const int i = (int)k / nb, j = (int)k % nb;
             pairs[dst] = make_int2(fa >= 0 ? n + fa + i : pr.x, fb >= 0 ? n + fb + j : pr.y);
