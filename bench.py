@@ -227,6 +227,48 @@ def broadphase_1m(steps=100, warmup=5):
             "overflow_flags": flags.value}
 
 
+def mesh_terrain(quads=100, steps=50, warmup=60):
+    """Concave mesh leg: 4,096 unit cubes (a 64 x 64 grid, 1.5 apart, from
+    y = 1) dropped on a static 100 x 100-quad terrain mesh (20,000
+    triangles over 100 x 100 units, height 0.2 sin(0.3 x) cos(0.25 z)),
+    8 solver iterations, graph replay.  Time per step after the cubes have
+    landed, the readback of the bodies included."""
+    import time
+    from experiments_amd import scenes
+    from experiments_amd.world import World, reference_config
+    g = np.linspace(-50, 50, quads + 1)
+    verts = np.array([(x, 0.2 * np.sin(0.3 * x) * np.cos(0.25 * z), z) for x in g for z in g],
+                     np.float32)
+    tris = []
+    for i in range(quads):
+        for j in range(quads):
+            a, b = i * (quads + 1) + j, i * (quads + 1) + j + 1
+            c, d = (i + 1) * (quads + 1) + j + 1, (i + 1) * (quads + 1) + j
+            tris += [(a, b, c), (a, c, d)]
+    w = World(reference_config(max_bodies=8192, max_shapes=2 * quads * quads + 8,
+                               max_pairs=1 << 20, max_contacts=1 << 18, max_pairs_per_body=64,
+                               solver_iterations=8))
+    cube = w.add_hull(scenes.cube_points(0.5))
+    mesh = w.add_concave_mesh(verts, np.array(tris, np.int32))
+    w.add_body(0.0, (0, 0, 0), (0, 0, 0, 1), mesh)
+    for i in range(64):
+        for j in range(64):
+            w.add_body(1.0, (-47 + 1.5 * i, 1.0, -47 + 1.5 * j), (0, 0, 0, 1), cube)
+    w.upload()
+    w.use_graph(True)
+    w.step(warmup)
+    w.get_state()
+    t = time.perf_counter()
+    w.step(steps)
+    w.get_state()
+    ms = (time.perf_counter() - t) / steps * 1e3
+    st = w.stats()
+    w.close()
+    return {"triangles": len(tris), "cubes": 4096, "steps": steps, "ms_per_step": round(ms, 4),
+            "body_steps_per_s": round(4097 / (ms / 1e3), 1), "pairs": st["num_pairs"],
+            "contacts": st["num_contacts"], "overflow_flags": st["overflow_flags"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -366,6 +408,7 @@ def main():
             w.close()
             w = None
         out["broadphase_1m"] = broadphase_1m()
+        out["mesh_terrain"] = mesh_terrain()
     if w is not None:
         w.close()
     print(json.dumps(out))
