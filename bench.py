@@ -36,7 +36,8 @@ HBM_PEAK_GBS = 8000.0
 SOLVE_TRAFFIC_BYTES = {"hulls64k": ((5371 + 5006) * 2.0 + 1478 + 1769) / 2 * 1024,
                        "batched4k": (1165580 * 2.0 + 233047) * 1024}
 
-# algorithmic bytes per unit for each stage (documented in DESIGN.md)
+# algorithmic bytes per unit for each stage: what each stage must read and
+# write once per body, pair, contact or sort pass (stage_bytes below)
 STAGES = ["setup", "sort", "pairs", "narrowphase", "constraints", "colouring", "solve",
           "integrate"]
 
@@ -291,8 +292,8 @@ def main():
                          "counter calibration for rocprofv3 --pmc) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only; --gpus %d is not supported (the multi-GPU "
-              "sharding is a design in DESIGN.md, not built)" % args.gpus, file=sys.stderr)
+        print("bench.py runs on one GPU only; --gpus %d is not supported (sharding the "
+              "scenes over several GPUs is designed but not built)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     if args.copy_probe_mib:
