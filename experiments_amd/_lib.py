@@ -162,6 +162,7 @@ SIGNATURES = {
     "mi_rb_radix_sort_create": (I, [I, PVP]),
     "mi_rb_radix_sort_destroy": (I, [VP]),
     "mi_rb_radix_sort_kv": (I, [VP, VP, I, I, VP]),
+    "mi_rb_radix_sort_kv_noalloc": (I, [VP, I, I, VP]),
     "mi_rb_exclusive_scan_u32": (I, [VP, VP, VP, I, VP, VP]),
     "mi_rb_bound_search": (I, [VP, I, VP, I, I, VP]),
     "mi_rb_grid_create": (I, [ctypes.POINTER(GridDesc), I, PVP]),

@@ -13,6 +13,8 @@
 // tiles that are already running.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "dmath.h"
 #include "rb_internal.h"
 
@@ -449,6 +451,67 @@ extern "C" mi_rb_status mi_rb_radix_sort_kv(void *sort, void *d_keyvalue, int n,
     if (!h || !d_keyvalue || n < 0 || n > h->w.max_n || sort_bits < 0 || sort_bits > 32)
         return MI_RB_INVALID_ARG;
     hipError_t e = launch_radix_sort(static_cast<uint2 *>(d_keyvalue), n, sort_bits, 8, h->w,
+                                     static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? MI_RB_OK : MI_RB_DEVICE;
+}
+
+// ---- the same sort with its workspace in a file-scope device array ----
+// Layout of `scratch` (32-bit words): the ping-pong copy of the pairs
+// (2 per element), the digit counts of every pass, the look-back status words
+// (passes x tiles x 256) and the tile counters with the look-back error word.
+constexpr int NA_MAX_N = MI_RB_SORT_NOALLOC_MAX;
+constexpr int NA_TILES = (NA_MAX_N + SORT_TILE - 1) / SORT_TILE;
+constexpr int NA_PASSES = 4;  // 32 key bits in 8-bit digits
+constexpr size_t NA_TMP = 0;
+constexpr size_t NA_HIST = NA_TMP + 2 * (size_t)NA_MAX_N;
+constexpr size_t NA_STATUS = NA_HIST + (size_t)MAX_PASSES * 256;
+constexpr size_t NA_CTR = NA_STATUS + (size_t)NA_PASSES * NA_TILES * 256;
+constexpr size_t NA_WORDS = NA_CTR + 64;
+static_assert(NA_HIST % 64 == 0 && NA_STATUS % 64 == 0 && NA_CTR % 64 == 0, "256-byte sections");
+
+static __device__ __align__(256) uint32_t scratch[NA_WORDS];
+
+namespace {
+constexpr int NA_MAX_DEVICES = 64;
+std::mutex g_na_mutex;
+uint32_t *g_na_ptr[NA_MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device (each device holds its own copy of the array).
+uint32_t *noalloc_scratch() {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NA_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(g_na_mutex);
+    if (!g_na_ptr[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return nullptr;
+        g_na_ptr[dev] = static_cast<uint32_t *>(p);
+    }
+    return g_na_ptr[dev];
+}
+}  // namespace
+
+extern "C" mi_rb_status mi_rb_radix_sort_kv_noalloc(void *d_keyvalue, int n, int sort_bits,
+                                                    void *stream) {
+    if (!d_keyvalue || n < 0 || sort_bits < 0 || sort_bits > 32) return MI_RB_INVALID_ARG;
+    if (n > NA_MAX_N) return MI_RB_CAPACITY;
+    uint32_t *base = noalloc_scratch();
+    if (!base) return MI_RB_DEVICE;
+    SortWork w;
+    w.tmp = reinterpret_cast<uint2 *>(base + NA_TMP);
+    w.hist = base + NA_HIST;
+    w.status = base + NA_STATUS;
+    w.tile_ctr = base + NA_CTR;
+    w.scan_status = nullptr;
+    w.max_n = NA_MAX_N;
+    w.max_tiles = NA_TILES;
+    // the look-back error word keeps the previous call's value: clear it too
+    if (n > 1 && sort_bits > 0) {
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, static_cast<hipStream_t>(stream),
+                           w.tile_ctr + MAX_PASSES, 0u, (size_t)1);
+        if (hipGetLastError() != hipSuccess) return MI_RB_DEVICE;
+    }
+    hipError_t e = launch_radix_sort(static_cast<uint2 *>(d_keyvalue), n, sort_bits, 8, w,
                                      static_cast<hipStream_t>(stream));
     return e == hipSuccess ? MI_RB_OK : MI_RB_DEVICE;
 }

@@ -374,6 +374,17 @@ mi_rb_status mi_rb_radix_sort_destroy(void *sort);
  * key bits, in place. */
 mi_rb_status mi_rb_radix_sort_kv(void *sort, void *d_keyvalue, int n, int sort_bits,
                                  void *stream);
+/* mi_rb_radix_sort_kv without a sort handle and without allocating: no
+ * counterpart in the reference.  For callers that cannot call hipMalloc
+ * (code under stream capture, tight serving loops, hosts that forbid device
+ * allocation).  The workspace is a file-scope device array of about 9 MiB
+ * that each GPU the library runs on holds until the process ends; its
+ * address is looked up once per device.  n above MI_RB_SORT_NOALLOC_MAX
+ * gives MI_RB_CAPACITY.  Every call on one GPU shares that GPU's workspace,
+ * so calls that overlap on one GPU (concurrent streams or host threads)
+ * race: serialise them.  Enqueued on `stream` of the current device. */
+#define MI_RB_SORT_NOALLOC_MAX (1 << 20)
+mi_rb_status mi_rb_radix_sort_kv_noalloc(void *d_keyvalue, int n, int sort_bits, void *stream);
 /* btPrefixScanCL::execute (btPrefixScanCL.cpp:46-99): exclusive scan of uint32;
  * the total is written to d_total when non-NULL (device pointer). */
 mi_rb_status mi_rb_exclusive_scan_u32(void *sort, const void *d_in, void *d_out, int n,
