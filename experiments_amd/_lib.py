@@ -119,6 +119,7 @@ SIGNATURES = {
     "mi_rb_create": (I, [ctypes.POINTER(Config), PVP]),
     "mi_rb_destroy": (I, [VP]),
     "mi_rb_register_convex_hull": (I, [VP, VP, I, I, VP, PI]),
+    "mi_rb_register_convex_polyhedron": (I, [VP, VP, I, I, VP, VP, I, VP, I, VP, I, I, PI]),
     "mi_rb_register_unsupported_shape": (I, [VP, I]),
     "mi_rb_register_sphere": (I, [VP, F, PI]),
     "mi_rb_register_plane": (I, [VP, VP, F, PI]),

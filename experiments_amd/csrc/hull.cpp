@@ -3,7 +3,9 @@
 // Replaces btConvexHullComputer + btConvexUtility::initializePolyhedralFeatures /
 // initialize (opencl/gpu_rigidbody_pipeline/btConvexUtility.cpp:31-510) as used by
 // CLPhysicsDemo::registerConvexPolyhedron (opencl/gpu_rigidbody_pipeline2/CLPhysicsDemo.cpp:491-530):
-//   * the hull of the points, as outward-wound triangles (incremental hull in double);
+//   * the points snapped to the integer grid btConvexHullComputer works on
+//     (btConvexHullComputer.cpp:1940-2082), and the hull of the grid points as
+//     outward-wound triangles (incremental hull with exact predicates);
 //   * the face plane of each triangle from its first two normalized edges, with
 //     plane w = -min over the face's vertices of dot(v, n) (btConvexUtility.cpp:73-113);
 //   * coplanar triangles (normal dot > 0.999) merged into one polygon ordered
@@ -11,9 +13,15 @@
 //     by another face (btConvexUtility.cpp:117-268);
 //   * unique edge directions, normalized, one per +-direction within 1e-6
 //     (btConvexUtility.cpp:339-390).
-// Vertices keep the order in which they appear in the input.
+// Vertices keep the order in which they appear in the input; the reference
+// lists them in the order btConvexHullComputer walks its edge graph, so
+// vertex, face and unique-edge order (and the sign of an edge direction)
+// differ from the reference while the sets are the same.
+// mi_rb_register_convex_polyhedron takes features built elsewhere (the
+// reference's own, in its order) unchanged.
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <vector>
 
@@ -37,50 +45,51 @@ struct Tri {
     bool alive;
 };
 
-// Incremental convex hull; returns outward-wound triangles over point indices.
+// Incremental convex hull of integer-valued points; returns outward-wound
+// triangles over point indices.  Every predicate is an exact sign of an
+// integer determinant (the coordinates are small enough for doubles to hold
+// every product), so points on a face's plane are never taken as outside it.
 bool build_hull(const std::vector<V3> &P, std::vector<Tri> &tris) {
     int n = (int)P.size();
     if (n < 4) return false;
-    double scale = 0;
-    for (auto &p : P) scale = std::max(scale, std::max(std::fabs(p.x), std::max(std::fabs(p.y), std::fabs(p.z))));
-    const double eps = 1e-9 * std::max(scale, 1.0);
+    auto same = [](V3 a, V3 b) { return a.x == b.x && a.y == b.y && a.z == b.z; };
+    auto nonzero = [](V3 a) { return a.x != 0.0 || a.y != 0.0 || a.z != 0.0; };
     // initial tetrahedron
     int i0 = 0, i1 = -1, i2 = -1, i3 = -1;
     for (int i = 1; i < n; i++)
-        if (len(sub(P[i], P[i0])) > eps * 10) { i1 = i; break; }
+        if (!same(P[i], P[i0])) { i1 = i; break; }
     if (i1 < 0) return false;
     for (int i = 1; i < n; i++)
-        if (len(cross(sub(P[i1], P[i0]), sub(P[i], P[i0]))) > eps * 10 * std::max(scale, 1.0)) { i2 = i; break; }
+        if (nonzero(cross(sub(P[i1], P[i0]), sub(P[i], P[i0])))) { i2 = i; break; }
     if (i2 < 0) return false;
     V3 nrm = cross(sub(P[i1], P[i0]), sub(P[i2], P[i0]));
     for (int i = 1; i < n; i++)
-        if (std::fabs(dot(nrm, sub(P[i], P[i0]))) > eps * 10 * std::max(scale * scale, 1.0)) { i3 = i; break; }
+        if (dot(nrm, sub(P[i], P[i0])) != 0.0) { i3 = i; break; }
     if (i3 < 0) return false;
     auto make = [&](int a, int b, int c) {
         Tri t;
         t.v[0] = a; t.v[1] = b; t.v[2] = c;
         t.n = cross(sub(P[b], P[a]), sub(P[c], P[a]));
-        double l = len(t.n);
-        t.n = {t.n.x / l, t.n.y / l, t.n.z / l};
         t.d = dot(t.n, P[a]);
         t.alive = true;
         return t;
     };
-    V3 cen = {(P[i0].x + P[i1].x + P[i2].x + P[i3].x) / 4, (P[i0].y + P[i1].y + P[i2].y + P[i3].y) / 4,
-              (P[i0].z + P[i1].z + P[i2].z + P[i3].z) / 4};
+    // four times the tetrahedron's centroid (integer)
+    V3 cen4 = {P[i0].x + P[i1].x + P[i2].x + P[i3].x, P[i0].y + P[i1].y + P[i2].y + P[i3].y,
+               P[i0].z + P[i1].z + P[i2].z + P[i3].z};
     int q[4] = {i0, i1, i2, i3};
     int faces[4][3] = {{0, 1, 2}, {0, 3, 1}, {0, 2, 3}, {1, 3, 2}};
     tris.clear();
     for (auto &f : faces) {
         Tri t = make(q[f[0]], q[f[1]], q[f[2]]);
-        if (dot(t.n, cen) - t.d > 0) t = make(q[f[0]], q[f[2]], q[f[1]]);
+        if (dot(t.n, cen4) - 4.0 * t.d > 0) t = make(q[f[0]], q[f[2]], q[f[1]]);
         tris.push_back(t);
     }
     for (int p = 0; p < n; p++) {
         if (p == i0 || p == i1 || p == i2 || p == i3) continue;
         std::vector<int> vis;
         for (int t = 0; t < (int)tris.size(); t++)
-            if (tris[t].alive && dot(tris[t].n, P[p]) - tris[t].d > eps) vis.push_back(t);
+            if (tris[t].alive && dot(tris[t].n, P[p]) - tris[t].d > 0.0) vis.push_back(t);
         if (vis.empty()) continue;
         // horizon: directed edges of visible faces whose twin is not visible
         std::vector<std::pair<int, int>> edges;
@@ -136,22 +145,77 @@ void face_plane(const std::vector<float> &V, FaceF &f) {
 
 }  // namespace
 
+// btConvexHullInternal::compute (bullet2/LinearMath/btConvexHullComputer.cpp:1940-2035)
+// builds the hull of integer points: every axis of the points' AABB is cut
+// into 10216 steps about the AABB centre (the step's sign flipped for one
+// axis order) and each coordinate is truncated to a whole number of steps;
+// the vertices it returns are those grid points mapped back as
+// step * integer + centre (getCoordinates, :2075-2082).  The same grid is
+// used here, in the same float operations, so registered vertices equal the
+// reference's bit for bit and the hull topology is decided exactly on the
+// integers.
+struct HullGrid {
+    float scaling[3], inv[3], center[3];
+};
+static HullGrid hull_grid(const std::vector<float> &p, int n) {
+    float mn[3] = {1e30f, 1e30f, 1e30f}, mx[3] = {-1e30f, -1e30f, -1e30f};
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < 3; k++) {
+            float v = p[3 * i + k];
+            if (v < mn[k]) mn[k] = v;
+            if (mx[k] < v) mx[k] = v;
+        }
+    float s[3] = {mx[0] - mn[0], mx[1] - mn[1], mx[2] - mn[2]};
+    int max_axis = s[0] < s[1] ? (s[1] < s[2] ? 2 : 1) : (s[0] < s[2] ? 2 : 0);
+    int min_axis = s[0] < s[1] ? (s[0] < s[2] ? 0 : 2) : (s[1] < s[2] ? 1 : 2);
+    if (min_axis == max_axis) min_axis = (max_axis + 1) % 3;
+    int med_axis = 3 - max_axis - min_axis;
+    const float step = 1.0f / 10216.0f;
+    HullGrid g;
+    for (int k = 0; k < 3; k++) {
+        s[k] *= step;
+        if ((med_axis + 1) % 3 != max_axis) s[k] *= -1.0f;
+        g.scaling[k] = s[k];
+        g.inv[k] = s[k] != 0.f ? 1.0f / s[k] : s[k];
+        g.center[k] = (mn[k] + mx[k]) * 0.5f;
+    }
+    return g;
+}
+
 int rb_build_hull(const float *xyz, int stride_bytes, int n, const float scale[3], HullData &out) {
     out = HullData();
     if (!xyz || n < 4 || stride_bytes < 12) return MI_RB_INVALID_ARG;
-    std::vector<V3> P(n);
     const unsigned char *base = reinterpret_cast<const unsigned char *>(xyz);
     float s[3] = {1.f, 1.f, 1.f};
     if (scale) { s[0] = scale[0]; s[1] = scale[1]; s[2] = scale[2]; }
-    std::vector<float> inF(3 * (size_t)n);
+    std::vector<float> raw(3 * (size_t)n);
     for (int i = 0; i < n; i++) {
         const float *v = reinterpret_cast<const float *>(base + (size_t)i * stride_bytes);
-        inF[3 * i] = v[0] * s[0]; inF[3 * i + 1] = v[1] * s[1]; inF[3 * i + 2] = v[2] * s[2];
-        P[i] = {inF[3 * i], inF[3 * i + 1], inF[3 * i + 2]};
+        raw[3 * i] = v[0] * s[0]; raw[3 * i + 1] = v[1] * s[1]; raw[3 * i + 2] = v[2] * s[2];
+        for (int k = 0; k < 3; k++)
+            if (!std::isfinite(raw[3 * i + k])) return MI_RB_INVALID_ARG;
+    }
+    // grid points: P holds the integer coordinates (exact in double, |x| <=
+    // 5108 per axis, so every orientation test below is exact), inF the
+    // vertex each maps back to.  The integer axes are given the sign of their
+    // step so that P has the orientation of the float space.
+    const HullGrid g = hull_grid(raw, n);
+    std::vector<V3> P(n);
+    std::vector<float> inF(3 * (size_t)n);
+    for (int i = 0; i < n; i++) {
+        double q[3];
+        for (int k = 0; k < 3; k++) {
+            float t = (raw[3 * i + k] - g.center[k]) * g.inv[k];
+            int32_t c = (int32_t)t;
+            inF[3 * i + k] = (float)c * g.scaling[k] + g.center[k];
+            q[k] = g.scaling[k] < 0.f ? -(double)c : (double)c;
+        }
+        P[i] = {q[0], q[1], q[2]};
     }
     std::vector<Tri> tris;
     if (!build_hull(P, tris)) return MI_RB_INVALID_ARG;
-    // hull vertices in input order (duplicates of a used point are dropped)
+    // hull vertices in input order (later points on the same grid point are
+    // dropped)
     std::vector<int> used(n, 0), remap(n, -1);
     for (auto &t : tris)
         for (int k = 0; k < 3; k++) used[t.v[k]] = 1;
@@ -161,10 +225,64 @@ int rb_build_hull(const float *xyz, int stride_bytes, int n, const float scale[3
             remap[i] = (int)(V.size() / 3);
             V.push_back(inF[3 * i]); V.push_back(inF[3 * i + 1]); V.push_back(inF[3 * i + 2]);
         }
-    std::vector<FaceF> tmp(tris.size());
-    for (size_t t = 0; t < tris.size(); t++) {
-        for (int k = 0; k < 3; k++) tmp[t].idx.push_back(remap[tris[t].v[k]]);
-        face_plane(V, tmp[t]);
+    // btConvexHullComputer's faces are the hull's exact planar polygons:
+    // triangles that share an edge and lie in one plane of the integer grid
+    // are joined, and each polygon is the loop of its boundary edges
+    const int nt = (int)tris.size();
+    std::vector<int> group(nt);
+    for (int t = 0; t < nt; t++) group[t] = t;
+    auto root = [&](int t) {
+        while (group[t] != t) t = group[t] = group[group[t]];
+        return t;
+    };
+    for (int t = 0; t < nt; t++)
+        for (int u = t + 1; u < nt; u++) {
+            int shared = 0, other = -1;
+            for (int k = 0; k < 3; k++) {
+                bool in_t = tris[t].v[0] == tris[u].v[k] || tris[t].v[1] == tris[u].v[k] ||
+                            tris[t].v[2] == tris[u].v[k];
+                if (in_t) shared++; else other = tris[u].v[k];
+            }
+            if (shared == 2 && dot(tris[t].n, P[other]) - tris[t].d == 0.0)
+                group[root(u)] = root(t);
+        }
+    std::vector<FaceF> tmp;
+    std::vector<int> done(nt, 0);
+    for (int t = 0; t < nt; t++) {
+        if (done[t]) continue;
+        const int r = root(t);
+        std::vector<std::pair<int, int>> dir;
+        for (int u = t; u < nt; u++)
+            if (root(u) == r) {
+                done[u] = 1;
+                for (int k = 0; k < 3; k++) dir.push_back({tris[u].v[k], tris[u].v[(k + 1) % 3]});
+            }
+        std::vector<std::pair<int, int>> boundary;
+        for (auto &e : dir) {
+            bool twin = false;
+            for (auto &f : dir)
+                if (f.first == e.second && f.second == e.first) { twin = true; break; }
+            if (!twin) boundary.push_back(e);
+        }
+        FaceF f;
+        int v = tris[t].v[0];
+        for (auto &e : boundary)
+            if (e.first == v) break;
+        // start at the first triangle's first vertex when it is on the
+        // boundary, else at the first boundary edge
+        bool on = false;
+        for (auto &e : boundary) on = on || e.first == v;
+        if (!on) v = boundary[0].first;
+        for (size_t k = 0; k < boundary.size(); k++) {
+            f.idx.push_back(remap[v]);
+            int next = -1;
+            for (auto &e : boundary)
+                if (e.first == v) { next = e.second; break; }
+            if (next < 0) return MI_RB_INVALID_ARG;
+            v = next;
+        }
+        face_plane(V, f);
+        tmp.push_back(f);
     }
     // merge coplanar groups (btConvexUtility.cpp:117-268)
     std::vector<FaceF> faces;
@@ -332,5 +450,77 @@ extern "C" mi_rb_status mi_rb_hull_build(const float *xyz, int stride_bytes, int
         for (int k = 0; k < 3; k++) if (edges4) edges4[4 * i + k] = h.edges[3 * i + k];
         if (edges4) edges4[4 * i + 3] = 0.f;
     }
+    return MI_RB_OK;
+}
+
+// Features built elsewhere (a btConvexUtility's vertices, faces and unique
+// edges), kept as given; the centre is the vertex average and the AABB that
+// of the vertices (CLPhysicsDemo.cpp:361-405), the inscribed radius the
+// smallest centre-to-plane distance over the faces' fan triangles.
+int rb_hull_from_features(const float *verts, int vert_stride, int nv, const float *planes4,
+                          const int *face_index2, int nf, const int *indices, int ni,
+                          const float *edges, int edge_stride, int ne, HullData &out) {
+    out = HullData();
+    if (!verts || !planes4 || !face_index2 || !indices || !edges || nv < 4 || nf < 4 ||
+        ni < 3 * nf || ne < 1 || vert_stride < 12 || edge_stride < 12)
+        return MI_RB_INVALID_ARG;
+    auto at = [](const float *base, int stride, int i) {
+        return reinterpret_cast<const float *>(reinterpret_cast<const unsigned char *>(base) +
+                                               (size_t)i * stride);
+    };
+    for (int i = 0; i < nv; i++)
+        for (int k = 0; k < 3; k++) {
+            float v = at(verts, vert_stride, i)[k];
+            if (!std::isfinite(v)) return MI_RB_INVALID_ARG;
+            out.verts.push_back(v);
+        }
+    for (int f = 0; f < nf; f++) {
+        int first = face_index2[2 * f], cnt = face_index2[2 * f + 1];
+        if (first < 0 || cnt < 3 || first > ni - cnt) return MI_RB_INVALID_ARG;
+        for (int k = 0; k < 4; k++) {
+            if (!std::isfinite(planes4[4 * f + k])) return MI_RB_INVALID_ARG;
+            out.planes.push_back(planes4[4 * f + k]);
+        }
+        out.face_index.push_back(first);
+        out.face_index.push_back(cnt);
+    }
+    for (int i = 0; i < ni; i++) {
+        if (indices[i] < 0 || indices[i] >= nv) return MI_RB_INVALID_ARG;
+        out.indices.push_back(indices[i]);
+    }
+    for (int i = 0; i < ne; i++)
+        for (int k = 0; k < 3; k++) {
+            float v = at(edges, edge_stride, i)[k];
+            if (!std::isfinite(v)) return MI_RB_INVALID_ARG;
+            out.edges.push_back(v);
+        }
+    const std::vector<float> &V = out.verts;
+    float c[3] = {0, 0, 0};
+    for (int i = 0; i < nv; i++)
+        for (int k = 0; k < 3; k++) c[k] += V[3 * i + k];
+    float inv = 1.f / (float)nv;
+    for (int k = 0; k < 3; k++) out.center[k] = c[k] * inv;
+    const V3 cc = {out.center[0], out.center[1], out.center[2]};
+    double rin = 1e300;
+    for (int f = 0; f < nf; f++) {
+        const int first = out.face_index[2 * f], cnt = out.face_index[2 * f + 1];
+        const float *p0 = &V[3 * out.indices[first]];
+        V3 a = {p0[0], p0[1], p0[2]};
+        for (int j = 1; j + 1 < cnt; j++) {
+            const float *p1 = &V[3 * out.indices[first + j]], *p2 = &V[3 * out.indices[first + j + 1]];
+            V3 b = {p1[0], p1[1], p1[2]}, d = {p2[0], p2[1], p2[2]};
+            V3 nn = cross(sub(b, a), sub(d, a));
+            const double l = len(nn);
+            if (!(l > 0.0)) continue;
+            rin = std::min(rin, std::fabs(dot(nn, sub(a, cc)) / l));
+        }
+    }
+    out.inradius = rin < 1e300 ? rin : 0.0;
+    for (int k = 0; k < 3; k++) { out.aabb_min[k] = 1e30f; out.aabb_max[k] = -1e30f; }
+    for (int i = 0; i < nv; i++)
+        for (int k = 0; k < 3; k++) {
+            out.aabb_min[k] = std::min(out.aabb_min[k], V[3 * i + k]);
+            out.aabb_max[k] = std::max(out.aabb_max[k], V[3 * i + k]);
+        }
     return MI_RB_OK;
 }

@@ -16,3 +16,7 @@ struct HullData {
 
 // Returns an mi_rb_status value.
 int rb_build_hull(const float *xyz, int stride_bytes, int n, const float scale[3], HullData &out);
+// Features given as they are (validated: indices in range, finite values).
+int rb_hull_from_features(const float *verts, int vert_stride, int nv, const float *planes4,
+                          const int *face_index2, int nf, const int *indices, int ni,
+                          const float *edges, int edge_stride, int ne, HullData &out);

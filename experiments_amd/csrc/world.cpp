@@ -129,27 +129,44 @@ World *W(void *w) { return static_cast<World *>(w); }
 
 // inertia at registration (btGpuNarrowPhaseAndSolver.cpp:566-657 with the
 // world AABB of CLPhysicsDemo::registerPhysicsInstance, CLPhysicsDemo.cpp:536-552)
-float4 local_inv_inertia(const World &w, float mass, int shape, const float q[4]) {
+float4 local_inv_inertia(const World &w, float mass, int shape, const float pos[3],
+                         const float q[4]) {
     if (mass == 0.f) return make_float4(0, 0, 0, 0);
     const HullInfo &h = w.hulls[shape];
-    float hx = (h.aabb_max.x - h.aabb_min.x) * 0.5f, hy = (h.aabb_max.y - h.aabb_min.y) * 0.5f,
-          hz = (h.aabb_max.z - h.aabb_min.z) * 0.5f;
+    const float lmin[3] = {h.aabb_min.x, h.aabb_min.y, h.aabb_min.z};
+    const float lmax[3] = {h.aabb_max.x, h.aabb_max.y, h.aabb_max.z};
     float lx, ly, lz;
     if (w.cfg.inertia_mode == MI_RB_INERTIA_BOX) {
-        lx = 2.f * hx; ly = 2.f * hy; lz = 2.f * hz;
+        lx = lmax[0] - lmin[0]; ly = lmax[1] - lmin[1]; lz = lmax[2] - lmin[2];
     } else {
-        // btTransformAabb with margin 0.01 and the body's orientation, then
-        // "fake larger inertia": l = 2 * (aabbMax - aabbMin)
-        float margin = 0.01f;
-        float ex = hx + margin, ey = hy + margin, ez = hz + margin;
-        float x = q[0], y = q[1], z = q[2], s = q[3];
-        float m[3][3] = {{1 - 2 * (y * y + z * z), 2 * (x * y - s * z), 2 * (x * z + s * y)},
-                         {2 * (x * y + s * z), 1 - 2 * (x * x + z * z), 2 * (y * z - s * x)},
-                         {2 * (x * z - s * y), 2 * (y * z + s * x), 1 - 2 * (x * x + y * y)}};
-        float e[3];
-        for (int r = 0; r < 3; r++)
-            e[r] = std::fabs(m[r][0]) * ex + std::fabs(m[r][1]) * ey + std::fabs(m[r][2]) * ez;
-        lx = 2.f * (2.f * e[0]); ly = 2.f * (2.f * e[1]); lz = 2.f * (2.f * e[2]);
+        // btTransformAabb (bullet2/LinearMath/btAabbUtil2.h:193-207) with
+        // margin 0.01 and the body's transform, the rotation as
+        // btMatrix3x3::setRotation forms it; then "fake larger inertia":
+        // l = 2 * (aabbMax - aabbMin)
+        const float margin = 0.01f;
+        float x = q[0], y = q[1], z = q[2], qw = q[3];
+        float d = ((x * x + y * y) + z * z) + qw * qw;
+        float s = 2.0f / d;
+        float xs = x * s, ys = y * s, zs = z * s;
+        float wx = qw * xs, wy = qw * ys, wz = qw * zs;
+        float xx = x * xs, xy = x * ys, xz = x * zs;
+        float yy = y * ys, yz = y * zs, zz = z * zs;
+        float m[3][3] = {{1.0f - (yy + zz), xy - wz, xz + wy},
+                         {xy + wz, 1.0f - (xx + zz), yz - wx},
+                         {xz - wy, yz + wx, 1.0f - (xx + yy)}};
+        float he[3], lc[3], l[3];
+        for (int k = 0; k < 3; k++) {
+            he[k] = 0.5f * (lmax[k] - lmin[k]);
+            he[k] = he[k] + margin;
+            lc[k] = 0.5f * (lmax[k] + lmin[k]);
+        }
+        for (int r = 0; r < 3; r++) {
+            float c = ((m[r][0] * lc[0] + m[r][1] * lc[1]) + m[r][2] * lc[2]) + pos[r];
+            float e = (he[0] * std::fabs(m[r][0]) + he[1] * std::fabs(m[r][1])) +
+                      he[2] * std::fabs(m[r][2]);
+            l[r] = 2.f * ((c + e) - (c - e));
+        }
+        lx = l[0]; ly = l[1]; lz = l[2];
     }
     float ix = (mass / 12.0f) * (ly * ly + lz * lz);
     float iy = (mass / 12.0f) * (lx * lx + lz * lz);
@@ -564,17 +581,15 @@ extern "C" mi_rb_status mi_rb_destroy(void *world) {
     return MI_RB_OK;
 }
 
-extern "C" mi_rb_status mi_rb_register_convex_hull(void *world, const float *xyz, int stride_bytes,
-                                                   int n, const float scale[3], int *out_shape) {
-    World *w = W(world);
-    if (!w || !xyz || n < 4 || !out_shape) return MI_RB_INVALID_ARG;
-    if ((int)w->hulls.size() >= w->cfg.max_shapes) return MI_RB_CAPACITY;
-    HullData h;
-    int st = rb_build_hull(xyz, stride_bytes, n, scale, h);
-    if (st != MI_RB_OK) return (mi_rb_status)st;
+// Appends one hull's features to the shape arrays (registerConvexHullShape,
+// btGpuNarrowPhaseAndSolver.cpp:423-490).
+static mi_rb_status add_hull(World *w, const HullData &h, int *out_shape) {
     int nv = (int)h.verts.size() / 3, nf = (int)h.planes.size() / 4;
     int ne = (int)h.edges.size() / 3, ni = (int)h.indices.size();
-    if (nv > RB_MAX_HULL_VERTS || ne > RB_MAX_HULL_EDGES) return MI_RB_CAPACITY;
+    if (nv > RB_MAX_HULL_VERTS || ne > RB_MAX_HULL_EDGES || nf > 2 * RB_MAX_HULL_FACES)
+        return MI_RB_CAPACITY;
+    for (int f = 0; f < nf; f++)
+        if (h.face_index[2 * f + 1] > RB_MAX_FACE_VERTS) return MI_RB_CAPACITY;
     if ((int)w->verts.size() + nv > w->cfg.max_hull_vertices) return MI_RB_CAPACITY;
     HullInfo hi;
     hi.face_offset = (int)w->planes.size();
@@ -617,6 +632,32 @@ extern "C" mi_rb_status mi_rb_register_convex_hull(void *world, const float *xyz
     *out_shape = (int)w->hulls.size() - 1;
     w->uploaded = false;
     return MI_RB_OK;
+}
+
+extern "C" mi_rb_status mi_rb_register_convex_hull(void *world, const float *xyz, int stride_bytes,
+                                                   int n, const float scale[3], int *out_shape) {
+    World *w = W(world);
+    if (!w || !xyz || n < 4 || !out_shape) return MI_RB_INVALID_ARG;
+    if ((int)w->hulls.size() >= w->cfg.max_shapes) return MI_RB_CAPACITY;
+    HullData h;
+    int st = rb_build_hull(xyz, stride_bytes, n, scale, h);
+    if (st != MI_RB_OK) return (mi_rb_status)st;
+    return add_hull(w, h, out_shape);
+}
+
+extern "C" mi_rb_status mi_rb_register_convex_polyhedron(
+    void *world, const float *verts, int vert_stride_bytes, int num_vertices, const float *planes4,
+    const int *face_index2, int num_faces, const int *indices, int num_indices, const float *edges,
+    int edge_stride_bytes, int num_edges, int *out_shape) {
+    World *w = W(world);
+    if (!w || !out_shape) return MI_RB_INVALID_ARG;
+    if ((int)w->hulls.size() >= w->cfg.max_shapes) return MI_RB_CAPACITY;
+    HullData h;
+    int st = rb_hull_from_features(verts, vert_stride_bytes, num_vertices, planes4, face_index2,
+                                   num_faces, indices, num_indices, edges, edge_stride_bytes,
+                                   num_edges, h);
+    if (st != MI_RB_OK) return (mi_rb_status)st;
+    return add_hull(w, h, out_shape);
 }
 
 // A shape without vertices (a sphere or a plane): no faces for a sphere, the
@@ -862,7 +903,7 @@ extern "C" mi_rb_status mi_rb_register_body(void *world, float mass, const float
     w->inv_mass.push_back(mass != 0.f ? 1.f / mass : 0.f);
     w->shape.push_back(shape);
     w->scene.push_back(scene);
-    w->inv_inertia_local.push_back(local_inv_inertia(*w, mass, shape, quat_xyzw));
+    w->inv_inertia_local.push_back(local_inv_inertia(*w, mass, shape, pos, quat_xyzw));
     *out_body = (int)w->pos.size() - 1;
     w->uploaded = false;
     return MI_RB_OK;

@@ -72,6 +72,21 @@ class World:
         self.num_shapes += 1
         return out.value
 
+    def add_polyhedron(self, verts, planes, face_index, indices, edges):
+        """Polyhedral features as given (a btConvexUtility's: vertices xyz,
+        face planes nx ny nz w, (first index, count) per face, face vertex
+        indices, unique edge directions xyz), kept in their order."""
+        v = _f32(verts).reshape(-1, 3).copy()
+        pl = _f32(planes).reshape(-1, 4).copy()
+        fi = np.ascontiguousarray(face_index, np.int32).reshape(-1, 2).copy()
+        ind = np.ascontiguousarray(indices, np.int32).ravel().copy()
+        e = _f32(edges).reshape(-1, 3).copy()
+        out = ctypes.c_int()
+        L.call("mi_rb_register_convex_polyhedron", self.h, _ptr(v), 12, len(v), _ptr(pl),
+               _ptr(fi), len(fi), _ptr(ind), len(ind), _ptr(e), 12, len(e), ctypes.byref(out))
+        self.num_shapes += 1
+        return out.value
+
     def add_sphere(self, radius):
         out = ctypes.c_int()
         L.call("mi_rb_register_sphere", self.h, ctypes.c_float(radius), ctypes.byref(out))

@@ -162,9 +162,30 @@ mi_rb_status mi_rb_destroy(void *world);
 
 /* CLPhysicsDemo::registerConvexPolyhedron(const float*, int strideBytes, int n,
  * const float* scaling, bool) (CLPhysicsDemo.cpp:491-530): convex hull of the
- * points (scaled), coplanar faces merged; returns the shape index. */
+ * points (scaled) on btConvexHullComputer's integer grid (so the vertices
+ * are the reference's), coplanar faces merged as btConvexUtility does;
+ * returns the shape index.  Vertices are listed in input order, where the
+ * reference lists them in its hull walk's order. */
 mi_rb_status mi_rb_register_convex_hull(void *world, const float *xyz, int stride_bytes, int n,
                                         const float scale[3], int *out_shape);
+/* CLPhysicsDemo::registerConvexPolyhedron(btConvexUtility*, bool)
+ * (CLPhysicsDemo.cpp:331-410) with btGpuNarrowphaseAndSolver::
+ * registerConvexHullShape (btGpuNarrowPhaseAndSolver.cpp:423-490): registers
+ * polyhedral features built elsewhere (a btConvexUtility's) unchanged and in
+ * their order: num_vertices points (xyz, vert_stride_bytes apart), num_faces
+ * faces (plane nx, ny, nz, w as float4 in planes4, and (first index, count)
+ * into indices in face_index2), num_indices vertex indices, num_edges unique
+ * edge directions (xyz, edge_stride_bytes apart).  As in the reference the
+ * centre is the vertex average and the local AABB that of the vertices.
+ * MI_RB_INVALID_ARG for indices out of range, faces of fewer than 3 vertices
+ * or values that are not finite; MI_RB_CAPACITY beyond 64 vertices, 128
+ * faces, 64 vertices in a face or 128 edges. */
+mi_rb_status mi_rb_register_convex_polyhedron(void *world, const float *verts,
+                                              int vert_stride_bytes, int num_vertices,
+                                              const float *planes4, const int *face_index2,
+                                              int num_faces, const int *indices, int num_indices,
+                                              const float *edges, int edge_stride_bytes,
+                                              int num_edges, int *out_shape);
 /* CLPhysicsDemo::registerSphereShape(radius) (CLPhysicsDemo.cpp:420-456): a
  * sphere; local AABB +-radius.  Contacts: sphere-sphere and sphere-plane,
  * one point each (satClipHullContacts.cl:1055-1118); a sphere against a hull

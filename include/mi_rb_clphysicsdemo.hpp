@@ -57,6 +57,36 @@ struct CLPhysicsDemo {
                                               scaling, &shape);
         return m_status == MI_RB_OK ? shape : -1;
     }
+    /* registerConvexPolyhedron(btConvexUtility* utilPtr, bool noHeightField)
+     * (CLPhysicsDemo.cpp:331-410): registers the caller's polyhedral
+     * features as they are, in their order.  Util: anything with the members
+     * of btConvexUtility that the reference reads -- m_vertices (points with
+     * operator[] giving x, y, z), m_faces (m_plane[4], m_indices) and
+     * m_uniqueEdges -- each a container with size() and operator[]. */
+    template <class Util>
+    int registerConvexPolyhedron(Util *utilPtr, bool noHeightField) {
+        (void)noHeightField;
+        const int nv = (int)utilPtr->m_vertices.size(), nf = (int)utilPtr->m_faces.size();
+        const int ne = (int)utilPtr->m_uniqueEdges.size();
+        std::vector<float> verts(3 * (size_t)nv), planes(4 * (size_t)nf), edges(3 * (size_t)ne);
+        std::vector<int> face_index(2 * (size_t)nf), indices;
+        for (int i = 0; i < nv; i++)
+            for (int k = 0; k < 3; k++) verts[3 * i + k] = utilPtr->m_vertices[i][k];
+        for (int f = 0; f < nf; f++) {
+            const auto &face = utilPtr->m_faces[f];
+            for (int k = 0; k < 4; k++) planes[4 * f + k] = face.m_plane[k];
+            face_index[2 * f] = (int)indices.size();
+            face_index[2 * f + 1] = (int)face.m_indices.size();
+            for (int k = 0; k < (int)face.m_indices.size(); k++) indices.push_back(face.m_indices[k]);
+        }
+        for (int i = 0; i < ne; i++)
+            for (int k = 0; k < 3; k++) edges[3 * i + k] = utilPtr->m_uniqueEdges[i][k];
+        int shape = -1;
+        m_status = mi_rb_register_convex_polyhedron(
+            m_world, verts.data(), 3 * (int)sizeof(float), nv, planes.data(), face_index.data(), nf,
+            indices.data(), (int)indices.size(), edges.data(), 3 * (int)sizeof(float), ne, &shape);
+        return m_status == MI_RB_OK ? shape : -1;
+    }
     int registerSphereShape(float radius) {
         int shape = -1;
         m_status = mi_rb_register_sphere(m_world, radius, &shape);

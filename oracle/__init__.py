@@ -79,6 +79,28 @@ def f4(a, n=None):
     return out
 
 
+def plane_space(n):
+    """The friction tangents (t0, t1) of normal n (orc_plane_space)."""
+    nn = np.ascontiguousarray(n[:3], np.float32)
+    t0 = np.zeros(4, np.float32)
+    t1 = np.zeros(4, np.float32)
+    lib().orc_plane_space(_p(nn), _p(t0), _p(t1))
+    return t0[:3], t1[:3]
+
+
+# ---------------------------------------------------------------- registration
+def register_inertia(mass, local_min, local_max, pos, quat, margin=0.01):
+    """Inverse local inertia diagonal (x, y, z, 0) a body gets at registration
+    (orc_register_inertia)."""
+    out = np.zeros(4, np.float32)
+    mn = np.ascontiguousarray(local_min[:3], np.float32)
+    mx = np.ascontiguousarray(local_max[:3], np.float32)
+    p = np.ascontiguousarray(pos[:3], np.float32)
+    q = np.ascontiguousarray(quat[:4], np.float32)
+    lib().orc_register_inertia(_f(mass), _p(mn), _p(mx), _p(p), _p(q), _f(margin), _p(out))
+    return out
+
+
 # ----------------------------------------------------------------- primitives
 def radix_sort_kv(keys, values, sort_bits=32):
     kv = np.stack([np.asarray(keys, np.uint32), np.asarray(values, np.uint32)], 1).copy()

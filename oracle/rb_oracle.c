@@ -772,6 +772,46 @@ int orc_narrowphase(int num_pairs, const int32_t *pairs, const float *pos, const
 
 /* ======================================================================== */
 /* Solver (dynamics/basic_demo/Stubs)                                       */
+/* Body registration inertia: CLPhysicsDemo::registerPhysicsInstance's world
+ * AABB of the shape (CLPhysicsDemo.cpp:536-552: btTransformAabb with margin
+ * 0.01, bullet2/LinearMath/btAabbUtil2.h:193-207, the rotation from
+ * btMatrix3x3::setRotation's scalar form, btMatrix3x3.h:204-249) and
+ * btGpuNarrowphaseAndSolver::registerRigidBody's box inertia of twice that
+ * AABB's size (btGpuNarrowPhaseAndSolver.cpp:614-640).  Writes the inverse
+ * diagonal (x, y, z, 0); a body without mass gets zeros. */
+void orc_register_inertia(float mass, const float local_min[3], const float local_max[3],
+                          const float pos[3], const float quat[4], float margin, float out[4])
+{
+    if (mass == 0.f) { out[0] = out[1] = out[2] = out[3] = 0.f; return; }
+    float x = quat[0], y = quat[1], z = quat[2], w = quat[3];
+    float d = ((x * x + y * y) + z * z) + w * w;
+    float s = 2.0f / d;
+    float xs = x * s, ys = y * s, zs = z * s;
+    float wx = w * xs, wy = w * ys, wz = w * zs;
+    float xx = x * xs, xy = x * ys, xz = x * zs;
+    float yy = y * ys, yz = y * zs, zz = z * zs;
+    float m[3][3] = {{1.0f - (yy + zz), xy - wz, xz + wy},
+                     {xy + wz, 1.0f - (xx + zz), yz - wx},
+                     {xz - wy, yz + wx, 1.0f - (xx + yy)}};
+    float he[3], lc[3], mn[3], mx[3];
+    for (int k = 0; k < 3; k++) {
+        he[k] = 0.5f * (local_max[k] - local_min[k]);
+        he[k] = he[k] + margin;
+        lc[k] = 0.5f * (local_max[k] + local_min[k]);
+    }
+    for (int r = 0; r < 3; r++) {
+        float c = ((m[r][0] * lc[0] + m[r][1] * lc[1]) + m[r][2] * lc[2]) + pos[r];
+        float e = (he[0] * fabsf(m[r][0]) + he[1] * fabsf(m[r][1])) + he[2] * fabsf(m[r][2]);
+        mn[r] = c - e;
+        mx[r] = c + e;
+    }
+    float lx = 2.f * (mx[0] - mn[0]), ly = 2.f * (mx[1] - mn[1]), lz = 2.f * (mx[2] - mn[2]);
+    float ix = (mass / 12.0f) * (ly * ly + lz * lz);
+    float iy = (mass / 12.0f) * (lx * lx + lz * lz);
+    float iz = (mass / 12.0f) * (lx * lx + ly * ly);
+    out[0] = 1.f / ix; out[1] = 1.f / iy; out[2] = 1.f / iz; out[3] = 0.f;
+}
+
 /* ======================================================================== */
 
 /* mtMul1 / mtMul3 (solveContact.cl:130-153) */
@@ -810,6 +850,14 @@ static void plane_space(f4 n, f4 *p, f4 *q)
         *p = mk4(-n.y * k, n.x * k, 0.f, 0.f);
         *q = mk4(-n.z * p->y, n.z * p->x, a * k, 0.f);
     }
+}
+
+void orc_plane_space(const float n[3], float t0[4], float t1[4])
+{
+    f4 p, q;
+    plane_space(mk4(n[0], n[1], n[2], 0.f), &p, &q);
+    st4(t0, p);
+    st4(t1, q);
 }
 
 /* ContactToConstraintKernel / setConstraint4 (solverSetup.cl:520-660), with

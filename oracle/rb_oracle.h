@@ -56,6 +56,10 @@ typedef struct {
     uint32_t body_a, body_b, batch, pad;
 } orc_constraint;
 
+/* ---- registration ---------------------------------------------------------- */
+void orc_register_inertia(float mass, const float local_min[3], const float local_max[3],
+                          const float pos[3], const float quat[4], float margin, float out[4]);
+
 /* ---- primitives ---------------------------------------------------------- */
 int orc_num_threads(void);
 void orc_radix_sort_kv(uint32_t *kv, int n, int sort_bits);
@@ -100,6 +104,8 @@ int orc_narrowphase(int num_pairs, const int32_t *pairs, const float *pos, const
                     float max_dist, orc_contact *slots, int *has_contact);
 
 /* ---- solver ---------------------------------------------------------------- */
+/* the two friction tangents of a contact normal (btPlaneSpace1) */
+void orc_plane_space(const float n[3], float t0[4], float t1[4]);
 void orc_contact_to_constraint(int nc, const orc_contact *contacts, const float *pos,
                                const float *linvel, const float *angvel, const float *inv_mass,
                                const float *inv_inertia_world, float dt, float drift,

@@ -61,14 +61,21 @@ def test_random_hull_invariants(nverts):
         nv, nf = len(v), len(pl)
         n_edges = fi[:, 1].sum() // 2
         assert nv - n_edges + nf == 2  # Euler
-        # every input point inside every triangle's plane; faces merged from
+        # every hull vertex inside every triangle's plane; faces merged from
         # near-coplanar triangles (normal dot > 0.999) keep their first
-        # triangle's plane, so points may sit outside it by at most the weld
+        # triangle's plane, so vertices may sit outside it by at most the weld
         # angle times the hull diameter
-        d = pts @ pl[:, :3].T + pl[:, 3]
+        d = v @ pl[:, :3].T + pl[:, 3]
         tri = fi[:, 1] == 3
         assert d[:, tri].max() < 1e-5
         assert d.max() < np.sin(np.arccos(0.999)) * 2.0
+        # the vertices are the input points snapped to a grid of 10216 steps
+        # per axis of their AABB: every input point lies within one step of
+        # the hull, and every vertex within one step of an input point
+        step = (pts.max(0) - pts.min(0)) / 10216.0
+        assert (pts @ pl[:, :3].T + pl[:, 3])[:, tri].max() < np.linalg.norm(step) + 1e-6
+        near = np.abs(v[:, None, :] - pts[None, :, :]) <= step + 1e-7
+        assert near.all(-1).any(1).all()
         # triangles are planar; every face is wound with the outward normal
         for f in range(nf):
             idx = ind[fi[f, 0]:fi[f, 0] + fi[f, 1]]
