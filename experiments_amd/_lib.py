@@ -17,6 +17,7 @@ WARM_ZERO, WARM_PREVIOUS, WARM_LEARNED = 0, 1, 2
 INERTIA_REFERENCE, INERTIA_BOX = 0, 1
 OVERFLOW_PAIRS, OVERFLOW_BODY_PAIRS, OVERFLOW_CONTACTS = 1, 2, 4
 OVERFLOW_COLORS, OVERFLOW_TILE, OVERFLOW_POLYGON, OVERFLOW_LEVEL = 8, 16, 32, 64
+OVERFLOW_LOOKBACK, OVERFLOW_UNSUPPORTED_PAIR = 128, 256
 (BUF_SORTED_KV, BUF_AABB, BUF_PAIRS, BUF_CONTACTS, BUF_CONSTRAINTS, BUF_SOLVE_ORDER, BUF_COLORS,
  BUF_INV_INERTIA, BUF_PHASES, BUF_HULLS, BUF_VERTS, BUF_PLANES, BUF_FACE_INDEX, BUF_INDICES,
  BUF_EDGES, BUF_INV_MASS, BUF_INV_INERTIA_LOCAL, BUF_SCENE, BUF_IS_LARGE, BUF_SHAPE,
@@ -145,6 +146,7 @@ SIGNATURES = {
     "mi_rb_read_buffer": (I, [VP, I, VP, ctypes.c_size_t, PI]),
     "mi_rb_save_state": (I, [VP, CP]),
     "mi_rb_load_state": (I, [VP, CP]),
+    "mi_rb_snapshot_check": (I, [CP, ctypes.POINTER(Config), PI]),
     "mi_rb_set_warmstart_weights": (I, [VP, VP, VP]),
     "mi_rb_copy_training_batch": (I, [VP, VP, VP, I, PI]),
     "mi_rb_set_tuning": (I, [VP, ctypes.POINTER(Tuning)]),

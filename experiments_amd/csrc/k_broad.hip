@@ -967,7 +967,7 @@ hipError_t launch_hgrid_pairs(const float4 *mn, const float4 *mx, int n, float i
                            level_used, smn, smx, g, counts, (const uint32_t *)nullptr, cursor,
                            max_pairs, ys);
     }
-    e = launch_exclusive_scan(counts, offsets, n, total, scan_status, s);
+    e = launch_exclusive_scan(counts, offsets, n, total, scan_status, s, nullptr, overflow);
     if (e != hipSuccess) return e;
     if (n > 0)
         hipLaunchKernelGGL(k_hg_pairs<1>, dim3(hg_blocks), dim3(256), 0, s, n, kv, cell_start, occ,
@@ -1006,7 +1006,7 @@ hipError_t launch_sap_pairs(const float4 *mn, const float4 *mx, const int *is_la
         hipLaunchKernelGGL(k_sap_large<0>, dim3(blocks), dim3(256), 0, s, n, mn, mx, smn, smx, lg,
                            lg_n, counts, (const uint32_t *)nullptr, cursor, max_pairs, ys);
     }
-    e = launch_exclusive_scan(counts, offsets, n, total, scan_status, s);
+    e = launch_exclusive_scan(counts, offsets, n, total, scan_status, s, nullptr, overflow);
     if (e != hipSuccess) return e;
     if (n > 0) {
         hipLaunchKernelGGL(k_sap_pairs<1>, dim3(blocks), dim3(256), 0, s, n, axis, smn, smx, lg_n,

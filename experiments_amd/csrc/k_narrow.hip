@@ -863,7 +863,8 @@ __global__ __launch_bounds__(256) void k_compound_count(int max_pairs, const int
                                                         const int *__restrict__ shape,
                                                         const float *__restrict__ inv_mass,
                                                         const HullInfo *__restrict__ hulls, MeshView mv,
-                                                        uint32_t *__restrict__ cnt) {
+                                                        uint32_t *__restrict__ cnt,
+                                                        int *__restrict__ overflow) {
     int npairs = *num_pairs_ptr;
     if (npairs > max_pairs) npairs = max_pairs;
     const int lane = threadIdx.x & 63;
@@ -875,6 +876,10 @@ __global__ __launch_bounds__(256) void k_compound_count(int max_pairs, const int
         const int side = mesh_side(ha, hb);
         uint32_t c = 0;
         if (side >= 0) {
+            // a mesh against a sphere, a plane, a compound or a mesh: no
+            // contacts, reported
+            if (side == 2 && !both_static && lane == 0)
+                atomicOr(overflow, (int)MI_RB_OVERFLOW_UNSUPPORTED_PAIR);
             if (side < 2 && !both_static) {
                 const int m = side == 0 ? pr.x : pr.y, b = side == 0 ? pr.y : pr.x;
                 const HullInfo &hm = side == 0 ? ha : hb;
@@ -970,7 +975,8 @@ hipError_t launch_narrowphase(const DeviceWorld &d, const StepParams &p, hipStre
                            d.hulls, d.planes, p.friction, d.contact_slots, d.has_contact);
     hipError_t e = launch_exclusive_scan(d.narrow_cand, d.cand_off, p.max_pairs,
                                          (uint32_t *)(d.counters + RB_CNT_NARROW_CAND), scan_status,
-                                         s, d.counters + RB_CNT_PAIRS);
+                                         s, d.counters + RB_CNT_PAIRS,
+                                         d.counters + RB_CNT_OVERFLOW);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_narrow_keys, dim3((p.max_pairs + 255) / 256), dim3(256), 0, s,
                        p.max_pairs, d.counters + RB_CNT_PAIRS, d.narrow_cand, d.cand_off,
@@ -993,10 +999,11 @@ hipError_t launch_compound_pairs(const DeviceWorld &d, const StepParams &p, hipS
     const MeshView mv{d.pos, d.quat, d.aabb_min, d.aabb_max, d.verts, d.children};
     hipLaunchKernelGGL(k_compound_count, dim3(grid), dim3(256), 0, s, p.max_pairs,
                        d.counters + RB_CNT_PAIRS, d.pairs, d.shape, d.inv_mass, d.hulls, mv,
-                       d.narrow_cand);
+                       d.narrow_cand, d.counters + RB_CNT_OVERFLOW);
     hipError_t e = launch_exclusive_scan(d.narrow_cand, d.cand_off, p.max_pairs,
                                          (uint32_t *)(d.counters + RB_CNT_NARROW_CAND), scan_status,
-                                         s, d.counters + RB_CNT_PAIRS);
+                                         s, d.counters + RB_CNT_PAIRS,
+                                         d.counters + RB_CNT_OVERFLOW);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_compound_write, dim3(grid), dim3(256), 0, s, p.max_pairs,
                        d.counters + RB_CNT_PAIRS, d.pairs, d.shape, d.hulls, mv, d.proxy_first,

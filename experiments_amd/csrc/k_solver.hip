@@ -80,7 +80,8 @@ __global__ __launch_bounds__(256) void k_constraints(
     const float4 *__restrict__ aabb_min, const float4 *__restrict__ aabb_max, ConstraintSoA cs,
     uint2 *__restrict__ tile_kv, int *counters, const int2 *__restrict__ prev_keys,
     const float4 *__restrict__ prev_imp, const float4 *__restrict__ prev_pts,
-    const int *__restrict__ prev_count, float *__restrict__ features, int *__restrict__ solve_pos) {
+    const int *__restrict__ prev_count, float *__restrict__ features, int *__restrict__ solve_pos,
+    const int2 *__restrict__ pairs) {
     int np = *num_pairs_ptr;
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np || !has_contact[p]) return;
@@ -138,18 +139,20 @@ __global__ __launch_bounds__(256) void k_constraints(
     cs.bodies[k] = make_int2(a, b);
     cs.pair[k] = p;
     solve_pos[k] = -1;  // set by the tile packing when the row is solved
-    // ---- warm start: previous step's impulses of the same body pair ----
+    // ---- warm start: previous step's impulses of the same pair entry (the
+    // body pair, or for a compound child or mesh triangle the child pair) ----
     int prev_n = *prev_count;
     float4 prev = mk4(0, 0, 0, 0);
     int has_prev = 0, lo_match = 0;
     {
+        const int2 key = pairs[p];
         int lo = 0, hi = prev_n;
         while (lo < hi) {
             int mid = (lo + hi) >> 1;
             int2 kk = prev_keys[mid];
-            if (kk.x < a || (kk.x == a && kk.y < b)) lo = mid + 1; else hi = mid;
+            if (kk.x < key.x || (kk.x == key.x && kk.y < key.y)) lo = mid + 1; else hi = mid;
         }
-        if (lo < prev_n && prev_keys[lo].x == a && prev_keys[lo].y == b) {
+        if (lo < prev_n && prev_keys[lo].x == key.x && prev_keys[lo].y == key.y) {
             prev = prev_imp[lo];
             has_prev = 1;
             lo_match = lo;
@@ -589,12 +592,15 @@ __device__ __forceinline__ void row_warm(const RowData &d, float4 &vA, float4 &w
 
 // After the solve: the impulses go back from the packed solve arrays to the
 // constraints, and into the previous-step impulse cache for the warm start,
-// keyed by the body pair (constraints are ordered by pair, so the keys are
-// sorted): the totals (sum of normal impulses, friction 0, friction 1, number
-// of points) and each point's position with its normal impulse.
+// keyed by the constraint's pair entry (unique per manifold; constraints are
+// ordered by pair, so without compound child pairs, which are appended after
+// the body pairs, the keys come out sorted): the totals (sum of normal
+// impulses, friction 0, friction 1, number of points) and each point's
+// position with its normal impulse.
 __global__ void k_save_prev(const int *counters, ConstraintSoA cs, const int *solve_pos,
                             const float4 *pk_lam, const float4 *pk_fric, int2 *prev_keys,
-                            float4 *prev_imp, float4 *prev_pts, int *prev_count) {
+                            float4 *prev_imp, float4 *prev_pts, int *prev_count,
+                            const int2 *pairs) {
     int nc = counters[RB_CNT_CONTACTS];
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k == 0) *prev_count = nc;
@@ -612,7 +618,7 @@ __global__ void k_save_prev(const int *counters, ConstraintSoA cs, const int *so
     }
     float4 jac = cs.jac_inv[k];
     int np = (jac.x != 0.f) + (jac.y != 0.f) + (jac.z != 0.f) + (jac.w != 0.f);
-    prev_keys[k] = cs.bodies[k];
+    prev_keys[k] = pairs[cs.pair[k]];
     prev_imp[k] = mk4(((l.x + l.y) + l.z) + l.w, f.z, f.w, (float)np);
     float lv[4] = {l.x, l.y, l.z, l.w};
     for (int i = 0; i < 4; i++) {
@@ -1172,7 +1178,7 @@ hipError_t launch_constraints(const DeviceWorld &d, const StepParams &p, hipStre
                        d.has_contact, d.contact_off, d.contact_slots, d.contacts, d.pos, d.linvel,
                        d.angvel, d.inv_mass, d.inv_inertia_world, d.scene, d.aabb_min, d.aabb_max,
                        d.cs, d.tile_kv, d.counters, d.prev_keys, d.prev_imp, d.prev_pts, d.prev_count,
-                       d.features, d.solve_pos);
+                       d.features, d.solve_pos, d.pairs);
     return hipGetLastError();
 }
 
@@ -1253,9 +1259,56 @@ hipError_t launch_solve(const DeviceWorld &d, const StepParams &p, hipStream_t s
                                p.friction_iters, d.counters);
     }
 #undef RB_SOLVE_ARGS
+    const bool staged = p.n_proxies > 0;
     hipLaunchKernelGGL(k_save_prev, dim3((p.max_contacts + 255) / 256), dim3(256), 0, s,
-                       d.counters, d.cs, d.solve_pos, d.pk.lam, d.pk.fric, d.prev_keys, d.prev_imp,
-                       d.prev_pts, d.prev_count);
+                       d.counters, d.cs, d.solve_pos, d.pk.lam, d.pk.fric,
+                       staged ? d.stage_keys : d.prev_keys, staged ? d.stage_imp : d.prev_imp,
+                       staged ? d.stage_pts : d.prev_pts, d.prev_count, d.pairs);
+    return hipGetLastError();
+}
+
+namespace {
+// (key part, row) of every cache entry: y first, then x of the order so far
+__global__ void k_cache_key_y(const int *counters, const int2 *keys, uint2 *kv) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < counters[RB_CNT_CONTACTS]) kv[k] = make_uint2((uint32_t)keys[k].y, (uint32_t)k);
+}
+__global__ void k_cache_key_x(const int *counters, const int2 *keys, uint2 *kv) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < counters[RB_CNT_CONTACTS]) kv[k].x = (uint32_t)keys[kv[k].y].x;
+}
+__global__ void k_cache_gather(const int *counters, const uint2 *kv, const int2 *keys,
+                               const float4 *imp, const float4 *pts, int2 *out_keys,
+                               float4 *out_imp, float4 *out_pts) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= counters[RB_CNT_CONTACTS]) return;
+    const int src = (int)kv[k].y;
+    out_keys[k] = keys[src];
+    out_imp[k] = imp[src];
+    for (int i = 0; i < 4; i++) out_pts[4 * (size_t)k + i] = pts[4 * (size_t)src + i];
+}
+}  // namespace
+
+hipError_t launch_cache_order(const DeviceWorld &d, const StepParams &p, hipStream_t s,
+                              const SortWork &sw) {
+    if (p.n_proxies <= 0) return hipSuccess;
+    // pair entries name bodies and compound/triangle proxies below n + proxies
+    const uint32_t top = (uint32_t)(p.n_bodies + p.n_proxies);
+    int bits = 1;
+    while (bits < 32 && (top >> bits) != 0) bits++;
+    const int grid = (p.max_contacts + 255) / 256;
+    int *nc = d.counters + RB_CNT_CONTACTS;
+    hipLaunchKernelGGL(k_cache_key_y, dim3(grid), dim3(256), 0, s, d.counters, d.stage_keys,
+                       d.tile_kv);
+    hipError_t e = launch_radix_sort(d.tile_kv, p.max_contacts, bits, 8, sw, s, nc);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_cache_key_x, dim3(grid), dim3(256), 0, s, d.counters, d.stage_keys,
+                       d.tile_kv);
+    e = launch_radix_sort(d.tile_kv, p.max_contacts, bits, 8, sw, s, nc);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_cache_gather, dim3(grid), dim3(256), 0, s, d.counters, d.tile_kv,
+                       d.stage_keys, d.stage_imp, d.stage_pts, d.prev_keys, d.prev_imp,
+                       d.prev_pts);
     return hipGetLastError();
 }
 

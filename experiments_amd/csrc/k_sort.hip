@@ -98,7 +98,8 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_onesweep(const uint2 *__restrict
                                                          uint2 *__restrict__ dst, int n, int shift,
                                                          int bits, const uint32_t *hist,
                                                          uint32_t *status, uint32_t *tile_ctr,
-                                                         uint32_t *err, const int *n_dev) {
+                                                         uint32_t *err, uint32_t err_bit,
+                                                         const int *n_dev) {
     constexpr int RADIX = 1 << RBITS;
     if (n_dev && *n_dev < n) n = *n_dev;
     __shared__ uint32_t s_wave_hist[WAVES][RADIX];
@@ -169,7 +170,7 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_onesweep(const uint2 *__restrict
                 uint32_t v = __hip_atomic_load(status + (size_t)t * RADIX + d, __ATOMIC_RELAXED,
                                                __HIP_MEMORY_SCOPE_AGENT);
                 if ((v & ~VAL_MASK) == 0) {  // predecessor not published yet
-                    if (++spins > (1u << 26)) { atomicOr(err, 1u); break; }
+                    if (++spins > (1u << 26)) { atomicOr(err, err_bit); break; }
                     continue;
                 }
                 prefix += v & VAL_MASK;
@@ -230,7 +231,8 @@ constexpr uint64_t SVAL_MASK = (1ull << 62) - 1ull;
 
 __global__ __launch_bounds__(SCAN_BLOCK) void k_scan(const uint32_t *in, uint32_t *out, int n,
                                                      uint32_t *total, uint64_t *status,
-                                                     uint32_t *tile_ctr, const int *n_dev) {
+                                                     uint32_t *tile_ctr, const int *n_dev,
+                                                     uint32_t *err, uint32_t err_bit) {
     if (n_dev && *n_dev < n) n = *n_dev;
     __shared__ uint32_t s_wave[SCAN_BLOCK / 64];
     __shared__ uint32_t s_tile;
@@ -275,7 +277,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan(const uint32_t *in, uint32_
             while (true) {
                 uint64_t w = __hip_atomic_load(status + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 if ((w & ~SVAL_MASK) == 0) {
-                    if (++spins > (1u << 26)) { atomicOr(tile_ctr + 1, 1u); break; }
+                    if (++spins > (1u << 26)) { atomicOr(err, err_bit); break; }
                     continue;
                 }
                 prefix += w & SVAL_MASK;
@@ -339,14 +341,18 @@ hipError_t radix_sort_impl(uint2 *kv, int n, int sort_bits, const SortWork &w, h
                        status_words, w.tile_ctr, (size_t)MAX_PASSES);
     hipLaunchKernelGGL(k_histogram<RBITS>, dim3(tiles), dim3(SORT_BLOCK), 0, s, kv, n, passes,
                        sort_bits, w.hist, n_dev);
+    // look-back timeouts: a sticky bit in the caller's overflow word, else
+    // in the word after the tile counters
+    uint32_t *err = w.err ? reinterpret_cast<uint32_t *>(w.err) : w.tile_ctr + MAX_PASSES;
+    const uint32_t err_bit = w.err ? (uint32_t)MI_RB_OVERFLOW_LOOKBACK : 1u;
     uint2 *src = kv, *dst = w.tmp;
     for (int p = 0; p < passes; p++) {
         int shift = p * RBITS;
         int bits = sort_bits - shift < RBITS ? sort_bits - shift : RBITS;
         hipLaunchKernelGGL(k_onesweep<RBITS>, dim3(tiles), dim3(SORT_BLOCK), 0, s, src, dst, n,
                            shift, bits, w.hist + (size_t)p * RADIX,
-                           w.status + (size_t)p * tiles * RADIX, w.tile_ctr + p,
-                           w.tile_ctr + MAX_PASSES, n_dev);
+                           w.status + (size_t)p * tiles * RADIX, w.tile_ctr + p, err, err_bit,
+                           n_dev);
         uint2 *t = src; src = dst; dst = t;
     }
     if (src != kv)  // odd pass count: copy back with a kernel (graph-capture friendly)
@@ -375,7 +381,7 @@ hipError_t launch_radix_sort(uint2 *kv, int n, int sort_bits, int digit_bits, co
 }
 
 hipError_t launch_exclusive_scan(const uint32_t *in, uint32_t *out, int n, uint32_t *total,
-                                 uint32_t *status, hipStream_t s, const int *n_dev) {
+                                 uint32_t *status, hipStream_t s, const int *n_dev, int *err) {
     if (n <= 0) {
         hipLaunchKernelGGL(k_scan_empty, dim3(1), dim3(1), 0, s, total);
         return hipGetLastError();
@@ -385,8 +391,12 @@ hipError_t launch_exclusive_scan(const uint32_t *in, uint32_t *out, int n, uint3
     uint32_t *ctr = status;
     uint64_t *st = reinterpret_cast<uint64_t *>(status + 2);
     hipLaunchKernelGGL(k_fill_u32, dim3(64), dim3(256), 0, s, status, 0u, (size_t)(2 + 2 * tiles));
+    // a look-back timeout sets MI_RB_OVERFLOW_LOOKBACK in *err, else the
+    // word after the tile counter (cleared with the status words)
+    uint32_t *e = err ? reinterpret_cast<uint32_t *>(err) : status + 1;
+    const uint32_t bit = err ? (uint32_t)MI_RB_OVERFLOW_LOOKBACK : 1u;
     hipLaunchKernelGGL(k_scan, dim3(tiles), dim3(SCAN_BLOCK), 0, s, in, out, n, total, st, ctr,
-                       n_dev);
+                       n_dev, e, bit);
     return hipGetLastError();
 }
 
@@ -503,6 +513,7 @@ extern "C" mi_rb_status mi_rb_radix_sort_kv_noalloc(void *d_keyvalue, int n, int
     w.status = base + NA_STATUS;
     w.tile_ctr = base + NA_CTR;
     w.scan_status = nullptr;
+    w.err = nullptr;
     w.max_n = NA_MAX_N;
     w.max_tiles = NA_TILES;
     // the look-back error word keeps the previous call's value: clear it too

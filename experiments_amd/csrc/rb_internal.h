@@ -169,7 +169,9 @@ struct DeviceWorld {
     uint2 *cand_kv;              // (cost key, pair) of the candidates, sorted
     float *residual;             // per sweep: max scaled dlambda, max lambda (as uint bits)
     // warm start
-    int2 *prev_keys;             // previous step's contact body pairs (a, b), sorted
+    int2 *prev_keys;             // previous step's contacts keyed by their pair entry (x, y), sorted
+    int2 *stage_keys;            // worlds with compound children: the cache before it is sorted
+    float4 *stage_imp, *stage_pts;
     float4 *prev_imp;            // their impulses: (sum of normal lambdas, friction 0, friction 1, points)
     float4 *prev_pts;            // 4 per entry: point position (xyz) and its normal lambda (w)
     int *prev_count;             // number of entries (persists across steps)
@@ -244,6 +246,7 @@ struct SortWork {
     uint32_t *status;     // look-back words: passes x tiles x radix
     uint32_t *tile_ctr;   // passes counters
     uint32_t *scan_status;// scan look-back words
+    int *err = nullptr;   // overflow word that gets MI_RB_OVERFLOW_LOOKBACK on a look-back timeout
     int max_n;
     int max_tiles;
 };
@@ -253,7 +256,8 @@ size_t sort_status_words(int max_n, int digit_bits);
 hipError_t launch_radix_sort(uint2 *kv, int n, int sort_bits, int digit_bits, const SortWork &w,
                              hipStream_t s, const int *n_dev = nullptr);
 hipError_t launch_exclusive_scan(const uint32_t *in, uint32_t *out, int n, uint32_t *total,
-                                 uint32_t *status, hipStream_t s, const int *n_dev = nullptr);
+                                 uint32_t *status, hipStream_t s, const int *n_dev = nullptr,
+                                 int *err = nullptr);
 hipError_t launch_bound_search(const uint2 *kv, int n, uint32_t *dst, int n_dst, int mode,
                                hipStream_t s);
 
@@ -272,6 +276,10 @@ hipError_t launch_contact_total(const DeviceWorld &d, const StepParams &p, hipSt
 hipError_t launch_tile_bounds(const DeviceWorld &d, const StepParams &p, hipStream_t s);
 hipError_t launch_tiles(const DeviceWorld &d, const StepParams &p, hipStream_t s);
 hipError_t launch_solve(const DeviceWorld &d, const StepParams &p, hipStream_t s);
+// worlds with compound children: the impulse cache written by the solve in
+// contact order, sorted by key into prev_* (a stable sort by y, then by x)
+hipError_t launch_cache_order(const DeviceWorld &d, const StepParams &p, hipStream_t s,
+                              const SortWork &sw);
 hipError_t launch_copy_transforms(const DeviceWorld &d, int n, float4 *dst, hipStream_t s);
 hipError_t launch_integrate(const DeviceWorld &d, const StepParams &p, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);

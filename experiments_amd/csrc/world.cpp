@@ -384,6 +384,10 @@ mi_rb_status derive_and_upload(World &w) {
     RB_CHECK(w.alloc(&d.prev_imp, maxC));
     RB_CHECK(w.alloc(&d.prev_pts, 4 * (size_t)maxC));
     RB_CHECK(w.alloc(&d.prev_count, 1));
+    const size_t stage_n = p.n_proxies > 0 ? (size_t)maxC : 1;
+    RB_CHECK(w.alloc(&d.stage_keys, stage_n));
+    RB_CHECK(w.alloc(&d.stage_imp, stage_n));
+    RB_CHECK(w.alloc(&d.stage_pts, 4 * stage_n));
     RB_CHECK(w.alloc(&d.features, 16 * (size_t)maxC));
     RB_CHECK(w.alloc(&d.mlp_out, 4 * (size_t)maxC));
     RB_CHECK(w.alloc(&d.mlp_w, RB_MLP_NW));
@@ -415,6 +419,7 @@ mi_rb_status derive_and_upload(World &w) {
     RB_CHECK(w.alloc(&w.sw.hist, 8 * 256));
     RB_CHECK(w.alloc(&w.sw.status, sort_status_words(sort_cap, 8)));
     RB_CHECK(w.alloc(&w.sw.tile_ctr, 16));
+    w.sw.err = d.counters + RB_CNT_OVERFLOW;
     int scan_cap = std::max(n, maxP);
     RB_CHECK(w.alloc(&w.scan_status, rb_scan_status_words(scan_cap)));
 
@@ -491,7 +496,8 @@ mi_rb_status enqueue_step(World &w, int last_stage) {
     RB_CHECK(launch_cell_start(d, p, s));
     RB_CHECK(launch_find_pairs(d, p, s));
     RB_CHECK(launch_exclusive_scan(d.pair_count, d.pair_off, p.n_bodies,
-                                   (uint32_t *)(d.counters + RB_CNT_SORT_TILE), w.scan_status, s));
+                                   (uint32_t *)(d.counters + RB_CNT_SORT_TILE), w.scan_status, s,
+                                   nullptr, d.counters + RB_CNT_OVERFLOW));
     RB_CHECK(launch_emit_pairs(d, p, s));
     if (p.n_proxies > 0) RB_CHECK(launch_compound_pairs(d, p, s, w.scan_status));
     if (ev) RB_CHECK(hipEventRecord(w.ev[3], s));
@@ -501,7 +507,7 @@ mi_rb_status enqueue_step(World &w, int last_stage) {
     if (last_stage < MI_RB_STAGE_CONSTRAINTS) return MI_RB_OK;
     RB_CHECK(launch_exclusive_scan(d.has_contact, d.contact_off, p.max_pairs,
                                    (uint32_t *)(d.counters + RB_CNT_CONTACT_TOTAL), w.scan_status,
-                                   s, d.counters + RB_CNT_PAIRS));
+                                   s, d.counters + RB_CNT_PAIRS, d.counters + RB_CNT_OVERFLOW));
     RB_CHECK(launch_contact_total(d, p, s));
     RB_CHECK(launch_constraints(d, p, s));
     if (p.warm_start == MI_RB_WARM_LEARNED) RB_CHECK(launch_mlp_warm(d, p, s));
@@ -514,6 +520,7 @@ mi_rb_status enqueue_step(World &w, int last_stage) {
     if (ev) RB_CHECK(hipEventRecord(w.ev[6], s));
     if (last_stage < MI_RB_STAGE_SOLVE) return MI_RB_OK;
     RB_CHECK(launch_solve(d, p, s));
+    RB_CHECK(launch_cache_order(d, p, s, w.sw));
     if (ev) RB_CHECK(hipEventRecord(w.ev[7], s));
     if (last_stage < MI_RB_STAGE_INTEGRATE) return MI_RB_OK;
     RB_CHECK(launch_integrate(d, p, s));
@@ -1428,9 +1435,34 @@ extern "C" mi_rb_status mi_rb_save_state(void *world, const char *path) {
     return wr == b.size() ? MI_RB_OK : MI_RB_INVALID_ARG;
 }
 
-extern "C" mi_rb_status mi_rb_load_state(void *world, const char *path) {
-    World *w = W(world);
-    if (!w || !path) return MI_RB_INVALID_ARG;
+namespace {
+// The arrays of a snapshot file, parsed and checked before any of them
+// replaces a world's.
+struct SnapshotArrays {
+    std::vector<HullInfo> hulls;
+    std::vector<ChildShape> children;
+    std::vector<float4> verts, planes, edges;
+    std::vector<int2> face_index;
+    std::vector<int> indices;
+    std::vector<float> hull_hext_norm;
+    std::vector<float4> pos, quat, linvel, angvel, inv_inertia_local;
+    std::vector<float> inv_mass;
+    std::vector<int> shape, scene;
+};
+
+bool finite4(const std::vector<float4> &v) {
+    for (const float4 &x : v)
+        if (!std::isfinite(x.x) || !std::isfinite(x.y) || !std::isfinite(x.z) || !std::isfinite(x.w))
+            return false;
+    return true;
+}
+
+// Reads a "MIRB" v1/v2 file and checks that every count, offset and index
+// stays inside the arrays and the configuration's capacities, so that the
+// device kernels can trust the shapes.  Returns MI_RB_OK,
+// MI_RB_INVALID_ARG (unreadable, truncated or inconsistent) or
+// MI_RB_CAPACITY (beyond the configuration).
+mi_rb_status parse_snapshot(const char *path, const mi_rb_config &cfg, SnapshotArrays &s) {
     FILE *f = fopen(path, "rb");
     if (!f) return MI_RB_INVALID_ARG;
     std::vector<char> b;
@@ -1442,16 +1474,111 @@ extern "C" mi_rb_status mi_rb_load_state(void *world, const char *path) {
     if (b.size() < 8 || std::memcmp(b.data(), "MIRB", 4) != 0 || (b[4] != 1 && b[4] != 2))
         return MI_RB_INVALID_ARG;
     size_t o = 8;
-    bool ok = get(b, o, w->hulls) && get(b, o, w->verts) && get(b, o, w->planes) &&
-              get(b, o, w->edges) && get(b, o, w->face_index) && get(b, o, w->indices) &&
-              get(b, o, w->hull_hext_norm) && get(b, o, w->pos) && get(b, o, w->quat) &&
-              get(b, o, w->linvel) && get(b, o, w->angvel) && get(b, o, w->inv_inertia_local) &&
-              get(b, o, w->inv_mass) && get(b, o, w->shape) && get(b, o, w->scene);
-    if (ok && b[4] == 2) ok = get(b, o, w->children);
-    else if (ok) w->children.clear();
-    if (!ok) return MI_RB_INVALID_ARG;
-    w->free_device();
+    bool ok = get(b, o, s.hulls) && get(b, o, s.verts) && get(b, o, s.planes) &&
+              get(b, o, s.edges) && get(b, o, s.face_index) && get(b, o, s.indices) &&
+              get(b, o, s.hull_hext_norm) && get(b, o, s.pos) && get(b, o, s.quat) &&
+              get(b, o, s.linvel) && get(b, o, s.angvel) && get(b, o, s.inv_inertia_local) &&
+              get(b, o, s.inv_mass) && get(b, o, s.shape) && get(b, o, s.scene);
+    if (ok && b[4] == 2) ok = get(b, o, s.children);
+    if (!ok || o != b.size()) return MI_RB_INVALID_ARG;
+    const size_t n = s.pos.size(), S = s.hulls.size();
+    if (s.quat.size() != n || s.linvel.size() != n || s.angvel.size() != n ||
+        s.inv_inertia_local.size() != n || s.inv_mass.size() != n || s.shape.size() != n ||
+        s.scene.size() != n || s.hull_hext_norm.size() != S ||
+        s.planes.size() != s.face_index.size())
+        return MI_RB_INVALID_ARG;
+    if (n > (size_t)cfg.max_bodies || S > (size_t)cfg.max_shapes ||
+        s.verts.size() > (size_t)cfg.max_hull_vertices)
+        return MI_RB_CAPACITY;
+    if (!finite4(s.pos) || !finite4(s.quat) || !finite4(s.linvel) || !finite4(s.angvel) ||
+        !finite4(s.inv_inertia_local) || !finite4(s.verts) || !finite4(s.planes) ||
+        !finite4(s.edges))
+        return MI_RB_INVALID_ARG;
+    auto in = [](long long off, long long cnt, size_t size) {
+        return off >= 0 && cnt >= 0 && off + cnt <= (long long)size;
+    };
+    for (size_t k = 0; k < S; k++) {
+        const HullInfo &h = s.hulls[k];
+        const int kind = shape_kind_of(h);
+        if (kind == SHAPE_KIND_HULL) {
+            if (h.num_vertices > RB_MAX_HULL_VERTS || h.num_edges > RB_MAX_HULL_EDGES ||
+                h.num_faces > 2 * RB_MAX_HULL_FACES || h.num_faces < 1 ||
+                !in(h.vertex_offset, h.num_vertices, s.verts.size()) ||
+                !in(h.face_offset, h.num_faces, s.planes.size()) ||
+                !in(h.edge_offset, h.num_edges, s.edges.size()) ||
+                !in(h.index_offset, h.num_indices, s.indices.size()))
+                return MI_RB_INVALID_ARG;
+            for (int fc = 0; fc < h.num_faces; fc++) {
+                const int2 fi = s.face_index[h.face_offset + fc];
+                if (fi.y < 1 || fi.y > RB_MAX_FACE_VERTS || !in(fi.x, fi.y, s.indices.size()))
+                    return MI_RB_INVALID_ARG;
+                for (int j = 0; j < fi.y; j++) {
+                    const int v = s.indices[fi.x + j];
+                    if (v < 0 || v >= h.num_vertices) return MI_RB_INVALID_ARG;
+                }
+            }
+        } else if (kind == SHAPE_KIND_COMPOUND) {
+            const int max_children = compound_is_mesh(h) ? MI_RB_MAX_MESH_TRIANGLES : 64;
+            if (h.num_edges > max_children || !in(h.edge_offset, h.num_edges, s.children.size()))
+                return MI_RB_INVALID_ARG;
+            for (int c = 0; c < h.num_edges; c++) {
+                const ChildShape &ch = s.children[h.edge_offset + c];
+                if (ch.shape < 0 || ch.shape >= (int)S ||
+                    shape_kind_of(s.hulls[ch.shape]) != SHAPE_KIND_HULL)
+                    return MI_RB_INVALID_ARG;
+            }
+        } else if (kind == SHAPE_KIND_PLANE) {
+            if (!in(h.face_offset, 1, s.planes.size())) return MI_RB_INVALID_ARG;
+        }
+    }
+    for (size_t i = 0; i < n; i++) {
+        const int sh = s.shape[i];
+        if (sh < 0 || sh >= (int)S || s.scene[i] < 0 || (i > 0 && s.scene[i] < s.scene[i - 1]))
+            return MI_RB_INVALID_ARG;
+        if (!std::isfinite(s.inv_mass[i]) || s.inv_mass[i] < 0.f) return MI_RB_INVALID_ARG;
+        const HullInfo &h = s.hulls[sh];
+        const int kind = shape_kind_of(h);
+        // planes and concave meshes are static only (as at registration)
+        if (s.inv_mass[i] != 0.f && (kind == SHAPE_KIND_PLANE || compound_is_mesh(h)))
+            return MI_RB_INVALID_ARG;
+    }
     return MI_RB_OK;
+}
+}  // namespace
+
+extern "C" mi_rb_status mi_rb_load_state(void *world, const char *path) {
+    World *w = W(world);
+    if (!w || !path) return MI_RB_INVALID_ARG;
+    SnapshotArrays s;
+    mi_rb_status st = parse_snapshot(path, w->cfg, s);
+    if (st != MI_RB_OK) return st;  // the world is left as it was
+    if (w->uploaded) {
+        hipSetDevice(w->device);
+        hipStreamSynchronize(w->stream);
+    }
+    w->free_device();
+    w->hulls.swap(s.hulls); w->children.swap(s.children);
+    w->verts.swap(s.verts); w->planes.swap(s.planes); w->edges.swap(s.edges);
+    w->face_index.swap(s.face_index); w->indices.swap(s.indices);
+    w->hull_hext_norm.swap(s.hull_hext_norm);
+    w->pos.swap(s.pos); w->quat.swap(s.quat); w->linvel.swap(s.linvel); w->angvel.swap(s.angvel);
+    w->inv_inertia_local.swap(s.inv_inertia_local); w->inv_mass.swap(s.inv_mass);
+    w->shape.swap(s.shape); w->scene.swap(s.scene);
+    return MI_RB_OK;
+}
+
+extern "C" mi_rb_status mi_rb_snapshot_check(const char *path, const mi_rb_config *cfg,
+                                             int counts[4]) {
+    if (!path || !cfg) return MI_RB_INVALID_ARG;
+    SnapshotArrays s;
+    mi_rb_status st = parse_snapshot(path, *cfg, s);
+    if (st == MI_RB_OK && counts) {
+        counts[0] = (int)s.pos.size();
+        counts[1] = (int)s.hulls.size();
+        counts[2] = (int)s.verts.size();
+        counts[3] = (int)s.children.size();
+    }
+    return st;
 }
 
 extern "C" mi_rb_status mi_rb_set_warmstart_weights(void *world, const uint16_t *w_bf16,
@@ -1540,6 +1667,7 @@ extern "C" mi_rb_status mi_rb_grid_create(const mi_rb_grid_desc *desc, int max_a
     if (!ok) { g->release(); delete g; return MI_RB_DEVICE; }
     g->sw.max_n = n;
     g->sw.max_tiles = rb_sort_tiles(n);
+    g->sw.err = g->overflow;
     hipMemset(g->cell_start, 0xff, sizeof(uint32_t) * ncells);
     hipMemset(g->overflow, 0, 16);
     for (auto &e : g->ev) hipEventCreate(&e);
@@ -1620,6 +1748,7 @@ extern "C" mi_rb_status mi_rb_sap_create(const mi_rb_sap_desc *desc, int max_aab
     if (!ok) { g->release(); delete g; return MI_RB_DEVICE; }
     g->sw.max_n = n;
     g->sw.max_tiles = rb_sort_tiles(n);
+    g->sw.err = g->overflow;
     hipMemset(g->overflow, 0, 16);
     for (auto &e : g->ev) hipEventCreate(&e);
     if (hipDeviceSynchronize() != hipSuccess) { g->release(); delete g; return MI_RB_DEVICE; }

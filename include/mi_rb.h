@@ -42,7 +42,10 @@ enum {
     MI_RB_OVERFLOW_COLORS = 8,         /* solver colouring ran out of colours */
     MI_RB_OVERFLOW_TILE = 16,          /* solver tile capacity (bodies or rows) */
     MI_RB_OVERFLOW_POLYGON = 32,       /* clipped polygon > 64 vertices */
-    MI_RB_OVERFLOW_LEVEL = 64          /* grid API: an AABB larger than the top level's cell */
+    MI_RB_OVERFLOW_LEVEL = 64,         /* grid API: an AABB larger than the top level's cell */
+    MI_RB_OVERFLOW_LOOKBACK = 128,     /* a sort or scan look-back gave up waiting (results wrong) */
+    MI_RB_OVERFLOW_UNSUPPORTED_PAIR = 256 /* a concave mesh met a body that is not a convex hull
+                                           * (sphere, plane, compound, mesh): no contacts */
 };
 
 typedef struct mi_rb_config {
@@ -134,7 +137,9 @@ enum {
     MI_RB_BUF_IS_LARGE = 18,   /* int per body (overflow list), after upload */
     MI_RB_BUF_SHAPE = 19,      /* int per body */
     MI_RB_BUF_FEATURES = 20,   /* float[16] per contact: warm-start network inputs */
-    MI_RB_BUF_PREV_KEYS = 21,  /* int2 (a, b) per cached pair: impulse cache of the last solve */
+    MI_RB_BUF_PREV_KEYS = 21,  /* int2 per cached manifold, its pair entry (x, y) (compound
+                                * child pairs name proxies): impulse cache of the last solve,
+                                * sorted */
     MI_RB_BUF_PREV_IMPULSES = 22, /* float4 per cached pair: sum lambda_n, friction 0, 1, points */
     MI_RB_BUF_PREDICTIONS = 23, /* float[3] per contact: network output (learned warm start) */
     MI_RB_BUF_PREV_POINTS = 24, /* float4[4] per cached pair: point xyz, its lambda_n in w */
@@ -216,7 +221,8 @@ mi_rb_status mi_rb_register_compound(void *world, int n, const int *child_shapes
  * the hull's gives a triangle-hull pair, found and clipped like a hull pair
  * (sat.cl:970-1191, satClipHullContacts.cl:1387); contacts name the mesh
  * body.  Triangles without area are left out; a mesh against a sphere, a
- * plane, a compound or a mesh gets no contacts.  Uses one shape slot and 3
+ * plane, a compound or a mesh gets no contacts and sets
+ * MI_RB_OVERFLOW_UNSUPPORTED_PAIR.  Uses one shape slot and 3
  * hull vertices per triangle plus one slot for the mesh. */
 #define MI_RB_MAX_MESH_TRIANGLES 65536
 mi_rb_status mi_rb_register_concave_mesh(void *world, const float *xyz, int stride_bytes,
@@ -287,9 +293,16 @@ mi_rb_status mi_rb_copy_transforms(void *world, void *d_vertex_buffer, int start
                                    int capacity_float4, void *stream);
 /* copy a step buffer (MI_RB_BUF_*) to host; *count = elements available */
 mi_rb_status mi_rb_read_buffer(void *world, int which, void *host, size_t max_bytes, int *count);
-/* versioned binary snapshot of shapes + bodies + velocities */
+/* versioned binary snapshot of shapes + bodies + velocities.  load_state
+ * checks the whole file first (array lengths, offsets and indices inside
+ * the arrays, capacities of the world's configuration): a truncated or
+ * inconsistent file gives MI_RB_INVALID_ARG (MI_RB_CAPACITY beyond the
+ * configuration) and leaves the world as it was. */
 mi_rb_status mi_rb_save_state(void *world, const char *path);
 mi_rb_status mi_rb_load_state(void *world, const char *path);
+/* The same check without a world (host only): counts = {bodies, shapes,
+ * vertices, compound children} of a valid file. */
+mi_rb_status mi_rb_snapshot_check(const char *path, const mi_rb_config *cfg, int counts[4]);
 
 /* ---- .bullet scene files ----
  * The reference reads Bullet's serialized scenes with bParse::btBulletFile and

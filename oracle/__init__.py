@@ -276,10 +276,11 @@ def contact_to_constraint(contacts, pos, linvel, angvel, inv_mass, inv_inertia_w
 
 
 def contact_features(contacts, cons, pos, linvel, angvel, inv_mass, prev_keys=None,
-                     prev_imp=None, prev_pts=None, apply_previous=False):
+                     prev_imp=None, prev_pts=None, apply_previous=False, contact_keys=None):
     """16 warm-start features per contact; with apply_previous the previous
-    impulses found for the pair (per point: the nearest previous point's) are
-    written into a copy of cons."""
+    impulses found for the contact's key (contact_keys: int2 per contact, the
+    contact's pair entry; None: its body pair) are written into a copy of
+    cons (per point: the nearest previous point's)."""
     contacts = np.ascontiguousarray(contacts, CONTACT_DTYPE)
     cons = np.array(cons, CONSTRAINT_DTYPE, copy=True)
     nc = len(contacts)
@@ -293,7 +294,9 @@ def contact_features(contacts, cons, pos, linvel, angvel, inv_mass, prev_keys=No
     lib().orc_contact_features(
         ctypes.c_int(nc), _p(contacts), _p(cons), _p(f4(pos)), _p(f4(linvel)), _p(f4(angvel)),
         _p(np.ascontiguousarray(inv_mass, np.float32)), ctypes.c_int(len(pk)), _p(pk), _p(pi),
-        _p(pp), ctypes.c_int(1 if apply_previous else 0), _p(feats))
+        _p(pp), ctypes.c_int(1 if apply_previous else 0),
+        _p(np.ascontiguousarray(contact_keys, np.int32)) if contact_keys is not None else None,
+        _p(feats))
     return feats[:nc], cons
 
 

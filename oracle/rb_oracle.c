@@ -938,7 +938,7 @@ void orc_contact_features(int nc, const orc_contact *contacts, orc_constraint *c
                           const float *pos, const float *linvel, const float *angvel,
                           const float *inv_mass, int nprev, const int *prev_keys,
                           const float *prev_imp, const float *prev_pts, int apply_previous,
-                          float *features)
+                          const int *contact_keys, float *features)
 {
     for (int k = 0; k < nc; k++) {
         const orc_contact *src = &contacts[k];
@@ -951,11 +951,15 @@ void orc_contact_features(int nc, const orc_contact *contacts, orc_constraint *c
         f4 vB = imB != 0.f ? ld4(linvel + 4 * b) : z, wB = imB != 0.f ? ld4(angvel + 4 * b) : z;
         f4 n = ld4(src->normal);
         float npts = src->normal[3];
-        /* linear search: the cache is small in tests, order does not matter */
+        /* linear search: the cache is small in tests, order does not matter;
+         * an entry is found by the contact's key (its pair entry) or, without
+         * keys, by the body pair */
         f4 prev = z;
         int found = -1;
+        const int ka = contact_keys ? contact_keys[2 * k] : a;
+        const int kb = contact_keys ? contact_keys[2 * k + 1] : b;
         for (int i = 0; i < nprev; i++)
-            if (prev_keys[2 * i] == a && prev_keys[2 * i + 1] == b) {
+            if (prev_keys[2 * i] == ka && prev_keys[2 * i + 1] == kb) {
                 prev = ld4(prev_imp + 4 * i);
                 found = i;
                 break;

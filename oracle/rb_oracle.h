@@ -116,6 +116,7 @@ void orc_contact_features(int nc, const orc_contact *contacts, orc_constraint *c
                           const float *pos, const float *linvel, const float *angvel,
                           const float *inv_mass, int nprev, const int *prev_keys,
                           const float *prev_imp, const float *prev_pts, int apply_previous,
+                          const int *contact_keys /* int2 per contact or NULL */,
                           float *features);
 void orc_warm_from_prediction(int nc, orc_constraint *cons, const float *y, float friction);
 void orc_solve(int nc, orc_constraint *cs, const int *order, float *linvel, float *angvel,

@@ -170,12 +170,15 @@ def gpu_tile_orders(w):
     return [order[first:end] for first, end, _, _ in tiles if end > first]
 
 
-def oracle_step(s, order, contacts=None, prev=None, warm=0, prediction=None, segments=None):
+def oracle_step(s, order, contacts=None, prev=None, warm=0, prediction=None, segments=None,
+                pairs=None):
     """Full step on the oracle from the snapshot state, visiting constraints in
     `order`.  With `segments` (a list of orders, one per independent tile) each
     segment is solved on its own, stopping at the residual tolerance of its
     own rows.  prev: (keys int2[m], totals float4[m], points float4[m][4]) of
-    the previous solve;
+    the previous solve, keyed by pair entry; pairs: the step's pair list
+    (body pairs, then compound child pairs) that `contacts` index, which
+    gives each contact its key;
     warm: 0 zero, 1 previous impulses, 2 impulses from `prediction` (n, 3).
     Returns pos, quat, linvel, angvel, the solved constraints and the
     warm-start features."""
@@ -188,8 +191,11 @@ def oracle_step(s, order, contacts=None, prev=None, warm=0, prediction=None, seg
                                    c.solver_dt if c.solver_dt > 0 else c.dt, c.position_drift,
                                    c.baumgarte, c.friction)
     keys, imp, pts = prev if prev is not None else (None, None, None)
+    ckeys = None
+    if pairs is not None:
+        ckeys = np.asarray(pairs, np.int32)[np.asarray(contacts["pair_index"], np.int64)]
     feats, cons = O.contact_features(contacts, cons, st["pos"], lv, av, sh["inv_mass"], keys, imp,
-                                     pts, apply_previous=(warm != 0))
+                                     pts, apply_previous=(warm != 0), contact_keys=ckeys)
     if warm == 2:
         cons = O.warm_from_prediction(cons, prediction, c.friction)
     if segments is None:

@@ -183,3 +183,26 @@ def test_obj_ground_mesh():
     assert np.allclose(st["pos"][1:, 1], 0.5, atol=0.05), st["pos"][1:, 1]
     assert np.abs(st["linvel"][1:, :3]).max() < 0.05
     w.close()
+
+
+def test_mesh_against_a_sphere_is_reported():
+    """A sphere (or a compound) meeting a concave mesh gets no contacts, as
+    documented, and the step says so with MI_RB_OVERFLOW_UNSUPPORTED_PAIR;
+    hull bodies on the same mesh do not raise it."""
+    for with_sphere in (False, True):
+        w = World(reference_config(max_bodies=64, max_shapes=512, max_pairs=1 << 12,
+                                   max_contacts=1 << 12, max_pairs_per_body=64))
+        cube = w.add_hull(scenes.cube_points(0.5))
+        verts, tris = terrain()
+        mesh = w.add_concave_mesh(verts, tris)
+        w.add_body(0.0, (0, 0, 0), (0, 0, 0, 1), mesh)
+        w.add_body(1.0, (0, 0.8, 0), (0, 0, 0, 1), cube)
+        if with_sphere:
+            ball = w.add_sphere(0.5)
+            w.add_body(1.0, (3, 0.4, 0), (0, 0, 0, 1), ball)
+        w.upload()
+        w.step(5)
+        flags = w.stats()["overflow_flags"]
+        assert bool(flags & L.OVERFLOW_UNSUPPORTED_PAIR) == with_sphere, flags
+        assert flags & ~L.OVERFLOW_UNSUPPORTED_PAIR == 0
+        w.close()

@@ -70,3 +70,27 @@ def test_saved_world_continues_identically(tmp_path):
     assert a.stats()["num_contacts"] == b.stats()["num_contacts"] > 0
     a.close()
     b.close()
+
+
+def test_corrupt_snapshot_leaves_the_world_as_it_was(tmp_path):
+    a = mixed_world()
+    a.step(10)
+    good = tmp_path / "world.mirb"
+    a.save(str(good))
+    raw = good.read_bytes()
+    n = a.body_count()
+    before = a.get_state()
+    # a truncated file: refused, nothing replaced
+    bad = tmp_path / "cut.mirb"
+    bad.write_bytes(raw[:len(raw) // 2])
+    with pytest.raises(L.MiRbError) as e:
+        a.load(str(bad))
+    assert e.value.status == L.INVALID_ARG
+    assert a.body_count() == n
+    after = a.get_state()
+    for k in before:
+        assert np.array_equal(before[k], after[k]), k
+    a.step(10)
+    s = a.get_state()
+    assert s["pos"].shape[0] == n and np.all(np.isfinite(s["pos"]))
+    a.close()

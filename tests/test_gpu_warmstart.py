@@ -158,7 +158,8 @@ def test_warm_step_matches_oracle(warm, kind):
     w.set_state(**{k: s.state[k] for k in ("pos", "quat", "linvel", "angvel")})
     w.step(1)
     g = w.get_state()
-    ref = oracle_step(s, order, contacts, prev=(keys, imp, pts), warm=warm, prediction=pred)
+    ref = oracle_step(s, order, contacts, prev=(keys, imp, pts), warm=warm, prediction=pred,
+                      pairs=w.pairs())
     assert np.allclose(feats, ref["features"], rtol=1e-5, atol=1e-6)
     assert np.mean(ref["features"][:, 13]) > 0.5  # most pairs were in contact last step
     if warm == L.WARM_LEARNED:
@@ -219,7 +220,8 @@ def test_scene_tolerance_matches_oracle(warm):
     w.step(1)
     g = w.get_state()
     st = w.stats()
-    ref = oracle_step(s, None, contacts, prev=(keys, imp, pts), warm=warm, segments=segs)
+    ref = oracle_step(s, None, contacts, prev=(keys, imp, pts), warm=warm, segments=segs,
+                      pairs=w.pairs())
     for k in ("linvel", "angvel", "pos", "quat"):
         c = 3 if k != "quat" else 4
         assert np.allclose(g[k][:, :c], ref[k][:, :c], rtol=1e-4, atol=1e-4), k
@@ -228,4 +230,49 @@ def test_scene_tolerance_matches_oracle(warm):
     assert st["friction_iterations"] == its[:, 1].max()
     assert abs(st["mean_normal_iterations"] - its[:, 0].mean()) < 1e-3
     assert its[:, 0].min() < its[:, 0].max()  # scenes really stop at different sweeps
+    w.close()
+
+
+@pytest.mark.parametrize("kind", ["compound", "mesh", "spheres"])
+def test_previous_warm_start_with_child_pairs_matches_oracle(kind):
+    """Worlds whose contacts come from compound child pairs, mesh triangles or
+    sphere/plane pairs: the impulse cache is keyed by each manifold's pair
+    entry (unique, sorted for the binary search), so every manifold finds its
+    own previous impulses; the features (with the found flag) and one step
+    with the previous-frame warm start equal the oracle's."""
+    if kind == "compound":
+        from test_gpu_compound import compound_pile
+        w = compound_pile()
+    elif kind == "mesh":
+        from test_gpu_mesh import mesh_world
+        w = mesh_world()[0]
+    else:
+        from test_gpu_spheres import sphere_box
+        w = sphere_box()[0]
+    w.set_solver(8, 0.0, L.WARM_PREVIOUS)
+    w.step(45)
+    s = Snapshot(w)
+    keys = w.read(L.BUF_PREV_KEYS, np.int32, (2,))
+    imp = w.read(L.BUF_PREV_IMPULSES, np.float32, (4,))
+    pts = w.read(L.BUF_PREV_POINTS, np.float32, (4, 4))
+    assert len(keys) > 0
+    kk = keys[:, 0].astype(np.int64) * (1 << 32) + keys[:, 1]
+    assert np.all(np.diff(kk) > 0)  # unique and sorted
+    w.step_partial(L.STAGE_COLOR)
+    order = gpu_order(w)
+    contacts = w.contacts(O.CONTACT_DTYPE)
+    pairs = w.pairs()
+    feats = w.read(L.BUF_FEATURES, np.float32, (16,))
+    w.set_state(**{k: s.state[k] for k in ("pos", "quat", "linvel", "angvel")})
+    w.step(1)
+    g = w.get_state()
+    ref = oracle_step(s, order, contacts, prev=(keys, imp, pts), warm=L.WARM_PREVIOUS,
+                      pairs=pairs)
+    assert np.array_equal(feats[:, 13], ref["features"][:, 13])
+    assert np.mean(feats[:, 13]) > 0.5
+    assert np.allclose(feats, ref["features"], rtol=1e-5, atol=1e-6)
+    for k in ("linvel", "angvel", "pos"):
+        assert np.allclose(g[k][:, :3], ref[k][:, :3], rtol=1e-4, atol=1e-4), k
+    gl = w.read(L.BUF_CONSTRAINTS, np.uint8, (176,)).view(O.CONSTRAINT_DTYPE).reshape(-1)
+    assert np.allclose(gl["lambda"], ref["cons"]["lambda"], rtol=1e-4, atol=1e-5)
     w.close()
