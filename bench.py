@@ -2,18 +2,23 @@
 
 Default workload: 65,536 convex hulls (a third unit cubes, a third random
 8-vertex and a third random 32-vertex hulls) falling onto a static ground,
-dt 1/60, 10 solver iterations, FP32 -- body-steps/s of the whole step
-(broadphase, narrowphase, solver, integration) with all state resident in
-HBM, the step replayed from one captured hipGraph.
+dt 1/60, 10 solver iterations, FP32, 500 steps -- body-steps/s of the whole
+step (broadphase, narrowphase, solver, integration) with all state resident
+in HBM, the step replayed from one captured hipGraph.  The scene is first
+advanced untimed into its contact-rich regime (step 200 of its 500, when
+most hulls have landed); the timed window follows the warmup steps after
+that, and the average over the whole 500-step run is reported beside it.
 
 Prints one JSON line.  Options:
   --gpus N        only 1 is supported (one GPU)
   --steps K       timed steps (default 200)
-  --warmup W      untimed steps before timing (default 20)
+  --warmup W      untimed steps before timing, after the settling steps (default 20)
+  --settle S      untimed steps before the warmup (default: the workload's)
   --workload      hulls64k (default) | lattice4k | batched4k
   --cpu-seconds S budget of the CPU-oracle baseline sample (default 15, 0 = skip)
   --per-scene-baseline  (batched4k) also time the same scenes stepped one
                   world and one launch set (captured graph) per scene
+  --no-extras     only the workload's own figures (profiling runs)
 """
 import argparse
 import json
@@ -28,21 +33,34 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
-# HBM bytes per k_tile_solve dispatch, 2.0 x FETCH_SIZE + 1.0 x WRITE_SIZE (KiB)
-# from rocprofv3 --pmc runs over the stage-timing steps of the default command
-# (copy-kernel calibration and commands in profiles/hbm_traffic_hulls64k.md and
-# profiles/hbm_traffic_batched4k.md).  Spatial tiles: the mean of the normal-
-# and the friction-sweep dispatches (40 of each per step).
-SOLVE_TRAFFIC_BYTES = {"hulls64k": ((5371 + 5006) * 2.0 + 1478 + 1769) / 2 * 1024,
-                       "batched4k": (1165580 * 2.0 + 233047) * 1024}
+# steps each workload is advanced untimed before the warmup: the 65,536
+# hulls fall from up to 122 m (most have landed by step 200 of 500), the
+# lattice lands within 40 steps, the piles settle within 120
+SETTLE = {"hulls64k": 200, "lattice4k": 40, "batched4k": 120}
 
 # algorithmic bytes per unit for each stage: what each stage must read and
 # write once per body, pair, contact or sort pass (stage_bytes below)
 STAGES = ["setup", "sort", "pairs", "narrowphase", "constraints", "colouring", "solve",
           "integrate"]
 
+# k_tile_solve, per row and sweep: normal sweeps on precomputed rows read
+# 368 B and write the 16 B of impulses; friction sweeps read the full 312 B
+# row and write 16 B; scene tiles run every sweep in one launch and write
+# both impulse vectors once (328 B).  Per tile body and launch: velocities
+# read and written (4 float4 = 64 B).
+ROW_NORMAL, ROW_FRICTION, ROW_SCENE, TILE_BODY = 384, 328, 328, 64
 
-def stage_bytes(stage, bodies, pairs, contacts, passes, sweeps):
+
+def solve_bytes(contacts, tile_bodies, sweeps, scene_tiles):
+    """Algorithmic bytes of the solver kernel per step."""
+    if scene_tiles:
+        return ROW_SCENE * contacts + TILE_BODY * tile_bodies
+    half = sweeps // 2
+    return half * (ROW_NORMAL * contacts + TILE_BODY * tile_bodies) + \
+        half * (ROW_FRICTION * contacts + TILE_BODY * tile_bodies)
+
+
+def stage_bytes(stage, bodies, pairs, contacts, passes, sweeps, tile_bodies, scene_tiles):
     if stage == "setup":      # read pos,quat,linvel,invI_local,mass,shape,scene,large; write aabb,kv,linvel,invI
         return 184 * bodies
     if stage == "sort":       # histogram read + P passes of read+write (key,value)
@@ -55,19 +73,24 @@ def stage_bytes(stage, bodies, pairs, contacts, passes, sweeps):
         return 8 * pairs + (96 + 96 + 176) * contacts
     if stage == "colouring":  # tile key sort + one pass over rows
         return (3 * 2 + 1) * 8 * contacts + 24 * contacts
-    if stage == "solve":      # per launch: read each row once and write its impulses, read
-        # and write each body's velocities (64 B).  Scene tiles run every sweep
-        # in one launch (rows: 312 B read, 16 B normal + 16 B friction impulses
-        # written); spatial tiles launch once per sweep and parity phase, the
-        # normal sweeps on precomputed rows (368 B read, 16 B written), the
-        # friction sweeps on the full rows (312 B read, 16 B written).
-        if sweeps == 1:
-            return 328 * contacts + 64 * bodies
-        half = sweeps // 2
-        return (384 * contacts + 64 * bodies) * half + (328 * contacts + 64 * bodies) * half
+    if stage == "solve":
+        return solve_bytes(contacts, tile_bodies, sweeps, scene_tiles)
     if stage == "integrate":  # read+write pos, quat, linvel, angvel; read inverse mass
         return 108 * bodies
     return 0
+
+
+def traffic_profile(workload):
+    """HBM bytes per k_tile_solve dispatch measured by rocprofv3 --pmc
+    (FETCH_SIZE, WRITE_SIZE, scaled by a copy kernel's readings) at the
+    state the default command times, from profiles/hbm_traffic_<workload>.json
+    (written with the counter runs; the .md beside it gives the commands)."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_%s.json" % workload)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d
 
 
 def build_scene(name):
@@ -270,23 +293,118 @@ def mesh_terrain(quads=100, steps=50, warmup=60):
             "contacts": st["num_contacts"], "overflow_flags": st["overflow_flags"]}
 
 
+def batched_leg(steps=20, warmup=5, sample_every=16):
+    """The 4,096 batched scenes (about 400k bodies): body-steps/s of the
+    batched graph-replayed step after the piles settle, and the same scenes
+    stepped one world and one captured graph per scene on a sample of every
+    `sample_every`-th scene (the per-scene rate of the sample stands for all
+    scenes)."""
+    from experiments_amd import scenes
+    scene = build_scene("batched4k")
+    n = scene.num_bodies
+    w = scenes.build_world(scene)
+    w.use_graph(True)
+    w.step(SETTLE["batched4k"] + warmup)
+    w.step(steps)
+    ms = w.last_step_ms()
+    st = w.stats()
+    value = n * steps / (ms / 1e3)
+    w.close()
+    subs = scenes.split_scenes(scene)[::sample_every]
+    worlds, bodies = [], 0
+    for sub in subs:
+        bodies += sub.num_bodies
+        ws = scenes.build_world(sub, max_pairs=max(4096, 16 * sub.num_bodies),
+                                max_contacts=max(2048, 8 * sub.num_bodies),
+                                max_hull_vertices=1 << 16)
+        ws.use_graph(True)
+        ws.step(SETTLE["batched4k"] + warmup)
+        worlds.append(ws)
+    t0 = time.perf_counter()
+    for ws in worlds:
+        ws.step(steps)
+    host_s = time.perf_counter() - t0
+    for ws in worlds:
+        ws.close()
+    per_scene = bodies * steps / host_s
+    return {"bodies": n, "scenes": 4096, "steps": steps,
+            "timed_steps": [SETTLE["batched4k"] + warmup + 1,
+                            SETTLE["batched4k"] + warmup + steps],
+            "ms_per_step": round(ms / steps, 4), "value": round(value, 1),
+            "contacts": st["num_contacts"],
+            "per_scene_launch_sets": {"value": round(per_scene, 1), "scenes": len(subs),
+                                      "bodies": bodies, "sample": "every %dth scene"
+                                      % sample_every, "host_s": round(host_s, 4)},
+            "batched_over_per_scene": round(value / per_scene, 2),
+            "overflow_flags": st["overflow_flags"]}
+
+
+def warm_start_leg(n_scenes=1024, settle=120, steps=20, tol=1e-4, max_sweeps=200):
+    """Solver sweeps to a 1e-4 residual for the zero, previous-step and
+    learned warm starts on held-out batched piles (another seed than the
+    training scenes), each run from the same settled state; the trained
+    weights come from experiments_amd/warmstart_weights.npz."""
+    from experiments_amd import _lib as L
+    from experiments_amd import scenes
+    from experiments_amd.warmstart import load_weights, run_iterations
+    weights = load_weights()
+    scene = scenes.batched_piles(n_scenes=n_scenes, seed=scenes.SEED + 1, iterations=max_sweeps)
+    out = {"scenes": n_scenes, "settle": settle, "steps": steps, "tol": tol,
+           "max_sweeps": max_sweeps}
+    starts = [("zero", L.WARM_ZERO), ("previous", L.WARM_PREVIOUS)]
+    if weights is not None:
+        starts.append(("learned", L.WARM_LEARNED))
+    for name, mode in starts:
+        w = scenes.build_world(scene, solver_iterations=10)
+        w.use_graph(True)
+        w.step(settle)
+        if mode == L.WARM_LEARNED:
+            w.set_warmstart_weights(*weights)
+        w.set_solver(max_sweeps, tol, mode)
+        w.use_graph(True)
+        r = run_iterations(w, steps)
+        a = r.pop("per_step")
+        out[name] = {"normal_sweeps": round(r["mean_iters_normal"], 2),
+                     "friction_sweeps": round(r["mean_iters_friction"], 2),
+                     "sweeps": round(r["mean_iters_normal"] + r["mean_iters_friction"], 2),
+                     "max_sweeps_used": max(r["max_iters_normal"], r["max_iters_friction"]),
+                     "final_residual_max": max(r["max_residual_normal"],
+                                               r["max_residual_friction"]),
+                     "unconverged_steps": int(np.sum((a[:, 4] >= tol) | (a[:, 5] >= tol)))}
+        w.close()
+    for name in ("previous", "learned"):
+        if name in out:
+            out[name]["fewer_sweeps_than_zero"] = round(out["zero"]["sweeps"] /
+                                                        out[name]["sweeps"], 3)
+    if "learned" in out:
+        out["learned"]["fewer_sweeps_than_previous"] = round(out["previous"]["sweeps"] /
+                                                             out["learned"]["sweeps"], 3)
+    else:
+        out["learned"] = "no trained weights (experiments_amd/warmstart_weights.npz)"
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--settle", type=int, default=-1)
     ap.add_argument("--workload", default="hulls64k")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--per-scene-baseline", action="store_true")
     ap.add_argument("--per-scene-max", type=int, default=0)
-    ap.add_argument("--no-broad1m", action="store_true",
-                    help="skip the 1M-body broadphase stress figures")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="only the workload's own figures (for profiler runs)")
+    ap.add_argument("--profile-only", action="store_true",
+                    help="stop after the timed window and the eager stage-timing steps (the "
+                         "kernel-trace and counter runs use this)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes instead of timing the candidates first")
     ap.add_argument("--eager", action="store_true",
                     help="launch the timed steps stage by stage instead of replaying the "
-                         "captured graph (for kernel tracers that do not follow graph launches)")
+                         "captured graph (the counter runs use this)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB 5 times (the HBM "
                          "counter calibration for rocprofv3 --pmc) and exit")
@@ -317,65 +435,97 @@ def main():
     from experiments_amd import scenes
     scene = build_scene(args.workload)
     n = scene.num_bodies
+    settle = SETTLE[args.workload] if args.settle < 0 else args.settle
     w = scenes.build_world(scene)
     w.use_graph(not args.eager)
+    run_ms = 0.0
+    if settle:
+        w.step(settle)
+        run_ms += w.last_step_ms()
     if args.warmup:
         w.step(args.warmup)
+        run_ms += w.last_step_ms()
     tuning, tune_ms = w.get_tuning(), None
-    if not args.no_autotune:
-        # launch shapes timed on a state from the middle of the timed window
-        # (where the contacts are), then back to the post-warmup state
+    if not args.no_autotune and not args.eager:
+        # launch shapes timed on a state from the middle of the timed window,
+        # then back to the post-warmup state
         snap = w.get_state()
         w.step(max(args.steps // 2, 1))
         tuning, tune_ms = w.autotune(steps=5)
         w.set_state(**snap)
-        w.use_graph(not args.eager)
+        w.use_graph(True)
     t0 = time.perf_counter()
     w.step(args.steps)           # synchronises on completion
     host_s = time.perf_counter() - t0
     gpu_ms = w.last_step_ms()
+    run_ms += gpu_ms
+    done = settle + args.warmup + args.steps
     st = w.stats()
     value = n * args.steps / (gpu_ms / 1e3)
-    # per-stage breakdown: a few eager steps with HIP events between stages
+    # per-stage breakdown and the solver kernel's own launch times: a few
+    # eager steps with HIP events from the end state of the timed window,
+    # which is then restored
+    after = w.get_state()
     w.use_graph(False)
     acc = np.zeros(8)
-    counts = dict(pairs=0, contacts=0)
+    counts = dict(pairs=0, contacts=0, tile_bodies=0, solve_ms=0.0, launches=0)
     for _ in range(args.profile_steps):
         w.step(1)
         s2 = w.stats()
         acc += np.array(s2["stage_ms"])
         counts["pairs"] += s2["num_pairs"]
         counts["contacts"] += s2["num_contacts"]
+        counts["tile_bodies"] += s2["solve_bodies"]
+        counts["solve_ms"] += s2["solve_kernel_ms"]
+        counts["launches"] += s2["solve_launches"]
     k = max(args.profile_steps, 1)
     stage_ms = acc / k
     pairs, contacts = counts["pairs"] / k, counts["contacts"] / k
+    tile_bodies = counts["tile_bodies"] / k
+    w.set_state(**after)
+    w.use_graph(not args.eager)
     prm = w.params()
     passes = (prm["key_bits"] + 7) // 8
     sweeps = 2 * scene.iterations
-    # the dominant kernel is the tile solve (spatial tiles: one launch per
-    # sweep and parity phase; scene tiles: one launch per step)
     scene_tiles = prm["tile_mode"] == 1
-    launches = 1 if scene_tiles else 4 * sweeps  # (the zero warm start has no warm pass)
-    solve_ms = float(stage_ms[STAGES.index("solve")])
-    per_launch_bytes = stage_bytes("solve", n, pairs, contacts, passes,
-                                   1 if scene_tiles else sweeps) / launches
-    launch_s = solve_ms / 1e3 / launches
+    # the dominant kernel is the tile solve (spatial tiles: one launch per
+    # sweep and parity phase; scene tiles: one launch per step); its launch
+    # duration is the mean of the HIP-event spans around each of its launches
+    launches = max(counts["launches"] / k, 1)
+    launch_s = counts["solve_ms"] / max(counts["launches"], 1) / 1e3
+    per_launch_bytes = solve_bytes(contacts, tile_bodies, sweeps, scene_tiles) / launches
     achieved = per_launch_bytes / launch_s / 1e9 if launch_s > 0 else 0.0
-    traffic = SOLVE_TRAFFIC_BYTES.get(args.workload)
+    prof = traffic_profile(args.workload)
+    traffic = None
+    if prof and prof.get("state_steps") == [settle + args.warmup + 1, done]:
+        traffic = prof.get("k_tile_solve_bytes_per_dispatch")
     roofline = {"bound": "hbm", "kernel": "k_tile_solve", "achieved": round(achieved, 2),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                 "traffic": round(traffic) if traffic else None,
+                "traffic_source": (prof or {}).get("source") if traffic else
+                ("no counter profile at this state" if prof is None else
+                 "counter profile taken at steps %s, not this window" % prof.get("state_steps")),
                 "algorithmic_bytes_per_launch": round(per_launch_bytes),
-                "launches_per_step": launches, "launch_us": round(launch_s * 1e6, 2)}
+                "launches_per_step": round(launches, 2), "launch_us": round(launch_s * 1e6, 2),
+                "rows": round(contacts), "tile_bodies": round(tile_bodies)}
     stage_hbm = {}
     for i, name in enumerate(STAGES):
-        sb = stage_bytes(name, n, pairs, contacts, passes,
-                         1 if (name == "solve" and scene_tiles) else sweeps)
+        sb = stage_bytes(name, n, pairs, contacts, passes, sweeps, tile_bodies, scene_tiles)
         if stage_ms[i] > 0:
             stage_hbm[name] = round(sb / (stage_ms[i] / 1e3) / 1e9, 1)
     breakdown = {STAGES[i]: round(float(stage_ms[i]), 4) for i in range(8)}
+    # the rest of the configured run, for the whole-run average
+    whole = None
+    if done < scene.steps and not args.profile_only:
+        w.step(scene.steps - done)
+        run_ms += w.last_step_ms()
+        whole = {"steps": scene.steps, "ms_per_step": round(run_ms / scene.steps, 4),
+                 "value": round(n * scene.steps / (run_ms / 1e3), 1),
+                 "note": "every step of the configured run from the initial state, graph "
+                         "replays, HIP-event time (the autotuner's trials excluded)"}
     cpu = None
-    if args.cpu_seconds > 0:
+    if args.cpu_seconds > 0 and not args.profile_only:
+        w.set_state(**after)
         cpu = cpu_baseline(w, scene, args.cpu_seconds)
     per_scene = None
     if args.per_scene_baseline and args.workload == "batched4k":
@@ -391,9 +541,11 @@ def main():
         "data": "synthetic: seeded PCG32 scene (random hull shapes, positions, orientations, "
                 "velocities, masses)",
         "config": {"workload": args.workload, "bodies": n, "solver_iterations": scene.iterations,
-                   "dt": scene.dt, "graph": not args.eager, "pairs": int(pairs),
-                   "contacts": int(contacts), "key_bits": prm["key_bits"],
+                   "dt": scene.dt, "graph": not args.eager, "settle_steps": settle,
+                   "timed_steps": [settle + args.warmup + 1, done],
+                   "pairs": int(pairs), "contacts": int(contacts), "key_bits": prm["key_bits"],
                    "sort_passes": passes, "host_s": round(host_s, 4), "tuning": tuning},
+        "whole_run": whole,
         "autotune_ms_per_step": tune_ms,
         "stage_ms": breakdown,
         "stage_hbm_GBs": stage_hbm,
@@ -403,13 +555,15 @@ def main():
     }
     if per_scene is not None:
         out["per_scene_launch_sets"] = per_scene
-    if not args.no_broad1m and args.workload == "hulls64k":
+    if not args.no_extras and args.workload == "hulls64k":
         if w is not None:
             out["sap_broadphase"] = sap_on_world(w)
             w.close()
             w = None
         out["broadphase_1m"] = broadphase_1m()
         out["mesh_terrain"] = mesh_terrain()
+        out["batched4k"] = batched_leg()
+        out["warm_start_iterations"] = warm_start_leg()
     if w is not None:
         w.close()
     print(json.dumps(out))

@@ -14,6 +14,8 @@ STATUS_NAMES = {0: "OK", 1: "INVALID_ARG", 2: "CAPACITY", 3: "DEVICE", 4: "UNSUP
                 5: "STATE", 6: "FORMAT"}
 
 WARM_ZERO, WARM_PREVIOUS, WARM_LEARNED = 0, 1, 2
+SAT_GPU, SAT_CPU_TWIN = 0, 1
+REDUCE_SEQUENTIAL, REDUCE_SPLIT_PATH = 0, 1
 INERTIA_REFERENCE, INERTIA_BOX = 0, 1
 OVERFLOW_PAIRS, OVERFLOW_BODY_PAIRS, OVERFLOW_CONTACTS = 1, 2, 4
 OVERFLOW_COLORS, OVERFLOW_TILE, OVERFLOW_POLYGON, OVERFLOW_LEVEL = 8, 16, 32, 64
@@ -44,7 +46,8 @@ class Config(ctypes.Structure):
         ("max_angular_motion", ctypes.c_float), ("cell_size", ctypes.c_float),
         ("grid_dims", ctypes.c_int * 3), ("residual_tolerance", ctypes.c_float),
         ("warm_start", ctypes.c_int), ("inertia_mode", ctypes.c_int),
-        ("tile_size", ctypes.c_float), ("device", ctypes.c_int)]
+        ("tile_size", ctypes.c_float), ("device", ctypes.c_int), ("sat_mode", ctypes.c_int),
+        ("reduction_mode", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -54,7 +57,9 @@ class Stats(ctypes.Structure):
         ("normal_iterations", ctypes.c_int), ("friction_iterations", ctypes.c_int),
         ("normal_residual", ctypes.c_float), ("friction_residual", ctypes.c_float),
         ("stage_ms", ctypes.c_float * 8), ("steps", ctypes.c_longlong),
-        ("mean_normal_iterations", ctypes.c_float), ("mean_friction_iterations", ctypes.c_float)]
+        ("mean_normal_iterations", ctypes.c_float), ("mean_friction_iterations", ctypes.c_float),
+        ("solve_bodies", ctypes.c_int), ("solve_kernel_ms", ctypes.c_float),
+        ("solve_launches", ctypes.c_int)]
 
 
 class Params(ctypes.Structure):

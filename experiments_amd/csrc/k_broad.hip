@@ -851,20 +851,20 @@ static GridP grid_of(const StepParams &p) {
 
 hipError_t launch_setup(const DeviceWorld &d, const StepParams &p, hipStream_t s) {
     if (p.n_bodies == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_setup, dim3((p.n_bodies + 255) / 256), dim3(256), 0, s, p.n_bodies,
+    RB_LAUNCH(k_setup, dim3((p.n_bodies + 255) / 256), dim3(256), 0, s, p.n_bodies,
                        d.pos, d.quat, d.linvel, d.angvel, d.inv_mass, d.shape, d.scene, d.is_large,
                        d.hulls, d.inv_inertia_local, d.inv_inertia_world, d.aabb_min, d.aabb_max,
                        d.kv, grid_of(p), p.gravity[0], p.gravity[1], p.gravity[2], p.dt,
                        d.occ, d.occ_words);
-    return hipGetLastError();
+    return rb_launch_status();
 }
 
 hipError_t launch_cell_start(const DeviceWorld &d, const StepParams &p, hipStream_t s) {
     const int n = p.n_bodies;
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_cell_start, dim3((n + 255) / 256), dim3(256), 0, s, d.kv, n, d.aabb_min,
+    RB_LAUNCH(k_cell_start, dim3((n + 255) / 256), dim3(256), 0, s, d.kv, n, d.aabb_min,
                        d.aabb_max, d.cell_start, d.occ, d.smn, d.smx);
-    return hipGetLastError();
+    return rb_launch_status();
 }
 
 hipError_t launch_find_pairs(const DeviceWorld &d, const StepParams &p, hipStream_t s) {
@@ -877,35 +877,35 @@ hipError_t launch_find_pairs(const DeviceWorld &d, const StepParams &p, hipStrea
     const int entries = FP_BLOCK / lanes;
     size_t lds = sizeof(int) * (size_t)entries * (K | 1);
     auto kern = lanes == 32 ? k_find_pairs<32> : k_find_pairs<1>;
-    hipLaunchKernelGGL(kern, dim3((n + entries - 1) / entries), dim3(FP_BLOCK), lds, s, n,
+    RB_LAUNCH(kern, dim3((n + entries - 1) / entries), dim3(FP_BLOCK), lds, s, n,
                        d.kv, d.cell_start, d.occ, d.smn, d.smx, d.aabb_min, d.aabb_max,
                        d.large_list,
                        p.n_large ? d.large_first : nullptr, p.n_large ? d.large_num : nullptr,
                        grid_of(p), K, d.body_slots, d.pair_count, d.counters + RB_CNT_OVERFLOW);
     if (p.n_large > 0) {
         dim3 g(p.n_large, p.large_chunks);
-        hipLaunchKernelGGL(k_large_count, g, dim3(256), 0, s, d.large_list, d.scene,
+        RB_LAUNCH(k_large_count, g, dim3(256), 0, s, d.large_list, d.scene,
                            d.scene_start, d.aabb_min, d.aabb_max, p.large_chunks, d.large_chunk_cnt);
-        hipLaunchKernelGGL(k_large_write, g, dim3(256), 0, s, d.large_list, d.large_off, d.scene,
+        RB_LAUNCH(k_large_write, g, dim3(256), 0, s, d.large_list, d.large_off, d.scene,
                            d.scene_start, d.aabb_min, d.aabb_max, p.large_chunks,
                            d.large_chunk_cnt, d.large_slots, d.pair_count);
     }
-    return hipGetLastError();
+    return rb_launch_status();
 }
 
 hipError_t launch_emit_pairs(const DeviceWorld &d, const StepParams &p, hipStream_t s) {
     int n = p.n_bodies;
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_emit_pairs, dim3((n + 255) / 256), dim3(256), 0, s, n, d.pair_count,
+    RB_LAUNCH(k_emit_pairs, dim3((n + 255) / 256), dim3(256), 0, s, n, d.pair_count,
                        d.pair_off, d.body_slots, p.max_pairs_per_body, d.is_large, d.large_index,
                        d.large_off, d.large_slots, p.max_pairs, d.pairs,
                        (const uint32_t *)(d.counters + RB_CNT_SORT_TILE),
                        d.counters + RB_CNT_PAIRS, d.counters + RB_CNT_OVERFLOW);
     if (p.n_large > 0)
-        hipLaunchKernelGGL(k_emit_large, dim3(p.n_large, 32), dim3(256), 0, s, d.large_list,
+        RB_LAUNCH(k_emit_large, dim3(p.n_large, 32), dim3(256), 0, s, d.large_list,
                            d.pair_count, d.pair_off, d.large_off, d.large_slots, p.max_pairs,
                            d.pairs);
-    return hipGetLastError();
+    return rb_launch_status();
 }
 
 // Each body's pair range ordered by y: ranges of up to GROUP_SMALL pairs in
@@ -919,14 +919,14 @@ static void launch_pairs_group(int n, uint32_t *counts, uint32_t *offsets, uint3
                                hipStream_t s) {
     const size_t bm_bytes = sizeof(uint32_t) * (((size_t)n + 31) / 32 + 1);
     const bool huge = bm_bytes <= HUGE_LDS_MAX;
-    hipLaunchKernelGGL(k_pairs_group, dim3(blocks), dim3(256), 0, s, n, counts, offsets, total,
+    RB_LAUNCH(k_pairs_group, dim3(blocks), dim3(256), 0, s, n, counts, offsets, total,
                        max_pairs, ys, pairs, num_pairs, overflow, big, big_n,
                        huge ? HUGE_MIN : INT_MAX);
     if (n > 0) {
-        hipLaunchKernelGGL(k_pairs_group_big, dim3(1024), dim3(256), 0, s, counts, offsets,
+        RB_LAUNCH(k_pairs_group_big, dim3(1024), dim3(256), 0, s, counts, offsets,
                            max_pairs, ys, pairs, big, big_n);
         if (huge)
-            hipLaunchKernelGGL(k_pairs_group_huge, dim3(64), dim3(HUGE_BLOCK), bm_bytes, s, n,
+            RB_LAUNCH(k_pairs_group_huge, dim3(64), dim3(HUGE_BLOCK), bm_bytes, s, n,
                                counts, offsets, max_pairs, ys, pairs, big, big_n);
     }
 }
@@ -949,7 +949,7 @@ hipError_t launch_hgrid_pairs(const float4 *mn, const float4 *mx, int n, float i
     const int hg_blocks = (int)(((int64_t)n * HG_LANES + 255) / 256);
     uint32_t *big_n = big + (n > 0 ? n : 0);
     if (ev) hipEventRecord(ev[0], s);
-    hipLaunchKernelGGL(k_hg_hash, dim3(blocks), dim3(256), 0, s, mn, mx, n, g, kv, counts, cursor,
+    RB_LAUNCH(k_hg_hash, dim3(blocks), dim3(256), 0, s, mn, mx, n, g, kv, counts, cursor,
                        big_n, overflow);
     if (ev) hipEventRecord(ev[1], s);
     hipError_t e = launch_radix_sort(kv, n, cell_bits + 2, 8, sw, s);
@@ -961,22 +961,22 @@ hipError_t launch_hgrid_pairs(const float4 *mn, const float4 *mx, int n, float i
     if (n > 0) {
         e = launch_fill_u32(occ, 0u, occ_words + 4, s);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_hg_cells, dim3(blocks), dim3(256), 0, s, kv, n, mn, mx, g, cell_start,
+        RB_LAUNCH(k_hg_cells, dim3(blocks), dim3(256), 0, s, kv, n, mn, mx, g, cell_start,
                            occ, level_used, smn, smx);
-        hipLaunchKernelGGL(k_hg_pairs<0>, dim3(hg_blocks), dim3(256), 0, s, n, kv, cell_start, occ,
+        RB_LAUNCH(k_hg_pairs<0>, dim3(hg_blocks), dim3(256), 0, s, n, kv, cell_start, occ,
                            level_used, smn, smx, g, counts, (const uint32_t *)nullptr, cursor,
                            max_pairs, ys);
     }
     e = launch_exclusive_scan(counts, offsets, n, total, scan_status, s, nullptr, overflow);
     if (e != hipSuccess) return e;
     if (n > 0)
-        hipLaunchKernelGGL(k_hg_pairs<1>, dim3(hg_blocks), dim3(256), 0, s, n, kv, cell_start, occ,
+        RB_LAUNCH(k_hg_pairs<1>, dim3(hg_blocks), dim3(256), 0, s, n, kv, cell_start, occ,
                            level_used, smn, smx, g, counts, offsets, cursor, max_pairs, ys);
     if (ev) hipEventRecord(ev[3], s);
     launch_pairs_group(n, counts, offsets, total, max_pairs, ys, pairs, num_pairs, overflow, big,
                        big_n, blocks, s);
     if (ev) hipEventRecord(ev[4], s);
-    return hipGetLastError();
+    return rb_launch_status();
 }
 
 // Sweep and prune of n AABBs along `axis`: keys, 32-bit sort, scatter, count
@@ -993,30 +993,30 @@ hipError_t launch_sap_pairs(const float4 *mn, const float4 *mx, const int *is_la
     if (ev) hipEventRecord(ev[0], s);
     hipError_t e = launch_fill_u32(lg_n, 0u, 1, s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_sap_keys, dim3(blocks), dim3(256), 0, s, mn, is_large, n, axis, kv, counts,
+    RB_LAUNCH(k_sap_keys, dim3(blocks), dim3(256), 0, s, mn, is_large, n, axis, kv, counts,
                        cursor, big_n, lg, lg_n);
     if (ev) hipEventRecord(ev[1], s);
     e = launch_radix_sort(kv, n, 32, 8, sw, s);
     if (e != hipSuccess) return e;
     if (ev) hipEventRecord(ev[2], s);
     if (n > 0) {
-        hipLaunchKernelGGL(k_sap_scatter, dim3(blocks), dim3(256), 0, s, kv, n, mn, mx, smn, smx);
-        hipLaunchKernelGGL(k_sap_pairs<0>, dim3(blocks), dim3(256), 0, s, n, axis, smn, smx, lg_n,
+        RB_LAUNCH(k_sap_scatter, dim3(blocks), dim3(256), 0, s, kv, n, mn, mx, smn, smx);
+        RB_LAUNCH(k_sap_pairs<0>, dim3(blocks), dim3(256), 0, s, n, axis, smn, smx, lg_n,
                            counts, (const uint32_t *)nullptr, cursor, max_pairs, ys);
-        hipLaunchKernelGGL(k_sap_large<0>, dim3(blocks), dim3(256), 0, s, n, mn, mx, smn, smx, lg,
+        RB_LAUNCH(k_sap_large<0>, dim3(blocks), dim3(256), 0, s, n, mn, mx, smn, smx, lg,
                            lg_n, counts, (const uint32_t *)nullptr, cursor, max_pairs, ys);
     }
     e = launch_exclusive_scan(counts, offsets, n, total, scan_status, s, nullptr, overflow);
     if (e != hipSuccess) return e;
     if (n > 0) {
-        hipLaunchKernelGGL(k_sap_pairs<1>, dim3(blocks), dim3(256), 0, s, n, axis, smn, smx, lg_n,
+        RB_LAUNCH(k_sap_pairs<1>, dim3(blocks), dim3(256), 0, s, n, axis, smn, smx, lg_n,
                            counts, offsets, cursor, max_pairs, ys);
-        hipLaunchKernelGGL(k_sap_large<1>, dim3(blocks), dim3(256), 0, s, n, mn, mx, smn, smx, lg,
+        RB_LAUNCH(k_sap_large<1>, dim3(blocks), dim3(256), 0, s, n, mn, mx, smn, smx, lg,
                            lg_n, counts, offsets, cursor, max_pairs, ys);
     }
     if (ev) hipEventRecord(ev[3], s);
     launch_pairs_group(n, counts, offsets, total, max_pairs, ys, pairs, num_pairs, overflow, big,
                        big_n, blocks, s);
     if (ev) hipEventRecord(ev[4], s);
-    return hipGetLastError();
+    return rb_launch_status();
 }

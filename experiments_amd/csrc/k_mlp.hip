@@ -366,17 +366,17 @@ int forward_grid(int n) {
 }  // namespace
 
 hipError_t launch_mlp_warm(const DeviceWorld &d, const StepParams &p, hipStream_t s) {
-    hipLaunchKernelGGL(k_mlp_forward<true>, dim3(forward_grid(p.max_contacts)), dim3(256), 0, s,
+    RB_LAUNCH(k_mlp_forward<true>, dim3(forward_grid(p.max_contacts)), dim3(256), 0, s,
                        d.features, p.max_contacts, d.counters + RB_CNT_CONTACTS, d.mlp_w, d.mlp_b,
                        d.mlp_out, p.friction, d.cs);
-    return hipGetLastError();
+    return rb_launch_status();
 }
 
 hipError_t launch_mlp_targets(const DeviceWorld &d, int capacity, float *x_out, float *t_out,
                               hipStream_t s) {
-    hipLaunchKernelGGL(k_mlp_targets, dim3((capacity + 255) / 256), dim3(256), 0, s, d.counters,
+    RB_LAUNCH(k_mlp_targets, dim3((capacity + 255) / 256), dim3(256), 0, s, d.counters,
                        d.cs, d.features, capacity, x_out, t_out);
-    return hipGetLastError();
+    return rb_launch_status();
 }
 
 // ---- C ABI: a network handle (weights, Adam state, scratch) ----
@@ -490,9 +490,9 @@ extern "C" mi_rb_status mi_rb_mlp_forward(void *handle, const float *x_dev, int 
     if (!h || n < 0 || (n > 0 && (!x_dev || !y_dev))) return MI_RB_INVALID_ARG;
     if (n == 0) return MI_RB_OK;
     ConstraintSoA none{};
-    hipLaunchKernelGGL(k_mlp_forward<false>, dim3(forward_grid(n)), dim3(256), 0, h->stream, x_dev,
+    RB_LAUNCH(k_mlp_forward<false>, dim3(forward_grid(n)), dim3(256), 0, h->stream, x_dev,
                        n, (const int *)nullptr, h->w, h->b, y_dev, 0.f, none);
-    RB_CHECK(hipGetLastError());
+    RB_CHECK(rb_launch_status());
     RB_CHECK(hipStreamSynchronize(h->stream));
     return MI_RB_OK;
 }
@@ -513,13 +513,13 @@ extern "C" mi_rb_status mi_rb_mlp_gradients(void *handle, const float *x_dev, co
         h->loss_cap = 1;
     }
     int nb = grad_blocks(n);
-    hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(256), 0, h->stream, x_dev, t_dev, n, n, -1, h->w,
+    RB_LAUNCH(k_mlp_grad, dim3(nb), dim3(256), 0, h->stream, x_dev, t_dev, n, n, -1, h->w,
                        h->b, h->part);
     AdamArgs a{0.f, 0.f, 0.f, 0.f, 0};
-    hipLaunchKernelGGL(k_mlp_reduce_adam, dim3((PSTRIDE + 255) / 256), dim3(256), 0, h->stream,
+    RB_LAUNCH(k_mlp_reduce_adam, dim3((PSTRIDE + 255) / 256), dim3(256), 0, h->stream,
                        h->part, nb, n, grad_dev, h->loss, (float *)nullptr, (float *)nullptr,
                        (float *)nullptr, (uint16_t *)nullptr, (float *)nullptr, a);
-    RB_CHECK(hipGetLastError());
+    RB_CHECK(rb_launch_status());
     float l = 0.f;
     RB_CHECK(hipMemcpyAsync(&l, h->loss, sizeof(float), hipMemcpyDeviceToHost, h->stream));
     RB_CHECK(hipStreamSynchronize(h->stream));
@@ -550,13 +550,13 @@ extern "C" mi_rb_status mi_rb_mlp_train(void *handle, const float *x_dev, const 
         AdamArgs a{(float)(lr * std::sqrt(c2) / c1), b1, b2, 1e-8f, i};
         int draw_seed =
             seed < 0 ? -1 : (int)(((uint32_t)seed * 0x9e3779b9u + (uint32_t)h->adam_t) & 0x7fffffffu);
-        hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(256), 0, h->stream, x_dev, t_dev, batch,
+        RB_LAUNCH(k_mlp_grad, dim3(nb), dim3(256), 0, h->stream, x_dev, t_dev, batch,
                            n_samples, draw_seed, h->w, h->b, h->part);
-        hipLaunchKernelGGL(k_mlp_reduce_adam, dim3((PSTRIDE + 255) / 256), dim3(256), 0, h->stream,
+        RB_LAUNCH(k_mlp_reduce_adam, dim3((PSTRIDE + 255) / 256), dim3(256), 0, h->stream,
                            h->part, nb, batch, (float *)nullptr, h->loss, h->master, h->mom, h->vel,
                            h->w, h->b, a);
     }
-    RB_CHECK(hipGetLastError());
+    RB_CHECK(rb_launch_status());
     if (loss_history)
         RB_CHECK(hipMemcpyAsync(loss_history, h->loss, sizeof(float) * (size_t)steps,
                                 hipMemcpyDeviceToHost, h->stream));

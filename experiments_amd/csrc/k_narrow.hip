@@ -507,7 +507,7 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_clip(
     const float4 *__restrict__ planes, const int2 *__restrict__ face_index,
     const int *__restrict__ indices, const float4 *__restrict__ edges, float friction,
     float max_dist, Contact4 *__restrict__ slots, uint32_t *__restrict__ has_contact,
-    const int *__restrict__ owner, int *overflow) {
+    const int *__restrict__ owner, int *overflow, int edge_reject, int reduce_mode) {
     __shared__ float4 sEA[RB_MAX_HULL_EDGES], sEB[RB_MAX_HULL_EDGES];
     __shared__ float4 sPoly[2][RB_MAX_POLY];
     __shared__ float4 sPts[RB_MAX_POLY];
@@ -633,7 +633,10 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_clip(
         __syncthreads();
         wave_argmin(best, best_ax);
         if (face_d < best || (face_d == best && fb0.y < best_ax)) { best = face_d; best_ax = fb0.y; }
-        if (best_ax == 0x7fffffff) {
+        // the CPU twin rejects on an edge axis whose projections are disjoint:
+        // every face axis overlaps here (pass 1 kept the pair), so a negative
+        // minimum is such an edge axis
+        if (best_ax == 0x7fffffff || (edge_reject && best < 0.f)) {
             if (lane == 0) has_contact[p] = 0;
             continue;
         }
@@ -762,7 +765,44 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_clip(
         float4 nrm = neg4(sep);
         int idx[4] = {0, 1, 2, 3};
         int nr = npts;
-        if (npts >= 4) {
+        if (reduce_mode == MI_RB_REDUCE_SPLIT_PATH) {
+            if (npts > 4) {
+                // extractManifoldSequentialGlobal (satClipHullContacts.cl:678-767):
+                // per direction the first point with the smallest projection
+                // below FLT_MIN, the first deepest point
+                float4 c = sPts[0];
+                for (int j = 1; j < npts; j++) c = add4(c, sPts[j]);
+                float fn = (float)npts;
+                c = mk4(c.x / fn, c.y / fn, c.z / fn, c.w / fn);
+                float4 av = sub4(sPts[0], c);
+                float4 u = cross3(nrm, av);
+                float4 v = cross3(nrm, u);
+                u = normalize3(u);
+                v = normalize3(v);
+                float f[4] = {FLT_MAX, FLT_MAX, FLT_MAX, FLT_MAX};
+                float dw = FLT_MAX;
+                int iw = 0x7fffffff;
+                if (lane < npts) {
+                    float4 q = sPts[lane];
+                    float4 r = sub4(q, c);
+                    f[0] = dot3(u, r); f[1] = dot3(neg4(u), r);
+                    f[2] = dot3(v, r); f[3] = dot3(neg4(v), r);
+                    for (int k = 0; k < 4; k++)
+                        if (!(f[k] < FLT_MIN)) f[k] = FLT_MAX;
+                    dw = q.w;
+                    iw = lane;
+                }
+                for (int k = 0; k < 4; k++) {
+                    int ik = f[k] < FLT_MAX ? lane : 0x7fffffff;
+                    float fk = f[k];
+                    wave_argmin(fk, ik);
+                    if (ik != 0x7fffffff) idx[k] = ik;
+                }
+                wave_argmin(dw, iw);
+                if (idx[0] != iw && idx[1] != iw && idx[2] != iw && idx[3] != iw) idx[0] = iw;
+                nr = 4;
+            }
+        } else if (npts >= 4) {
             float4 c = sPts[0];
             for (int j = 1; j < npts; j++) c = add4(c, sPts[j]);
             float fn = (float)npts;
@@ -965,12 +1005,12 @@ __global__ void k_compound_total(int max_pairs, int *num_pairs, const uint32_t *
 hipError_t launch_narrowphase(const DeviceWorld &d, const StepParams &p, hipStream_t s,
                               const SortWork &sw, uint32_t *scan_status) {
     const int grid = rb_num_cus() * 4 * (p.narrow_waves > 0 ? p.narrow_waves : 4);
-    hipLaunchKernelGGL(k_narrow_face, dim3(grid), dim3(NP_BLOCK), 0, s, p.max_pairs,
+    RB_LAUNCH(k_narrow_face, dim3(grid), dim3(NP_BLOCK), 0, s, p.max_pairs,
                        d.counters + RB_CNT_PAIRS, d.pairs, d.pos, d.quat, d.inv_mass, d.shape,
                        d.hulls, d.verts, d.planes, d.has_contact, d.narrow_cand, d.face_best,
                        d.counters + RB_CNT_OVERFLOW);
     if (p.has_vertexless)
-        hipLaunchKernelGGL(k_narrow_special, dim3(rb_num_cus() * 4), dim3(256), 0, s, p.max_pairs,
+        RB_LAUNCH(k_narrow_special, dim3(rb_num_cus() * 4), dim3(256), 0, s, p.max_pairs,
                            d.counters + RB_CNT_PAIRS, d.pairs, d.pos, d.quat, d.inv_mass, d.shape,
                            d.hulls, d.planes, p.friction, d.contact_slots, d.has_contact);
     hipError_t e = launch_exclusive_scan(d.narrow_cand, d.cand_off, p.max_pairs,
@@ -978,26 +1018,27 @@ hipError_t launch_narrowphase(const DeviceWorld &d, const StepParams &p, hipStre
                                          s, d.counters + RB_CNT_PAIRS,
                                          d.counters + RB_CNT_OVERFLOW);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_narrow_keys, dim3((p.max_pairs + 255) / 256), dim3(256), 0, s,
+    RB_LAUNCH(k_narrow_keys, dim3((p.max_pairs + 255) / 256), dim3(256), 0, s,
                        p.max_pairs, d.counters + RB_CNT_PAIRS, d.narrow_cand, d.cand_off,
                        d.face_best, d.cand_kv);
     e = launch_radix_sort(d.cand_kv, p.max_pairs, 16, 8, sw, s, d.counters + RB_CNT_NARROW_CAND);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_narrow_clip, dim3(grid), dim3(NP_BLOCK), 0, s,
+    RB_LAUNCH(k_narrow_clip, dim3(grid), dim3(NP_BLOCK), 0, s,
                        (const uint32_t *)(d.counters + RB_CNT_NARROW_CAND), d.cand_kv, d.face_best,
                        d.pairs, d.pos, d.quat, d.inv_mass, d.shape, d.hulls, d.verts, d.planes,
                        d.face_index, d.indices, d.edges, p.friction, p.threshold, d.contact_slots,
-                       d.has_contact, d.owner, d.counters + RB_CNT_OVERFLOW);
-    return hipGetLastError();
+                       d.has_contact, d.owner, d.counters + RB_CNT_OVERFLOW, p.sat_edge_reject,
+                       p.reduce_mode);
+    return rb_launch_status();
 }
 
 hipError_t launch_compound_pairs(const DeviceWorld &d, const StepParams &p, hipStream_t s,
                                  uint32_t *scan_status) {
-    hipLaunchKernelGGL(k_proxy_transforms, dim3((p.n_proxies + 255) / 256), dim3(256), 0, s,
+    RB_LAUNCH(k_proxy_transforms, dim3((p.n_proxies + 255) / 256), dim3(256), 0, s,
                        p.n_proxies, p.n_bodies, d.proxy, d.children, d.pos, d.quat);
     const int grid = rb_num_cus() * 4;
     const MeshView mv{d.pos, d.quat, d.aabb_min, d.aabb_max, d.verts, d.children};
-    hipLaunchKernelGGL(k_compound_count, dim3(grid), dim3(256), 0, s, p.max_pairs,
+    RB_LAUNCH(k_compound_count, dim3(grid), dim3(256), 0, s, p.max_pairs,
                        d.counters + RB_CNT_PAIRS, d.pairs, d.shape, d.inv_mass, d.hulls, mv,
                        d.narrow_cand, d.counters + RB_CNT_OVERFLOW);
     hipError_t e = launch_exclusive_scan(d.narrow_cand, d.cand_off, p.max_pairs,
@@ -1005,11 +1046,11 @@ hipError_t launch_compound_pairs(const DeviceWorld &d, const StepParams &p, hipS
                                          s, d.counters + RB_CNT_PAIRS,
                                          d.counters + RB_CNT_OVERFLOW);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_compound_write, dim3(grid), dim3(256), 0, s, p.max_pairs,
+    RB_LAUNCH(k_compound_write, dim3(grid), dim3(256), 0, s, p.max_pairs,
                        d.counters + RB_CNT_PAIRS, d.pairs, d.shape, d.hulls, mv, d.proxy_first,
                        p.n_bodies, d.narrow_cand, d.cand_off);
-    hipLaunchKernelGGL(k_compound_total, dim3(1), dim3(1), 0, s, p.max_pairs,
+    RB_LAUNCH(k_compound_total, dim3(1), dim3(1), 0, s, p.max_pairs,
                        d.counters + RB_CNT_PAIRS, (const uint32_t *)(d.counters + RB_CNT_NARROW_CAND),
                        d.counters + RB_CNT_OVERFLOW);
-    return hipGetLastError();
+    return rb_launch_status();
 }

@@ -676,6 +676,7 @@ __global__ __launch_bounds__(TB) void k_tile_pack(
     if (tid == 0) {
         tile_nb[t] = nb;
         tile_hdr[t] = make_int4(first, R, tile_ncolors[t], nb < SOLVE_SLOTS ? nb : SOLVE_SLOTS);
+        atomicAdd(counters + RB_CNT_SOLVE_BODIES, nb);
     }
     for (int r = tid; r < R; r += TB) {
         const int i = first + r;
@@ -1174,43 +1175,49 @@ __global__ __launch_bounds__(256) void k_integrate(int n, float4 *__restrict__ p
 // ---------------------------------------------------------------- launchers
 hipError_t launch_constraints(const DeviceWorld &d, const StepParams &p, hipStream_t s) {
     int grid = (p.max_pairs + 255) / 256;
-    hipLaunchKernelGGL(k_constraints, dim3(grid), dim3(256), 0, s, p, d.counters + RB_CNT_PAIRS,
+    RB_LAUNCH(k_constraints, dim3(grid), dim3(256), 0, s, p, d.counters + RB_CNT_PAIRS,
                        d.has_contact, d.contact_off, d.contact_slots, d.contacts, d.pos, d.linvel,
                        d.angvel, d.inv_mass, d.inv_inertia_world, d.scene, d.aabb_min, d.aabb_max,
                        d.cs, d.tile_kv, d.counters, d.prev_keys, d.prev_imp, d.prev_pts, d.prev_count,
                        d.features, d.solve_pos, d.pairs);
-    return hipGetLastError();
+    return rb_launch_status();
 }
 
 hipError_t launch_contact_total(const DeviceWorld &d, const StepParams &p, hipStream_t s) {
-    hipLaunchKernelGGL(k_contact_total, dim3(1), dim3(1), 0, s,
+    RB_LAUNCH(k_contact_total, dim3(1), dim3(1), 0, s,
                        (const uint32_t *)(d.counters + RB_CNT_CONTACT_TOTAL), d.counters,
                        p.max_contacts);
-    return hipGetLastError();
+    return rb_launch_status();
 }
 
 hipError_t launch_tile_bounds(const DeviceWorld &d, const StepParams &p, hipStream_t s) {
-    hipLaunchKernelGGL(k_tile_clear, dim3((p.n_tiles + 255) / 256), dim3(256), 0, s, p.n_tiles,
+    RB_LAUNCH(k_tile_clear, dim3((p.n_tiles + 255) / 256), dim3(256), 0, s, p.n_tiles,
                        d.tile_first, d.tile_cnt);
-    hipLaunchKernelGGL(k_tile_bounds2, dim3((p.max_contacts + 255) / 256), dim3(256), 0, s,
+    RB_LAUNCH(k_tile_bounds2, dim3((p.max_contacts + 255) / 256), dim3(256), 0, s,
                        d.tile_kv, d.counters, d.tile_first, d.tile_cnt);
-    return hipGetLastError();
+    return rb_launch_status();
 }
 
 hipError_t launch_tiles(const DeviceWorld &d, const StepParams &p, hipStream_t s) {
-    hipLaunchKernelGGL(k_tile_color, dim3(p.n_tiles), dim3(TB), 0, s, d.tile_kv, d.tile_first,
+    RB_LAUNCH(k_tile_color, dim3(p.n_tiles), dim3(TB), 0, s, d.tile_kv, d.tile_first,
                        d.tile_cnt, d.cs.bodies, d.inv_mass, d.solve_order, d.row_color,
                        d.tile_color_off, d.tile_ncolors, d.counters);
-    hipLaunchKernelGGL(k_tile_pack, dim3(p.n_tiles), dim3(TB), 0, s, d.tile_first, d.tile_cnt,
+    RB_LAUNCH(k_tile_pack, dim3(p.n_tiles), dim3(TB), 0, s, d.tile_first, d.tile_cnt,
                        d.solve_order, d.cs, d.pos, d.inv_mass, d.inv_inertia_world, d.pk,
                        d.solve_pos, d.tile_bodies, d.tile_nb, d.tile_ncolors, d.tile_hdr,
                        d.counters);
-    return hipGetLastError();
+    return rb_launch_status();
 }
 
-hipError_t launch_solve(const DeviceWorld &d, const StepParams &p, hipStream_t s) {
+hipError_t launch_solve(const DeviceWorld &d, const StepParams &p, hipStream_t s,
+                        hipEvent_t *kev, int *n_launches) {
     const int S = p.normal_iters + p.friction_iters;
     const int warm = p.warm_start != MI_RB_WARM_ZERO;
+    int nl = 0;
+    auto mark = [&](int end) {
+        if (kev) hipEventRecord(kev[2 * nl + end], s);
+        if (end) nl++;
+    };
     // (the step's first kernel has cleared the converged flag and the residuals)
     // scene tiles: one workgroup per scene, 256 or 512 threads
 #define RB_SOLVE_ARGS                                                                         \
@@ -1219,17 +1226,19 @@ hipError_t launch_solve(const DeviceWorld &d, const StepParams &p, hipStream_t s
     if (p.tile_mode == 1) {
         // independent scenes: every sweep of a scene in one workgroup, one
         // launch; with a tolerance each scene stops on its own residual
+        mark(0);
         if (p.solve_block == 256)
-            hipLaunchKernelGGL(k_tile_solve<256>, dim3(p.n_tiles), dim3(256), 0, s, p, 0, 0,
+            RB_LAUNCH(k_tile_solve<256>, dim3(p.n_tiles), dim3(256), 0, s, p, 0, 0,
                                p.normal_iters, p.friction_iters, warm, RB_SOLVE_ARGS);
         else
-            hipLaunchKernelGGL(k_tile_solve<TB_SCENE>, dim3(p.n_tiles), dim3(TB_SCENE), 0, s, p,
+            RB_LAUNCH(k_tile_solve<TB_SCENE>, dim3(p.n_tiles), dim3(TB_SCENE), 0, s, p,
                                0, 0, p.normal_iters, p.friction_iters, warm, RB_SOLVE_ARGS);
+        mark(1);
         if (p.tol > 0.f)
-            hipLaunchKernelGGL(k_tile_summary, dim3(1), dim3(256), 0, s, p.n_tiles, d.tile_stats,
+            RB_LAUNCH(k_tile_summary, dim3(1), dim3(256), 0, s, p.n_tiles, d.tile_stats,
                                d.counters);
         else
-            hipLaunchKernelGGL(k_sweep_summary, dim3(1), dim3(1), 0, s, d.residual, p.normal_iters,
+            RB_LAUNCH(k_sweep_summary, dim3(1), dim3(1), 0, s, d.residual, p.normal_iters,
                                p.friction_iters, d.counters);
     } else {
         for (int sw = warm ? -1 : 0; sw < S; sw++) {
@@ -1237,34 +1246,37 @@ hipError_t launch_solve(const DeviceWorld &d, const StepParams &p, hipStream_t s
             for (int ph = 0; ph < 4; ph++) {
                 int nx = (p.tiles_x - (ph & 1) + 1) / 2, nz = (p.tiles_z - (ph >> 1) + 1) / 2;
                 if (nx <= 0 || nz <= 0) continue;
+                mark(0);
                 if (sw < 0)
-                    hipLaunchKernelGGL(k_tile_solve<TB>, dim3(nx * nz), dim3(TB), 0, s, p, ph, 0, 0, 0,
+                    RB_LAUNCH(k_tile_solve<TB>, dim3(nx * nz), dim3(TB), 0, s, p, ph, 0, 0, 0,
                                        1, RB_SOLVE_ARGS);
                 else if (!fr)
-                    hipLaunchKernelGGL((k_tile_solve<TB, true>), dim3(nx * nz), dim3(TB), 0, s, p,
+                    RB_LAUNCH((k_tile_solve<TB, true>), dim3(nx * nz), dim3(TB), 0, s, p,
                                        ph, sw, 1, 0, 0, RB_SOLVE_ARGS);
                 else
-                    hipLaunchKernelGGL(k_tile_solve<TB>, dim3(nx * nz), dim3(TB), 0, s, p, ph, sw,
+                    RB_LAUNCH(k_tile_solve<TB>, dim3(nx * nz), dim3(TB), 0, s, p, ph, sw,
                                        0, 1, 0, RB_SOLVE_ARGS);
+                mark(1);
             }
             if (sw >= 0 && p.tol > 0.f) {
-                hipLaunchKernelGGL(k_sweep_done, dim3(1), dim3(1), 0, s, d.residual, sw, fr, p.tol,
+                RB_LAUNCH(k_sweep_done, dim3(1), dim3(1), 0, s, d.residual, sw, fr, p.tol,
                                    d.counters);
                 if (sw == p.normal_iters - 1)
-                    hipLaunchKernelGGL(k_clear_done, dim3(1), dim3(1), 0, s, d.counters);
+                    RB_LAUNCH(k_clear_done, dim3(1), dim3(1), 0, s, d.counters);
             }
         }
         if (p.tol <= 0.f)
-            hipLaunchKernelGGL(k_sweep_summary, dim3(1), dim3(1), 0, s, d.residual, p.normal_iters,
+            RB_LAUNCH(k_sweep_summary, dim3(1), dim3(1), 0, s, d.residual, p.normal_iters,
                                p.friction_iters, d.counters);
     }
 #undef RB_SOLVE_ARGS
+    if (n_launches) *n_launches = nl;
     const bool staged = p.n_proxies > 0;
-    hipLaunchKernelGGL(k_save_prev, dim3((p.max_contacts + 255) / 256), dim3(256), 0, s,
+    RB_LAUNCH(k_save_prev, dim3((p.max_contacts + 255) / 256), dim3(256), 0, s,
                        d.counters, d.cs, d.solve_pos, d.pk.lam, d.pk.fric,
                        staged ? d.stage_keys : d.prev_keys, staged ? d.stage_imp : d.prev_imp,
                        staged ? d.stage_pts : d.prev_pts, d.prev_count, d.pairs);
-    return hipGetLastError();
+    return rb_launch_status();
 }
 
 namespace {
@@ -1298,18 +1310,18 @@ hipError_t launch_cache_order(const DeviceWorld &d, const StepParams &p, hipStre
     while (bits < 32 && (top >> bits) != 0) bits++;
     const int grid = (p.max_contacts + 255) / 256;
     int *nc = d.counters + RB_CNT_CONTACTS;
-    hipLaunchKernelGGL(k_cache_key_y, dim3(grid), dim3(256), 0, s, d.counters, d.stage_keys,
+    RB_LAUNCH(k_cache_key_y, dim3(grid), dim3(256), 0, s, d.counters, d.stage_keys,
                        d.tile_kv);
     hipError_t e = launch_radix_sort(d.tile_kv, p.max_contacts, bits, 8, sw, s, nc);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_cache_key_x, dim3(grid), dim3(256), 0, s, d.counters, d.stage_keys,
+    RB_LAUNCH(k_cache_key_x, dim3(grid), dim3(256), 0, s, d.counters, d.stage_keys,
                        d.tile_kv);
     e = launch_radix_sort(d.tile_kv, p.max_contacts, bits, 8, sw, s, nc);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_cache_gather, dim3(grid), dim3(256), 0, s, d.counters, d.tile_kv,
+    RB_LAUNCH(k_cache_gather, dim3(grid), dim3(256), 0, s, d.counters, d.tile_kv,
                        d.stage_keys, d.stage_imp, d.stage_pts, d.prev_keys, d.prev_imp,
                        d.prev_pts);
-    return hipGetLastError();
+    return rb_launch_status();
 }
 
 // Body transforms into a renderer's vertex buffer (copyTransformsToVBOKernel,
@@ -1327,14 +1339,14 @@ __global__ __launch_bounds__(256) void k_copy_transforms(int n, const float4 *__
 
 hipError_t launch_copy_transforms(const DeviceWorld &d, int n, float4 *dst, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_copy_transforms, dim3((n + 255) / 256), dim3(256), 0, s, n, d.pos, d.quat,
+    RB_LAUNCH(k_copy_transforms, dim3((n + 255) / 256), dim3(256), 0, s, n, d.pos, d.quat,
                        dst);
-    return hipGetLastError();
+    return rb_launch_status();
 }
 
 hipError_t launch_integrate(const DeviceWorld &d, const StepParams &p, hipStream_t s) {
     if (p.n_bodies == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_integrate, dim3((p.n_bodies + 255) / 256), dim3(256), 0, s, p.n_bodies,
+    RB_LAUNCH(k_integrate, dim3((p.n_bodies + 255) / 256), dim3(256), 0, s, p.n_bodies,
                        d.pos, d.quat, d.linvel, d.angvel, d.inv_mass, p.dt, p.damping, p.max_ang);
-    return hipGetLastError();
+    return rb_launch_status();
 }

@@ -13,6 +13,7 @@
 // tiles that are already running.
 #include <hip/hip_runtime.h>
 
+#include <cstdio>
 #include <mutex>
 
 #include "dmath.h"
@@ -337,9 +338,9 @@ hipError_t radix_sort_impl(uint2 *kv, int n, int sort_bits, const SortWork &w, h
     size_t zero_words = (size_t)passes * RADIX;
     size_t status_words = (size_t)passes * tiles * RADIX;
     // (the error word after the tile counters is sticky: not cleared)
-    hipLaunchKernelGGL(k_sort_clear, dim3(256), dim3(256), 0, s, w.hist, zero_words, w.status,
+    RB_LAUNCH(k_sort_clear, dim3(256), dim3(256), 0, s, w.hist, zero_words, w.status,
                        status_words, w.tile_ctr, (size_t)MAX_PASSES);
-    hipLaunchKernelGGL(k_histogram<RBITS>, dim3(tiles), dim3(SORT_BLOCK), 0, s, kv, n, passes,
+    RB_LAUNCH(k_histogram<RBITS>, dim3(tiles), dim3(SORT_BLOCK), 0, s, kv, n, passes,
                        sort_bits, w.hist, n_dev);
     // look-back timeouts: a sticky bit in the caller's overflow word, else
     // in the word after the tile counters
@@ -349,18 +350,43 @@ hipError_t radix_sort_impl(uint2 *kv, int n, int sort_bits, const SortWork &w, h
     for (int p = 0; p < passes; p++) {
         int shift = p * RBITS;
         int bits = sort_bits - shift < RBITS ? sort_bits - shift : RBITS;
-        hipLaunchKernelGGL(k_onesweep<RBITS>, dim3(tiles), dim3(SORT_BLOCK), 0, s, src, dst, n,
+        RB_LAUNCH(k_onesweep<RBITS>, dim3(tiles), dim3(SORT_BLOCK), 0, s, src, dst, n,
                            shift, bits, w.hist + (size_t)p * RADIX,
                            w.status + (size_t)p * tiles * RADIX, w.tile_ctr + p, err, err_bit,
                            n_dev);
         uint2 *t = src; src = dst; dst = t;
     }
     if (src != kv)  // odd pass count: copy back with a kernel (graph-capture friendly)
-        hipLaunchKernelGGL(k_copy_kv, dim3(256), dim3(256), 0, s, src, kv, n, n_dev);
-    return hipGetLastError();
+        RB_LAUNCH(k_copy_kv, dim3(256), dim3(256), 0, s, src, kv, n, n_dev);
+    return rb_launch_status();
 }
 
 }  // namespace
+
+namespace {
+thread_local const char *g_bad_launch = nullptr;
+}
+
+bool rb_launch_ok(dim3 grid, dim3 block, const char *kernel) {
+    const unsigned long long threads = (unsigned long long)block.x * block.y * block.z;
+    const bool ok = grid.x >= 1 && grid.y >= 1 && grid.z >= 1 && grid.x <= 0x7fffffffu &&
+                    grid.y <= 65535u && grid.z <= 65535u && threads >= 1 && threads <= 1024;
+    if (!ok) {
+        std::fprintf(stderr, "mi_rb: launch of %s refused: grid (%u, %u, %u), block (%u, %u, %u)\n",
+                     kernel, grid.x, grid.y, grid.z, block.x, block.y, block.z);
+        g_bad_launch = kernel;
+    }
+    return ok;
+}
+
+hipError_t rb_launch_status() {
+    hipError_t e = hipGetLastError();
+    if (g_bad_launch) {
+        g_bad_launch = nullptr;
+        return hipErrorInvalidConfiguration;
+    }
+    return e;
+}
 
 size_t sort_status_words(int max_n, int digit_bits) {
     int tiles = (max_n + SORT_TILE - 1) / SORT_TILE;
@@ -383,38 +409,38 @@ hipError_t launch_radix_sort(uint2 *kv, int n, int sort_bits, int digit_bits, co
 hipError_t launch_exclusive_scan(const uint32_t *in, uint32_t *out, int n, uint32_t *total,
                                  uint32_t *status, hipStream_t s, const int *n_dev, int *err) {
     if (n <= 0) {
-        hipLaunchKernelGGL(k_scan_empty, dim3(1), dim3(1), 0, s, total);
-        return hipGetLastError();
+        RB_LAUNCH(k_scan_empty, dim3(1), dim3(1), 0, s, total);
+        return rb_launch_status();
     }
     int tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
     // status: [0] tile counter, then 64-bit words (8-byte aligned)
     uint32_t *ctr = status;
     uint64_t *st = reinterpret_cast<uint64_t *>(status + 2);
-    hipLaunchKernelGGL(k_fill_u32, dim3(64), dim3(256), 0, s, status, 0u, (size_t)(2 + 2 * tiles));
+    RB_LAUNCH(k_fill_u32, dim3(64), dim3(256), 0, s, status, 0u, (size_t)(2 + 2 * tiles));
     // a look-back timeout sets MI_RB_OVERFLOW_LOOKBACK in *err, else the
     // word after the tile counter (cleared with the status words)
     uint32_t *e = err ? reinterpret_cast<uint32_t *>(err) : status + 1;
     const uint32_t bit = err ? (uint32_t)MI_RB_OVERFLOW_LOOKBACK : 1u;
-    hipLaunchKernelGGL(k_scan, dim3(tiles), dim3(SCAN_BLOCK), 0, s, in, out, n, total, st, ctr,
+    RB_LAUNCH(k_scan, dim3(tiles), dim3(SCAN_BLOCK), 0, s, in, out, n, total, st, ctr,
                        n_dev, e, bit);
-    return hipGetLastError();
+    return rb_launch_status();
 }
 
 hipError_t launch_bound_search(const uint2 *kv, int n, uint32_t *dst, int n_dst, int mode,
                                hipStream_t s) {
     if (mode == 2)
-        hipLaunchKernelGGL(k_fill_u32, dim3(256), dim3(256), 0, s, dst, 0u, (size_t)n_dst);
-    hipLaunchKernelGGL(k_bound_search, dim3((n + 1 + 255) / 256), dim3(256), 0, s, kv, n, dst,
+        RB_LAUNCH(k_fill_u32, dim3(256), dim3(256), 0, s, dst, 0u, (size_t)n_dst);
+    RB_LAUNCH(k_bound_search, dim3((n + 1 + 255) / 256), dim3(256), 0, s, kv, n, dst,
                        n_dst, mode);
-    return hipGetLastError();
+    return rb_launch_status();
 }
 
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;
     int blocks = (int)((n + 255) / 256);
     if (blocks > 1024) blocks = 1024;
-    hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, s, p, v, n);
-    return hipGetLastError();
+    RB_LAUNCH(k_fill_u32, dim3(blocks), dim3(256), 0, s, p, v, n);
+    return rb_launch_status();
 }
 
 // ---------------------------------------------------------------- C ABI
@@ -518,9 +544,9 @@ extern "C" mi_rb_status mi_rb_radix_sort_kv_noalloc(void *d_keyvalue, int n, int
     w.max_tiles = NA_TILES;
     // the look-back error word keeps the previous call's value: clear it too
     if (n > 1 && sort_bits > 0) {
-        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, static_cast<hipStream_t>(stream),
+        RB_LAUNCH(k_fill_u32, dim3(1), dim3(64), 0, static_cast<hipStream_t>(stream),
                            w.tile_ctr + MAX_PASSES, 0u, (size_t)1);
-        if (hipGetLastError() != hipSuccess) return MI_RB_DEVICE;
+        if (rb_launch_status() != hipSuccess) return MI_RB_DEVICE;
     }
     hipError_t e = launch_radix_sort(static_cast<uint2 *>(d_keyvalue), n, sort_bits, 8, w,
                                      static_cast<hipStream_t>(stream));

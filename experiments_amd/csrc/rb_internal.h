@@ -195,6 +195,7 @@ enum {
     RB_CNT_SORT_TILE = 8,     // uint32 total of the pair-count scan
     RB_CNT_CONTACT_TOTAL = 9, // uint32 total of the contact scan
     RB_CNT_NARROW_CAND = 10,  // uint32 number of narrowphase pass-2 candidates
+    RB_CNT_SOLVE_BODIES = 11, // dynamic bodies of the solver tiles, summed over tiles
     RB_CNT_DONE = 12,         // solver converged flag for the remaining sweeps
     RB_CNT_SUM_ITERS_N = 13,  // scene tiles with a tolerance: sweeps summed over tiles
     RB_CNT_SUM_ITERS_F = 14,
@@ -223,6 +224,8 @@ struct StepParams {
     float threshold;
     float damping;
     float max_ang;
+    int sat_edge_reject;  // 1: an edge axis with disjoint projections separates (CPU twin)
+    int reduce_mode;      // MI_RB_REDUCE_*
     // solver tiling
     float tile_inv;
     int tiles_x, tiles_z;
@@ -238,6 +241,21 @@ struct StepParams {
     int has_vertexless;  // some shape is a sphere, a plane or a compound
     int n_proxies;       // compound children bodies (after the n bodies)
 };
+
+// ---- checked kernel launches ----
+// Every kernel launch goes through RB_LAUNCH: a grid or workgroup outside
+// what a dispatch may carry (an empty grid, a dimension above 2^31-1, more
+// than 1024 work-items per group) is refused on the host and named on
+// stderr instead of reaching the queue (inside a captured graph it would
+// become a dispatch packet).  rb_launch_status() then reports it as a failed
+// launch (hipErrorInvalidConfiguration) once, else hipGetLastError().
+bool rb_launch_ok(dim3 grid, dim3 block, const char *kernel);
+hipError_t rb_launch_status();
+#define RB_LAUNCH(kernel, grid, block, shmem, stream, ...)                         \
+    do {                                                                          \
+        if (rb_launch_ok(dim3(grid), dim3(block), #kernel))                       \
+            hipLaunchKernelGGL(kernel, grid, block, shmem, stream, __VA_ARGS__);  \
+    } while (0)
 
 // ---- launch interfaces (implemented in the .hip files) ----
 struct SortWork {
@@ -275,7 +293,11 @@ hipError_t launch_constraints(const DeviceWorld &d, const StepParams &p, hipStre
 hipError_t launch_contact_total(const DeviceWorld &d, const StepParams &p, hipStream_t s);
 hipError_t launch_tile_bounds(const DeviceWorld &d, const StepParams &p, hipStream_t s);
 hipError_t launch_tiles(const DeviceWorld &d, const StepParams &p, hipStream_t s);
-hipError_t launch_solve(const DeviceWorld &d, const StepParams &p, hipStream_t s);
+// kev: with timing, events recorded before and after every k_tile_solve
+// launch (2 per launch, at least 2 * (4 * (sweeps + 1)) of them), else null;
+// *n_launches gets the number of launches
+hipError_t launch_solve(const DeviceWorld &d, const StepParams &p, hipStream_t s,
+                        hipEvent_t *kev = nullptr, int *n_launches = nullptr);
 // worlds with compound children: the impulse cache written by the solve in
 // contact order, sorted by key into prev_* (a stable sort by y, then by x)
 hipError_t launch_cache_order(const DeviceWorld &d, const StepParams &p, hipStream_t s,

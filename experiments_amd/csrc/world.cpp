@@ -81,6 +81,8 @@ struct World {
     hipEvent_t ev_order = nullptr;  // orders work on a caller's stream after the world's steps
     bool timing = true;
     bool events_full = false;  // every stage event of the last eager step was recorded
+    std::vector<hipEvent_t> kev;  // eager timing: around every k_tile_solve launch
+    int kev_launches = 0;         // launches timed by kev in the last eager step
     // learned warm-start weights (host copy; uploaded with the world)
     std::vector<uint16_t> mlp_w_host;
     std::vector<float> mlp_b_host;
@@ -250,6 +252,8 @@ mi_rb_status derive_and_upload(World &w) {
     p.threshold = c.contact_threshold;
     p.damping = c.angular_damping;
     p.max_ang = c.max_angular_motion;
+    p.sat_edge_reject = c.sat_mode == MI_RB_SAT_CPU_TWIN ? 1 : 0;
+    p.reduce_mode = c.reduction_mode;
     p.normal_iters = c.solver_iterations;
     p.friction_iters = c.solver_iterations;
     p.tol = c.residual_tolerance;
@@ -485,7 +489,7 @@ mi_rb_status enqueue_step(World &w, int last_stage) {
     (void)hipGetLastError();  // start from a clean error state: launches below are checked one by one
     w.events_full = ev && last_stage >= MI_RB_STAGE_INTEGRATE;
     if (ev) RB_CHECK(hipEventRecord(w.ev[0], s));
-    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(256), 0, s, d.counters, d.residual,
+    RB_LAUNCH(k_zero_counters, dim3(1), dim3(256), 0, s, d.counters, d.residual,
                        2 * (p.normal_iters + p.friction_iters + 1));
     RB_CHECK(launch_setup(d, p, s));
     if (ev) RB_CHECK(hipEventRecord(w.ev[1], s));
@@ -519,7 +523,16 @@ mi_rb_status enqueue_step(World &w, int last_stage) {
     RB_CHECK(launch_tiles(d, p, s));
     if (ev) RB_CHECK(hipEventRecord(w.ev[6], s));
     if (last_stage < MI_RB_STAGE_SOLVE) return MI_RB_OK;
-    RB_CHECK(launch_solve(d, p, s));
+    if (ev) {
+        const size_t need = 2 * (size_t)4 * (p.normal_iters + p.friction_iters + 1) + 2;
+        while (w.kev.size() < need) {
+            hipEvent_t e = nullptr;
+            RB_CHECK(hipEventCreate(&e));
+            w.kev.push_back(e);
+        }
+    }
+    w.kev_launches = 0;
+    RB_CHECK(launch_solve(d, p, s, ev ? w.kev.data() : nullptr, &w.kev_launches));
     RB_CHECK(launch_cache_order(d, p, s, w.sw));
     if (ev) RB_CHECK(hipEventRecord(w.ev[7], s));
     if (last_stage < MI_RB_STAGE_INTEGRATE) return MI_RB_OK;
@@ -558,7 +571,8 @@ extern "C" mi_rb_status mi_rb_create(const mi_rb_config *cfg, void **out_world) 
     if (!cfg || !out_world) return MI_RB_INVALID_ARG;
     if (cfg->max_bodies <= 0 || cfg->max_pairs <= 0 || cfg->max_contacts <= 0 ||
         cfg->max_pairs_per_body <= 0 || cfg->max_pairs_per_body > 64 || cfg->dt <= 0.f ||
-        cfg->solver_iterations < 0 || cfg->solver_iterations > 500)
+        cfg->solver_iterations < 0 || cfg->solver_iterations > 500 || cfg->sat_mode < 0 ||
+        cfg->sat_mode > 1 || cfg->reduction_mode < 0 || cfg->reduction_mode > 1)
         return MI_RB_INVALID_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= cfg->device) return MI_RB_DEVICE;
@@ -582,6 +596,7 @@ extern "C" mi_rb_status mi_rb_destroy(void *world) {
     if (w->stream) hipStreamSynchronize(w->stream);
     w->free_device();
     for (auto &e : w->ev) if (e) hipEventDestroy(e);
+    for (auto &e : w->kev) if (e) hipEventDestroy(e);
     if (w->ev_order) hipEventDestroy(w->ev_order);
     if (w->stream) hipStreamDestroy(w->stream);
     delete w;
@@ -1178,6 +1193,7 @@ extern "C" mi_rb_status mi_rb_get_stats(void *world, mi_rb_stats *out) {
     out->num_pairs = c[RB_CNT_PAIRS];
     out->num_contacts = c[RB_CNT_CONTACTS];
     out->num_colors = c[RB_CNT_NCOLORS];
+    out->solve_bodies = c[RB_CNT_SOLVE_BODIES];
     out->overflow_flags = c[RB_CNT_OVERFLOW];
     out->normal_iterations = c[RB_CNT_ITERS_N];
     out->friction_iterations = c[RB_CNT_ITERS_F];
@@ -1196,6 +1212,13 @@ extern "C" mi_rb_status mi_rb_get_stats(void *world, mi_rb_stats *out) {
             float ms = 0.f;
             if (hipEventElapsedTime(&ms, w->ev[k], w->ev[k + 1]) == hipSuccess) out->stage_ms[k] = ms;
         }
+        float sum = 0.f;
+        for (int k = 0; k < w->kev_launches; k++) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, w->kev[2 * k], w->kev[2 * k + 1]) == hipSuccess) sum += ms;
+        }
+        out->solve_kernel_ms = sum;
+        out->solve_launches = w->kev_launches;
         (void)hipGetLastError();  // an event query must not leave a sticky error behind
     }
     return MI_RB_OK;

@@ -8,10 +8,13 @@ reference has no counterpart: its solver starts every step from zero impulses
 (dynamics/basic_demo/Stubs/solverSetup.cl:529-531).
 """
 import ctypes
+import os
 
 import numpy as np
 
 from . import _lib as L
+
+WEIGHTS_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "warmstart_weights.npz")
 
 FEATURES = ["n.x", "n.y", "n.z", "10*min depth", "vr.n", "vr.t0", "vr.t1", "1/mA", "1/mB",
             "10*prev lambda_n", "10*prev friction0", "10*prev friction1", "npts/4", "has prev",
@@ -122,3 +125,13 @@ def run_iterations(world, steps):
                 max_iters_normal=int(a[:, 2].max()), max_iters_friction=int(a[:, 3].max()),
                 max_residual_normal=float(a[:, 4].max()), max_residual_friction=float(a[:, 5].max()),
                 per_step=a)
+
+
+def load_weights(path=WEIGHTS_PATH):
+    """The trained network shipped with the package (written by
+    experiments_amd.train_warmstart --out): (bf16 weights, fp32 biases), or
+    None when there is none."""
+    if not os.path.exists(path):
+        return None
+    z = np.load(path)
+    return z["w_bf16"].astype(np.uint16), z["bias"].astype(np.float32)

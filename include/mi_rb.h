@@ -72,7 +72,21 @@ typedef struct mi_rb_config {
     int inertia_mode;          /* MI_RB_INERTIA_* */
     float tile_size;           /* solver tile edge in XZ; 0 = derived */
     int device;                /* HIP device ordinal */
+    int sat_mode;              /* MI_RB_SAT_*: the kernel's SAT or the CPU twin's */
+    int reduction_mode;        /* MI_RB_REDUCE_*: contact reduction of the clip kernel */
 } mi_rb_config;
+
+/* SAT of the hull narrowphase: as the GPU kernel (findSeparatingAxisKernel,
+ * sat.cl:928, edge x edge axes never reject a pair), or as the CPU twin
+ * (findSeparatingAxis, gpu_rigidbody_pipeline2/ConvexHullContact.cpp:326-453:
+ * an edge axis with disjoint projections separates the pair) */
+enum { MI_RB_SAT_GPU = 0, MI_RB_SAT_CPU_TWIN = 1 };
+/* contact reduction to <= 4 points: extractManifoldSequential of
+ * clipHullHullKernel (satClipHullContacts.cl:788-884, the default), or
+ * extractManifoldSequentialGlobal of the split path (:678-767: up to 4 points
+ * pass through, else the extremes along +-u, +-v below FLT_MIN with the
+ * deepest point replacing the first if missing; points may repeat) */
+enum { MI_RB_REDUCE_SEQUENTIAL = 0, MI_RB_REDUCE_SPLIT_PATH = 1 };
 
 typedef struct mi_rb_stats {
     int num_bodies;
@@ -90,6 +104,13 @@ typedef struct mi_rb_stats {
      * are the maximum over scenes, these the mean over scenes with contacts */
     float mean_normal_iterations;
     float mean_friction_iterations;
+    /* dynamic bodies of the solver's tiles, summed over tiles (a body in
+     * several tiles counts in each): what one sweep gathers */
+    int solve_bodies;
+    /* eager steps with timing: the HIP-event time of the solver kernel's
+     * launches alone (k_tile_solve) and their number */
+    float solve_kernel_ms;
+    int solve_launches;
 } mi_rb_stats;
 
 /* parameters derived at upload (grid, overflow list, solver tiles) */

@@ -46,6 +46,7 @@ def lib():
         _lib.orc_exclusive_scan.restype = ctypes.c_uint32
         for name in ("orc_find_pairs_grid", "orc_find_pairs_brute", "orc_sap_pairs", "orc_sat",
                      "orc_clip_hull_hull", "orc_clip_face", "orc_reduce_manifold",
+                     "orc_reduce_manifold_global",
                      "orc_narrowphase", "orc_color_contacts"):
             getattr(_lib, name).restype = ctypes.c_int
         _lib.orc_sweep_residual.restype = ctypes.c_float
@@ -204,12 +205,13 @@ class Shapes:
         self.edges = np.ascontiguousarray(edges, np.float32).reshape(-1, 4)
 
 
-def sat(shapes, sa, sb, posA, ornA, posB, ornB):
+def sat(shapes, sa, sb, posA, ornA, posB, ornB, edge_reject=False):
     sep = np.zeros(4, np.float32)
     H = shapes.hulls
     r = lib().orc_sat(_p(H[sa:sa + 1]), _p(H[sb:sb + 1]), _p(shapes.verts), _p(shapes.planes),
                       _p(shapes.edges), _p(f4(posA)[0]), _p(np.asarray(ornA, np.float32).copy()),
-                      _p(f4(posB)[0]), _p(np.asarray(ornB, np.float32).copy()), _p(sep))
+                      _p(f4(posB)[0]), _p(np.asarray(ornB, np.float32).copy()),
+                      ctypes.c_int(1 if edge_reject else 0), _p(sep))
     return r, sep
 
 
@@ -220,11 +222,13 @@ def clip_face(poly, normal, eq):
     return out[:n].copy()
 
 
-def reduce_manifold(points, normal):
+def reduce_manifold(points, normal, split_path=False):
+    """Indices of the kept points: extractManifoldSequential, or with
+    split_path extractManifoldSequentialGlobal (indices may repeat)."""
     pts = np.ascontiguousarray(points, np.float32).reshape(-1, 4)
     idx = np.full(4, -1, np.int32)
-    n = lib().orc_reduce_manifold(_p(pts), ctypes.c_int(len(pts)),
-                                  _p(np.asarray(normal, np.float32).copy()), _p(idx))
+    fn = lib().orc_reduce_manifold_global if split_path else lib().orc_reduce_manifold
+    n = fn(_p(pts), ctypes.c_int(len(pts)), _p(np.asarray(normal, np.float32).copy()), _p(idx))
     return idx[:n].copy()
 
 
@@ -247,7 +251,8 @@ def compound_proxies(proxy, children, pos, quat):
     return pos4, quat4
 
 
-def narrowphase(pairs, pos, quat, inv_mass, shape, shapes, friction=0.7, max_dist=0.02):
+def narrowphase(pairs, pos, quat, inv_mass, shape, shapes, friction=0.7, max_dist=0.02,
+                sat_mode=0, reduction_mode=0):
     pairs = np.ascontiguousarray(pairs, np.int32).reshape(-1, 2)
     P = len(pairs)
     slots = np.zeros(max(P, 1), CONTACT_DTYPE)
@@ -258,7 +263,8 @@ def narrowphase(pairs, pos, quat, inv_mass, shape, shapes, friction=0.7, max_dis
                           _p(np.ascontiguousarray(shape, np.int32)), _p(shapes.hulls),
                           _p(shapes.verts), _p(shapes.planes), _p(shapes.face_index),
                           _p(shapes.indices), _p(shapes.edges), _f(friction), _f(max_dist),
-                          _p(slots), _p(has))
+                          ctypes.c_int(sat_mode), ctypes.c_int(reduction_mode), _p(slots),
+                          _p(has))
     return slots[:P], has[:P]
 
 

@@ -451,9 +451,13 @@ static int sep_faces(const orc_hull *hA, const orc_hull *hB, const float *verts,
     return 1;
 }
 
-/* findSeparatingAxisEdgeEdge (sat.cl:556-635): edge axes never reject. */
-static void sep_edges(const orc_hull *hA, const orc_hull *hB, const float *verts,
-                      const float *edges, f4 pA, f4 oA, f4 pB, f4 oB, f4 dc, f4 *sep, float *dmin)
+/* findSeparatingAxisEdgeEdge (sat.cl:556-635): edge axes never reject; the
+ * CPU twin (ConvexHullContact.cpp:415-453) rejects on the first edge axis
+ * whose projections are disjoint.  Returns 0 when edge_reject is set and an
+ * edge axis separates. */
+static int sep_edges(const orc_hull *hA, const orc_hull *hB, const float *verts,
+                     const float *edges, f4 pA, f4 oA, f4 pB, f4 oB, f4 dc, f4 *sep, float *dmin,
+                     int edge_reject)
 {
     for (int e0 = 0; e0 < hA->num_edges; e0++) {
         f4 ew0 = qt_rotate(oA, ld4(edges + 4 * (hA->edge_offset + e0)));
@@ -466,6 +470,7 @@ static void sep_edges(const orc_hull *hA, const orc_hull *hB, const float *verts
                 float mn0, mx0, mn1, mx1;
                 project(hA, verts, pA, oA, c, &mn0, &mx0);
                 project(hB, verts, pB, oB, c, &mn1, &mx1);
+                if (edge_reject && (mx0 < mn1 || mx1 < mn0)) return 0;
                 float d0 = mx0 - mn1, d1 = mx1 - mn0;
                 float dist = d0 < d1 ? d0 : d1;
                 if (dist < *dmin) { *dmin = dist; *sep = c; }
@@ -473,6 +478,7 @@ static void sep_edges(const orc_hull *hA, const orc_hull *hB, const float *verts
         }
     }
     if (dot3(neg4(dc), *sep) > 0.0f) *sep = neg4(*sep);
+    return 1;
 }
 
 /* Convex-convex branch of findSeparatingAxisKernel (sat.cl:1195-1258).
@@ -480,7 +486,7 @@ static void sep_edges(const orc_hull *hA, const orc_hull *hB, const float *verts
  * the minimum-overlap axis oriented along c0 - c1. */
 int orc_sat(const orc_hull *hA, const orc_hull *hB, const float *verts, const float *planes,
             const float *edges, const float *posA, const float *ornA, const float *posB,
-            const float *ornB, float sep_out[4])
+            const float *ornB, int edge_reject, float sep_out[4])
 {
     f4 pA = ld4(posA), pB = ld4(posB), oA = ld4(ornA), oB = ld4(ornB);
     pA.w = 0.f; pB.w = 0.f;
@@ -491,7 +497,7 @@ int orc_sat(const orc_hull *hA, const orc_hull *hB, const float *verts, const fl
     f4 sep = mk4(0, 0, 0, 0);
     if (!sep_faces(hA, hB, verts, planes, pA, oA, pB, oB, dc, &sep, &dmin)) return 0;
     if (!sep_faces(hB, hA, verts, planes, pB, oB, pA, oA, dc, &sep, &dmin)) return 0;
-    sep_edges(hA, hB, verts, edges, pA, oA, pB, oB, dc, &sep, &dmin);
+    if (!sep_edges(hA, hB, verts, edges, pA, oA, pB, oB, dc, &sep, &dmin, edge_reject)) return 0;
     st4(sep_out, sep);
     return 1;
 }
@@ -606,6 +612,46 @@ int orc_clip_hull_hull(const float sep_in[4], const orc_hull *hA, const orc_hull
  * of the clipped points: extremes along +u, -u, +v, -v in the contact plane,
  * compared as the float bits with the low 8 bits replaced by the point index,
  * taking the signed-integer maximum. */
+/* extractManifoldSequentialGlobal (satClipHullContacts.cl:678-767), the split
+ * path's reduction: up to 4 points pass through; else the points with the
+ * smallest projection on u, -u, v and -v about the centre (each only from
+ * projections below FLT_MIN, the reference's initial "maximum"; first of
+ * equals), the deepest point replacing the first if it is none of them; 4
+ * indices, which may repeat.  idx keeps 0, 1, 2, 3 where nothing qualifies. */
+int orc_reduce_manifold_global(const float *pts, int n, const float normal[4], int idx[4])
+{
+    for (int i = 0; i < 4; i++) idx[i] = i;
+    if (n == 0) return 0;
+    if (n <= 4) return n;
+    if (n > 64) n = 64;
+    f4 c = ld4(pts);
+    for (int j = 1; j < n; j++) c = add4(c, ld4(pts + 4 * j));
+    {
+        float fn = (float)n;
+        c = mk4(c.x / fn, c.y / fn, c.z / fn, c.w / fn);
+    }
+    f4 nn = ld4(normal);
+    f4 av = sub4(ld4(pts), c);
+    f4 u = cross3(nn, av);
+    f4 v = cross3(nn, u);
+    u = normalize3(u);
+    v = normalize3(v);
+    float minw = FLT_MAX;
+    int min_index = -1;
+    float lim[4] = {FLT_MIN, FLT_MIN, FLT_MIN, FLT_MIN};
+    for (int ie = 0; ie < n; ie++) {
+        f4 p = ld4(pts + 4 * ie);
+        if (p.w < minw) { minw = p.w; min_index = ie; }
+        f4 r = sub4(p, c);
+        float f[4] = {dot3(u, r), dot3(neg4(u), r), dot3(v, r), dot3(neg4(v), r)};
+        for (int k = 0; k < 4; k++)
+            if (f[k] < lim[k]) { lim[k] = f[k]; idx[k] = ie; }
+    }
+    if (idx[0] != min_index && idx[1] != min_index && idx[2] != min_index && idx[3] != min_index)
+        idx[0] = min_index;
+    return 4;
+}
+
 int orc_reduce_manifold(const float *pts, int n, const float normal[4], int idx[4])
 {
     if (n == 0) return 0;
@@ -724,7 +770,8 @@ int orc_narrowphase(int num_pairs, const int32_t *pairs, const float *pos, const
                     const float *inv_mass, const int *shape, const orc_hull *hulls,
                     const float *verts, const float *planes, const int *face_index,
                     const int *indices, const float *edges, float friction,
-                    float max_dist, orc_contact *slots, int *has_contact)
+                    float max_dist, int sat_mode, int reduction_mode, orc_contact *slots,
+                    int *has_contact)
 {
     int total = 0;
     /* pairs are independent: spread them over the host cores (each writes
@@ -747,7 +794,8 @@ int orc_narrowphase(int num_pairs, const int32_t *pairs, const float *pos, const
         float pa[4], pb[4];
         memcpy(pa, pos + 4 * a, 16); memcpy(pb, pos + 4 * b, 16);
         pa[3] = 0.f; pb[3] = 0.f;
-        if (!orc_sat(hA, hB, verts, planes, edges, pa, quat + 4 * a, pb, quat + 4 * b, sep))
+        if (!orc_sat(hA, hB, verts, planes, edges, pa, quat + 4 * a, pb, quat + 4 * b,
+                     sat_mode == 1, sep))
             continue;
         float pts[64 * 4];
         int n = orc_clip_hull_hull(sep, hA, hB, verts, planes, face_index, indices, pos + 4 * a,
@@ -755,7 +803,8 @@ int orc_narrowphase(int num_pairs, const int32_t *pairs, const float *pos, const
         if (n <= 0) continue;
         float nrm[4] = {-sep[0], -sep[1], -sep[2], -sep[3]};
         int idx[4] = {-1, -1, -1, -1};
-        int nr = orc_reduce_manifold(pts, n, nrm, idx);
+        int nr = reduction_mode == 1 ? orc_reduce_manifold_global(pts, n, nrm, idx)
+                                     : orc_reduce_manifold(pts, n, nrm, idx);
         orc_contact *c = &slots[p];
         c->normal[0] = nrm[0]; c->normal[1] = nrm[1]; c->normal[2] = nrm[2];
         c->normal[3] = (float)nr;

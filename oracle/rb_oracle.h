@@ -85,9 +85,11 @@ void orc_setup_bodies(int n, float *linvel, const float *angvel, const float *qu
                       float *inv_inertia_world, const float gravity[3], float dt);
 
 /* ---- narrowphase ---------------------------------------------------------- */
+/* edge_reject: 0 as the GPU kernel (edge axes never reject), 1 as the CPU
+ * twin (an edge axis with disjoint projections separates the pair) */
 int orc_sat(const orc_hull *hA, const orc_hull *hB, const float *verts, const float *planes,
             const float *edges, const float *posA, const float *ornA, const float *posB,
-            const float *ornB, float sep[4]);
+            const float *ornB, int edge_reject, float sep[4]);
 int orc_clip_hull_hull(const float sep[4], const orc_hull *hA, const orc_hull *hB,
                        const float *verts, const float *planes, const int *face_index,
                        const int *indices, const float *posA, const float *ornA,
@@ -95,13 +97,15 @@ int orc_clip_hull_hull(const float sep[4], const orc_hull *hA, const orc_hull *h
                        float *out /* 64 float4 */);
 int orc_clip_face(const float *in, int n_in, const float nrm[4], float eq, float *out);
 int orc_reduce_manifold(const float *pts, int n, const float normal[4], int idx[4]);
+int orc_reduce_manifold_global(const float *pts, int n, const float normal[4], int idx[4]);
 void orc_compound_proxies(int P, int n, const int32_t *proxy, const float *children, float *pos,
                           float *quat);
 int orc_narrowphase(int num_pairs, const int32_t *pairs, const float *pos, const float *quat,
                     const float *inv_mass, const int *shape, const orc_hull *hulls,
                     const float *verts, const float *planes, const int *face_index,
                     const int *indices, const float *edges, float friction,
-                    float max_dist, orc_contact *slots, int *has_contact);
+                    float max_dist, int sat_mode, int reduction_mode, orc_contact *slots,
+                    int *has_contact);
 
 /* ---- solver ---------------------------------------------------------------- */
 /* the two friction tangents of a contact normal (btPlaneSpace1) */

@@ -139,7 +139,8 @@ def oracle_contacts(s, pairs):
         shape = np.concatenate([shape, s.children["shape"][s.proxy[:, 1]]]).astype(np.int32)
         owner = np.concatenate([owner, s.proxy[:, 0]])
     slots, has = O.narrowphase(pairs, pos, quat, im, shape, s.shapes, c.friction,
-                               c.contact_threshold)
+                               c.contact_threshold, sat_mode=getattr(c, "sat_mode", 0),
+                               reduction_mode=getattr(c, "reduction_mode", 0))
     out = slots[has == 1].copy()
     if len(s.proxy):
         # a compound child names its compound (sign: static)

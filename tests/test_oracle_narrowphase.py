@@ -137,3 +137,108 @@ def test_reduce_regular_polygon_extremes():
     assert len(set(idx.tolist())) == 4
     a = np.sort(ang[idx])
     assert np.allclose(np.diff(a), np.pi / 2)
+
+
+# ---- the CPU twin's SAT and the split path's reduction (options) -----------
+
+def _sat64_separated(va, vb):
+    """Exact separation of two convex vertex sets in float64: some face
+    normal or edge cross product of their hulls separates them."""
+    from scipy.spatial import ConvexHull
+    axes = []
+    for v in (va, vb):
+        h = ConvexHull(v)
+        axes += [eq[:3] for eq in h.equations]
+    ea = [va[j] - va[i] for i in range(len(va)) for j in range(i + 1, len(va))]
+    eb = [vb[j] - vb[i] for i in range(len(vb)) for j in range(i + 1, len(vb))]
+    for a in ea:
+        for b in eb:
+            c = np.cross(a, b)
+            if np.linalg.norm(c) > 1e-9:
+                axes.append(c / np.linalg.norm(c))
+    for ax in axes:
+        pa, pb = va @ ax, vb @ ax
+        if pa.max() < pb.min() - 1e-7 or pb.max() < pa.min() - 1e-7:
+            return True
+    return False
+
+
+def test_cpu_twin_sat_rejects_pairs_separated_by_an_edge_axis():
+    """findSeparatingAxis of the CPU twin (ConvexHullContact.cpp:326-453) ends
+    at an edge axis with disjoint projections; the GPU kernel's never does.
+    On cube pairs placed edge to edge the twin's answer is the exact
+    (float64) separation test, and the kernel's keeps the pairs that only an
+    edge axis separates."""
+    from test_abi import hull as build
+    v, pl, fi, ind, ed = build(scenes.cube_points(0.5))
+    H = np.zeros(1, O.HULL_DTYPE)
+    H["num_faces"], H["num_vertices"], H["num_edges"], H["num_indices"] = len(pl), len(v), len(ed), len(ind)
+    sh = O.Shapes(H, O.f4(v), pl.astype(np.float32), fi.astype(np.int32), ind.astype(np.int32),
+                  O.f4(ed))
+    rng = np.random.default_rng(8)
+    edge_only = both = 0
+    for trial in range(700):
+        qa = rng.normal(size=4); qa /= np.linalg.norm(qa)
+        qb = rng.normal(size=4); qb /= np.linalg.norm(qb)
+        d = rng.normal(size=3); d /= np.linalg.norm(d)
+        pa = np.zeros(3, np.float32)
+        pb = (d * rng.uniform(0.9, 1.8)).astype(np.float32)
+        qa, qb = qa.astype(np.float32), qb.astype(np.float32)
+        ok_gpu, _ = O.sat(sh, 0, 0, pa, qa, pb, qb)
+        ok_twin, _ = O.sat(sh, 0, 0, pa, qa, pb, qb, edge_reject=True)
+        R = lambda q: _rot(q)
+        va = v @ R(qa).T + pa
+        vb = v @ R(qb).T + pb
+        sep = _sat64_separated(va.astype(np.float64), vb.astype(np.float64))
+        if ok_twin:
+            assert not sep, trial      # the twin keeps only overlapping pairs
+            assert ok_gpu
+        if ok_gpu and not ok_twin:
+            edge_only += 1
+            assert sep, trial          # ... and rejects only separated ones
+        both += ok_twin
+    assert edge_only > 5 and both > 25, (edge_only, both)
+
+
+def _rot(q):
+    x, y, z, w = np.asarray(q, np.float64) / np.linalg.norm(q)
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+def test_split_path_reduction_known_answers():
+    """extractManifoldSequentialGlobal (satClipHullContacts.cl:678-767)."""
+    n = np.array([0, 1, 0, 0], np.float32)
+    # up to four points pass through
+    for k in range(5):
+        pts = np.zeros((k, 4), np.float32)
+        pts[:, 0] = np.arange(k)
+        assert list(O.reduce_manifold(pts, n, split_path=True)) == list(range(k))
+    # eight points on a circle in the plane of the normal: u = n x (p0 - c)
+    # = -z, v = n x u = -x; the smallest projections on u, -u, v, -v are the
+    # points at 90, 270, 0 and 180 degrees
+    ang = np.arange(8) * (np.pi / 4)
+    pts = np.zeros((8, 4), np.float32)
+    pts[:, 0], pts[:, 2], pts[:, 3] = np.cos(ang), np.sin(ang), -0.01
+    assert list(O.reduce_manifold(pts, n, split_path=True)) == [2, 6, 0, 4]
+    # the deepest point replaces the first when it is none of them
+    pts[5, 3] = -0.1
+    assert list(O.reduce_manifold(pts, n, split_path=True)) == [5, 6, 0, 4]
+    # when it is one of them, nothing is replaced
+    pts[5, 3] = -0.01
+    pts[6, 3] = -0.1
+    assert list(O.reduce_manifold(pts, n, split_path=True)) == [2, 6, 0, 4]
+
+
+def test_split_path_reduction_may_repeat_points():
+    """Unlike extractManifoldSequential, the split path returns four indices
+    even when one point is the extreme of two directions."""
+    n = np.array([0, 1, 0, 0], np.float32)
+    xz = [(0.82, 0.17), (0.33, 0.11), (-1.3, 0.09), (0.91, 0.01), (0.45, 0.16), (-0.54, -0.22)]
+    pts = np.array([(x, 0, z, -.01) for x, z in xz], np.float32)
+    idx = O.reduce_manifold(pts, n, split_path=True)
+    # point 3 has the smallest projection on -u and on v
+    assert list(idx) == [0, 3, 3, 2]
+    idx1 = O.reduce_manifold(pts, n)
+    assert len(idx1) == 4
