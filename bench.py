@@ -367,10 +367,12 @@ def warm_start_leg(n_scenes=1024, settle=120, steps=20, tol=1e-4, max_sweeps=200
         out[name] = {"normal_sweeps": round(r["mean_iters_normal"], 2),
                      "friction_sweeps": round(r["mean_iters_friction"], 2),
                      "sweeps": round(r["mean_iters_normal"] + r["mean_iters_friction"], 2),
-                     "max_sweeps_used": max(r["max_iters_normal"], r["max_iters_friction"]),
-                     "final_residual_max": max(r["max_residual_normal"],
-                                               r["max_residual_friction"]),
-                     "unconverged_steps": int(np.sum((a[:, 4] >= tol) | (a[:, 5] >= tol)))}
+                     "max_normal_sweeps": r["max_iters_normal"],
+                     "max_friction_sweeps": r["max_iters_friction"],
+                     "final_residual_max_normal": r["max_residual_normal"],
+                     "final_residual_max_friction": r["max_residual_friction"],
+                     "steps_with_an_unconverged_scene_normal": int(np.sum(a[:, 4] >= tol)),
+                     "steps_with_an_unconverged_scene_friction": int(np.sum(a[:, 5] >= tol))}
         w.close()
     for name in ("previous", "learned"):
         if name in out:
@@ -466,9 +468,23 @@ def main():
     # eager steps with HIP events from the end state of the timed window,
     # which is then restored
     after = w.get_state()
+    # the solver kernel's launch duration: two eager steps from the end state
+    # of the timed window with HIP events before the first and after the last
+    # of its back-to-back launches (a kernel trace gives the same duration
+    # for the eager and the graph-replayed launches)
+    w.use_graph(False)
+    w.set_solve_timing(True)
+    span_ms, span_launches = 0.0, 0
+    for _ in range(2):
+        w.step(1)
+        s2 = w.stats()
+        span_ms += s2["solve_kernel_ms"]
+        span_launches += s2["solve_launches"]
+    w.set_solve_timing(False)
+    w.set_state(**after)
     w.use_graph(False)
     acc = np.zeros(8)
-    counts = dict(pairs=0, contacts=0, tile_bodies=0, solve_ms=0.0, launches=0)
+    counts = dict(pairs=0, contacts=0, tile_bodies=0, solve_ms=0.0, launches=0, events_ms=0.0)
     for _ in range(args.profile_steps):
         w.step(1)
         s2 = w.stats()
@@ -476,7 +492,7 @@ def main():
         counts["pairs"] += s2["num_pairs"]
         counts["contacts"] += s2["num_contacts"]
         counts["tile_bodies"] += s2["solve_bodies"]
-        counts["solve_ms"] += s2["solve_kernel_ms"]
+        counts["events_ms"] += s2["solve_events_ms"]
         counts["launches"] += s2["solve_launches"]
     k = max(args.profile_steps, 1)
     stage_ms = acc / k
@@ -490,9 +506,11 @@ def main():
     scene_tiles = prm["tile_mode"] == 1
     # the dominant kernel is the tile solve (spatial tiles: one launch per
     # sweep and parity phase; scene tiles: one launch per step); its launch
-    # duration is the mean of the HIP-event spans around each of its launches
+    # duration is the span of its back-to-back launches over the launch
+    # count; the HIP-event spans around each single launch beside it
     launches = max(counts["launches"] / k, 1)
-    launch_s = counts["solve_ms"] / max(counts["launches"], 1) / 1e3
+    launch_s = span_ms / max(span_launches, 1) / 1e3
+    launch_events_s = counts["events_ms"] / max(counts["launches"], 1) / 1e3
     per_launch_bytes = solve_bytes(contacts, tile_bodies, sweeps, scene_tiles) / launches
     achieved = per_launch_bytes / launch_s / 1e9 if launch_s > 0 else 0.0
     prof = traffic_profile(args.workload)
@@ -507,6 +525,11 @@ def main():
                  "counter profile taken at steps %s, not this window" % prof.get("state_steps")),
                 "algorithmic_bytes_per_launch": round(per_launch_bytes),
                 "launches_per_step": round(launches, 2), "launch_us": round(launch_s * 1e6, 2),
+                "launch_us_events": round(launch_events_s * 1e6, 2),
+                "launch_us_source": "HIP events before the first and after the last of the "
+                                    "back-to-back solver launches in two eager steps from the "
+                                    "timed window's end state, over the launch count; "
+                                    "launch_us_events: HIP events around each launch",
                 "rows": round(contacts), "tile_bodies": round(tile_bodies)}
     stage_hbm = {}
     for i, name in enumerate(STAGES):

@@ -59,7 +59,7 @@ class Stats(ctypes.Structure):
         ("stage_ms", ctypes.c_float * 8), ("steps", ctypes.c_longlong),
         ("mean_normal_iterations", ctypes.c_float), ("mean_friction_iterations", ctypes.c_float),
         ("solve_bodies", ctypes.c_int), ("solve_kernel_ms", ctypes.c_float),
-        ("solve_launches", ctypes.c_int)]
+        ("solve_launches", ctypes.c_int), ("solve_events_ms", ctypes.c_float)]
 
 
 class Params(ctypes.Structure):
@@ -166,6 +166,7 @@ SIGNATURES = {
     "mi_rb_mlp_gradients": (I, [VP, VP, VP, I, VP, ctypes.POINTER(ctypes.c_float)]),
     "mi_rb_mlp_train": (I, [VP, VP, VP, I, I, I, ctypes.c_float, I, VP]),
     "mi_rb_last_step_ms": (I, [VP, ctypes.POINTER(ctypes.c_float)]),
+    "mi_rb_set_solve_timing": (I, [VP, I]),
     "mi_rb_hull_build": (I, [VP, I, I, VP, VP, I, VP, VP, I, VP, I, VP, I, VP]),
     "mi_rb_radix_sort_create": (I, [I, PVP]),
     "mi_rb_radix_sort_destroy": (I, [VP]),

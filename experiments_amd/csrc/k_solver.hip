@@ -1210,10 +1210,11 @@ hipError_t launch_tiles(const DeviceWorld &d, const StepParams &p, hipStream_t s
 }
 
 hipError_t launch_solve(const DeviceWorld &d, const StepParams &p, hipStream_t s,
-                        hipEvent_t *kev, int *n_launches) {
+                        hipEvent_t *kev, int *n_launches, hipEvent_t *span) {
     const int S = p.normal_iters + p.friction_iters;
     const int warm = p.warm_start != MI_RB_WARM_ZERO;
     int nl = 0;
+    if (span) hipEventRecord(span[0], s);
     auto mark = [&](int end) {
         if (kev) hipEventRecord(kev[2 * nl + end], s);
         if (end) nl++;
@@ -1270,6 +1271,7 @@ hipError_t launch_solve(const DeviceWorld &d, const StepParams &p, hipStream_t s
                                p.friction_iters, d.counters);
     }
 #undef RB_SOLVE_ARGS
+    if (span) hipEventRecord(span[1], s);
     if (n_launches) *n_launches = nl;
     const bool staged = p.n_proxies > 0;
     RB_LAUNCH(k_save_prev, dim3((p.max_contacts + 255) / 256), dim3(256), 0, s,

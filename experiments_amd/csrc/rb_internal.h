@@ -296,8 +296,11 @@ hipError_t launch_tiles(const DeviceWorld &d, const StepParams &p, hipStream_t s
 // kev: with timing, events recorded before and after every k_tile_solve
 // launch (2 per launch, at least 2 * (4 * (sweeps + 1)) of them), else null;
 // *n_launches gets the number of launches
+// span: two events recorded before the first and after the last solver
+// launch (the launches run back to back), else null
 hipError_t launch_solve(const DeviceWorld &d, const StepParams &p, hipStream_t s,
-                        hipEvent_t *kev = nullptr, int *n_launches = nullptr);
+                        hipEvent_t *kev = nullptr, int *n_launches = nullptr,
+                        hipEvent_t *span = nullptr);
 // worlds with compound children: the impulse cache written by the solve in
 // contact order, sorted by key into prev_* (a stable sort by y, then by x)
 hipError_t launch_cache_order(const DeviceWorld &d, const StepParams &p, hipStream_t s,

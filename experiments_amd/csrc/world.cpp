@@ -83,6 +83,9 @@ struct World {
     bool events_full = false;  // every stage event of the last eager step was recorded
     std::vector<hipEvent_t> kev;  // eager timing: around every k_tile_solve launch
     int kev_launches = 0;         // launches timed by kev in the last eager step
+    bool solve_timing = false;    // events around the solver launches of every step
+    hipEvent_t span[2] = {};      // ... recorded before the first and after the last
+    int span_launches = 0;        // solver launches between them
     // learned warm-start weights (host copy; uploaded with the world)
     std::vector<uint16_t> mlp_w_host;
     std::vector<float> mlp_b_host;
@@ -531,8 +534,14 @@ mi_rb_status enqueue_step(World &w, int last_stage) {
             w.kev.push_back(e);
         }
     }
+    // (eager steps only: events recorded inside a capture do not time replays)
+    const bool span = ev && w.solve_timing && w.span[0] && w.span[1];
     w.kev_launches = 0;
-    RB_CHECK(launch_solve(d, p, s, ev ? w.kev.data() : nullptr, &w.kev_launches));
+    // (per-launch events and the span are not recorded together: the span
+    // serves the captured steps, the per-launch events the eager ones)
+    RB_CHECK(launch_solve(d, p, s, ev && !span ? w.kev.data() : nullptr, &w.kev_launches,
+                          span ? w.span : nullptr));
+    w.span_launches = span ? w.kev_launches : 0;
     RB_CHECK(launch_cache_order(d, p, s, w.sw));
     if (ev) RB_CHECK(hipEventRecord(w.ev[7], s));
     if (last_stage < MI_RB_STAGE_INTEGRATE) return MI_RB_OK;
@@ -597,6 +606,7 @@ extern "C" mi_rb_status mi_rb_destroy(void *world) {
     w->free_device();
     for (auto &e : w->ev) if (e) hipEventDestroy(e);
     for (auto &e : w->kev) if (e) hipEventDestroy(e);
+    for (auto &e : w->span) if (e) hipEventDestroy(e);
     if (w->ev_order) hipEventDestroy(w->ev_order);
     if (w->stream) hipStreamDestroy(w->stream);
     delete w;
@@ -1008,12 +1018,24 @@ extern "C" mi_rb_status mi_rb_use_graph(void *world, int enable) {
     return MI_RB_OK;
 }
 
+extern "C" mi_rb_status mi_rb_set_solve_timing(void *world, int enable) {
+    World *w = W(world);
+    if (!w) return MI_RB_INVALID_ARG;
+    if (w->graph) { hipGraphExecDestroy(w->graph); w->graph = nullptr; }
+    w->solve_timing = enable != 0;
+    hipSetDevice(w->device);
+    for (auto &e : w->span)
+        if (!e && hipEventCreate(&e) != hipSuccess) return MI_RB_DEVICE;
+    return MI_RB_OK;
+}
+
 extern "C" mi_rb_status mi_rb_step(void *world, int nsteps) {
     World *w = W(world);
     if (!w || nsteps < 0) return MI_RB_INVALID_ARG;
     if (!w->uploaded) return MI_RB_STATE;
     if (w->sp.warm_start == MI_RB_WARM_LEARNED && w->mlp_w_host.empty()) return MI_RB_STATE;
     hipSetDevice(w->device);
+
     if (w->use_graph && !w->graph) {
         hipGraph_t g;
         RB_CHECK(hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal));
@@ -1217,8 +1239,16 @@ extern "C" mi_rb_status mi_rb_get_stats(void *world, mi_rb_stats *out) {
             float ms = 0.f;
             if (hipEventElapsedTime(&ms, w->kev[2 * k], w->kev[2 * k + 1]) == hipSuccess) sum += ms;
         }
-        out->solve_kernel_ms = sum;
+        out->solve_events_ms = sum;
         out->solve_launches = w->kev_launches;
+        (void)hipGetLastError();
+    }
+    if (w->solve_timing && w->span_launches > 0) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, w->span[0], w->span[1]) == hipSuccess) {
+            out->solve_kernel_ms = ms;
+            out->solve_launches = w->span_launches;
+        }
         (void)hipGetLastError();  // an event query must not leave a sticky error behind
     }
     return MI_RB_OK;

@@ -193,6 +193,11 @@ class World:
     def step_partial(self, stage):
         L.call("mi_rb_step_partial", self.h, stage)
 
+    def set_solve_timing(self, enable=True):
+        """HIP events around the solver kernel's launches in every step
+        (stats: solve_kernel_ms, solve_launches of the last step)."""
+        L.call("mi_rb_set_solve_timing", self.h, 1 if enable else 0)
+
     def use_graph(self, enable=True):
         L.call("mi_rb_use_graph", self.h, 1 if enable else 0)
 

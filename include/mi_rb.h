@@ -107,10 +107,14 @@ typedef struct mi_rb_stats {
     /* dynamic bodies of the solver's tiles, summed over tiles (a body in
      * several tiles counts in each): what one sweep gathers */
     int solve_bodies;
-    /* eager steps with timing: the HIP-event time of the solver kernel's
-     * launches alone (k_tile_solve) and their number */
+    /* the solver kernel's (k_tile_solve) launches of the last eager step:
+     * with mi_rb_set_solve_timing, their number and the HIP-event span from
+     * before the first to after the last (they run back to back); otherwise
+     * the sum of HIP-event spans around each launch (which add the launch
+     * overhead) */
     float solve_kernel_ms;
     int solve_launches;
+    float solve_events_ms;
 } mi_rb_stats;
 
 /* parameters derived at upload (grid, overflow list, solver tiles) */
@@ -393,6 +397,12 @@ mi_rb_status mi_rb_copy_training_batch(void *world, float *x_dev, float *t_dev, 
 mi_rb_status mi_rb_set_solver(void *world, int iterations, float tolerance, int warm_start);
 /* HIP-event time of the last `steps` steps, milliseconds */
 mi_rb_status mi_rb_last_step_ms(void *world, float *ms);
+/* Measurement (no reference counterpart): with enable, every eager step
+ * records HIP events before the first and after the last launch of the
+ * solver kernel, which run back to back, so mi_rb_get_stats reports their
+ * span (solve_kernel_ms) and number (solve_launches) for the last step.
+ * Drops the captured graph. */
+mi_rb_status mi_rb_set_solve_timing(void *world, int enable);
 
 /* Launch shapes of the step (no reference counterpart: the reference fixes
  * its work-group sizes in the host code).  0 = default.  Changing them drops
