@@ -339,6 +339,26 @@ def solve(cons, order, linvel, angvel, pos, inv_mass, inv_inertia_world, normal_
     return cons, lv, av, iters, res
 
 
+def solve_segments(cons, segments, linvel, angvel, pos, inv_mass, inv_inertia_world,
+                   normal_iters, friction_iters, friction=0.7, tol=0.0):
+    """solve() on each segment (a list of row orders, one per independent
+    tile) in turn, in one call; iterations and residuals per segment."""
+    cons = np.ascontiguousarray(cons, CONSTRAINT_DTYPE).copy()
+    lv, av = f4(linvel), f4(angvel)
+    segs = [np.asarray(x, np.int32) for x in segments]
+    off = np.zeros(len(segs) + 1, np.int32)
+    off[1:] = np.cumsum([len(x) for x in segs])
+    order = np.ascontiguousarray(np.concatenate(segs) if segs else np.zeros(1, np.int32), np.int32)
+    iters = np.zeros((max(len(segs), 1), 2), np.int32)
+    res = np.zeros((max(len(segs), 1), 2), np.float32)
+    lib().orc_solve_segments(ctypes.c_int(len(segs)), _p(off), _p(cons), _p(order), _p(lv),
+                             _p(av), _p(f4(pos)), _p(np.ascontiguousarray(inv_mass, np.float32)),
+                             _p(np.ascontiguousarray(inv_inertia_world, np.float32)),
+                             ctypes.c_int(normal_iters), ctypes.c_int(friction_iters),
+                             _f(friction), _f(tol), _p(iters), _p(res))
+    return cons, lv, av, iters[:len(segs)], res[:len(segs)]
+
+
 def integrate(pos, quat, linvel, angvel, inv_mass, dt, damping=0.99,
               max_ang=0.25 * 3.14159254):
     pos = f4(pos)

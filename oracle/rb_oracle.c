@@ -1324,6 +1324,20 @@ void orc_solve(int nc, orc_constraint *cs, const int *order, float *linvel, floa
     }
 }
 
+/* Independent segments of the visiting order (one per scene tile), each
+ * solved on its own with its own tolerance stop: orc_solve on rows
+ * order[seg_off[k] .. seg_off[k+1]).  iters/residual: 2 per segment. */
+void orc_solve_segments(int nseg, const int *seg_off, orc_constraint *cs, const int *order,
+                        float *linvel, float *angvel, const float *pos, const float *inv_mass,
+                        const float *inv_inertia_world, int normal_iters, int friction_iters,
+                        float friction, float tol, int *iters_used, float *residual)
+{
+    for (int k = 0; k < nseg; k++)
+        orc_solve(seg_off[k + 1] - seg_off[k], cs, order + seg_off[k], linvel, angvel, pos,
+                  inv_mass, inv_inertia_world, normal_iters, friction_iters, friction, tol,
+                  iters_used + 2 * k, residual + 2 * k);
+}
+
 /* quatMult / quatNorm and integrateTransformsKernel2
  * (opencl/broadphase_benchmark/integrateKernel.cl:3-26, 102-158), together
  * with copyVelocitiesKernel's zeroing of static bodies (broadphaseKernel.cl:216-233). */

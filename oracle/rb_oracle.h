@@ -127,6 +127,10 @@ void orc_solve(int nc, orc_constraint *cs, const int *order, float *linvel, floa
                const float *pos, const float *inv_mass, const float *inv_inertia_world,
                int normal_iters, int friction_iters, float friction, float tol,
                int *iters_used, float *residual);
+void orc_solve_segments(int nseg, const int *seg_off, orc_constraint *cs, const int *order,
+                        float *linvel, float *angvel, const float *pos, const float *inv_mass,
+                        const float *inv_inertia_world, int normal_iters, int friction_iters,
+                        float friction, float tol, int *iters_used, float *residual);
 void orc_integrate(int n, float *pos, float *quat, const float *linvel, float *angvel,
                    const float *inv_mass, float dt, float damping, float max_ang);
 

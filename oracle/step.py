@@ -204,13 +204,11 @@ def oracle_step(s, order, contacts=None, prev=None, warm=0, prediction=None, seg
                                           c.solver_iterations, c.solver_iterations, c.friction,
                                           c.residual_tolerance)
     else:
-        lv2, av2, it, res = lv, av, [], []
-        for seg in segments:
-            cons, lv2, av2, i_s, r_s = O.solve(cons, seg, lv2, av2, st["pos"], sh["inv_mass"],
-                                               invI, c.solver_iterations, c.solver_iterations,
-                                               c.friction, c.residual_tolerance)
-            it.append(i_s)
-            res.append(r_s)
+        cons, lv2, av2, it, res = O.solve_segments(cons, segments, lv, av, st["pos"],
+                                                   sh["inv_mass"], invI, c.solver_iterations,
+                                                   c.solver_iterations, c.friction,
+                                                   c.residual_tolerance)
+        it, res = list(it), list(res)
     pos, quat, av3 = O.integrate(st["pos"], st["quat"], lv2, av2, sh["inv_mass"], c.dt,
                                  c.angular_damping, c.max_angular_motion)
     return dict(pos=pos, quat=quat, linvel=lv2, angvel=av3, cons=cons, iters=it, res=res,
