@@ -339,7 +339,7 @@ def batched_leg(steps=20, warmup=5, sample_every=16):
             "overflow_flags": st["overflow_flags"]}
 
 
-def warm_start_leg(n_scenes=1024, settle=120, steps=20, tol=1e-4, max_sweeps=200):
+def warm_start_leg(n_scenes=1024, settle=240, steps=20, tol=1e-4, max_sweeps=500):
     """Solver sweeps to a 1e-4 residual for the zero, previous-step and
     learned warm starts on held-out batched piles (another seed than the
     training scenes), each run from the same settled state; the trained
