@@ -484,13 +484,15 @@ def main():
     w.set_state(**after)
     w.use_graph(False)
     acc = np.zeros(8)
-    counts = dict(pairs=0, contacts=0, tile_bodies=0, solve_ms=0.0, launches=0, events_ms=0.0)
+    counts = dict(pairs=0, contacts=0, tile_bodies=0, solve_ms=0.0, launches=0, events_ms=0.0,
+                  face_pass=0)
     for _ in range(args.profile_steps):
         w.step(1)
         s2 = w.stats()
         acc += np.array(s2["stage_ms"])
         counts["pairs"] += s2["num_pairs"]
         counts["contacts"] += s2["num_contacts"]
+        counts["face_pass"] += s2["face_pass_pairs"]
         counts["tile_bodies"] += s2["solve_bodies"]
         counts["events_ms"] += s2["solve_events_ms"]
         counts["launches"] += s2["solve_launches"]
@@ -566,7 +568,8 @@ def main():
         "config": {"workload": args.workload, "bodies": n, "solver_iterations": scene.iterations,
                    "dt": scene.dt, "graph": not args.eager, "settle_steps": settle,
                    "timed_steps": [settle + args.warmup + 1, done],
-                   "pairs": int(pairs), "contacts": int(contacts), "key_bits": prm["key_bits"],
+                   "pairs": int(pairs), "face_pass_pairs": int(counts["face_pass"] / k),
+                   "contacts": int(contacts), "key_bits": prm["key_bits"],
                    "sort_passes": passes, "host_s": round(host_s, 4), "tuning": tuning},
         "whole_run": whole,
         "autotune_ms_per_step": tune_ms,

@@ -59,7 +59,8 @@ class Stats(ctypes.Structure):
         ("stage_ms", ctypes.c_float * 8), ("steps", ctypes.c_longlong),
         ("mean_normal_iterations", ctypes.c_float), ("mean_friction_iterations", ctypes.c_float),
         ("solve_bodies", ctypes.c_int), ("solve_kernel_ms", ctypes.c_float),
-        ("solve_launches", ctypes.c_int), ("solve_events_ms", ctypes.c_float)]
+        ("solve_launches", ctypes.c_int), ("solve_events_ms", ctypes.c_float),
+        ("face_pass_pairs", ctypes.c_int)]
 
 
 class Params(ctypes.Structure):

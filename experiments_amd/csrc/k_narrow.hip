@@ -290,12 +290,130 @@ __global__ __launch_bounds__(256) void k_narrow_special(
     }
 }
 
+// Pass 0, QL lanes per pair: the face of A that B's centre lies furthest
+// outside of, and the face of B that A's centre lies furthest outside of.
+// Both axes are built and projected with pass 1's arithmetic, so an axis
+// that separates here separates in pass 1 too, and pass 1 (which projects
+// every face axis, or leaves out only axes that cannot separate) would give
+// the pair no contact: it gets none here.  The other pairs (and every pair
+// that is not hull-hull within the limits) are flagged in has_contact for
+// pass 1, which takes them in pair order from the compacted flags and sets
+// has_contact itself (pass 2 for its candidates).  Most broadphase pairs are
+// apart, so pass 1's wave per pair runs on a fraction of them.  With QL > 1
+// the lanes of a pair split its faces and vertices.
+// lanes per pair of pass 0 (4 timed faster than 1 and 8 on the 65,537-hull
+// scene: fewer cache lines per load than 1, less repeated work than 8)
+constexpr int QUICK_LANES = 4;
+template <int QL>
+__device__ __forceinline__ void group_argmax(float &d, int &idx) {
+#pragma unroll
+    for (int o = 1; o < QL; o <<= 1) {
+        const float od = __shfl_xor(d, o, 64);
+        const int oi = __shfl_xor(idx, o, 64);
+        if (od > d || (od == d && oi < idx)) { d = od; idx = oi; }
+    }
+}
+// project() with the group's own hull (vector loads), its vertices split
+// over the QL lanes of the group (every lane of the group must call it)
+template <int QL>
+__device__ __forceinline__ void project_group(const float4 *__restrict__ v, int nv, int sub,
+                                              float4 pos, float4 orn, float4 dir, float &mn,
+                                              float &mx) {
+    const float4 ld = qt_inv_rotate(orn, dir);
+    const float off = dot3(pos, dir);
+    float lo = FLT_MAX, hi = -FLT_MAX;
+    for (int i = sub; i < nv; i += QL) {
+        const float4 q = v[i];
+        const float dp = (q.x * ld.x + q.y * ld.y) + q.z * ld.z;
+        lo = fminf(lo, dp);
+        hi = fmaxf(hi, dp);
+    }
+#pragma unroll
+    for (int o = 1; o < QL; o <<= 1) {
+        lo = fminf(lo, __shfl_xor(lo, o, 64));
+        hi = fmaxf(hi, __shfl_xor(hi, o, 64));
+    }
+    mn = lo + off;
+    mx = hi + off;
+}
+template <int QL>
+__global__ __launch_bounds__(256) void k_narrow_quick(
+    int max_pairs, const int *__restrict__ num_pairs_ptr, const int2 *__restrict__ pairs,
+    const float4 *__restrict__ pos, const float4 *__restrict__ quat,
+    const float *__restrict__ inv_mass, const int *__restrict__ shape,
+    const HullInfo *__restrict__ hulls, const float4 *__restrict__ verts,
+    const float4 *__restrict__ planes, uint32_t *__restrict__ has_contact,
+    uint32_t *__restrict__ cand) {
+    int npairs = *num_pairs_ptr;
+    if (npairs > max_pairs) npairs = max_pairs;
+    const int sub = threadIdx.x & (QL - 1);
+    const int p = (int)((blockIdx.x * blockDim.x + threadIdx.x) / QL);
+    // a group's lanes share p, so they take the same branches
+    if (p >= npairs) return;
+    const int2 pr = pairs[p];
+    bool sep = false;
+    if (inv_mass[pr.x] != 0.f || inv_mass[pr.y] != 0.f) {
+        const HullInfo hA = hulls[shape[pr.x]];
+        const HullInfo hB = hulls[shape[pr.y]];
+        if (hA.num_vertices > 0 && hB.num_vertices > 0 && hull_limits_ok(hA, hB)) {
+            float4 pA = pos[pr.x], pB = pos[pr.y];
+            const float4 oA = quat[pr.x], oB = quat[pr.y];
+            pA.w = 0.f;
+            pB.w = 0.f;
+            const float4 cA = xform(hA.center, pA, oA), cB = xform(hB.center, pB, oB);
+            const float4 dc = sub4(cA, cB);
+            const float4 *pa = planes + hA.face_offset, *pb = planes + hB.face_offset;
+            // the faces: largest signed distance of the other centre (local
+            // frame); any choice keeps the result exact
+            const float4 lB = qt_inv_rotate(oA, sub4(cB, pA));
+            const float4 lA = qt_inv_rotate(oB, sub4(cA, pB));
+            float sa = -FLT_MAX, sb = -FLT_MAX;
+            int fa = 0x7fffffff, fb = 0x7fffffff;
+            for (int f = sub; f < hA.num_faces; f += QL) {
+                const float4 pl = pa[f];
+                const float d = dot3(pl, lB) + pl.w;
+                if (d > sa) { sa = d; fa = f; }
+            }
+            for (int f = sub; f < hB.num_faces; f += QL) {
+                const float4 pl = pb[f];
+                const float d = dot3(pl, lA) + pl.w;
+                if (d > sb) { sb = d; fb = f; }
+            }
+            group_argmax<QL>(sa, fa);
+            group_argmax<QL>(sb, fb);
+            if (fa >= hA.num_faces) fa = 0;  // NaN distances: any face will do
+            if (fb >= hB.num_faces) fb = 0;
+            const float4 *vA = verts + hA.vertex_offset, *vB = verts + hB.vertex_offset;
+            float mn0, mx0, mn1, mx1;
+            if (hA.num_faces > 0) {
+                float4 w = qt_rotate(oA, pa[fa]);
+                if (dot3(dc, w) < 0) w = scl4(w, -1.f);
+                project_group<QL>(vA, hA.num_vertices, sub, pA, oA, w, mn0, mx0);
+                project_group<QL>(vB, hB.num_vertices, sub, pB, oB, w, mn1, mx1);
+                sep = mx0 < mn1 || mx1 < mn0;
+            }
+            if (!sep && hB.num_faces > 0) {
+                float4 w = qt_rotate(oB, pb[fb]);
+                if (dot3(dc, w) < 0) w = scl4(w, -1.f);
+                project_group<QL>(vB, hB.num_vertices, sub, pB, oB, w, mn0, mx0);
+                project_group<QL>(vA, hA.num_vertices, sub, pA, oA, w, mn1, mx1);
+                sep = mx0 < mn1 || mx1 < mn0;
+            }
+        }
+    }
+    if (sub == 0) {
+        has_contact[p] = sep ? 0u : 1u;
+        if (sep) cand[p] = 0;
+    }
+}
+
 // Pass 1, every pair: face axes of A and B.  A pair separated by a face axis
 // (or static-static, or beyond the hull limits) has no contact; the others
 // become candidates for pass 2 with their face minimum (depth, axis) and the
 // number of edge-edge axes still to test, the cost key of pass 2's order.
 __global__ __launch_bounds__(NP_BLOCK) void k_narrow_face(
-    int max_pairs, const int *__restrict__ num_pairs_ptr, const int2 *__restrict__ pairs,
+    int max_pairs, const uint32_t *__restrict__ n_work_ptr, const uint32_t *__restrict__ work,
+    const int2 *__restrict__ pairs,
     const float4 *__restrict__ pos, const float4 *__restrict__ quat,
     const float *__restrict__ inv_mass, const int *__restrict__ shape,
     const HullInfo *__restrict__ hulls, const float4 *__restrict__ verts,
@@ -306,11 +424,10 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_face(
     __shared__ float sLB[4 * NP_BLOCK];  // their overlap bounds
     __shared__ int sQ[4 * NP_BLOCK];     // axes left to project
     const int lane = threadIdx.x;
-    int npairs = *num_pairs_ptr;
-    if (npairs > max_pairs) npairs = max_pairs;
+    int nwork = (int)min(*n_work_ptr, (uint32_t)max_pairs);
     // The pair and body loads of the next pair go out at the top of each
-    // iteration (and the pair index of the one after), so their latency
-    // overlaps the current pair's work.
+    // iteration (and the pair of the one after), so their latency overlaps
+    // the current pair's work.
     struct Head {
         int a, b, sA, sB;
         float imA, imB;
@@ -330,12 +447,19 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_face(
     };
     const int G = gridDim.x;
     Head cur, nxt;
-    int2 pr_next = (int)blockIdx.x + G < npairs ? pairs[blockIdx.x + G] : make_int2(0, 0);
+    int p_next = (int)blockIdx.x + G < nwork ? (int)work[blockIdx.x + G] : 0;
+    int2 pr_next = (int)blockIdx.x + G < nwork ? pairs[p_next] : make_int2(0, 0);
+    int p_after = 0;
     int2 pr_after = make_int2(0, 0);
-    if ((int)blockIdx.x < npairs) load_head(pairs[blockIdx.x], cur);
-    for (int p = blockIdx.x; p < npairs; p += G, cur = nxt, pr_next = pr_after) {
-        if (p + G < npairs) load_head(pr_next, nxt);
-        pr_after = p + 2 * G < npairs ? pairs[p + 2 * G] : make_int2(0, 0);
+    int p = (int)blockIdx.x < nwork ? (int)work[blockIdx.x] : 0;
+    if ((int)blockIdx.x < nwork) load_head(pairs[p], cur);
+    for (int i = blockIdx.x; i < nwork;
+         i += G, cur = nxt, p = p_next, p_next = p_after, pr_next = pr_after) {
+        if (i + G < nwork) load_head(pr_next, nxt);
+        if (i + 2 * G < nwork) {
+            p_after = (int)work[i + 2 * G];
+            pr_after = pairs[p_after];
+        }
         const int a = cur.a, b = cur.b;
         const float imA = cur.imA, imB = cur.imB;
         if (imA == 0.f && imB == 0.f) {
@@ -587,12 +711,35 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_clip(
             if (dist < best || (dist == best && ax < best_ax)) { best = dist; best_ax = ax; }
         };
         int qn = 0;
+        // The bound test takes the unit axis from the hardware's approximate
+        // reciprocal square root (error ~1 ulp, far below the margin) and
+        // orients it by the sign of dot(dc, c): the bound is that of
+        // sat_bound up to rounding.  The degenerate-axis test is axis_of's.
+        // Lane's axis k = base + lane as (e0, e1), stepped by 64 without a
+        // division per axis.
+        const int q64 = nEB > 0 ? NP_BLOCK / nEB : 0, r64 = NP_BLOCK - q64 * nEB;
+        int e0 = nEB > 0 ? lane / nEB : 0, e1 = lane - e0 * nEB;
         for (int base = 0; base < ne; base += NP_BLOCK) {
             const int k = base + lane;
             bool keep = false;
             if (k < ne) {
-                float4 c = axis_of(k);
-                keep = c.w == 0.f && !(sat_bound(rho, dc, use_dv, dv, c) - margin > face_d);
+                const float4 c = cross3(sEA[e0], sEB[e1]);
+                if (fabsf(c.x) > 1e-6f || fabsf(c.y) > 1e-6f || fabsf(c.z) > 1e-6f) {
+                    const float inv = __builtin_amdgcn_rsqf(dot3(c, c));
+                    float dd = dot3(dc, c) * inv, dvc = dot3(dv, c) * inv;
+                    if (dd < 0.f) {
+                        dd = -dd;
+                        dvc = -dvc;
+                    }
+                    const float lb1 = use_dv ? fmaxf(rho - dd, dvc) : rho - dd;
+                    keep = !(fminf(rho + dd, lb1) - margin > face_d);
+                }
+            }
+            e0 += q64;
+            e1 += r64;
+            if (e1 >= nEB) {
+                e1 -= nEB;
+                e0++;
             }
             const uint64_t m = __ballot(keep);
             if (keep) sQ[qn + __popcll(m & lanemask_lt())] = k;
@@ -1005,8 +1152,19 @@ __global__ void k_compound_total(int max_pairs, int *num_pairs, const uint32_t *
 hipError_t launch_narrowphase(const DeviceWorld &d, const StepParams &p, hipStream_t s,
                               const SortWork &sw, uint32_t *scan_status) {
     const int grid = rb_num_cus() * 4 * (p.narrow_waves > 0 ? p.narrow_waves : 4);
-    RB_LAUNCH(k_narrow_face, dim3(grid), dim3(NP_BLOCK), 0, s, p.max_pairs,
-                       d.counters + RB_CNT_PAIRS, d.pairs, d.pos, d.quat, d.inv_mass, d.shape,
+    // pass 1's pairs: the flags of pass 0 compacted into cand_kv, which
+    // k_narrow_keys fills only after pass 1
+    uint32_t *work = (uint32_t *)d.cand_kv;
+    uint32_t *n_work = (uint32_t *)(d.counters + RB_CNT_FACE_WORK);
+    RB_LAUNCH(k_narrow_quick<QUICK_LANES>,
+              dim3(((long long)p.max_pairs * QUICK_LANES + 255) / 256), dim3(256), 0, s,
+              p.max_pairs, d.counters + RB_CNT_PAIRS, d.pairs, d.pos, d.quat, d.inv_mass, d.shape,
+              d.hulls, d.verts, d.planes, d.has_contact, d.narrow_cand);
+    hipError_t e0 = launch_compact_flags(d.has_contact, work, p.max_pairs, n_work, scan_status, s,
+                                         d.counters + RB_CNT_PAIRS, d.counters + RB_CNT_OVERFLOW);
+    if (e0 != hipSuccess) return e0;
+    RB_LAUNCH(k_narrow_face, dim3(grid), dim3(NP_BLOCK), 0, s, p.max_pairs, n_work, work,
+                       d.pairs, d.pos, d.quat, d.inv_mass, d.shape,
                        d.hulls, d.verts, d.planes, d.has_contact, d.narrow_cand, d.face_best,
                        d.counters + RB_CNT_OVERFLOW);
     if (p.has_vertexless)

@@ -200,7 +200,8 @@ enum {
     RB_CNT_SUM_ITERS_N = 13,  // scene tiles with a tolerance: sweeps summed over tiles
     RB_CNT_SUM_ITERS_F = 14,
     RB_CNT_TILES_SOLVED = 15, // ... and the number of tiles that had rows
-    RB_CNT_COUNT = 16
+    RB_CNT_FACE_WORK = 16,    // uint32 pairs the quick face test left for the face pass
+    RB_CNT_COUNT = 17
 };
 
 // Step parameters passed by value to kernels.
@@ -273,9 +274,13 @@ size_t sort_status_words(int max_n, int digit_bits);
 // from device memory (min(n, *n_dev)), so the launch needs no host sync.
 hipError_t launch_radix_sort(uint2 *kv, int n, int sort_bits, int digit_bits, const SortWork &w,
                              hipStream_t s, const int *n_dev = nullptr);
+// out (may be null): exclusive prefix sums of in; idx_out (may be null, in
+// holding 0/1 flags): the indices of the set flags in increasing order
 hipError_t launch_exclusive_scan(const uint32_t *in, uint32_t *out, int n, uint32_t *total,
                                  uint32_t *status, hipStream_t s, const int *n_dev = nullptr,
-                                 int *err = nullptr);
+                                 int *err = nullptr, uint32_t *idx_out = nullptr);
+hipError_t launch_compact_flags(const uint32_t *flags, uint32_t *idx_out, int n, uint32_t *total,
+                                uint32_t *status, hipStream_t s, const int *n_dev, int *err);
 hipError_t launch_bound_search(const uint2 *kv, int n, uint32_t *dst, int n_dst, int mode,
                                hipStream_t s);
 

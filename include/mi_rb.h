@@ -115,6 +115,9 @@ typedef struct mi_rb_stats {
     float solve_kernel_ms;
     int solve_launches;
     float solve_events_ms;
+    /* narrowphase pairs that the quick face test (one face of each hull) did
+     * not separate: the pairs of the wave-per-pair face pass */
+    int face_pass_pairs;
 } mi_rb_stats;
 
 /* parameters derived at upload (grid, overflow list, solver tiles) */
