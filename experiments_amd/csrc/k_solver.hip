@@ -466,6 +466,10 @@ __device__ __forceinline__ void load_packed(int i, const PackedRows &pk, RowData
 
 // solveContact (solveContact.cl:272-312): sequential impulses on the normal
 // rows, lambda >= 0.
+// RES: compute the sweep residual terms (the largest scaled impulse change
+// and the largest impulse); launches whose sweep reports no residual skip
+// them, which leaves every velocity and impulse unchanged.
+template <bool RES = true>
 __device__ __forceinline__ void row_normal(RowData &d, float4 &vA, float4 &wA, float4 &vB,
                                            float4 &wB, const RowBodies &rb, float &rmax,
                                            float &lmax) {
@@ -495,15 +499,18 @@ __device__ __forceinline__ void row_normal(RowData &d, float4 &vA, float4 &wA, f
         float4 ai1 = scl4(mt_mul1(rb.IB, a1), rl);
         vA = add4(vA, li0); wA = add4(wA, ai0);
         vB = add4(vB, li1); wB = add4(wB, ai1);
-        float sc = fabsf(rl) / fabsf(jv[ic]);
-        rmax = fmaxf(rmax, sc);
-        lmax = fmaxf(lmax, fabsf(upd));
+        if (RES) {
+            float sc = fabsf(rl) / fabsf(jv[ic]);
+            rmax = fmaxf(rmax, sc);
+            lmax = fmaxf(lmax, fabsf(upd));
+        }
     }
     d.lam = mk4(lv[0], lv[1], lv[2], lv[3]);
 }
 
 // solveFriction (solveFriction.cl:297-430): two tangents at the manifold
 // centre, |lambda| <= mu * sum lambda_n, then the angular damping about n.
+template <bool RES = true>
 __device__ __forceinline__ void row_friction(RowData &d, float4 &vA, float4 &wA, float4 &vB,
                                              float4 &wB, const RowBodies &rb, float friction,
                                              float &rmax, float &lmax) {
@@ -538,9 +545,11 @@ __device__ __forceinline__ void row_friction(RowData &d, float4 &vA, float4 &wA,
         float4 ai1 = scl4(mt_mul1(rb.IB, a1), rl);
         vA = add4(vA, li0); wA = add4(wA, ai0);
         vB = add4(vB, li1); wB = add4(wB, ai1);
-        float sc = fabsf(rl) / fabsf(fj[i]);
-        rmax = fmaxf(rmax, sc);
-        lmax = fmaxf(lmax, fabsf(upd));
+        if (RES) {
+            float sc = fabsf(rl) / fabsf(fj[i]);
+            rmax = fmaxf(rmax, sc);
+            lmax = fmaxf(lmax, fabsf(upd));
+        }
     }
     d.fr = mk4(fr.x, fr.y, fl[0], fl[1]);
     float4 ab = normalize3(sub4(rb.pB, rb.pA));
@@ -735,6 +744,7 @@ __global__ __launch_bounds__(TB) void k_tile_pack(
 
 // One row of a sweep (mode 0 normal, 1 friction, 2 warm start), velocities
 // from the LDS cache or, for bodies that did not fit it, global memory.
+template <bool RES = true>
 __device__ __forceinline__ void run_row(int mode, RowData &d, const RowBodies &rb, SolveLDS &L,
                                         float4 *linvel, float4 *angvel, float friction,
                                         float &rmax, float &lmax) {
@@ -743,8 +753,8 @@ __device__ __forceinline__ void run_row(int mode, RowData &d, const RowBodies &r
     float4 wA = rb.ha >= 0 ? L.w[rb.ha] : (rb.ha == SLOT_GLOBAL ? angvel[rb.a] : z);
     float4 vB = rb.hb >= 0 ? L.v[rb.hb] : (rb.hb == SLOT_GLOBAL ? linvel[rb.b] : z);
     float4 wB = rb.hb >= 0 ? L.w[rb.hb] : (rb.hb == SLOT_GLOBAL ? angvel[rb.b] : z);
-    if (mode == 0) row_normal(d, vA, wA, vB, wB, rb, rmax, lmax);
-    else if (mode == 1) row_friction(d, vA, wA, vB, wB, rb, friction, rmax, lmax);
+    if (mode == 0) row_normal<RES>(d, vA, wA, vB, wB, rb, rmax, lmax);
+    else if (mode == 1) row_friction<RES>(d, vA, wA, vB, wB, rb, friction, rmax, lmax);
     else row_warm(d, vA, wA, vB, wB, rb);
     if (rb.ha >= 0) { L.v[rb.ha] = vA; L.w[rb.ha] = wA; }
     else if (rb.ha == SLOT_GLOBAL) { linvel[rb.a] = vA; angvel[rb.a] = wA; }
@@ -781,6 +791,7 @@ __device__ __forceinline__ void load_normal(int i, const PackedRows &pk, NormalR
     r.hb = m.w;
 }
 // row_normal on a NormalRow
+template <bool RES = true>
 __device__ __forceinline__ void row_normal_pre(NormalRow &d, float4 &vA, float4 &wA, float4 &vB,
                                                float4 &wB, float &rmax, float &lmax) {
     const float4 lin0 = d.lin0;
@@ -800,12 +811,15 @@ __device__ __forceinline__ void row_normal_pre(NormalRow &d, float4 &vA, float4 
         lv[ic] = upd;
         vA = add4(vA, scl4(d.l0, rl)); wA = add4(wA, scl4(d.ia0[ic], rl));
         vB = add4(vB, scl4(d.l1, rl)); wB = add4(wB, scl4(d.ia1[ic], rl));
-        float sc = fabsf(rl) / fabsf(jv[ic]);
-        rmax = fmaxf(rmax, sc);
-        lmax = fmaxf(lmax, fabsf(upd));
+        if (RES) {
+            float sc = fabsf(rl) / fabsf(jv[ic]);
+            rmax = fmaxf(rmax, sc);
+            lmax = fmaxf(lmax, fabsf(upd));
+        }
     }
     d.lam = mk4(lv[0], lv[1], lv[2], lv[3]);
 }
+template <bool RES = true>
 __device__ __forceinline__ void run_row_normal(NormalRow &d, SolveLDS &L, float4 *linvel,
                                                float4 *angvel, float &rmax, float &lmax) {
     const float4 z = mk4(0, 0, 0, 0);
@@ -813,7 +827,7 @@ __device__ __forceinline__ void run_row_normal(NormalRow &d, SolveLDS &L, float4
     float4 wA = d.ha >= 0 ? L.w[d.ha] : (d.ha == SLOT_GLOBAL ? angvel[d.a] : z);
     float4 vB = d.hb >= 0 ? L.v[d.hb] : (d.hb == SLOT_GLOBAL ? linvel[d.b] : z);
     float4 wB = d.hb >= 0 ? L.w[d.hb] : (d.hb == SLOT_GLOBAL ? angvel[d.b] : z);
-    row_normal_pre(d, vA, wA, vB, wB, rmax, lmax);
+    row_normal_pre<RES>(d, vA, wA, vB, wB, rmax, lmax);
     if (d.ha >= 0) { L.v[d.ha] = vA; L.w[d.ha] = wA; }
     else if (d.ha == SLOT_GLOBAL) { linvel[d.a] = vA; angvel[d.a] = wA; }
     if (d.hb >= 0) { L.v[d.hb] = vB; L.w[d.hb] = wB; }
@@ -826,7 +840,7 @@ __device__ __forceinline__ void run_row_normal(NormalRow &d, SolveLDS &L, float4
 // values and the result equals per-lane atomics).  They are needed after
 // every sweep with a tolerance, otherwise only for the last normal and the
 // last friction sweep, the residuals a step reports.
-__device__ __forceinline__ bool sweep_residual_needed(const StepParams &sp, int sweep) {
+__host__ __device__ __forceinline__ bool sweep_residual_needed(const StepParams &sp, int sweep) {
     return sp.tol > 0.f || sweep == sp.normal_iters - 1 ||
            sweep == sp.normal_iters + sp.friction_iters - 1;
 }
@@ -860,7 +874,7 @@ struct TileSolveLDS {
 // colour rows share no dynamic body, so the thread that runs a row does not
 // change the result.  local_tol: a scene tile ends each phase as soon as its
 // own residual is below sp.tol.
-template <int TBK, bool REG>
+template <int TBK, bool REG, bool RES>
 __device__ __forceinline__ void tile_sweeps(const StepParams &sp, int sweep0, int nsweeps_n,
                                             int nsweeps_f, int warm, bool local_tol, int first,
                                             int R, int nc, const int *coff, const PackedRows &pk,
@@ -881,7 +895,7 @@ __device__ __forceinline__ void tile_sweeps(const StepParams &sp, int sweep0, in
             for (int c = 0; c < nc; c++) {
                 if (REG) {
                     if (rc == c)
-                        run_row(mode, rd, rb, L.c, linvel, angvel, sp.friction, rmax, lmax);
+                        run_row<RES>(mode, rd, rb, L.c, linvel, angvel, sp.friction, rmax, lmax);
                 } else {
                     int c0 = coff[c], c1 = coff[c + 1];
                     if (c1 > R) c1 = R;
@@ -889,7 +903,7 @@ __device__ __forceinline__ void tile_sweeps(const StepParams &sp, int sweep0, in
                         RowData d;
                         RowBodies b;
                         load_packed(first + r, pk, d, b);
-                        run_row(mode, d, b, L.c, linvel, angvel, sp.friction, rmax, lmax);
+                        run_row<RES>(mode, d, b, L.c, linvel, angvel, sp.friction, rmax, lmax);
                         if (mode == 0) pk.lam[first + r] = d.lam;
                         else if (mode == 1) pk.fric[first + r] = d.fr;
                     }
@@ -899,7 +913,7 @@ __device__ __forceinline__ void tile_sweeps(const StepParams &sp, int sweep0, in
             if (phase < 0) continue;
             swept = true;
             const int sweep = sweep0 + (phase == 0 ? it : nsweeps_n + it);
-            if (sweep_residual_needed(sp, sweep)) add_sweep_residual(residual, sweep, rmax, lmax);
+            if (RES && sweep_residual_needed(sp, sweep)) add_sweep_residual(residual, sweep, rmax, lmax);
             if (local_tol) {
                 // rmax, lmax >= 0: their float bits order like the values
                 if (rmax > 0.f) atomicMax(&L.red[0], __float_as_uint(rmax));
@@ -922,7 +936,7 @@ __device__ __forceinline__ void tile_sweeps(const StepParams &sp, int sweep0, in
 
 // One normal sweep of a tile on precomputed rows (spatial tiles: a launch
 // per sweep and parity phase).  REG: row tid is nr, colour rc.
-template <int TBK, bool REG>
+template <int TBK, bool REG, bool RES>
 __device__ __forceinline__ void tile_normal_sweep(const StepParams &sp, int sweep, int first,
                                                   int R, int nc, const int *coff,
                                                   const PackedRows &pk,
@@ -933,20 +947,20 @@ __device__ __forceinline__ void tile_normal_sweep(const StepParams &sp, int swee
     float rmax = 0.f, lmax = 0.f;
     for (int c = 0; c < nc; c++) {
         if (REG) {
-            if (rc == c) run_row_normal(nr, L.c, linvel, angvel, rmax, lmax);
+            if (rc == c) run_row_normal<RES>(nr, L.c, linvel, angvel, rmax, lmax);
         } else {
             int c0 = coff[c], c1 = coff[c + 1];
             if (c1 > R) c1 = R;
             for (int r = c0 + tid; r < c1; r += TBK) {
                 NormalRow d;
                 load_normal(first + r, pk, d);
-                run_row_normal(d, L.c, linvel, angvel, rmax, lmax);
+                run_row_normal<RES>(d, L.c, linvel, angvel, rmax, lmax);
                 pk.lam[first + r] = d.lam;
             }
         }
         __syncthreads();
     }
-    if (sweep_residual_needed(sp, sweep)) add_sweep_residual(residual, sweep, rmax, lmax);
+    if (RES && sweep_residual_needed(sp, sweep)) add_sweep_residual(residual, sweep, rmax, lmax);
     if (REG && tid < R) pk.lam[first + tid] = nr.lam;
 }
 
@@ -955,7 +969,9 @@ __device__ __forceinline__ void tile_normal_sweep(const StepParams &sp, int swee
 // The solve of one tile (rows first..first+R of the packed arrays): its
 // dynamic bodies' velocities gathered into LDS, the sweeps, the velocities
 // stored back.  NORMAL: one normal sweep on the precomputed rows.
-template <int TBK, bool NORMAL>
+// RES: the launch's sweeps compute their residual terms (a launch of one
+// sweep whose residual is not reported runs without them)
+template <int TBK, bool NORMAL, bool RES>
 __device__ __forceinline__ void solve_tile(
     const StepParams &sp, int t, int sweep0, int nsweeps_n, int nsweeps_f, int warm,
     const int4 *__restrict__ tile_hdr, const int *__restrict__ tile_color_off,
@@ -990,11 +1006,11 @@ __device__ __forceinline__ void solve_tile(
         }
         __syncthreads();
         if (R <= TBK)
-            tile_normal_sweep<TBK, true>(sp, sweep0, first, R, nc, coff, pk, linvel, angvel,
-                                         residual, L, nr, rc);
+            tile_normal_sweep<TBK, true, RES>(sp, sweep0, first, R, nc, coff, pk, linvel, angvel,
+                                              residual, L, nr, rc);
         else
-            tile_normal_sweep<TBK, false>(sp, sweep0, first, R, nc, coff, pk, linvel, angvel,
-                                          residual, L, nr, rc);
+            tile_normal_sweep<TBK, false, RES>(sp, sweep0, first, R, nc, coff, pk, linvel,
+                                               angvel, residual, L, nr, rc);
         __syncthreads();
         for (int s = tid; s < nb; s += TBK) {
             int b = bodies[s];
@@ -1022,11 +1038,13 @@ __device__ __forceinline__ void solve_tile(
     int used[2] = {0, 0};
     float rfin[2] = {0.f, 0.f};
     if (R <= TBK)
-        tile_sweeps<TBK, true>(sp, sweep0, nsweeps_n, nsweeps_f, warm, local_tol, first, R, nc,
-                               coff, pk, linvel, angvel, residual, L, used, rfin, rd, rb, rc);
+        tile_sweeps<TBK, true, RES>(sp, sweep0, nsweeps_n, nsweeps_f, warm, local_tol, first, R,
+                                    nc, coff, pk, linvel, angvel, residual, L, used, rfin, rd, rb,
+                                    rc);
     else
-        tile_sweeps<TBK, false>(sp, sweep0, nsweeps_n, nsweeps_f, warm, local_tol, first, R, nc,
-                                coff, pk, linvel, angvel, residual, L, used, rfin, rd, rb, rc);
+        tile_sweeps<TBK, false, RES>(sp, sweep0, nsweeps_n, nsweeps_f, warm, local_tol, first, R,
+                                     nc, coff, pk, linvel, angvel, residual, L, used, rfin, rd,
+                                     rb, rc);
     if (local_tol && tid == 0)
         tile_stats[t] = make_int4(used[0], used[1], (int)__float_as_uint(rfin[0]),
                                   (int)__float_as_uint(rfin[1]));
@@ -1039,7 +1057,7 @@ __device__ __forceinline__ void solve_tile(
 }
 
 
-template <int TBK, bool NORMAL = false>
+template <int TBK, bool NORMAL = false, bool RES = true>
 __global__ __launch_bounds__(TBK) void k_tile_solve(
     StepParams sp, int phase, int sweep0, int nsweeps_n, int nsweeps_f, int warm,
     const int4 *__restrict__ tile_hdr, const int *__restrict__ tile_color_off,
@@ -1059,7 +1077,7 @@ __global__ __launch_bounds__(TBK) void k_tile_solve(
         if (tx >= sp.tiles_x || tz >= sp.tiles_z) return;
         t = tz * sp.tiles_x + tx;
     }
-    solve_tile<TBK, NORMAL>(sp, t, sweep0, nsweeps_n, nsweeps_f, warm, tile_hdr, tile_color_off,
+    solve_tile<TBK, NORMAL, RES>(sp, t, sweep0, nsweeps_n, nsweeps_f, warm, tile_hdr, tile_color_off,
                             tile_bodies, pk, linvel, angvel, residual, counters, tile_stats, L);
 }
 
@@ -1251,12 +1269,18 @@ hipError_t launch_solve(const DeviceWorld &d, const StepParams &p, hipStream_t s
                 if (sw < 0)
                     RB_LAUNCH(k_tile_solve<TB>, dim3(nx * nz), dim3(TB), 0, s, p, ph, 0, 0, 0,
                                        1, RB_SOLVE_ARGS);
-                else if (!fr)
+                else if (!fr && sweep_residual_needed(p, sw))
                     RB_LAUNCH((k_tile_solve<TB, true>), dim3(nx * nz), dim3(TB), 0, s, p,
                                        ph, sw, 1, 0, 0, RB_SOLVE_ARGS);
-                else
+                else if (!fr)
+                    RB_LAUNCH((k_tile_solve<TB, true, false>), dim3(nx * nz), dim3(TB), 0, s, p,
+                              ph, sw, 1, 0, 0, RB_SOLVE_ARGS);
+                else if (sweep_residual_needed(p, sw))
                     RB_LAUNCH(k_tile_solve<TB>, dim3(nx * nz), dim3(TB), 0, s, p, ph, sw,
                                        0, 1, 0, RB_SOLVE_ARGS);
+                else
+                    RB_LAUNCH((k_tile_solve<TB, false, false>), dim3(nx * nz), dim3(TB), 0, s,
+                              p, ph, sw, 0, 1, 0, RB_SOLVE_ARGS);
                 mark(1);
             }
             if (sw >= 0 && p.tol > 0.f) {
