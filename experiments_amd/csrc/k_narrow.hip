@@ -184,6 +184,20 @@ __device__ __forceinline__ float4 support_diff(const float4 *__restrict__ verts,
 }
 
 constexpr int NP_BLOCK = 64;
+// wave-max key of a projection: float order as signed integers, the lane in
+// the low 6 bits (ties and the dropped bits go to the higher lane); a NaN
+// projection ranks below every number and a lane without a vertex below all
+__device__ __forceinline__ int probe_key(float v, int lane, bool valid) {
+    int f = __float_as_int(v);
+    f = f >= 0 ? f : (f ^ 0x7fffffff);
+    if (!(v >= -FLT_MAX)) f = INT_MIN + 64;
+    if (f < INT_MIN + 64) f = INT_MIN + 64;
+    if (!valid) f = INT_MIN;
+    return (f & ~63) | lane;
+}
+// probe directions of the clip pass's second edge-axis bound: dc, four at
+// 0.35 and four at 0.8 (rotated by 45 degrees) around it
+constexpr int NPROBE = 9;
 static_assert(RB_MAX_HULL_VERTS <= RB_MAX_POLY, "a face's side planes fit the point buffer");
 // face axes up to which pass 1 projects every axis instead of ordering them
 // by an overlap bound first
@@ -680,10 +694,51 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_clip(
         const float face_d = __int_as_float(fb0.x);
         const float margin = sat_margin(pA, pB);
         const float rho = hA.center.w + hB.center.w;
+        // Pairs with many edge axes take a second bound from probe vertices:
+        // for any vertex sB of B and sA of A, max_B(c) - min_A(c) >=
+        // dot(sB - sA, c), so with the vertices furthest along NPROBE
+        // directions around dc (B) and against them (A), max over the B
+        // probes minus min over the A probes bounds that overlap term for
+        // every axis near dc, where the competitive edge axes are.  Probe
+        // positions are taken relative to B's origin (no large coordinates).
+        const bool use_probe = ne > NP_BLOCK;
+        float4 *sProbe = sPts;  // B's probes, then A's (sPts is free until the clipping)
+        if (use_probe) {
+            float4 n = dot3(dc, dc) > 0.f ? normalize3(dc) : mk4(0.f, 1.f, 0.f, 0.f);
+            float4 e1 = normalize3(cross3(n, fabsf(n.x) < 0.9f ? mk4(1.f, 0.f, 0.f, 0.f)
+                                                                 : mk4(0.f, 1.f, 0.f, 0.f)));
+            float4 e2 = cross3(n, e1);
+            const float4 dAB = sub4(pA, pB);
+            const float4 wB4 = lane < hB.num_vertices
+                                   ? qt_rotate(oB, verts[hB.vertex_offset + lane])
+                                   : mk4(0.f, 0.f, 0.f, 0.f);
+            const float4 wA4 = lane < hA.num_vertices
+                                   ? add4(qt_rotate(oA, verts[hA.vertex_offset + lane]), dAB)
+                                   : mk4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+            for (int k = 0; k < NPROBE; k++) {
+                const float t = k == 0 ? 0.f : (k <= 4 ? 0.35f : 0.8f * 0.70710678f);
+                const float a = k <= 4 ? ((k & 1) ? 1.f : 0.f) * ((k & 2) ? -1.f : 1.f) : ((k & 1) ? 1.f : -1.f);
+                const float b = k <= 4 ? ((k & 1) ? 0.f : 1.f) * ((k & 2) ? -1.f : 1.f) : ((k & 2) ? 1.f : -1.f);
+                const float4 u = add4(n, add4(scl4(e1, t * a), scl4(e2, t * b)));
+                // Any vertex gives a valid bound, so the furthest one is
+                // found on one ordered integer key per lane (the projection's
+                // ordered bits with the lane in the low 6 bits); a NaN
+                // projection counts as none and lanes without a vertex lose.
+                const int ib =
+                    wave_max_i32(probe_key(dot3(wB4, u), lane, lane < hB.num_vertices)) & 63;
+                const int ia =
+                    wave_max_i32(probe_key(-dot3(wA4, u), lane, lane < hA.num_vertices)) & 63;
+                if (lane == ib) sProbe[k] = wB4;
+                if (lane == ia) sProbe[NPROBE + k] = wA4;
+            }
+            __syncthreads();
+        }
         // (the vertex bound only pays for pairs with many edge axes)
         const bool use_dv = ne > 512;
-        const float4 dv = use_dv ? support_diff(verts, hA, hB, pA, oA, pB, oB, dc, lane)
-                                 : mk4(0.f, 0.f, 0.f, 0.f);
+        const float4 dv = !use_dv ? mk4(0.f, 0.f, 0.f, 0.f)
+                          : use_probe ? sub4(sProbe[0], sProbe[NPROBE])
+                                      : support_diff(verts, hA, hB, pA, oA, pB, oB, dc, lane);
         // The bound test runs on all lanes, one axis each; the surviving
         // axes are compacted into an LDS queue and projected 64 at a time, so
         // the projection loops run with full waves instead of with the few
@@ -709,6 +764,47 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_clip(
             float dist = d0 < d1 ? d0 : d1;
             int ax = nf + k;
             if (dist < best || (dist == best && ax < best_ax)) { best = dist; best_ax = ax; }
+        };
+        // Stage 2 (pairs with probes): the probe bound over up to 64 queued
+        // axes of sQ, one per lane; survivors go to sP, projected 64 at a
+        // time.  Without probes every queued axis survives.
+        int *sP = (int *)&sPoly[0][0];  // (sPoly is free until the clipping)
+        int pn = 0;
+        auto stage2 = [&](int cnt) {
+            const int k = lane < cnt ? sQ[lane] : 0;
+            bool keep = lane < cnt;
+            if (keep && use_probe) {
+                const int e0 = k / nEB, e1 = k - e0 * nEB;
+                float4 c = cross3(sEA[e0], sEB[e1]);
+                const float inv = __builtin_amdgcn_rsqf(dot3(c, c));
+                c = scl4(c, inv);
+                float dd = dot3(dc, c);
+                if (dd < 0.f) {
+                    c = neg4(c);
+                    dd = -dd;
+                }
+                float hb = -FLT_MAX, la = FLT_MAX;
+#pragma unroll
+                for (int j = 0; j < NPROBE; j++) {
+                    hb = fmaxf(hb, dot3(sProbe[j], c));
+                    la = fminf(la, dot3(sProbe[NPROBE + j], c));
+                }
+                const float t2 = fmaxf(rho - dd, hb - la);
+                keep = !(fminf(rho + dd, t2) - margin > face_d);
+            }
+            const uint64_t m = __ballot(keep);
+            if (keep) sP[pn + __popcll(m & lanemask_lt())] = k;
+            pn += __popcll(m);
+            __syncthreads();
+            if (pn >= NP_BLOCK) {
+                project_axis(sP[lane]);
+                const int rest = pn - NP_BLOCK;
+                const int moved = lane < rest ? sP[NP_BLOCK + lane] : 0;
+                __syncthreads();
+                if (lane < rest) sP[lane] = moved;
+                pn = rest;
+                __syncthreads();
+            }
         };
         int qn = 0;
         // The bound test takes the unit axis from the hardware's approximate
@@ -746,7 +842,7 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_clip(
             qn += __popcll(m);
             __syncthreads();
             if (qn >= NP_BLOCK) {
-                project_axis(sQ[lane]);
+                stage2(NP_BLOCK);
                 int rest = qn - NP_BLOCK;
                 int moved = lane < rest ? sQ[NP_BLOCK + lane] : 0;
                 __syncthreads();
@@ -755,13 +851,14 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_clip(
                 __syncthreads();
             }
         }
-        if (qn > 0) {
-            // the last partial batch: 64 / next_pow2(qn) lanes per axis
-            const int lg = lanes_log2(qn);
+        if (qn > 0) stage2(qn);
+        if (pn > 0) {
+            // the last partial batch: 64 / next_pow2(pn) lanes per axis
+            const int lg = lanes_log2(pn);
             const int slot = lane >> lg, part = lane & ((1 << lg) - 1);
-            const int k = sQ[slot < qn ? slot : 0];
+            const int k = sP[slot < pn ? slot : 0];
             if (lg == 0) {
-                if (slot < qn) project_axis(k);
+                if (slot < pn) project_axis(k);
             } else {
                 float4 c = axis_of(k);
                 float mn0, mx0, mn1, mx1;
@@ -770,7 +867,7 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_clip(
                 float d0 = mx0 - mn1, d1 = mx1 - mn0;
                 float dist = d0 < d1 ? d0 : d1;
                 int ax = nf + k;
-                if (slot < qn && part == 0 &&
+                if (slot < pn && part == 0 &&
                     (dist < best || (dist == best && ax < best_ax))) {
                     best = dist;
                     best_ax = ax;
