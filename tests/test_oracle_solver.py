@@ -291,3 +291,38 @@ def test_friction_is_clamped_to_the_cone():
     f = cs["f_lambda"][0]
     assert np.max(np.abs(f)) == lim
     assert lv[0, 0] > 4.0 - 2 * lim - 1e-6
+
+
+def test_solve_segments_is_solve_per_segment():
+    """orc_solve_segments (the GPU's tile order: independent segments in one
+    call) equals orc_solve run on each segment's row order in turn, and a
+    segment's rows touch only its own bodies, so the segments' order does
+    not matter.  Two stacks of two boxes on their own grounds, sliding."""
+    dt = np.float32(1 / 60)
+    rng = np.random.default_rng(3)
+    # bodies: grounds 0 and 3, stack A = 1, 2 on 0; stack B = 4, 5 on 3
+    pos = np.array([[0, -0.5, 0], [0, 0.5, 0], [0, 1.5, 0],
+                    [9, -0.5, 0], [9, 0.5, 0], [9, 1.5, 0]], np.float32)
+    im = np.array([0, 1, 0.5, 0, 2, 1], np.float32)
+    I = world_inertia(6, [(0, 0, 0), (6, 6, 6), (3, 3, 3), (0, 0, 0), (12, 12, 12), (6, 6, 6)])
+    v0 = rng.normal(size=(6, 3)).astype(np.float32) * 0.5
+    v0[[0, 3]] = 0
+    w0 = rng.normal(size=(6, 3)).astype(np.float32) * 0.3
+    w0[[0, 3]] = 0
+    cs = []
+    for a, b, x in ((1, 0, 0), (2, 1, 0), (4, 3, 9), (5, 4, 9)):
+        c = ground_contact(0.01, body=a)
+        c["body_b"] = b
+        c["pos"][0, :, 0] += x
+        c["pos"][0, :, 1] += pos[a, 1] - 0.5
+        cs.append(O.contact_to_constraint(c, pos, v0, w0, im, I, dt))
+    cons = np.concatenate(cs)
+    segs = [np.array([1, 0], np.int32), np.array([3, 2], np.int32)]
+    got = O.solve_segments(cons, segs, v0, w0, pos, im, I, 7, 5, friction=0.6)
+    c1, lv, av = cons, v0, w0
+    for seg in segs:
+        c1, lv, av, _, _ = O.solve(c1, seg, lv, av, pos, im, I, 7, 5, friction=0.6)
+    assert np.array_equal(got[0], c1)
+    assert np.array_equal(got[1], lv) and np.array_equal(got[2], av)
+    rev = O.solve_segments(cons, segs[::-1], v0, w0, pos, im, I, 7, 5, friction=0.6)
+    assert np.array_equal(rev[1], lv) and np.array_equal(rev[0], c1)
