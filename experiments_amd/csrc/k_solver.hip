@@ -425,6 +425,28 @@ struct SolveLDS {
 constexpr int SLOT_STATIC = -1;    // static body: zero velocity, no writes
 constexpr int SLOT_GLOBAL = -2;    // dynamic body beyond the LDS cache: global memory
 
+// A row body's velocities by its slot.  The LDS read is unconditional (slot
+// clamped to 0) and the rare global and static cases overwrite it, so the
+// compiler keeps an LDS read (ds_read) instead of merging the two sources
+// into one generic-address load through a select of pointers (a flat load,
+// whose latency is that of the vector memory path, plus a zero vector
+// materialised in scratch).
+__device__ __forceinline__ void fetch_vel(int h, int b, const SolveLDS &L,
+                                          const float4 *__restrict__ linvel,
+                                          const float4 *__restrict__ angvel, float4 &v,
+                                          float4 &w) {
+    const int s = h >= 0 ? h : 0;
+    v = L.v[s];
+    w = L.w[s];
+    if (h < 0) {
+        // (both slots read the body's global velocity and select by value)
+        const float4 gv = linvel[b], gw = angvel[b];
+        const bool g = h == SLOT_GLOBAL;
+        v = g ? gv : mk4(0.f, 0.f, 0.f, 0.f);
+        w = g ? gw : mk4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
 // Constraint fields one row needs (Constraint4 minus the body indices).
 struct RowData {
     float4 lin0, jac, bb, lam, fr, cen;
@@ -748,11 +770,9 @@ template <bool RES = true>
 __device__ __forceinline__ void run_row(int mode, RowData &d, const RowBodies &rb, SolveLDS &L,
                                         float4 *linvel, float4 *angvel, float friction,
                                         float &rmax, float &lmax) {
-    const float4 z = mk4(0, 0, 0, 0);
-    float4 vA = rb.ha >= 0 ? L.v[rb.ha] : (rb.ha == SLOT_GLOBAL ? linvel[rb.a] : z);
-    float4 wA = rb.ha >= 0 ? L.w[rb.ha] : (rb.ha == SLOT_GLOBAL ? angvel[rb.a] : z);
-    float4 vB = rb.hb >= 0 ? L.v[rb.hb] : (rb.hb == SLOT_GLOBAL ? linvel[rb.b] : z);
-    float4 wB = rb.hb >= 0 ? L.w[rb.hb] : (rb.hb == SLOT_GLOBAL ? angvel[rb.b] : z);
+    float4 vA, wA, vB, wB;
+    fetch_vel(rb.ha, rb.a, L, linvel, angvel, vA, wA);
+    fetch_vel(rb.hb, rb.b, L, linvel, angvel, vB, wB);
     if (mode == 0) row_normal<RES>(d, vA, wA, vB, wB, rb, rmax, lmax);
     else if (mode == 1) row_friction<RES>(d, vA, wA, vB, wB, rb, friction, rmax, lmax);
     else row_warm(d, vA, wA, vB, wB, rb);
@@ -822,11 +842,9 @@ __device__ __forceinline__ void row_normal_pre(NormalRow &d, float4 &vA, float4 
 template <bool RES = true>
 __device__ __forceinline__ void run_row_normal(NormalRow &d, SolveLDS &L, float4 *linvel,
                                                float4 *angvel, float &rmax, float &lmax) {
-    const float4 z = mk4(0, 0, 0, 0);
-    float4 vA = d.ha >= 0 ? L.v[d.ha] : (d.ha == SLOT_GLOBAL ? linvel[d.a] : z);
-    float4 wA = d.ha >= 0 ? L.w[d.ha] : (d.ha == SLOT_GLOBAL ? angvel[d.a] : z);
-    float4 vB = d.hb >= 0 ? L.v[d.hb] : (d.hb == SLOT_GLOBAL ? linvel[d.b] : z);
-    float4 wB = d.hb >= 0 ? L.w[d.hb] : (d.hb == SLOT_GLOBAL ? angvel[d.b] : z);
+    float4 vA, wA, vB, wB;
+    fetch_vel(d.ha, d.a, L, linvel, angvel, vA, wA);
+    fetch_vel(d.hb, d.b, L, linvel, angvel, vB, wB);
     row_normal_pre<RES>(d, vA, wA, vB, wB, rmax, lmax);
     if (d.ha >= 0) { L.v[d.ha] = vA; L.w[d.ha] = wA; }
     else if (d.ha == SLOT_GLOBAL) { linvel[d.a] = vA; angvel[d.a] = wA; }
