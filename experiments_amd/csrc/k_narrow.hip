@@ -198,6 +198,18 @@ __device__ __forceinline__ int probe_key(float v, int lane, bool valid) {
 // probe directions of the clip pass's second edge-axis bound: dc, four at
 // 0.35 and four at 0.8 (rotated by 45 degrees) around it
 constexpr int NPROBE = 9;
+// the small-hull clip pass (k_narrow_clip_small): SG lanes per candidate for
+// hulls of at most SG_VERTS vertices
+constexpr int SG = 16;
+constexpr int SG_VERTS = 8;
+constexpr int SG_EDGES = 18;
+constexpr int SG_FACES = 16;
+constexpr int SG_POLY = 2 * SG_VERTS;
+constexpr int SG_GROUPS = NP_BLOCK / SG;
+static_assert(SG_POLY <= SG, "one lane per polygon vertex");
+__device__ __forceinline__ bool small_hull(const HullInfo &h) {
+    return h.num_vertices <= SG_VERTS && h.num_edges <= SG_EDGES && h.num_faces <= SG_FACES;
+}
 static_assert(RB_MAX_HULL_VERTS <= RB_MAX_POLY, "a face's side planes fit the point buffer");
 // face axes up to which pass 1 projects every axis instead of ordering them
 // by an overlap bound first
@@ -612,12 +624,14 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_face(
         wave_argmin(best, best_ax);
         if (lane == 0) {
             cand[p] = 1;
-            face_best[p] = make_int4(__float_as_int(best), best_ax, hA.num_edges * hB.num_edges, 0);
+            face_best[p] = make_int4(__float_as_int(best), best_ax, hA.num_edges * hB.num_edges,
+                                     small_hull(hA) && small_hull(hB) ? 1 : 0);
         }
     }
 }
 
-// Candidates in descending order of edge-edge axis count (stable).
+// Candidates in descending order of edge-edge axis count (stable), pairs of
+// small hulls (the SG-lane clip pass) after the others.
 __global__ __launch_bounds__(256) void k_narrow_keys(int max_pairs, const int *__restrict__ num_pairs_ptr,
                                                      const uint32_t *__restrict__ cand,
                                                      const uint32_t *__restrict__ cand_off,
@@ -627,8 +641,11 @@ __global__ __launch_bounds__(256) void k_narrow_keys(int max_pairs, const int *_
     if (npairs > max_pairs) npairs = max_pairs;
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npairs || !cand[p]) return;
-    int ne = face_best[p].z;
-    kv[cand_off[p]] = make_uint2(0xffffu - (uint32_t)(ne < 0xffff ? ne : 0xffff), (uint32_t)p);
+    // large pairs first, each class by descending edge-axis count (at most
+    // 128 x 128 < 0x8000)
+    const int4 fb = face_best[p];
+    const uint32_t ne = (uint32_t)(fb.z < 0x7fff ? fb.z : 0x7fff);
+    kv[cand_off[p]] = make_uint2((fb.w ? 0x8000u : 0u) | (0x7fffu - ne), (uint32_t)p);
 }
 
 // Pass 2, the candidates: edge x edge axes (never reject), then clipping of
@@ -637,7 +654,7 @@ __global__ __launch_bounds__(256) void k_narrow_keys(int max_pairs, const int *_
 // cost-sorted list (round r: workgroup b takes r*G + b, or r*G + G-1-b on odd
 // rounds), which spreads the expensive pairs over all workgroups.
 __global__ __launch_bounds__(NP_BLOCK) void k_narrow_clip(
-    const uint32_t *__restrict__ n_cand_ptr, const uint2 *__restrict__ order,
+    const int *__restrict__ n_cand_ptr, const uint2 *__restrict__ order,
     const int4 *__restrict__ face_best, const int2 *__restrict__ pairs,
     const float4 *__restrict__ pos, const float4 *__restrict__ quat,
     const float *__restrict__ inv_mass, const int *__restrict__ shape,
@@ -1082,6 +1099,449 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_clip(
     }
 }
 
+// ---- pass 2 for small hulls: SG lanes per candidate ----
+// Candidates whose hulls both have at most 8 vertices (cubes, the 8-vertex
+// hulls: at most 18 unique edges, 16 faces, 8 vertices in a face) run SG = 16
+// lanes each, four per wave, instead of a wave each: their edge axes (at most
+// 324), faces and clipped polygons (at most 8 + 8 points) fit 16 lanes, and a
+// wave per pair left most lanes idle.  Every step is pass 2's with the same
+// arithmetic and the same tie rules, so the manifold is identical; the lanes
+// of a group take the same branches, reductions stay inside the group, and
+// LDS is the group's own.
+
+struct SmallClipLDS {
+    float4 ea[SG_EDGES], eb[SG_EDGES];
+    float4 va[SG_VERTS], vb[SG_VERTS];
+    float4 poly[2][SG_POLY];
+    float4 pts[2 * NPROBE];  // probes, then the side planes, then the kept points
+    int q[2 * SG], pq[2 * SG];
+};
+
+// LDS written by lanes of this wave, then read by others of it
+__device__ __forceinline__ void group_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+// the group's bits of a wave ballot, and the group lanes below this one
+__device__ __forceinline__ uint32_t group_ballot(bool x, int gbase) {
+    return (uint32_t)((__ballot(x) >> gbase) & ((1ull << SG) - 1ull));
+}
+__device__ __forceinline__ void group_argmin(float &d, int &idx) {
+#pragma unroll
+    for (int o = SG / 2; o >= 1; o >>= 1) {
+        float od = __shfl_xor(d, o, 64);
+        int oi = __shfl_xor(idx, o, 64);
+        if (od < d || (od == d && oi < idx)) { d = od; idx = oi; }
+    }
+}
+__device__ __forceinline__ void group_argmax(float &d, int &idx) {
+#pragma unroll
+    for (int o = SG / 2; o >= 1; o >>= 1) {
+        float od = __shfl_xor(d, o, 64);
+        int oi = __shfl_xor(idx, o, 64);
+        if (od > d || (od == d && oi < idx)) { d = od; idx = oi; }
+    }
+}
+__device__ __forceinline__ int group_max_i32(int v) {
+#pragma unroll
+    for (int o = SG / 2; o >= 1; o >>= 1) {
+        int ov = __shfl_xor(v, o, 64);
+        v = ov > v ? ov : v;
+    }
+    return v;
+}
+// log2 of the lanes per axis for a batch of cnt <= SG axes
+__device__ __forceinline__ int group_lanes_log2(int cnt) {
+    return cnt > 8 ? 0 : cnt > 4 ? 1 : cnt > 2 ? 2 : cnt > 1 ? 3 : 4;
+}
+
+__global__ __launch_bounds__(NP_BLOCK) void k_narrow_clip_small(
+    const uint32_t *__restrict__ n_cand_ptr, const int *__restrict__ split_ptr,
+    const uint2 *__restrict__ order, const int4 *__restrict__ face_best,
+    const int2 *__restrict__ pairs, const float4 *__restrict__ pos,
+    const float4 *__restrict__ quat, const float *__restrict__ inv_mass,
+    const int *__restrict__ shape, const HullInfo *__restrict__ hulls,
+    const float4 *__restrict__ verts, const float4 *__restrict__ planes,
+    const int2 *__restrict__ face_index, const int *__restrict__ indices,
+    const float4 *__restrict__ edges, float friction, float max_dist, Contact4 *__restrict__ slots,
+    uint32_t *__restrict__ has_contact, const int *__restrict__ owner, int *overflow,
+    int edge_reject, int reduce_mode) {
+    __shared__ SmallClipLDS SL[SG_GROUPS];
+    const int lane = threadIdx.x;
+    const int grp = lane / SG, gl = lane & (SG - 1), gbase = grp * SG;
+    const uint32_t lt = (1u << gl) - 1u;
+    SmallClipLDS &S = SL[grp];
+    const int first = *split_ptr;
+    const int n_cand = (int)*n_cand_ptr;
+    const int W = (int)gridDim.x * SG_GROUPS;
+    const int wid = (int)blockIdx.x * SG_GROUPS + grp;
+    for (int r = 0;; r++) {
+        const int i = first + r * W + ((r & 1) ? (W - 1 - wid) : wid);
+        if (i >= n_cand) break;
+        const int p = (int)order[i].y;
+        const int2 pr = pairs[p];
+        const int a = pr.x, b = pr.y;
+        const float imA = inv_mass[a], imB = inv_mass[b];
+        const HullInfo hA = hulls[shape[a]];
+        const HullInfo hB = hulls[shape[b]];
+        float4 pA = pos[a], pB = pos[b];
+        const float4 oA = quat[a], oB = quat[b];
+        pA.w = 0.f;
+        pB.w = 0.f;
+        const int nvA = hA.num_vertices, nvB = hB.num_vertices;
+        group_sync();  // the previous candidate's reads of S are done
+        if (gl < hA.num_edges) S.ea[gl] = qt_rotate(oA, edges[hA.edge_offset + gl]);
+        if (gl + SG < hA.num_edges) S.ea[gl + SG] = qt_rotate(oA, edges[hA.edge_offset + gl + SG]);
+        if (gl < hB.num_edges) S.eb[gl] = qt_rotate(oB, edges[hB.edge_offset + gl]);
+        if (gl + SG < hB.num_edges) S.eb[gl + SG] = qt_rotate(oB, edges[hB.edge_offset + gl + SG]);
+        if (gl < nvA) S.va[gl] = verts[hA.vertex_offset + gl];
+        if (gl < nvB) S.vb[gl] = verts[hB.vertex_offset + gl];
+        group_sync();
+
+        const float4 dc = sub4(xform(hA.center, pA, oA), xform(hB.center, pB, oB));
+        float best = FLT_MAX;
+        int best_ax = 0x7fffffff;
+        const int nf = hA.num_faces + hB.num_faces;
+        const int ne = hA.num_edges * hB.num_edges;
+        const int4 fb0 = face_best[p];
+        const float face_d = __int_as_float(fb0.x);
+        const float margin = sat_margin(pA, pB);
+        const float rho = hA.center.w + hB.center.w;
+        // probe vertices (as pass 2 takes them) for pairs with many edge axes
+        const bool use_probe = ne > NP_BLOCK;
+        if (use_probe) {
+            float4 n = dot3(dc, dc) > 0.f ? normalize3(dc) : mk4(0.f, 1.f, 0.f, 0.f);
+            float4 e1 = normalize3(cross3(n, fabsf(n.x) < 0.9f ? mk4(1.f, 0.f, 0.f, 0.f)
+                                                                 : mk4(0.f, 1.f, 0.f, 0.f)));
+            float4 e2 = cross3(n, e1);
+            const float4 dAB = sub4(pA, pB);
+            const float4 wB4 = gl < nvB ? qt_rotate(oB, S.vb[gl]) : mk4(0.f, 0.f, 0.f, 0.f);
+            const float4 wA4 = gl < nvA ? add4(qt_rotate(oA, S.va[gl]), dAB) : mk4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+            for (int k = 0; k < NPROBE; k++) {
+                const float t = k == 0 ? 0.f : (k <= 4 ? 0.35f : 0.8f * 0.70710678f);
+                const float a2 = k <= 4 ? ((k & 1) ? 1.f : 0.f) * ((k & 2) ? -1.f : 1.f) : ((k & 1) ? 1.f : -1.f);
+                const float b2 = k <= 4 ? ((k & 1) ? 0.f : 1.f) * ((k & 2) ? -1.f : 1.f) : ((k & 2) ? 1.f : -1.f);
+                const float4 u = add4(n, add4(scl4(e1, t * a2), scl4(e2, t * b2)));
+                const int ib = group_max_i32(probe_key(dot3(wB4, u), gl, gl < nvB)) & (SG - 1);
+                const int ia = group_max_i32(probe_key(-dot3(wA4, u), gl, gl < nvA)) & (SG - 1);
+                if (gl == ib) S.pts[k] = wB4;
+                if (gl == ia) S.pts[NPROBE + k] = wA4;
+            }
+            group_sync();
+        }
+        const int nEB = hB.num_edges;
+        auto axis_of = [&](int k) {
+            int e0 = k / nEB, e1 = k - e0 * nEB;
+            float4 c = cross3(S.ea[e0], S.eb[e1]);
+            bool ok = fabsf(c.x) > 1e-6f || fabsf(c.y) > 1e-6f || fabsf(c.z) > 1e-6f;
+            if (ok) {
+                c = normalize3(c);
+                if (dot3(dc, c) < 0) c = scl4(c, -1.f);
+            }
+            return ok ? c : mk4(0.f, 0.f, 0.f, -1.f);
+        };
+        // axis k on 2^lg lanes (part of them), min into (best, best_ax)
+        auto project_axis = [&](int k, int lg, int part, bool valid) {
+            const float4 c = axis_of(k);
+            float mn0, mx0, mn1, mx1;
+            project_part(S.va, nvA, part, lg, pA, oA, c, mn0, mx0);
+            project_part(S.vb, nvB, part, lg, pB, oB, c, mn1, mx1);
+            const float d0 = mx0 - mn1, d1 = mx1 - mn0;
+            const float dist = d0 < d1 ? d0 : d1;
+            const int ax = nf + k;
+            if (valid && part == 0 && (dist < best || (dist == best && ax < best_ax))) {
+                best = dist;
+                best_ax = ax;
+            }
+        };
+        int pn = 0;
+        auto stage2 = [&](int cnt) {
+            const int k = gl < cnt ? S.q[gl] : 0;
+            bool keep = gl < cnt;
+            if (keep && use_probe) {
+                const int e0 = k / nEB, e1 = k - e0 * nEB;
+                float4 c = cross3(S.ea[e0], S.eb[e1]);
+                const float inv = __builtin_amdgcn_rsqf(dot3(c, c));
+                c = scl4(c, inv);
+                float dd = dot3(dc, c);
+                if (dd < 0.f) {
+                    c = neg4(c);
+                    dd = -dd;
+                }
+                float hb = -FLT_MAX, la = FLT_MAX;
+#pragma unroll
+                for (int j = 0; j < NPROBE; j++) {
+                    hb = fmaxf(hb, dot3(S.pts[j], c));
+                    la = fminf(la, dot3(S.pts[NPROBE + j], c));
+                }
+                const float t2 = fmaxf(rho - dd, hb - la);
+                keep = !(fminf(rho + dd, t2) - margin > face_d);
+            }
+            const uint32_t m = group_ballot(keep, gbase);
+            if (keep) S.pq[pn + __popc(m & lt)] = k;
+            pn += __popc(m);
+            group_sync();
+            if (pn >= SG) {
+                project_axis(S.pq[gl], 0, 0, true);
+                const int rest = pn - SG;
+                const int moved = gl < rest ? S.pq[SG + gl] : 0;
+                group_sync();
+                if (gl < rest) S.pq[gl] = moved;
+                pn = rest;
+                group_sync();
+            }
+        };
+        int qn = 0;
+        // the first bound, as pass 2 takes it (no vertex bound: ne <= 324)
+        const int qs = nEB > 0 ? SG / nEB : 0, rs = SG - qs * nEB;
+        int e0 = nEB > 0 ? gl / nEB : 0, e1 = gl - e0 * nEB;
+        for (int base = 0; base < ne; base += SG) {
+            const int k = base + gl;
+            bool keep = false;
+            if (k < ne) {
+                const float4 c = cross3(S.ea[e0], S.eb[e1]);
+                if (fabsf(c.x) > 1e-6f || fabsf(c.y) > 1e-6f || fabsf(c.z) > 1e-6f) {
+                    const float inv = __builtin_amdgcn_rsqf(dot3(c, c));
+                    float dd = dot3(dc, c) * inv;
+                    if (dd < 0.f) dd = -dd;
+                    keep = !(fminf(rho + dd, rho - dd) - margin > face_d);
+                }
+            }
+            e0 += qs;
+            e1 += rs;
+            if (e1 >= nEB) {
+                e1 -= nEB;
+                e0++;
+            }
+            const uint32_t m = group_ballot(keep, gbase);
+            if (keep) S.q[qn + __popc(m & lt)] = k;
+            qn += __popc(m);
+            group_sync();
+            if (qn >= SG) {
+                stage2(SG);
+                const int rest = qn - SG;
+                const int moved = gl < rest ? S.q[SG + gl] : 0;
+                group_sync();
+                if (gl < rest) S.q[gl] = moved;
+                qn = rest;
+                group_sync();
+            }
+        }
+        if (qn > 0) stage2(qn);
+        if (pn > 0) {
+            const int lg = group_lanes_log2(pn);
+            const int slot = gl >> lg, part = gl & ((1 << lg) - 1);
+            project_axis(S.pq[slot < pn ? slot : 0], lg, part, slot < pn);
+        }
+        group_argmin(best, best_ax);
+        if (face_d < best || (face_d == best && fb0.y < best_ax)) { best = face_d; best_ax = fb0.y; }
+        if (best_ax == 0x7fffffff || (edge_reject && best < 0.f)) {
+            if (gl == 0) has_contact[p] = 0;
+            continue;
+        }
+        float4 sep;
+        if (best_ax < hA.num_faces) {
+            sep = qt_rotate(oA, planes[hA.face_offset + best_ax]);
+            if (dot3(dc, sep) < 0) sep = scl4(sep, -1.f);
+        } else if (best_ax < nf) {
+            sep = qt_rotate(oB, planes[hB.face_offset + best_ax - hA.num_faces]);
+            if (dot3(dc, sep) < 0) sep = scl4(sep, -1.f);
+        } else {
+            const int k = best_ax - nf;
+            const int f0 = k / nEB, f1 = k - f0 * nEB;
+            sep = normalize3(cross3(S.ea[f0], S.eb[f1]));
+            if (dot3(dc, sep) < 0) sep = scl4(sep, -1.f);
+        }
+        if (dot3(neg4(dc), sep) > 0.0f) sep = neg4(sep);
+
+        // ---- clipping: incident face of B, reference face of A ----
+        float dB = -FLT_MAX;
+        int fB = 0x7fffffff;
+        if (gl < hB.num_faces) {
+            const float4 pl = planes[hB.face_offset + gl];
+            const float d = dot3(qt_rotate(oB, mk4(pl.x, pl.y, pl.z, 0.f)), sep);
+            if (d > dB) { dB = d; fB = gl; }
+        }
+        group_argmax(dB, fB);
+        float dA = FLT_MAX;
+        int fA = 0x7fffffff;
+        if (gl < hA.num_faces) {
+            const float4 pl = planes[hA.face_offset + gl];
+            const float d = dot3(qt_rotate(oA, mk4(pl.x, pl.y, pl.z, 0.f)), sep);
+            if (d < dA) { dA = d; fA = gl; }
+        }
+        group_argmin(dA, fA);
+        if (fB == 0x7fffffff || fA == 0x7fffffff) {
+            if (gl == 0) has_contact[p] = 0;
+            continue;
+        }
+        const int2 fiB = face_index[hB.face_offset + fB];
+        int nin = fiB.y < SG_VERTS ? fiB.y : SG_VERTS;
+        if (gl < nin) S.poly[0][gl] = xform(S.vb[indices[fiB.x + gl]], pB, oB);
+        const int2 fiA = face_index[hA.face_offset + fA];
+        const float4 plA = planes[hA.face_offset + fA];
+        const float4 wpn = qt_rotate(oA, mk4(plA.x, plA.y, plA.z, 0.f));
+        float4 *sSide = S.pts;
+        const int nside = fiA.y < SG_VERTS ? fiA.y : SG_VERTS;
+        if (gl < nside) {
+            const float4 va = S.va[indices[fiA.x + gl]];
+            const float4 vb = S.va[indices[fiA.x + (gl + 1) % fiA.y]];
+            const float4 we = qt_rotate(oA, sub4(va, vb));
+            const float4 pn4 = neg4(cross3(we, wpn));
+            const float4 wa = xform(va, pA, oA);
+            sSide[gl] = mk4(pn4.x, pn4.y, pn4.z, -dot3(wa, pn4));
+        }
+        int cur = 0;
+        bool poly_over = fiB.y > SG_VERTS || fiA.y > SG_VERTS;
+        group_sync();
+        for (int s0 = 0; s0 < nside; s0++) {
+            const float4 side = sSide[s0];
+            const float4 pn4 = mk4(side.x, side.y, side.z, 0.f);
+            const float eq = side.w;
+            const float4 *in = S.poly[cur];
+            float4 *outp = S.poly[cur ^ 1];
+            int cnt = 0;
+            float4 o0 = mk4(0, 0, 0, 0), o1 = o0;
+            if (gl < nin) {
+                const float4 f0 = in[gl == 0 ? nin - 1 : gl - 1];
+                const float4 end = in[gl];
+                const float ds = dot3(pn4, f0) + eq;
+                const float de = dot3(pn4, end) + eq;
+                if (ds < 0) {
+                    if (de < 0) {
+                        o0 = end; cnt = 1;
+                    } else {
+                        float t = ds * 1.f / (ds - de);
+                        o0 = mk4(f0.x + (end.x - f0.x) * t, f0.y + (end.y - f0.y) * t,
+                                 f0.z + (end.z - f0.z) * t, 0.f);
+                        cnt = 1;
+                    }
+                } else if (de < 0) {
+                    float t = ds * 1.f / (ds - de);
+                    o0 = mk4(f0.x + (end.x - f0.x) * t, f0.y + (end.y - f0.y) * t,
+                             f0.z + (end.z - f0.z) * t, 0.f);
+                    o1 = end;
+                    cnt = 2;
+                }
+            }
+            const uint32_t b1 = group_ballot(cnt >= 1, gbase), b2 = group_ballot(cnt == 2, gbase);
+            const int off = __popc(b1 & lt) + __popc(b2 & lt);
+            const int total = __popc(b1) + __popc(b2);
+            if (total > SG_POLY) poly_over = true;
+            if (cnt >= 1 && off < SG_POLY) outp[off] = o0;
+            if (cnt == 2 && off + 1 < SG_POLY) outp[off + 1] = o1;
+            group_sync();
+            nin = total < SG_POLY ? total : SG_POLY;
+            cur ^= 1;
+        }
+        if (poly_over && gl == 0) atomicOr(overflow, (int)MI_RB_OVERFLOW_POLYGON);
+        const float peq = plA.w - dot3(wpn, pA);
+        int keep = 0;
+        float4 kp = mk4(0, 0, 0, 0);
+        if (gl < nin) {
+            const float4 q = S.poly[cur][gl];
+            float depth = dot3(wpn, q) + peq;
+            if (depth <= -1e30f) depth = -1e30f;
+            if (depth <= max_dist) { keep = 1; kp = mk4(q.x, q.y, q.z, depth); }
+        }
+        const uint32_t kb = group_ballot(keep, gbase);
+        const int npts = __popc(kb);
+        float4 *sPts = S.pts;
+        group_sync();  // every lane has read the side planes
+        if (keep) sPts[__popc(kb & lt)] = kp;
+        group_sync();
+        if (npts == 0) {
+            if (gl == 0) has_contact[p] = 0;
+            continue;
+        }
+        // ---- reduction to <= 4 points (pass 2's, on the group) ----
+        const float4 nrm = neg4(sep);
+        int idx[4] = {0, 1, 2, 3};
+        int nr = npts;
+        if (reduce_mode == MI_RB_REDUCE_SPLIT_PATH) {
+            if (npts > 4) {
+                float4 c = sPts[0];
+                for (int j = 1; j < npts; j++) c = add4(c, sPts[j]);
+                const float fn = (float)npts;
+                c = mk4(c.x / fn, c.y / fn, c.z / fn, c.w / fn);
+                const float4 av = sub4(sPts[0], c);
+                float4 u = cross3(nrm, av);
+                float4 v = cross3(nrm, u);
+                u = normalize3(u);
+                v = normalize3(v);
+                float f[4] = {FLT_MAX, FLT_MAX, FLT_MAX, FLT_MAX};
+                float dw = FLT_MAX;
+                int iw = 0x7fffffff;
+                if (gl < npts) {
+                    const float4 q = sPts[gl];
+                    const float4 rr = sub4(q, c);
+                    f[0] = dot3(u, rr); f[1] = dot3(neg4(u), rr);
+                    f[2] = dot3(v, rr); f[3] = dot3(neg4(v), rr);
+                    for (int k = 0; k < 4; k++)
+                        if (!(f[k] < FLT_MIN)) f[k] = FLT_MAX;
+                    dw = q.w;
+                    iw = gl;
+                }
+                for (int k = 0; k < 4; k++) {
+                    int ik = f[k] < FLT_MAX ? gl : 0x7fffffff;
+                    float fk = f[k];
+                    group_argmin(fk, ik);
+                    if (ik != 0x7fffffff) idx[k] = ik;
+                }
+                group_argmin(dw, iw);
+                if (idx[0] != iw && idx[1] != iw && idx[2] != iw && idx[3] != iw) idx[0] = iw;
+                nr = 4;
+            }
+        } else if (npts >= 4) {
+            float4 c = sPts[0];
+            for (int j = 1; j < npts; j++) c = add4(c, sPts[j]);
+            const float fn = (float)npts;
+            c = mk4(c.x / fn, c.y / fn, c.z / fn, c.w / fn);
+            const float4 av = sub4(sPts[0], c);
+            float4 u = cross3(nrm, av);
+            float4 v = cross3(nrm, u);
+            u = normalize3(u);
+            v = normalize3(v);
+            int key[4] = {INT_MIN, INT_MIN, INT_MIN, INT_MIN};
+            if (gl < npts) {
+                const float4 rr = sub4(sPts[gl], c);
+                const float f[4] = {dot3(u, rr), dot3(neg4(u), rr), dot3(v, rr), dot3(neg4(v), rr)};
+                for (int k = 0; k < 4; k++)
+                    key[k] = (int)((__float_as_uint(f[k]) & 0xffffff00u) | (0xffu & (uint32_t)gl));
+            }
+            for (int k = 0; k < 4; k++) idx[k] = group_max_i32(key[k]) & 0xff;
+            nr = 4;
+        }
+        if (gl == 0) {
+            Contact4 ct;
+            ct.normal = mk4(nrm.x, nrm.y, nrm.z, (float)nr);
+            for (int k = 0; k < 4; k++) ct.pos[k] = k < nr ? sPts[idx[k]] : mk4(0, 0, 0, 0);
+            ct.coeffs = (uint32_t)(0.f * 0xffff) | ((uint32_t)(friction * 0xffff) << 16);
+            ct.pair_index = p;
+            const int oa = owner[a], ob = owner[b];
+            ct.body_a = imA == 0.f ? -oa : oa;
+            ct.body_b = imB == 0.f ? -ob : ob;
+            slots[p] = ct;
+            has_contact[p] = 1;
+        }
+    }
+}
+
+// First position of the small-hull candidates in the cost-sorted order (the
+// sort key's bit 15; small candidates sort after the others).
+__global__ __launch_bounds__(256) void k_clip_split(const uint32_t *__restrict__ n_cand_ptr,
+                                                    const uint2 *__restrict__ order, int *split) {
+    const int n = (int)*n_cand_ptr;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t s = (order[i].x >> 15) & 1u;
+    const uint32_t prev = i > 0 ? (order[i - 1].x >> 15) & 1u : 0u;
+    if (s && !prev) *split = i;
+    if (i == n - 1 && !s) *split = n;
+}
+
 // ---- compound shapes (findCompoundPairsKernel, sat.cl:763-925) ----
 // Proxy body n + k is child proxy[k].y of body proxy[k].x: its world
 // transform is the body's times the child's (qtRotate(orn, childPos) + pos,
@@ -1278,12 +1738,23 @@ hipError_t launch_narrowphase(const DeviceWorld &d, const StepParams &p, hipStre
                        d.face_best, d.cand_kv);
     e = launch_radix_sort(d.cand_kv, p.max_pairs, 16, 8, sw, s, d.counters + RB_CNT_NARROW_CAND);
     if (e != hipSuccess) return e;
+    // large-hull candidates [0, split) a wave each, then the small-hull ones
+    // [split, n) SG lanes each
+    RB_LAUNCH(k_clip_split, dim3((p.max_pairs + 255) / 256), dim3(256), 0, s,
+              (const uint32_t *)(d.counters + RB_CNT_NARROW_CAND), d.cand_kv,
+              d.counters + RB_CNT_CLIP_SPLIT);
     RB_LAUNCH(k_narrow_clip, dim3(grid), dim3(NP_BLOCK), 0, s,
-                       (const uint32_t *)(d.counters + RB_CNT_NARROW_CAND), d.cand_kv, d.face_best,
+                       d.counters + RB_CNT_CLIP_SPLIT, d.cand_kv, d.face_best,
                        d.pairs, d.pos, d.quat, d.inv_mass, d.shape, d.hulls, d.verts, d.planes,
                        d.face_index, d.indices, d.edges, p.friction, p.threshold, d.contact_slots,
                        d.has_contact, d.owner, d.counters + RB_CNT_OVERFLOW, p.sat_edge_reject,
                        p.reduce_mode);
+    RB_LAUNCH(k_narrow_clip_small, dim3(grid), dim3(NP_BLOCK), 0, s,
+              (const uint32_t *)(d.counters + RB_CNT_NARROW_CAND), d.counters + RB_CNT_CLIP_SPLIT,
+              d.cand_kv, d.face_best, d.pairs, d.pos, d.quat, d.inv_mass, d.shape, d.hulls,
+              d.verts, d.planes, d.face_index, d.indices, d.edges, p.friction, p.threshold,
+              d.contact_slots, d.has_contact, d.owner, d.counters + RB_CNT_OVERFLOW,
+              p.sat_edge_reject, p.reduce_mode);
     return rb_launch_status();
 }
 

@@ -201,7 +201,8 @@ enum {
     RB_CNT_SUM_ITERS_F = 14,
     RB_CNT_TILES_SOLVED = 15, // ... and the number of tiles that had rows
     RB_CNT_FACE_WORK = 16,    // uint32 pairs the quick face test left for the face pass
-    RB_CNT_COUNT = 17
+    RB_CNT_CLIP_SPLIT = 17,   // first small-hull candidate in the clip order
+    RB_CNT_COUNT = 18
 };
 
 // Step parameters passed by value to kernels.
