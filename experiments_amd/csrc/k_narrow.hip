@@ -210,6 +210,45 @@ static_assert(SG_POLY <= SG, "one lane per polygon vertex");
 __device__ __forceinline__ bool small_hull(const HullInfo &h) {
     return h.num_vertices <= SG_VERTS && h.num_edges <= SG_EDGES && h.num_faces <= SG_FACES;
 }
+// LDS written by lanes of this wave, then read by others of it
+__device__ __forceinline__ void group_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+// the group's bits of a wave ballot, and the group lanes below this one
+__device__ __forceinline__ uint32_t group_ballot(bool x, int gbase) {
+    return (uint32_t)((__ballot(x) >> gbase) & ((1ull << SG) - 1ull));
+}
+__device__ __forceinline__ void group_argmin(float &d, int &idx) {
+#pragma unroll
+    for (int o = SG / 2; o >= 1; o >>= 1) {
+        float od = __shfl_xor(d, o, 64);
+        int oi = __shfl_xor(idx, o, 64);
+        if (od < d || (od == d && oi < idx)) { d = od; idx = oi; }
+    }
+}
+__device__ __forceinline__ void group_argmax(float &d, int &idx) {
+#pragma unroll
+    for (int o = SG / 2; o >= 1; o >>= 1) {
+        float od = __shfl_xor(d, o, 64);
+        int oi = __shfl_xor(idx, o, 64);
+        if (od > d || (od == d && oi < idx)) { d = od; idx = oi; }
+    }
+}
+__device__ __forceinline__ int group_max_i32(int v) {
+#pragma unroll
+    for (int o = SG / 2; o >= 1; o >>= 1) {
+        int ov = __shfl_xor(v, o, 64);
+        v = ov > v ? ov : v;
+    }
+    return v;
+}
+// log2 of the lanes per axis for a batch of cnt <= SG axes
+__device__ __forceinline__ int group_lanes_log2(int cnt) {
+    return cnt > 8 ? 0 : cnt > 4 ? 1 : cnt > 2 ? 2 : cnt > 1 ? 3 : 4;
+}
+
 static_assert(RB_MAX_HULL_VERTS <= RB_MAX_POLY, "a face's side planes fit the point buffer");
 // face axes up to which pass 1 projects every axis instead of ordering them
 // by an overlap bound first
@@ -322,10 +361,12 @@ __global__ __launch_bounds__(256) void k_narrow_special(
 // that separates here separates in pass 1 too, and pass 1 (which projects
 // every face axis, or leaves out only axes that cannot separate) would give
 // the pair no contact: it gets none here.  The other pairs (and every pair
-// that is not hull-hull within the limits) are flagged in has_contact for
-// pass 1, which takes them in pair order from the compacted flags and sets
-// has_contact itself (pass 2 for its candidates).  Most broadphase pairs are
-// apart, so pass 1's wave per pair runs on a fraction of them.  With QL > 1
+// that is not hull-hull within the limits) are flagged for pass 1 (pairs of
+// two small hulls in cand for its SG-lane kernel, the others in has_contact
+// for its wave-per-pair kernel), which takes them in pair order from the
+// compacted flags and sets cand and has_contact itself (pass 2 for its
+// candidates).  Most broadphase pairs are apart, so pass 1 runs on a
+// fraction of them.  With QL > 1
 // the lanes of a pair split its faces and vertices.
 // lanes per pair of pass 0 (4 timed faster than 1 and 8 on the 65,537-hull
 // scene: fewer cache lines per load than 1, less repeated work than 8)
@@ -377,11 +418,12 @@ __global__ __launch_bounds__(256) void k_narrow_quick(
     // a group's lanes share p, so they take the same branches
     if (p >= npairs) return;
     const int2 pr = pairs[p];
-    bool sep = false;
+    bool sep = false, small = false;
     if (inv_mass[pr.x] != 0.f || inv_mass[pr.y] != 0.f) {
         const HullInfo hA = hulls[shape[pr.x]];
         const HullInfo hB = hulls[shape[pr.y]];
         if (hA.num_vertices > 0 && hB.num_vertices > 0 && hull_limits_ok(hA, hB)) {
+            small = small_hull(hA) && small_hull(hB);
             float4 pA = pos[pr.x], pB = pos[pr.y];
             const float4 oA = quat[pr.x], oB = quat[pr.y];
             pA.w = 0.f;
@@ -427,9 +469,11 @@ __global__ __launch_bounds__(256) void k_narrow_quick(
             }
         }
     }
+    // listed pairs: has_contact flags the wave-per-pair face pass, cand the
+    // small-hull one (each compacted into its part of the work list)
     if (sub == 0) {
-        has_contact[p] = sep ? 0u : 1u;
-        if (sep) cand[p] = 0;
+        has_contact[p] = !sep && !small ? 1u : 0u;
+        cand[p] = !sep && small ? 1u : 0u;
     }
 }
 
@@ -626,6 +670,81 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_face(
             cand[p] = 1;
             face_best[p] = make_int4(__float_as_int(best), best_ax, hA.num_edges * hB.num_edges,
                                      small_hull(hA) && small_hull(hB) ? 1 : 0);
+        }
+    }
+}
+
+// Pass 1 for pairs of two small hulls (listed by pass 0 after the others):
+// SG lanes per pair, four pairs per wave.  The face axes (at most 32) are
+// those of the wave kernel's plain path, built and projected with the same
+// arithmetic (two axes per lane, every vertex of both hulls from the
+// group's LDS), so the separation and the lexicographic (overlap, axis)
+// minimum are the same.
+__global__ __launch_bounds__(NP_BLOCK) void k_narrow_face_small(
+    int max_pairs, const uint32_t *__restrict__ n_large_ptr,
+    const uint32_t *__restrict__ n_small_ptr, const uint32_t *__restrict__ work,
+    const int2 *__restrict__ pairs, const float4 *__restrict__ pos,
+    const float4 *__restrict__ quat, const float *__restrict__ inv_mass,
+    const int *__restrict__ shape, const HullInfo *__restrict__ hulls,
+    const float4 *__restrict__ verts, const float4 *__restrict__ planes,
+    uint32_t *__restrict__ has_contact, uint32_t *__restrict__ cand,
+    int4 *__restrict__ face_best) {
+    __shared__ float4 sVA[SG_GROUPS][SG_VERTS], sVB[SG_GROUPS][SG_VERTS];
+    const int lane = threadIdx.x;
+    const int grp = lane / SG, gl = lane & (SG - 1), gbase = grp * SG;
+    const uint32_t n_large = *n_large_ptr;
+    const uint32_t n_small = *n_small_ptr;
+    int first = (int)min(n_large, (uint32_t)max_pairs);
+    int end = (int)min(n_large + n_small, (uint32_t)max_pairs);
+    const int W = (int)gridDim.x * SG_GROUPS;
+    for (int i = first + (int)blockIdx.x * SG_GROUPS + grp; i < end; i += W) {
+        const int p = (int)work[i];
+        const int2 pr = pairs[p];
+        const HullInfo hA = hulls[shape[pr.x]];
+        const HullInfo hB = hulls[shape[pr.y]];
+        float4 pA = pos[pr.x], pB = pos[pr.y];
+        const float4 oA = quat[pr.x], oB = quat[pr.y];
+        pA.w = 0.f;
+        pB.w = 0.f;
+        const int nvA = hA.num_vertices, nvB = hB.num_vertices;
+        group_sync();  // the previous pair's reads of the group's vertices are done
+        if (gl < nvA) sVA[grp][gl] = verts[hA.vertex_offset + gl];
+        if (gl < nvB) sVB[grp][gl] = verts[hB.vertex_offset + gl];
+        group_sync();
+        const float4 dc = sub4(xform(hA.center, pA, oA), xform(hB.center, pB, oB));
+        const int nfA = hA.num_faces, nf = nfA + hB.num_faces;
+        bool sep_found = false;
+        float best = FLT_MAX;
+        int best_ax = 0x7fffffff;
+        for (int ax = gl; ax < nf; ax += SG) {
+            const bool onA = ax < nfA;
+            const float4 nrm = onA ? planes[hA.face_offset + ax] : planes[hB.face_offset + ax - nfA];
+            float4 w = qt_rotate(onA ? oA : oB, nrm);
+            if (dot3(dc, w) < 0) w = scl4(w, -1.f);
+            float mn0, mx0, mn1, mx1;
+            if (onA) {
+                project_part(sVA[grp], nvA, 0, 0, pA, oA, w, mn0, mx0);
+                project_part(sVB[grp], nvB, 0, 0, pB, oB, w, mn1, mx1);
+            } else {
+                project_part(sVB[grp], nvB, 0, 0, pB, oB, w, mn0, mx0);
+                project_part(sVA[grp], nvA, 0, 0, pA, oA, w, mn1, mx1);
+            }
+            if (mx0 < mn1 || mx1 < mn0) {
+                sep_found = true;
+            } else {
+                const float d0 = mx0 - mn1, d1 = mx1 - mn0;
+                const float d = d0 < d1 ? d0 : d1;
+                if (d < best || (d == best && ax < best_ax)) { best = d; best_ax = ax; }
+            }
+        }
+        if (group_ballot(sep_found, gbase)) {
+            if (gl == 0) { has_contact[p] = 0; cand[p] = 0; }
+            continue;
+        }
+        group_argmin(best, best_ax);
+        if (gl == 0) {
+            cand[p] = 1;
+            face_best[p] = make_int4(__float_as_int(best), best_ax, hA.num_edges * hB.num_edges, 1);
         }
     }
 }
@@ -1116,45 +1235,6 @@ struct SmallClipLDS {
     float4 pts[2 * NPROBE];  // probes, then the side planes, then the kept points
     int q[2 * SG], pq[2 * SG];
 };
-
-// LDS written by lanes of this wave, then read by others of it
-__device__ __forceinline__ void group_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-// the group's bits of a wave ballot, and the group lanes below this one
-__device__ __forceinline__ uint32_t group_ballot(bool x, int gbase) {
-    return (uint32_t)((__ballot(x) >> gbase) & ((1ull << SG) - 1ull));
-}
-__device__ __forceinline__ void group_argmin(float &d, int &idx) {
-#pragma unroll
-    for (int o = SG / 2; o >= 1; o >>= 1) {
-        float od = __shfl_xor(d, o, 64);
-        int oi = __shfl_xor(idx, o, 64);
-        if (od < d || (od == d && oi < idx)) { d = od; idx = oi; }
-    }
-}
-__device__ __forceinline__ void group_argmax(float &d, int &idx) {
-#pragma unroll
-    for (int o = SG / 2; o >= 1; o >>= 1) {
-        float od = __shfl_xor(d, o, 64);
-        int oi = __shfl_xor(idx, o, 64);
-        if (od > d || (od == d && oi < idx)) { d = od; idx = oi; }
-    }
-}
-__device__ __forceinline__ int group_max_i32(int v) {
-#pragma unroll
-    for (int o = SG / 2; o >= 1; o >>= 1) {
-        int ov = __shfl_xor(v, o, 64);
-        v = ov > v ? ov : v;
-    }
-    return v;
-}
-// log2 of the lanes per axis for a batch of cnt <= SG axes
-__device__ __forceinline__ int group_lanes_log2(int cnt) {
-    return cnt > 8 ? 0 : cnt > 4 ? 1 : cnt > 2 ? 2 : cnt > 1 ? 3 : 4;
-}
 
 __global__ __launch_bounds__(NP_BLOCK) void k_narrow_clip_small(
     const uint32_t *__restrict__ n_cand_ptr, const int *__restrict__ split_ptr,
@@ -1717,13 +1797,21 @@ hipError_t launch_narrowphase(const DeviceWorld &d, const StepParams &p, hipStre
               dim3(((long long)p.max_pairs * QUICK_LANES + 255) / 256), dim3(256), 0, s,
               p.max_pairs, d.counters + RB_CNT_PAIRS, d.pairs, d.pos, d.quat, d.inv_mass, d.shape,
               d.hulls, d.verts, d.planes, d.has_contact, d.narrow_cand);
+    // the wave kernel's pairs, then the small-hull kernel's after them
+    uint32_t *n_small = (uint32_t *)(d.counters + RB_CNT_FACE_SMALL);
     hipError_t e0 = launch_compact_flags(d.has_contact, work, p.max_pairs, n_work, scan_status, s,
                                          d.counters + RB_CNT_PAIRS, d.counters + RB_CNT_OVERFLOW);
+    if (e0 != hipSuccess) return e0;
+    e0 = launch_compact_flags(d.narrow_cand, work, p.max_pairs, n_small, scan_status, s,
+                              d.counters + RB_CNT_PAIRS, d.counters + RB_CNT_OVERFLOW, n_work);
     if (e0 != hipSuccess) return e0;
     RB_LAUNCH(k_narrow_face, dim3(grid), dim3(NP_BLOCK), 0, s, p.max_pairs, n_work, work,
                        d.pairs, d.pos, d.quat, d.inv_mass, d.shape,
                        d.hulls, d.verts, d.planes, d.has_contact, d.narrow_cand, d.face_best,
                        d.counters + RB_CNT_OVERFLOW);
+    RB_LAUNCH(k_narrow_face_small, dim3(grid), dim3(NP_BLOCK), 0, s, p.max_pairs, n_work, n_small,
+              work, d.pairs, d.pos, d.quat, d.inv_mass, d.shape, d.hulls, d.verts, d.planes,
+              d.has_contact, d.narrow_cand, d.face_best);
     if (p.has_vertexless)
         RB_LAUNCH(k_narrow_special, dim3(rb_num_cus() * 4), dim3(256), 0, s, p.max_pairs,
                            d.counters + RB_CNT_PAIRS, d.pairs, d.pos, d.quat, d.inv_mass, d.shape,

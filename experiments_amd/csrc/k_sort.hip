@@ -234,7 +234,8 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan(const uint32_t *in, uint32_
                                                      uint32_t *total, uint64_t *status,
                                                      uint32_t *tile_ctr, const int *n_dev,
                                                      uint32_t *err, uint32_t err_bit,
-                                                     uint32_t *idx_out) {
+                                                     uint32_t *idx_out,
+                                                     const uint32_t *idx_base) {
     if (n_dev && *n_dev < n) n = *n_dev;
     __shared__ uint32_t s_wave[SCAN_BLOCK / 64];
     __shared__ uint32_t s_tile;
@@ -293,11 +294,12 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan(const uint32_t *in, uint32_
     }
     __syncthreads();
     uint32_t run = (uint32_t)s_prefix + wave_off + (incl - sum);
+    const uint32_t ibase = idx_out && idx_base ? *idx_base : 0u;
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; k++) {
         int i = base + k;
         if (out && i < n) out[i] = run;
-        if (idx_out && v[k]) idx_out[run] = (uint32_t)i;  // flags: the set indices, in order
+        if (idx_out && v[k]) idx_out[ibase + run] = (uint32_t)i;  // flags: the set indices, in order
         run += v[k];
     }
     int last_tile = (n - 1) / SCAN_TILE;
@@ -410,7 +412,7 @@ hipError_t launch_radix_sort(uint2 *kv, int n, int sort_bits, int digit_bits, co
 
 hipError_t launch_exclusive_scan(const uint32_t *in, uint32_t *out, int n, uint32_t *total,
                                  uint32_t *status, hipStream_t s, const int *n_dev, int *err,
-                                 uint32_t *idx_out) {
+                                 uint32_t *idx_out, const uint32_t *idx_base) {
     if (n <= 0) {
         RB_LAUNCH(k_scan_empty, dim3(1), dim3(1), 0, s, total);
         return rb_launch_status();
@@ -425,13 +427,15 @@ hipError_t launch_exclusive_scan(const uint32_t *in, uint32_t *out, int n, uint3
     uint32_t *e = err ? reinterpret_cast<uint32_t *>(err) : status + 1;
     const uint32_t bit = err ? (uint32_t)MI_RB_OVERFLOW_LOOKBACK : 1u;
     RB_LAUNCH(k_scan, dim3(tiles), dim3(SCAN_BLOCK), 0, s, in, out, n, total, st, ctr,
-                       n_dev, e, bit, idx_out);
+                       n_dev, e, bit, idx_out, idx_base);
     return rb_launch_status();
 }
 
 hipError_t launch_compact_flags(const uint32_t *flags, uint32_t *idx_out, int n, uint32_t *total,
-                                uint32_t *status, hipStream_t s, const int *n_dev, int *err) {
-    return launch_exclusive_scan(flags, nullptr, n, total, status, s, n_dev, err, idx_out);
+                                uint32_t *status, hipStream_t s, const int *n_dev, int *err,
+                                const uint32_t *idx_base) {
+    return launch_exclusive_scan(flags, nullptr, n, total, status, s, n_dev, err, idx_out,
+                                 idx_base);
 }
 
 hipError_t launch_bound_search(const uint2 *kv, int n, uint32_t *dst, int n_dst, int mode,

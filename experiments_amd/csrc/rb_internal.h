@@ -202,7 +202,8 @@ enum {
     RB_CNT_TILES_SOLVED = 15, // ... and the number of tiles that had rows
     RB_CNT_FACE_WORK = 16,    // uint32 pairs the quick face test left for the face pass
     RB_CNT_CLIP_SPLIT = 17,   // first small-hull candidate in the clip order
-    RB_CNT_COUNT = 18
+    RB_CNT_FACE_SMALL = 18,   // uint32 small-hull pairs of the face pass (after FACE_WORK's)
+    RB_CNT_COUNT = 19
 };
 
 // Step parameters passed by value to kernels.
@@ -276,12 +277,15 @@ size_t sort_status_words(int max_n, int digit_bits);
 hipError_t launch_radix_sort(uint2 *kv, int n, int sort_bits, int digit_bits, const SortWork &w,
                              hipStream_t s, const int *n_dev = nullptr);
 // out (may be null): exclusive prefix sums of in; idx_out (may be null, in
-// holding 0/1 flags): the indices of the set flags in increasing order
+// holding 0/1 flags): the indices of the set flags in increasing order, from
+// idx_out[*idx_base] when idx_base (a device count) is given
 hipError_t launch_exclusive_scan(const uint32_t *in, uint32_t *out, int n, uint32_t *total,
                                  uint32_t *status, hipStream_t s, const int *n_dev = nullptr,
-                                 int *err = nullptr, uint32_t *idx_out = nullptr);
+                                 int *err = nullptr, uint32_t *idx_out = nullptr,
+                                 const uint32_t *idx_base = nullptr);
 hipError_t launch_compact_flags(const uint32_t *flags, uint32_t *idx_out, int n, uint32_t *total,
-                                uint32_t *status, hipStream_t s, const int *n_dev, int *err);
+                                uint32_t *status, hipStream_t s, const int *n_dev, int *err,
+                                const uint32_t *idx_base = nullptr);
 hipError_t launch_bound_search(const uint2 *kv, int n, uint32_t *dst, int n_dst, int mode,
                                hipStream_t s);
 

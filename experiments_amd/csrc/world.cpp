@@ -1216,7 +1216,7 @@ extern "C" mi_rb_status mi_rb_get_stats(void *world, mi_rb_stats *out) {
     out->num_contacts = c[RB_CNT_CONTACTS];
     out->num_colors = c[RB_CNT_NCOLORS];
     out->solve_bodies = c[RB_CNT_SOLVE_BODIES];
-    out->face_pass_pairs = c[RB_CNT_FACE_WORK];
+    out->face_pass_pairs = c[RB_CNT_FACE_WORK] + c[RB_CNT_FACE_SMALL];
     out->overflow_flags = c[RB_CNT_OVERFLOW];
     out->normal_iterations = c[RB_CNT_ITERS_N];
     out->friction_iterations = c[RB_CNT_ITERS_F];
