@@ -105,7 +105,10 @@ __global__ __launch_bounds__(256) void k_constraints(
     float4 n = src.normal;
     float npts = n.w;
     cs.linear[k] = mk4(-n.x, -n.y, -n.z, sp.friction);
+    // (the loops over the 4 points are unrolled so that the point arrays and
+    // the contact stay in registers)
     float jac[4], bb[4];
+#pragma unroll
     for (int ic = 0; ic < 4; ic++) {
         float4 wp = src.pos[ic];
         float4 r0 = sub4(wp, pA), r1 = sub4(wp, pB);
@@ -126,7 +129,9 @@ __global__ __launch_bounds__(256) void k_constraints(
     float4 fr = mk4(0, 0, 0, 0);
     float4 cen = mk4(0, 0, 0, 0);
     if (npts > 0) {
-        for (int i = 0; (float)i < npts; i++) cen = add4(cen, src.pos[i]);
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            if ((float)i < npts) cen = add4(cen, src.pos[i]);
         cen = mk4(cen.x / npts, cen.y / npts, cen.z / npts, cen.w / npts);
         float4 t0, t1;
         plane_space(n, t0, t1);
@@ -135,6 +140,7 @@ __global__ __launch_bounds__(256) void k_constraints(
         fr.y = jac_coeff(neg4(cross3(r0, t1)), cross3(r1, t1), imA, IA, imB, IB);
     }
     cs.center[k] = cen;
+#pragma unroll
     for (int i = 0; i < 4; i++) cs.pos[4 * k + i] = (float)i < npts ? src.pos[i] : mk4(0, 0, 0, 0);
     cs.bodies[k] = make_int2(a, b);
     cs.pair[k] = p;
@@ -166,6 +172,7 @@ __global__ __launch_bounds__(256) void k_constraints(
         float4 va = add4(vA, cross3(wA, rA)), vb = add4(vB, cross3(wB, rB));
         float4 vr = sub4(va, vb);
         float dmin = FLT_MAX, jsum = 0.f, bsum = 0.f;
+#pragma unroll
         for (int i = 0; i < 4; i++)
             if ((float)i < npts) {
                 dmin = fminf(dmin, src.pos[i].w);
@@ -189,6 +196,7 @@ __global__ __launch_bounds__(256) void k_constraints(
         // rescales these to its predicted total
         const int pn = (int)prev.w;
         float lv[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
         for (int i = 0; i < 4; i++) {
             if ((float)i >= npts || jac[i] == 0.f) continue;
             float4 q = src.pos[i];
