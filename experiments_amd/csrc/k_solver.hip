@@ -681,7 +681,8 @@ __global__ __launch_bounds__(TB) void k_tile_pack(
     const int *__restrict__ order, ConstraintSoA cs, const float4 *__restrict__ pos,
     const float *__restrict__ inv_mass, const float4 *__restrict__ invI, PackedRows pk,
     int *__restrict__ solve_pos, int *__restrict__ tile_bodies, int *__restrict__ tile_nb,
-    const int *__restrict__ tile_ncolors, int4 *__restrict__ tile_hdr, int *counters) {
+    const int *__restrict__ tile_ncolors, int4 *__restrict__ tile_hdr, int *counters,
+    int precompute) {
     __shared__ PackLDS L;
     const int t = blockIdx.x, tid = threadIdx.x;
     const int first = tile_first[t];
@@ -748,8 +749,9 @@ __global__ __launch_bounds__(TB) void k_tile_pack(
         pk.pA[i] = pA;
         pk.pB[i] = pB;
         solve_pos[k] = i;
-        // velocity-independent terms of the normal row (as row_normal forms them)
-        {
+        // velocity-independent terms of the normal row (as row_normal forms
+        // them), read by the spatial tiles' normal sweeps only
+        if (precompute) {
             const float4 lin0 = cs.linear[k];
             const float4 n = neg4(lin0), lin = neg4(n);
             float4 IA[3], IB[3];
@@ -1089,11 +1091,12 @@ __global__ __launch_bounds__(TBK) void k_tile_solve(
     const int4 *__restrict__ tile_hdr, const int *__restrict__ tile_color_off,
     const int *__restrict__ tile_bodies, PackedRows pk,
     float4 *__restrict__ linvel, float4 *__restrict__ angvel, float *__restrict__ residual,
-    int *counters, int4 *__restrict__ tile_stats) {
+    int *counters, int4 *__restrict__ tile_stats, const uint2 *__restrict__ tile_order) {
     __shared__ TileSolveLDS L;
     int t;
     if (sp.tile_mode == 1) {
-        t = blockIdx.x;
+        // largest scenes first: the workgroups dispatched last are the short ones
+        t = (int)tile_order[blockIdx.x].y;
     } else {
         // tiles of this phase: (tx & 1, tz & 1) == (phase & 1, phase >> 1)
         int px = phase & 1, pz = phase >> 1;
@@ -1249,8 +1252,31 @@ hipError_t launch_tiles(const DeviceWorld &d, const StepParams &p, hipStream_t s
     RB_LAUNCH(k_tile_pack, dim3(p.n_tiles), dim3(TB), 0, s, d.tile_first, d.tile_cnt,
                        d.solve_order, d.cs, d.pos, d.inv_mass, d.inv_inertia_world, d.pk,
                        d.solve_pos, d.tile_bodies, d.tile_nb, d.tile_ncolors, d.tile_hdr,
-                       d.counters);
+                       d.counters, p.tile_mode == 0 ? 1 : 0);
     return rb_launch_status();
+}
+
+namespace {
+__global__ void k_tile_order_keys(int n_tiles, const int4 *__restrict__ tile_hdr,
+                                  uint2 *__restrict__ kv) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_tiles) return;
+    const int R = tile_hdr[t].y;
+    const uint32_t r = (uint32_t)(R < 0 ? 0 : (R > 0xffff ? 0xffff : R));
+    kv[t] = make_uint2(0xffffu - r, (uint32_t)t);
+}
+}  // namespace
+
+hipError_t launch_tile_order(const DeviceWorld &d, const StepParams &p, hipStream_t s,
+                             const SortWork &sw) {
+    if (p.tile_mode != 1) return hipSuccess;
+    RB_LAUNCH(k_tile_order_keys, dim3((p.n_tiles + 255) / 256), dim3(256), 0, s, p.n_tiles,
+              d.tile_hdr, d.tile_order);
+    hipError_t e = rb_launch_status();
+    if (e != hipSuccess) return e;
+    // (the order is a schedule only: every scene is solved by its own
+    // workgroup, so results do not depend on it)
+    return launch_radix_sort(d.tile_order, p.n_tiles, 16, 8, sw, s);
 }
 
 hipError_t launch_solve(const DeviceWorld &d, const StepParams &p, hipStream_t s,
@@ -1267,7 +1293,7 @@ hipError_t launch_solve(const DeviceWorld &d, const StepParams &p, hipStream_t s
     // scene tiles: one workgroup per scene, 256 or 512 threads
 #define RB_SOLVE_ARGS                                                                         \
     d.tile_hdr, d.tile_color_off, d.tile_bodies, d.pk, d.linvel, d.angvel, d.residual,        \
-        d.counters, d.tile_stats
+        d.counters, d.tile_stats, d.tile_order
     if (p.tile_mode == 1) {
         // independent scenes: every sweep of a scene in one workgroup, one
         // launch; with a tolerance each scene stops on its own residual

@@ -162,6 +162,7 @@ struct DeviceWorld {
     int *tile_bodies;            // per tile at 2 * first: its dynamic bodies (dense slots)
     int *tile_nb;                // per tile: number of those bodies
     int4 *tile_hdr;              // per tile: first row, rows, colours, cached bodies
+    uint2 *tile_order;           // scene tiles by descending rows: (key, tile)
     // narrowphase pass 1 -> pass 2
     uint32_t *narrow_cand;       // per pair: 1 if not separated by a face axis
     uint32_t *cand_off;          // its exclusive scan
@@ -303,6 +304,9 @@ hipError_t launch_constraints(const DeviceWorld &d, const StepParams &p, hipStre
 hipError_t launch_contact_total(const DeviceWorld &d, const StepParams &p, hipStream_t s);
 hipError_t launch_tile_bounds(const DeviceWorld &d, const StepParams &p, hipStream_t s);
 hipError_t launch_tiles(const DeviceWorld &d, const StepParams &p, hipStream_t s);
+// scene tiles: the solve's workgroup order, largest scene first
+hipError_t launch_tile_order(const DeviceWorld &d, const StepParams &p, hipStream_t s,
+                             const SortWork &sw);
 // kev: with timing, events recorded before and after every k_tile_solve
 // launch (2 per launch, at least 2 * (4 * (sweeps + 1)) of them), else null;
 // *n_launches gets the number of launches

@@ -381,6 +381,7 @@ mi_rb_status derive_and_upload(World &w) {
     RB_CHECK(w.alloc(&d.tile_bodies, 2 * (size_t)maxC));
     RB_CHECK(w.alloc(&d.tile_nb, p.n_tiles));
     RB_CHECK(w.alloc(&d.tile_hdr, p.n_tiles));
+    RB_CHECK(w.alloc(&d.tile_order, p.n_tiles));
     RB_CHECK(w.alloc(&d.narrow_cand, maxP));
     RB_CHECK(w.alloc(&d.cand_off, maxP));
     RB_CHECK(w.alloc(&d.face_best, maxP));
@@ -524,6 +525,7 @@ mi_rb_status enqueue_step(World &w, int last_stage) {
                                d.counters + RB_CNT_CONTACTS));
     RB_CHECK(launch_tile_bounds(d, p, s));
     RB_CHECK(launch_tiles(d, p, s));
+    RB_CHECK(launch_tile_order(d, p, s, w.sw));
     if (ev) RB_CHECK(hipEventRecord(w.ev[6], s));
     if (last_stage < MI_RB_STAGE_SOLVE) return MI_RB_OK;
     if (ev) {
