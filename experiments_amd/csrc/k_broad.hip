@@ -202,8 +202,13 @@ __global__ __launch_bounds__(FP_BLOCK) void k_find_pairs(
     if (t >= n) return;
     const uint2 e = kv[t];
     if (e.x == KEY_NONE) return;  // large body: k_large_count / k_large_write
-    int *lst = s_lists + slot * (K | 1);  // odd stride: the lists start in distinct banks
     const int i = (int)e.y;
+    int *dst = body_slots + (size_t)i * K;
+    // the list: in LDS for the lanes' rank sort (odd stride: the lists start
+    // in distinct banks); one lane keeps it in its own slots of body_slots,
+    // so its workgroups need no LDS (260 B per lane limited them to two per
+    // CU) and sort it there
+    int *lst = LANES == 1 ? dst : s_lists + slot * (K | 1);
     const int sc = (int)(e.x >> g.cell_bits);
     const float4 mn0 = smn[t], mx0 = smx[t];
     int gp[3];
@@ -300,13 +305,25 @@ __global__ __launch_bounds__(FP_BLOCK) void k_find_pairs(
         if (lane == 0) atomicOr(overflow, (int)MI_RB_OVERFLOW_BODY_PAIRS);
         c = (uint32_t)K;
     }
-    wave_sync();  // the list's LDS stores come before the other lanes' loads
-    int *dst = body_slots + (size_t)i * K;
-    for (uint32_t w = lane; w < c; w += LANES) {
-        const int j = lst[w];
-        uint32_t r = 0;
-        for (uint32_t k = 0; k < c; k++) r += lst[k] < j ? 1u : 0u;
-        dst[r] = j;
+    if constexpr (LANES == 1) {
+        // insertion sort in place (distinct indices, a few per body)
+        for (uint32_t k = 1; k < c; k++) {
+            const int x = dst[k];
+            int m = (int)k - 1;
+            while (m >= 0 && dst[m] > x) {
+                dst[m + 1] = dst[m];
+                m--;
+            }
+            dst[m + 1] = x;
+        }
+    } else {
+        wave_sync();  // the list's LDS stores come before the other lanes' loads
+        for (uint32_t w = lane; w < c; w += LANES) {
+            const int j = lst[w];
+            uint32_t r = 0;
+            for (uint32_t k = 0; k < c; k++) r += lst[k] < j ? 1u : 0u;
+            dst[r] = j;
+        }
     }
     if (lane == 0) pair_count[i] = c;
 }
@@ -875,7 +892,7 @@ hipError_t launch_find_pairs(const DeviceWorld &d, const StepParams &p, hipStrea
     // one lane per body (its cells walked in turn)
     const int lanes = n < (1 << 18) ? 32 : 1;
     const int entries = FP_BLOCK / lanes;
-    size_t lds = sizeof(int) * (size_t)entries * (K | 1);
+    size_t lds = lanes == 1 ? 0 : sizeof(int) * (size_t)entries * (K | 1);
     auto kern = lanes == 32 ? k_find_pairs<32> : k_find_pairs<1>;
     RB_LAUNCH(kern, dim3((n + entries - 1) / entries), dim3(FP_BLOCK), lds, s, n,
                        d.kv, d.cell_start, d.occ, d.smn, d.smx, d.aabb_min, d.aabb_max,
