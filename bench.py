@@ -539,6 +539,17 @@ def main():
         if stage_ms[i] > 0:
             stage_hbm[name] = round(sb / (stage_ms[i] / 1e3) / 1e9, 1)
     breakdown = {STAGES[i]: round(float(stage_ms[i]), 4) for i in range(8)}
+    # hash + sort of this workload's bodies against the HBM peak: the setup
+    # stage (which hashes the cell keys with the rest of the body setup) and
+    # the key sort, their algorithmic bytes over their eager-step time
+    hs_ms = float(stage_ms[0] + stage_ms[1])
+    hs_bytes = stage_bytes("setup", n, 0, 0, passes, 0, 0, 0) + \
+        stage_bytes("sort", n, 0, 0, passes, 0, 0, 0)
+    hash_sort = {"bytes_per_body": round(hs_bytes / n), "ms": round(hs_ms, 4),
+                 "achieved_GBs": round(hs_bytes / (hs_ms / 1e3) / 1e9, 1) if hs_ms > 0 else None,
+                 "frac_of_8TBs": round(hs_bytes / (hs_ms / 1e3) / (HBM_PEAK_GBS * 1e9), 4)
+                 if hs_ms > 0 else None,
+                 "note": "setup (gravity, inertia, AABB, cell key) + cell-key sort, eager stage times"}
     # the rest of the configured run, for the whole-run average
     whole = None
     if done < scene.steps and not args.profile_only:
@@ -575,6 +586,7 @@ def main():
         "autotune_ms_per_step": tune_ms,
         "stage_ms": breakdown,
         "stage_hbm_GBs": stage_hbm,
+        "hash_sort_hbm": hash_sort,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "overflow_flags": st["overflow_flags"],
