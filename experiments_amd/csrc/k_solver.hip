@@ -863,17 +863,21 @@ __device__ __forceinline__ void run_row_normal(NormalRow &d, SolveLDS &L, float4
 }
 
 // The sweep residual terms (largest scaled impulse change, largest impulse)
-// go into per-sweep maxima, one atomic per wave after a max over the wave's
-// lanes (on the float bits: the terms are >= 0, so the bits order like the
-// values and the result equals per-lane atomics).  They are needed after
+// go into per-sweep maxima: a max over the wave's lanes, then over the
+// workgroup's waves in LDS, then one atomic per workgroup (on the float bits:
+// the terms are >= 0, so the bits order like the values and the result
+// equals per-lane atomics; one atomic per wave on two addresses had cost a
+// spatial launch about 5 us).  They are needed after
 // every sweep with a tolerance, otherwise only for the last normal and the
 // last friction sweep, the residuals a step reports.
 __host__ __device__ __forceinline__ bool sweep_residual_needed(const StepParams &sp, int sweep) {
     return sp.tol > 0.f || sweep == sp.normal_iters - 1 ||
            sweep == sp.normal_iters + sp.friction_iters - 1;
 }
+// Every thread of the workgroup calls it; gred (LDS, two words) is zero on
+// entry and left zero.
 __device__ __forceinline__ void add_sweep_residual(float *residual, int sweep, float rmax,
-                                                   float lmax) {
+                                                   float lmax, unsigned int *gred) {
     unsigned r = __float_as_uint(rmax), l = __float_as_uint(lmax);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -881,9 +885,20 @@ __device__ __forceinline__ void add_sweep_residual(float *residual, int sweep, f
         l = max(l, (unsigned)__shfl_xor((int)l, o));
     }
     if ((threadIdx.x & 63) == 0 && (r != 0u || l != 0u)) {
-        atomicMax((unsigned int *)&residual[2 * sweep], r);
-        atomicMax((unsigned int *)&residual[2 * sweep + 1], l);
+        atomicMax(&gred[0], r);
+        atomicMax(&gred[1], l);
     }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned r0 = gred[0], l0 = gred[1];
+        if (r0 != 0u || l0 != 0u) {
+            atomicMax((unsigned int *)&residual[2 * sweep], r0);
+            atomicMax((unsigned int *)&residual[2 * sweep + 1], l0);
+        }
+        gred[0] = 0u;
+        gred[1] = 0u;
+    }
+    __syncthreads();
 }
 
 // Shared state of one tile's solve: the LDS velocity cache and the
@@ -891,6 +906,7 @@ __device__ __forceinline__ void add_sweep_residual(float *residual, int sweep, f
 struct TileSolveLDS {
     SolveLDS c;
     unsigned int red[2];
+    unsigned int gred[2];  // add_sweep_residual's workgroup maxima
 };
 
 // The sweeps of one tile: the optional warm-start pass, then normal sweeps,
@@ -941,7 +957,7 @@ __device__ __forceinline__ void tile_sweeps(const StepParams &sp, int sweep0, in
             if (phase < 0) continue;
             swept = true;
             const int sweep = sweep0 + (phase == 0 ? it : nsweeps_n + it);
-            if (RES && sweep_residual_needed(sp, sweep)) add_sweep_residual(residual, sweep, rmax, lmax);
+            if (RES && sweep_residual_needed(sp, sweep)) add_sweep_residual(residual, sweep, rmax, lmax, L.gred);
             if (local_tol) {
                 // rmax, lmax >= 0: their float bits order like the values
                 if (rmax > 0.f) atomicMax(&L.red[0], __float_as_uint(rmax));
@@ -988,7 +1004,7 @@ __device__ __forceinline__ void tile_normal_sweep(const StepParams &sp, int swee
         }
         __syncthreads();
     }
-    if (RES && sweep_residual_needed(sp, sweep)) add_sweep_residual(residual, sweep, rmax, lmax);
+    if (RES && sweep_residual_needed(sp, sweep)) add_sweep_residual(residual, sweep, rmax, lmax, L.gred);
     if (REG && tid < R) pk.lam[first + tid] = nr.lam;
 }
 
@@ -1018,6 +1034,11 @@ __device__ __forceinline__ void solve_tile(
     }
     const int *coff = tile_color_off + t * (RB_MAX_LOCAL_COLORS + 1);
     const int *bodies = tile_bodies + 2 * (size_t)first;
+    // (the barrier after the velocity gather orders this before any use)
+    if (tid == 0) {
+        L.gred[0] = 0u;
+        L.gred[1] = 0u;
+    }
     if constexpr (NORMAL) {
         // one normal sweep (nsweeps_n == 1) on the precomputed rows
         NormalRow nr;
