@@ -339,50 +339,25 @@ def batched_leg(steps=20, warmup=5, sample_every=16):
             "overflow_flags": st["overflow_flags"]}
 
 
-def warm_start_leg(n_scenes=1024, settle=240, steps=20, tol=1e-4, max_sweeps=500):
+def warm_start_leg(n_scenes=1024, settles=(240, 600), steps=20, tol=1e-4, max_sweeps=4096):
     """Solver sweeps to a 1e-4 residual for the zero, previous-step and
     learned warm starts on held-out batched piles (another seed than the
-    training scenes), each run from the same settled state; the trained
-    weights come from experiments_amd/warmstart_weights.npz."""
-    from experiments_amd import _lib as L
+    training scenes), each run from the same state: the piles advanced
+    `settle` fixed-sweep steps (at 240 many piles are still moving, at 600
+    nearly all rest).  Sweeps are counted per scene and step from the
+    solver's per-scene record; the cap is high enough that scenes reaching
+    it are reported, not averaged in silently.  The trained weights come
+    from experiments_amd/warmstart_weights.npz."""
     from experiments_amd import scenes
-    from experiments_amd.warmstart import load_weights, run_iterations
+    from experiments_amd.warmstart import compare_starts, load_weights
     weights = load_weights()
-    scene = scenes.batched_piles(n_scenes=n_scenes, seed=scenes.SEED + 1, iterations=max_sweeps)
-    out = {"scenes": n_scenes, "settle": settle, "steps": steps, "tol": tol,
-           "max_sweeps": max_sweeps}
-    starts = [("zero", L.WARM_ZERO), ("previous", L.WARM_PREVIOUS)]
-    if weights is not None:
-        starts.append(("learned", L.WARM_LEARNED))
-    for name, mode in starts:
-        w = scenes.build_world(scene, solver_iterations=10)
-        w.use_graph(True)
-        w.step(settle)
-        if mode == L.WARM_LEARNED:
-            w.set_warmstart_weights(*weights)
-        w.set_solver(max_sweeps, tol, mode)
-        w.use_graph(True)
-        r = run_iterations(w, steps)
-        a = r.pop("per_step")
-        out[name] = {"normal_sweeps": round(r["mean_iters_normal"], 2),
-                     "friction_sweeps": round(r["mean_iters_friction"], 2),
-                     "sweeps": round(r["mean_iters_normal"] + r["mean_iters_friction"], 2),
-                     "max_normal_sweeps": r["max_iters_normal"],
-                     "max_friction_sweeps": r["max_iters_friction"],
-                     "final_residual_max_normal": r["max_residual_normal"],
-                     "final_residual_max_friction": r["max_residual_friction"],
-                     "steps_with_an_unconverged_scene_normal": int(np.sum(a[:, 4] >= tol)),
-                     "steps_with_an_unconverged_scene_friction": int(np.sum(a[:, 5] >= tol))}
-        w.close()
-    for name in ("previous", "learned"):
-        if name in out:
-            out[name]["fewer_sweeps_than_zero"] = round(out["zero"]["sweeps"] /
-                                                        out[name]["sweeps"], 3)
-    if "learned" in out:
-        out["learned"]["fewer_sweeps_than_previous"] = round(out["previous"]["sweeps"] /
-                                                             out["learned"]["sweeps"], 3)
-    else:
+    scene = scenes.batched_piles(n_scenes=n_scenes, seed=scenes.SEED + 1, iterations=10)
+    out = {"scenes": n_scenes, "steps": steps, "tol": tol, "max_sweeps": max_sweeps}
+    if weights is None:
         out["learned"] = "no trained weights (experiments_amd/warmstart_weights.npz)"
+    for settle in settles:
+        out["settled_%d" % settle] = compare_starts(scene, weights, settle, steps, tol,
+                                                    max_sweeps)
     return out
 
 

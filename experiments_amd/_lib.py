@@ -24,7 +24,7 @@ OVERFLOW_LOOKBACK, OVERFLOW_UNSUPPORTED_PAIR = 128, 256
  BUF_INV_INERTIA, BUF_PHASES, BUF_HULLS, BUF_VERTS, BUF_PLANES, BUF_FACE_INDEX, BUF_INDICES,
  BUF_EDGES, BUF_INV_MASS, BUF_INV_INERTIA_LOCAL, BUF_SCENE, BUF_IS_LARGE, BUF_SHAPE,
  BUF_FEATURES, BUF_PREV_KEYS, BUF_PREV_IMPULSES, BUF_PREDICTIONS, BUF_PREV_POINTS,
- BUF_CHILDREN) = range(26)
+ BUF_CHILDREN, BUF_TILE_STATS) = range(27)
 MLP_NW, MLP_NB = 64 * 16 + 64 * 64 + 3 * 64, 64 + 64 + 3
 (STAGE_SETUP, STAGE_SORT, STAGE_PAIRS, STAGE_NARROWPHASE, STAGE_CONSTRAINTS, STAGE_COLOR,
  STAGE_SOLVE, STAGE_INTEGRATE) = range(8)

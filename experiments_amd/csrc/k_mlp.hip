@@ -126,8 +126,10 @@ __device__ __forceinline__ void mlp_tile(const MlpRegs &m, float4 x, f32x4 a1[4]
 // Impulses from a prediction (network units are RB_MLP_SCALE x impulse):
 // the predicted normal impulse, clamped at zero, is shared over the points in
 // proportion to the impulses the constraint setup matched from the previous
-// step (evenly over the points with a row when there are none); friction is
-// clamped to the cone of the predicted normal impulse.
+// step (evenly over the points with a row when there are none).  Friction:
+// a manifold with matched previous impulses keeps the previous step's
+// friction impulses the setup gave it, others take the predicted ones; either
+// is clamped to the cone of the predicted normal impulse.
 __device__ __forceinline__ void apply_prediction(int k, float y0, float y1, float y2,
                                                  float friction, ConstraintSoA cs) {
     const float4 jac = cs.jac_inv[k], lp = cs.lambda[k];
@@ -145,7 +147,7 @@ __device__ __forceinline__ void apply_prediction(int k, float y0, float y1, floa
     cs.lambda[k] = make_float4(lv[0], lv[1], lv[2], lv[3]);
     float4 fr = cs.fric[k];
     float lim = friction * ln;
-    float f0 = y1 / RB_MLP_SCALE, f1 = y2 / RB_MLP_SCALE;
+    float f0 = sum > 0.f ? fr.z : y1 / RB_MLP_SCALE, f1 = sum > 0.f ? fr.w : y2 / RB_MLP_SCALE;
     f0 = fminf(fmaxf(f0, -lim), lim);
     f1 = fminf(fmaxf(f1, -lim), lim);
     if (fr.x == 0.f) { f0 = 0.f; f1 = 0.f; }

@@ -77,6 +77,9 @@ static_assert(sizeof(Contact4) == 96, "Contact4 layout");
 #define RB_MLP_NB (2 * RB_MLP_H + RB_MLP_OUT)
 #define RB_MLP_SCALE 10.f  // network units per unit of impulse (inputs and outputs)
 
+// Solver sweeps per phase (normal, then friction), fixed or the cap under a tolerance
+#define RB_MAX_SWEEPS 4096
+
 // Solver tiles
 #define RB_TILE_MAX_ROWS 2048     // constraints per tile held by one workgroup
 #define RB_TILE_MAX_BODIES 1024   // distinct bodies per tile cached in LDS

@@ -386,7 +386,7 @@ mi_rb_status derive_and_upload(World &w) {
     RB_CHECK(w.alloc(&d.cand_off, maxP));
     RB_CHECK(w.alloc(&d.face_best, maxP));
     RB_CHECK(w.alloc(&d.cand_kv, maxP));
-    RB_CHECK(w.alloc(&d.residual, 2 * 1024));
+    RB_CHECK(w.alloc(&d.residual, 4 * RB_MAX_SWEEPS + 2));  // 2 floats per sweep of both phases
     RB_CHECK(w.alloc(&d.counters, RB_CNT_COUNT));
     RB_CHECK(w.alloc(&d.prev_keys, maxC));
     RB_CHECK(w.alloc(&d.prev_imp, maxC));
@@ -582,7 +582,7 @@ extern "C" mi_rb_status mi_rb_create(const mi_rb_config *cfg, void **out_world) 
     if (!cfg || !out_world) return MI_RB_INVALID_ARG;
     if (cfg->max_bodies <= 0 || cfg->max_pairs <= 0 || cfg->max_contacts <= 0 ||
         cfg->max_pairs_per_body <= 0 || cfg->max_pairs_per_body > 64 || cfg->dt <= 0.f ||
-        cfg->solver_iterations < 0 || cfg->solver_iterations > 500 || cfg->sat_mode < 0 ||
+        cfg->solver_iterations < 0 || cfg->solver_iterations > RB_MAX_SWEEPS || cfg->sat_mode < 0 ||
         cfg->sat_mode > 1 || cfg->reduction_mode < 0 || cfg->reduction_mode > 1)
         return MI_RB_INVALID_ARG;
     int ndev = 0;
@@ -1093,7 +1093,8 @@ extern "C" mi_rb_status mi_rb_set_tuning(void *world, const mi_rb_tuning *t) {
 extern "C" mi_rb_status mi_rb_set_solver(void *world, int iterations, float tolerance,
                                          int warm_start) {
     World *w = W(world);
-    if (!w || iterations < 0 || iterations > 500 || !(tolerance >= 0.f)) return MI_RB_INVALID_ARG;
+    if (!w || iterations < 0 || iterations > RB_MAX_SWEEPS || !(tolerance >= 0.f))
+        return MI_RB_INVALID_ARG;
     if (warm_start < MI_RB_WARM_ZERO || warm_start > MI_RB_WARM_LEARNED) return MI_RB_INVALID_ARG;
     w->cfg.solver_iterations = iterations;
     w->cfg.residual_tolerance = tolerance;
@@ -1419,6 +1420,7 @@ extern "C" mi_rb_status mi_rb_read_buffer(void *world, int which, void *host, si
             break;
         }
         case MI_RB_BUF_COLORS: src = w->d.row_color; elem = 4; cnt = nc; break;
+        case MI_RB_BUF_TILE_STATS: src = w->d.tile_stats; elem = 16; cnt = w->sp.n_tiles; break;
         case MI_RB_BUF_INV_INERTIA: src = w->d.inv_inertia_world; elem = 48; cnt = n; break;
         case MI_RB_BUF_PHASES: {
             // per tile: first, end, phase (spatial: (tx%3)+3*(tz%3); scene: 0), ncolors

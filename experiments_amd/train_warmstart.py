@@ -4,7 +4,7 @@ warm starts.
 
     python -m experiments_amd.train_warmstart [--scenes 4096] [--settle 240]
         [--log-steps 200] [--train-steps 20000] [--batch 65536]
-        [--eval-steps 100] [--out w.npz]
+        [--eval-steps 20] [--max-sweeps 4096] [--out w.npz]
 
 Protocol:
   1. Training scenes settle for `settle` steps with the fixed-sweep plain
@@ -15,8 +15,9 @@ Protocol:
      kernels.
   3. Held-out scenes (another seed) settle the same way, once per warm start.
      Each is then stepped `eval-steps` times, every solve running to the same
-     tolerance.  The sweeps used (mean over scenes, per phase) and the final
-     residuals are reported.
+     tolerance.  The sweeps used per scene and step (means per phase, also
+     over the scene-steps every start solved to the tolerance), the
+     scene-steps left above it and the final residuals are reported.
 Prints one JSON line.
 """
 import argparse
@@ -29,7 +30,7 @@ import numpy as np
 
 from . import _lib as L
 from . import scenes
-from .warmstart import Mlp, log_plain_solver, run_iterations
+from .warmstart import Mlp, compare_starts, log_plain_solver
 
 
 def settled_world(scene, settle, warm, tol, max_sweeps, settle_sweeps=10):
@@ -54,29 +55,17 @@ def train(X, T, steps, batch, lr=2e-3, lr_min=2e-5, chunk=1000, seed=1):
         lr_k = lr_min + 0.5 * (lr - lr_min) * (1 + math.cos(math.pi * mid))
         hist.append(net.train(X, T, batch=batch, steps=k, lr=lr_k, seed=seed + done))
         done += k
+        print("trained %d of %d steps: loss %.5f" % (done, steps, float(hist[-1][-1])),
+              file=sys.stderr, flush=True)
     return net, np.concatenate(hist) if hist else np.zeros(0, np.float32)
 
 
 def evaluate(scene, weights, settle, steps, tol, max_sweeps):
-    out = {}
-    for name, mode in (("zero", L.WARM_ZERO), ("previous", L.WARM_PREVIOUS),
-                       ("learned", L.WARM_LEARNED)):
-        w = settled_world(scene, settle, L.WARM_ZERO, tol, max_sweeps)
-        if mode == L.WARM_LEARNED:
-            w.set_warmstart_weights(*weights)
-        w.set_solver(max_sweeps, tol, mode)
-        r = run_iterations(w, steps)
-        r.pop("per_step")
+    out = compare_starts(scene, weights, settle, steps, tol, max_sweeps)
+    for name in ("zero", "previous", "learned"):
         print("evaluated %s: %.1f normal / %.1f friction sweeps per scene" %
-              (name, r["mean_iters_normal"], r["mean_iters_friction"]), file=sys.stderr, flush=True)
-        r["mean_sweeps"] = r["mean_iters_normal"] + r["mean_iters_friction"]
-        out[name] = r
-        w.close()
-    z = out["zero"]["mean_sweeps"]
-    zn = out["zero"]["mean_iters_normal"]
-    for name in ("previous", "learned"):
-        out[name]["fewer_sweeps_than_zero"] = z / max(out[name]["mean_sweeps"], 1e-9)
-        out[name]["fewer_normal_sweeps_than_zero"] = zn / max(out[name]["mean_iters_normal"], 1e-9)
+              (name, out[name]["normal_sweeps"], out[name]["friction_sweeps"]),
+              file=sys.stderr, flush=True)
     return out
 
 
@@ -87,9 +76,9 @@ def main(argv=None):
     ap.add_argument("--log-steps", type=int, default=200)
     ap.add_argument("--train-steps", type=int, default=20000)
     ap.add_argument("--batch", type=int, default=65536)
-    ap.add_argument("--eval-steps", type=int, default=100)
+    ap.add_argument("--eval-steps", type=int, default=20)
     ap.add_argument("--tol", type=float, default=1e-4)
-    ap.add_argument("--max-sweeps", type=int, default=100)
+    ap.add_argument("--max-sweeps", type=int, default=4096)
     ap.add_argument("--out", default="")
     args = ap.parse_args(argv)
     # torch provides the device buffers: initialise it before the library's

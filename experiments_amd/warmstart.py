@@ -127,6 +127,95 @@ def run_iterations(world, steps):
                 per_step=a)
 
 
+def run_scene_iterations(world, steps):
+    """Step a batched world solved to a tolerance and record, per step and
+    scene, the normal and friction sweeps used and the final residuals (the
+    solver's per-scene record).  Returns int arrays normal, friction (steps,
+    scenes), float arrays res_normal, res_friction and a bool mask `active`
+    of the scenes that had contacts."""
+    nn, nf, rn, rf, act = [], [], [], [], []
+    for _ in range(steps):
+        world.step(1)
+        st = world.read(L.BUF_TILE_STATS, np.int32, (4,))
+        act.append(st[:, 0] >= 0)
+        nn.append(st[:, 0].copy())
+        nf.append(st[:, 1].copy())
+        rn.append(st[:, 2].view(np.float32).copy())
+        rf.append(st[:, 3].view(np.float32).copy())
+    return dict(normal=np.array(nn), friction=np.array(nf), res_normal=np.array(rn),
+                res_friction=np.array(rf), active=np.array(act))
+
+
+def _start_summary(r, tol):
+    a = r["active"]
+    return {"normal_sweeps": round(float(r["normal"][a].mean()), 2),
+            "friction_sweeps": round(float(r["friction"][a].mean()), 2),
+            "sweeps": round(float((r["normal"] + r["friction"])[a].mean()), 2),
+            "max_normal_sweeps": int(r["normal"][a].max()),
+            "max_friction_sweeps": int(r["friction"][a].max()),
+            "unconverged_scene_steps_normal": int((a & (r["res_normal"] >= tol)).sum()),
+            "unconverged_scene_steps_friction": int((a & (r["res_friction"] >= tol)).sum()),
+            "final_residual_max_normal": float(r["res_normal"][a].max()),
+            "final_residual_max_friction": float(r["res_friction"][a].max())}
+
+
+def compare_starts(scene, weights, settle, steps, tol=1e-4, max_sweeps=4096, settle_sweeps=10,
+                   skip=5):
+    """Sweeps to the residual tolerance for the zero, previous-step and (with
+    weights) learned warm starts, each run from the same state: `scene`
+    advanced `settle` steps with `settle_sweeps` fixed sweeps, then `skip`
+    unrecorded and `steps` recorded steps solved to `tol` (at most
+    `max_sweeps` per phase; the skipped steps keep the first warm starts, which
+    read impulses of fixed-sweep solves, out of the record).  Sweeps are
+    counted per scene and step from the solver's per-scene record: means over
+    the scene-steps with contacts, the scene-steps that ended above the
+    tolerance, and the means over the scene-steps that every start solved to
+    the tolerance (`converged_in_every_start`), with the ratios between
+    starts."""
+    from . import scenes
+    starts = [("zero", L.WARM_ZERO), ("previous", L.WARM_PREVIOUS)]
+    if weights is not None:
+        starts.append(("learned", L.WARM_LEARNED))
+    runs, out = {}, {}
+    for name, mode in starts:
+        w = scenes.build_world(scene, solver_iterations=settle_sweeps)
+        try:
+            w.use_graph(True)
+            w.step(settle)
+            if mode == L.WARM_LEARNED:
+                w.set_warmstart_weights(*weights)
+            w.set_solver(max_sweeps, tol, mode)
+            w.use_graph(True)
+            w.step(skip)
+            runs[name] = run_scene_iterations(w, steps)
+            flags = w.stats()["overflow_flags"]
+        finally:
+            w.close()
+        out[name] = _start_summary(runs[name], tol)
+        out[name]["overflow_flags"] = flags
+    common = np.ones_like(runs["zero"]["active"])
+    for r in runs.values():
+        common &= r["active"] & (r["res_normal"] < tol) & (r["res_friction"] < tol)
+    conv = {"scene_steps": int(common.sum()), "of": int(runs["zero"]["active"].sum())}
+    for name, r in runs.items():
+        conv[name] = {"normal_sweeps": round(float(r["normal"][common].mean()), 2),
+                      "sweeps": round(float((r["normal"] + r["friction"])[common].mean()), 2)}
+    out["converged_in_every_start"] = conv
+
+    def ratios(d, ref, tag):
+        d["fewer_sweeps_than_" + tag] = round(ref["sweeps"] / d["sweeps"], 3)
+        d["fewer_normal_sweeps_than_" + tag] = round(ref["normal_sweeps"] / d["normal_sweeps"], 3)
+
+    for name in runs:
+        if name != "zero":
+            ratios(out[name], out["zero"], "zero")
+            ratios(conv[name], conv["zero"], "zero")
+    if "learned" in runs:
+        ratios(out["learned"], out["previous"], "previous")
+        ratios(conv["learned"], conv["previous"], "previous")
+    return out
+
+
 def load_weights(path=WEIGHTS_PATH):
     """The trained network shipped with the package (written by
     experiments_amd.train_warmstart --out): (bf16 weights, fp32 biases), or

@@ -58,7 +58,7 @@ typedef struct mi_rb_config {
     float dt;                  /* step for gravity and integration */
     float solver_dt;           /* step in the Baumgarte bias; 0 = dt */
     float gravity[3];
-    int solver_iterations;     /* normal sweeps, then as many friction sweeps */
+    int solver_iterations;     /* normal sweeps, then as many friction sweeps (<= 4096) */
     float friction;            /* Coulomb coefficient (reference: 0.7) */
     float baumgarte;           /* position constraint coefficient (0.2) */
     float position_drift;      /* allowed penetration in the bias (0.005) */
@@ -171,7 +171,11 @@ enum {
     MI_RB_BUF_PREV_IMPULSES = 22, /* float4 per cached pair: sum lambda_n, friction 0, 1, points */
     MI_RB_BUF_PREDICTIONS = 23, /* float[3] per contact: network output (learned warm start) */
     MI_RB_BUF_PREV_POINTS = 24, /* float4[4] per cached pair: point xyz, its lambda_n in w */
-    MI_RB_BUF_CHILDREN = 25    /* 48 B per compound child: float4 pos, float4 orn, int shape, pad */
+    MI_RB_BUF_CHILDREN = 25,   /* 48 B per compound child: float4 pos, float4 orn, int shape, pad */
+    MI_RB_BUF_TILE_STATS = 26  /* int4 per solver tile (per scene in a batched world): normal and
+                                * friction sweeps used and the final normal and friction
+                                * residuals (float bits); x = -1 for a tile without rows.
+                                * Written by steps with a tolerance on scene tiles only */
 };
 
 /* stages at which mi_rb_step_partial stops */

@@ -1057,8 +1057,10 @@ void orc_contact_features(int nc, const orc_contact *contacts, orc_constraint *c
 /* Initial lambdas from network predictions y[k][3] (10 x impulse): the
  * normal impulse, clamped at zero, is shared over the points in proportion
  * to the lambdas cons already holds (the matched previous impulses), or
- * evenly over the points with a row when they sum to zero; friction is
- * clamped to friction x that normal impulse, none without friction rows. */
+ * evenly over the points with a row when they sum to zero.  Friction: the
+ * friction lambdas cons holds (the previous step's) where the matched
+ * impulses sum above zero, else the predicted ones; clamped to friction x that
+ * normal impulse, none without friction rows. */
 void orc_warm_from_prediction(int nc, orc_constraint *cons, const float *y, float friction)
 {
     for (int k = 0; k < nc; k++) {
@@ -1076,8 +1078,11 @@ void orc_warm_from_prediction(int nc, orc_constraint *cons, const float *y, floa
         }
         for (int i = 0; i < 4; i++) c->lambda[i] = lv[i];
         float lim = friction * ln;
-        float f0 = fminf(fmaxf(y[3 * k + 1] / 10.f, -lim), lim);
-        float f1 = fminf(fmaxf(y[3 * k + 2] / 10.f, -lim), lim);
+        /* matched manifolds keep the previous step's friction impulses */
+        float f0 = sum > 0.f ? c->f_lambda[0] : y[3 * k + 1] / 10.f;
+        float f1 = sum > 0.f ? c->f_lambda[1] : y[3 * k + 2] / 10.f;
+        f0 = fminf(fmaxf(f0, -lim), lim);
+        f1 = fminf(fmaxf(f1, -lim), lim);
         if (c->f_jac_inv[0] == 0.f) { f0 = 0.f; f1 = 0.f; }
         c->f_lambda[0] = f0;
         c->f_lambda[1] = f1;

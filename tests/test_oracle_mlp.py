@@ -217,9 +217,13 @@ def test_warm_from_prediction_clamps():
     pos, lin, ang, im, invI = _bodies()
     cons = O.contact_to_constraint(c, pos, lin, ang, im, invI, 1 / 60.0)
     cons["lambda"][2] = [0.1, 0.3, 0.0, 0.4]                 # matched previous impulses
+    cons["f_lambda"][2] = [0.5, -2.0]                        # and previous friction
     y = np.array([[20.0, 30.0, -1.0], [-5.0, 1.0, 1.0], [16.0, 0.0, 0.0]], np.float32)
     out = O.warm_from_prediction(cons, y, 0.7)
     assert np.allclose(out["lambda"][2], [0.2, 0.6, 0.0, 0.8])  # 1.6 in proportion 1:3:0:4
+    # a matched manifold keeps its previous friction (not the predicted 0, 0),
+    # clamped to the cone 0.7 x 1.6
+    assert np.allclose(out["f_lambda"][2], [0.5, -0.7 * 1.6])
     assert np.allclose(out["lambda"][0], 0.5)               # no previous: 2.0 split over 4 points
     assert np.isclose(out["f_lambda"][0, 0], 0.7 * 2.0)     # clamped to the cone
     assert np.isclose(out["f_lambda"][0, 1], -0.1)
