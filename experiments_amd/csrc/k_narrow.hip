@@ -1231,6 +1231,7 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_clip(
 struct SmallClipLDS {
     float4 ea[SG_EDGES], eb[SG_EDGES];
     float4 va[SG_VERTS], vb[SG_VERTS];
+    float4 pa[SG_FACES], pb[SG_FACES];  // face planes (local), loaded with the edges
     float4 poly[2][SG_POLY];
     float4 pts[2 * NPROBE];  // probes, then the side planes, then the kept points
     int q[2 * SG], pq[2 * SG];
@@ -1277,6 +1278,13 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_clip_small(
         if (gl + SG < hB.num_edges) S.eb[gl + SG] = qt_rotate(oB, edges[hB.edge_offset + gl + SG]);
         if (gl < nvA) S.va[gl] = verts[hA.vertex_offset + gl];
         if (gl < nvB) S.vb[gl] = verts[hB.vertex_offset + gl];
+        // the faces' planes and (first index, count) records are read after
+        // the SAT; loading them here overlaps their latency with it (lane gl
+        // keeps face gl's records of both hulls)
+        if (gl < hA.num_faces) S.pa[gl] = planes[hA.face_offset + gl];
+        if (gl < hB.num_faces) S.pb[gl] = planes[hB.face_offset + gl];
+        const int2 fiA_l = gl < hA.num_faces ? face_index[hA.face_offset + gl] : make_int2(0, 0);
+        const int2 fiB_l = gl < hB.num_faces ? face_index[hB.face_offset + gl] : make_int2(0, 0);
         group_sync();
 
         const float4 dc = sub4(xform(hA.center, pA, oA), xform(hB.center, pB, oB));
@@ -1423,10 +1431,10 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_clip_small(
         }
         float4 sep;
         if (best_ax < hA.num_faces) {
-            sep = qt_rotate(oA, planes[hA.face_offset + best_ax]);
+            sep = qt_rotate(oA, S.pa[best_ax]);
             if (dot3(dc, sep) < 0) sep = scl4(sep, -1.f);
         } else if (best_ax < nf) {
-            sep = qt_rotate(oB, planes[hB.face_offset + best_ax - hA.num_faces]);
+            sep = qt_rotate(oB, S.pb[best_ax - hA.num_faces]);
             if (dot3(dc, sep) < 0) sep = scl4(sep, -1.f);
         } else {
             const int k = best_ax - nf;
@@ -1440,7 +1448,7 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_clip_small(
         float dB = -FLT_MAX;
         int fB = 0x7fffffff;
         if (gl < hB.num_faces) {
-            const float4 pl = planes[hB.face_offset + gl];
+            const float4 pl = S.pb[gl];
             const float d = dot3(qt_rotate(oB, mk4(pl.x, pl.y, pl.z, 0.f)), sep);
             if (d > dB) { dB = d; fB = gl; }
         }
@@ -1448,7 +1456,7 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_clip_small(
         float dA = FLT_MAX;
         int fA = 0x7fffffff;
         if (gl < hA.num_faces) {
-            const float4 pl = planes[hA.face_offset + gl];
+            const float4 pl = S.pa[gl];
             const float d = dot3(qt_rotate(oA, mk4(pl.x, pl.y, pl.z, 0.f)), sep);
             if (d < dA) { dA = d; fA = gl; }
         }
@@ -1457,11 +1465,12 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_clip_small(
             if (gl == 0) has_contact[p] = 0;
             continue;
         }
-        const int2 fiB = face_index[hB.face_offset + fB];
+        // fB, fA < SG_FACES: the group's lanes fB, fA hold those faces' records
+        const int2 fiB = make_int2(__shfl(fiB_l.x, gbase + fB, 64), __shfl(fiB_l.y, gbase + fB, 64));
         int nin = fiB.y < SG_VERTS ? fiB.y : SG_VERTS;
         if (gl < nin) S.poly[0][gl] = xform(S.vb[indices[fiB.x + gl]], pB, oB);
-        const int2 fiA = face_index[hA.face_offset + fA];
-        const float4 plA = planes[hA.face_offset + fA];
+        const int2 fiA = make_int2(__shfl(fiA_l.x, gbase + fA, 64), __shfl(fiA_l.y, gbase + fA, 64));
+        const float4 plA = S.pa[fA];
         const float4 wpn = qt_rotate(oA, mk4(plA.x, plA.y, plA.z, 0.f));
         float4 *sSide = S.pts;
         const int nside = fiA.y < SG_VERTS ? fiA.y : SG_VERTS;
