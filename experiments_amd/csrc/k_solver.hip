@@ -1008,6 +1008,42 @@ __device__ __forceinline__ void tile_normal_sweep(const StepParams &sp, int swee
     if (REG && tid < R) pk.lam[first + tid] = nr.lam;
 }
 
+// The tile's dynamic bodies' velocities into LDS slots (slot s: body
+// bodies[s]).  Thread tid keeps the body of slot tid (every slot when nb <=
+// TBK) in a register for the store back, which then waits on no index load.
+template <int TBK>
+__device__ __forceinline__ int gather_velocities(const int *__restrict__ bodies, int nb,
+                                                 const float4 *__restrict__ linvel,
+                                                 const float4 *__restrict__ angvel, SolveLDS &L) {
+    const int tid = threadIdx.x;
+    const int b0 = tid < nb ? bodies[tid] : 0;
+    if (tid < nb) {
+        L.v[tid] = linvel[b0];
+        L.w[tid] = angvel[b0];
+    }
+    for (int s = tid + TBK; s < nb; s += TBK) {
+        const int b = bodies[s];
+        L.v[s] = linvel[b];
+        L.w[s] = angvel[b];
+    }
+    return b0;
+}
+template <int TBK>
+__device__ __forceinline__ void store_velocities(const int *__restrict__ bodies, int nb, int b0,
+                                                 float4 *__restrict__ linvel,
+                                                 float4 *__restrict__ angvel, const SolveLDS &L) {
+    const int tid = threadIdx.x;
+    if (tid < nb) {
+        linvel[b0] = L.v[tid];
+        angvel[b0] = L.w[tid];
+    }
+    for (int s = tid + TBK; s < nb; s += TBK) {
+        const int b = bodies[s];
+        linvel[b] = L.v[s];
+        angvel[b] = L.w[s];
+    }
+}
+
 // One workgroup per tile (spatial: the tiles of one parity phase; scene: one
 // scene); the tile's dynamic body velocities live in LDS for the launch.
 // The solve of one tile (rows first..first+R of the packed arrays): its
@@ -1048,11 +1084,7 @@ __device__ __forceinline__ void solve_tile(
             for (int c = 0; c < nc; c++)
                 if (coff[c] <= tid && tid < coff[c + 1]) rc = c;
         }
-        for (int s = tid; s < nb; s += TBK) {
-            int b = bodies[s];
-            L.c.v[s] = linvel[b];
-            L.c.w[s] = angvel[b];
-        }
+        const int b0 = gather_velocities<TBK>(bodies, nb, linvel, angvel, L.c);
         __syncthreads();
         if (R <= TBK)
             tile_normal_sweep<TBK, true, RES>(sp, sweep0, first, R, nc, coff, pk, linvel, angvel,
@@ -1061,11 +1093,7 @@ __device__ __forceinline__ void solve_tile(
             tile_normal_sweep<TBK, false, RES>(sp, sweep0, first, R, nc, coff, pk, linvel,
                                                angvel, residual, L, nr, rc);
         __syncthreads();
-        for (int s = tid; s < nb; s += TBK) {
-            int b = bodies[s];
-            linvel[b] = L.c.v[s];
-            angvel[b] = L.c.w[s];
-        }
+        store_velocities<TBK>(bodies, nb, b0, linvel, angvel, L.c);
         return;
     }
     // a tile of at most TBK rows keeps row tid in thread tid's registers;
@@ -1078,11 +1106,7 @@ __device__ __forceinline__ void solve_tile(
         for (int c = 0; c < nc; c++)
             if (coff[c] <= tid && tid < coff[c + 1]) rc = c;
     }
-    for (int s = tid; s < nb; s += TBK) {
-        int b = bodies[s];
-        L.c.v[s] = linvel[b];
-        L.c.w[s] = angvel[b];
-    }
+    const int b0 = gather_velocities<TBK>(bodies, nb, linvel, angvel, L.c);
     __syncthreads();
     int used[2] = {0, 0};
     float rfin[2] = {0.f, 0.f};
@@ -1098,11 +1122,7 @@ __device__ __forceinline__ void solve_tile(
         tile_stats[t] = make_int4(used[0], used[1], (int)__float_as_uint(rfin[0]),
                                   (int)__float_as_uint(rfin[1]));
     __syncthreads();
-    for (int s = tid; s < nb; s += TBK) {
-        int b = bodies[s];
-        linvel[b] = L.c.v[s];
-        angvel[b] = L.c.w[s];
-    }
+    store_velocities<TBK>(bodies, nb, b0, linvel, angvel, L.c);
 }
 
 

@@ -5,7 +5,8 @@
 //
 // Sort: one histogram kernel counts every digit of every pass in a single
 // read of the keys; then one "onesweep" scatter kernel per pass ranks a
-// 4096-element tile in LDS (wave ballots give the stable rank of each key
+// tile of 4096 elements (1024 for sorts of up to 256k elements, so that
+// they still span many workgroups) in LDS (wave ballots give the stable rank of each key
 // among equal digits), obtains the tile's global digit offsets through a
 // decoupled look-back over the preceding tiles, stages the tile in digit
 // order in LDS and writes it out in contiguous runs.  Tiles take their index
@@ -24,10 +25,14 @@ namespace {
 constexpr int SORT_BLOCK = 256;            // 4 waves
 constexpr int SORT_ITEMS = 16;             // keys per lane
 constexpr int SORT_TILE = SORT_BLOCK * SORT_ITEMS;  // 4096
+constexpr int SORT_ITEMS_SMALL = 4;        // keys per lane of the small tile
+constexpr int SORT_SMALL_MAX_N = 1 << 18;  // sorts up to this size use the small tile
 constexpr int WAVES = SORT_BLOCK / 64;
 static_assert(SORT_TILE == RB_SORT_TILE, "host-side tile count");
+static_assert(SORT_BLOCK * SORT_ITEMS_SMALL == RB_SORT_TILE_SMALL, "host-side tile count");
 constexpr int MAX_PASSES = 8;
 
+constexpr int LB = 8;  // look-back words read per round trip (per digit)
 constexpr uint32_t FLAG_AGG = 1u << 30;
 constexpr uint32_t FLAG_INC = 2u << 30;
 constexpr uint32_t VAL_MASK = (1u << 30) - 1u;
@@ -59,25 +64,26 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
     return v;
 }
 
-template <int RBITS>
+template <int RBITS, int ITEMS>
 __global__ __launch_bounds__(SORT_BLOCK) void k_histogram(const uint2 *kv, int n, int passes,
                                                           int sort_bits, uint32_t *hist,
                                                           const int *n_dev) {
     constexpr int RADIX = 1 << RBITS;
+    constexpr int TILE = SORT_BLOCK * ITEMS;
     if (n_dev && *n_dev < n) n = *n_dev;
-    if ((int)blockIdx.x * SORT_TILE >= n) return;
+    if ((int)blockIdx.x * TILE >= n) return;
     __shared__ uint32_t h[MAX_PASSES * RADIX];
     for (int i = threadIdx.x; i < passes * RADIX; i += SORT_BLOCK) h[i] = 0;
     __syncthreads();
-    int base = blockIdx.x * SORT_TILE;
-    uint32_t keys[SORT_ITEMS];
+    int base = blockIdx.x * TILE;
+    uint32_t keys[ITEMS];
 #pragma unroll
-    for (int k = 0; k < SORT_ITEMS; k++) {
+    for (int k = 0; k < ITEMS; k++) {
         int i = base + k * SORT_BLOCK + threadIdx.x;
         keys[k] = i < n ? kv[i].x : 0u;
     }
 #pragma unroll
-    for (int k = 0; k < SORT_ITEMS; k++) {
+    for (int k = 0; k < ITEMS; k++) {
         int i = base + k * SORT_BLOCK + threadIdx.x;
         if (i < n) {
             uint32_t key = keys[k];
@@ -94,7 +100,7 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_histogram(const uint2 *kv, int n
         if (h[i]) atomicAdd(&hist[i], h[i]);
 }
 
-template <int RBITS>
+template <int RBITS, int ITEMS>
 __global__ __launch_bounds__(SORT_BLOCK) void k_onesweep(const uint2 *__restrict__ src,
                                                          uint2 *__restrict__ dst, int n, int shift,
                                                          int bits, const uint32_t *hist,
@@ -102,11 +108,12 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_onesweep(const uint2 *__restrict
                                                          uint32_t *err, uint32_t err_bit,
                                                          const int *n_dev) {
     constexpr int RADIX = 1 << RBITS;
+    constexpr int TILE = SORT_BLOCK * ITEMS;
     if (n_dev && *n_dev < n) n = *n_dev;
     __shared__ uint32_t s_wave_hist[WAVES][RADIX];
     __shared__ uint32_t s_digit_start[RADIX];   // tile-local start of each digit
     __shared__ uint32_t s_global[RADIX];        // global base + tile prefix of each digit
-    __shared__ uint2 s_stage[SORT_TILE];
+    __shared__ uint2 s_stage[TILE];
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_scan[RADIX];
 
@@ -117,20 +124,20 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_onesweep(const uint2 *__restrict
     for (int i = tid; i < WAVES * RADIX; i += SORT_BLOCK) (&s_wave_hist[0][0])[i] = 0;
     __syncthreads();
     const int tile = (int)s_tile;
-    if (tile * SORT_TILE >= n) return;  // beyond the device-side count: nothing looks back here
-    const int base = tile * SORT_TILE + wave * (SORT_ITEMS * 64);
+    if (tile * TILE >= n) return;  // beyond the device-side count: nothing looks back here
+    const int base = tile * TILE + wave * (ITEMS * 64);
     const uint32_t dmask = (1u << bits) - 1u;
 
-    uint2 item[SORT_ITEMS];
-    uint32_t rank[SORT_ITEMS];
+    uint2 item[ITEMS];
+    uint32_t rank[ITEMS];
     // every load of the tile is in flight before the first ranking step
 #pragma unroll
-    for (int r = 0; r < SORT_ITEMS; r++) {
+    for (int r = 0; r < ITEMS; r++) {
         int i = base + r * 64 + lane;
         item[r] = i < n ? src[i] : make_uint2(0xffffffffu, 0xffffffffu);
     }
 #pragma unroll
-    for (int r = 0; r < SORT_ITEMS; r++) {
+    for (int r = 0; r < ITEMS; r++) {
         int i = base + r * 64 + lane;
         bool valid = i < n;
         uint32_t d = (item[r].x >> shift) & dmask;
@@ -165,18 +172,31 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_onesweep(const uint2 *__restrict
             s_global[d] = 0;
         } else {
             __hip_atomic_store(st, FLAG_AGG | run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // the predecessors' words LB at a time (independent loads, one
+            // round trip), summed back to the nearest inclusive one; an
+            // unpublished word is read again
             uint32_t prefix = 0, spins = 0;
             int t = tile - 1;
             while (true) {
-                uint32_t v = __hip_atomic_load(status + (size_t)t * RADIX + d, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT);
-                if ((v & ~VAL_MASK) == 0) {  // predecessor not published yet
-                    if (++spins > (1u << 26)) { atomicOr(err, err_bit); break; }
-                    continue;
+                uint32_t v[LB];
+#pragma unroll
+                for (int k = 0; k < LB; k++)
+                    v[k] = t - k >= 0 ? __hip_atomic_load(status + (size_t)(t - k) * RADIX + d,
+                                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                      : FLAG_INC;  // (never reached: tile 0 is inclusive)
+                int k = 0;
+                bool done = false;
+                for (; k < LB; k++) {
+                    if ((v[k] & ~VAL_MASK) == 0) break;  // not published yet
+                    prefix += v[k] & VAL_MASK;
+                    if (v[k] & FLAG_INC) {
+                        done = true;
+                        break;
+                    }
                 }
-                prefix += v & VAL_MASK;
-                if (v & FLAG_INC) break;
-                t--;
+                if (done) break;
+                t -= k;
+                if (k < LB && ++spins > (1u << 26)) { atomicOr(err, err_bit); break; }
             }
             __hip_atomic_store(st, FLAG_INC | (prefix + run), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
@@ -202,7 +222,7 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_onesweep(const uint2 *__restrict
     }
     __syncthreads();
     // stage in digit order
-    for (int r = 0; r < SORT_ITEMS; r++) {
+    for (int r = 0; r < ITEMS; r++) {
         int i = base + r * 64 + lane;
         if (i < n) {
             uint32_t d = (item[r].x >> shift) & dmask;
@@ -211,8 +231,8 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_onesweep(const uint2 *__restrict
         }
     }
     __syncthreads();
-    int tile_n = n - tile * SORT_TILE;
-    if (tile_n > SORT_TILE) tile_n = SORT_TILE;
+    int tile_n = n - tile * TILE;
+    if (tile_n > TILE) tile_n = TILE;
     for (int k = tid; k < tile_n; k += SORT_BLOCK) {
         uint2 e = s_stage[k];
         uint32_t d = (e.x >> shift) & dmask;
@@ -267,29 +287,50 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan(const uint32_t *in, uint32_
         if (w < wave) wave_off += c;
         tile_sum += c;
     }
-    if (tid == 0) {
+    if (wave == 0) {
         uint64_t *st = status + tile;
         if (tile == 0) {
-            __hip_atomic_store(st, SFLAG_INC | tile_sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            s_prefix = 0;
+            if (lane == 0) {
+                __hip_atomic_store(st, SFLAG_INC | tile_sum, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+                s_prefix = 0;
+            }
         } else {
-            __hip_atomic_store(st, SFLAG_AGG | tile_sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0)
+                __hip_atomic_store(st, SFLAG_AGG | tile_sum, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            // lane k reads the word of tile t - k: 64 predecessors per round
+            // trip, summed back to the nearest inclusive one; from an
+            // unpublished word on, the words are read again
             uint64_t prefix = 0;
             uint32_t spins = 0;
             int t = tile - 1;
             while (true) {
-                uint64_t w = __hip_atomic_load(status + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if ((w & ~SVAL_MASK) == 0) {
-                    if (++spins > (1u << 26)) { atomicOr(err, err_bit); break; }
-                    continue;
+                const int tt = t - lane;
+                const uint64_t w = tt >= 0 ? __hip_atomic_load(status + tt, __ATOMIC_RELAXED,
+                                                               __HIP_MEMORY_SCOPE_AGENT)
+                                           : SFLAG_INC;  // (never reached: tile 0 is inclusive)
+                const uint64_t unpub = __ballot((w & ~SVAL_MASK) == 0);
+                const uint64_t inc = __ballot((w & SFLAG_INC) != 0);
+                const uint64_t stop = unpub | inc;
+                const int j = stop ? __ffsll((long long)stop) - 1 : 64;
+                const bool j_inc = j < 64 && ((inc >> j) & 1ull);
+                uint64_t mine = (lane < j || (lane == j && j_inc)) ? (w & SVAL_MASK) : 0ull;
+#pragma unroll
+                for (int o = 32; o >= 1; o >>= 1) mine += __shfl_xor(mine, o, 64);
+                prefix += mine;
+                if (j_inc) break;
+                t -= j;
+                if (j < 64 && ++spins > (1u << 26)) {
+                    if (lane == 0) atomicOr(err, err_bit);
+                    break;
                 }
-                prefix += w & SVAL_MASK;
-                if (w & SFLAG_INC) break;
-                t--;
             }
-            __hip_atomic_store(st, SFLAG_INC | (prefix + tile_sum), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-            s_prefix = prefix;
+            if (lane == 0) {
+                __hip_atomic_store(st, SFLAG_INC | (prefix + tile_sum), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+                s_prefix = prefix;
+            }
         }
     }
     __syncthreads();
@@ -332,19 +373,20 @@ __global__ void k_bound_search(const uint2 *kv, int n, uint32_t *dst, int n_dst,
     }
 }
 
-template <int RBITS>
-hipError_t radix_sort_impl(uint2 *kv, int n, int sort_bits, const SortWork &w, hipStream_t s,
-                           const int *n_dev) {
+template <int RBITS, int ITEMS>
+hipError_t radix_sort_tiles(uint2 *kv, int n, int sort_bits, const SortWork &w, hipStream_t s,
+                            const int *n_dev) {
     constexpr int RADIX = 1 << RBITS;
+    constexpr int TILE = SORT_BLOCK * ITEMS;
     const int passes = (sort_bits + RBITS - 1) / RBITS;
-    const int tiles = (n + SORT_TILE - 1) / SORT_TILE;
+    const int tiles = (n + TILE - 1) / TILE;
     if (passes > MAX_PASSES || tiles > w.max_tiles) return hipErrorInvalidValue;
     size_t zero_words = (size_t)passes * RADIX;
     size_t status_words = (size_t)passes * tiles * RADIX;
     // (the error word after the tile counters is sticky: not cleared)
     RB_LAUNCH(k_sort_clear, dim3(256), dim3(256), 0, s, w.hist, zero_words, w.status,
                        status_words, w.tile_ctr, (size_t)MAX_PASSES);
-    RB_LAUNCH(k_histogram<RBITS>, dim3(tiles), dim3(SORT_BLOCK), 0, s, kv, n, passes,
+    RB_LAUNCH((k_histogram<RBITS, ITEMS>), dim3(tiles), dim3(SORT_BLOCK), 0, s, kv, n, passes,
                        sort_bits, w.hist, n_dev);
     // look-back timeouts: a sticky bit in the caller's overflow word, else
     // in the word after the tile counters
@@ -354,7 +396,7 @@ hipError_t radix_sort_impl(uint2 *kv, int n, int sort_bits, const SortWork &w, h
     for (int p = 0; p < passes; p++) {
         int shift = p * RBITS;
         int bits = sort_bits - shift < RBITS ? sort_bits - shift : RBITS;
-        RB_LAUNCH(k_onesweep<RBITS>, dim3(tiles), dim3(SORT_BLOCK), 0, s, src, dst, n,
+        RB_LAUNCH((k_onesweep<RBITS, ITEMS>), dim3(tiles), dim3(SORT_BLOCK), 0, s, src, dst, n,
                            shift, bits, w.hist + (size_t)p * RADIX,
                            w.status + (size_t)p * tiles * RADIX, w.tile_ctr + p, err, err_bit,
                            n_dev);
@@ -363,6 +405,18 @@ hipError_t radix_sort_impl(uint2 *kv, int n, int sort_bits, const SortWork &w, h
     if (src != kv)  // odd pass count: copy back with a kernel (graph-capture friendly)
         RB_LAUNCH(k_copy_kv, dim3(256), dim3(256), 0, s, src, kv, n, n_dev);
     return rb_launch_status();
+}
+
+// The tile size: 1024 elements for sorts of up to SORT_SMALL_MAX_N (more
+// workgroups, fewer keys ranked per lane) when the workspace holds that many
+// tiles, else 4096.
+template <int RBITS>
+hipError_t radix_sort_impl(uint2 *kv, int n, int sort_bits, const SortWork &w, hipStream_t s,
+                           const int *n_dev) {
+    const int small_tiles = (n + RB_SORT_TILE_SMALL - 1) / RB_SORT_TILE_SMALL;
+    if (n <= SORT_SMALL_MAX_N && small_tiles <= w.max_tiles)
+        return radix_sort_tiles<RBITS, SORT_ITEMS_SMALL>(kv, n, sort_bits, w, s, n_dev);
+    return radix_sort_tiles<RBITS, SORT_ITEMS>(kv, n, sort_bits, w, s, n_dev);
 }
 
 }  // namespace
@@ -393,7 +447,7 @@ hipError_t rb_launch_status() {
 }
 
 size_t sort_status_words(int max_n, int digit_bits) {
-    int tiles = (max_n + SORT_TILE - 1) / SORT_TILE;
+    int tiles = rb_sort_tiles(max_n);  // small tiles
     if (tiles < 1) tiles = 1;
     return (size_t)MAX_PASSES * tiles * (1u << digit_bits);
 }
@@ -464,9 +518,8 @@ struct RadixSortHandle {
 extern "C" mi_rb_status mi_rb_radix_sort_create(int max_elements, void **out_sort) {
     if (!out_sort || max_elements <= 0) return MI_RB_INVALID_ARG;
     RadixSortHandle *h = new RadixSortHandle();
-    int tiles = (max_elements + SORT_TILE - 1) / SORT_TILE;
     h->w.max_n = max_elements;
-    h->w.max_tiles = tiles;
+    h->w.max_tiles = rb_sort_tiles(max_elements);
     size_t status_words = sort_status_words(max_elements, 8);
     int scan_tiles = (max_elements + SCAN_TILE - 1) / SCAN_TILE;
     if (hipMalloc(&h->w.tmp, sizeof(uint2) * (size_t)max_elements) != hipSuccess ||

@@ -325,12 +325,14 @@ hipError_t launch_cache_order(const DeviceWorld &d, const StepParams &p, hipStre
 hipError_t launch_copy_transforms(const DeviceWorld &d, int n, float4 *dst, hipStream_t s);
 hipError_t launch_integrate(const DeviceWorld &d, const StepParams &p, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
-// elements per tile of the radix sort and of the scan (their look-back
-// status arrays hold one entry per tile)
+// elements per tile of the radix sort (4096, or 1024 for small sorts: the
+// status arrays are sized for the small tile) and of the scan (their
+// look-back status arrays hold one entry per tile)
 constexpr int RB_SORT_TILE = 4096;
+constexpr int RB_SORT_TILE_SMALL = 1024;
 constexpr int RB_SCAN_TILE = 4096;
 constexpr int RB_LARGE_CHUNK = 4096;  // bodies per chunk of the large-body pair search
-inline int rb_sort_tiles(int n) { return (n + RB_SORT_TILE - 1) / RB_SORT_TILE; }
+inline int rb_sort_tiles(int n) { return (n + RB_SORT_TILE_SMALL - 1) / RB_SORT_TILE_SMALL; }
 inline size_t rb_scan_status_words(int n) { return 4 + 2 * (size_t)((n + RB_SCAN_TILE - 1) / RB_SCAN_TILE); }
 hipError_t launch_mlp_warm(const DeviceWorld &d, const StepParams &p, hipStream_t s);
 hipError_t launch_mlp_targets(const DeviceWorld &d, int capacity, float *x_out, float *t_out,
