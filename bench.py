@@ -478,7 +478,8 @@ def main():
     w.set_state(**after)
     w.use_graph(not args.eager)
     prm = w.params()
-    passes = (prm["key_bits"] + 7) // 8
+    digit = tuning.get("sort_digit_bits") or 8  # the body-key sort's digit width
+    passes = (prm["key_bits"] + digit - 1) // digit
     sweeps = 2 * scene.iterations
     scene_tiles = prm["tile_mode"] == 1
     # the dominant kernel is the tile solve (spatial tiles: one launch per

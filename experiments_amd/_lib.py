@@ -31,7 +31,8 @@ MLP_NW, MLP_NB = 64 * 16 + 64 * 64 + 3 * 64, 64 + 64 + 3
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [("solve_block", ctypes.c_int), ("narrow_waves", ctypes.c_int)]
+    _fields_ = [("solve_block", ctypes.c_int), ("narrow_waves", ctypes.c_int),
+                ("sort_tile", ctypes.c_int), ("sort_digit_bits", ctypes.c_int)]
 
 
 class Config(ctypes.Structure):

@@ -409,12 +409,14 @@ hipError_t radix_sort_tiles(uint2 *kv, int n, int sort_bits, const SortWork &w, 
 
 // The tile size: 1024 elements for sorts of up to SORT_SMALL_MAX_N (more
 // workgroups, fewer keys ranked per lane) when the workspace holds that many
-// tiles, else 4096.
+// tiles, else 4096; tile_keys (1024 or 4096) overrides the size rule.
 template <int RBITS>
 hipError_t radix_sort_impl(uint2 *kv, int n, int sort_bits, const SortWork &w, hipStream_t s,
-                           const int *n_dev) {
+                           const int *n_dev, int tile_keys) {
     const int small_tiles = (n + RB_SORT_TILE_SMALL - 1) / RB_SORT_TILE_SMALL;
-    if (n <= SORT_SMALL_MAX_N && small_tiles <= w.max_tiles)
+    const bool small = tile_keys == RB_SORT_TILE_SMALL ||
+                       (tile_keys != RB_SORT_TILE && n <= SORT_SMALL_MAX_N);
+    if (small && small_tiles <= w.max_tiles)
         return radix_sort_tiles<RBITS, SORT_ITEMS_SMALL>(kv, n, sort_bits, w, s, n_dev);
     return radix_sort_tiles<RBITS, SORT_ITEMS>(kv, n, sort_bits, w, s, n_dev);
 }
@@ -453,13 +455,13 @@ size_t sort_status_words(int max_n, int digit_bits) {
 }
 
 hipError_t launch_radix_sort(uint2 *kv, int n, int sort_bits, int digit_bits, const SortWork &w,
-                             hipStream_t s, const int *n_dev) {
+                             hipStream_t s, const int *n_dev, int tile_keys) {
     if (n <= 1 || sort_bits <= 0) return hipSuccess;
     switch (digit_bits) {
-        case 4: return radix_sort_impl<4>(kv, n, sort_bits, w, s, n_dev);
-        case 6: return radix_sort_impl<6>(kv, n, sort_bits, w, s, n_dev);
-        case 7: return radix_sort_impl<7>(kv, n, sort_bits, w, s, n_dev);
-        case 8: return radix_sort_impl<8>(kv, n, sort_bits, w, s, n_dev);
+        case 4: return radix_sort_impl<4>(kv, n, sort_bits, w, s, n_dev, tile_keys);
+        case 6: return radix_sort_impl<6>(kv, n, sort_bits, w, s, n_dev, tile_keys);
+        case 7: return radix_sort_impl<7>(kv, n, sort_bits, w, s, n_dev, tile_keys);
+        case 8: return radix_sort_impl<8>(kv, n, sort_bits, w, s, n_dev, tile_keys);
         default: return hipErrorInvalidValue;
     }
 }

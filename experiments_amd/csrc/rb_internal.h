@@ -278,8 +278,9 @@ struct SortWork {
 size_t sort_status_words(int max_n, int digit_bits);
 // n is the capacity; when n_dev is given the kernels read the element count
 // from device memory (min(n, *n_dev)), so the launch needs no host sync.
+// tile_keys: 1024 or 4096 elements per tile, 0 = chosen by n.
 hipError_t launch_radix_sort(uint2 *kv, int n, int sort_bits, int digit_bits, const SortWork &w,
-                             hipStream_t s, const int *n_dev = nullptr);
+                             hipStream_t s, const int *n_dev = nullptr, int tile_keys = 0);
 // out (may be null): exclusive prefix sums of in; idx_out (may be null, in
 // holding 0/1 flags): the indices of the set flags in increasing order, from
 // idx_out[*idx_base] when idx_base (a device count) is given

@@ -112,7 +112,7 @@ struct World {
     uint32_t *scan_status = nullptr;
     long long steps = 0;
     float last_ms = 0.f;
-    mi_rb_tuning tuning{0, 0};  // launch shapes; 0 = default
+    mi_rb_tuning tuning{0, 0, 0, 0};  // launch shapes; 0 = default
     std::vector<void *> allocs;
 
     template <typename T>
@@ -498,7 +498,9 @@ mi_rb_status enqueue_step(World &w, int last_stage) {
     RB_CHECK(launch_setup(d, p, s));
     if (ev) RB_CHECK(hipEventRecord(w.ev[1], s));
     if (last_stage < MI_RB_STAGE_SORT) return MI_RB_OK;
-    RB_CHECK(launch_radix_sort(d.kv, p.n_bodies, w.key_bits, 8, w.sw, s));
+    RB_CHECK(launch_radix_sort(d.kv, p.n_bodies, w.key_bits,
+                               w.tuning.sort_digit_bits ? w.tuning.sort_digit_bits : 8, w.sw, s,
+                               nullptr, w.tuning.sort_tile));
     if (ev) RB_CHECK(hipEventRecord(w.ev[2], s));
     if (last_stage < MI_RB_STAGE_PAIRS) return MI_RB_OK;
     RB_CHECK(launch_cell_start(d, p, s));
@@ -1083,6 +1085,10 @@ extern "C" mi_rb_status mi_rb_set_tuning(void *world, const mi_rb_tuning *t) {
     if (!w || !t) return MI_RB_INVALID_ARG;
     if (!(t->solve_block == 0 || t->solve_block == 256 || t->solve_block == 512)) return MI_RB_INVALID_ARG;
     if (t->narrow_waves < 0 || t->narrow_waves > 8) return MI_RB_INVALID_ARG;
+    if (!(t->sort_tile == 0 || t->sort_tile == 1024 || t->sort_tile == 4096)) return MI_RB_INVALID_ARG;
+    if (!(t->sort_digit_bits == 0 || t->sort_digit_bits == 4 || t->sort_digit_bits == 6 ||
+          t->sort_digit_bits == 7 || t->sort_digit_bits == 8))
+        return MI_RB_INVALID_ARG;
     w->tuning = *t;
     w->sp.solve_block = t->solve_block ? t->solve_block : DEFAULT_SOLVE_BLOCK;
     w->sp.narrow_waves = t->narrow_waves ? t->narrow_waves : DEFAULT_NARROW_WAVES;
@@ -1112,6 +1118,8 @@ extern "C" mi_rb_status mi_rb_get_tuning(void *world, mi_rb_tuning *t) {
     if (!w || !t) return MI_RB_INVALID_ARG;
     t->solve_block = w->tuning.solve_block ? w->tuning.solve_block : DEFAULT_SOLVE_BLOCK;
     t->narrow_waves = w->tuning.narrow_waves ? w->tuning.narrow_waves : DEFAULT_NARROW_WAVES;
+    t->sort_tile = w->tuning.sort_tile;
+    t->sort_digit_bits = w->tuning.sort_digit_bits;
     return MI_RB_OK;
 }
 
@@ -1148,24 +1156,41 @@ extern "C" mi_rb_status mi_rb_autotune(void *world, int steps, mi_rb_tuning *bes
     const int blocks[2] = {256, 512}, waves[4] = {2, 4, 6, 8};
     float best_ms = 0.f;
     mi_rb_tuning bt = keep;
+    // one trial: the step time of tuning t from the saved state
+    auto trial = [&](const mi_rb_tuning &t, float &ms) -> mi_rb_status {
+        mi_rb_status stt = mi_rb_set_tuning(world, &t);
+        if (stt == MI_RB_OK) stt = restore();
+        w->use_graph = true;
+        if (stt == MI_RB_OK) stt = mi_rb_step(world, 1);  // capture + warm
+        if (stt == MI_RB_OK) stt = restore();
+        if (stt == MI_RB_OK) stt = mi_rb_step(world, steps);
+        if (stt != MI_RB_OK) {
+            mi_rb_set_tuning(world, &keep);
+            w->use_graph = keep_graph;
+            return stt;
+        }
+        ms = w->last_ms / (float)steps;
+        return MI_RB_OK;
+    };
     int idx = 0;
     for (int bi = 0; bi < 2; bi++)
         for (int wi = 0; wi < 4; wi++, idx++) {
-            mi_rb_tuning t{blocks[bi], waves[wi]};
-            mi_rb_status stt = mi_rb_set_tuning(world, &t);
-            if (stt == MI_RB_OK) stt = restore();
-            w->use_graph = true;
-            if (stt == MI_RB_OK) stt = mi_rb_step(world, 1);  // capture + warm
-            if (stt == MI_RB_OK) stt = restore();
-            if (stt == MI_RB_OK) stt = mi_rb_step(world, steps);
-            if (stt != MI_RB_OK) {
-                mi_rb_set_tuning(world, &keep);
-                w->use_graph = keep_graph;
-                return stt;
-            }
-            float ms = w->last_ms / (float)steps;
+            const mi_rb_tuning t{blocks[bi], waves[wi], keep.sort_tile, keep.sort_digit_bits};
+            float ms = 0.f;
+            mi_rb_status stt = trial(t, ms);
+            if (stt != MI_RB_OK) return stt;
             if (ms_out) ms_out[idx] = ms;
             if (idx == 0 || ms < best_ms) { best_ms = ms; bt = t; }
+        }
+    // the body-key sort's tile and digit width with the fastest launch shape
+    const int tiles[2] = {1024, 4096}, digits[2] = {7, 8};
+    for (int ti = 0; ti < 2; ti++)
+        for (int di = 0; di < 2; di++) {
+            const mi_rb_tuning t{bt.solve_block, bt.narrow_waves, tiles[ti], digits[di]};
+            float ms = 0.f;
+            mi_rb_status stt = trial(t, ms);
+            if (stt != MI_RB_OK) return stt;
+            if (ms < best_ms) { best_ms = ms; bt = t; }
         }
     mi_rb_status stt = mi_rb_set_tuning(world, &bt);
     if (stt != MI_RB_OK) return stt;

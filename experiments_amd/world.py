@@ -248,14 +248,14 @@ class World:
         self.cfg.residual_tolerance = float(tolerance)
         self.cfg.warm_start = int(warm_start)
 
-    def set_tuning(self, solve_block=0, narrow_waves=0):
-        t = L.Tuning(solve_block, narrow_waves)
+    def set_tuning(self, solve_block=0, narrow_waves=0, sort_tile=0, sort_digit_bits=0):
+        t = L.Tuning(solve_block, narrow_waves, sort_tile, sort_digit_bits)
         L.call("mi_rb_set_tuning", self.h, ctypes.byref(t))
 
     def get_tuning(self):
         t = L.Tuning()
         L.call("mi_rb_get_tuning", self.h, ctypes.byref(t))
-        return {"solve_block": t.solve_block, "narrow_waves": t.narrow_waves}
+        return {k: getattr(t, k) for k, _ in L.Tuning._fields_}
 
     def autotune(self, steps=5):
         """Time every launch shape from the current state (restored after
@@ -264,7 +264,7 @@ class World:
         ms = np.zeros(8, np.float32)
         L.call("mi_rb_autotune", self.h, int(steps), ctypes.byref(t), _ptr(ms))
         shapes = [(b, wv) for b in (256, 512) for wv in (2, 4, 6, 8)]
-        return ({"solve_block": t.solve_block, "narrow_waves": t.narrow_waves},
+        return ({k: getattr(t, k) for k, _ in L.Tuning._fields_},
                 {"%d/%d" % sh: float(m) for sh, m in zip(shapes, ms)})
 
     def set_warmstart_weights(self, w_bf16, bias):

@@ -415,8 +415,10 @@ mi_rb_status mi_rb_set_solve_timing(void *world, int enable);
  * its work-group sizes in the host code).  0 = default.  Changing them drops
  * the captured graph; results do not depend on them. */
 typedef struct mi_rb_tuning {
-    int solve_block;   /* threads per scene-tile solver workgroup: 256 or 512 */
-    int narrow_waves;  /* narrowphase waves per SIMD: 1..8 */
+    int solve_block;     /* threads per scene-tile solver workgroup: 256 or 512 */
+    int narrow_waves;    /* narrowphase waves per SIMD: 1..8 */
+    int sort_tile;       /* body-key sort, keys per tile: 1024 or 4096; 0 = by key count */
+    int sort_digit_bits; /* body-key sort, bits per radix digit: 4, 6, 7 or 8; 0 = 8 */
 } mi_rb_tuning;
 mi_rb_status mi_rb_set_tuning(void *world, const mi_rb_tuning *t);
 mi_rb_status mi_rb_get_tuning(void *world, mi_rb_tuning *t);
@@ -424,7 +426,9 @@ mi_rb_status mi_rb_get_tuning(void *world, mi_rb_tuning *t);
  * the world's current state (restored after each trial, so the world ends
  * where it started), applies the fastest and returns it.  ms_out (may be
  * NULL) receives the per-candidate step times in the order solve_block in
- * {256, 512} x narrow_waves in {2, 4, 6, 8}. */
+ * {256, 512} x narrow_waves in {2, 4, 6, 8}.  With the fastest of those, the
+ * body-key sort's sort_tile in {1024, 4096} x sort_digit_bits in {7, 8} is
+ * timed the same way and the fastest kept (those times are not reported). */
 mi_rb_status mi_rb_autotune(void *world, int steps, mi_rb_tuning *best, float ms_out[8]);
 
 /* ---- host utility (no device work): convex hull + polyhedral features ---- */

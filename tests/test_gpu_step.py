@@ -62,6 +62,23 @@ def test_broadphase_bit_exact(world):
     assert w.stats()["overflow_flags"] == 0
 
 
+@pytest.mark.parametrize("sort_tile,digit_bits", [(1024, 7), (4096, 8), (4096, 6), (1024, 4)])
+def test_body_sort_tunings_bit_exact(world, sort_tile, digit_bits):
+    """Every tile size and digit width the tuner may pick for the body-key
+    sort gives the oracle's sorted pairs and pair list."""
+    name, w = world
+    s = Snapshot(w)
+    w.set_tuning(sort_tile=sort_tile, sort_digit_bits=digit_bits)
+    try:
+        w.step_partial(L.STAGE_PAIRS)
+        _, _, skv, pairs = oracle_broadphase(s)
+        assert np.array_equal(w.sorted_kv(), skv)
+        assert np.array_equal(w.pairs(), pairs)
+    finally:
+        w.set_tuning()
+    assert w.stats()["overflow_flags"] == 0
+
+
 def test_narrowphase_contacts(world):
     name, w = world
     s = Snapshot(w)
