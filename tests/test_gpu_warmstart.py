@@ -230,6 +230,14 @@ def test_scene_tolerance_matches_oracle(warm):
     assert st["friction_iterations"] == its[:, 1].max()
     assert abs(st["mean_normal_iterations"] - its[:, 0].mean()) < 1e-3
     assert its[:, 0].min() < its[:, 0].max()  # scenes really stop at different sweeps
+    # the per-scene record: sweeps per phase of every scene with rows, in
+    # scene order (the oracle's segments), and final residuals below the
+    # tolerance wherever a phase stopped before its cap
+    ts = w.read(L.BUF_TILE_STATS, np.int32, (4,))
+    act = ts[:, 0] >= 0
+    assert np.array_equal(ts[act, 0], its[:, 0]) and np.array_equal(ts[act, 1], its[:, 1])
+    res = ts[act, 2:4].copy().view(np.float32)
+    assert np.all(res[its[:, 0] < 60, 0] < 1e-3) and np.all(res[its[:, 1] < 60, 1] < 1e-3)
     w.close()
 
 
