@@ -949,12 +949,49 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_clip(
         // sat_bound up to rounding.  The degenerate-axis test is axis_of's.
         // Lane's axis k = base + lane as (e0, e1), stepped by 64 without a
         // division per axis.
-        const int q64 = nEB > 0 ? NP_BLOCK / nEB : 0, r64 = NP_BLOCK - q64 * nEB;
-        int e0 = nEB > 0 ? lane / nEB : 0, e1 = lane - e0 * nEB;
-        for (int base = 0; base < ne; base += NP_BLOCK) {
-            const int k = base + lane;
+        // Rows and columns of the edge-axis grid that the bound drops whole:
+        // every axis c = ea x eb is perpendicular to ea, so |dot(dc, c)| is
+        // at most dc's distance from ea's direction, |dc x ea| / |ea|.  When
+        // that is below rho - margin - face_d (less a rounding allowance),
+        // rho - |dot(dc, c)| - margin exceeds face_d on every axis of the row
+        // and the bound drops them all.  The other rows and columns are
+        // listed in increasing order, so the axes reaching the bound keep
+        // their order and their index k.
+        int *sRA = (int *)&sPoly[0][0] + 2 * NP_BLOCK, *sRB = sRA + RB_MAX_HULL_EDGES;
+        const float T = rho - margin - face_d - 1e-5f * (1.f + sqrtf(dot3(dc, dc)));
+        int naA = 0, naB = 0;
+        for (int base = 0; base < hA.num_edges; base += NP_BLOCK) {
+            const int i = base + lane;
+            bool act = false;
+            if (i < hA.num_edges) {
+                const float4 e = sEA[i], x = cross3(dc, e);
+                act = !(T > 0.f && dot3(x, x) < T * T * dot3(e, e));
+            }
+            const uint64_t m = __ballot(act);
+            if (act) sRA[naA + __popcll(m & lanemask_lt())] = i;
+            naA += __popcll(m);
+        }
+        for (int base = 0; base < nEB; base += NP_BLOCK) {
+            const int i = base + lane;
+            bool act = false;
+            if (i < nEB) {
+                const float4 e = sEB[i], x = cross3(dc, e);
+                act = !(T > 0.f && dot3(x, x) < T * T * dot3(e, e));
+            }
+            const uint64_t m = __ballot(act);
+            if (act) sRB[naB + __popcll(m & lanemask_lt())] = i;
+            naB += __popcll(m);
+        }
+        __syncthreads();
+        const int nact = naA * naB;
+        const int q64 = naB > 0 ? NP_BLOCK / naB : 0, r64 = NP_BLOCK - q64 * naB;
+        int i0 = naB > 0 ? lane / naB : 0, i1 = lane - i0 * naB;
+        for (int base = 0; base < nact; base += NP_BLOCK) {
             bool keep = false;
-            if (k < ne) {
+            int k = 0;
+            if (base + lane < nact) {
+                const int e0 = sRA[i0], e1 = sRB[i1];
+                k = e0 * nEB + e1;
                 const float4 c = cross3(sEA[e0], sEB[e1]);
                 if (fabsf(c.x) > 1e-6f || fabsf(c.y) > 1e-6f || fabsf(c.z) > 1e-6f) {
                     const float inv = __builtin_amdgcn_rsqf(dot3(c, c));
@@ -967,11 +1004,11 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_clip(
                     keep = !(fminf(rho + dd, lb1) - margin > face_d);
                 }
             }
-            e0 += q64;
-            e1 += r64;
-            if (e1 >= nEB) {
-                e1 -= nEB;
-                e0++;
+            i0 += q64;
+            i1 += r64;
+            if (i1 >= naB) {
+                i1 -= naB;
+                i0++;
             }
             const uint64_t m = __ballot(keep);
             if (keep) sQ[qn + __popcll(m & lanemask_lt())] = k;
