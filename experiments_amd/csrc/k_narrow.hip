@@ -789,12 +789,23 @@ __global__ __launch_bounds__(NP_BLOCK) void k_narrow_clip(
 
     const int lane = threadIdx.x;
     const int n_cand = (int)*n_cand_ptr;
-    const int G = gridDim.x;
-    for (int r = 0;; r++) {
-        const int i = r * G + ((r & 1) ? (G - 1 - (int)blockIdx.x) : (int)blockIdx.x);
-        if (i >= n_cand) break;
-        const int p = (int)order[i].y;
-        const int2 pr = pairs[p];
+    const int G = gridDim.x, bid = (int)blockIdx.x;
+    // candidate of round r (boustrophedon over the cost-sorted order); the
+    // next round's pair and the index of the one after are loaded while
+    // this round's candidate runs
+    auto cand_at = [&](int r) { return r * G + ((r & 1) ? (G - 1 - bid) : bid); };
+    int i = cand_at(0);
+    int p = i < n_cand ? (int)order[i].y : 0;
+    int2 pr = i < n_cand ? pairs[p] : make_int2(0, 0);
+    int i_nx = cand_at(1);
+    int p_nx = i_nx < n_cand ? (int)order[i_nx].y : 0;
+    int2 pr_nx = make_int2(0, 0);
+    int i_nx2 = 0, p_nx2 = 0;
+    for (int r = 0; i < n_cand;
+         r++, i = i_nx, p = p_nx, pr = pr_nx, i_nx = i_nx2, p_nx = p_nx2) {
+        pr_nx = i_nx < n_cand ? pairs[p_nx] : make_int2(0, 0);
+        i_nx2 = cand_at(r + 2);
+        p_nx2 = i_nx2 < n_cand ? (int)order[i_nx2].y : 0;
         int a = pr.x, b = pr.y;
         float imA = inv_mass[a], imB = inv_mass[b];
         const HullInfo hA = hulls[shape[a]];
