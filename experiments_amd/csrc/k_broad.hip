@@ -255,13 +255,22 @@ __global__ __launch_bounds__(FP_BLOCK) void k_find_pairs(
                 if ((occ[nk[k] >> 5] >> (nk[k] & 31u)) & 1u) u0[k] = cell_start[nk[k]];
             }
         }
+        // one walk: a lane's first HITS overlaps stay in registers, so the
+        // members are walked again only by a lane that found more
+        constexpr int HITS = 4;
+        int hit[HITS] = {0, 0, 0, 0};
         uint32_t cnt = 0;
 #pragma unroll
         for (int k = 0; k < C; k++)
             for (uint32_t u = u0[k]; u < (uint32_t)n; u++) {
                 const uint2 f = kv[u];
                 if (f.x != nk[k]) break;
-                if ((int)f.y < i && aabb_overlap(mn0, mx0, smn[u], smx[u])) cnt++;
+                if ((int)f.y < i && aabb_overlap(mn0, mx0, smn[u], smx[u])) {
+#pragma unroll
+                    for (int h = 0; h < HITS; h++)
+                        if (cnt == (uint32_t)h) hit[h] = (int)f.y;
+                    cnt++;
+                }
             }
         uint32_t incl = cnt;
 #pragma unroll
@@ -271,13 +280,19 @@ __global__ __launch_bounds__(FP_BLOCK) void k_find_pairs(
         }
         c = __shfl(incl, LANES - 1, LANES);
         uint32_t v = incl - cnt;
+        if (cnt <= (uint32_t)HITS) {
 #pragma unroll
-        for (int k = 0; k < C; k++)
-            for (uint32_t u = u0[k]; u < (uint32_t)n && v < (uint32_t)K; u++) {
-                const uint2 f = kv[u];
-                if (f.x != nk[k]) break;
-                if ((int)f.y < i && aabb_overlap(mn0, mx0, smn[u], smx[u])) lst[v++] = (int)f.y;
-            }
+            for (int h = 0; h < HITS; h++)
+                if ((uint32_t)h < cnt && v + h < (uint32_t)K) lst[v + h] = hit[h];
+        } else {
+#pragma unroll
+            for (int k = 0; k < C; k++)
+                for (uint32_t u = u0[k]; u < (uint32_t)n && v < (uint32_t)K; u++) {
+                    const uint2 f = kv[u];
+                    if (f.x != nk[k]) break;
+                    if ((int)f.y < i && aabb_overlap(mn0, mx0, smn[u], smx[u])) lst[v++] = (int)f.y;
+                }
+        }
     }
     if (large_num) {
         const int l0 = large_first[sc], nl = large_num[sc];
