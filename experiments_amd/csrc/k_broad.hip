@@ -546,8 +546,11 @@ __global__ __launch_bounds__(256) void k_hg_cells(const uint2 *__restrict__ kv, 
 // the scanned counts (slot order within a range is that of the atomics;
 // k_pairs_group orders it).  The entry visits the 27 cells around it on its
 // own level and on every coarser level that holds bodies, and walks only the
-// occupied ones.
-constexpr int HG_LANES = 8;
+// occupied ones.  With lane_hits the count pass also records each lane's
+// pair count and its first HG_KEEP partners, and the write pass stores
+// those without walking again (only a lane that found more walks again).
+constexpr int HG_LANES = RB_HG_LANES;
+constexpr int HG_KEEP = RB_HG_KEEP;
 template <int WRITE>
 __global__ __launch_bounds__(256) void k_hg_pairs(int n, const uint2 *__restrict__ kv,
                                                   const uint32_t *__restrict__ cell_start,
@@ -558,7 +561,8 @@ __global__ __launch_bounds__(256) void k_hg_pairs(int n, const uint2 *__restrict
                                                   uint32_t *__restrict__ counts,
                                                   const uint32_t *__restrict__ offsets,
                                                   uint32_t *__restrict__ cursor, int max_pairs,
-                                                  uint32_t *__restrict__ ys) {
+                                                  uint32_t *__restrict__ ys,
+                                                  uint32_t *__restrict__ lane_hits) {
     constexpr int C = (27 + HG_LANES - 1) / HG_LANES;
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     const int t = gid / HG_LANES, lane = gid % HG_LANES;
@@ -566,6 +570,20 @@ __global__ __launch_bounds__(256) void k_hg_pairs(int n, const uint2 *__restrict
     uint2 e = kv[t];
     if (e.x == KEY_NONE) return;
     const int x = (int)e.y;
+    uint32_t *rec = lane_hits ? lane_hits + (size_t)gid * (1 + HG_KEEP) : nullptr;
+    if (WRITE && rec) {
+        const uint32_t found = rec[0];
+        if (found <= (uint32_t)HG_KEEP) {
+            for (uint32_t h = 0; h < found; h++) {
+                const int y = (int)rec[1 + h];
+                const uint32_t hi = (uint32_t)(x > y ? x : y), lo = (uint32_t)(x > y ? y : x);
+                const uint32_t pos = offsets[hi] + atomicAdd(&cursor[hi], 1u);
+                if (pos < (uint32_t)max_pairs) ys[pos] = lo;
+            }
+            return;
+        }
+    }
+    uint32_t found = 0, keep[HG_KEEP] = {0, 0};
     const int lx = (int)(e.x >> g.cell_bits);
     const float4 mn0 = smn[t], mx0 = smx[t];
     for (int l = lx; l < g.levels; l++) {
@@ -598,11 +616,20 @@ __global__ __launch_bounds__(256) void k_hg_pairs(int n, const uint2 *__restrict
                 const uint32_t hi = (uint32_t)(x > y ? x : y), lo = (uint32_t)(x > y ? y : x);
                 if (!WRITE) {
                     atomicAdd(&counts[hi], 1u);
+#pragma unroll
+                    for (int h = 0; h < HG_KEEP; h++)
+                        if (found == (uint32_t)h) keep[h] = (uint32_t)y;
+                    found++;
                 } else {
                     uint32_t pos = offsets[hi] + atomicAdd(&cursor[hi], 1u);
                     if (pos < (uint32_t)max_pairs) ys[pos] = lo;
                 }
             }
+    }
+    if (!WRITE && rec) {
+        rec[0] = found;
+#pragma unroll
+        for (int h = 0; h < HG_KEEP; h++) rec[1 + h] = keep[h];
     }
 }
 
@@ -971,7 +998,8 @@ hipError_t launch_hgrid_pairs(const float4 *mn, const float4 *mx, int n, float i
                               uint32_t *counts, uint32_t *offsets, uint32_t *cursor,
                               uint32_t *big, uint32_t *total, uint32_t *ys, int max_pairs,
                               int2 *pairs, int *num_pairs, int *overflow, const SortWork &sw,
-                              uint32_t *scan_status, hipStream_t s, hipEvent_t *ev) {
+                              uint32_t *scan_status, hipStream_t s, hipEvent_t *ev,
+                              uint32_t *lane_hits) {
     HGridP g;
     g.inv_cell0 = inv_cell0;
     g.dims[0] = dims[0]; g.dims[1] = dims[1]; g.dims[2] = dims[2];
@@ -997,13 +1025,14 @@ hipError_t launch_hgrid_pairs(const float4 *mn, const float4 *mx, int n, float i
                            occ, level_used, smn, smx);
         RB_LAUNCH(k_hg_pairs<0>, dim3(hg_blocks), dim3(256), 0, s, n, kv, cell_start, occ,
                            level_used, smn, smx, g, counts, (const uint32_t *)nullptr, cursor,
-                           max_pairs, ys);
+                           max_pairs, ys, lane_hits);
     }
     e = launch_exclusive_scan(counts, offsets, n, total, scan_status, s, nullptr, overflow);
     if (e != hipSuccess) return e;
     if (n > 0)
         RB_LAUNCH(k_hg_pairs<1>, dim3(hg_blocks), dim3(256), 0, s, n, kv, cell_start, occ,
-                           level_used, smn, smx, g, counts, offsets, cursor, max_pairs, ys);
+                           level_used, smn, smx, g, counts, offsets, cursor, max_pairs, ys,
+                           lane_hits);
     if (ev) hipEventRecord(ev[3], s);
     launch_pairs_group(n, counts, offsets, total, max_pairs, ys, pairs, num_pairs, overflow, big,
                        big_n, blocks, s);

@@ -339,14 +339,19 @@ hipError_t launch_mlp_warm(const DeviceWorld &d, const StepParams &p, hipStream_
 hipError_t launch_mlp_targets(const DeviceWorld &d, int capacity, float *x_out, float *t_out,
                               hipStream_t s);
 
-// the stand-alone grid API
+// the stand-alone grid API.  lane_hits (may be null): RB_HG_LANES x (1 +
+// RB_HG_KEEP) words per body, where the count pass leaves each lane's pairs
+// for the write pass
+#define RB_HG_LANES 8
+#define RB_HG_KEEP 2
 hipError_t launch_hgrid_pairs(const float4 *mn, const float4 *mx, int n, float inv_cell0,
                               const int dims[3], int cell_bits, int levels, uint2 *kv,
                               uint32_t *cell_start, uint32_t *occ, float4 *smn, float4 *smx,
                               uint32_t *counts, uint32_t *offsets, uint32_t *cursor,
                               uint32_t *big, uint32_t *total, uint32_t *ys, int max_pairs,
                               int2 *pairs, int *num_pairs, int *overflow, const SortWork &sw,
-                              uint32_t *scan_status, hipStream_t s, hipEvent_t *ev);
+                              uint32_t *scan_status, hipStream_t s, hipEvent_t *ev,
+                              uint32_t *lane_hits);
 hipError_t launch_sap_pairs(const float4 *mn, const float4 *mx, const int *is_large, int n,
                             int axis, uint2 *kv, float4 *smn, float4 *smx, uint32_t *counts,
                             uint32_t *offsets, uint32_t *cursor, uint32_t *big, uint32_t *lg,

@@ -1711,11 +1711,12 @@ struct Grid {
     int *overflow = nullptr;
     SortWork sw{};
     uint32_t *scan_status = nullptr;
+    uint32_t *lane_hits = nullptr;       // per body and search lane: pair count, first partners
     int cell_bits = 0;
     hipEvent_t ev[5] = {};
     void release() {
         void *ps[] = {kv, ys, cursor, big, cell_start, counts, offsets, total, overflow, sw.tmp,
-                      sw.hist, sw.status, sw.tile_ctr, scan_status, occ, smn, smx};
+                      sw.hist, sw.status, sw.tile_ctr, scan_status, occ, smn, smx, lane_hits};
         for (void *p : ps) if (p) hipFree(p);
         for (auto &e : ev) if (e) hipEventDestroy(e);
     }
@@ -1746,7 +1747,8 @@ extern "C" mi_rb_status mi_rb_grid_create(const mi_rb_grid_desc *desc, int max_a
               dalloc(&g->sw.hist, 8 * 256) == hipSuccess &&
               dalloc(&g->sw.status, sort_status_words(n, 8)) == hipSuccess &&
               dalloc(&g->sw.tile_ctr, 16) == hipSuccess &&
-              dalloc(&g->scan_status, rb_scan_status_words(n)) == hipSuccess;
+              dalloc(&g->scan_status, rb_scan_status_words(n)) == hipSuccess &&
+              dalloc(&g->lane_hits, (size_t)n * RB_HG_LANES * (1 + RB_HG_KEEP)) == hipSuccess;
     if (!ok) { g->release(); delete g; return MI_RB_DEVICE; }
     g->sw.max_n = n;
     g->sw.max_tiles = rb_sort_tiles(n);
@@ -1785,7 +1787,7 @@ extern "C" mi_rb_status mi_rb_grid_find_pairs(void *grid, const void *d_aabb_min
                                 g->counts, g->offsets, g->cursor, g->big, g->total, g->ys,
                                 g->desc.max_pairs,
                                 static_cast<int2 *>(d_pairs), static_cast<int *>(d_num_pairs),
-                                g->overflow, g->sw, g->scan_status, s, g->ev));
+                                g->overflow, g->sw, g->scan_status, s, g->ev, g->lane_hits));
     return MI_RB_OK;
 }
 
