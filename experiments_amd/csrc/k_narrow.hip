@@ -1844,7 +1844,7 @@ __global__ void k_compound_total(int max_pairs, int *num_pairs, const uint32_t *
 }  // namespace
 
 hipError_t launch_narrowphase(const DeviceWorld &d, const StepParams &p, hipStream_t s,
-                              const SortWork &sw, uint32_t *scan_status) {
+                              const SortWork &sw, uint32_t *scan_status, size_t scan_stride) {
     const int grid = rb_num_cus() * 4 * (p.narrow_waves > 0 ? p.narrow_waves : 4);
     // pass 1's pairs: the flags of pass 0 compacted into cand_kv, which
     // k_narrow_keys fills only after pass 1
@@ -1856,11 +1856,14 @@ hipError_t launch_narrowphase(const DeviceWorld &d, const StepParams &p, hipStre
               d.hulls, d.verts, d.planes, d.has_contact, d.narrow_cand);
     // the wave kernel's pairs, then the small-hull kernel's after them
     uint32_t *n_small = (uint32_t *)(d.counters + RB_CNT_FACE_SMALL);
-    hipError_t e0 = launch_compact_flags(d.has_contact, work, p.max_pairs, n_work, scan_status, s,
-                                         d.counters + RB_CNT_PAIRS, d.counters + RB_CNT_OVERFLOW);
+    hipError_t e0 = launch_compact_flags(d.has_contact, work, p.max_pairs, n_work,
+                                         scan_status + RB_SCAN_WORK * scan_stride, s,
+                                         d.counters + RB_CNT_PAIRS, d.counters + RB_CNT_OVERFLOW,
+                                         nullptr, true);
     if (e0 != hipSuccess) return e0;
-    e0 = launch_compact_flags(d.narrow_cand, work, p.max_pairs, n_small, scan_status, s,
-                              d.counters + RB_CNT_PAIRS, d.counters + RB_CNT_OVERFLOW, n_work);
+    e0 = launch_compact_flags(d.narrow_cand, work, p.max_pairs, n_small,
+                              scan_status + RB_SCAN_SMALL * scan_stride, s,
+                              d.counters + RB_CNT_PAIRS, d.counters + RB_CNT_OVERFLOW, n_work, true);
     if (e0 != hipSuccess) return e0;
     RB_LAUNCH(k_narrow_face, dim3(grid), dim3(NP_BLOCK), 0, s, p.max_pairs, n_work, work,
                        d.pairs, d.pos, d.quat, d.inv_mass, d.shape,
@@ -1874,9 +1877,10 @@ hipError_t launch_narrowphase(const DeviceWorld &d, const StepParams &p, hipStre
                            d.counters + RB_CNT_PAIRS, d.pairs, d.pos, d.quat, d.inv_mass, d.shape,
                            d.hulls, d.planes, p.friction, d.contact_slots, d.has_contact);
     hipError_t e = launch_exclusive_scan(d.narrow_cand, d.cand_off, p.max_pairs,
-                                         (uint32_t *)(d.counters + RB_CNT_NARROW_CAND), scan_status,
-                                         s, d.counters + RB_CNT_PAIRS,
-                                         d.counters + RB_CNT_OVERFLOW);
+                                         (uint32_t *)(d.counters + RB_CNT_NARROW_CAND),
+                                         scan_status + RB_SCAN_CAND * scan_stride, s,
+                                         d.counters + RB_CNT_PAIRS, d.counters + RB_CNT_OVERFLOW,
+                                         nullptr, nullptr, true);
     if (e != hipSuccess) return e;
     RB_LAUNCH(k_narrow_keys, dim3((p.max_pairs + 255) / 256), dim3(256), 0, s,
                        p.max_pairs, d.counters + RB_CNT_PAIRS, d.narrow_cand, d.cand_off,
@@ -1904,7 +1908,7 @@ hipError_t launch_narrowphase(const DeviceWorld &d, const StepParams &p, hipStre
 }
 
 hipError_t launch_compound_pairs(const DeviceWorld &d, const StepParams &p, hipStream_t s,
-                                 uint32_t *scan_status) {
+                                 uint32_t *scan_status, size_t scan_stride) {
     RB_LAUNCH(k_proxy_transforms, dim3((p.n_proxies + 255) / 256), dim3(256), 0, s,
                        p.n_proxies, p.n_bodies, d.proxy, d.children, d.pos, d.quat);
     const int grid = rb_num_cus() * 4;
@@ -1913,9 +1917,10 @@ hipError_t launch_compound_pairs(const DeviceWorld &d, const StepParams &p, hipS
                        d.counters + RB_CNT_PAIRS, d.pairs, d.shape, d.inv_mass, d.hulls, mv,
                        d.narrow_cand, d.counters + RB_CNT_OVERFLOW);
     hipError_t e = launch_exclusive_scan(d.narrow_cand, d.cand_off, p.max_pairs,
-                                         (uint32_t *)(d.counters + RB_CNT_NARROW_CAND), scan_status,
-                                         s, d.counters + RB_CNT_PAIRS,
-                                         d.counters + RB_CNT_OVERFLOW);
+                                         (uint32_t *)(d.counters + RB_CNT_NARROW_CAND),
+                                         scan_status + RB_SCAN_COMPOUND * scan_stride, s,
+                                         d.counters + RB_CNT_PAIRS, d.counters + RB_CNT_OVERFLOW,
+                                         nullptr, nullptr, true);
     if (e != hipSuccess) return e;
     RB_LAUNCH(k_compound_write, dim3(grid), dim3(256), 0, s, p.max_pairs,
                        d.counters + RB_CNT_PAIRS, d.pairs, d.shape, d.hulls, mv, d.proxy_first,

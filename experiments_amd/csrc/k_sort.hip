@@ -468,7 +468,7 @@ hipError_t launch_radix_sort(uint2 *kv, int n, int sort_bits, int digit_bits, co
 
 hipError_t launch_exclusive_scan(const uint32_t *in, uint32_t *out, int n, uint32_t *total,
                                  uint32_t *status, hipStream_t s, const int *n_dev, int *err,
-                                 uint32_t *idx_out, const uint32_t *idx_base) {
+                                 uint32_t *idx_out, const uint32_t *idx_base, bool precleared) {
     if (n <= 0) {
         RB_LAUNCH(k_scan_empty, dim3(1), dim3(1), 0, s, total);
         return rb_launch_status();
@@ -477,7 +477,8 @@ hipError_t launch_exclusive_scan(const uint32_t *in, uint32_t *out, int n, uint3
     // status: [0] tile counter, then 64-bit words (8-byte aligned)
     uint32_t *ctr = status;
     uint64_t *st = reinterpret_cast<uint64_t *>(status + 2);
-    RB_LAUNCH(k_fill_u32, dim3(64), dim3(256), 0, s, status, 0u, (size_t)(2 + 2 * tiles));
+    if (!precleared)
+        RB_LAUNCH(k_fill_u32, dim3(64), dim3(256), 0, s, status, 0u, (size_t)(2 + 2 * tiles));
     // a look-back timeout sets MI_RB_OVERFLOW_LOOKBACK in *err, else the
     // word after the tile counter (cleared with the status words)
     uint32_t *e = err ? reinterpret_cast<uint32_t *>(err) : status + 1;
@@ -489,9 +490,9 @@ hipError_t launch_exclusive_scan(const uint32_t *in, uint32_t *out, int n, uint3
 
 hipError_t launch_compact_flags(const uint32_t *flags, uint32_t *idx_out, int n, uint32_t *total,
                                 uint32_t *status, hipStream_t s, const int *n_dev, int *err,
-                                const uint32_t *idx_base) {
+                                const uint32_t *idx_base, bool precleared) {
     return launch_exclusive_scan(flags, nullptr, n, total, status, s, n_dev, err, idx_out,
-                                 idx_base);
+                                 idx_base, precleared);
 }
 
 hipError_t launch_bound_search(const uint2 *kv, int n, uint32_t *dst, int n_dst, int mode,

@@ -283,14 +283,21 @@ hipError_t launch_radix_sort(uint2 *kv, int n, int sort_bits, int digit_bits, co
                              hipStream_t s, const int *n_dev = nullptr, int tile_keys = 0);
 // out (may be null): exclusive prefix sums of in; idx_out (may be null, in
 // holding 0/1 flags): the indices of the set flags in increasing order, from
-// idx_out[*idx_base] when idx_base (a device count) is given
+// idx_out[*idx_base] when idx_base (a device count) is given.  precleared:
+// the status words are already zero (a step clears the regions of its scans
+// in its first kernel), so no clearing launch precedes the scan.
 hipError_t launch_exclusive_scan(const uint32_t *in, uint32_t *out, int n, uint32_t *total,
                                  uint32_t *status, hipStream_t s, const int *n_dev = nullptr,
                                  int *err = nullptr, uint32_t *idx_out = nullptr,
-                                 const uint32_t *idx_base = nullptr);
+                                 const uint32_t *idx_base = nullptr, bool precleared = false);
 hipError_t launch_compact_flags(const uint32_t *flags, uint32_t *idx_out, int n, uint32_t *total,
                                 uint32_t *status, hipStream_t s, const int *n_dev, int *err,
-                                const uint32_t *idx_base = nullptr);
+                                const uint32_t *idx_base = nullptr, bool precleared = false);
+// the step's scans, each on its own region of the scan status words
+enum {
+    RB_SCAN_PAIRS = 0, RB_SCAN_CONTACTS = 1, RB_SCAN_COMPOUND = 2,
+    RB_SCAN_WORK = 3, RB_SCAN_SMALL = 4, RB_SCAN_CAND = 5, RB_SCAN_REGIONS = 6
+};
 hipError_t launch_bound_search(const uint2 *kv, int n, uint32_t *dst, int n_dst, int mode,
                                hipStream_t s);
 
@@ -300,10 +307,12 @@ hipError_t launch_find_pairs(const DeviceWorld &d, const StepParams &p, hipStrea
 hipError_t launch_emit_pairs(const DeviceWorld &d, const StepParams &p, hipStream_t s);
 // compound pairs: proxy transforms, then one child pair per (child of A,
 // child of B) appended after the pairs (the compound pair itself gets none)
+// scan_status, scan_stride: the step's scan regions (zeroed by its first
+// kernel), region r at scan_status + r * scan_stride
 hipError_t launch_compound_pairs(const DeviceWorld &d, const StepParams &p, hipStream_t s,
-                                 uint32_t *scan_status);
+                                 uint32_t *scan_status, size_t scan_stride);
 hipError_t launch_narrowphase(const DeviceWorld &d, const StepParams &p, hipStream_t s,
-                              const SortWork &sw, uint32_t *scan_status);
+                              const SortWork &sw, uint32_t *scan_status, size_t scan_stride);
 hipError_t launch_constraints(const DeviceWorld &d, const StepParams &p, hipStream_t s);
 hipError_t launch_contact_total(const DeviceWorld &d, const StepParams &p, hipStream_t s);
 hipError_t launch_tile_bounds(const DeviceWorld &d, const StepParams &p, hipStream_t s);

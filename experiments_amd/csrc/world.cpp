@@ -57,12 +57,18 @@ int ilog2(uint64_t v) {
     return b;
 }
 
-// Start of a step: the counters (the overflow bits are sticky) and the
-// solver's per-sweep residual maxima.
-__global__ void k_zero_counters(int *c, float *residual, int n_residual) {
-    int i = threadIdx.x;
-    if (i < RB_CNT_COUNT && i != RB_CNT_OVERFLOW) c[i] = 0;
-    for (int k = i; k < n_residual; k += blockDim.x) residual[k] = 0.f;
+// Start of a step: the counters (the overflow bits are sticky), the
+// solver's per-sweep residual maxima and the scans' status words.
+__global__ void k_zero_counters(int *c, float *residual, int n_residual, uint32_t *scan_status,
+                                int n_scan_words) {
+    const int i = threadIdx.x;
+    if (blockIdx.x == 0) {
+        if (i < RB_CNT_COUNT && i != RB_CNT_OVERFLOW) c[i] = 0;
+        for (int k = i; k < n_residual; k += blockDim.x) residual[k] = 0.f;
+    }
+    // the status words of the step's scans (one region per scan)
+    for (int k = blockIdx.x * blockDim.x + i; k < n_scan_words; k += gridDim.x * blockDim.x)
+        scan_status[k] = 0u;
 }
 
 // Launch shapes used when mi_rb_tuning leaves a field 0 (the autotuner's
@@ -110,6 +116,7 @@ struct World {
     DeviceWorld d{};
     SortWork sw{};
     uint32_t *scan_status = nullptr;
+    size_t scan_stride = 0;  // words per scan region (RB_SCAN_REGIONS regions)
     long long steps = 0;
     float last_ms = 0.f;
     mi_rb_tuning tuning{0, 0, 0, 0};  // launch shapes; 0 = default
@@ -429,7 +436,8 @@ mi_rb_status derive_and_upload(World &w) {
     RB_CHECK(w.alloc(&w.sw.tile_ctr, 16));
     w.sw.err = d.counters + RB_CNT_OVERFLOW;
     int scan_cap = std::max(n, maxP);
-    RB_CHECK(w.alloc(&w.scan_status, rb_scan_status_words(scan_cap)));
+    w.scan_stride = rb_scan_status_words(scan_cap);
+    RB_CHECK(w.alloc(&w.scan_status, RB_SCAN_REGIONS * w.scan_stride));
 
     // ---- uploads ----
     hipStream_t s = w.stream;
@@ -493,8 +501,9 @@ mi_rb_status enqueue_step(World &w, int last_stage) {
     (void)hipGetLastError();  // start from a clean error state: launches below are checked one by one
     w.events_full = ev && last_stage >= MI_RB_STAGE_INTEGRATE;
     if (ev) RB_CHECK(hipEventRecord(w.ev[0], s));
-    RB_LAUNCH(k_zero_counters, dim3(1), dim3(256), 0, s, d.counters, d.residual,
-                       2 * (p.normal_iters + p.friction_iters + 1));
+    RB_LAUNCH(k_zero_counters, dim3(16), dim3(256), 0, s, d.counters, d.residual,
+                       2 * (p.normal_iters + p.friction_iters + 1), w.scan_status,
+                       (int)(RB_SCAN_REGIONS * w.scan_stride));
     RB_CHECK(launch_setup(d, p, s));
     if (ev) RB_CHECK(hipEventRecord(w.ev[1], s));
     if (last_stage < MI_RB_STAGE_SORT) return MI_RB_OK;
@@ -506,18 +515,22 @@ mi_rb_status enqueue_step(World &w, int last_stage) {
     RB_CHECK(launch_cell_start(d, p, s));
     RB_CHECK(launch_find_pairs(d, p, s));
     RB_CHECK(launch_exclusive_scan(d.pair_count, d.pair_off, p.n_bodies,
-                                   (uint32_t *)(d.counters + RB_CNT_SORT_TILE), w.scan_status, s,
-                                   nullptr, d.counters + RB_CNT_OVERFLOW));
+                                   (uint32_t *)(d.counters + RB_CNT_SORT_TILE),
+                                   w.scan_status + RB_SCAN_PAIRS * w.scan_stride, s, nullptr,
+                                   d.counters + RB_CNT_OVERFLOW, nullptr, nullptr, true));
     RB_CHECK(launch_emit_pairs(d, p, s));
-    if (p.n_proxies > 0) RB_CHECK(launch_compound_pairs(d, p, s, w.scan_status));
+    if (p.n_proxies > 0)
+        RB_CHECK(launch_compound_pairs(d, p, s, w.scan_status, w.scan_stride));
     if (ev) RB_CHECK(hipEventRecord(w.ev[3], s));
     if (last_stage < MI_RB_STAGE_NARROWPHASE) return MI_RB_OK;
-    RB_CHECK(launch_narrowphase(d, p, s, w.sw, w.scan_status));
+    RB_CHECK(launch_narrowphase(d, p, s, w.sw, w.scan_status, w.scan_stride));
     if (ev) RB_CHECK(hipEventRecord(w.ev[4], s));
     if (last_stage < MI_RB_STAGE_CONSTRAINTS) return MI_RB_OK;
     RB_CHECK(launch_exclusive_scan(d.has_contact, d.contact_off, p.max_pairs,
-                                   (uint32_t *)(d.counters + RB_CNT_CONTACT_TOTAL), w.scan_status,
-                                   s, d.counters + RB_CNT_PAIRS, d.counters + RB_CNT_OVERFLOW));
+                                   (uint32_t *)(d.counters + RB_CNT_CONTACT_TOTAL),
+                                   w.scan_status + RB_SCAN_CONTACTS * w.scan_stride, s,
+                                   d.counters + RB_CNT_PAIRS, d.counters + RB_CNT_OVERFLOW,
+                                   nullptr, nullptr, true));
     RB_CHECK(launch_contact_total(d, p, s));
     RB_CHECK(launch_constraints(d, p, s));
     if (p.warm_start == MI_RB_WARM_LEARNED) RB_CHECK(launch_mlp_warm(d, p, s));
