@@ -84,6 +84,13 @@ __global__ __launch_bounds__(256) void k_constraints(
     const int2 *__restrict__ pairs) {
     int np = *num_pairs_ptr;
     int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p == 0) {
+        // the contact count for the later kernels: the contact scan's total,
+        // capped at the capacity (no kernel of this launch reads it)
+        uint32_t t = (uint32_t)counters[RB_CNT_CONTACT_TOTAL];
+        if (t > (uint32_t)sp.max_contacts) t = (uint32_t)sp.max_contacts;
+        counters[RB_CNT_CONTACTS] = (int)t;
+    }
     if (p >= np || !has_contact[p]) return;
     int k = (int)contact_off[p];
     if (k >= sp.max_contacts) {
@@ -238,18 +245,8 @@ __global__ __launch_bounds__(256) void k_constraints(
     tile_kv[k] = make_uint2((uint32_t)t, (uint32_t)k);
 }
 
-__global__ void k_contact_total(const uint32_t *total, int *counters, int max_contacts) {
-    uint32_t t = *total;
-    if (t > (uint32_t)max_contacts) t = (uint32_t)max_contacts;
-    counters[RB_CNT_CONTACTS] = (int)t;
-}
-
-// tile_first/tile_cnt of every tile: zero, then the run boundaries of the
-// sorted tile keys
-__global__ void k_tile_clear(int n_tiles, int *tile_first, int *tile_cnt) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n_tiles) { tile_first[k] = 0; tile_cnt[k] = 0; }
-}
+// tile_first/tile_cnt of every tile (zeroed by the step's first kernel):
+// the run boundaries of the sorted tile keys
 __global__ void k_tile_bounds2(const uint2 *__restrict__ tile_kv, const int *counters,
                                int *tile_first, int *tile_cnt) {
     int nc = counters[RB_CNT_CONTACTS];
@@ -1271,16 +1268,7 @@ hipError_t launch_constraints(const DeviceWorld &d, const StepParams &p, hipStre
     return rb_launch_status();
 }
 
-hipError_t launch_contact_total(const DeviceWorld &d, const StepParams &p, hipStream_t s) {
-    RB_LAUNCH(k_contact_total, dim3(1), dim3(1), 0, s,
-                       (const uint32_t *)(d.counters + RB_CNT_CONTACT_TOTAL), d.counters,
-                       p.max_contacts);
-    return rb_launch_status();
-}
-
 hipError_t launch_tile_bounds(const DeviceWorld &d, const StepParams &p, hipStream_t s) {
-    RB_LAUNCH(k_tile_clear, dim3((p.n_tiles + 255) / 256), dim3(256), 0, s, p.n_tiles,
-                       d.tile_first, d.tile_cnt);
     RB_LAUNCH(k_tile_bounds2, dim3((p.max_contacts + 255) / 256), dim3(256), 0, s,
                        d.tile_kv, d.counters, d.tile_first, d.tile_cnt);
     return rb_launch_status();

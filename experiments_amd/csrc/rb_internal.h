@@ -314,7 +314,6 @@ hipError_t launch_compound_pairs(const DeviceWorld &d, const StepParams &p, hipS
 hipError_t launch_narrowphase(const DeviceWorld &d, const StepParams &p, hipStream_t s,
                               const SortWork &sw, uint32_t *scan_status, size_t scan_stride);
 hipError_t launch_constraints(const DeviceWorld &d, const StepParams &p, hipStream_t s);
-hipError_t launch_contact_total(const DeviceWorld &d, const StepParams &p, hipStream_t s);
 hipError_t launch_tile_bounds(const DeviceWorld &d, const StepParams &p, hipStream_t s);
 hipError_t launch_tiles(const DeviceWorld &d, const StepParams &p, hipStream_t s);
 // scene tiles: the solve's workgroup order, largest scene first

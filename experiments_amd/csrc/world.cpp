@@ -58,9 +58,10 @@ int ilog2(uint64_t v) {
 }
 
 // Start of a step: the counters (the overflow bits are sticky), the
-// solver's per-sweep residual maxima and the scans' status words.
+// solver's per-sweep residual maxima, the scans' status words and the solver
+// tiles' row ranges.
 __global__ void k_zero_counters(int *c, float *residual, int n_residual, uint32_t *scan_status,
-                                int n_scan_words) {
+                                int n_scan_words, int *tile_first, int *tile_cnt, int n_tiles) {
     const int i = threadIdx.x;
     if (blockIdx.x == 0) {
         if (i < RB_CNT_COUNT && i != RB_CNT_OVERFLOW) c[i] = 0;
@@ -69,6 +70,11 @@ __global__ void k_zero_counters(int *c, float *residual, int n_residual, uint32_
     // the status words of the step's scans (one region per scan)
     for (int k = blockIdx.x * blockDim.x + i; k < n_scan_words; k += gridDim.x * blockDim.x)
         scan_status[k] = 0u;
+    // the solver tiles' row ranges (k_tile_bounds2 sets the non-empty ones)
+    for (int k = blockIdx.x * blockDim.x + i; k < n_tiles; k += gridDim.x * blockDim.x) {
+        tile_first[k] = 0;
+        tile_cnt[k] = 0;
+    }
 }
 
 // Launch shapes used when mi_rb_tuning leaves a field 0 (the autotuner's
@@ -503,7 +509,8 @@ mi_rb_status enqueue_step(World &w, int last_stage) {
     if (ev) RB_CHECK(hipEventRecord(w.ev[0], s));
     RB_LAUNCH(k_zero_counters, dim3(16), dim3(256), 0, s, d.counters, d.residual,
                        2 * (p.normal_iters + p.friction_iters + 1), w.scan_status,
-                       (int)(RB_SCAN_REGIONS * w.scan_stride));
+                       (int)(RB_SCAN_REGIONS * w.scan_stride), d.tile_first, d.tile_cnt,
+                       p.n_tiles);
     RB_CHECK(launch_setup(d, p, s));
     if (ev) RB_CHECK(hipEventRecord(w.ev[1], s));
     if (last_stage < MI_RB_STAGE_SORT) return MI_RB_OK;
@@ -531,7 +538,6 @@ mi_rb_status enqueue_step(World &w, int last_stage) {
                                    w.scan_status + RB_SCAN_CONTACTS * w.scan_stride, s,
                                    d.counters + RB_CNT_PAIRS, d.counters + RB_CNT_OVERFLOW,
                                    nullptr, nullptr, true));
-    RB_CHECK(launch_contact_total(d, p, s));
     RB_CHECK(launch_constraints(d, p, s));
     if (p.warm_start == MI_RB_WARM_LEARNED) RB_CHECK(launch_mlp_warm(d, p, s));
     if (ev) RB_CHECK(hipEventRecord(w.ev[5], s));
