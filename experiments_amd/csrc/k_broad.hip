@@ -583,7 +583,7 @@ __global__ __launch_bounds__(256) void k_hg_pairs(int n, const uint2 *__restrict
             return;
         }
     }
-    uint32_t found = 0, keep[HG_KEEP] = {0, 0};
+    uint32_t found = 0, keep[HG_KEEP] = {};
     const int lx = (int)(e.x >> g.cell_bits);
     const float4 mn0 = smn[t], mx0 = smx[t];
     for (int l = lx; l < g.levels; l++) {
