@@ -43,13 +43,11 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n) {
     for (; i < n; i += stride) p[i] = v;
 }
 
-// The three zero fills a sort starts from (digit counts, look-back status
-// words, tile counters) in one launch.
-__global__ void k_sort_clear(uint32_t *hist, size_t n_hist, uint32_t *status, size_t n_status,
-                             uint32_t *tile_ctr, size_t n_ctr) {
+// The zero fills a sort starts from (digit counts, tile counters) in one
+// launch; the histogram kernel clears the look-back status words.
+__global__ void k_sort_clear(uint32_t *hist, size_t n_hist, uint32_t *tile_ctr, size_t n_ctr) {
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     const size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (size_t i = i0; i < n_status; i += stride) status[i] = 0u;
     for (size_t i = i0; i < n_hist; i += stride) hist[i] = 0u;
     for (size_t i = i0; i < n_ctr; i += stride) tile_ctr[i] = 0u;
 }
@@ -67,11 +65,16 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
 template <int RBITS, int ITEMS>
 __global__ __launch_bounds__(SORT_BLOCK) void k_histogram(const uint2 *kv, int n, int passes,
                                                           int sort_bits, uint32_t *hist,
-                                                          const int *n_dev) {
+                                                          const int *n_dev, uint32_t *status,
+                                                          int tiles) {
     constexpr int RADIX = 1 << RBITS;
     constexpr int TILE = SORT_BLOCK * ITEMS;
     if (n_dev && *n_dev < n) n = *n_dev;
     if ((int)blockIdx.x * TILE >= n) return;
+    // this tile's look-back status words of every pass (the tiles beyond the
+    // element count neither publish nor are read)
+    for (int i = threadIdx.x; i < passes * RADIX; i += SORT_BLOCK)
+        status[((size_t)(i / RADIX) * tiles + blockIdx.x) * RADIX + (i % RADIX)] = 0u;
     __shared__ uint32_t h[MAX_PASSES * RADIX];
     for (int i = threadIdx.x; i < passes * RADIX; i += SORT_BLOCK) h[i] = 0;
     __syncthreads();
@@ -381,13 +384,12 @@ hipError_t radix_sort_tiles(uint2 *kv, int n, int sort_bits, const SortWork &w, 
     const int passes = (sort_bits + RBITS - 1) / RBITS;
     const int tiles = (n + TILE - 1) / TILE;
     if (passes > MAX_PASSES || tiles > w.max_tiles) return hipErrorInvalidValue;
-    size_t zero_words = (size_t)passes * RADIX;
-    size_t status_words = (size_t)passes * tiles * RADIX;
     // (the error word after the tile counters is sticky: not cleared)
-    RB_LAUNCH(k_sort_clear, dim3(256), dim3(256), 0, s, w.hist, zero_words, w.status,
-                       status_words, w.tile_ctr, (size_t)MAX_PASSES);
+    if (!w.precleared)
+        RB_LAUNCH(k_sort_clear, dim3(8), dim3(256), 0, s, w.hist, (size_t)passes * RADIX,
+                  w.tile_ctr, (size_t)MAX_PASSES);
     RB_LAUNCH((k_histogram<RBITS, ITEMS>), dim3(tiles), dim3(SORT_BLOCK), 0, s, kv, n, passes,
-                       sort_bits, w.hist, n_dev);
+                       sort_bits, w.hist, n_dev, w.status, tiles);
     // look-back timeouts: a sticky bit in the caller's overflow word, else
     // in the word after the tile counters
     uint32_t *err = w.err ? reinterpret_cast<uint32_t *>(w.err) : w.tile_ctr + MAX_PASSES;

@@ -274,7 +274,15 @@ struct SortWork {
     int *err = nullptr;   // overflow word that gets MI_RB_OVERFLOW_LOOKBACK on a look-back timeout
     int max_n;
     int max_tiles;
+    // hist and tile_ctr are zero on entry (a step clears one head per sort
+    // in its first kernel), so no clearing launch precedes the sort
+    bool precleared = false;
 };
+// A sort's digit counters and tile counters (RB_SORT_HEAD_WORDS words:
+// 8 passes x 256 digits, then 8 tile counters); a step keeps one head per
+// sort after its scans' status regions.
+#define RB_SORT_HEAD_WORDS (8 * 256 + 16)
+enum { RB_STEP_SORT_BODY = 0, RB_STEP_SORT_CAND = 1, RB_STEP_SORT_TILE = 2, RB_STEP_SORTS = 3 };
 size_t sort_status_words(int max_n, int digit_bits);
 // n is the capacity; when n_dev is given the kernels read the element count
 // from device memory (min(n, *n_dev)), so the launch needs no host sync.

@@ -58,8 +58,8 @@ int ilog2(uint64_t v) {
 }
 
 // Start of a step: the counters (the overflow bits are sticky), the
-// solver's per-sweep residual maxima, the scans' status words and the solver
-// tiles' row ranges.
+// solver's per-sweep residual maxima, the scans' status words and the sorts'
+// digit and tile counters, and the solver tiles' row ranges.
 __global__ void k_zero_counters(int *c, float *residual, int n_residual, uint32_t *scan_status,
                                 int n_scan_words, int *tile_first, int *tile_cnt, int n_tiles) {
     const int i = threadIdx.x;
@@ -67,7 +67,8 @@ __global__ void k_zero_counters(int *c, float *residual, int n_residual, uint32_
         if (i < RB_CNT_COUNT && i != RB_CNT_OVERFLOW) c[i] = 0;
         for (int k = i; k < n_residual; k += blockDim.x) residual[k] = 0.f;
     }
-    // the status words of the step's scans (one region per scan)
+    // the status words of the step's scans (one region per scan), then the
+    // step's sort heads
     for (int k = blockIdx.x * blockDim.x + i; k < n_scan_words; k += gridDim.x * blockDim.x)
         scan_status[k] = 0u;
     // the solver tiles' row ranges (k_tile_bounds2 sets the non-empty ones)
@@ -121,7 +122,8 @@ struct World {
     std::vector<int> is_large, large_list;
     DeviceWorld d{};
     SortWork sw{};
-    uint32_t *scan_status = nullptr;
+    SortWork sw_step[RB_STEP_SORTS];  // sw with the step's precleared heads
+    uint32_t *scan_status = nullptr;  // scan regions, then the step's sort heads
     size_t scan_stride = 0;  // words per scan region (RB_SCAN_REGIONS regions)
     long long steps = 0;
     float last_ms = 0.f;
@@ -443,7 +445,14 @@ mi_rb_status derive_and_upload(World &w) {
     w.sw.err = d.counters + RB_CNT_OVERFLOW;
     int scan_cap = std::max(n, maxP);
     w.scan_stride = rb_scan_status_words(scan_cap);
-    RB_CHECK(w.alloc(&w.scan_status, RB_SCAN_REGIONS * w.scan_stride));
+    RB_CHECK(w.alloc(&w.scan_status,
+                     RB_SCAN_REGIONS * w.scan_stride + RB_STEP_SORTS * RB_SORT_HEAD_WORDS));
+    for (int k = 0; k < RB_STEP_SORTS; k++) {
+        w.sw_step[k] = w.sw;
+        w.sw_step[k].hist = w.scan_status + RB_SCAN_REGIONS * w.scan_stride + k * RB_SORT_HEAD_WORDS;
+        w.sw_step[k].tile_ctr = w.sw_step[k].hist + 8 * 256;
+        w.sw_step[k].precleared = true;
+    }
 
     // ---- uploads ----
     hipStream_t s = w.stream;
@@ -509,13 +518,15 @@ mi_rb_status enqueue_step(World &w, int last_stage) {
     if (ev) RB_CHECK(hipEventRecord(w.ev[0], s));
     RB_LAUNCH(k_zero_counters, dim3(16), dim3(256), 0, s, d.counters, d.residual,
                        2 * (p.normal_iters + p.friction_iters + 1), w.scan_status,
-                       (int)(RB_SCAN_REGIONS * w.scan_stride), d.tile_first, d.tile_cnt,
+                       (int)(RB_SCAN_REGIONS * w.scan_stride + RB_STEP_SORTS * RB_SORT_HEAD_WORDS),
+                       d.tile_first, d.tile_cnt,
                        p.n_tiles);
     RB_CHECK(launch_setup(d, p, s));
     if (ev) RB_CHECK(hipEventRecord(w.ev[1], s));
     if (last_stage < MI_RB_STAGE_SORT) return MI_RB_OK;
     RB_CHECK(launch_radix_sort(d.kv, p.n_bodies, w.key_bits,
-                               w.tuning.sort_digit_bits ? w.tuning.sort_digit_bits : 8, w.sw, s,
+                               w.tuning.sort_digit_bits ? w.tuning.sort_digit_bits : 8,
+                               w.sw_step[RB_STEP_SORT_BODY], s,
                                nullptr, w.tuning.sort_tile));
     if (ev) RB_CHECK(hipEventRecord(w.ev[2], s));
     if (last_stage < MI_RB_STAGE_PAIRS) return MI_RB_OK;
@@ -530,7 +541,8 @@ mi_rb_status enqueue_step(World &w, int last_stage) {
         RB_CHECK(launch_compound_pairs(d, p, s, w.scan_status, w.scan_stride));
     if (ev) RB_CHECK(hipEventRecord(w.ev[3], s));
     if (last_stage < MI_RB_STAGE_NARROWPHASE) return MI_RB_OK;
-    RB_CHECK(launch_narrowphase(d, p, s, w.sw, w.scan_status, w.scan_stride));
+    RB_CHECK(launch_narrowphase(d, p, s, w.sw_step[RB_STEP_SORT_CAND], w.scan_status,
+                                w.scan_stride));
     if (ev) RB_CHECK(hipEventRecord(w.ev[4], s));
     if (last_stage < MI_RB_STAGE_CONSTRAINTS) return MI_RB_OK;
     RB_CHECK(launch_exclusive_scan(d.has_contact, d.contact_off, p.max_pairs,
@@ -542,7 +554,8 @@ mi_rb_status enqueue_step(World &w, int last_stage) {
     if (p.warm_start == MI_RB_WARM_LEARNED) RB_CHECK(launch_mlp_warm(d, p, s));
     if (ev) RB_CHECK(hipEventRecord(w.ev[5], s));
     if (last_stage < MI_RB_STAGE_COLOR) return MI_RB_OK;
-    RB_CHECK(launch_radix_sort(d.tile_kv, p.max_contacts, w.tile_bits, 8, w.sw, s,
+    RB_CHECK(launch_radix_sort(d.tile_kv, p.max_contacts, w.tile_bits, 8,
+                               w.sw_step[RB_STEP_SORT_TILE], s,
                                d.counters + RB_CNT_CONTACTS));
     RB_CHECK(launch_tile_bounds(d, p, s));
     RB_CHECK(launch_tiles(d, p, s));
