@@ -459,7 +459,6 @@ size_t sort_status_words(int max_n, int digit_bits) {
 hipError_t launch_radix_sort(uint2 *kv, int n, int sort_bits, int digit_bits, const SortWork &w,
                              hipStream_t s, const int *n_dev, int tile_keys) {
     if (n <= 1 || sort_bits <= 0) return hipSuccess;
-    if (tile_keys == 0) tile_keys = w.tile_keys;
     switch (digit_bits) {
         case 4: return radix_sort_impl<4>(kv, n, sort_bits, w, s, n_dev, tile_keys);
         case 6: return radix_sort_impl<6>(kv, n, sort_bits, w, s, n_dev, tile_keys);

@@ -277,8 +277,6 @@ struct SortWork {
     // hist and tile_ctr are zero on entry (a step clears one head per sort
     // in its first kernel), so no clearing launch precedes the sort
     bool precleared = false;
-    // tile size when a sort call names none (0: chosen by the element count)
-    int tile_keys = 0;
 };
 // A sort's digit counters and tile counters (RB_SORT_HEAD_WORDS words:
 // 8 passes x 256 digits, then 8 tile counters); a step keeps one head per

@@ -453,11 +453,6 @@ mi_rb_status derive_and_upload(World &w) {
         w.sw_step[k].tile_ctr = w.sw_step[k].hist + 8 * 256;
         w.sw_step[k].precleared = true;
     }
-    // the candidate and contact-tile sorts are sized by their capacities but
-    // sort the counts found this step: small tiles spread those over more
-    // workgroups
-    w.sw_step[RB_STEP_SORT_CAND].tile_keys = RB_SORT_TILE_SMALL;
-    w.sw_step[RB_STEP_SORT_TILE].tile_keys = RB_SORT_TILE_SMALL;
 
     // ---- uploads ----
     hipStream_t s = w.stream;
