@@ -282,7 +282,10 @@ struct SortWork {
 // 8 passes x 256 digits, then 8 tile counters); a step keeps one head per
 // sort after its scans' status regions.
 #define RB_SORT_HEAD_WORDS (8 * 256 + 16)
-enum { RB_STEP_SORT_BODY = 0, RB_STEP_SORT_CAND = 1, RB_STEP_SORT_TILE = 2, RB_STEP_SORTS = 3 };
+enum {
+    RB_STEP_SORT_BODY = 0, RB_STEP_SORT_CAND = 1, RB_STEP_SORT_TILE = 2, RB_STEP_SORT_ORDER = 3,
+    RB_STEP_SORTS = 4
+};
 size_t sort_status_words(int max_n, int digit_bits);
 // n is the capacity; when n_dev is given the kernels read the element count
 // from device memory (min(n, *n_dev)), so the launch needs no host sync.

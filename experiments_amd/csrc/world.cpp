@@ -559,7 +559,7 @@ mi_rb_status enqueue_step(World &w, int last_stage) {
                                d.counters + RB_CNT_CONTACTS));
     RB_CHECK(launch_tile_bounds(d, p, s));
     RB_CHECK(launch_tiles(d, p, s));
-    RB_CHECK(launch_tile_order(d, p, s, w.sw));
+    RB_CHECK(launch_tile_order(d, p, s, w.sw_step[RB_STEP_SORT_ORDER]));
     if (ev) RB_CHECK(hipEventRecord(w.ev[6], s));
     if (last_stage < MI_RB_STAGE_SOLVE) return MI_RB_OK;
     if (ev) {
